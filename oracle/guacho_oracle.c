@@ -1,0 +1,933 @@
+/*
+ * guacho_oracle.c -- CPU restatement of Guacho-3D's Godunov MHD step.
+ *
+ * TEST INFRASTRUCTURE: this is the checker for the HIP product path, never
+ * the thing measured as the product and never linked into it.  Every
+ * function restates the reference routine cited beside it, keeping the
+ * reference's array layout (AoS, neq fastest), sweep structure and
+ * floating-point operation order (Fortran evaluates left to right and
+ * keeps parentheses; x**2 is x*x).  Build with -ffp-contract=off so no
+ * multiply-add is fused.
+ *
+ * Parity pins (tests/test_oracle_*.py):
+ *   - per-function vectors against the reference compiled from its own
+ *     sources (oracle/_ref, built by oracle/Makefile) and the committed
+ *     fixtures under tests/golden/ made from it;
+ *   - whole sweeps (hlldfluxes/hllefluxes/hllfluxes/hllcfluxes, calcprim,
+ *     boundaryI/II, get_timestep) against the same reference build;
+ *   - get_efield / flux_cd_update / step / viscous_copy / boundaryI_ef /
+ *     tstep: the reference's flux_cd_module.f90 does not compile without
+ *     MPI (serial branch at flux_cd_module.f90:120 is never closed), so
+ *     these are pinned by invariants: flux-CD keeps the central-difference
+ *     div B of boundaryI's dump formula at round-off, periodic boxes
+ *     conserve mass/momentum/energy/B to round-off, 180-degree rotation
+ *     symmetry of Orszag-Tang, axis-permutation symmetry, and a step
+ *     composed from the reference's own sweeps with restated flux-CD.
+ *   - UR* degenerate branch (hlld.f90:167-172) : parity unpinned -- the
+ *     reference assigns the L variables there and leaves vstr..bzstr
+ *     undefined; this restatement mirrors the L logic for R.
+ */
+#include "guacho_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define NEQ 8
+
+struct orc_state {
+  orc_params p;
+  int NX, NY, NZ;   /* extents incl. ghosts: n+4 */
+  long ncell;
+  double *u, *up, *primit, *f, *g, *h, *e, *temp;
+};
+
+/* Fortran MAX/MIN on two reals. */
+static inline double fmax2(double a, double b) { return (a > b) ? a : b; }
+static inline double fmin2(double a, double b) { return (a < b) ? a : b; }
+/* Fortran SIGN(1., x) (copysign semantics, -0 gives -1). */
+static inline double sign1(double x) { return copysign(1.0, x); }
+
+/* offset of u(1, i, j, k) for reference 1-based i,j,k (ghosts -1..n+2) */
+static inline long cidx(const orc_state *s, int i, int j, int k) {
+  return ((long)(k + 1) * s->NY + (j + 1)) * s->NX + (i + 1);
+}
+#define AT(arr, s, q, i, j, k) ((arr)[cidx((s), (i), (j), (k)) * NEQ + (q)])
+#define EAT(s, q, i, j, k) ((s)->e[cidx((s), (i), (j), (k)) * 3 + (q)])
+
+/* ===================================================================== */
+/* per-cell physics: src/hydro_core.f90                                  */
+/* ===================================================================== */
+
+/* u2prim, hydro_core.f90:46-129 (EOS_ADIABATIC, no passives) */
+void orc_u2prim(const orc_params *p, const double *uu, double *prim, double *T) {
+  double r = fmax2(uu[0], 1e-15);                                   /* :62 */
+  prim[0] = r;
+  prim[1] = uu[1] / r;                                               /* :65-67 */
+  prim[2] = uu[2] / r;
+  prim[3] = uu[3] / r;
+  if (p->mhd) {                                                      /* :71-72 */
+    prim[4] = ((uu[4] - (0.5 * r) * ((prim[1] * prim[1] + prim[2] * prim[2]) + prim[3] * prim[3]))
+               - 0.5 * ((uu[5] * uu[5] + uu[6] * uu[6]) + uu[7] * uu[7])) / p->cv;
+  } else {                                                           /* :75 */
+    prim[4] = (uu[4] - (0.5 * r) * ((prim[1] * prim[1] + prim[2] * prim[2]) + prim[3] * prim[3])) / p->cv;
+  }
+  prim[4] = fmax2(prim[4], 1e-16);                                   /* :78 */
+  if (p->mhd || p->pmhd) {                                           /* :80-84 */
+    prim[5] = uu[5];
+    prim[6] = uu[6];
+    prim[7] = uu[7];
+  } else {
+    prim[5] = uu[5];
+    prim[6] = uu[6];
+    prim[7] = uu[7];
+  }
+  if (T) *T = (prim[4] / r) * 1.0;                                   /* :94, Tempsc=1 */
+}
+
+/* prim2u, hydro_core.f90:331-383 (non-split branch) */
+void orc_prim2u(const orc_params *p, const double *prim, double *uu) {
+  uu[0] = prim[0];                                                   /* :349-352 */
+  uu[1] = prim[0] * prim[1];
+  uu[2] = prim[0] * prim[2];
+  uu[3] = prim[0] * prim[3];
+  uu[4] = (0.5 * prim[0]) * ((prim[1] * prim[1] + prim[2] * prim[2]) + prim[3] * prim[3])
+          + p->cv * prim[4];                                         /* :356 */
+  if (p->mhd)
+    uu[4] = uu[4] + 0.5 * ((prim[5] * prim[5] + prim[6] * prim[6]) + prim[7] * prim[7]); /* :366 */
+  if (p->mhd || p->pmhd) {                                           /* :372-374 */
+    uu[5] = prim[5];
+    uu[6] = prim[6];
+    uu[7] = prim[7];
+  } else {
+    uu[5] = uu[6] = uu[7] = 0.0;
+  }
+}
+
+/* prim2f, hydro_core.f90:395-476 (non-split branches) */
+void orc_prim2f(const orc_params *p, const double *prim, double *ff) {
+  double etot;
+  if (p->mhd) {                                                      /* :432-441 */
+    etot = 0.5 * (((prim[0] * ((prim[1] * prim[1] + prim[2] * prim[2]) + prim[3] * prim[3])
+                    + prim[5] * prim[5]) + prim[6] * prim[6]) + prim[7] * prim[7])
+           + p->cv * prim[4];
+    ff[0] = prim[0] * prim[1];
+    ff[1] = ((prim[0] * prim[1]) * prim[1] + prim[4])
+            + 0.5 * ((prim[6] * prim[6] + prim[7] * prim[7]) - prim[5] * prim[5]);
+    ff[2] = (prim[0] * prim[1]) * prim[2] - prim[5] * prim[6];
+    ff[3] = (prim[0] * prim[1]) * prim[3] - prim[5] * prim[7];
+    ff[4] = prim[1] * ((etot + prim[4]) + 0.5 * ((prim[5] * prim[5] + prim[6] * prim[6]) + prim[7] * prim[7]))
+            - prim[5] * ((prim[1] * prim[5] + prim[2] * prim[6]) + prim[3] * prim[7]);
+  } else {                                                           /* :446-452 */
+    etot = (0.5 * prim[0]) * ((prim[1] * prim[1] + prim[2] * prim[2]) + prim[3] * prim[3]) + p->cv * prim[4];
+    ff[0] = prim[0] * prim[1];
+    ff[1] = (prim[0] * prim[1]) * prim[1] + prim[4];
+    ff[2] = (prim[0] * prim[1]) * prim[2];
+    ff[3] = (prim[0] * prim[1]) * prim[3];
+    ff[4] = prim[1] * (etot + prim[4]);
+  }
+  if (p->mhd || p->pmhd) {                                           /* :463-465 */
+    ff[5] = 0.0;
+    ff[6] = prim[1] * prim[6] - prim[5] * prim[2];
+    ff[7] = prim[1] * prim[7] - prim[5] * prim[3];
+  } else {
+    ff[5] = ff[6] = ff[7] = 0.0;
+  }
+}
+
+/* swapy / swapz, hydro_core.f90:485-534 */
+void orc_swapy(double *v) {
+  double a = v[1]; v[1] = v[2]; v[2] = a;
+  a = v[5]; v[5] = v[6]; v[6] = a;
+}
+void orc_swapz(double *v) {
+  double a = v[1]; v[1] = v[3]; v[3] = a;
+  a = v[5]; v[5] = v[7]; v[7] = a;
+}
+
+/* csound, hydro_core.f90:544-553 */
+double orc_csound(const orc_params *p, double pr, double d) {
+  return sqrt((p->gamma * pr) / d);
+}
+
+/* cfast, hydro_core.f90:568-581 */
+void orc_cfast(const orc_params *p, double pr, double d, double bx, double by, double bz,
+               double *cfx, double *cfy, double *cfz) {
+  double g = p->gamma;
+  double b2 = (bx * bx + by * by) + bz * bz;
+  double t = g * pr + b2;
+  *cfx = sqrt((0.5 * (t + sqrt(t * t - (((4.0 * g) * pr) * bx) * bx))) / d);
+  *cfy = sqrt((0.5 * (t + sqrt(t * t - (((4.0 * g) * pr) * by) * by))) / d);
+  *cfz = sqrt((0.5 * (t + sqrt(t * t - (((4.0 * g) * pr) * bz) * bz))) / d);
+}
+
+/* cfastX, hydro_core.f90:591-604 */
+double orc_cfastX(const orc_params *p, const double *prim) {
+  double g = p->gamma;
+  double b2 = (prim[5] * prim[5] + prim[6] * prim[6]) + prim[7] * prim[7];
+  double cs2va2 = (g * prim[4] + b2) / prim[0];
+  return sqrt(0.5 * (cs2va2 + sqrt(cs2va2 * cs2va2
+                                   - ((((4.0 * g) * prim[4]) * (prim[5] * prim[5])) / prim[0]) / prim[0])));
+}
+
+/* average(), the slope limiters of hydro_core.f90:735-796 */
+static double lim_average(int lim, double a, double b) {
+  double s, c, d, av1, av2, av = 0.0;
+  switch (lim) {
+    case ORC_LIMITER_NO_AVERAGE: av = 0.0; break;                    /* :741-744 */
+    case ORC_LIMITER_NO_LIMIT: av = 0.5 * (a + b); break;            /* :746-749 */
+    case ORC_LIMITER_MINMOD:                                         /* :751-755 */
+      s = sign1(a);
+      av = s * fmax2(0.0, fmin2(fabs(a), s * b));
+      break;
+    case ORC_LIMITER_VAN_LEER:                                       /* :757-764 */
+      if (a * b <= 0.0) av = 0.0;
+      else av = ((a * b) * (a + b)) / (a * a + b * b);
+      break;
+    case ORC_LIMITER_VAN_ALBADA: {                                   /* :766-769 */
+      const double delta = 1e-7;
+      av = (a * (b * b + delta) + b * (a * a + delta)) / ((a * a + b * b) + delta);
+      break;
+    }
+    case ORC_LIMITER_UMIST:                                          /* :771-778 */
+      s = sign1(a);
+      c = 0.25 * a + 0.75 * b;
+      d = 0.75 * a + 0.25 * b;
+      av = fmin2(fmin2(fmin2(2.0 * fabs(a), (2.0 * s) * b), s * c), s * d);
+      av = s * fmax2(0.0, av);
+      break;
+    case ORC_LIMITER_WOODWARD:                                       /* :780-786 */
+      s = sign1(a);
+      c = 0.5 * (a + b);
+      av = fmin2(fmin2(2.0 * fabs(a), (2.0 * s) * b), s * c);
+      av = s * fmax2(0.0, av);
+      break;
+    case ORC_LIMITER_SUPERBEE:                                       /* :788-794 */
+      s = sign1(b);
+      av1 = fmin2(2.0 * fabs(b), s * a);
+      av2 = fmin2(fabs(b), (2.0 * s) * a);
+      av = s * fmax2(fmax2(0.0, av1), av2);
+      break;
+  }
+  return av;
+}
+
+/* limiter, hydro_core.f90:712-731 */
+void orc_limiter(const orc_params *p, const double *pll, double *pl, double *pr, const double *prr) {
+  for (int q = 0; q < NEQ; q++) {
+    double dl = pl[q] - pll[q];
+    double dm = pr[q] - pl[q];
+    double dr = prr[q] - pr[q];
+    double al = lim_average(p->limiter, dl, dm);
+    double ar = lim_average(p->limiter, dm, dr);
+    pl[q] = pl[q] + al * 0.5;
+    pr[q] = pr[q] - ar * 0.5;
+  }
+}
+
+/* ===================================================================== */
+/* Riemann solvers                                                        */
+/* ===================================================================== */
+
+/* prim2fhlld, src/hlld.f90:48-319 */
+int orc_hlld(const orc_params *p, const double *pl, const double *pr, double *ff) {
+  const double cv = p->cv;
+  double csl = orc_cfastX(p, pl), csr = orc_cfastX(p, pr);          /* :64-65 */
+  double sr = fmax2(pl[1] + csl, pr[1] + csr);                       /* :67 */
+  double sl = fmin2(pl[1] - csl, pr[1] - csr);                       /* :68 */
+  if (sl > 0) { orc_prim2f(p, pl, ff); return 0; }                   /* :71-74 */
+  if (sr < 0) { orc_prim2f(p, pr, ff); return 0; }                   /* :77-80 */
+
+  double bx = 0.5 * (pl[5] + pr[5]);                                 /* :82 */
+  double signbx = sign1(bx);                                         /* :83 */
+  double ptl = pl[4] + 0.5 * ((bx * bx + pl[6] * pl[6]) + pl[7] * pl[7]);   /* :86 */
+  double ptr = pr[4] + 0.5 * ((bx * bx + pr[6] * pr[6]) + pr[7] * pr[7]);   /* :87 */
+  double slmul = sl - pl[1], srmur = sr - pr[1];                     /* :89-90 */
+  double rholul = pl[0] * pl[1], rhorur = pr[0] * pr[1];             /* :92-93 */
+  double sM = (((srmur * rhorur - slmul * rholul) - ptr) + ptl)
+              / (srmur * pr[0] - slmul * pl[0]);                     /* :95 */
+  double srmsM = sr - sM, slmsM = sl - sM;                           /* :97-98 */
+  double rhostl = (pl[0] * slmul) / slmsM;                           /* :100 */
+  double rhostr = (pr[0] * srmur) / srmsM;                           /* :101 */
+  double sstl = sM - fabs(bx) / sqrt(rhostl);                        /* :103 */
+  double sstr = sM + fabs(bx) / sqrt(rhostr);                        /* :104 */
+  double pst = (((srmur * pr[0]) * ptl - (slmul * pl[0]) * ptr)
+                + (((pl[0] * pr[0]) * srmur) * slmul) * (pr[1] - pl[1]))
+               / (srmur * pr[0] - slmul * pl[0]);                    /* :106-108 */
+
+  double vstl, wstl, bystl, bzstl, vstr, wstr, bystr, bzstr;
+  double el, er, denl, denr, vdotbl, vdotbr, vstdotbstl, vstdotbstr, estl, estr;
+
+  if (sstl >= 0) {                                                   /* UL*, :111-156 */
+    el = ((0.5 * pl[0]) * ((pl[1] * pl[1] + pl[2] * pl[2]) + pl[3] * pl[3]) + cv * pl[4])
+         + 0.5 * ((bx * bx + pl[6] * pl[6]) + pl[7] * pl[7]);
+    double sMmul = sM - pl[1];
+    denl = (pl[0] * slmul) * slmsM - bx * bx;
+    if (denl == 0) {
+      vstl = pl[2]; wstl = pl[3]; bystl = 0.0; bzstl = 0.0;
+    } else {
+      vstl = pl[2] - ((bx * pl[6]) * sMmul) / denl;
+      wstl = pl[3] - ((bx * pl[7]) * sMmul) / denl;
+      bystl = (pl[6] * (pl[0] * (slmul * slmul) - bx * bx)) / denl;
+      bzstl = (pl[7] * (pl[0] * (slmul * slmul) - bx * bx)) / denl;
+    }
+    vdotbl = (pl[1] * bx + pl[2] * pl[6]) + pl[3] * pl[7];
+    vstdotbstl = (sM * bx + vstl * bystl) + wstl * bzstl;
+    estl = (((slmul * el - ptl * pl[1]) + pst * sM) + bx * (vdotbl - vstdotbstl)) / slmsM;
+    ff[0] = rhostl * sM;
+    ff[1] = (rhostl * (sM * sM) + pst) - bx * bx;
+    ff[2] = (rhostl * sM) * vstl - bx * bystl;
+    ff[3] = (rhostl * sM) * wstl - bx * bzstl;
+    ff[4] = sM * (estl + pst) - bx * vstdotbstl;
+    ff[5] = 0.0;
+    ff[6] = bystl * sM - bx * vstl;
+    ff[7] = bzstl * sM - bx * wstl;
+    return 0;
+  }
+
+  if (sstr <= 0) {                                                   /* UR*, :159-204 */
+    er = ((0.5 * pr[0]) * ((pr[1] * pr[1] + pr[2] * pr[2]) + pr[3] * pr[3]) + cv * pr[4])
+         + 0.5 * ((bx * bx + pr[6] * pr[6]) + pr[7] * pr[7]);
+    double sMmur = sM - pr[1];
+    denr = (pr[0] * srmur) * srmsM - bx * bx;
+    if (denr == 0) {
+      /* reference assigns the L names here (hlld.f90:167-172); mirrored for R */
+      vstr = pr[2]; wstr = pr[3]; bystr = 0.0; bzstr = 0.0;
+    } else {
+      vstr = pr[2] - ((bx * pr[6]) * sMmur) / denr;
+      wstr = pr[3] - ((bx * pr[7]) * sMmur) / denr;
+      bystr = (pr[6] * (pr[0] * (srmur * srmur) - bx * bx)) / denr;
+      bzstr = (pr[7] * (pr[0] * (srmur * srmur) - bx * bx)) / denr;
+    }
+    vdotbr = (pr[1] * bx + pr[2] * pr[6]) + pr[3] * pr[7];
+    vstdotbstr = (sM * bx + vstr * bystr) + wstr * bzstr;
+    estr = (((srmur * er - ptr * pr[1]) + pst * sM) + bx * (vdotbr - vstdotbstr)) / srmsM;
+    ff[0] = rhostr * sM;
+    ff[1] = (rhostr * (sM * sM) + pst) - bx * bx;
+    ff[2] = (rhostr * sM) * vstr - bx * bystr;
+    ff[3] = (rhostr * sM) * wstr - bx * bzstr;
+    ff[4] = sM * (estr + pst) - bx * vstdotbstr;
+    ff[5] = 0.0;
+    ff[6] = bystr * sM - bx * vstr;
+    ff[7] = bzstr * sM - bx * wstr;
+    return 0;
+  }
+
+  /* quantities shared by UL** and UR**, :207-252 */
+  double sMmul = sM - pl[1], sMmur = sM - pr[1];
+  denl = (pl[0] * slmul) * slmsM - bx * bx;
+  denr = (pr[0] * srmur) * srmsM - bx * bx;
+  if (denl == 0) {
+    vstl = pl[2]; wstl = pl[3]; bystl = 0.0; bzstl = 0.0;
+  } else {
+    vstl = pl[2] - ((bx * pl[6]) * sMmul) / denl;
+    wstl = pl[3] - ((bx * pl[7]) * sMmul) / denl;
+    bystl = (pl[6] * (pl[0] * (slmul * slmul) - bx * bx)) / denl;
+    bzstl = (pl[7] * (pl[0] * (slmul * slmul) - bx * bx)) / denl;
+  }
+  if (denr == 0) {
+    vstr = pr[2]; wstr = pr[3]; bystr = 0.0; bzstr = 0.0;
+  } else {
+    vstr = pr[2] - ((bx * pr[6]) * sMmur) / denr;
+    wstr = pr[3] - ((bx * pr[7]) * sMmur) / denr;
+    bystr = (pr[6] * (pr[0] * (srmur * srmur) - bx * bx)) / denr;
+    bzstr = (pr[7] * (pr[0] * (srmur * srmur) - bx * bx)) / denr;
+  }
+  double dd = sqrt(rhostl) + sqrt(rhostr);
+  double vstst = ((sqrt(rhostl) * vstl + sqrt(rhostr) * vstr) + (bystr - bystl) * signbx) / dd;
+  double wstst = ((sqrt(rhostl) * wstl + sqrt(rhostr) * wstr) + (bzstr - bzstl) * signbx) / dd;
+  double bystst = ((sqrt(rhostl) * bystr + sqrt(rhostr) * bystl)
+                   + (sqrt(rhostl * rhostr) * (vstr - vstl)) * signbx) / dd;
+  double bzstst = ((sqrt(rhostl) * bzstr + sqrt(rhostr) * bzstl)
+                   + (sqrt(rhostl * rhostr) * (wstr - wstl)) * signbx) / dd;
+  double vststdotbstst = (sM * bx + vstst * bystst) + wstst * bzstst;
+
+  if (sM >= 0) {                                                     /* UL**, :255-283 */
+    el = ((0.5 * pl[0]) * ((pl[1] * pl[1] + pl[2] * pl[2]) + pl[3] * pl[3]) + cv * pl[4])
+         + 0.5 * ((bx * bx + pl[6] * pl[6]) + pl[7] * pl[7]);
+    vdotbl = (pl[1] * bx + pl[2] * pl[6]) + pl[3] * pl[7];
+    vstdotbstl = (sM * bx + vstl * bystl) + wstl * bzstl;
+    estl = (((slmul * el - ptl * pl[1]) + pst * sM) + bx * (vdotbl - vstdotbstl)) / slmsM;
+    double eststl = estl - (sqrt(rhostl) * (vstdotbstl - vststdotbstst)) * signbx;
+    ff[0] = rhostl * sM;
+    ff[1] = (rhostl * (sM * sM) + pst) - bx * bx;
+    ff[2] = (rhostl * sM) * vstst - bx * bystst;
+    ff[3] = (rhostl * sM) * wstst - bx * bzstst;
+    ff[4] = sM * (eststl + pst) - bx * vststdotbstst;
+    ff[5] = 0.0;
+    ff[6] = bystst * sM - bx * vstst;
+    ff[7] = bzstst * sM - bx * wstst;
+    return 0;
+  }
+
+  if (sM <= 0) {                                                     /* UR**, :286-314 */
+    er = ((0.5 * pr[0]) * ((pr[1] * pr[1] + pr[2] * pr[2]) + pr[3] * pr[3]) + cv * pr[4])
+         + 0.5 * ((bx * bx + pr[6] * pr[6]) + pr[7] * pr[7]);
+    vdotbr = (pr[1] * bx + pr[2] * pr[6]) + pr[3] * pr[7];
+    vstdotbstr = (sM * bx + vstr * bystr) + wstr * bzstr;
+    estr = (((srmur * er - ptr * pr[1]) + pst * sM) + bx * (vdotbr - vstdotbstr)) / srmsM;
+    double eststr = estr + (sqrt(rhostr) * (vstdotbstr - vststdotbstst)) * signbx;
+    ff[0] = rhostr * sM;
+    ff[1] = (rhostr * (sM * sM) + pst) - bx * bx;
+    ff[2] = (rhostr * sM) * vstst - bx * bystst;
+    ff[3] = (rhostr * sM) * wstst - bx * bzstst;
+    ff[4] = sM * (eststr + pst) - bx * vststdotbstst;
+    ff[5] = 0.0;
+    ff[6] = bystst * sM - bx * vstst;
+    ff[7] = bzstst * sM - bx * wstst;
+    return 0;
+  }
+  return 1;                                                          /* :316-317 */
+}
+
+/* HLL average shared by prim2fhlle (hlle.f90:74-79) and prim2fhll (hll.f90:73-78) */
+static void hll_average(const orc_params *p, const double *pl, const double *pr,
+                        double sl, double sr, double *ff) {
+  double fl[NEQ], fr[NEQ], ul[NEQ], ur[NEQ];
+  orc_prim2f(p, pl, fl);
+  orc_prim2f(p, pr, fr);
+  orc_prim2u(p, pl, ul);
+  orc_prim2u(p, pr, ur);
+  for (int q = 0; q < NEQ; q++)
+    ff[q] = ((sr * fl[q] - sl * fr[q]) + (sl * sr) * (ur[q] - ul[q])) / (sr - sl);
+}
+
+/* prim2fhlle, src/hlle.f90:48-83 */
+int orc_hlle(const orc_params *p, const double *pl, const double *pr, double *ff) {
+  double csl = orc_cfastX(p, pl), csr = orc_cfastX(p, pr);
+  double sr = fmax2(pl[1] + csl, pr[1] + csr);
+  double sl = fmin2(pl[1] - csl, pr[1] - csr);
+  if (sl > 0) { orc_prim2f(p, pl, ff); return 0; }
+  if (sr < 0) { orc_prim2f(p, pr, ff); return 0; }
+  hll_average(p, pl, pr, sl, sr, ff);
+  return 0;
+}
+
+/* prim2fhll, src/hll.f90:47-82 */
+int orc_hll(const orc_params *p, const double *pl, const double *pr, double *ff) {
+  double csl = orc_csound(p, pl[4], pl[0]), csr = orc_csound(p, pr[4], pr[0]);
+  double sr = fmax2(pl[1] + csl, pr[1] + csr);
+  double sl = fmin2(pl[1] - csl, pr[1] - csr);
+  if (sl > 0) { orc_prim2f(p, pl, ff); return 0; }
+  if (sr < 0) { orc_prim2f(p, pr, ff); return 0; }
+  hll_average(p, pl, pr, sl, sr, ff);
+  return 0;
+}
+
+/* prim2fhllc, src/hllc.f90:44-140 (hydro / passive-B; full MHD leaves
+ * uuk(6:8) undefined in the reference and is rejected by orc_create) */
+int orc_hllc(const orc_params *p, const double *pl, const double *pr, double *ff) {
+  const double cv = p->cv;
+  double csl = orc_csound(p, pl[4], pl[0]), csr = orc_csound(p, pr[4], pr[0]);
+  double sr = fmax2(pl[1] + csl, pr[1] + csr);
+  double sl = fmin2(pl[1] - csl, pr[1] - csr);
+  if (sl > 0) { orc_prim2f(p, pl, ff); return 0; }
+  if (sr < 0) { orc_prim2f(p, pr, ff); return 0; }
+  double slmul = sl - pl[1], srmur = sr - pr[1];
+  double rholul = pl[0] * pl[1], rhorur = pr[0] * pr[1];
+  double sst = (((srmur * rhorur - slmul * rholul) - pr[4]) + pl[4]) / (srmur * pr[0] - slmul * pl[0]);
+  double uuk[NEQ], uu[NEQ];
+  if (sst >= 0.0) {                                                  /* :79-104 */
+    double rhost = (pl[0] * slmul) / (sl - sst);
+    double ek = (0.5 * pl[0]) * ((pl[1] * pl[1] + pl[2] * pl[2]) + pl[3] * pl[3]) + cv * pl[4];
+    uuk[0] = rhost;
+    uuk[1] = rhost * sst;
+    uuk[2] = rhost * pl[2];
+    uuk[3] = rhost * pl[3];
+    uuk[4] = rhost * (ek / pl[0] + (sst - pl[1]) * (sst + pl[4] / (pl[0] * slmul)));
+    for (int q = 5; q < 8; q++) uuk[q] = p->pmhd ? (rhost * pl[q]) / pl[0] : 0.0;
+    orc_prim2f(p, pl, ff);
+    orc_prim2u(p, pl, uu);
+    for (int q = 0; q < NEQ; q++) ff[q] = ff[q] + sl * (uuk[q] - uu[q]);
+    return 0;
+  }
+  if (sst <= 0.0) {                                                  /* :106-133 */
+    double rhost = (pr[0] * srmur) / (sr - sst);
+    double ek = (0.5 * pr[0]) * ((pr[1] * pr[1] + pr[2] * pr[2]) + pr[3] * pr[3]) + cv * pr[4];
+    uuk[0] = rhost;
+    uuk[1] = rhost * sst;
+    uuk[2] = rhost * pr[2];
+    uuk[3] = rhost * pr[3];
+    uuk[4] = rhost * (ek / pr[0] + (sst - pr[1]) * (sst + pr[4] / (pr[0] * srmur)));
+    for (int q = 5; q < 8; q++) uuk[q] = p->pmhd ? (rhost * pr[q]) / pr[0] : 0.0;
+    orc_prim2f(p, pr, ff);
+    orc_prim2u(p, pr, uu);
+    for (int q = 0; q < NEQ; q++) ff[q] = ff[q] + sr * (uuk[q] - uu[q]);
+    return 0;
+  }
+  return 1;                                                          /* :135-138 */
+}
+
+int orc_riemann(const orc_params *p, const double *pl, const double *pr, double *ff) {
+  switch (p->solver) {
+    case ORC_SOLVER_HLLD: return orc_hlld(p, pl, pr, ff);
+    case ORC_SOLVER_HLLE: return orc_hlle(p, pl, pr, ff);
+    case ORC_SOLVER_HLL: return orc_hll(p, pl, pr, ff);
+    case ORC_SOLVER_HLLC: return orc_hllc(p, pl, pr, ff);
+  }
+  return 2;
+}
+
+/* ===================================================================== */
+/* whole-domain state                                                     */
+/* ===================================================================== */
+
+int orc_create(const orc_params *p, orc_state **out) {
+  if (p->nx < 1 || p->ny < 1 || p->nz < 1) return 1;
+  if (p->solver < 1 || p->solver > 4) return 1;
+  if (p->limiter < -1 || p->limiter > 6) return 1;
+  if (p->solver == ORC_SOLVER_HLLC && p->mhd) return 1;
+  for (int b = 0; b < 6; b++)
+    if (p->bc[b] < 1 || p->bc[b] > 3) return 1;
+  /* periodic must pair on an axis, as the reference's tests do (boundaries.f90:111,122,133) */
+  for (int a = 0; a < 3; a++)
+    if ((p->bc[2 * a] == ORC_BC_PERIODIC) != (p->bc[2 * a + 1] == ORC_BC_PERIODIC)) return 1;
+  orc_state *s = (orc_state *)calloc(1, sizeof(orc_state));
+  if (!s) return 3;
+  s->p = *p;
+  s->NX = p->nx + 4;
+  s->NY = p->ny + 4;
+  s->NZ = p->nz + 4;
+  s->ncell = (long)s->NX * s->NY * s->NZ;
+  size_t n8 = (size_t)s->ncell * NEQ * sizeof(double);
+  s->u = (double *)calloc(1, n8);
+  s->up = (double *)calloc(1, n8);
+  s->primit = (double *)calloc(1, n8);
+  s->f = (double *)calloc(1, n8);
+  s->g = (double *)calloc(1, n8);
+  s->h = (double *)calloc(1, n8);
+  s->e = (double *)calloc(1, (size_t)s->ncell * 3 * sizeof(double));
+  s->temp = (double *)calloc(1, (size_t)s->ncell * sizeof(double));
+  if (!s->u || !s->up || !s->primit || !s->f || !s->g || !s->h || !s->e || !s->temp) {
+    orc_destroy(s);
+    return 3;
+  }
+#ifdef _OPENMP
+  if (p->nthreads > 0) omp_set_num_threads(p->nthreads);
+#endif
+  *out = s;
+  return 0;
+}
+
+void orc_destroy(orc_state *s) {
+  if (!s) return;
+  free(s->u); free(s->up); free(s->primit); free(s->f); free(s->g); free(s->h);
+  free(s->e); free(s->temp);
+  free(s);
+}
+
+long orc_cells(const orc_state *s) { return s->ncell; }
+void orc_set_u(orc_state *s, const double *u) { memcpy(s->u, u, (size_t)s->ncell * NEQ * sizeof(double)); }
+void orc_get_u(const orc_state *s, double *u) { memcpy(u, s->u, (size_t)s->ncell * NEQ * sizeof(double)); }
+void orc_set_up(orc_state *s, const double *u) { memcpy(s->up, u, (size_t)s->ncell * NEQ * sizeof(double)); }
+void orc_get_up(const orc_state *s, double *u) { memcpy(u, s->up, (size_t)s->ncell * NEQ * sizeof(double)); }
+void orc_set_primit(orc_state *s, const double *u) { memcpy(s->primit, u, (size_t)s->ncell * NEQ * sizeof(double)); }
+void orc_get_primit(const orc_state *s, double *u) { memcpy(u, s->primit, (size_t)s->ncell * NEQ * sizeof(double)); }
+void orc_get_temp(const orc_state *s, double *t) { memcpy(t, s->temp, (size_t)s->ncell * sizeof(double)); }
+void orc_get_flux(const orc_state *s, int axis, double *out) {
+  const double *src = axis == 0 ? s->f : axis == 1 ? s->g : s->h;
+  memcpy(out, src, (size_t)s->ncell * NEQ * sizeof(double));
+}
+void orc_get_e(const orc_state *s, double *e) { memcpy(e, s->e, (size_t)s->ncell * 3 * sizeof(double)); }
+
+/* copy the 8-vector of cell (si,sj,sk) of arr into cell (di,dj,dk), optionally
+ * negating component q_flip (0-based; -1 for none) */
+static inline void copy_cell(orc_state *s, double *arr, int di, int dj, int dk,
+                             int si, int sj, int sk, int q_flip) {
+  double *d = &AT(arr, s, 0, di, dj, dk);
+  const double *src = &AT(arr, s, 0, si, sj, sk);
+  for (int q = 0; q < NEQ; q++) d[q] = (q == q_flip) ? -src[q] : src[q];
+}
+
+/* boundaryI, src/boundaries.f90:44-247, serial branch (:108-144) for
+ * periodic; closed (:146-199) and outflow (:201-242) on ranges 0:n+1 */
+void orc_boundaryI(orc_state *s) {
+  const orc_params *p = &s->p;
+  const int nx = p->nx, ny = p->ny, nz = p->nz;
+  double *u = s->u;
+  if (p->bc[0] == ORC_BC_PERIODIC && p->bc[1] == ORC_BC_PERIODIC) {
+    for (int k = -1; k <= nz + 2; k++)
+      for (int j = -1; j <= ny + 2; j++) copy_cell(s, u, 0, j, k, nx, j, k, -1);
+    for (int k = -1; k <= nz + 2; k++)
+      for (int j = -1; j <= ny + 2; j++) copy_cell(s, u, nx + 1, j, k, 1, j, k, -1);
+  }
+  if (p->bc[2] == ORC_BC_PERIODIC && p->bc[3] == ORC_BC_PERIODIC) {
+    for (int k = -1; k <= nz + 2; k++)
+      for (int i = -1; i <= nx + 2; i++) copy_cell(s, u, i, 0, k, i, ny, k, -1);
+    for (int k = -1; k <= nz + 2; k++)
+      for (int i = -1; i <= nx + 2; i++) copy_cell(s, u, i, ny + 1, k, i, 1, k, -1);
+  }
+  if (p->bc[4] == ORC_BC_PERIODIC && p->bc[5] == ORC_BC_PERIODIC) {
+    for (int j = -1; j <= ny + 2; j++)
+      for (int i = -1; i <= nx + 2; i++) copy_cell(s, u, i, j, 0, i, j, nz, -1);
+    for (int j = -1; j <= ny + 2; j++)
+      for (int i = -1; i <= nx + 2; i++) copy_cell(s, u, i, j, nz + 1, i, j, 1, -1);
+  }
+  /* closed then outflow, each in the order left,right,bottom,top,out,in */
+  for (int pass = 0; pass < 2; pass++) {
+    int kind = pass == 0 ? ORC_BC_CLOSED : ORC_BC_OUTFLOW;
+    int flip_x = pass == 0 ? 1 : -1, flip_y = pass == 0 ? 2 : -1, flip_z = pass == 0 ? 3 : -1;
+    if (p->bc[0] == kind)
+      for (int k = 0; k <= nz + 1; k++)
+        for (int j = 0; j <= ny + 1; j++) copy_cell(s, u, 0, j, k, 1, j, k, flip_x);
+    if (p->bc[1] == kind)
+      for (int k = 0; k <= nz + 1; k++)
+        for (int j = 0; j <= ny + 1; j++) copy_cell(s, u, nx + 1, j, k, nx, j, k, flip_x);
+    if (p->bc[2] == kind)
+      for (int k = 0; k <= nz + 1; k++)
+        for (int i = 0; i <= nx + 1; i++) copy_cell(s, u, i, 0, k, i, 1, k, flip_y);
+    if (p->bc[3] == kind)
+      for (int k = 0; k <= nz + 1; k++)
+        for (int i = 0; i <= nx + 1; i++) copy_cell(s, u, i, ny + 1, k, i, ny, k, flip_y);
+    if (p->bc[4] == kind)
+      for (int j = 0; j <= ny + 1; j++)
+        for (int i = 0; i <= nx + 1; i++) copy_cell(s, u, i, j, 0, i, j, 1, flip_z);
+    if (p->bc[5] == kind)
+      for (int j = 0; j <= ny + 1; j++)
+        for (int i = 0; i <= nx + 1; i++) copy_cell(s, u, i, j, nz + 1, i, j, nz, flip_z);
+  }
+}
+
+/* boundaryII, src/boundaries.f90:256-512, serial branch (:323-359) for
+ * periodic; closed (:361-438) and outflow (:440-505) mirror the two ghost
+ * layers (ghost 0 <- cell 1, ghost -1 <- cell 2) over full transverse ranges */
+void orc_boundaryII(orc_state *s) {
+  const orc_params *p = &s->p;
+  const int nx = p->nx, ny = p->ny, nz = p->nz;
+  double *u = s->up;
+  if (p->bc[0] == ORC_BC_PERIODIC && p->bc[1] == ORC_BC_PERIODIC) {
+    for (int k = -1; k <= nz + 2; k++)
+      for (int j = -1; j <= ny + 2; j++) {
+        copy_cell(s, u, -1, j, k, nx - 1, j, k, -1);
+        copy_cell(s, u, 0, j, k, nx, j, k, -1);
+      }
+    for (int k = -1; k <= nz + 2; k++)
+      for (int j = -1; j <= ny + 2; j++) {
+        copy_cell(s, u, nx + 1, j, k, 1, j, k, -1);
+        copy_cell(s, u, nx + 2, j, k, 2, j, k, -1);
+      }
+  }
+  if (p->bc[2] == ORC_BC_PERIODIC && p->bc[3] == ORC_BC_PERIODIC) {
+    for (int k = -1; k <= nz + 2; k++)
+      for (int i = -1; i <= nx + 2; i++) {
+        copy_cell(s, u, i, -1, k, i, ny - 1, k, -1);
+        copy_cell(s, u, i, 0, k, i, ny, k, -1);
+      }
+    for (int k = -1; k <= nz + 2; k++)
+      for (int i = -1; i <= nx + 2; i++) {
+        copy_cell(s, u, i, ny + 1, k, i, 1, k, -1);
+        copy_cell(s, u, i, ny + 2, k, i, 2, k, -1);
+      }
+  }
+  if (p->bc[4] == ORC_BC_PERIODIC && p->bc[5] == ORC_BC_PERIODIC) {
+    for (int j = -1; j <= ny + 2; j++)
+      for (int i = -1; i <= nx + 2; i++) {
+        copy_cell(s, u, i, j, -1, i, j, nz - 1, -1);
+        copy_cell(s, u, i, j, 0, i, j, nz, -1);
+      }
+    for (int j = -1; j <= ny + 2; j++)
+      for (int i = -1; i <= nx + 2; i++) {
+        copy_cell(s, u, i, j, nz + 1, i, j, 1, -1);
+        copy_cell(s, u, i, j, nz + 2, i, j, 2, -1);
+      }
+  }
+  for (int pass = 0; pass < 2; pass++) {
+    int kind = pass == 0 ? ORC_BC_CLOSED : ORC_BC_OUTFLOW;
+    int fx = pass == 0 ? 1 : -1, fy = pass == 0 ? 2 : -1, fz = pass == 0 ? 3 : -1;
+    if (p->bc[0] == kind)
+      for (int k = -1; k <= nz + 2; k++)
+        for (int j = -1; j <= ny + 2; j++) {
+          copy_cell(s, u, -1, j, k, 2, j, k, fx);
+          copy_cell(s, u, 0, j, k, 1, j, k, fx);
+        }
+    if (p->bc[1] == kind)
+      for (int k = -1; k <= nz + 2; k++)
+        for (int j = -1; j <= ny + 2; j++) {
+          copy_cell(s, u, nx + 1, j, k, nx, j, k, fx);
+          copy_cell(s, u, nx + 2, j, k, nx - 1, j, k, fx);
+        }
+    if (p->bc[2] == kind)
+      for (int k = -1; k <= nz + 2; k++)
+        for (int i = -1; i <= nx + 2; i++) {
+          copy_cell(s, u, i, -1, k, i, 2, k, fy);
+          copy_cell(s, u, i, 0, k, i, 1, k, fy);
+        }
+    if (p->bc[3] == kind)
+      for (int k = -1; k <= nz + 2; k++)
+        for (int i = -1; i <= nx + 2; i++) {
+          copy_cell(s, u, i, ny + 1, k, i, ny, k, fy);
+          copy_cell(s, u, i, ny + 2, k, i, ny - 1, k, fy);
+        }
+    if (p->bc[4] == kind)
+      for (int j = -1; j <= ny + 2; j++)
+        for (int i = -1; i <= nx + 2; i++) {
+          copy_cell(s, u, i, j, -1, i, j, 2, fz);
+          copy_cell(s, u, i, j, 0, i, j, 1, fz);
+        }
+    if (p->bc[5] == kind)
+      for (int j = -1; j <= ny + 2; j++)
+        for (int i = -1; i <= nx + 2; i++) {
+          copy_cell(s, u, i, j, nz + 1, i, j, nz, fz);
+          copy_cell(s, u, i, j, nz + 2, i, j, nz - 1, fz);
+        }
+  }
+}
+
+/* calcprim over the entire domain, hydro_core.f90:304-316 */
+static void calcprim_arr(orc_state *s, const double *src) {
+  const orc_params *p = &s->p;
+  long n = s->ncell;
+#pragma omp parallel for schedule(static)
+  for (long c = 0; c < n; c++) orc_u2prim(p, &src[c * NEQ], &s->primit[c * NEQ], &s->temp[c]);
+}
+void orc_calcprim_u(orc_state *s) { calcprim_arr(s, s->u); }
+void orc_calcprim_up(orc_state *s) { calcprim_arr(s, s->up); }
+
+/* hlldfluxes / hllefluxes / hllfluxes / hllcfluxes, e.g. hlld.f90:331-432:
+ * for i,j,k in 0..n, x/y/z interface states (swapped for y/z), limited
+ * for choice 2, solver, fluxes swapped back. */
+int orc_fluxes(orc_state *s, int choice) {
+  const orc_params *p = &s->p;
+  const int nx = p->nx, ny = p->ny, nz = p->nz;
+  int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+  for (int k = 0; k <= nz; k++)
+    for (int j = 0; j <= ny; j++)
+      for (int i = 0; i <= nx; i++) {
+        double pl[NEQ], pr[NEQ], pll[NEQ], prr[NEQ], ff[NEQ];
+        for (int axis = 0; axis < 3; axis++) {
+          int di = axis == 0, dj = axis == 1, dk = axis == 2;
+          memcpy(pl, &AT(s->primit, s, 0, i, j, k), sizeof pl);
+          memcpy(pr, &AT(s->primit, s, 0, i + di, j + dj, k + dk), sizeof pr);
+          if (choice == 2) {
+            memcpy(pll, &AT(s->primit, s, 0, i - di, j - dj, k - dk), sizeof pll);
+            memcpy(prr, &AT(s->primit, s, 0, i + 2 * di, j + 2 * dj, k + 2 * dk), sizeof prr);
+          }
+          if (axis == 1) {
+            orc_swapy(pl); orc_swapy(pr);
+            if (choice == 2) { orc_swapy(pll); orc_swapy(prr); }
+          } else if (axis == 2) {
+            orc_swapz(pl); orc_swapz(pr);
+            if (choice == 2) { orc_swapz(pll); orc_swapz(prr); }
+          }
+          if (choice == 2) orc_limiter(p, pll, pl, pr, prr);
+          bad |= orc_riemann(p, pl, pr, ff);
+          if (axis == 1) orc_swapy(ff);
+          else if (axis == 2) orc_swapz(ff);
+          double *dst = axis == 0 ? s->f : axis == 1 ? s->g : s->h;
+          memcpy(&dst[cidx(s, i, j, k) * NEQ], ff, sizeof ff);
+        }
+      }
+  return bad ? 1 : 0;
+}
+
+/* boundaryI_ef, flux_cd_module.f90:45-237; serial periodic semantics
+ * (:108-139), closed (:143-192) and outflow (:194-235) on 0:n+1 ranges */
+static void boundaryI_ef(orc_state *s) {
+  const orc_params *p = &s->p;
+  const int nx = p->nx, ny = p->ny, nz = p->nz;
+  if (p->bc[0] == ORC_BC_PERIODIC && p->bc[1] == ORC_BC_PERIODIC) {
+    for (int k = -1; k <= nz + 2; k++)
+      for (int j = -1; j <= ny + 2; j++)
+        for (int q = 0; q < 3; q++) EAT(s, q, 0, j, k) = EAT(s, q, nx, j, k);
+    for (int k = -1; k <= nz + 2; k++)
+      for (int j = -1; j <= ny + 2; j++)
+        for (int q = 0; q < 3; q++) EAT(s, q, nx + 1, j, k) = EAT(s, q, 1, j, k);
+  }
+  if (p->bc[2] == ORC_BC_PERIODIC && p->bc[3] == ORC_BC_PERIODIC) {
+    for (int k = -1; k <= nz + 2; k++)
+      for (int i = -1; i <= nx + 2; i++)
+        for (int q = 0; q < 3; q++) EAT(s, q, i, 0, k) = EAT(s, q, i, ny, k);
+    for (int k = -1; k <= nz + 2; k++)
+      for (int i = -1; i <= nx + 2; i++)
+        for (int q = 0; q < 3; q++) EAT(s, q, i, ny + 1, k) = EAT(s, q, i, 1, k);
+  }
+  if (p->bc[4] == ORC_BC_PERIODIC && p->bc[5] == ORC_BC_PERIODIC) {
+    for (int j = -1; j <= ny + 2; j++)
+      for (int i = -1; i <= nx + 2; i++)
+        for (int q = 0; q < 3; q++) EAT(s, q, i, j, 0) = EAT(s, q, i, j, nz);
+    for (int j = -1; j <= ny + 2; j++)
+      for (int i = -1; i <= nx + 2; i++)
+        for (int q = 0; q < 3; q++) EAT(s, q, i, j, nz + 1) = EAT(s, q, i, j, 1);
+  }
+  /* closed: x flips e1 (:147), y flips e2 (:164), z flips nothing (:181-182) */
+  for (int pass = 0; pass < 2; pass++) {
+    int kind = pass == 0 ? ORC_BC_CLOSED : ORC_BC_OUTFLOW;
+    int fx = pass == 0 ? 0 : -1, fy = pass == 0 ? 1 : -1;
+    if (p->bc[0] == kind)
+      for (int k = 0; k <= nz + 1; k++)
+        for (int j = 0; j <= ny + 1; j++)
+          for (int q = 0; q < 3; q++) EAT(s, q, 0, j, k) = q == fx ? -EAT(s, q, 1, j, k) : EAT(s, q, 1, j, k);
+    if (p->bc[1] == kind)
+      for (int k = 0; k <= nz + 1; k++)
+        for (int j = 0; j <= ny + 1; j++)
+          for (int q = 0; q < 3; q++)
+            EAT(s, q, nx + 1, j, k) = q == fx ? -EAT(s, q, nx, j, k) : EAT(s, q, nx, j, k);
+    if (p->bc[2] == kind)
+      for (int k = 0; k <= nz + 1; k++)
+        for (int i = 0; i <= nx + 1; i++)
+          for (int q = 0; q < 3; q++) EAT(s, q, i, 0, k) = q == fy ? -EAT(s, q, i, 1, k) : EAT(s, q, i, 1, k);
+    if (p->bc[3] == kind)
+      for (int k = 0; k <= nz + 1; k++)
+        for (int i = 0; i <= nx + 1; i++)
+          for (int q = 0; q < 3; q++)
+            EAT(s, q, i, ny + 1, k) = q == fy ? -EAT(s, q, i, ny, k) : EAT(s, q, i, ny, k);
+    if (p->bc[4] == kind)
+      for (int j = 0; j <= ny + 1; j++)
+        for (int i = 0; i <= nx + 1; i++)
+          for (int q = 0; q < 3; q++) EAT(s, q, i, j, 0) = EAT(s, q, i, j, 1);
+    if (p->bc[5] == kind)
+      for (int j = 0; j <= ny + 1; j++)
+        for (int i = 0; i <= nx + 1; i++)
+          for (int q = 0; q < 3; q++) EAT(s, q, i, j, nz + 1) = EAT(s, q, i, j, nz);
+  }
+}
+
+/* get_efield, flux_cd_module.f90:245-273 */
+void orc_get_efield(orc_state *s) {
+  const orc_params *p = &s->p;
+  const int nx = p->nx, ny = p->ny, nz = p->nz;
+  const double *f = s->f, *g = s->g, *h = s->h;
+#pragma omp parallel for schedule(static)
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        EAT(s, 0, i, j, k) = 0.25 * (((-AT(g, s, 7, i, j - 1, k) - AT(g, s, 7, i, j, k))
+                                      + AT(h, s, 6, i, j, k - 1)) + AT(h, s, 6, i, j, k));
+        EAT(s, 1, i, j, k) = 0.25 * (((+AT(f, s, 7, i - 1, j, k) + AT(f, s, 7, i, j, k))
+                                      - AT(h, s, 5, i, j, k - 1)) - AT(h, s, 5, i, j, k));
+        EAT(s, 2, i, j, k) = 0.25 * (((-AT(f, s, 6, i - 1, j, k) - AT(f, s, 6, i, j, k))
+                                      + AT(g, s, 5, i, j - 1, k)) + AT(g, s, 5, i, j, k));
+      }
+  boundaryI_ef(s);
+}
+
+/* step(dt), hydro_solver.f90:77-127, with flux_cd_update
+ * (flux_cd_module.f90:285-323); sources are off */
+void orc_step(orc_state *s, double dt) {
+  const orc_params *p = &s->p;
+  const int nx = p->nx, ny = p->ny, nz = p->nz;
+  const double dtdx = dt / p->dx, dtdy = dt / p->dy, dtdz = dt / p->dz;
+  const double *u = s->u, *f = s->f, *g = s->g, *h = s->h;
+  double *up = s->up;
+  if (p->flux_cd) orc_get_efield(s);
+#pragma omp parallel for schedule(static)
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        if (!p->flux_cd) {
+          for (int q = 0; q < NEQ; q++)                              /* :105-107 */
+            AT(up, s, q, i, j, k) = ((AT(u, s, q, i, j, k)
+                                      - dtdx * (AT(f, s, q, i, j, k) - AT(f, s, q, i - 1, j, k)))
+                                     - dtdy * (AT(g, s, q, i, j, k) - AT(g, s, q, i, j - 1, k)))
+                                    - dtdz * (AT(h, s, q, i, j, k) - AT(h, s, q, i, j, k - 1));
+        } else {
+          for (int q = 0; q < 5; q++)                                /* cd :299-301 */
+            AT(up, s, q, i, j, k) = ((AT(u, s, q, i, j, k)
+                                      - dtdx * (AT(f, s, q, i, j, k) - AT(f, s, q, i - 1, j, k)))
+                                     - dtdy * (AT(g, s, q, i, j, k) - AT(g, s, q, i, j - 1, k)))
+                                    - dtdz * (AT(h, s, q, i, j, k) - AT(h, s, q, i, j, k - 1));
+          AT(up, s, 5, i, j, k) = (AT(u, s, 5, i, j, k)              /* cd :311-313 */
+                                   - (0.5 * dtdy) * (EAT(s, 2, i, j + 1, k) - EAT(s, 2, i, j - 1, k)))
+                                  + (0.5 * dtdz) * (EAT(s, 1, i, j, k + 1) - EAT(s, 1, i, j, k - 1));
+          AT(up, s, 6, i, j, k) = (AT(u, s, 6, i, j, k)              /* cd :315-317 */
+                                   + (0.5 * dtdx) * (EAT(s, 2, i + 1, j, k) - EAT(s, 2, i - 1, j, k)))
+                                  - (0.5 * dtdz) * (EAT(s, 0, i, j, k + 1) - EAT(s, 0, i, j, k - 1));
+          AT(up, s, 7, i, j, k) = (AT(u, s, 7, i, j, k)              /* cd :319-321 */
+                                   - (0.5 * dtdx) * (EAT(s, 1, i + 1, j, k) - EAT(s, 1, i - 1, j, k)))
+                                  + (0.5 * dtdy) * (EAT(s, 0, i, j + 1, k) - EAT(s, 0, i, j - 1, k));
+        }
+      }
+}
+
+/* viscous_copy, hydro_solver.f90:47-65 */
+void orc_viscous_copy(orc_state *s) {
+  const orc_params *p = &s->p;
+  const int nx = p->nx, ny = p->ny, nz = p->nz;
+  const double eta = p->eta;
+  const double *up = s->up;
+  double *u = s->u;
+#pragma omp parallel for schedule(static)
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++)
+        for (int q = 0; q < NEQ; q++)
+          AT(u, s, q, i, j, k) = AT(up, s, q, i, j, k)
+              + eta * ((((((AT(up, s, q, i + 1, j, k) + AT(up, s, q, i - 1, j, k))
+                           + AT(up, s, q, i, j + 1, k)) + AT(up, s, q, i, j - 1, k))
+                         + AT(up, s, q, i, j, k + 1)) + AT(up, s, q, i, j, k - 1))
+                       - 6.0 * AT(up, s, q, i, j, k));
+}
+
+/* tstep, hydro_solver.f90:134-229 with every add-on off */
+int orc_tstep(orc_state *s, double dt) {
+  int bad;
+  double dtm = dt / 2.0;                                             /* :152 */
+  bad = orc_fluxes(s, 1);                                            /* :158 */
+  orc_step(s, dtm);                                                  /* :165 */
+  orc_boundaryII(s);                                                 /* :169 */
+  orc_calcprim_up(s);                                                /* :170 */
+  bad |= orc_fluxes(s, 2);                                           /* :177 */
+  orc_step(s, dt);                                                   /* :184 */
+  orc_viscous_copy(s);                                               /* :188 */
+  orc_boundaryI(s);                                                  /* :216 */
+  orc_calcprim_u(s);                                                 /* :220 */
+  return bad;
+}
+
+/* min loop of get_timestep, hydro_core.f90:642-675 */
+double orc_cfl_dtp(const orc_state *s) {
+  const orc_params *p = &s->p;
+  const int nx = p->nx, ny = p->ny, nz = p->nz;
+  double dtp = 1e30;
+#pragma omp parallel for schedule(static) reduction(min : dtp)
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++) {
+        const double *pm = &s->primit[cidx(s, i, j, k) * NEQ];
+        if (p->mhd) {
+          double cx, cy, cz;
+          orc_cfast(p, pm[4], pm[0], pm[5], pm[6], pm[7], &cx, &cy, &cz);
+          dtp = fmin2(dtp, p->dx / (fabs(pm[1]) + cx));
+          dtp = fmin2(dtp, p->dy / (fabs(pm[2]) + cy));
+          dtp = fmin2(dtp, p->dz / (fabs(pm[3]) + cz));
+        } else {
+          double c = orc_csound(p, pm[4], pm[0]);
+          dtp = fmin2(dtp, p->dx / (fabs(pm[1]) + c));
+          dtp = fmin2(dtp, p->dy / (fabs(pm[2]) + c));
+          dtp = fmin2(dtp, p->dz / (fabs(pm[3]) + c));
+        }
+      }
+  return dtp;
+}
+
+/* CFL ramp and dump clamp, hydro_core.f90:677-694 */
+void orc_timestep_finish(int current_iter, int n_iter, double cfl, double dtp,
+                         double current_time, double tprint, double *dt, int *dump_flag) {
+  if (current_iter <= n_iter)
+    dtp = (cfl * pow(2.0, -(double)(n_iter + 1 - current_iter))) * dtp;
+  else
+    dtp = cfl * dtp;
+  *dt = dtp;
+  if ((current_time + *dt) >= tprint) {
+    *dt = tprint - current_time;
+    *dump_flag = 1;
+  }
+}
+
+/* divB dump formula, Out_BIN_Module.f90:214-222 */
+void orc_divb(const orc_state *s, double *divb) {
+  const orc_params *p = &s->p;
+  const int nx = p->nx, ny = p->ny, nz = p->nz;
+  const double *u = s->u;
+  for (int k = 1; k <= nz; k++)
+    for (int j = 1; j <= ny; j++)
+      for (int i = 1; i <= nx; i++)
+        divb[((long)(k - 1) * ny + (j - 1)) * nx + (i - 1)] =
+            ((AT(u, s, 5, i + 1, j, k) - AT(u, s, 5, i - 1, j, k)) / (2.0 * p->dx)
+             + (AT(u, s, 6, i, j + 1, k) - AT(u, s, 6, i, j - 1, k)) / (2.0 * p->dy))
+            + (AT(u, s, 7, i, j, k + 1) - AT(u, s, 7, i, j, k - 1)) / (2.0 * p->dz);
+}
