@@ -1,0 +1,144 @@
+"""Benchmark of the Godunov MHD step on one MI355X.
+
+Workload: the seeded 3D MHD turbulence box, 256^3 periodic, plain FP64,
+HLLD with minmod reconstruction and flux-CD, CFL 0.4 -- 20 steps from the
+seeded initial state.  A "step" is one iteration of the reference's time
+loop (src/main.f90:94-125): the CFL reduction (get_timestep) and one tstep,
+state resident in HBM.
+
+Prints one JSON line: zone-updates/s over the timed steps, the HBM roofline
+of the dominant kernel (limited-reconstruction flux sweep) from HIP events
+on the library's stream, the step-level roofline at the 128 B/zone-update
+compulsory traffic, and the C restatement of the reference's CPU path
+timed on a bounded sample on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
+BYTES_PER_ZONE_FP64 = 128      # read 8 + write 8 conserved doubles per zone-update
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256, help="cells per side of the turbulence box")
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    return ap.parse_args()
+
+
+def cpu_baseline(seconds: float):
+    """The C restatement (oracle/) on the host cores: turbulence 128^3, steps
+    until `seconds` of wall time have passed (at most 200)."""
+    import numpy as np  # noqa: F401
+    from guacho_amd import problems
+    from guacho_amd.host import timestep_clamp
+    from oracle.oracle import Config, Oracle
+
+    threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")), 16)
+    n = 128
+    u0, dx, dy, dz = problems.turbulence(n, seed=2024)
+    O = Oracle(Config(nx=n, ny=n, nz=n, dx=dx, dy=dy, dz=dz), nthreads=threads)
+    O.set_u(u0)
+    O.boundaryI()
+    O.calcprim_u()
+    t0 = time.perf_counter()
+    steps = 0
+    t = 0.0
+    while steps < 200:
+        dt, _ = timestep_clamp(steps + 1, 10, 0.4, O.cfl_dtp(), t, 1e30)
+        O.tstep(dt)
+        t += dt
+        steps += 1
+        if time.perf_counter() - t0 >= seconds:
+            break
+    el = time.perf_counter() - t0
+    O.close()
+    return {"value": n ** 3 * steps / el, "unit": "zone-updates/s", "cores": threads, "kind": "port",
+            "sample": f"C restatement (oracle/guacho_oracle.c, gcc -O2, OpenMP {threads} threads), "
+                      f"turbulence {n}^3 FP64 HLLD minmod flux-CD, {steps} steps in {el:.2f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs one subdomain per GPU and no "
+              f"multi-GPU path exists (asked for --gpus {args.gpus})", file=sys.stderr)
+        sys.exit(2)
+    from guacho_amd import _capi, problems
+    from guacho_amd.host import timestep_clamp
+
+    n = args.n
+    u0, dx, dy, dz = problems.turbulence(n, seed=args.seed)
+    prm = _capi.make_params(n, n, n, dx, dy, dz)
+    cfl = 0.4
+    with _capi.Subdomain(prm) as dev:
+        # warm-up on the same state, then restart from the seeded state
+        dev.upload_u(u0)
+        dev.boundary_u()
+        t = 0.0
+        for it in range(1, args.warmup + 1):
+            dt, _ = timestep_clamp(it, 10, cfl, dev.cfl_dtp(), t, 1e30)
+            dev.tstep(dt)
+            t += dt
+        dev.sync()
+        dev.upload_u(u0)
+        dev.boundary_u()
+        dev.sync()
+        dev.kernel_times()                      # reset counters
+        dev.set_timing(True)
+        t = 0.0
+        t0 = time.perf_counter()
+        for it in range(1, args.steps + 1):
+            dt, _ = timestep_clamp(it, 10, cfl, dev.cfl_dtp(), t, 1e30)
+            dev.tstep(dt)
+            t += dt
+        dev.sync()
+        el = time.perf_counter() - t0
+        kt = dev.kernel_times()
+        dev.set_timing(False)
+
+    zones = n ** 3 * args.steps
+    value = zones / el
+    ms_per_step = 1e3 * el / args.steps
+    ms, launches = kt["flux_corrector"]
+    avg_s = ms / launches / 1e3
+    faces = (n + 1) * n * n
+    # per launch: 8 primitives read per cell + 8 flux components written per face
+    alg_bytes = 128.0 * faces
+    achieved = alg_bytes / avg_s / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_flux<HLLD,2> (limited reconstruction + HLLD, one axis per launch)",
+            "avg_launch_ms": round(ms / launches, 4)}
+    step_ach = BYTES_PER_ZONE_FP64 * n ** 3 / (el / args.steps) / 1e9
+    out = {
+        "metric": "zone-updates/s", "value": round(value, 1), "unit": "zone-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"turbulence-{n}^3-fp64-hlld-minmod-fluxcd-cfl0.4", "grid": [n, n, n],
+                   "solver": "HLLD", "limiter": "minmod", "flux_cd": True, "precision": "fp64",
+                   "boundaries": "periodic", "seed": args.seed},
+        "roofline": roof,
+        "step_roofline": {"bytes_per_zone_update": BYTES_PER_ZONE_FP64, "achieved": round(step_ach, 1),
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_ach / HBM_PEAK_GBS, 4)},
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items() if v[1]},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,191 @@
+"""ctypes binding of the C ABI in include/guacho_amd.h.
+
+The product path: every compute call goes to guacho_amd/libguacho_amd.so.
+There is no fallback; a missing library raises at load time.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libguacho_amd.so")
+
+GM_OK, GM_ERR_CONFIG, GM_ERR_HIP, GM_ERR_NONFINITE, GM_ERR_OOM, GM_ERR_ARG = 0, 1, 2, 3, 4, 5
+GM_PRECISION_FP64, GM_PRECISION_MIXED = 0, 1
+GM_NKCLASS = 8
+
+# exported entry points, in the order include/guacho_amd.h declares them
+EXPORTS = (
+    "gm_create", "gm_destroy", "gm_last_error", "gm_upload_u", "gm_download_u",
+    "gm_download_primit", "gm_boundary_u", "gm_cfl_dtp", "gm_tstep", "gm_sync",
+    "gm_download_divb", "gm_set_timing", "gm_kernel_times", "gm_device_bytes",
+)
+
+
+class GmError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+        self.message = message
+
+
+class GmParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("neq", C.c_int32), ("nghost", C.c_int32),
+        ("riemann_solver", C.c_int32), ("slope_limiter", C.c_int32),
+        ("bc", C.c_int32 * 6),
+        ("mhd", C.c_int32), ("pmhd", C.c_int32),
+        ("enable_flux_cd", C.c_int32), ("eight_wave", C.c_int32),
+        ("precision", C.c_int32), ("device", C.c_int32), ("reserved", C.c_int32),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
+        ("cv", C.c_double), ("gamma", C.c_double), ("eta", C.c_double),
+    ]
+
+
+_LIB: Optional[C.CDLL] = None
+
+
+def lib() -> C.CDLL:
+    """Load libguacho_amd.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C guacho_amd` "
+                           "(or __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
+    L.gm_create.argtypes = [C.POINTER(GmParams), C.POINTER(vp)]
+    L.gm_destroy.argtypes = [vp]
+    L.gm_last_error.argtypes = [vp]
+    L.gm_last_error.restype = C.c_char_p
+    L.gm_upload_u.argtypes = [vp, dp]
+    L.gm_download_u.argtypes = [vp, dp]
+    L.gm_download_primit.argtypes = [vp, dp, C.c_void_p]
+    L.gm_boundary_u.argtypes = [vp]
+    L.gm_cfl_dtp.argtypes = [vp, C.POINTER(C.c_double)]
+    L.gm_tstep.argtypes = [vp, C.c_double]
+    L.gm_sync.argtypes = [vp]
+    L.gm_download_divb.argtypes = [vp, dp]
+    L.gm_set_timing.argtypes = [vp, C.c_int]
+    L.gm_kernel_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_char_p)]
+    L.gm_device_bytes.argtypes = [vp, C.POINTER(C.c_int64)]
+    for name in EXPORTS:
+        if name != "gm_last_error":
+            getattr(L, name).restype = C.c_int
+    _LIB = L
+    return L
+
+
+def make_params(nx: int, ny: int, nz: int, dx: float, dy: float, dz: float, *, cv: float = 1.5,
+                gamma: Optional[float] = None, riemann_solver: int = 4, slope_limiter: int = 1,
+                bc: Sequence[int] = (3, 3, 3, 3, 3, 3), mhd: bool = True, pmhd: bool = False,
+                enable_flux_cd: bool = True, eight_wave: bool = False,
+                precision: int = GM_PRECISION_FP64, device: int = 0, eta: float = 0.0) -> GmParams:
+    p = GmParams()
+    p.nx, p.ny, p.nz = nx, ny, nz
+    p.neq, p.nghost = 8, 2
+    p.riemann_solver, p.slope_limiter = riemann_solver, slope_limiter
+    for b in range(6):
+        p.bc[b] = int(bc[b])
+    p.mhd, p.pmhd = int(mhd), int(pmhd)
+    p.enable_flux_cd, p.eight_wave = int(enable_flux_cd), int(eight_wave)
+    p.precision, p.device, p.reserved = precision, device, 0
+    p.dx, p.dy, p.dz = dx, dy, dz
+    p.cv = cv
+    p.gamma = (cv + 1.0) / cv if gamma is None else gamma
+    p.eta = eta
+    return p
+
+
+class Subdomain:
+    """One device-resident subdomain (a gm_ctx)."""
+
+    def __init__(self, params: GmParams):
+        self.L = lib()
+        self.params = params
+        h = C.c_void_p()
+        rc = self.L.gm_create(C.byref(params), C.byref(h))
+        if rc != GM_OK:
+            raise GmError(rc, self.L.gm_last_error(None).decode())
+        self.h = h
+        self.shape = (8, params.nx + 4, params.ny + 4, params.nz + 4)
+
+    def _check(self, rc: int):
+        if rc != GM_OK:
+            raise GmError(rc, self.L.gm_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.gm_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload_u(self, u: np.ndarray):
+        u = np.asfortranarray(u, dtype=np.float64)
+        if u.shape != self.shape:
+            raise ValueError(f"u has shape {u.shape}, expected {self.shape}")
+        self._check(self.L.gm_upload_u(self.h, u))
+
+    def download_u(self) -> np.ndarray:
+        u = np.zeros(self.shape, dtype=np.float64, order="F")
+        self._check(self.L.gm_download_u(self.h, u))
+        return u
+
+    def download_primit(self, with_temp: bool = False):
+        p = np.zeros(self.shape, dtype=np.float64, order="F")
+        t = np.zeros(self.shape[1:], dtype=np.float64, order="F") if with_temp else None
+        self._check(self.L.gm_download_primit(self.h, p, t.ctypes.data if t is not None else None))
+        return (p, t) if with_temp else p
+
+    def boundary_u(self):
+        self._check(self.L.gm_boundary_u(self.h))
+
+    def cfl_dtp(self) -> float:
+        v = C.c_double()
+        self._check(self.L.gm_cfl_dtp(self.h, C.byref(v)))
+        return v.value
+
+    def tstep(self, dt: float):
+        self._check(self.L.gm_tstep(self.h, float(dt)))
+
+    def sync(self):
+        self._check(self.L.gm_sync(self.h))
+
+    def divb(self) -> np.ndarray:
+        p = self.params
+        d = np.zeros((p.nx, p.ny, p.nz), dtype=np.float64, order="F")
+        self._check(self.L.gm_download_divb(self.h, d))
+        return d
+
+    def set_timing(self, enable: bool):
+        self._check(self.L.gm_set_timing(self.h, int(enable)))
+
+    def kernel_times(self) -> Dict[str, tuple]:
+        ms = (C.c_double * GM_NKCLASS)()
+        n = (C.c_int64 * GM_NKCLASS)()
+        names = (C.c_char_p * GM_NKCLASS)()
+        self._check(self.L.gm_kernel_times(self.h, ms, n, names))
+        return {names[k].decode(): (ms[k], n[k]) for k in range(GM_NKCLASS)}
+
+    def device_bytes(self) -> int:
+        v = C.c_int64()
+        self._check(self.L.gm_device_bytes(self.h, C.byref(v)))
+        return v.value

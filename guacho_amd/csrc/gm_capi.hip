@@ -1,0 +1,435 @@
+// gm_capi.hip -- the C ABI of include/guacho_amd.h over the step kernels.
+//
+// A context owns one subdomain's device state and one HIP stream.  The
+// step (gm_tstep) follows tstep (src/hydro_solver.f90:134-229):
+//   predictor: primitives of U^n -> first-order fluxes -> E -> U* = U^n - dt/2 div F
+//   corrector: primitives of U*  -> limited fluxes     -> E -> U^{n+1} = U^n - dt div F
+// with ghosts resolved by boundary mapping inside the kernels, so neither
+// boundaryII nor boundaryI runs on the step path; gm_download_u and
+// gm_boundary_u materialise boundaryI's ghost layer when a caller needs it.
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gm_kernels.hpp"
+#include "guacho_amd.h"
+
+namespace {
+
+const char* kClassNames[GM_NKCLASS] = {"prim", "flux_predictor", "flux_corrector", "efield",
+                                       "update", "cfl_dt", "ghost", "layout"};
+enum { KC_PRIM, KC_FLUX1, KC_FLUX2, KC_EFIELD, KC_UPDATE, KC_CFL, KC_GHOST, KC_LAYOUT };
+
+thread_local std::string g_create_error;
+
+struct Scalars {
+  unsigned long long dtmin;
+  int err;
+  int pad;
+};
+
+__global__ void k_init_dtmin(Scalars* s) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) s->dtmin = (unsigned long long)__double_as_longlong(1e30);
+}
+
+__global__ void k_clear_err(Scalars* s) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) s->err = 0;
+}
+
+}  // namespace
+
+struct gm_ctx {
+  gm_params p;
+  gm::Geom g;
+  gm::Phys ph;
+  hipStream_t stream = nullptr;
+  double* U = nullptr;   // state U^n (SoA, 8 vars, ghosts as uploaded / filled)
+  double* Us = nullptr;  // U* after the predictor
+  double* P = nullptr;   // primitives; also the staging buffer for layout conversion
+  double* F[3] = {nullptr, nullptr, nullptr};
+  double* E = nullptr;   // cell-centred E (3 vars); also staging for Temp / divB
+  Scalars* dscal = nullptr;  // device scalars
+  Scalars* hscal = nullptr;  // pinned host mirror
+  bool ghosts_dirty = true;
+  int timing = 0;
+  struct Rec { int cls; hipEvent_t a, b; };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  double ms[GM_NKCLASS] = {0};
+  int64_t launches[GM_NKCLASS] = {0};
+  int64_t bytes = 0;
+  std::string msg;
+};
+
+namespace {
+
+int fail(gm_ctx* c, int code, const std::string& m) {
+  if (c) c->msg = m;
+  else g_create_error = m;
+  return code;
+}
+
+int hip_check(gm_ctx* c, hipError_t e, const char* what) {
+  if (e == hipSuccess) return GM_OK;
+  return fail(c, GM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define GM_HIP(c, call)                                   \
+  do {                                                    \
+    int _rc = hip_check((c), (call), #call);              \
+    if (_rc != GM_OK) return _rc;                         \
+  } while (0)
+
+hipEvent_t take_event(gm_ctx* c) {
+  if (!c->pool.empty()) {
+    hipEvent_t e = c->pool.back();
+    c->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+template <class Fn>
+void timed(gm_ctx* c, int cls, Fn&& fn) {
+  if (!c->timing) {
+    fn();
+    return;
+  }
+  hipEvent_t a = take_event(c), b = take_event(c);
+  if (!a || !b) {
+    fn();
+    return;
+  }
+  (void)hipEventRecord(a, c->stream);
+  fn();
+  (void)hipEventRecord(b, c->stream);
+  c->recs.push_back({cls, a, b});
+}
+
+int validate(const gm_params* p) {
+  if (!p) return fail(nullptr, GM_ERR_ARG, "params is NULL");
+  if (p->neq != 8) return fail(nullptr, GM_ERR_CONFIG, "neq must be 8 (BFIELD, no passive scalars)");
+  if (p->nghost != 2) return fail(nullptr, GM_ERR_CONFIG, "nghost must be 2");
+  if (p->nx < 2 || p->ny < 2 || p->nz < 2)
+    return fail(nullptr, GM_ERR_CONFIG, "nx, ny, nz must each be at least 2 (two-layer ghost mirror)");
+  if ((long long)(p->nx + 4) * (p->ny + 4) * (p->nz + 4) > (1LL << 36))
+    return fail(nullptr, GM_ERR_CONFIG, "grid too large");
+  if (p->riemann_solver < GM_SOLVER_HLL || p->riemann_solver > GM_SOLVER_HLLD)
+    return fail(nullptr, GM_ERR_CONFIG, "riemann_solver must be HLL, HLLC, HLLE or HLLD (split solvers not built)");
+  if (!p->mhd && !p->pmhd)
+    return fail(nullptr, GM_ERR_CONFIG, "pure hydro is a neq=5 build in the reference; set mhd or pmhd");
+  if (p->mhd && p->pmhd) return fail(nullptr, GM_ERR_CONFIG, "mhd and pmhd are exclusive");
+  if (p->riemann_solver == GM_SOLVER_HLLC && p->mhd)
+    return fail(nullptr, GM_ERR_CONFIG,
+                "HLLC with full MHD leaves the star-state field undefined in the reference (hllc.f90:89-93); use pmhd");
+  if (p->slope_limiter < GM_LIMITER_NO_AVERAGE || p->slope_limiter > GM_LIMITER_SUPERBEE)
+    return fail(nullptr, GM_ERR_CONFIG, "slope_limiter out of range");
+  for (int b = 0; b < 6; b++) {
+    if (p->bc[b] == GM_BC_OTHER)
+      return fail(nullptr, GM_ERR_CONFIG, "BC_OTHER / bc_user needs the user's impose_user_bc hook on the host; not supported on device");
+    if (p->bc[b] < GM_BC_OUTFLOW || p->bc[b] > GM_BC_PERIODIC)
+      return fail(nullptr, GM_ERR_CONFIG, "boundary kind out of range");
+  }
+  for (int a = 0; a < 3; a++)
+    if ((p->bc[2 * a] == GM_BC_PERIODIC) != (p->bc[2 * a + 1] == GM_BC_PERIODIC))
+      return fail(nullptr, GM_ERR_CONFIG, "a periodic boundary must pair with a periodic boundary on the same axis");
+  if (p->eight_wave) return fail(nullptr, GM_ERR_CONFIG, "eight_wave (Powell sources) is not built");
+  if (p->eta != 0.0) return fail(nullptr, GM_ERR_CONFIG, "eta (artificial viscosity) must be 0");
+  if (p->precision != GM_PRECISION_FP64)
+    return fail(nullptr, GM_ERR_CONFIG, "only GM_PRECISION_FP64 is available in this build");
+  if (!(p->dx > 0) || !(p->dy > 0) || !(p->dz > 0) || !std::isfinite(p->dx) || !std::isfinite(p->dy) ||
+      !std::isfinite(p->dz))
+    return fail(nullptr, GM_ERR_CONFIG, "dx, dy, dz must be positive and finite");
+  if (!(p->cv > 0) || !(p->gamma > 0) || !std::isfinite(p->cv) || !std::isfinite(p->gamma))
+    return fail(nullptr, GM_ERR_CONFIG, "cv and gamma must be positive and finite");
+  if (p->reserved != 0) return fail(nullptr, GM_ERR_CONFIG, "reserved must be 0");
+  return GM_OK;
+}
+
+void release(gm_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& r : c->recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (auto e : c->pool) (void)hipEventDestroy(e);
+  double* bufs[] = {c->U, c->Us, c->P, c->F[0], c->F[1], c->F[2], c->E};
+  for (double* b : bufs)
+    if (b) (void)hipFree(b);
+  if (c->dscal) (void)hipFree(c->dscal);
+  if (c->hscal) (void)hipHostFree(c->hscal);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int set_device(gm_ctx* c) { return hip_check(c, hipSetDevice(c->p.device), "hipSetDevice"); }
+
+// boundaryI ghost layer on U, in the reference's serial order: periodic
+// x, y, z; then closed left..in; then outflow left..in.
+void fill_ghost1(gm_ctx* c) {
+  const gm::Geom& g = c->g;
+  timed(c, KC_GHOST, [&] {
+    for (int a = 0; a < 3; a++)
+      if (g.bc[2 * a] == GM_BC_PERIODIC) {
+        gm::launch_ghost1(c->stream, g, c->U, a, 0, GM_BC_PERIODIC);
+        gm::launch_ghost1(c->stream, g, c->U, a, 1, GM_BC_PERIODIC);
+      }
+    for (int kind : {GM_BC_CLOSED, GM_BC_OUTFLOW})
+      for (int f = 0; f < 6; f++)
+        if (g.bc[f] == kind) gm::launch_ghost1(c->stream, g, c->U, f / 2, f % 2, kind);
+  });
+  c->ghosts_dirty = false;
+}
+
+int read_errors(gm_ctx* c) {
+  GM_HIP(c, hipMemcpyAsync(&c->hscal->err, &c->dscal->err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  GM_HIP(c, hipStreamSynchronize(c->stream));
+  const int e = c->hscal->err;
+  if (e) {
+    hipLaunchKernelGGL(k_clear_err, dim3(1), dim3(1), 0, c->stream, c->dscal);
+    GM_HIP(c, hipGetLastError());
+    return fail(c, GM_ERR_NONFINITE,
+                (e & 1) ? "Riemann solver fell through on a non-finite state (hlld.f90:316-317)"
+                        : "non-finite value in the conserved state");
+  }
+  return GM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gm_create(const gm_params* params, gm_ctx** out) {
+  if (!out) return fail(nullptr, GM_ERR_ARG, "ctx out-pointer is NULL");
+  *out = nullptr;
+  int rc = validate(params);
+  if (rc != GM_OK) return rc;
+  gm_ctx* c = new gm_ctx();
+  c->p = *params;
+  gm::Geom& g = c->g;
+  g.nx = params->nx; g.ny = params->ny; g.nz = params->nz;
+  g.NX = g.nx + 4; g.NY = g.ny + 4; g.NZ = g.nz + 4;
+  g.ncell = (long long)g.NX * g.NY * g.NZ;
+  for (int b = 0; b < 6; b++) g.bc[b] = params->bc[b];
+  c->ph.cv = params->cv;
+  c->ph.gamma = params->gamma;
+  c->ph.mhd = params->mhd ? 1 : 0;
+  c->ph.pmhd = params->pmhd ? 1 : 0;
+  c->ph.limiter = params->slope_limiter;
+
+  auto bail = [&](int code) {
+    g_create_error = c->msg;
+    release(c);
+    return code;
+  };
+  if ((rc = set_device(c)) != GM_OK) return bail(rc);
+  if ((rc = hip_check(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate")) != GM_OK)
+    return bail(rc);
+  const size_t n8 = (size_t)g.ncell * 8 * sizeof(double);
+  const size_t n3 = (size_t)g.ncell * 3 * sizeof(double);
+  double** bufs8[] = {&c->U, &c->Us, &c->P, &c->F[0], &c->F[1], &c->F[2]};
+  for (double** b : bufs8) {
+    if (hipMalloc((void**)b, n8) != hipSuccess) {
+      (void)hipGetLastError();
+      c->msg = "device allocation failed";
+      return bail(GM_ERR_OOM);
+    }
+    c->bytes += (int64_t)n8;
+  }
+  if (hipMalloc((void**)&c->E, n3) != hipSuccess) {
+    (void)hipGetLastError();
+    c->msg = "device allocation failed";
+    return bail(GM_ERR_OOM);
+  }
+  c->bytes += (int64_t)n3;
+  if ((rc = hip_check(c, hipMalloc((void**)&c->dscal, sizeof(Scalars)), "hipMalloc")) != GM_OK) return bail(rc);
+  if ((rc = hip_check(c, hipHostMalloc((void**)&c->hscal, sizeof(Scalars), hipHostMallocDefault),
+                      "hipHostMalloc")) != GM_OK)
+    return bail(rc);
+  // zero everything once so no kernel ever reads uninitialised memory
+  for (double** b : bufs8) (void)hipMemsetAsync(*b, 0, n8, c->stream);
+  (void)hipMemsetAsync(c->E, 0, n3, c->stream);
+  (void)hipMemsetAsync(c->dscal, 0, sizeof(Scalars), c->stream);
+  if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize")) != GM_OK) return bail(rc);
+  *out = c;
+  return GM_OK;
+}
+
+int gm_destroy(gm_ctx* c) {
+  if (!c) return GM_ERR_ARG;
+  (void)hipSetDevice(c->p.device);
+  release(c);
+  return GM_OK;
+}
+
+const char* gm_last_error(const gm_ctx* c) { return c ? c->msg.c_str() : g_create_error.c_str(); }
+
+int gm_upload_u(gm_ctx* c, const double* u) {
+  if (!c) return GM_ERR_ARG;
+  if (!u) return fail(c, GM_ERR_ARG, "u is NULL");
+  int rc = set_device(c);
+  if (rc) return rc;
+  const size_t n8 = (size_t)c->g.ncell * 8 * sizeof(double);
+  GM_HIP(c, hipMemcpyAsync(c->P, u, n8, hipMemcpyHostToDevice, c->stream));
+  timed(c, KC_LAYOUT, [&] { gm::launch_aos_to_soa(c->stream, c->g.ncell, c->P, c->U); });
+  GM_HIP(c, hipGetLastError());
+  GM_HIP(c, hipStreamSynchronize(c->stream));
+  c->ghosts_dirty = false;  // the caller's ghosts are the reference's ghosts
+  return GM_OK;
+}
+
+int gm_boundary_u(gm_ctx* c) {
+  if (!c) return GM_ERR_ARG;
+  int rc = set_device(c);
+  if (rc) return rc;
+  fill_ghost1(c);
+  GM_HIP(c, hipGetLastError());
+  return GM_OK;
+}
+
+int gm_download_u(gm_ctx* c, double* u) {
+  if (!c) return GM_ERR_ARG;
+  if (!u) return fail(c, GM_ERR_ARG, "u is NULL");
+  int rc = set_device(c);
+  if (rc) return rc;
+  if (c->ghosts_dirty) fill_ghost1(c);
+  timed(c, KC_LAYOUT, [&] { gm::launch_soa_to_aos(c->stream, c->g.ncell, 8, c->U, c->P); });
+  GM_HIP(c, hipGetLastError());
+  GM_HIP(c, hipMemcpyAsync(u, c->P, (size_t)c->g.ncell * 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  GM_HIP(c, hipStreamSynchronize(c->stream));
+  return GM_OK;
+}
+
+int gm_download_primit(gm_ctx* c, double* primit, double* temp) {
+  if (!c) return GM_ERR_ARG;
+  if (!primit) return fail(c, GM_ERR_ARG, "primit is NULL");
+  int rc = set_device(c);
+  if (rc) return rc;
+  if (c->ghosts_dirty) fill_ghost1(c);
+  double* T = temp ? c->E : nullptr;
+  timed(c, KC_PRIM, [&] { gm::launch_prim_all(c->stream, c->g.ncell, c->ph, c->U, c->P, T); });
+  GM_HIP(c, hipGetLastError());
+  GM_HIP(c, hipMemcpyAsync(primit, c->P, (size_t)c->g.ncell * 8 * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  if (temp)
+    GM_HIP(c, hipMemcpyAsync(temp, c->E, (size_t)c->g.ncell * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  GM_HIP(c, hipStreamSynchronize(c->stream));
+  return GM_OK;
+}
+
+int gm_download_divb(gm_ctx* c, double* divb) {
+  if (!c) return GM_ERR_ARG;
+  if (!divb) return fail(c, GM_ERR_ARG, "divb is NULL");
+  int rc = set_device(c);
+  if (rc) return rc;
+  gm::launch_divb(c->stream, c->g, c->U, c->p.dx, c->p.dy, c->p.dz, c->E);
+  GM_HIP(c, hipGetLastError());
+  const size_t n = (size_t)c->g.nx * c->g.ny * c->g.nz;
+  GM_HIP(c, hipMemcpyAsync(divb, c->E, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  GM_HIP(c, hipStreamSynchronize(c->stream));
+  return GM_OK;
+}
+
+int gm_cfl_dtp(gm_ctx* c, double* dtp) {
+  if (!c) return GM_ERR_ARG;
+  if (!dtp) return fail(c, GM_ERR_ARG, "dtp is NULL");
+  int rc = set_device(c);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_init_dtmin, dim3(1), dim3(1), 0, c->stream, c->dscal);
+  timed(c, KC_CFL, [&] {
+    gm::launch_cfl(c->stream, c->g, c->ph, c->U, c->p.dx, c->p.dy, c->p.dz, &c->dscal->dtmin, &c->dscal->err);
+  });
+  GM_HIP(c, hipGetLastError());
+  GM_HIP(c, hipMemcpyAsync(c->hscal, c->dscal, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
+  GM_HIP(c, hipStreamSynchronize(c->stream));
+  const unsigned long long bits = c->hscal->dtmin;
+  std::memcpy(dtp, &bits, sizeof(double));
+  if (c->hscal->err) return read_errors(c);
+  return GM_OK;
+}
+
+int gm_tstep(gm_ctx* c, double dt) {
+  if (!c) return GM_ERR_ARG;
+  if (!std::isfinite(dt)) return fail(c, GM_ERR_ARG, "dt is not finite");
+  int rc = set_device(c);
+  if (rc) return rc;
+  const gm::Geom& g = c->g;
+  const gm_params& p = c->p;
+  hipStream_t s = c->stream;
+  int* err = &c->dscal->err;
+  const double dtm = dt / 2.0;  // hydro_solver.f90:152
+  // predictor: first-order fluxes of U^n, U* = U^n - dt/2 div F
+  timed(c, KC_PRIM, [&] { gm::launch_prim(s, g, c->ph, c->U, c->P, 0); });
+  for (int a = 0; a < 3; a++)
+    timed(c, KC_FLUX1, [&] { gm::launch_flux(s, g, c->ph, p.riemann_solver, 1, a, c->P, c->F[a], err); });
+  if (p.enable_flux_cd) timed(c, KC_EFIELD, [&] { gm::launch_efield(s, g, c->F[0], c->F[1], c->F[2], c->E); });
+  timed(c, KC_UPDATE, [&] {
+    gm::launch_update(s, g, c->U, c->Us, c->F[0], c->F[1], c->F[2], c->E, dtm, p.dx, p.dy, p.dz,
+                      p.enable_flux_cd);
+  });
+  // corrector: limited fluxes of U*, U^{n+1} = U^n - dt div F (in place;
+  // viscous_copy with eta = 0 is the identity on values)
+  timed(c, KC_PRIM, [&] { gm::launch_prim(s, g, c->ph, c->Us, c->P, -1); });
+  for (int a = 0; a < 3; a++)
+    timed(c, KC_FLUX2, [&] { gm::launch_flux(s, g, c->ph, p.riemann_solver, 2, a, c->P, c->F[a], err); });
+  if (p.enable_flux_cd) timed(c, KC_EFIELD, [&] { gm::launch_efield(s, g, c->F[0], c->F[1], c->F[2], c->E); });
+  timed(c, KC_UPDATE, [&] {
+    gm::launch_update(s, g, c->U, c->U, c->F[0], c->F[1], c->F[2], c->E, dt, p.dx, p.dy, p.dz, p.enable_flux_cd);
+  });
+  c->ghosts_dirty = true;
+  GM_HIP(c, hipGetLastError());
+  return GM_OK;
+}
+
+int gm_sync(gm_ctx* c) {
+  if (!c) return GM_ERR_ARG;
+  int rc = set_device(c);
+  if (rc) return rc;
+  GM_HIP(c, hipStreamSynchronize(c->stream));
+  return read_errors(c);
+}
+
+int gm_set_timing(gm_ctx* c, int enable) {
+  if (!c) return GM_ERR_ARG;
+  c->timing = enable ? 1 : 0;
+  return GM_OK;
+}
+
+int gm_kernel_times(gm_ctx* c, double* ms, int64_t* launches, const char** names) {
+  if (!c) return GM_ERR_ARG;
+  int rc = set_device(c);
+  if (rc) return rc;
+  GM_HIP(c, hipStreamSynchronize(c->stream));
+  for (auto& r : c->recs) {
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
+      c->ms[r.cls] += t;
+      c->launches[r.cls] += 1;
+    }
+    c->pool.push_back(r.a);
+    c->pool.push_back(r.b);
+  }
+  c->recs.clear();
+  for (int k = 0; k < GM_NKCLASS; k++) {
+    if (ms) ms[k] = c->ms[k];
+    if (launches) launches[k] = c->launches[k];
+    if (names) names[k] = kClassNames[k];
+    c->ms[k] = 0;
+    c->launches[k] = 0;
+  }
+  return GM_OK;
+}
+
+int gm_device_bytes(const gm_ctx* c, int64_t* bytes) {
+  if (!c || !bytes) return GM_ERR_ARG;
+  *bytes = c->bytes;
+  return GM_OK;
+}
+
+}  // extern "C"

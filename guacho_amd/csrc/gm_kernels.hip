@@ -1,0 +1,459 @@
+// gm_kernels.hip -- gfx950 kernels of the Godunov MHD step.
+//
+// Device layout: structure of arrays.  Variable q of the conserved state
+// lives at base + q*ncell; cell (i, j, k), with the reference's 1-based
+// indices and two ghost layers (i in -1..nx+2), sits at
+//   c = ((k+1)*NY + (j+1))*NX + (i+1),   NX = nx+4, NY = ny+4.
+// x is fastest, so a wavefront of 64 consecutive i reads 512 contiguous
+// bytes per variable.
+//
+// The step never writes ghost cells of the state.  Where a stencil reaches
+// past the domain, the kernels read the interior cell the reference's ghost
+// fill would have copied there (boundaryI: src/boundaries.f90:44-247 for
+// U^n; boundaryII: :256-512 for U*; boundaryI_ef:
+// src/flux_cd_module.f90:45-237 for E), negating the momentum or E
+// component that the closed boundary flips.  No interior update reads an
+// edge or corner ghost, so a per-axis mapping is exact.
+#include "gm_kernels.hpp"
+
+namespace gm {
+
+__device__ __forceinline__ long long cidx(const Geom& g, int i, int j, int k) {
+  return ((long long)(k + 1) * g.NY + (j + 1)) * g.NX + (i + 1);
+}
+
+// Interior source of ghost index i on an axis of n cells; flips when the
+// boundary on that side is closed.  Valid for i in -1..n+2 (n >= 2).
+__device__ __forceinline__ int map_ghost(int i, int n, int lo, int hi, bool& flip) {
+  if (i < 1) {
+    if (lo == BC_PERIODIC) return i + n;
+    flip = (lo == BC_CLOSED);
+    return 1 - i;
+  }
+  if (i > n) {
+    if (hi == BC_PERIODIC) return i - n;
+    flip = (hi == BC_CLOSED);
+    return 2 * n + 1 - i;
+  }
+  return i;
+}
+
+// Conserved 8-vector of cell (i,j,k), ghosts resolved through the
+// boundary conditions.
+__device__ __forceinline__ void load_cons(const Geom& g, const double* __restrict__ U, int i, int j, int k,
+                                          double u[8]) {
+  bool fx = false, fy = false, fz = false;
+  const int si = map_ghost(i, g.nx, g.bc[0], g.bc[1], fx);
+  const int sj = map_ghost(j, g.ny, g.bc[2], g.bc[3], fy);
+  const int sk = map_ghost(k, g.nz, g.bc[4], g.bc[5], fz);
+  const long long c = cidx(g, si, sj, sk);
+#pragma unroll
+  for (int q = 0; q < 8; q++) u[q] = U[q * g.ncell + c];
+  if (fx) u[1] = -u[1];
+  if (fy) u[2] = -u[2];
+  if (fz) u[3] = -u[3];
+}
+
+// ---------------------------------------------------------------------------
+// primitives: calcprim over the cells the flux stencils read
+// (src/hydro_core.f90:245-320), ghost values by boundary mapping.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_prim(Geom g, Phys ph, const double* __restrict__ U,
+                                              double* __restrict__ P, int lo) {
+  // cells lo..n+1-lo on each axis (lo = 0: one ghost layer, lo = -1: two)
+  const int ex = g.nx + 2 - 2 * lo, ey = g.ny + 2 - 2 * lo, ez = g.nz + 2 - 2 * lo;
+  const long long n = (long long)ex * ey * ez;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % ex) + lo;
+    const long long r = t / ex;
+    const int j = (int)(r % ey) + lo;
+    const int k = (int)(r / ey) + lo;
+    // only face ghosts are read downstream; skip edges and corners
+    const int out = (i < 1 || i > g.nx) + (j < 1 || j > g.ny) + (k < 1 || k > g.nz);
+    if (out > 1) continue;
+    double u[8], p[8];
+    load_cons(g, U, i, j, k, u);
+    u2prim(ph, u, p);
+    const long long c = cidx(g, i, j, k);
+#pragma unroll
+    for (int q = 0; q < 8; q++) P[q * g.ncell + c] = p[q];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// interface fluxes: the sweep of hlldfluxes / hllefluxes / hllfluxes /
+// hllcfluxes (e.g. src/hlld.f90:331-432) for one axis.  Face f(:,i,j,k)
+// lies between cells i and i+1 (i = 0..n along the axis); only faces with
+// interior transverse indices feed the update.
+// ---------------------------------------------------------------------------
+template <int SOLVER, int ORDER>
+__global__ void __launch_bounds__(256) k_flux(Geom g, Phys ph, int axis, const double* __restrict__ P,
+                                              double* __restrict__ F, int* __restrict__ err) {
+  const int na = axis == 0 ? g.nx : axis == 1 ? g.ny : g.nz;
+  // face grid: along-axis index a in 0..na, transverse interior
+  const int e0 = axis == 0 ? na + 1 : g.nx;
+  const int e1 = axis == 1 ? na + 1 : g.ny;
+  const int e2 = axis == 2 ? na + 1 : g.nz;
+  const long long n = (long long)e0 * e1 * e2;
+  const long long stride = axis == 0 ? 1 : axis == 1 ? (long long)g.NX : (long long)g.NX * g.NY;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    int i = (int)(t % e0);
+    const long long r = t / e0;
+    int j = (int)(r % e1);
+    int k = (int)(r / e1);
+    if (axis != 0) i += 1;
+    if (axis != 1) j += 1;
+    if (axis != 2) k += 1;
+    const long long c = cidx(g, i, j, k);
+    double pl[8], pr[8], ff[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      pl[q] = P[q * g.ncell + c];
+      pr[q] = P[q * g.ncell + c + stride];
+    }
+    if constexpr (ORDER == 2) {
+      double pll[8], prr[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        pll[q] = P[q * g.ncell + c - stride];
+        prr[q] = P[q * g.ncell + c + 2 * stride];
+      }
+      limiter(ph.limiter, pll, pl, pr, prr);
+    }
+    axis_perm(axis, pl);
+    axis_perm(axis, pr);
+    const bool ok = riemann<SOLVER>(ph, pl, pr, ff);
+    if (!ok) atomicOr(err, 1);
+    axis_perm(axis, ff);
+#pragma unroll
+    for (int q = 0; q < 8; q++) F[q * g.ncell + c] = ff[q];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// cell-centred electric field, get_efield (src/flux_cd_module.f90:245-273)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_efield(Geom g, const double* __restrict__ Fx,
+                                                const double* __restrict__ Fy,
+                                                const double* __restrict__ Fz, double* __restrict__ E) {
+  const long long n = (long long)g.nx * g.ny * g.nz;
+  const long long sx = 1, sy = g.NX, sz = (long long)g.NX * g.NY;
+  const long long N = g.ncell;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % g.nx) + 1;
+    const long long r = t / g.nx;
+    const int j = (int)(r % g.ny) + 1;
+    const int k = (int)(r / g.ny) + 1;
+    const long long c = cidx(g, i, j, k);
+    E[0 * N + c] = 0.25 * (((-Fy[7 * N + c - sy] - Fy[7 * N + c]) + Fz[6 * N + c - sz]) + Fz[6 * N + c]);
+    E[1 * N + c] = 0.25 * (((+Fx[7 * N + c - sx] + Fx[7 * N + c]) - Fz[5 * N + c - sz]) - Fz[5 * N + c]);
+    E[2 * N + c] = 0.25 * (((-Fx[6 * N + c - sx] - Fx[6 * N + c]) + Fy[5 * N + c - sy]) + Fy[5 * N + c]);
+  }
+}
+
+// E component q at cell (i,j,k) where at most one index is a ghost:
+// boundaryI_ef semantics (periodic copy; closed flips e1 on x faces and e2
+// on y faces, flux_cd_module.f90:145-192; outflow copies).
+__device__ __forceinline__ double load_e(const Geom& g, const double* __restrict__ E, int q, int i, int j,
+                                         int k) {
+  bool flip = false;
+  if (i < 1 || i > g.nx) {
+    const int side = i < 1 ? g.bc[0] : g.bc[1];
+    if (side == BC_PERIODIC) i = i < 1 ? i + g.nx : i - g.nx;
+    else { flip = (side == BC_CLOSED) && q == 0; i = i < 1 ? 1 : g.nx; }
+  } else if (j < 1 || j > g.ny) {
+    const int side = j < 1 ? g.bc[2] : g.bc[3];
+    if (side == BC_PERIODIC) j = j < 1 ? j + g.ny : j - g.ny;
+    else { flip = (side == BC_CLOSED) && q == 1; j = j < 1 ? 1 : g.ny; }
+  } else if (k < 1 || k > g.nz) {
+    const int side = k < 1 ? g.bc[4] : g.bc[5];
+    if (side == BC_PERIODIC) k = k < 1 ? k + g.nz : k - g.nz;
+    else { k = k < 1 ? 1 : g.nz; }
+  }
+  const double v = E[q * g.ncell + cidx(g, i, j, k)];
+  return flip ? -v : v;
+}
+
+// ---------------------------------------------------------------------------
+// upwind update, step(dt) (src/hydro_solver.f90:77-127) with flux_cd_update
+// (src/flux_cd_module.f90:285-323).  Uout may alias Uin (each cell reads
+// only its own conserved state).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_update(Geom g, const double* Uin, double* Uout,
+                                                const double* __restrict__ Fx,
+                                                const double* __restrict__ Fy,
+                                                const double* __restrict__ Fz,
+                                                const double* __restrict__ E, double dt, double dx,
+                                                double dy, double dz, int flux_cd) {
+  const long long n = (long long)g.nx * g.ny * g.nz;
+  const long long sx = 1, sy = g.NX, sz = (long long)g.NX * g.NY;
+  const long long N = g.ncell;
+  const double dtdx = dt / dx, dtdy = dt / dy, dtdz = dt / dz;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % g.nx) + 1;
+    const long long r = t / g.nx;
+    const int j = (int)(r % g.ny) + 1;
+    const int k = (int)(r / g.ny) + 1;
+    const long long c = cidx(g, i, j, k);
+    const int nh = flux_cd ? 5 : 8;
+    double out[8];
+    for (int q = 0; q < nh; q++) {
+      const long long o = q * N + c;
+      out[q] = ((Uin[o] - dtdx * (Fx[o] - Fx[o - sx])) - dtdy * (Fy[o] - Fy[o - sy])) - dtdz * (Fz[o] - Fz[o - sz]);
+    }
+    if (flux_cd) {
+      out[5] = (Uin[5 * N + c] - (0.5 * dtdy) * (load_e(g, E, 2, i, j + 1, k) - load_e(g, E, 2, i, j - 1, k)))
+               + (0.5 * dtdz) * (load_e(g, E, 1, i, j, k + 1) - load_e(g, E, 1, i, j, k - 1));
+      out[6] = (Uin[6 * N + c] + (0.5 * dtdx) * (load_e(g, E, 2, i + 1, j, k) - load_e(g, E, 2, i - 1, j, k)))
+               - (0.5 * dtdz) * (load_e(g, E, 0, i, j, k + 1) - load_e(g, E, 0, i, j, k - 1));
+      out[7] = (Uin[7 * N + c] - (0.5 * dtdx) * (load_e(g, E, 1, i + 1, j, k) - load_e(g, E, 1, i - 1, j, k)))
+               + (0.5 * dtdy) * (load_e(g, E, 0, i, j + 1, k) - load_e(g, E, 0, i, j - 1, k));
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) Uout[q * N + c] = out[q];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// CFL candidate reduction: the min loop of get_timestep
+// (src/hydro_core.f90:642-675) on the primitives of the interior.  Exact
+// (min is order-free); block minima meet in one 64-bit atomicMin on the
+// bit pattern, which orders positive doubles like their values.  Any
+// non-finite conserved value sets err bit 2.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = dmin(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+__global__ void __launch_bounds__(256) k_cfl(Geom g, Phys ph, const double* __restrict__ U, double dx,
+                                             double dy, double dz, unsigned long long* __restrict__ dtmin,
+                                             int* __restrict__ err) {
+  __shared__ double red[4];
+  const long long n = (long long)g.nx * g.ny * g.nz;
+  double dtp = 1e30;
+  bool bad = false;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % g.nx) + 1;
+    const long long r = t / g.nx;
+    const int j = (int)(r % g.ny) + 1;
+    const int k = (int)(r / g.ny) + 1;
+    const long long c = cidx(g, i, j, k);
+    double u[8], p[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      u[q] = U[q * g.ncell + c];
+      bad |= !__builtin_isfinite(u[q]);
+    }
+    u2prim(ph, u, p);
+    if (ph.mhd) {
+      const double b2 = (p[5] * p[5] + p[6] * p[6]) + p[7] * p[7];
+      const double cx = cfast1(ph.gamma, p[4], p[0], b2, p[5]);
+      const double cy = cfast1(ph.gamma, p[4], p[0], b2, p[6]);
+      const double cz = cfast1(ph.gamma, p[4], p[0], b2, p[7]);
+      dtp = dmin(dtp, dx / (__builtin_fabs(p[1]) + cx));
+      dtp = dmin(dtp, dy / (__builtin_fabs(p[2]) + cy));
+      dtp = dmin(dtp, dz / (__builtin_fabs(p[3]) + cz));
+    } else {
+      const double cs = csound(ph, p[4], p[0]);
+      dtp = dmin(dtp, dx / (__builtin_fabs(p[1]) + cs));
+      dtp = dmin(dtp, dy / (__builtin_fabs(p[2]) + cs));
+      dtp = dmin(dtp, dz / (__builtin_fabs(p[3]) + cs));
+    }
+  }
+  if (bad) atomicOr(err, 2);
+  dtp = wave_min(dtp);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = dtp;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = dmin(m, red[w]);
+    atomicMin(dtmin, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ghost fill of one face of the conserved state, boundaryI
+// (src/boundaries.f90:44-247, serial branch): periodic copies over the full
+// transverse range, closed/outflow over 0..n+1.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_ghost1(Geom g, double* __restrict__ U, int axis, int side,
+                                                int kind) {
+  const int n0 = axis == 0 ? g.ny : g.nx;  // transverse extents (interior)
+  const int n1 = axis == 2 ? g.ny : g.nz;
+  const int na = axis == 0 ? g.nx : axis == 1 ? g.ny : g.nz;
+  const int lo = kind == BC_PERIODIC ? -1 : 0;
+  const int e0 = n0 + 2 - 2 * lo, e1 = n1 + 2 - 2 * lo;
+  const long long n = (long long)e0 * e1;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int a = (int)(t % e0) + lo;
+    const int b = (int)(t / e0) + lo;
+    const int dst = side == 0 ? 0 : na + 1;
+    int src;
+    if (kind == BC_PERIODIC) src = side == 0 ? na : 1;
+    else src = side == 0 ? 1 : na;
+    int di, dj, dk, si, sj, sk;
+    if (axis == 0) { di = dst; si = src; dj = sj = a; dk = sk = b; }
+    else if (axis == 1) { dj = dst; sj = src; di = si = a; dk = sk = b; }
+    else { dk = dst; sk = src; di = si = a; dj = sj = b; }
+    const long long cd = cidx(g, di, dj, dk), cs = cidx(g, si, sj, sk);
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const double v = U[q * g.ncell + cs];
+      U[q * g.ncell + cd] = (kind == BC_CLOSED && q == 1 + axis) ? -v : v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// layout conversion between the reference's AoS u(neq,i,j,k) and SoA
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_aos_to_soa(long long ncell, const double* __restrict__ A,
+                                                    double* __restrict__ S) {
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < ncell;
+       c += (long long)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) S[q * ncell + c] = A[c * 8 + q];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_soa_to_aos(long long ncell, int nvar, const double* __restrict__ S,
+                                                    double* __restrict__ A) {
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < ncell;
+       c += (long long)gridDim.x * blockDim.x) {
+    for (int q = 0; q < nvar; q++) A[c * nvar + q] = S[q * ncell + c];
+  }
+}
+
+// primitives and temperature of every cell as stored (calcprim over the
+// whole array, hydro_core.f90:304-316), AoS output
+__global__ void __launch_bounds__(256) k_prim_all(long long ncell, Phys ph, const double* __restrict__ U,
+                                                  double* __restrict__ Paos, double* __restrict__ T) {
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < ncell;
+       c += (long long)gridDim.x * blockDim.x) {
+    double u[8], p[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) u[q] = U[q * ncell + c];
+    u2prim(ph, u, p);
+#pragma unroll
+    for (int q = 0; q < 8; q++) Paos[c * 8 + q] = p[q];
+    if (T) T[c] = (p[4] / p[0]) * 1.0;
+  }
+}
+
+// divB of the dump (src/Out_BIN_Module.f90:214-222), divb(nx,ny,nz)
+__global__ void __launch_bounds__(256) k_divb(Geom g, const double* __restrict__ U, double dx, double dy,
+                                              double dz, double* __restrict__ D) {
+  const long long n = (long long)g.nx * g.ny * g.nz;
+  const long long sx = 1, sy = g.NX, sz = (long long)g.NX * g.NY;
+  const long long N = g.ncell;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % g.nx) + 1;
+    const long long r = t / g.nx;
+    const int j = (int)(r % g.ny) + 1;
+    const int k = (int)(r / g.ny) + 1;
+    const long long c = cidx(g, i, j, k);
+    D[t] = ((U[5 * N + c + sx] - U[5 * N + c - sx]) / (2.0 * dx)
+            + (U[6 * N + c + sy] - U[6 * N + c - sy]) / (2.0 * dy))
+           + (U[7 * N + c + sz] - U[7 * N + c - sz]) / (2.0 * dz);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers
+// ---------------------------------------------------------------------------
+static inline unsigned grid_for(long long n, int block = 256) {
+  long long b = (n + block - 1) / block;
+  const long long cap = 256LL * 1024;  // grid-stride loops cover the rest
+  if (b > cap) b = cap;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+void launch_prim(hipStream_t s, const Geom& g, const Phys& ph, const double* U, double* P, int lo) {
+  const long long n = (long long)(g.nx + 2 - 2 * lo) * (g.ny + 2 - 2 * lo) * (g.nz + 2 - 2 * lo);
+  hipLaunchKernelGGL(k_prim, dim3(grid_for(n)), dim3(256), 0, s, g, ph, U, P, lo);
+}
+
+template <int SOLVER, int ORDER>
+static void launch_flux_t(hipStream_t s, const Geom& g, const Phys& ph, int axis, const double* P, double* F,
+                          int* err) {
+  const long long e0 = axis == 0 ? g.nx + 1 : g.nx;
+  const long long e1 = axis == 1 ? g.ny + 1 : g.ny;
+  const long long e2 = axis == 2 ? g.nz + 1 : g.nz;
+  const long long n = e0 * e1 * e2;
+  hipLaunchKernelGGL((k_flux<SOLVER, ORDER>), dim3(grid_for(n)), dim3(256), 0, s, g, ph, axis, P, F, err);
+}
+
+void launch_flux(hipStream_t s, const Geom& g, const Phys& ph, int solver, int order, int axis,
+                 const double* P, double* F, int* err) {
+#define GM_FLUX_CASE(S)                                                      \
+  case S:                                                                    \
+    if (order == 1) launch_flux_t<S, 1>(s, g, ph, axis, P, F, err);          \
+    else launch_flux_t<S, 2>(s, g, ph, axis, P, F, err);                     \
+    break;
+  switch (solver) {
+    GM_FLUX_CASE(SOLVER_HLLD)
+    GM_FLUX_CASE(SOLVER_HLLE)
+    GM_FLUX_CASE(SOLVER_HLL)
+    GM_FLUX_CASE(SOLVER_HLLC)
+  }
+#undef GM_FLUX_CASE
+}
+
+void launch_efield(hipStream_t s, const Geom& g, const double* Fx, const double* Fy, const double* Fz,
+                   double* E) {
+  const long long n = (long long)g.nx * g.ny * g.nz;
+  hipLaunchKernelGGL(k_efield, dim3(grid_for(n)), dim3(256), 0, s, g, Fx, Fy, Fz, E);
+}
+
+void launch_update(hipStream_t s, const Geom& g, const double* Uin, double* Uout, const double* Fx,
+                   const double* Fy, const double* Fz, const double* E, double dt, double dx, double dy,
+                   double dz, int flux_cd) {
+  const long long n = (long long)g.nx * g.ny * g.nz;
+  hipLaunchKernelGGL(k_update, dim3(grid_for(n)), dim3(256), 0, s, g, Uin, Uout, Fx, Fy, Fz, E, dt, dx, dy,
+                     dz, flux_cd);
+}
+
+void launch_cfl(hipStream_t s, const Geom& g, const Phys& ph, const double* U, double dx, double dy, double dz,
+                unsigned long long* dtmin, int* err) {
+  const long long n = (long long)g.nx * g.ny * g.nz;
+  unsigned b = grid_for(n);
+  if (b > 4096) b = 4096;
+  hipLaunchKernelGGL(k_cfl, dim3(b), dim3(256), 0, s, g, ph, U, dx, dy, dz, dtmin, err);
+}
+
+void launch_ghost1(hipStream_t s, const Geom& g, double* U, int axis, int side, int kind) {
+  const int n0 = axis == 0 ? g.ny : g.nx;
+  const int n1 = axis == 2 ? g.ny : g.nz;
+  const long long n = (long long)(n0 + 4) * (n1 + 4);
+  hipLaunchKernelGGL(k_ghost1, dim3(grid_for(n)), dim3(256), 0, s, g, U, axis, side, kind);
+}
+
+void launch_aos_to_soa(hipStream_t s, long long ncell, const double* A, double* S) {
+  hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(ncell)), dim3(256), 0, s, ncell, A, S);
+}
+
+void launch_soa_to_aos(hipStream_t s, long long ncell, int nvar, const double* S, double* A) {
+  hipLaunchKernelGGL(k_soa_to_aos, dim3(grid_for(ncell)), dim3(256), 0, s, ncell, nvar, S, A);
+}
+
+void launch_prim_all(hipStream_t s, long long ncell, const Phys& ph, const double* U, double* Paos, double* T) {
+  hipLaunchKernelGGL(k_prim_all, dim3(grid_for(ncell)), dim3(256), 0, s, ncell, ph, U, Paos, T);
+}
+
+void launch_divb(hipStream_t s, const Geom& g, const double* U, double dx, double dy, double dz, double* D) {
+  const long long n = (long long)g.nx * g.ny * g.nz;
+  hipLaunchKernelGGL(k_divb, dim3(grid_for(n)), dim3(256), 0, s, g, U, dx, dy, dz, D);
+}
+
+}  // namespace gm

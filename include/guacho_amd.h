@@ -1,0 +1,152 @@
+/*
+ * guacho_amd.h -- C ABI of the MI355X Godunov MHD step library
+ * (guacho_amd/libguacho_amd.so).
+ *
+ * The reference has no FFI: its hot path is Fortran called from the driver
+ * loop src/main.f90:94-125.  Each entry point below replaces one reference
+ * routine on that path; a Fortran host binds them with ISO_C_BINDING
+ * (host/guacho_capi.f90) and Python with ctypes (guacho_amd/_capi.py).
+ *
+ * Conventions
+ *  - Host arrays are caller-allocated in the reference's globals layout,
+ *    Fortran column-major u(neq, nxmin:nxmax, nymin:nymax, nzmin:nzmax)
+ *    with neq = 8 fastest and nghost = 2 (src/globals.f90:33-38,
+ *    src/init.f90:145-151).  The library stores structure-of-arrays in
+ *    device memory and transposes on upload/download.
+ *  - Every call returns GM_OK (0) or a positive GM_ERR_* code; the message
+ *    is available from gm_last_error().  Nothing aborts the process.
+ *  - One host thread per context.  Calls are ordered on the context's HIP
+ *    stream; downloads and gm_cfl_dtp synchronize with it.
+ */
+#ifndef GUACHO_AMD_H
+#define GUACHO_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GM_OK 0
+#define GM_ERR_CONFIG 1    /* unsupported or invalid configuration */
+#define GM_ERR_HIP 2       /* HIP runtime error */
+#define GM_ERR_NONFINITE 3 /* non-finite state; mirrors the HLLD/HLLC
+                              fall-through stop at hlld.f90:316-317,
+                              hllc.f90:135-138 */
+#define GM_ERR_OOM 4       /* device allocation failed */
+#define GM_ERR_ARG 5       /* bad argument (null pointer, size) */
+
+/* Named constants: same values as src/constants.f90:56-93. */
+#define GM_SOLVER_HLL 1
+#define GM_SOLVER_HLLC 2
+#define GM_SOLVER_HLLE 3
+#define GM_SOLVER_HLLD 4
+#define GM_BC_OUTFLOW 1
+#define GM_BC_CLOSED 2
+#define GM_BC_PERIODIC 3
+#define GM_BC_OTHER 4
+#define GM_LIMITER_NO_AVERAGE (-1)
+#define GM_LIMITER_NO_LIMIT 0
+#define GM_LIMITER_MINMOD 1
+#define GM_LIMITER_VAN_LEER 2
+#define GM_LIMITER_VAN_ALBADA 3
+#define GM_LIMITER_UMIST 4
+#define GM_LIMITER_WOODWARD 5
+#define GM_LIMITER_SUPERBEE 6
+
+#define GM_PRECISION_FP64 0  /* the drop-in: bitwise the reference's arithmetic */
+#define GM_PRECISION_MIXED 1 /* FP32 states/fluxes, FP64 pressure recovery,
+                                update accumulation and reductions */
+
+/* Run configuration: the subset of the reference's parameters module
+ * (OT/parameters.f90:48-177) that the step reads, plus the grid spacing
+ * computed by initmain (src/init.f90:119-122). */
+typedef struct gm_params {
+  int32_t nx, ny, nz;     /* interior cells of the subdomain */
+  int32_t neq;            /* must be 8 (BFIELD, no passives) */
+  int32_t nghost;         /* must be 2 */
+  int32_t riemann_solver; /* GM_SOLVER_* */
+  int32_t slope_limiter;  /* GM_LIMITER_* */
+  int32_t bc[6];          /* bc_left, bc_right, bc_bottom, bc_top, bc_out, bc_in */
+  int32_t mhd;            /* full MHD */
+  int32_t pmhd;           /* passive B (HLLC) */
+  int32_t enable_flux_cd; /* flux-CD div B control */
+  int32_t eight_wave;     /* must be 0 (Powell sources not built) */
+  int32_t precision;      /* GM_PRECISION_* */
+  int32_t device;         /* HIP device ordinal */
+  int32_t reserved;       /* must be 0 */
+  double dx, dy, dz;
+  double cv;              /* specific heat at constant volume */
+  double gamma;           /* (cv+1)/cv, passed as the caller computed it */
+  double eta;             /* artificial viscosity; must be 0 */
+} gm_params;
+
+typedef struct gm_ctx gm_ctx;
+
+/* Allocate device state for one subdomain.  Replaces the allocations of
+ * initmain (src/init.f90:144-159) and validates the configuration the way
+ * its report does (src/init.f90:209-404). */
+int gm_create(const gm_params *params, gm_ctx **ctx);
+
+/* Free the context and its device memory. */
+int gm_destroy(gm_ctx *ctx);
+
+/* Message for the last error on ctx (or of the last failed gm_create when
+ * ctx is NULL).  Valid until the next call on the same context. */
+const char *gm_last_error(const gm_ctx *ctx);
+
+/* Copy the full conserved array, ghosts included, host -> device.
+ * Replaces writes to globals::u by initflow (src/init.f90:432-477). */
+int gm_upload_u(gm_ctx *ctx, const double *u);
+
+/* Copy the full conserved array, ghosts included, device -> host, after
+ * the one-layer ghost fill of boundaryI.  Replaces reads of globals::u by
+ * write_BIN (src/Out_BIN_Module.f90:162).  Synchronizes. */
+int gm_download_u(gm_ctx *ctx, double *u);
+
+/* Primitives (and temperature when temp != NULL) of every cell, ghosts
+ * included: calcprim(u, primit) (src/hydro_core.f90:245-320).  temp has
+ * the layout Temp(nxmin:nxmax, nymin:nymax, nzmin:nzmax).  Synchronizes. */
+int gm_download_primit(gm_ctx *ctx, double *primit, double *temp);
+
+/* One-layer ghost fill of the conserved state: boundaryI
+ * (src/boundaries.f90:44-247).  gm_tstep applies boundary conditions
+ * itself; this is for callers that change u and need the ghosts. */
+int gm_boundary_u(gm_ctx *ctx);
+
+/* The minimum over interior cells of dx/(|vx|+cfx), dy/(|vy|+cfy),
+ * dz/(|vz|+cfz) (src/hydro_core.f90:642-675; csound for pmhd).  The host
+ * applies the CFL ramp, the MIN allreduce and the dump clamp of
+ * hydro_core.f90:677-694.  Returns GM_ERR_NONFINITE if any interior cell
+ * holds a non-finite value.  Synchronizes. */
+int gm_cfl_dtp(gm_ctx *ctx, double *dtp);
+
+/* Advance the state by dt: tstep (src/hydro_solver.f90:134-229) with the
+ * add-ons (cooling, chemistry, radiation, conduction) off.  Asynchronous;
+ * a non-finite flux sets an error reported by the next gm_cfl_dtp or
+ * gm_sync. */
+int gm_tstep(gm_ctx *ctx, double dt);
+
+/* Wait for queued work; report any deferred device error. */
+int gm_sync(gm_ctx *ctx);
+
+/* Divergence of B with the dump formula of write_BIN
+ * (src/Out_BIN_Module.f90:214-222), divb(nx, ny, nz) column-major. */
+int gm_download_divb(gm_ctx *ctx, double *divb);
+
+/* ---- measurement ---------------------------------------------------- */
+/* Kernel timing: while enabled, gm_tstep records HIP events around each
+ * kernel on the context's stream.  gm_kernel_times returns, per kernel
+ * class, the accumulated milliseconds and launch count since the last
+ * reset, and the kernel names (static strings). */
+#define GM_NKCLASS 8
+int gm_set_timing(gm_ctx *ctx, int enable);
+int gm_kernel_times(gm_ctx *ctx, double *ms, int64_t *launches, const char **names);
+
+/* Device memory held by the context, in bytes. */
+int gm_device_bytes(const gm_ctx *ctx, int64_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GUACHO_AMD_H */

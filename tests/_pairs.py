@@ -1,0 +1,59 @@
+"""Helpers shared by the GPU parity tests: run the same seeded set-up
+through the device step (libguacho_amd.so) and the C restatement (oracle)
+with the reference's host logic, step by step."""
+import numpy as np
+
+from guacho_amd import _capi
+from guacho_amd.host import timestep_clamp
+from oracle.oracle import Config, Oracle
+
+
+def oracle_config(params: _capi.GmParams, cfl: float = 0.4) -> Config:
+    return Config(nx=params.nx, ny=params.ny, nz=params.nz, dx=params.dx, dy=params.dy, dz=params.dz,
+                  cv=params.cv, riemann_solver=params.riemann_solver, slope_limiter=params.slope_limiter,
+                  bc=list(params.bc), mhd=bool(params.mhd), pmhd=bool(params.pmhd),
+                  enable_flux_cd=bool(params.enable_flux_cd), cfl=cfl)
+
+
+def random_state(rng, nx, ny, nz, cv=1.5, contrast=0.3, vmax=1.5, bmax=1.0):
+    shape = (nx + 4, ny + 4, nz + 4)
+    rho = rng.uniform(1 - contrast, 1 + contrast, shape)
+    v = rng.uniform(-vmax, vmax, (3,) + shape)
+    B = rng.uniform(-bmax, bmax, (3,) + shape)
+    p = rng.uniform(1 - contrast, 1 + contrast, shape)
+    u = np.zeros((8,) + shape, order="F")
+    u[0] = rho
+    u[1:4] = rho * v
+    u[4] = 0.5 * rho * (v ** 2).sum(0) + cv * p + 0.5 * (B ** 2).sum(0)
+    u[5:8] = B
+    return u
+
+
+def run_pair(params: _capi.GmParams, u0: np.ndarray, nsteps: int, cfl: float = 0.4, check_each=False):
+    """Advance u0 nsteps on both paths with dt from each path's own CFL
+    reduction; asserts the reductions agree bitwise every step.  Returns
+    (u_device, u_oracle, dts)."""
+    O = Oracle(oracle_config(params, cfl))
+    O.set_u(u0)
+    O.boundaryI()
+    O.calcprim_u()
+    dts = []
+    with _capi.Subdomain(params) as dev:
+        dev.upload_u(u0)
+        dev.boundary_u()
+        t = 0.0
+        for it in range(1, nsteps + 1):
+            dtp_o = O.cfl_dtp()
+            dtp_d = dev.cfl_dtp()
+            assert dtp_o == dtp_d, f"step {it}: CFL reduction differs {dtp_o!r} vs {dtp_d!r}"
+            dt, _ = timestep_clamp(it, 10, cfl, dtp_o, t, 1e30)
+            O.tstep(dt)
+            dev.tstep(dt)
+            t += dt
+            dts.append(dt)
+            if check_each:
+                np.testing.assert_array_equal(dev.download_u(), O.get_u(), err_msg=f"step {it}")
+        ud = dev.download_u()
+    uo = O.get_u()
+    O.close()
+    return ud, uo, dts

@@ -1,0 +1,121 @@
+"""Write tests/golden/ref_<variant>.npz from the reference's own routines.
+
+Runs in the build container only (needs oracle/_ref, compiled from the
+reference's sources by `make -C oracle ref`).  Each fixture holds seeded
+inputs and the reference's outputs for
+  - per-interface solves (prim2fhlld / hlle / hll / hllc of the variant),
+  - the slope limiter, u2prim, prim2u, prim2f, cfast, cfastX,
+  - one boundaryI + calcprim + first-order sweep on a random state,
+  - one boundaryII + calcprim + limited sweep on a random state,
+  - get_timestep's ramp and clamp.
+The fixtures are data (inputs and expected outputs); tests replay the
+inputs through the C restatement and require identical bits.
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+from oracle.oracle import RefLib, ref_variants  # noqa: E402
+
+NVEC = 600
+
+
+def random_prims(rng, n, pmhd=False):
+    p = np.empty((n, 8))
+    p[:, 0] = rng.uniform(0.1, 2.0, n)
+    p[:, 1:4] = rng.uniform(-3.0, 3.0, (n, 3))
+    p[:, 4] = rng.uniform(0.02, 2.0, n)
+    p[:, 5:8] = rng.uniform(-2.0, 2.0, (n, 3))
+    return p
+
+
+def random_state(rng, shape, cv):
+    _, NX, NY, NZ = shape
+    # moderate contrasts: unlimited reconstruction (LIMITER_NO_LIMIT) must
+    # keep interface density and pressure positive
+    rho = rng.uniform(0.7, 1.3, (NX, NY, NZ))
+    v = rng.uniform(-1.5, 1.5, (3, NX, NY, NZ))
+    B = rng.uniform(-1.0, 1.0, (3, NX, NY, NZ))
+    p = rng.uniform(0.7, 1.3, (NX, NY, NZ))
+    u = np.zeros(shape, order="F")
+    u[0] = rho
+    u[1:4] = rho * v
+    u[4] = 0.5 * rho * (v ** 2).sum(0) + cv * p + 0.5 * (B ** 2).sum(0)
+    u[5:8] = B
+    return u
+
+
+def make(variant, seed):
+    R = RefLib(variant)
+    rng = np.random.default_rng(seed)
+    out = {}
+    pl = random_prims(rng, NVEC)
+    pr = random_prims(rng, NVEC)
+    pr[::7, 5] = pl[::7, 5]            # continuous normal field
+    pl[::11, 5] = 0.0
+    pr[::11, 5] = 0.0                  # Bx = 0: single-star regions only
+    pr[::13] = pl[::13]                # consistency F(U,U) = F(U)
+    out["riem_pl"], out["riem_pr"] = pl, pr
+    out["riem_ff"] = np.array([R.riemann(a, b) for a, b in zip(pl, pr)])
+    pll = pl + rng.normal(0, 0.3, pl.shape)
+    prr = pr + rng.normal(0, 0.3, pr.shape)
+    pll[::5] = pl[::5]                 # flat left slope
+    out["lim_in"] = np.stack([pll, pl, pr, prr])
+    out["lim_out"] = np.array([np.concatenate(R.limiter(a, b, c, d)) for a, b, c, d in zip(pll, pl, pr, prr)])
+    uu = np.array([np.concatenate([[a[0]], a[0] * a[1:4], [a[4] * 2.0], a[5:8]]) for a in pl])
+    uu[::17, 0] = 1e-20                # density floor
+    uu[::19, 4] = -1.0                 # pressure floor
+    out["u2p_in"] = uu
+    out["u2p_out"] = np.array([np.concatenate([R.u2prim(a)[0], [R.u2prim(a)[1]]]) for a in uu])
+    out["p2u_out"] = np.array([R.prim2u(a) for a in pl])
+    out["p2f_out"] = np.array([R.prim2f(a) for a in pl])
+    out["cfast_out"] = np.array([R.cfast(a[4], a[0], a[5], a[6], a[7]) for a in pl])
+    out["cfastx_out"] = np.array([R.cfastx(a) for a in pl])
+    # sweeps on a random state
+    dx, dy, dz = 0.1, 0.07, 0.13
+    R.init(dx, dy, dz)
+    u = random_state(rng, R.shape(), R.cv)
+    out["sweep_u_in"] = u
+    R.set_array(0, u)
+    R.boundary(1)
+    out["sweep_u_bc1"] = R.get_array(0)
+    R.calcprim(0)
+    out["sweep_primit1"] = R.get_array(2)
+    out["sweep_temp1"] = R.get_temp()
+    R.fluxes(1)
+    out["sweep_f1"], out["sweep_g1"], out["sweep_h1"] = (R.get_array(3 + a) for a in range(3))
+    up = random_state(rng, R.shape(), R.cv)
+    out["sweep_up_in"] = up
+    R.set_array(1, up)
+    R.boundary(2)
+    out["sweep_up_bc2"] = R.get_array(1)
+    R.calcprim(1)
+    R.fluxes(2)
+    out["sweep_f2"], out["sweep_g2"], out["sweep_h2"] = (R.get_array(3 + a) for a in range(3))
+    ts = []
+    for it in (1, 4, 10, 11, 25):
+        for t, tp in ((0.0, 10.0), (0.3, 0.3000001)):
+            dt, dump = R.get_timestep(it, 10, t, tp)
+            ts.append([it, t, tp, dt, float(dump)])
+    out["timestep"] = np.array(ts)
+    out["dxyz"] = np.array([dx, dy, dz])
+    cfg = np.array([R.nx, R.ny, R.nz, R.riemann_solver, R.slope_limiter, *R.bc, int(R.mhd), int(R.pmhd)])
+    out["config_int"] = cfg
+    out["config_real"] = np.array([R.cv, R.gamma, R.cfl])
+    return out
+
+
+def main():
+    vs = ref_variants()
+    if not vs:
+        raise SystemExit("oracle/_ref is not built (make -C oracle ref)")
+    for n, v in enumerate(vs):
+        np.savez_compressed(os.path.join(HERE, f"ref_{v}.npz"), **make(v, 1000 + n))
+        print("wrote", v)
+
+
+if __name__ == "__main__":
+    main()

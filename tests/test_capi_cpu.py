@@ -1,0 +1,74 @@
+"""The C ABI without a GPU: the library loads, exports every entry point
+include/guacho_amd.h declares, lays out gm_params as the header does, and
+rejects unsupported configurations before touching the device."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from guacho_amd import _capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "guacho_amd.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return re.findall(r"\b(gm_[a-z_0-9]+)\s*\(", src)
+
+
+def test_header_and_binding_agree():
+    names = declared_functions()
+    assert set(names) == set(_capi.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    L = _capi.lib()
+    for name in declared_functions():
+        assert hasattr(L, name), name
+
+
+def test_params_layout():
+    # 20 int32 then 6 doubles, no padding
+    assert C.sizeof(_capi.GmParams) == 20 * 4 + 6 * 8
+    assert _capi.GmParams.dx.offset == 80
+
+
+def _create(**kw):
+    base = dict(nx=8, ny=8, nz=8, dx=0.1, dy=0.1, dz=0.1)
+    base.update(kw)
+    p = _capi.make_params(base.pop("nx"), base.pop("ny"), base.pop("nz"), base.pop("dx"), base.pop("dy"),
+                          base.pop("dz"), **base)
+    return p
+
+
+@pytest.mark.parametrize("kw,needle", [
+    (dict(riemann_solver=7), "riemann_solver"),
+    (dict(riemann_solver=2, mhd=True), "HLLC"),
+    (dict(mhd=False, pmhd=False), "hydro"),
+    (dict(slope_limiter=9), "slope_limiter"),
+    (dict(bc=(4, 4, 3, 3, 3, 3)), "BC_OTHER"),
+    (dict(bc=(3, 1, 3, 3, 3, 3)), "periodic"),
+    (dict(eight_wave=True), "eight_wave"),
+    (dict(eta=0.1), "eta"),
+    (dict(nz=1), "at least 2"),
+    (dict(dx=0.0), "dx"),
+    (dict(precision=7), "PRECISION"),
+])
+def test_rejects_unsupported_configuration(kw, needle):
+    L = _capi.lib()
+    p = _create(**kw)
+    h = C.c_void_p()
+    rc = L.gm_create(C.byref(p), C.byref(h))
+    assert rc == _capi.GM_ERR_CONFIG
+    assert not h.value
+    assert needle.lower() in L.gm_last_error(None).decode().lower()
+
+
+def test_null_arguments():
+    L = _capi.lib()
+    assert L.gm_create(None, None) == _capi.GM_ERR_ARG
+    assert L.gm_tstep(None, 0.1) == _capi.GM_ERR_ARG
+    assert L.gm_destroy(None) == _capi.GM_ERR_ARG
