@@ -69,6 +69,51 @@ def cpu_baseline(seconds: float):
                       f"turbulence {n}^3 FP64 HLLD minmod flux-CD, {steps} steps in {el:.2f} s"}
 
 
+# Compulsory bytes one launch of each kernel class moves (FP64), per the
+# step's decomposition: N interior cells, faces ~ (n+1) n^2 per axis launch.
+def algorithmic_bytes(cls: str, n: int) -> float:
+    N = float(n) ** 3
+    return {
+        # read the 8 conserved values of the cells the sweeps read, write 8 primitives
+        "prim": 128.0 * (n + 4) ** 2 * (n + 4) * (1.0 - 12.0 / (n + 4) ** 2),
+        # read 8 primitives per cell once, write 8 flux components per face
+        "flux_predictor": 128.0 * (n + 1) * n * n,
+        "flux_corrector": 128.0 * (n + 1) * n * n,
+        # read the 6 field fluxes, write 3 E components
+        "efield": 72.0 * N,
+        # read U (8) + 5 hydro fluxes x 3 axes + 3 E, write U (8)
+        "update": 272.0 * N,
+        # read U (8)
+        "cfl_dt": 64.0 * N,
+    }.get(cls, 0.0)
+
+
+def measured_traffic(cls: str):
+    """Calibrated HBM bytes per launch of `cls` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary of this workload, if one exists."""
+    path = os.path.join(ROOT, "profiles", "traffic_fp64_turb256.json")
+    try:
+        with open(path) as fh:
+            d = json.load(fh)
+        return d["per_launch_bytes"].get(cls), os.path.relpath(path, ROOT)
+    except (OSError, KeyError, ValueError):
+        return None, None
+
+
+def dominant_roofline(kt, n, steps):
+    """HBM roofline of the kernel class with the largest total time."""
+    cls = max((k for k in kt if kt[k][1] and algorithmic_bytes(k, n) > 0), key=lambda k: kt[k][0])
+    ms, launches = kt[cls]
+    avg_s = ms / launches / 1e3
+    ach = algorithmic_bytes(cls, n) / avg_s / 1e9
+    traffic, src = measured_traffic(cls)
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": cls,
+            "launches_per_step": launches // steps,
+            "avg_launch_ms": round(ms / launches, 4), "algorithmic_bytes_per_launch": algorithmic_bytes(cls, n),
+            "traffic_source": src}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -111,16 +156,7 @@ def main():
     zones = n ** 3 * args.steps
     value = zones / el
     ms_per_step = 1e3 * el / args.steps
-    ms, launches = kt["flux_corrector"]
-    avg_s = ms / launches / 1e3
-    faces = (n + 1) * n * n
-    # per launch: 8 primitives read per cell + 8 flux components written per face
-    alg_bytes = 128.0 * faces
-    achieved = alg_bytes / avg_s / 1e9
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_flux<HLLD,2> (limited reconstruction + HLLD, one axis per launch)",
-            "avg_launch_ms": round(ms / launches, 4)}
+    roof = dominant_roofline(kt, n, args.steps)
     step_ach = BYTES_PER_ZONE_FP64 * n ** 3 / (el / args.steps) / 1e9
     out = {
         "metric": "zone-updates/s", "value": round(value, 1), "unit": "zone-updates/s", "n_gpus": 1,

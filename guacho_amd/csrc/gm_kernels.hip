@@ -22,6 +22,26 @@ __device__ __forceinline__ long long cidx(const Geom& g, int i, int j, int k) {
   return ((long long)(k + 1) * g.NY + (j + 1)) * g.NX + (i + 1);
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin by linear id.  Renumber
+// them so each XCD runs one contiguous range of logical tiles (whole
+// neighbouring rows and planes), so stencil neighbours hit its own L2.
+constexpr int kXCD = 8;
+__device__ __forceinline__ unsigned xcd_logical_block() {
+  const unsigned b = blockIdx.x, nb = gridDim.x;
+  const unsigned q = nb / kXCD, r = nb % kXCD;
+  const unsigned x = b % kXCD, idx = b / kXCD;
+  return x < r ? x * (q + 1) + idx : r * (q + 1) + (x - r) * q + idx;
+}
+
+// Logical tile -> (flat index within a plane of `plane_n` items, plane).
+// Tiles of 256 items; `bpp` tiles per plane.  Returns false past the end.
+__device__ __forceinline__ bool tile_item(unsigned bpp, unsigned plane_n, int& t, int& plane) {
+  const unsigned lb = xcd_logical_block();
+  plane = (int)(lb / bpp);
+  t = (int)((lb - (unsigned)plane * bpp) * 256u + threadIdx.x);
+  return (unsigned)t < plane_n;
+}
+
 // Interior source of ghost index i on an axis of n cells; flips when the
 // boundary on that side is closed.  Valid for i in -1..n+2 (n >= 2).
 __device__ __forceinline__ int map_ghost(int i, int n, int lo, int hi, bool& flip) {
@@ -59,26 +79,21 @@ __device__ __forceinline__ void load_cons(const Geom& g, const double* __restric
 // (src/hydro_core.f90:245-320), ghost values by boundary mapping.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_prim(Geom g, Phys ph, const double* __restrict__ U,
-                                              double* __restrict__ P, int lo) {
+                                              double* __restrict__ P, int lo, unsigned bpp) {
   // cells lo..n+1-lo on each axis (lo = 0: one ghost layer, lo = -1: two)
-  const int ex = g.nx + 2 - 2 * lo, ey = g.ny + 2 - 2 * lo, ez = g.nz + 2 - 2 * lo;
-  const long long n = (long long)ex * ey * ez;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
-       t += (long long)gridDim.x * blockDim.x) {
-    const int i = (int)(t % ex) + lo;
-    const long long r = t / ex;
-    const int j = (int)(r % ey) + lo;
-    const int k = (int)(r / ey) + lo;
-    // only face ghosts are read downstream; skip edges and corners
-    const int out = (i < 1 || i > g.nx) + (j < 1 || j > g.ny) + (k < 1 || k > g.nz);
-    if (out > 1) continue;
-    double u[8], p[8];
-    load_cons(g, U, i, j, k, u);
-    u2prim(ph, u, p);
-    const long long c = cidx(g, i, j, k);
+  const int ex = g.nx + 2 - 2 * lo, ey = g.ny + 2 - 2 * lo;
+  int t, kk;
+  if (!tile_item(bpp, (unsigned)(ex * ey), t, kk)) return;
+  const int i = t % ex + lo, j = t / ex + lo, k = kk + lo;
+  // only face ghosts are read downstream; skip edges and corners
+  const int out = (i < 1 || i > g.nx) + (j < 1 || j > g.ny) + (k < 1 || k > g.nz);
+  if (out > 1) return;
+  double u[8], p[8];
+  load_cons(g, U, i, j, k, u);
+  u2prim(ph, u, p);
+  const long long c = cidx(g, i, j, k);
 #pragma unroll
-    for (int q = 0; q < 8; q++) P[q * g.ncell + c] = p[q];
-  }
+  for (int q = 0; q < 8; q++) P[q * g.ncell + c] = p[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -87,49 +102,42 @@ __global__ void __launch_bounds__(256) k_prim(Geom g, Phys ph, const double* __r
 // lies between cells i and i+1 (i = 0..n along the axis); only faces with
 // interior transverse indices feed the update.
 // ---------------------------------------------------------------------------
-template <int SOLVER, int ORDER>
-__global__ void __launch_bounds__(256) k_flux(Geom g, Phys ph, int axis, const double* __restrict__ P,
-                                              double* __restrict__ F, int* __restrict__ err) {
-  const int na = axis == 0 ? g.nx : axis == 1 ? g.ny : g.nz;
-  // face grid: along-axis index a in 0..na, transverse interior
-  const int e0 = axis == 0 ? na + 1 : g.nx;
-  const int e1 = axis == 1 ? na + 1 : g.ny;
-  const int e2 = axis == 2 ? na + 1 : g.nz;
-  const long long n = (long long)e0 * e1 * e2;
+template <int SOLVER, int ORDER, int axis>
+__global__ void __launch_bounds__(256) k_flux(Geom g, Phys ph, const double* __restrict__ P,
+                                              double* __restrict__ F, int* __restrict__ err, unsigned bpp) {
+  // face grid: along-axis index 0..n, transverse indices interior
+  const int e0 = axis == 0 ? g.nx + 1 : g.nx;
+  const int e1 = axis == 1 ? g.ny + 1 : g.ny;
+  int t, k;
+  if (!tile_item(bpp, (unsigned)(e0 * e1), t, k)) return;
+  int i = t % e0, j = t / e0;
+  if (axis != 0) i += 1;
+  if (axis != 1) j += 1;
+  if (axis != 2) k += 1;
   const long long stride = axis == 0 ? 1 : axis == 1 ? (long long)g.NX : (long long)g.NX * g.NY;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
-       t += (long long)gridDim.x * blockDim.x) {
-    int i = (int)(t % e0);
-    const long long r = t / e0;
-    int j = (int)(r % e1);
-    int k = (int)(r / e1);
-    if (axis != 0) i += 1;
-    if (axis != 1) j += 1;
-    if (axis != 2) k += 1;
-    const long long c = cidx(g, i, j, k);
-    double pl[8], pr[8], ff[8];
+  const long long c = cidx(g, i, j, k);
+  double pl[8], pr[8], ff[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    pl[q] = P[q * g.ncell + c];
+    pr[q] = P[q * g.ncell + c + stride];
+  }
+  if constexpr (ORDER == 2) {
+    double pll[8], prr[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-      pl[q] = P[q * g.ncell + c];
-      pr[q] = P[q * g.ncell + c + stride];
+      pll[q] = P[q * g.ncell + c - stride];
+      prr[q] = P[q * g.ncell + c + 2 * stride];
     }
-    if constexpr (ORDER == 2) {
-      double pll[8], prr[8];
-#pragma unroll
-      for (int q = 0; q < 8; q++) {
-        pll[q] = P[q * g.ncell + c - stride];
-        prr[q] = P[q * g.ncell + c + 2 * stride];
-      }
-      limiter(ph.limiter, pll, pl, pr, prr);
-    }
-    axis_perm(axis, pl);
-    axis_perm(axis, pr);
-    const bool ok = riemann<SOLVER>(ph, pl, pr, ff);
-    if (!ok) atomicOr(err, 1);
-    axis_perm(axis, ff);
-#pragma unroll
-    for (int q = 0; q < 8; q++) F[q * g.ncell + c] = ff[q];
+    limiter(ph.limiter, pll, pl, pr, prr);
   }
+  axis_perm<axis>(pl);
+  axis_perm<axis>(pr);
+  const bool ok = riemann<SOLVER>(ph, pl, pr, ff);
+  if (!ok) atomicOr(err, 1);
+  axis_perm<axis>(ff);
+#pragma unroll
+  for (int q = 0; q < 8; q++) F[q * g.ncell + c] = ff[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -137,81 +145,100 @@ __global__ void __launch_bounds__(256) k_flux(Geom g, Phys ph, int axis, const d
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_efield(Geom g, const double* __restrict__ Fx,
                                                 const double* __restrict__ Fy,
-                                                const double* __restrict__ Fz, double* __restrict__ E) {
-  const long long n = (long long)g.nx * g.ny * g.nz;
+                                                const double* __restrict__ Fz, double* __restrict__ E,
+                                                unsigned bpp) {
+  int t, k;
+  if (!tile_item(bpp, (unsigned)(g.nx * g.ny), t, k)) return;
+  const int i = t % g.nx + 1, j = t / g.nx + 1;
+  k += 1;
   const long long sx = 1, sy = g.NX, sz = (long long)g.NX * g.NY;
   const long long N = g.ncell;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
-       t += (long long)gridDim.x * blockDim.x) {
-    const int i = (int)(t % g.nx) + 1;
-    const long long r = t / g.nx;
-    const int j = (int)(r % g.ny) + 1;
-    const int k = (int)(r / g.ny) + 1;
-    const long long c = cidx(g, i, j, k);
-    E[0 * N + c] = 0.25 * (((-Fy[7 * N + c - sy] - Fy[7 * N + c]) + Fz[6 * N + c - sz]) + Fz[6 * N + c]);
-    E[1 * N + c] = 0.25 * (((+Fx[7 * N + c - sx] + Fx[7 * N + c]) - Fz[5 * N + c - sz]) - Fz[5 * N + c]);
-    E[2 * N + c] = 0.25 * (((-Fx[6 * N + c - sx] - Fx[6 * N + c]) + Fy[5 * N + c - sy]) + Fy[5 * N + c]);
-  }
-}
-
-// E component q at cell (i,j,k) where at most one index is a ghost:
-// boundaryI_ef semantics (periodic copy; closed flips e1 on x faces and e2
-// on y faces, flux_cd_module.f90:145-192; outflow copies).
-__device__ __forceinline__ double load_e(const Geom& g, const double* __restrict__ E, int q, int i, int j,
-                                         int k) {
-  bool flip = false;
-  if (i < 1 || i > g.nx) {
-    const int side = i < 1 ? g.bc[0] : g.bc[1];
-    if (side == BC_PERIODIC) i = i < 1 ? i + g.nx : i - g.nx;
-    else { flip = (side == BC_CLOSED) && q == 0; i = i < 1 ? 1 : g.nx; }
-  } else if (j < 1 || j > g.ny) {
-    const int side = j < 1 ? g.bc[2] : g.bc[3];
-    if (side == BC_PERIODIC) j = j < 1 ? j + g.ny : j - g.ny;
-    else { flip = (side == BC_CLOSED) && q == 1; j = j < 1 ? 1 : g.ny; }
-  } else if (k < 1 || k > g.nz) {
-    const int side = k < 1 ? g.bc[4] : g.bc[5];
-    if (side == BC_PERIODIC) k = k < 1 ? k + g.nz : k - g.nz;
-    else { k = k < 1 ? 1 : g.nz; }
-  }
-  const double v = E[q * g.ncell + cidx(g, i, j, k)];
-  return flip ? -v : v;
+  const long long c = cidx(g, i, j, k);
+  E[0 * N + c] = 0.25 * (((-Fy[7 * N + c - sy] - Fy[7 * N + c]) + Fz[6 * N + c - sz]) + Fz[6 * N + c]);
+  E[1 * N + c] = 0.25 * (((+Fx[7 * N + c - sx] + Fx[7 * N + c]) - Fz[5 * N + c - sz]) - Fz[5 * N + c]);
+  E[2 * N + c] = 0.25 * (((-Fx[6 * N + c - sx] - Fx[6 * N + c]) + Fy[5 * N + c - sy]) + Fy[5 * N + c]);
 }
 
 // ---------------------------------------------------------------------------
 // upwind update, step(dt) (src/hydro_solver.f90:77-127) with flux_cd_update
-// (src/flux_cd_module.f90:285-323).  Uout may alias Uin (each cell reads
-// only its own conserved state).
+// (src/flux_cd_module.f90:285-323).  Each thread marches up a z-column
+// segment of KCHUNK cells, carrying the z-face flux below the cell and E
+// one plane below and at the cell, so every z neighbour is read once.
+//
+// E ghosts (boundaryI_ef): the update reads e2, e3 at x neighbours, e1, e3
+// at y neighbours and e1, e2 at z neighbours.  A closed boundary negates e1
+// on x ghosts and e2 on y ghosts only (flux_cd_module.f90:145-176), never a
+// component read there, so every E ghost resolves to a fixed interior
+// neighbour offset: periodic wraps, closed and outflow copy the edge cell.
+// Uout may alias Uin (each cell reads only its own conserved state).
 // ---------------------------------------------------------------------------
+constexpr int KCHUNK = 16;
+
+// offset (in cells along the axis) of the E ghost-resolved neighbour
+__device__ __forceinline__ int e_nbr(int i, int n, int d, int bc) {
+  const int t = i + d;
+  if (t >= 1 && t <= n) return d;
+  if (bc == BC_PERIODIC) return (t < 1 ? t + n : t - n) - i;
+  return 0;  // closed / outflow: ghost takes the edge cell's value
+}
+
+template <int FLUXCD>
 __global__ void __launch_bounds__(256) k_update(Geom g, const double* Uin, double* Uout,
                                                 const double* __restrict__ Fx,
                                                 const double* __restrict__ Fy,
                                                 const double* __restrict__ Fz,
                                                 const double* __restrict__ E, double dt, double dx,
-                                                double dy, double dz, int flux_cd) {
-  const long long n = (long long)g.nx * g.ny * g.nz;
+                                                double dy, double dz, unsigned bpp) {
+  constexpr int NH = FLUXCD ? 5 : 8;
+  int t, kc;
+  if (!tile_item(bpp, (unsigned)(g.nx * g.ny), t, kc)) return;
+  const int i = t % g.nx + 1, j = t / g.nx + 1;
+  const int k0 = kc * KCHUNK + 1;
+  const int k1 = min(k0 + KCHUNK - 1, g.nz);
   const long long sx = 1, sy = g.NX, sz = (long long)g.NX * g.NY;
   const long long N = g.ncell;
   const double dtdx = dt / dx, dtdy = dt / dy, dtdz = dt / dz;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
-       t += (long long)gridDim.x * blockDim.x) {
-    const int i = (int)(t % g.nx) + 1;
-    const long long r = t / g.nx;
-    const int j = (int)(r % g.ny) + 1;
-    const int k = (int)(r / g.ny) + 1;
-    const long long c = cidx(g, i, j, k);
-    const int nh = flux_cd ? 5 : 8;
+  long long c = cidx(g, i, j, k0);
+  double fzm[NH];
+#pragma unroll
+  for (int q = 0; q < NH; q++) fzm[q] = Fz[q * N + c - sz];
+  long long oxm = 0, oxp = 0, oym = 0, oyp = 0;
+  double e1m = 0, e2m = 0, e1c = 0, e2c = 0;
+  if constexpr (FLUXCD) {
+    oxm = e_nbr(i, g.nx, -1, g.bc[0]) * sx;
+    oxp = e_nbr(i, g.nx, +1, g.bc[1]) * sx;
+    oym = e_nbr(j, g.ny, -1, g.bc[2]) * sy;
+    oyp = e_nbr(j, g.ny, +1, g.bc[3]) * sy;
+    const long long ozm = e_nbr(k0, g.nz, -1, g.bc[4]) * sz;
+    e1m = E[0 * N + c + ozm];
+    e2m = E[1 * N + c + ozm];
+    e1c = E[0 * N + c];
+    e2c = E[1 * N + c];
+  }
+#pragma unroll 1
+  for (int k = k0; k <= k1; k++, c += sz) {
     double out[8];
-    for (int q = 0; q < nh; q++) {
+#pragma unroll
+    for (int q = 0; q < NH; q++) {
       const long long o = q * N + c;
-      out[q] = ((Uin[o] - dtdx * (Fx[o] - Fx[o - sx])) - dtdy * (Fy[o] - Fy[o - sy])) - dtdz * (Fz[o] - Fz[o - sz]);
+      const double fz = Fz[o];
+      out[q] = ((Uin[o] - dtdx * (Fx[o] - Fx[o - sx])) - dtdy * (Fy[o] - Fy[o - sy])) - dtdz * (fz - fzm[q]);
+      fzm[q] = fz;
     }
-    if (flux_cd) {
-      out[5] = (Uin[5 * N + c] - (0.5 * dtdy) * (load_e(g, E, 2, i, j + 1, k) - load_e(g, E, 2, i, j - 1, k)))
-               + (0.5 * dtdz) * (load_e(g, E, 1, i, j, k + 1) - load_e(g, E, 1, i, j, k - 1));
-      out[6] = (Uin[6 * N + c] + (0.5 * dtdx) * (load_e(g, E, 2, i + 1, j, k) - load_e(g, E, 2, i - 1, j, k)))
-               - (0.5 * dtdz) * (load_e(g, E, 0, i, j, k + 1) - load_e(g, E, 0, i, j, k - 1));
-      out[7] = (Uin[7 * N + c] - (0.5 * dtdx) * (load_e(g, E, 1, i + 1, j, k) - load_e(g, E, 1, i - 1, j, k)))
-               + (0.5 * dtdy) * (load_e(g, E, 0, i, j + 1, k) - load_e(g, E, 0, i, j - 1, k));
+    if constexpr (FLUXCD) {
+      const long long ozp = e_nbr(k, g.nz, +1, g.bc[5]) * sz;
+      const double e1p = E[0 * N + c + ozp];
+      const double e2p = E[1 * N + c + ozp];
+      const double* E1 = E;
+      const double* E2 = E + N;
+      const double* E3 = E + 2 * N;
+      out[5] = (Uin[5 * N + c] - (0.5 * dtdy) * (E3[c + oyp] - E3[c + oym])) + (0.5 * dtdz) * (e2p - e2m);
+      out[6] = (Uin[6 * N + c] + (0.5 * dtdx) * (E3[c + oxp] - E3[c + oxm])) - (0.5 * dtdz) * (e1p - e1m);
+      out[7] = (Uin[7 * N + c] - (0.5 * dtdx) * (E2[c + oxp] - E2[c + oxm])) + (0.5 * dtdy) * (E1[c + oyp] - E1[c + oym]);
+      e1m = e1c;
+      e2m = e2c;
+      e1c = e1p;
+      e2c = e2p;
     }
 #pragma unroll
     for (int q = 0; q < 8; q++) Uout[q * N + c] = out[q];
@@ -233,41 +260,43 @@ __device__ __forceinline__ double wave_min(double v) {
 
 __global__ void __launch_bounds__(256) k_cfl(Geom g, Phys ph, const double* __restrict__ U, double dx,
                                              double dy, double dz, unsigned long long* __restrict__ dtmin,
-                                             int* __restrict__ err) {
+                                             int* __restrict__ err, unsigned bpp) {
   __shared__ double red[4];
-  const long long n = (long long)g.nx * g.ny * g.nz;
   double dtp = 1e30;
   bool bad = false;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
-       t += (long long)gridDim.x * blockDim.x) {
-    const int i = (int)(t % g.nx) + 1;
-    const long long r = t / g.nx;
-    const int j = (int)(r % g.ny) + 1;
-    const int k = (int)(r / g.ny) + 1;
-    const long long c = cidx(g, i, j, k);
-    double u[8], p[8];
+  int t, kc;
+  if (tile_item(bpp, (unsigned)(g.nx * g.ny), t, kc)) {
+    const int i = t % g.nx + 1, j = t / g.nx + 1;
+    const int k0 = kc * KCHUNK + 1;
+    const int k1 = min(k0 + KCHUNK - 1, g.nz);
+    for (int k = k0; k <= k1; k++) {
+      const long long c = cidx(g, i, j, k);
+      double u[8], p[8];
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-      u[q] = U[q * g.ncell + c];
-      bad |= !__builtin_isfinite(u[q]);
-    }
-    u2prim(ph, u, p);
-    if (ph.mhd) {
-      const double b2 = (p[5] * p[5] + p[6] * p[6]) + p[7] * p[7];
-      const double cx = cfast1(ph.gamma, p[4], p[0], b2, p[5]);
-      const double cy = cfast1(ph.gamma, p[4], p[0], b2, p[6]);
-      const double cz = cfast1(ph.gamma, p[4], p[0], b2, p[7]);
-      dtp = dmin(dtp, dx / (__builtin_fabs(p[1]) + cx));
-      dtp = dmin(dtp, dy / (__builtin_fabs(p[2]) + cy));
-      dtp = dmin(dtp, dz / (__builtin_fabs(p[3]) + cz));
-    } else {
-      const double cs = csound(ph, p[4], p[0]);
-      dtp = dmin(dtp, dx / (__builtin_fabs(p[1]) + cs));
-      dtp = dmin(dtp, dy / (__builtin_fabs(p[2]) + cs));
-      dtp = dmin(dtp, dz / (__builtin_fabs(p[3]) + cs));
+      for (int q = 0; q < 8; q++) {
+        u[q] = U[q * g.ncell + c];
+        bad |= !__builtin_isfinite(u[q]);
+      }
+      u2prim(ph, u, p);
+      if (ph.mhd) {
+        const double b2 = (p[5] * p[5] + p[6] * p[6]) + p[7] * p[7];
+        const double cx = cfast1(ph.gamma, p[4], p[0], b2, p[5]);
+        const double cy = cfast1(ph.gamma, p[4], p[0], b2, p[6]);
+        const double cz = cfast1(ph.gamma, p[4], p[0], b2, p[7]);
+        dtp = dmin(dtp, dx / (__builtin_fabs(p[1]) + cx));
+        dtp = dmin(dtp, dy / (__builtin_fabs(p[2]) + cy));
+        dtp = dmin(dtp, dz / (__builtin_fabs(p[3]) + cz));
+      } else {
+        const double cs = csound(ph, p[4], p[0]);
+        dtp = dmin(dtp, dx / (__builtin_fabs(p[1]) + cs));
+        dtp = dmin(dtp, dy / (__builtin_fabs(p[2]) + cs));
+        dtp = dmin(dtp, dz / (__builtin_fabs(p[3]) + cs));
+      }
     }
   }
-  if (bad) atomicOr(err, 2);
+  if (__any(bad)) {
+    if ((threadIdx.x & 63) == 0) atomicOr(err, 2);
+  }
   dtp = wave_min(dtp);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 0) red[wid] = dtp;
@@ -379,19 +408,30 @@ static inline unsigned grid_for(long long n, int block = 256) {
   return (unsigned)b;
 }
 
+static inline unsigned tiles(long long items) { return (unsigned)((items + 255) / 256); }
+
 void launch_prim(hipStream_t s, const Geom& g, const Phys& ph, const double* U, double* P, int lo) {
-  const long long n = (long long)(g.nx + 2 - 2 * lo) * (g.ny + 2 - 2 * lo) * (g.nz + 2 - 2 * lo);
-  hipLaunchKernelGGL(k_prim, dim3(grid_for(n)), dim3(256), 0, s, g, ph, U, P, lo);
+  const int ex = g.nx + 2 - 2 * lo, ey = g.ny + 2 - 2 * lo, ez = g.nz + 2 - 2 * lo;
+  const unsigned bpp = tiles((long long)ex * ey);
+  hipLaunchKernelGGL(k_prim, dim3(bpp * ez), dim3(256), 0, s, g, ph, U, P, lo, bpp);
+}
+
+template <int SOLVER, int ORDER, int AXIS>
+static void launch_flux_a(hipStream_t s, const Geom& g, const Phys& ph, const double* P, double* F, int* err) {
+  const long long e0 = AXIS == 0 ? g.nx + 1 : g.nx;
+  const long long e1 = AXIS == 1 ? g.ny + 1 : g.ny;
+  const long long e2 = AXIS == 2 ? g.nz + 1 : g.nz;
+  const unsigned bpp = tiles(e0 * e1);
+  hipLaunchKernelGGL((k_flux<SOLVER, ORDER, AXIS>), dim3(bpp * (unsigned)e2), dim3(256), 0, s, g, ph, P, F, err,
+                     bpp);
 }
 
 template <int SOLVER, int ORDER>
 static void launch_flux_t(hipStream_t s, const Geom& g, const Phys& ph, int axis, const double* P, double* F,
                           int* err) {
-  const long long e0 = axis == 0 ? g.nx + 1 : g.nx;
-  const long long e1 = axis == 1 ? g.ny + 1 : g.ny;
-  const long long e2 = axis == 2 ? g.nz + 1 : g.nz;
-  const long long n = e0 * e1 * e2;
-  hipLaunchKernelGGL((k_flux<SOLVER, ORDER>), dim3(grid_for(n)), dim3(256), 0, s, g, ph, axis, P, F, err);
+  if (axis == 0) launch_flux_a<SOLVER, ORDER, 0>(s, g, ph, P, F, err);
+  else if (axis == 1) launch_flux_a<SOLVER, ORDER, 1>(s, g, ph, P, F, err);
+  else launch_flux_a<SOLVER, ORDER, 2>(s, g, ph, P, F, err);
 }
 
 void launch_flux(hipStream_t s, const Geom& g, const Phys& ph, int solver, int order, int axis,
@@ -412,24 +452,28 @@ void launch_flux(hipStream_t s, const Geom& g, const Phys& ph, int solver, int o
 
 void launch_efield(hipStream_t s, const Geom& g, const double* Fx, const double* Fy, const double* Fz,
                    double* E) {
-  const long long n = (long long)g.nx * g.ny * g.nz;
-  hipLaunchKernelGGL(k_efield, dim3(grid_for(n)), dim3(256), 0, s, g, Fx, Fy, Fz, E);
+  const unsigned bpp = tiles((long long)g.nx * g.ny);
+  hipLaunchKernelGGL(k_efield, dim3(bpp * g.nz), dim3(256), 0, s, g, Fx, Fy, Fz, E, bpp);
 }
 
 void launch_update(hipStream_t s, const Geom& g, const double* Uin, double* Uout, const double* Fx,
                    const double* Fy, const double* Fz, const double* E, double dt, double dx, double dy,
                    double dz, int flux_cd) {
-  const long long n = (long long)g.nx * g.ny * g.nz;
-  hipLaunchKernelGGL(k_update, dim3(grid_for(n)), dim3(256), 0, s, g, Uin, Uout, Fx, Fy, Fz, E, dt, dx, dy,
-                     dz, flux_cd);
+  const unsigned bpp = tiles((long long)g.nx * g.ny);
+  const unsigned nchunk = (unsigned)((g.nz + KCHUNK - 1) / KCHUNK);
+  if (flux_cd)
+    hipLaunchKernelGGL(k_update<1>, dim3(bpp * nchunk), dim3(256), 0, s, g, Uin, Uout, Fx, Fy, Fz, E, dt, dx, dy,
+                       dz, bpp);
+  else
+    hipLaunchKernelGGL(k_update<0>, dim3(bpp * nchunk), dim3(256), 0, s, g, Uin, Uout, Fx, Fy, Fz, E, dt, dx, dy,
+                       dz, bpp);
 }
 
 void launch_cfl(hipStream_t s, const Geom& g, const Phys& ph, const double* U, double dx, double dy, double dz,
                 unsigned long long* dtmin, int* err) {
-  const long long n = (long long)g.nx * g.ny * g.nz;
-  unsigned b = grid_for(n);
-  if (b > 4096) b = 4096;
-  hipLaunchKernelGGL(k_cfl, dim3(b), dim3(256), 0, s, g, ph, U, dx, dy, dz, dtmin, err);
+  const unsigned bpp = tiles((long long)g.nx * g.ny);
+  const unsigned nchunk = (unsigned)((g.nz + KCHUNK - 1) / KCHUNK);
+  hipLaunchKernelGGL(k_cfl, dim3(bpp * nchunk), dim3(256), 0, s, g, ph, U, dx, dy, dz, dtmin, err, bpp);
 }
 
 void launch_ghost1(hipStream_t s, const Geom& g, double* U, int axis, int side, int kind) {

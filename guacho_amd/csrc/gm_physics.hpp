@@ -35,12 +35,16 @@ __device__ __forceinline__ double dmax(double a, double b) { return (a > b) ? a 
 __device__ __forceinline__ double dmin(double a, double b) { return (a < b) ? a : b; }
 __device__ __forceinline__ double sign1(double x) { return __builtin_copysign(1.0, x); }
 
-// y faces exchange slots (1,2) and (5,6); z faces (1,3) and (5,7).
-__device__ __forceinline__ void axis_perm(int axis, double v[8]) {
-  if (axis == 1) {
+// y faces exchange slots (1,2) and (5,6); z faces (1,3) and (5,7).  The
+// axis is a compile-time constant so the permutation is pure register
+// renaming (a run-time axis makes the compiler index the array, which
+// moves it to scratch memory).
+template <int AXIS>
+__device__ __forceinline__ void axis_perm(double v[8]) {
+  if constexpr (AXIS == 1) {
     double a = v[1]; v[1] = v[2]; v[2] = a;
     a = v[5]; v[5] = v[6]; v[6] = a;
-  } else if (axis == 2) {
+  } else if constexpr (AXIS == 2) {
     double a = v[1]; v[1] = v[3]; v[3] = a;
     a = v[5]; v[5] = v[7]; v[7] = a;
   }
@@ -203,6 +207,14 @@ __device__ __forceinline__ double hlld_energy(double cv, const double s[8], doub
 
 // prim2fhlld, src/hlld.f90:48-319.  Returns false on the fall-through
 // (non-finite input) that stops the reference at :316-317.
+//
+// Organised for a 64-wide wavefront: both sides' star states are built for
+// every subsonic interface (the double-star regions need both anyway), the
+// side that feeds the flux is chosen by value selects, and only the
+// double-star averages (:241-252) sit in a divergent branch.  Each region's
+// flux is the same expression, on the same operands, as the reference's
+// branch for that region: UL* :111-156, UR* :159-204, UL** :255-283,
+// UR** :286-314.
 __device__ __forceinline__ bool hlld(const Phys& ph, const double pl[8], const double pr[8], double ff[8]) {
   const double cv = ph.cv;
   const double csl = cfastX(ph, pl), csr = cfastX(ph, pr);
@@ -228,46 +240,16 @@ __device__ __forceinline__ bool hlld(const Phys& ph, const double pl[8], const d
   const double pst = (((srmur * pr[0]) * ptl - (slmul * pl[0]) * ptr)
                       + (((pl[0] * pr[0]) * srmur) * slmul) * (pr[1] - pl[1])) / den_m;
 
-  if (sstl >= 0 || sstr <= 0) {
-    // single-star regions UL* (:111-156) and UR* (:159-204): one side's
-    // star state; the two formulas differ only by which side feeds them.
-    const bool left = (sstl >= 0);
-    const double* s = left ? pl : pr;
-    const double smu = left ? slmul : srmur;
-    const double smsM = left ? slmsM : srmsM;
-    const double rhost = left ? rhostl : rhostr;
-    const double pt = left ? ptl : ptr;
-    const double e = hlld_energy(cv, s, bx);
-    double vst, wst, byst, bzst;
-    hlld_star_side(s, bx, smu, smsM, sM - s[1], vst, wst, byst, bzst);
-    const double vdotb = (s[1] * bx + s[2] * s[6]) + s[3] * s[7];
-    const double vstdotbst = (sM * bx + vst * byst) + wst * bzst;
-    const double est = (((smu * e - pt * s[1]) + pst * sM) + bx * (vdotb - vstdotbst)) / smsM;
-    ff[0] = rhost * sM;
-    ff[1] = (rhost * (sM * sM) + pst) - bx * bx;
-    ff[2] = (rhost * sM) * vst - bx * byst;
-    ff[3] = (rhost * sM) * wst - bx * bzst;
-    ff[4] = sM * (est + pst) - bx * vstdotbst;
-    ff[5] = 0.0;
-    ff[6] = byst * sM - bx * vst;
-    ff[7] = bzst * sM - bx * wst;
-    return true;
-  }
-
-  // double-star regions UL** (:255-283) and UR** (:286-314)
   double vstl, wstl, bystl, bzstl, vstr, wstr, bystr, bzstr;
   hlld_star_side(pl, bx, slmul, slmsM, sM - pl[1], vstl, wstl, bystl, bzstl);
   hlld_star_side(pr, bx, srmur, srmsM, sM - pr[1], vstr, wstr, bystr, bzstr);
-  const double dd = sqrl + sqrr;
-  const double vstst = ((sqrl * vstl + sqrr * vstr) + (bystr - bystl) * signbx) / dd;
-  const double wstst = ((sqrl * wstl + sqrr * wstr) + (bzstr - bzstl) * signbx) / dd;
-  const double sqlr = __builtin_sqrt(rhostl * rhostr);
-  const double bystst = ((sqrl * bystr + sqrr * bystl) + (sqlr * (vstr - vstl)) * signbx) / dd;
-  const double bzstst = ((sqrl * bzstr + sqrr * bzstl) + (sqlr * (wstr - wstl)) * signbx) / dd;
-  const double vststdotbstst = (sM * bx + vstst * bystst) + wstst * bzstst;
-  if (!(sM >= 0) && !(sM <= 0)) return false;
-  const bool left = (sM >= 0);
-  const double* s = left ? pl : pr;
+
+  const bool single = (sstl >= 0) || (sstr <= 0);
+  if (!single && !(sM >= 0) && !(sM <= 0)) return false;
+  const bool left = single ? (sstl >= 0) : (sM >= 0);
+  double s[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) s[q] = left ? pl[q] : pr[q];
   const double smu = left ? slmul : srmur;
   const double smsM = left ? slmsM : srmsM;
   const double rhost = left ? rhostl : rhostr;
@@ -279,17 +261,33 @@ __device__ __forceinline__ bool hlld(const Phys& ph, const double pl[8], const d
   const double vdotb = (s[1] * bx + s[2] * s[6]) + s[3] * s[7];
   const double vstdotbst = (sM * bx + vst * byst) + wst * bzst;
   const double est = (((smu * e - pt * s[1]) + pst * sM) + bx * (vdotb - vstdotbst)) / smsM;
-  // eL** = eL* - sqrt(rhoL*)(...)signBx ; eR** = eR* + sqrt(rhoR*)(...)signBx
-  const double corr = (sqr * (vstdotbst - vststdotbstst)) * signbx;
-  const double estst = left ? est - corr : est + corr;
+
+  double vt = vst, wt = wst, byt = byst, bzt = bzst, et = est, vdb = vstdotbst;
+  if (!single) {
+    const double dd = sqrl + sqrr;
+    const double vstst = ((sqrl * vstl + sqrr * vstr) + (bystr - bystl) * signbx) / dd;
+    const double wstst = ((sqrl * wstl + sqrr * wstr) + (bzstr - bzstl) * signbx) / dd;
+    const double sqlr = __builtin_sqrt(rhostl * rhostr);
+    const double bystst = ((sqrl * bystr + sqrr * bystl) + (sqlr * (vstr - vstl)) * signbx) / dd;
+    const double bzstst = ((sqrl * bzstr + sqrr * bzstl) + (sqlr * (wstr - wstl)) * signbx) / dd;
+    const double vststdotbstst = (sM * bx + vstst * bystst) + wstst * bzstst;
+    // eL** = eL* - sqrt(rhoL*)(...)signBx ; eR** = eR* + sqrt(rhoR*)(...)signBx
+    const double corr = (sqr * (vstdotbst - vststdotbstst)) * signbx;
+    et = left ? est - corr : est + corr;
+    vt = vstst;
+    wt = wstst;
+    byt = bystst;
+    bzt = bzstst;
+    vdb = vststdotbstst;
+  }
   ff[0] = rhost * sM;
   ff[1] = (rhost * (sM * sM) + pst) - bx * bx;
-  ff[2] = (rhost * sM) * vstst - bx * bystst;
-  ff[3] = (rhost * sM) * wstst - bx * bzstst;
-  ff[4] = sM * (estst + pst) - bx * vststdotbstst;
+  ff[2] = (rhost * sM) * vt - bx * byt;
+  ff[3] = (rhost * sM) * wt - bx * bzt;
+  ff[4] = sM * (et + pst) - bx * vdb;
   ff[5] = 0.0;
-  ff[6] = bystst * sM - bx * vstst;
-  ff[7] = bzstst * sM - bx * wstst;
+  ff[6] = byt * sM - bx * vt;
+  ff[7] = bzt * sM - bx * wt;
   return true;
 }
 
@@ -339,7 +337,9 @@ __device__ __forceinline__ bool hllc(const Phys& ph, const double pl[8], const d
   const double sst = (((srmur * rhorur - slmul * rholul) - pr[4]) + pl[4]) / (srmur * pr[0] - slmul * pl[0]);
   if (!(sst >= 0.0) && !(sst <= 0.0)) return false;
   const bool left = (sst >= 0.0);
-  const double* s = left ? pl : pr;
+  double s[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) s[q] = left ? pl[q] : pr[q];
   const double smu = left ? slmul : srmur;
   const double sw = left ? sl : sr;
   const double rhost = (s[0] * smu) / (sw - sst);

@@ -1,0 +1,110 @@
+!=======================================================================
+! The device drop-in for the reference's step routines.  Each subroutine
+! keeps the name and argument meaning of the routine it replaces and runs
+! it on the GPU through guacho_capi:
+!   boundaryI     -> gm_boundary_u   (src/boundaries.f90:44-247)
+!   calcprim      -> gm_download_primit (src/hydro_core.f90:245-320)
+!   get_timestep  -> gm_cfl_dtp + the ramp and clamp of
+!                    src/hydro_core.f90:677-694, evaluated here
+!   tstep         -> gm_tstep        (src/hydro_solver.f90:134-229)
+! Errors print the library's message and stop, as the reference stops on
+! its own errors.
+!=======================================================================
+module gm_step
+  use iso_c_binding
+  use guacho_capi
+  use parameters
+  use globals
+  implicit none
+  type(c_ptr), save :: gm_ctx = c_null_ptr
+
+contains
+
+  subroutine gm_check(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: what
+    if (rc /= GM_OK) then
+      print '(a,a,a,i0,a,a)', 'guacho_amd: ', what, ' failed (', rc, '): ', gm_error_string(gm_ctx)
+      stop 1
+    end if
+  end subroutine gm_check
+
+  !> create the device subdomain and upload u (after initflow)
+  subroutine gm_device_init()
+    type(gm_params) :: p
+    integer :: dev
+    character(len=16) :: env
+    integer :: envlen, envstat
+    dev = 0
+    call get_environment_variable('GUACHO_DEVICE', env, envlen, envstat)
+    if (envstat == 0 .and. envlen > 0) read(env, *) dev
+    p%nx = nx; p%ny = ny; p%nz = nz
+    p%neq = neq
+    p%nghost = nghost
+    p%riemann_solver = riemann_solver
+    p%slope_limiter = slope_limiter
+    p%bc = (/ bc_left, bc_right, bc_bottom, bc_top, bc_out, bc_in /)
+    if (bc_user) p%bc = BC_OTHER
+    p%mhd = merge(1, 0, mhd)
+    p%pmhd = merge(1, 0, pmhd)
+    p%enable_flux_cd = merge(1, 0, enable_flux_cd)
+    p%eight_wave = merge(1, 0, eight_wave .or. user_source_terms .or. radiation_pressure)
+    p%precision = GM_PRECISION_FP64
+    p%device = dev
+    p%reserved = 0
+    p%dx = dx; p%dy = dy; p%dz = dz
+    p%cv = cv
+    p%gamma = gamma
+    p%eta = eta
+    if (cooling /= COOL_NONE .or. th_cond /= TC_OFF .or. dif_rad .or. eq_of_state /= EOS_ADIABATIC) then
+      print '(a)', 'guacho_amd: cooling, conduction, diffuse radiation and non-adiabatic EOS are not built'
+      stop 1
+    end if
+    call gm_check(gm_create(p, gm_ctx), 'gm_create')
+    call gm_check(gm_upload_u(gm_ctx, u), 'gm_upload_u')
+  end subroutine gm_device_init
+
+  subroutine gm_device_finalize()
+    integer(c_int) :: rc
+    if (c_associated(gm_ctx)) rc = gm_destroy(gm_ctx)
+    gm_ctx = c_null_ptr
+  end subroutine gm_device_finalize
+
+  !> host copy of u (ghosts after boundaryI), e.g. for dumps
+  subroutine gm_download()
+    call gm_check(gm_download_u(gm_ctx, u), 'gm_download_u')
+  end subroutine gm_download
+
+  subroutine boundaryI()
+    call gm_check(gm_boundary_u(gm_ctx), 'gm_boundary_u')
+  end subroutine boundaryI
+
+  !> primitives of the device state into primit (and Temp)
+  subroutine calcprim()
+    call gm_check(gm_download_primit(gm_ctx, primit, Temp), 'gm_download_primit')
+  end subroutine calcprim
+
+  subroutine get_timestep(current_iter, n_iter, current_time, tprint, dt, dump_flag)
+    integer, intent(in) :: current_iter, n_iter
+    real, intent(in) :: current_time, tprint
+    real, intent(out) :: dt
+    logical, intent(inout) :: dump_flag
+    real(c_double) :: dtp
+    call gm_check(gm_cfl_dtp(gm_ctx, dtp), 'gm_cfl_dtp')
+    if (current_iter <= n_iter) then
+      dtp = cfl * (2.**(-real(n_iter + 1 - current_iter))) * dtp
+    else
+      dtp = cfl * dtp
+    end if
+    dt = dtp
+    if ((current_time + dt) >= tprint) then
+      dt = tprint - current_time
+      dump_flag = .true.
+    end if
+  end subroutine get_timestep
+
+  subroutine tstep()
+    call gm_check(gm_tstep(gm_ctx, dt_CFL), 'gm_tstep')
+  end subroutine tstep
+
+end module gm_step
