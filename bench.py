@@ -7,10 +7,11 @@ loop (src/main.f90:94-125): the CFL reduction (get_timestep) and one tstep,
 state resident in HBM.
 
 Prints one JSON line: zone-updates/s over the timed steps, the HBM roofline
-of the dominant kernel (limited-reconstruction flux sweep) from HIP events
-on the library's stream, the step-level roofline at the 128 B/zone-update
-compulsory traffic, and the C restatement of the reference's CPU path
-timed on a bounded sample on the host cores.
+of the dominant kernel class from HIP events on the library's stream, the
+step-level roofline at the 128 B/zone-update compulsory traffic, the
+mixed-precision form on the 512^3 box against the FP64 path (speed, per-field
+error, drift, div B), and the C restatement of the reference's CPU path timed
+on a bounded sample on the host cores.  Progress goes to stderr.
 """
 import argparse
 import json
@@ -33,6 +34,9 @@ def parse():
     ap.add_argument("--n", type=int, default=256, help="cells per side of the turbulence box")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision section")
+    ap.add_argument("--mixed-n", type=int, default=512)
+    ap.add_argument("--mixed-steps", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     return ap.parse_args()
 
@@ -114,6 +118,72 @@ def dominant_roofline(kt, n, steps):
             "traffic_source": src}
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def run_steps(dev, u0, steps, warmup, cfl=0.4):
+    """warm-up, re-upload the seeded state, then `steps` timed steps; returns seconds."""
+    from guacho_amd.host import timestep_clamp
+    dev.upload_u(u0)
+    dev.boundary_u()
+    t = 0.0
+    for it in range(1, warmup + 1):
+        dt, _ = timestep_clamp(it, 10, cfl, dev.cfl_dtp(), t, 1e30)
+        dev.tstep(dt)
+        t += dt
+    dev.sync()
+    dev.upload_u(u0)
+    dev.boundary_u()
+    dev.sync()
+    t = 0.0
+    t0 = time.perf_counter()
+    for it in range(1, steps + 1):
+        dt, _ = timestep_clamp(it, 10, cfl, dev.cfl_dtp(), t, 1e30)
+        dev.tstep(dt)
+        t += dt
+    dev.sync()
+    return time.perf_counter() - t0
+
+
+def mixed_section(n: int, steps: int, seed: int):
+    """The mixed-precision form on the seeded turbulence box (FP32 states,
+    reconstruction and fluxes; FP64 pressure recovery, update accumulation
+    and reductions) against the FP64 path: zone-updates/s of both, per-field
+    relative L1 error, mass and energy drift, max |div B|."""
+    import numpy as np
+    from guacho_amd import _capi, problems
+    log(f"mixed section: generating turbulence {n}^3")
+    u0, dx, dy, dz = problems.turbulence(n, seed=seed)
+    res = {}
+    for name, prec in (("fp64", _capi.GM_PRECISION_FP64), ("mixed", _capi.GM_PRECISION_MIXED)):
+        log(f"mixed section: {name} {steps} steps")
+        prm = _capi.make_params(n, n, n, dx, dy, dz, precision=prec)
+        with _capi.Subdomain(prm) as dev:
+            el = run_steps(dev, u0, steps, warmup=2)
+            res[name] = {"value": n ** 3 * steps / el, "ms_per_step": 1e3 * el / steps,
+                         "u": dev.download_u(), "divb": float(np.abs(dev.divb()).max())}
+    a = res["mixed"]["u"][:, 2:-2, 2:-2, 2:-2]
+    b = res["fp64"]["u"][:, 2:-2, 2:-2, 2:-2]
+    names = ["rho", "mx", "my", "mz", "E", "Bx", "By", "Bz"]
+    l1 = {names[q]: float(np.abs(a[q] - b[q]).sum() / np.abs(b[q]).sum()) for q in range(8)}
+    s0 = u0[:, 2:-2, 2:-2, 2:-2].sum(axis=(1, 2, 3))
+    drift = {k: {"mass": float(abs(res[k]["u"][0, 2:-2, 2:-2, 2:-2].sum() - s0[0]) / abs(s0[0])),
+                 "energy": float(abs(res[k]["u"][4, 2:-2, 2:-2, 2:-2].sum() - s0[4]) / abs(s0[4]))}
+             for k in ("fp64", "mixed")}
+    return {
+        "workload": f"turbulence-{n}^3-hlld-minmod-fluxcd-cfl0.4, {steps} steps",
+        "value": round(res["mixed"]["value"], 1), "unit": "zone-updates/s",
+        "ms_per_step": round(res["mixed"]["ms_per_step"], 4),
+        "fp64_value": round(res["fp64"]["value"], 1), "fp64_ms_per_step": round(res["fp64"]["ms_per_step"], 4),
+        "speedup_vs_fp64": round(res["mixed"]["value"] / res["fp64"]["value"], 3),
+        "rel_l1_vs_fp64": l1, "drift": drift,
+        "max_abs_divb": {"fp64": res["fp64"]["divb"], "mixed": res["mixed"]["divb"]},
+        "bytes_per_zone_update": 64,
+        "step_roofline_frac": round(64.0 * n ** 3 / (res["mixed"]["ms_per_step"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -127,8 +197,10 @@ def main():
     u0, dx, dy, dz = problems.turbulence(n, seed=args.seed)
     prm = _capi.make_params(n, n, n, dx, dy, dz)
     cfl = 0.4
+    log(f"FP64 turbulence {n}^3: {args.warmup} warm-up + {args.steps} timed steps")
     with _capi.Subdomain(prm) as dev:
-        # warm-up on the same state, then restart from the seeded state
+        dev.kernel_times()                      # reset counters
+        # warm-up runs untimed inside run_steps; kernel timing covers the timed steps only
         dev.upload_u(u0)
         dev.boundary_u()
         t = 0.0
@@ -140,7 +212,7 @@ def main():
         dev.upload_u(u0)
         dev.boundary_u()
         dev.sync()
-        dev.kernel_times()                      # reset counters
+        dev.kernel_times()
         dev.set_timing(True)
         t = 0.0
         t0 = time.perf_counter()
@@ -171,7 +243,10 @@ def main():
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_ach / HBM_PEAK_GBS, 4)},
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items() if v[1]},
     }
+    if not args.no_mixed:
+        out["mixed"] = mixed_section(args.mixed_n, args.mixed_steps, args.seed)
     if not args.no_cpu_baseline:
+        log("CPU baseline (C restatement on the host cores)")
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out), flush=True)
 

@@ -46,11 +46,13 @@ struct gm_ctx {
   gm::Geom g;
   gm::Phys ph;
   hipStream_t stream = nullptr;
-  double* U = nullptr;   // state U^n (SoA, 8 vars, ghosts as uploaded / filled)
-  double* Us = nullptr;  // U* after the predictor
-  double* P = nullptr;   // primitives; also the staging buffer for layout conversion
-  double* F[3] = {nullptr, nullptr, nullptr};
-  double* E = nullptr;   // cell-centred E (3 vars); also staging for Temp / divB
+  size_t esize = 8;      // bytes per stored value: 8 (FP64) or 4 (mixed, FP32 storage)
+  void* U = nullptr;     // state U^n (SoA, 8 vars, ghosts as uploaded / filled)
+  void* Us = nullptr;    // U* after the predictor
+  void* P = nullptr;     // primitives
+  void* Fb = nullptr;    // fluxes of the three axes (3 x 8 vars); also the staging
+                         // buffer for layout conversion (>= 8 doubles per cell)
+  void* E = nullptr;     // cell-centred E (3 vars); also staging for Temp / divB
   Scalars* dscal = nullptr;  // device scalars
   Scalars* hscal = nullptr;  // pinned host mirror
   bool ghosts_dirty = true;
@@ -140,8 +142,8 @@ int validate(const gm_params* p) {
       return fail(nullptr, GM_ERR_CONFIG, "a periodic boundary must pair with a periodic boundary on the same axis");
   if (p->eight_wave) return fail(nullptr, GM_ERR_CONFIG, "eight_wave (Powell sources) is not built");
   if (p->eta != 0.0) return fail(nullptr, GM_ERR_CONFIG, "eta (artificial viscosity) must be 0");
-  if (p->precision != GM_PRECISION_FP64)
-    return fail(nullptr, GM_ERR_CONFIG, "only GM_PRECISION_FP64 is available in this build");
+  if (p->precision != GM_PRECISION_FP64 && p->precision != GM_PRECISION_MIXED)
+    return fail(nullptr, GM_ERR_CONFIG, "precision must be GM_PRECISION_FP64 or GM_PRECISION_MIXED");
   if (!(p->dx > 0) || !(p->dy > 0) || !(p->dz > 0) || !std::isfinite(p->dx) || !std::isfinite(p->dy) ||
       !std::isfinite(p->dz))
     return fail(nullptr, GM_ERR_CONFIG, "dx, dy, dz must be positive and finite");
@@ -159,8 +161,8 @@ void release(gm_ctx* c) {
     (void)hipEventDestroy(r.b);
   }
   for (auto e : c->pool) (void)hipEventDestroy(e);
-  double* bufs[] = {c->U, c->Us, c->P, c->F[0], c->F[1], c->F[2], c->E};
-  for (double* b : bufs)
+  void* bufs[] = {c->U, c->Us, c->P, c->Fb, c->E};
+  for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->dscal) (void)hipFree(c->dscal);
   if (c->hscal) (void)hipHostFree(c->hscal);
@@ -172,19 +174,62 @@ int set_device(gm_ctx* c) { return hip_check(c, hipSetDevice(c->p.device), "hipS
 
 // boundaryI ghost layer on U, in the reference's serial order: periodic
 // x, y, z; then closed left..in; then outflow left..in.
-void fill_ghost1(gm_ctx* c) {
+template <typename S>
+void fill_ghost1_t(gm_ctx* c) {
   const gm::Geom& g = c->g;
+  S* U = (S*)c->U;
   timed(c, KC_GHOST, [&] {
     for (int a = 0; a < 3; a++)
       if (g.bc[2 * a] == GM_BC_PERIODIC) {
-        gm::launch_ghost1(c->stream, g, c->U, a, 0, GM_BC_PERIODIC);
-        gm::launch_ghost1(c->stream, g, c->U, a, 1, GM_BC_PERIODIC);
+        gm::launch_ghost1(c->stream, g, U, a, 0, GM_BC_PERIODIC);
+        gm::launch_ghost1(c->stream, g, U, a, 1, GM_BC_PERIODIC);
       }
     for (int kind : {GM_BC_CLOSED, GM_BC_OUTFLOW})
       for (int f = 0; f < 6; f++)
-        if (g.bc[f] == kind) gm::launch_ghost1(c->stream, g, c->U, f / 2, f % 2, kind);
+        if (g.bc[f] == kind) gm::launch_ghost1(c->stream, g, U, f / 2, f % 2, kind);
   });
   c->ghosts_dirty = false;
+}
+
+void fill_ghost1(gm_ctx* c) {
+  if (c->esize == 8) fill_ghost1_t<double>(c);
+  else fill_ghost1_t<float>(c);
+}
+
+// the step of one precision: tstep (src/hydro_solver.f90:134-229)
+template <typename S>
+void tstep_t(gm_ctx* c, double dt) {
+  const gm::Geom& g = c->g;
+  const gm_params& p = c->p;
+  hipStream_t s = c->stream;
+  int* err = &c->dscal->err;
+  S* U = (S*)c->U;
+  S* Us = (S*)c->Us;
+  S* P = (S*)c->P;
+  S* F[3] = {(S*)c->Fb, (S*)c->Fb + 8 * g.ncell, (S*)c->Fb + 16 * g.ncell};
+  S* E = (S*)c->E;
+  const double dtm = dt / 2.0;  // hydro_solver.f90:152
+  // predictor: first-order fluxes of U^n, U* = U^n - dt/2 div F
+  timed(c, KC_PRIM, [&] { gm::launch_prim(s, g, c->ph, (const S*)U, P, 0); });
+  for (int a = 0; a < 3; a++)
+    timed(c, KC_FLUX1, [&] { gm::launch_flux(s, g, c->ph, p.riemann_solver, 1, a, (const S*)P, F[a], err); });
+  if (p.enable_flux_cd)
+    timed(c, KC_EFIELD, [&] { gm::launch_efield(s, g, (const S*)F[0], (const S*)F[1], (const S*)F[2], E); });
+  timed(c, KC_UPDATE, [&] {
+    gm::launch_update(s, g, (const S*)U, Us, (const S*)F[0], (const S*)F[1], (const S*)F[2], (const S*)E, dtm,
+                      p.dx, p.dy, p.dz, p.enable_flux_cd);
+  });
+  // corrector: limited fluxes of U*, U^{n+1} = U^n - dt div F (in place;
+  // viscous_copy with eta = 0 is the identity on values)
+  timed(c, KC_PRIM, [&] { gm::launch_prim(s, g, c->ph, (const S*)Us, P, -1); });
+  for (int a = 0; a < 3; a++)
+    timed(c, KC_FLUX2, [&] { gm::launch_flux(s, g, c->ph, p.riemann_solver, 2, a, (const S*)P, F[a], err); });
+  if (p.enable_flux_cd)
+    timed(c, KC_EFIELD, [&] { gm::launch_efield(s, g, (const S*)F[0], (const S*)F[1], (const S*)F[2], E); });
+  timed(c, KC_UPDATE, [&] {
+    gm::launch_update(s, g, (const S*)U, U, (const S*)F[0], (const S*)F[1], (const S*)F[2], (const S*)E, dt, p.dx,
+                      p.dy, p.dz, p.enable_flux_cd);
+  });
 }
 
 int read_errors(gm_ctx* c) {
@@ -231,30 +276,24 @@ int gm_create(const gm_params* params, gm_ctx** out) {
   if ((rc = set_device(c)) != GM_OK) return bail(rc);
   if ((rc = hip_check(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate")) != GM_OK)
     return bail(rc);
-  const size_t n8 = (size_t)g.ncell * 8 * sizeof(double);
-  const size_t n3 = (size_t)g.ncell * 3 * sizeof(double);
-  double** bufs8[] = {&c->U, &c->Us, &c->P, &c->F[0], &c->F[1], &c->F[2]};
-  for (double** b : bufs8) {
-    if (hipMalloc((void**)b, n8) != hipSuccess) {
+  c->esize = params->precision == GM_PRECISION_MIXED ? 4 : 8;
+  const size_t n8 = (size_t)g.ncell * 8 * c->esize;
+  const size_t n3 = (size_t)g.ncell * 3 * c->esize;
+  struct { void** p; size_t n; } allocs[] = {{&c->U, n8}, {&c->Us, n8}, {&c->P, n8}, {&c->Fb, 3 * n8}, {&c->E, n3}};
+  for (auto& a : allocs) {
+    if (hipMalloc(a.p, a.n) != hipSuccess) {
       (void)hipGetLastError();
       c->msg = "device allocation failed";
       return bail(GM_ERR_OOM);
     }
-    c->bytes += (int64_t)n8;
+    c->bytes += (int64_t)a.n;
   }
-  if (hipMalloc((void**)&c->E, n3) != hipSuccess) {
-    (void)hipGetLastError();
-    c->msg = "device allocation failed";
-    return bail(GM_ERR_OOM);
-  }
-  c->bytes += (int64_t)n3;
   if ((rc = hip_check(c, hipMalloc((void**)&c->dscal, sizeof(Scalars)), "hipMalloc")) != GM_OK) return bail(rc);
   if ((rc = hip_check(c, hipHostMalloc((void**)&c->hscal, sizeof(Scalars), hipHostMallocDefault),
                       "hipHostMalloc")) != GM_OK)
     return bail(rc);
   // zero everything once so no kernel ever reads uninitialised memory
-  for (double** b : bufs8) (void)hipMemsetAsync(*b, 0, n8, c->stream);
-  (void)hipMemsetAsync(c->E, 0, n3, c->stream);
+  for (auto& a : allocs) (void)hipMemsetAsync(*a.p, 0, a.n, c->stream);
   (void)hipMemsetAsync(c->dscal, 0, sizeof(Scalars), c->stream);
   if ((rc = hip_check(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize")) != GM_OK) return bail(rc);
   *out = c;
@@ -276,8 +315,11 @@ int gm_upload_u(gm_ctx* c, const double* u) {
   int rc = set_device(c);
   if (rc) return rc;
   const size_t n8 = (size_t)c->g.ncell * 8 * sizeof(double);
-  GM_HIP(c, hipMemcpyAsync(c->P, u, n8, hipMemcpyHostToDevice, c->stream));
-  timed(c, KC_LAYOUT, [&] { gm::launch_aos_to_soa(c->stream, c->g.ncell, c->P, c->U); });
+  GM_HIP(c, hipMemcpyAsync(c->Fb, u, n8, hipMemcpyHostToDevice, c->stream));
+  timed(c, KC_LAYOUT, [&] {
+    if (c->esize == 8) gm::launch_aos_to_soa(c->stream, c->g.ncell, (const double*)c->Fb, (double*)c->U);
+    else gm::launch_aos_to_soa(c->stream, c->g.ncell, (const double*)c->Fb, (float*)c->U);
+  });
   GM_HIP(c, hipGetLastError());
   GM_HIP(c, hipStreamSynchronize(c->stream));
   c->ghosts_dirty = false;  // the caller's ghosts are the reference's ghosts
@@ -299,9 +341,12 @@ int gm_download_u(gm_ctx* c, double* u) {
   int rc = set_device(c);
   if (rc) return rc;
   if (c->ghosts_dirty) fill_ghost1(c);
-  timed(c, KC_LAYOUT, [&] { gm::launch_soa_to_aos(c->stream, c->g.ncell, 8, c->U, c->P); });
+  timed(c, KC_LAYOUT, [&] {
+    if (c->esize == 8) gm::launch_soa_to_aos(c->stream, c->g.ncell, (const double*)c->U, (double*)c->Fb);
+    else gm::launch_soa_to_aos(c->stream, c->g.ncell, (const float*)c->U, (double*)c->Fb);
+  });
   GM_HIP(c, hipGetLastError());
-  GM_HIP(c, hipMemcpyAsync(u, c->P, (size_t)c->g.ncell * 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  GM_HIP(c, hipMemcpyAsync(u, c->Fb, (size_t)c->g.ncell * 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   GM_HIP(c, hipStreamSynchronize(c->stream));
   return GM_OK;
 }
@@ -312,10 +357,13 @@ int gm_download_primit(gm_ctx* c, double* primit, double* temp) {
   int rc = set_device(c);
   if (rc) return rc;
   if (c->ghosts_dirty) fill_ghost1(c);
-  double* T = temp ? c->E : nullptr;
-  timed(c, KC_PRIM, [&] { gm::launch_prim_all(c->stream, c->g.ncell, c->ph, c->U, c->P, T); });
+  double* T = temp ? (double*)c->E : nullptr;
+  timed(c, KC_PRIM, [&] {
+    if (c->esize == 8) gm::launch_prim_all(c->stream, c->g.ncell, c->ph, (const double*)c->U, (double*)c->Fb, T);
+    else gm::launch_prim_all(c->stream, c->g.ncell, c->ph, (const float*)c->U, (double*)c->Fb, T);
+  });
   GM_HIP(c, hipGetLastError());
-  GM_HIP(c, hipMemcpyAsync(primit, c->P, (size_t)c->g.ncell * 8 * sizeof(double), hipMemcpyDeviceToHost,
+  GM_HIP(c, hipMemcpyAsync(primit, c->Fb, (size_t)c->g.ncell * 8 * sizeof(double), hipMemcpyDeviceToHost,
                            c->stream));
   if (temp)
     GM_HIP(c, hipMemcpyAsync(temp, c->E, (size_t)c->g.ncell * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -328,7 +376,8 @@ int gm_download_divb(gm_ctx* c, double* divb) {
   if (!divb) return fail(c, GM_ERR_ARG, "divb is NULL");
   int rc = set_device(c);
   if (rc) return rc;
-  gm::launch_divb(c->stream, c->g, c->U, c->p.dx, c->p.dy, c->p.dz, c->E);
+  if (c->esize == 8) gm::launch_divb(c->stream, c->g, (const double*)c->U, c->p.dx, c->p.dy, c->p.dz, (double*)c->E);
+  else gm::launch_divb(c->stream, c->g, (const float*)c->U, c->p.dx, c->p.dy, c->p.dz, (double*)c->E);
   GM_HIP(c, hipGetLastError());
   const size_t n = (size_t)c->g.nx * c->g.ny * c->g.nz;
   GM_HIP(c, hipMemcpyAsync(divb, c->E, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -343,7 +392,12 @@ int gm_cfl_dtp(gm_ctx* c, double* dtp) {
   if (rc) return rc;
   hipLaunchKernelGGL(k_init_dtmin, dim3(1), dim3(1), 0, c->stream, c->dscal);
   timed(c, KC_CFL, [&] {
-    gm::launch_cfl(c->stream, c->g, c->ph, c->U, c->p.dx, c->p.dy, c->p.dz, &c->dscal->dtmin, &c->dscal->err);
+    if (c->esize == 8)
+      gm::launch_cfl(c->stream, c->g, c->ph, (const double*)c->U, c->p.dx, c->p.dy, c->p.dz, &c->dscal->dtmin,
+                     &c->dscal->err);
+    else
+      gm::launch_cfl(c->stream, c->g, c->ph, (const float*)c->U, c->p.dx, c->p.dy, c->p.dz, &c->dscal->dtmin,
+                     &c->dscal->err);
   });
   GM_HIP(c, hipGetLastError());
   GM_HIP(c, hipMemcpyAsync(c->hscal, c->dscal, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
@@ -359,29 +413,8 @@ int gm_tstep(gm_ctx* c, double dt) {
   if (!std::isfinite(dt)) return fail(c, GM_ERR_ARG, "dt is not finite");
   int rc = set_device(c);
   if (rc) return rc;
-  const gm::Geom& g = c->g;
-  const gm_params& p = c->p;
-  hipStream_t s = c->stream;
-  int* err = &c->dscal->err;
-  const double dtm = dt / 2.0;  // hydro_solver.f90:152
-  // predictor: first-order fluxes of U^n, U* = U^n - dt/2 div F
-  timed(c, KC_PRIM, [&] { gm::launch_prim(s, g, c->ph, c->U, c->P, 0); });
-  for (int a = 0; a < 3; a++)
-    timed(c, KC_FLUX1, [&] { gm::launch_flux(s, g, c->ph, p.riemann_solver, 1, a, c->P, c->F[a], err); });
-  if (p.enable_flux_cd) timed(c, KC_EFIELD, [&] { gm::launch_efield(s, g, c->F[0], c->F[1], c->F[2], c->E); });
-  timed(c, KC_UPDATE, [&] {
-    gm::launch_update(s, g, c->U, c->Us, c->F[0], c->F[1], c->F[2], c->E, dtm, p.dx, p.dy, p.dz,
-                      p.enable_flux_cd);
-  });
-  // corrector: limited fluxes of U*, U^{n+1} = U^n - dt div F (in place;
-  // viscous_copy with eta = 0 is the identity on values)
-  timed(c, KC_PRIM, [&] { gm::launch_prim(s, g, c->ph, c->Us, c->P, -1); });
-  for (int a = 0; a < 3; a++)
-    timed(c, KC_FLUX2, [&] { gm::launch_flux(s, g, c->ph, p.riemann_solver, 2, a, c->P, c->F[a], err); });
-  if (p.enable_flux_cd) timed(c, KC_EFIELD, [&] { gm::launch_efield(s, g, c->F[0], c->F[1], c->F[2], c->E); });
-  timed(c, KC_UPDATE, [&] {
-    gm::launch_update(s, g, c->U, c->U, c->F[0], c->F[1], c->F[2], c->E, dt, p.dx, p.dy, p.dz, p.enable_flux_cd);
-  });
+  if (c->esize == 8) tstep_t<double>(c, dt);
+  else tstep_t<float>(c, dt);
   c->ghosts_dirty = true;
   GM_HIP(c, hipGetLastError());
   return GM_OK;

@@ -58,9 +58,10 @@ __device__ __forceinline__ int map_ghost(int i, int n, int lo, int hi, bool& fli
   return i;
 }
 
-// Conserved 8-vector of cell (i,j,k), ghosts resolved through the
+// Conserved 8-vector of cell (i,j,k) in FP64, ghosts resolved through the
 // boundary conditions.
-__device__ __forceinline__ void load_cons(const Geom& g, const double* __restrict__ U, int i, int j, int k,
+template <typename S>
+__device__ __forceinline__ void load_cons(const Geom& g, const S* __restrict__ U, int i, int j, int k,
                                           double u[8]) {
   bool fx = false, fy = false, fz = false;
   const int si = map_ghost(i, g.nx, g.bc[0], g.bc[1], fx);
@@ -68,7 +69,7 @@ __device__ __forceinline__ void load_cons(const Geom& g, const double* __restric
   const int sk = map_ghost(k, g.nz, g.bc[4], g.bc[5], fz);
   const long long c = cidx(g, si, sj, sk);
 #pragma unroll
-  for (int q = 0; q < 8; q++) u[q] = U[q * g.ncell + c];
+  for (int q = 0; q < 8; q++) u[q] = (double)U[q * g.ncell + c];
   if (fx) u[1] = -u[1];
   if (fy) u[2] = -u[2];
   if (fz) u[3] = -u[3];
@@ -78,8 +79,9 @@ __device__ __forceinline__ void load_cons(const Geom& g, const double* __restric
 // primitives: calcprim over the cells the flux stencils read
 // (src/hydro_core.f90:245-320), ghost values by boundary mapping.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_prim(Geom g, Phys ph, const double* __restrict__ U,
-                                              double* __restrict__ P, int lo, unsigned bpp) {
+template <typename S>
+__global__ void __launch_bounds__(256) k_prim(Geom g, Phys ph, const S* __restrict__ U, S* __restrict__ P,
+                                              int lo, unsigned bpp) {
   // cells lo..n+1-lo on each axis (lo = 0: one ghost layer, lo = -1: two)
   const int ex = g.nx + 2 - 2 * lo, ey = g.ny + 2 - 2 * lo;
   int t, kk;
@@ -93,7 +95,7 @@ __global__ void __launch_bounds__(256) k_prim(Geom g, Phys ph, const double* __r
   u2prim(ph, u, p);
   const long long c = cidx(g, i, j, k);
 #pragma unroll
-  for (int q = 0; q < 8; q++) P[q * g.ncell + c] = p[q];
+  for (int q = 0; q < 8; q++) P[q * g.ncell + c] = (S)p[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -102,9 +104,9 @@ __global__ void __launch_bounds__(256) k_prim(Geom g, Phys ph, const double* __r
 // lies between cells i and i+1 (i = 0..n along the axis); only faces with
 // interior transverse indices feed the update.
 // ---------------------------------------------------------------------------
-template <int SOLVER, int ORDER, int axis>
-__global__ void __launch_bounds__(256) k_flux(Geom g, Phys ph, const double* __restrict__ P,
-                                              double* __restrict__ F, int* __restrict__ err, unsigned bpp) {
+template <int SOLVER, int ORDER, int axis, typename S>
+__global__ void __launch_bounds__(256) k_flux(Geom g, Phys ph, const S* __restrict__ P, S* __restrict__ F,
+                                              int* __restrict__ err, unsigned bpp) {
   // face grid: along-axis index 0..n, transverse indices interior
   const int e0 = axis == 0 ? g.nx + 1 : g.nx;
   const int e1 = axis == 1 ? g.ny + 1 : g.ny;
@@ -116,14 +118,14 @@ __global__ void __launch_bounds__(256) k_flux(Geom g, Phys ph, const double* __r
   if (axis != 2) k += 1;
   const long long stride = axis == 0 ? 1 : axis == 1 ? (long long)g.NX : (long long)g.NX * g.NY;
   const long long c = cidx(g, i, j, k);
-  double pl[8], pr[8], ff[8];
+  S pl[8], pr[8], ff[8];
 #pragma unroll
   for (int q = 0; q < 8; q++) {
     pl[q] = P[q * g.ncell + c];
     pr[q] = P[q * g.ncell + c + stride];
   }
   if constexpr (ORDER == 2) {
-    double pll[8], prr[8];
+    S pll[8], prr[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) {
       pll[q] = P[q * g.ncell + c - stride];
@@ -143,10 +145,9 @@ __global__ void __launch_bounds__(256) k_flux(Geom g, Phys ph, const double* __r
 // ---------------------------------------------------------------------------
 // cell-centred electric field, get_efield (src/flux_cd_module.f90:245-273)
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_efield(Geom g, const double* __restrict__ Fx,
-                                                const double* __restrict__ Fy,
-                                                const double* __restrict__ Fz, double* __restrict__ E,
-                                                unsigned bpp) {
+template <typename S>
+__global__ void __launch_bounds__(256) k_efield(Geom g, const S* __restrict__ Fx, const S* __restrict__ Fy,
+                                                const S* __restrict__ Fz, S* __restrict__ E, unsigned bpp) {
   int t, k;
   if (!tile_item(bpp, (unsigned)(g.nx * g.ny), t, k)) return;
   const int i = t % g.nx + 1, j = t / g.nx + 1;
@@ -154,9 +155,10 @@ __global__ void __launch_bounds__(256) k_efield(Geom g, const double* __restrict
   const long long sx = 1, sy = g.NX, sz = (long long)g.NX * g.NY;
   const long long N = g.ncell;
   const long long c = cidx(g, i, j, k);
-  E[0 * N + c] = 0.25 * (((-Fy[7 * N + c - sy] - Fy[7 * N + c]) + Fz[6 * N + c - sz]) + Fz[6 * N + c]);
-  E[1 * N + c] = 0.25 * (((+Fx[7 * N + c - sx] + Fx[7 * N + c]) - Fz[5 * N + c - sz]) - Fz[5 * N + c]);
-  E[2 * N + c] = 0.25 * (((-Fx[6 * N + c - sx] - Fx[6 * N + c]) + Fy[5 * N + c - sy]) + Fy[5 * N + c]);
+  auto ld = [](const S* a, long long o) { return (double)a[o]; };
+  E[0 * N + c] = (S)(0.25 * (((-ld(Fy, 7 * N + c - sy) - ld(Fy, 7 * N + c)) + ld(Fz, 6 * N + c - sz)) + ld(Fz, 6 * N + c)));
+  E[1 * N + c] = (S)(0.25 * (((+ld(Fx, 7 * N + c - sx) + ld(Fx, 7 * N + c)) - ld(Fz, 5 * N + c - sz)) - ld(Fz, 5 * N + c)));
+  E[2 * N + c] = (S)(0.25 * (((-ld(Fx, 6 * N + c - sx) - ld(Fx, 6 * N + c)) + ld(Fy, 5 * N + c - sy)) + ld(Fy, 5 * N + c)));
 }
 
 // ---------------------------------------------------------------------------
@@ -182,13 +184,11 @@ __device__ __forceinline__ int e_nbr(int i, int n, int d, int bc) {
   return 0;  // closed / outflow: ghost takes the edge cell's value
 }
 
-template <int FLUXCD>
-__global__ void __launch_bounds__(256) k_update(Geom g, const double* Uin, double* Uout,
-                                                const double* __restrict__ Fx,
-                                                const double* __restrict__ Fy,
-                                                const double* __restrict__ Fz,
-                                                const double* __restrict__ E, double dt, double dx,
-                                                double dy, double dz, unsigned bpp) {
+template <int FLUXCD, typename S>
+__global__ void __launch_bounds__(256) k_update(Geom g, const S* Uin, S* Uout, const S* __restrict__ Fx,
+                                                const S* __restrict__ Fy, const S* __restrict__ Fz,
+                                                const S* __restrict__ E, double dt, double dx, double dy,
+                                                double dz, unsigned bpp) {
   constexpr int NH = FLUXCD ? 5 : 8;
   int t, kc;
   if (!tile_item(bpp, (unsigned)(g.nx * g.ny), t, kc)) return;
@@ -201,7 +201,7 @@ __global__ void __launch_bounds__(256) k_update(Geom g, const double* Uin, doubl
   long long c = cidx(g, i, j, k0);
   double fzm[NH];
 #pragma unroll
-  for (int q = 0; q < NH; q++) fzm[q] = Fz[q * N + c - sz];
+  for (int q = 0; q < NH; q++) fzm[q] = (double)Fz[q * N + c - sz];
   long long oxm = 0, oxp = 0, oym = 0, oyp = 0;
   double e1m = 0, e2m = 0, e1c = 0, e2c = 0;
   if constexpr (FLUXCD) {
@@ -210,10 +210,10 @@ __global__ void __launch_bounds__(256) k_update(Geom g, const double* Uin, doubl
     oym = e_nbr(j, g.ny, -1, g.bc[2]) * sy;
     oyp = e_nbr(j, g.ny, +1, g.bc[3]) * sy;
     const long long ozm = e_nbr(k0, g.nz, -1, g.bc[4]) * sz;
-    e1m = E[0 * N + c + ozm];
-    e2m = E[1 * N + c + ozm];
-    e1c = E[0 * N + c];
-    e2c = E[1 * N + c];
+    e1m = (double)E[0 * N + c + ozm];
+    e2m = (double)E[1 * N + c + ozm];
+    e1c = (double)E[0 * N + c];
+    e2c = (double)E[1 * N + c];
   }
 #pragma unroll 1
   for (int k = k0; k <= k1; k++, c += sz) {
@@ -221,27 +221,30 @@ __global__ void __launch_bounds__(256) k_update(Geom g, const double* Uin, doubl
 #pragma unroll
     for (int q = 0; q < NH; q++) {
       const long long o = q * N + c;
-      const double fz = Fz[o];
-      out[q] = ((Uin[o] - dtdx * (Fx[o] - Fx[o - sx])) - dtdy * (Fy[o] - Fy[o - sy])) - dtdz * (fz - fzm[q]);
+      const double fz = (double)Fz[o];
+      out[q] = (((double)Uin[o] - dtdx * ((double)Fx[o] - (double)Fx[o - sx]))
+                - dtdy * ((double)Fy[o] - (double)Fy[o - sy]))
+               - dtdz * (fz - fzm[q]);
       fzm[q] = fz;
     }
     if constexpr (FLUXCD) {
       const long long ozp = e_nbr(k, g.nz, +1, g.bc[5]) * sz;
-      const double e1p = E[0 * N + c + ozp];
-      const double e2p = E[1 * N + c + ozp];
-      const double* E1 = E;
-      const double* E2 = E + N;
-      const double* E3 = E + 2 * N;
-      out[5] = (Uin[5 * N + c] - (0.5 * dtdy) * (E3[c + oyp] - E3[c + oym])) + (0.5 * dtdz) * (e2p - e2m);
-      out[6] = (Uin[6 * N + c] + (0.5 * dtdx) * (E3[c + oxp] - E3[c + oxm])) - (0.5 * dtdz) * (e1p - e1m);
-      out[7] = (Uin[7 * N + c] - (0.5 * dtdx) * (E2[c + oxp] - E2[c + oxm])) + (0.5 * dtdy) * (E1[c + oyp] - E1[c + oym]);
+      const double e1p = (double)E[0 * N + c + ozp];
+      const double e2p = (double)E[1 * N + c + ozp];
+      const S* E1 = E;
+      const S* E2 = E + N;
+      const S* E3 = E + 2 * N;
+      auto d = [](S v) { return (double)v; };
+      out[5] = (d(Uin[5 * N + c]) - (0.5 * dtdy) * (d(E3[c + oyp]) - d(E3[c + oym]))) + (0.5 * dtdz) * (e2p - e2m);
+      out[6] = (d(Uin[6 * N + c]) + (0.5 * dtdx) * (d(E3[c + oxp]) - d(E3[c + oxm]))) - (0.5 * dtdz) * (e1p - e1m);
+      out[7] = (d(Uin[7 * N + c]) - (0.5 * dtdx) * (d(E2[c + oxp]) - d(E2[c + oxm]))) + (0.5 * dtdy) * (d(E1[c + oyp]) - d(E1[c + oym]));
       e1m = e1c;
       e2m = e2c;
       e1c = e1p;
       e2c = e2p;
     }
 #pragma unroll
-    for (int q = 0; q < 8; q++) Uout[q * N + c] = out[q];
+    for (int q = 0; q < 8; q++) Uout[q * N + c] = (S)out[q];
   }
 }
 
@@ -258,7 +261,8 @@ __device__ __forceinline__ double wave_min(double v) {
   return v;
 }
 
-__global__ void __launch_bounds__(256) k_cfl(Geom g, Phys ph, const double* __restrict__ U, double dx,
+template <typename S>
+__global__ void __launch_bounds__(256) k_cfl(Geom g, Phys ph, const S* __restrict__ U, double dx,
                                              double dy, double dz, unsigned long long* __restrict__ dtmin,
                                              int* __restrict__ err, unsigned bpp) {
   __shared__ double red[4];
@@ -274,7 +278,7 @@ __global__ void __launch_bounds__(256) k_cfl(Geom g, Phys ph, const double* __re
       double u[8], p[8];
 #pragma unroll
       for (int q = 0; q < 8; q++) {
-        u[q] = U[q * g.ncell + c];
+        u[q] = (double)U[q * g.ncell + c];
         bad |= !__builtin_isfinite(u[q]);
       }
       u2prim(ph, u, p);
@@ -313,8 +317,8 @@ __global__ void __launch_bounds__(256) k_cfl(Geom g, Phys ph, const double* __re
 // (src/boundaries.f90:44-247, serial branch): periodic copies over the full
 // transverse range, closed/outflow over 0..n+1.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_ghost1(Geom g, double* __restrict__ U, int axis, int side,
-                                                int kind) {
+template <typename S>
+__global__ void __launch_bounds__(256) k_ghost1(Geom g, S* __restrict__ U, int axis, int side, int kind) {
   const int n0 = axis == 0 ? g.ny : g.nx;  // transverse extents (interior)
   const int n1 = axis == 2 ? g.ny : g.nz;
   const int na = axis == 0 ? g.nx : axis == 1 ? g.ny : g.nz;
@@ -336,7 +340,7 @@ __global__ void __launch_bounds__(256) k_ghost1(Geom g, double* __restrict__ U, 
     const long long cd = cidx(g, di, dj, dk), cs = cidx(g, si, sj, sk);
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-      const double v = U[q * g.ncell + cs];
+      const S v = U[q * g.ncell + cs];
       U[q * g.ncell + cd] = (kind == BC_CLOSED && q == 1 + axis) ? -v : v;
     }
   }
@@ -345,32 +349,36 @@ __global__ void __launch_bounds__(256) k_ghost1(Geom g, double* __restrict__ U, 
 // ---------------------------------------------------------------------------
 // layout conversion between the reference's AoS u(neq,i,j,k) and SoA
 // ---------------------------------------------------------------------------
+template <typename S>
 __global__ void __launch_bounds__(256) k_aos_to_soa(long long ncell, const double* __restrict__ A,
-                                                    double* __restrict__ S) {
+                                                    S* __restrict__ D) {
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < ncell;
        c += (long long)gridDim.x * blockDim.x) {
 #pragma unroll
-    for (int q = 0; q < 8; q++) S[q * ncell + c] = A[c * 8 + q];
+    for (int q = 0; q < 8; q++) D[q * ncell + c] = (S)A[c * 8 + q];
   }
 }
 
-__global__ void __launch_bounds__(256) k_soa_to_aos(long long ncell, int nvar, const double* __restrict__ S,
+template <typename S>
+__global__ void __launch_bounds__(256) k_soa_to_aos(long long ncell, const S* __restrict__ D,
                                                     double* __restrict__ A) {
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < ncell;
        c += (long long)gridDim.x * blockDim.x) {
-    for (int q = 0; q < nvar; q++) A[c * nvar + q] = S[q * ncell + c];
+#pragma unroll
+    for (int q = 0; q < 8; q++) A[c * 8 + q] = (double)D[q * ncell + c];
   }
 }
 
 // primitives and temperature of every cell as stored (calcprim over the
 // whole array, hydro_core.f90:304-316), AoS output
-__global__ void __launch_bounds__(256) k_prim_all(long long ncell, Phys ph, const double* __restrict__ U,
+template <typename S>
+__global__ void __launch_bounds__(256) k_prim_all(long long ncell, Phys ph, const S* __restrict__ U,
                                                   double* __restrict__ Paos, double* __restrict__ T) {
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < ncell;
        c += (long long)gridDim.x * blockDim.x) {
     double u[8], p[8];
 #pragma unroll
-    for (int q = 0; q < 8; q++) u[q] = U[q * ncell + c];
+    for (int q = 0; q < 8; q++) u[q] = (double)U[q * ncell + c];
     u2prim(ph, u, p);
 #pragma unroll
     for (int q = 0; q < 8; q++) Paos[c * 8 + q] = p[q];
@@ -379,7 +387,8 @@ __global__ void __launch_bounds__(256) k_prim_all(long long ncell, Phys ph, cons
 }
 
 // divB of the dump (src/Out_BIN_Module.f90:214-222), divb(nx,ny,nz)
-__global__ void __launch_bounds__(256) k_divb(Geom g, const double* __restrict__ U, double dx, double dy,
+template <typename S>
+__global__ void __launch_bounds__(256) k_divb(Geom g, const S* __restrict__ U, double dx, double dy,
                                               double dz, double* __restrict__ D) {
   const long long n = (long long)g.nx * g.ny * g.nz;
   const long long sx = 1, sy = g.NX, sz = (long long)g.NX * g.NY;
@@ -391,14 +400,15 @@ __global__ void __launch_bounds__(256) k_divb(Geom g, const double* __restrict__
     const int j = (int)(r % g.ny) + 1;
     const int k = (int)(r / g.ny) + 1;
     const long long c = cidx(g, i, j, k);
-    D[t] = ((U[5 * N + c + sx] - U[5 * N + c - sx]) / (2.0 * dx)
-            + (U[6 * N + c + sy] - U[6 * N + c - sy]) / (2.0 * dy))
-           + (U[7 * N + c + sz] - U[7 * N + c - sz]) / (2.0 * dz);
+    auto d = [](S v) { return (double)v; };
+    D[t] = ((d(U[5 * N + c + sx]) - d(U[5 * N + c - sx])) / (2.0 * dx)
+            + (d(U[6 * N + c + sy]) - d(U[6 * N + c - sy])) / (2.0 * dy))
+           + (d(U[7 * N + c + sz]) - d(U[7 * N + c - sz])) / (2.0 * dz);
   }
 }
 
 // ---------------------------------------------------------------------------
-// host-side launchers
+// host-side launchers, instantiated for FP64 and FP32 storage
 // ---------------------------------------------------------------------------
 static inline unsigned grid_for(long long n, int block = 256) {
   long long b = (n + block - 1) / block;
@@ -410,36 +420,37 @@ static inline unsigned grid_for(long long n, int block = 256) {
 
 static inline unsigned tiles(long long items) { return (unsigned)((items + 255) / 256); }
 
-void launch_prim(hipStream_t s, const Geom& g, const Phys& ph, const double* U, double* P, int lo) {
+template <typename S>
+void launch_prim(hipStream_t s, const Geom& g, const Phys& ph, const S* U, S* P, int lo) {
   const int ex = g.nx + 2 - 2 * lo, ey = g.ny + 2 - 2 * lo, ez = g.nz + 2 - 2 * lo;
   const unsigned bpp = tiles((long long)ex * ey);
-  hipLaunchKernelGGL(k_prim, dim3(bpp * ez), dim3(256), 0, s, g, ph, U, P, lo, bpp);
+  hipLaunchKernelGGL(k_prim<S>, dim3(bpp * ez), dim3(256), 0, s, g, ph, U, P, lo, bpp);
 }
 
-template <int SOLVER, int ORDER, int AXIS>
-static void launch_flux_a(hipStream_t s, const Geom& g, const Phys& ph, const double* P, double* F, int* err) {
+template <int SOLVER, int ORDER, int AXIS, typename S>
+static void launch_flux_a(hipStream_t s, const Geom& g, const Phys& ph, const S* P, S* F, int* err) {
   const long long e0 = AXIS == 0 ? g.nx + 1 : g.nx;
   const long long e1 = AXIS == 1 ? g.ny + 1 : g.ny;
   const long long e2 = AXIS == 2 ? g.nz + 1 : g.nz;
   const unsigned bpp = tiles(e0 * e1);
-  hipLaunchKernelGGL((k_flux<SOLVER, ORDER, AXIS>), dim3(bpp * (unsigned)e2), dim3(256), 0, s, g, ph, P, F, err,
-                     bpp);
+  hipLaunchKernelGGL((k_flux<SOLVER, ORDER, AXIS, S>), dim3(bpp * (unsigned)e2), dim3(256), 0, s, g, ph, P, F,
+                     err, bpp);
 }
 
-template <int SOLVER, int ORDER>
-static void launch_flux_t(hipStream_t s, const Geom& g, const Phys& ph, int axis, const double* P, double* F,
-                          int* err) {
+template <int SOLVER, int ORDER, typename S>
+static void launch_flux_t(hipStream_t s, const Geom& g, const Phys& ph, int axis, const S* P, S* F, int* err) {
   if (axis == 0) launch_flux_a<SOLVER, ORDER, 0>(s, g, ph, P, F, err);
   else if (axis == 1) launch_flux_a<SOLVER, ORDER, 1>(s, g, ph, P, F, err);
   else launch_flux_a<SOLVER, ORDER, 2>(s, g, ph, P, F, err);
 }
 
-void launch_flux(hipStream_t s, const Geom& g, const Phys& ph, int solver, int order, int axis,
-                 const double* P, double* F, int* err) {
-#define GM_FLUX_CASE(S)                                                      \
-  case S:                                                                    \
-    if (order == 1) launch_flux_t<S, 1>(s, g, ph, axis, P, F, err);          \
-    else launch_flux_t<S, 2>(s, g, ph, axis, P, F, err);                     \
+template <typename S>
+void launch_flux(hipStream_t s, const Geom& g, const Phys& ph, int solver, int order, int axis, const S* P,
+                 S* F, int* err) {
+#define GM_FLUX_CASE(SV)                                                     \
+  case SV:                                                                   \
+    if (order == 1) launch_flux_t<SV, 1>(s, g, ph, axis, P, F, err);         \
+    else launch_flux_t<SV, 2>(s, g, ph, axis, P, F, err);                    \
     break;
   switch (solver) {
     GM_FLUX_CASE(SOLVER_HLLD)
@@ -450,54 +461,77 @@ void launch_flux(hipStream_t s, const Geom& g, const Phys& ph, int solver, int o
 #undef GM_FLUX_CASE
 }
 
-void launch_efield(hipStream_t s, const Geom& g, const double* Fx, const double* Fy, const double* Fz,
-                   double* E) {
+template <typename S>
+void launch_efield(hipStream_t s, const Geom& g, const S* Fx, const S* Fy, const S* Fz, S* E) {
   const unsigned bpp = tiles((long long)g.nx * g.ny);
-  hipLaunchKernelGGL(k_efield, dim3(bpp * g.nz), dim3(256), 0, s, g, Fx, Fy, Fz, E, bpp);
+  hipLaunchKernelGGL(k_efield<S>, dim3(bpp * g.nz), dim3(256), 0, s, g, Fx, Fy, Fz, E, bpp);
 }
 
-void launch_update(hipStream_t s, const Geom& g, const double* Uin, double* Uout, const double* Fx,
-                   const double* Fy, const double* Fz, const double* E, double dt, double dx, double dy,
-                   double dz, int flux_cd) {
+template <typename S>
+void launch_update(hipStream_t s, const Geom& g, const S* Uin, S* Uout, const S* Fx, const S* Fy, const S* Fz,
+                   const S* E, double dt, double dx, double dy, double dz, int flux_cd) {
   const unsigned bpp = tiles((long long)g.nx * g.ny);
   const unsigned nchunk = (unsigned)((g.nz + KCHUNK - 1) / KCHUNK);
   if (flux_cd)
-    hipLaunchKernelGGL(k_update<1>, dim3(bpp * nchunk), dim3(256), 0, s, g, Uin, Uout, Fx, Fy, Fz, E, dt, dx, dy,
-                       dz, bpp);
+    hipLaunchKernelGGL((k_update<1, S>), dim3(bpp * nchunk), dim3(256), 0, s, g, Uin, Uout, Fx, Fy, Fz, E, dt,
+                       dx, dy, dz, bpp);
   else
-    hipLaunchKernelGGL(k_update<0>, dim3(bpp * nchunk), dim3(256), 0, s, g, Uin, Uout, Fx, Fy, Fz, E, dt, dx, dy,
-                       dz, bpp);
+    hipLaunchKernelGGL((k_update<0, S>), dim3(bpp * nchunk), dim3(256), 0, s, g, Uin, Uout, Fx, Fy, Fz, E, dt,
+                       dx, dy, dz, bpp);
 }
 
-void launch_cfl(hipStream_t s, const Geom& g, const Phys& ph, const double* U, double dx, double dy, double dz,
+template <typename S>
+void launch_cfl(hipStream_t s, const Geom& g, const Phys& ph, const S* U, double dx, double dy, double dz,
                 unsigned long long* dtmin, int* err) {
   const unsigned bpp = tiles((long long)g.nx * g.ny);
   const unsigned nchunk = (unsigned)((g.nz + KCHUNK - 1) / KCHUNK);
-  hipLaunchKernelGGL(k_cfl, dim3(bpp * nchunk), dim3(256), 0, s, g, ph, U, dx, dy, dz, dtmin, err, bpp);
+  hipLaunchKernelGGL(k_cfl<S>, dim3(bpp * nchunk), dim3(256), 0, s, g, ph, U, dx, dy, dz, dtmin, err, bpp);
 }
 
-void launch_ghost1(hipStream_t s, const Geom& g, double* U, int axis, int side, int kind) {
+template <typename S>
+void launch_ghost1(hipStream_t s, const Geom& g, S* U, int axis, int side, int kind) {
   const int n0 = axis == 0 ? g.ny : g.nx;
   const int n1 = axis == 2 ? g.ny : g.nz;
   const long long n = (long long)(n0 + 4) * (n1 + 4);
-  hipLaunchKernelGGL(k_ghost1, dim3(grid_for(n)), dim3(256), 0, s, g, U, axis, side, kind);
+  hipLaunchKernelGGL(k_ghost1<S>, dim3(grid_for(n)), dim3(256), 0, s, g, U, axis, side, kind);
 }
 
-void launch_aos_to_soa(hipStream_t s, long long ncell, const double* A, double* S) {
-  hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(ncell)), dim3(256), 0, s, ncell, A, S);
+template <typename S>
+void launch_aos_to_soa(hipStream_t s, long long ncell, const double* A, S* D) {
+  hipLaunchKernelGGL(k_aos_to_soa<S>, dim3(grid_for(ncell)), dim3(256), 0, s, ncell, A, D);
 }
 
-void launch_soa_to_aos(hipStream_t s, long long ncell, int nvar, const double* S, double* A) {
-  hipLaunchKernelGGL(k_soa_to_aos, dim3(grid_for(ncell)), dim3(256), 0, s, ncell, nvar, S, A);
+template <typename S>
+void launch_soa_to_aos(hipStream_t s, long long ncell, const S* D, double* A) {
+  hipLaunchKernelGGL(k_soa_to_aos<S>, dim3(grid_for(ncell)), dim3(256), 0, s, ncell, D, A);
 }
 
-void launch_prim_all(hipStream_t s, long long ncell, const Phys& ph, const double* U, double* Paos, double* T) {
-  hipLaunchKernelGGL(k_prim_all, dim3(grid_for(ncell)), dim3(256), 0, s, ncell, ph, U, Paos, T);
+template <typename S>
+void launch_prim_all(hipStream_t s, long long ncell, const Phys& ph, const S* U, double* Paos, double* T) {
+  hipLaunchKernelGGL(k_prim_all<S>, dim3(grid_for(ncell)), dim3(256), 0, s, ncell, ph, U, Paos, T);
 }
 
-void launch_divb(hipStream_t s, const Geom& g, const double* U, double dx, double dy, double dz, double* D) {
+template <typename S>
+void launch_divb(hipStream_t s, const Geom& g, const S* U, double dx, double dy, double dz, double* D) {
   const long long n = (long long)g.nx * g.ny * g.nz;
-  hipLaunchKernelGGL(k_divb, dim3(grid_for(n)), dim3(256), 0, s, g, U, dx, dy, dz, D);
+  hipLaunchKernelGGL(k_divb<S>, dim3(grid_for(n)), dim3(256), 0, s, g, U, dx, dy, dz, D);
 }
+
+#define GM_INSTANTIATE(S)                                                                                       \
+  template void launch_prim<S>(hipStream_t, const Geom&, const Phys&, const S*, S*, int);                        \
+  template void launch_flux<S>(hipStream_t, const Geom&, const Phys&, int, int, int, const S*, S*, int*);        \
+  template void launch_efield<S>(hipStream_t, const Geom&, const S*, const S*, const S*, S*);                    \
+  template void launch_update<S>(hipStream_t, const Geom&, const S*, S*, const S*, const S*, const S*, const S*, \
+                                 double, double, double, double, int);                                           \
+  template void launch_cfl<S>(hipStream_t, const Geom&, const Phys&, const S*, double, double, double,           \
+                              unsigned long long*, int*);                                                        \
+  template void launch_ghost1<S>(hipStream_t, const Geom&, S*, int, int, int);                                   \
+  template void launch_aos_to_soa<S>(hipStream_t, long long, const double*, S*);                                 \
+  template void launch_soa_to_aos<S>(hipStream_t, long long, const S*, double*);                                 \
+  template void launch_prim_all<S>(hipStream_t, long long, const Phys&, const S*, double*, double*);             \
+  template void launch_divb<S>(hipStream_t, const Geom&, const S*, double, double, double, double*);
+GM_INSTANTIATE(double)
+GM_INSTANTIATE(float)
+#undef GM_INSTANTIATE
 
 }  // namespace gm

@@ -17,6 +17,24 @@
 
 namespace gm {
 
+// Arithmetic of one precision.  FP64 uses IEEE division and sqrt (the
+// drop-in's bitwise parity depends on it).  FP32 -- the mixed form's
+// reconstruction and fluxes -- uses the hardware reciprocal and square
+// root (v_rcp_f32, v_sqrt_f32, about 1 ulp), since its accuracy is judged
+// against the FP64 path by tolerance.
+template <typename T>
+struct Ar;
+template <>
+struct Ar<double> {
+  static __device__ __forceinline__ double div(double a, double b) { return a / b; }
+  static __device__ __forceinline__ double sqrt(double a) { return __builtin_sqrt(a); }
+};
+template <>
+struct Ar<float> {
+  static __device__ __forceinline__ float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+  static __device__ __forceinline__ float sqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
+};
+
 enum : int { SOLVER_HLL = 1, SOLVER_HLLC = 2, SOLVER_HLLE = 3, SOLVER_HLLD = 4 };
 enum : int { BC_OUTFLOW = 1, BC_CLOSED = 2, BC_PERIODIC = 3, BC_OTHER = 4 };
 enum : int {
@@ -31,26 +49,31 @@ struct Phys {
   int limiter;
 };
 
-__device__ __forceinline__ double dmax(double a, double b) { return (a > b) ? a : b; }
-__device__ __forceinline__ double dmin(double a, double b) { return (a < b) ? a : b; }
-__device__ __forceinline__ double sign1(double x) { return __builtin_copysign(1.0, x); }
+template <typename T> __device__ __forceinline__ T dmax(T a, T b) { return (a > b) ? a : b; }
+template <typename T> __device__ __forceinline__ T dmin(T a, T b) { return (a < b) ? a : b; }
+template <typename T> __device__ __forceinline__ T sign1(T x) { return __builtin_copysign(T(1), x); }
+template <typename T> __device__ __forceinline__ T tabs(T x) { return __builtin_fabs(x); }
+__device__ __forceinline__ float tabs(float x) { return __builtin_fabsf(x); }
+__device__ __forceinline__ float sign1(float x) { return __builtin_copysignf(1.0f, x); }
 
 // y faces exchange slots (1,2) and (5,6); z faces (1,3) and (5,7).  The
 // axis is a compile-time constant so the permutation is pure register
 // renaming (a run-time axis makes the compiler index the array, which
 // moves it to scratch memory).
-template <int AXIS>
-__device__ __forceinline__ void axis_perm(double v[8]) {
+template <int AXIS, typename T>
+__device__ __forceinline__ void axis_perm(T v[8]) {
   if constexpr (AXIS == 1) {
-    double a = v[1]; v[1] = v[2]; v[2] = a;
+    T a = v[1]; v[1] = v[2]; v[2] = a;
     a = v[5]; v[5] = v[6]; v[6] = a;
   } else if constexpr (AXIS == 2) {
-    double a = v[1]; v[1] = v[3]; v[3] = a;
+    T a = v[1]; v[1] = v[3]; v[3] = a;
     a = v[5]; v[5] = v[7]; v[7] = a;
   }
 }
 
-// u2prim, src/hydro_core.f90:46-129 (EOS_ADIABATIC).  Returns prims in p.
+// u2prim, src/hydro_core.f90:46-129 (EOS_ADIABATIC).  Always FP64: this is
+// the pressure recovery E - rho v^2/2 - B^2/2, whose cancellation at low
+// plasma beta the mixed form keeps in double.
 __device__ __forceinline__ void u2prim(const Phys& ph, const double u[8], double p[8]) {
   const double r = dmax(u[0], 1e-15);
   p[0] = r;
@@ -70,13 +93,15 @@ __device__ __forceinline__ void u2prim(const Phys& ph, const double u[8], double
 }
 
 // prim2u, src/hydro_core.f90:331-383
-__device__ __forceinline__ void prim2u(const Phys& ph, const double p[8], double u[8]) {
+template <typename T>
+__device__ __forceinline__ void prim2u(const Phys& ph, const T p[8], T u[8]) {
+  const T cv = (T)ph.cv;
   u[0] = p[0];
   u[1] = p[0] * p[1];
   u[2] = p[0] * p[2];
   u[3] = p[0] * p[3];
-  double e = (0.5 * p[0]) * ((p[1] * p[1] + p[2] * p[2]) + p[3] * p[3]) + ph.cv * p[4];
-  if (ph.mhd) e = e + 0.5 * ((p[5] * p[5] + p[6] * p[6]) + p[7] * p[7]);
+  T e = (T(0.5) * p[0]) * ((p[1] * p[1] + p[2] * p[2]) + p[3] * p[3]) + cv * p[4];
+  if (ph.mhd) e = e + T(0.5) * ((p[5] * p[5] + p[6] * p[6]) + p[7] * p[7]);
   u[4] = e;
   u[5] = p[5];
   u[6] = p[6];
@@ -84,125 +109,134 @@ __device__ __forceinline__ void prim2u(const Phys& ph, const double p[8], double
 }
 
 // prim2f, src/hydro_core.f90:395-476 (x-direction flux)
-__device__ __forceinline__ void prim2f(const Phys& ph, const double p[8], double f[8]) {
-  const double rv = p[0] * p[1];
+template <typename T>
+__device__ __forceinline__ void prim2f(const Phys& ph, const T p[8], T f[8]) {
+  const T cv = (T)ph.cv;
+  const T rv = p[0] * p[1];
   if (ph.mhd) {
-    const double b2 = (p[5] * p[5] + p[6] * p[6]) + p[7] * p[7];
-    const double etot = 0.5 * (((p[0] * ((p[1] * p[1] + p[2] * p[2]) + p[3] * p[3])
-                                 + p[5] * p[5]) + p[6] * p[6]) + p[7] * p[7]) + ph.cv * p[4];
+    const T b2 = (p[5] * p[5] + p[6] * p[6]) + p[7] * p[7];
+    const T etot = T(0.5) * (((p[0] * ((p[1] * p[1] + p[2] * p[2]) + p[3] * p[3])
+                              + p[5] * p[5]) + p[6] * p[6]) + p[7] * p[7]) + cv * p[4];
     f[0] = rv;
-    f[1] = (rv * p[1] + p[4]) + 0.5 * ((p[6] * p[6] + p[7] * p[7]) - p[5] * p[5]);
+    f[1] = (rv * p[1] + p[4]) + T(0.5) * ((p[6] * p[6] + p[7] * p[7]) - p[5] * p[5]);
     f[2] = rv * p[2] - p[5] * p[6];
     f[3] = rv * p[3] - p[5] * p[7];
-    f[4] = p[1] * ((etot + p[4]) + 0.5 * b2)
+    f[4] = p[1] * ((etot + p[4]) + T(0.5) * b2)
            - p[5] * ((p[1] * p[5] + p[2] * p[6]) + p[3] * p[7]);
   } else {
-    const double etot = (0.5 * p[0]) * ((p[1] * p[1] + p[2] * p[2]) + p[3] * p[3]) + ph.cv * p[4];
+    const T etot = (T(0.5) * p[0]) * ((p[1] * p[1] + p[2] * p[2]) + p[3] * p[3]) + cv * p[4];
     f[0] = rv;
     f[1] = rv * p[1] + p[4];
     f[2] = rv * p[2];
     f[3] = rv * p[3];
     f[4] = p[1] * (etot + p[4]);
   }
-  f[5] = 0.0;
+  f[5] = T(0);
   f[6] = p[1] * p[6] - p[5] * p[2];
   f[7] = p[1] * p[7] - p[5] * p[3];
 }
 
 // csound, src/hydro_core.f90:544-553
-__device__ __forceinline__ double csound(const Phys& ph, double pr, double d) {
-  return __builtin_sqrt((ph.gamma * pr) / d);
+template <typename T>
+__device__ __forceinline__ T csound(const Phys& ph, T pr, T d) {
+  return Ar<T>::sqrt(Ar<T>::div((T)ph.gamma * pr, d));
 }
 
-// one component of cfast, src/hydro_core.f90:576-579
+// one component of cfast, src/hydro_core.f90:576-579 (FP64: the CFL reduction)
 __device__ __forceinline__ double cfast1(double g, double pr, double d, double b2, double bn) {
   const double t = g * pr + b2;
   return __builtin_sqrt((0.5 * (t + __builtin_sqrt(t * t - (((4.0 * g) * pr) * bn) * bn))) / d);
 }
 
 // cfastX, src/hydro_core.f90:591-604
-__device__ __forceinline__ double cfastX(const Phys& ph, const double p[8]) {
-  const double g = ph.gamma;
-  const double b2 = (p[5] * p[5] + p[6] * p[6]) + p[7] * p[7];
-  const double cs2va2 = (g * p[4] + b2) / p[0];
-  return __builtin_sqrt(0.5 * (cs2va2 + __builtin_sqrt(cs2va2 * cs2va2
-                                                       - ((((4.0 * g) * p[4]) * (p[5] * p[5])) / p[0]) / p[0])));
+template <typename T>
+__device__ __forceinline__ T cfastX(const Phys& ph, const T p[8]) {
+  using A = Ar<T>;
+  const T g = (T)ph.gamma;
+  const T b2 = (p[5] * p[5] + p[6] * p[6]) + p[7] * p[7];
+  const T cs2va2 = A::div(g * p[4] + b2, p[0]);
+  return A::sqrt(T(0.5) * (cs2va2 + A::sqrt(cs2va2 * cs2va2
+                                             - A::div(A::div(((T(4) * g) * p[4]) * (p[5] * p[5]), p[0]), p[0]))));
 }
 
 // average() of the slope limiters, src/hydro_core.f90:735-796
-__device__ __forceinline__ double lim_average(int lim, double a, double b) {
-  double s, c, d;
+template <typename T>
+__device__ __forceinline__ T lim_average(int lim, T a, T b) {
+  using A = Ar<T>;
+  T s, c, d;
   switch (lim) {
     case LIM_NO_LIMIT:
-      return 0.5 * (a + b);
+      return T(0.5) * (a + b);
     case LIM_MINMOD:
       s = sign1(a);
-      return s * dmax(0.0, dmin(__builtin_fabs(a), s * b));
+      return s * dmax(T(0), dmin(tabs(a), s * b));
     case LIM_VAN_LEER:
-      if (a * b <= 0.0) return 0.0;
-      return ((a * b) * (a + b)) / (a * a + b * b);
+      if (a * b <= T(0)) return T(0);
+      return A::div((a * b) * (a + b), a * a + b * b);
     case LIM_VAN_ALBADA: {
-      const double delta = 1e-7;
-      return (a * (b * b + delta) + b * (a * a + delta)) / ((a * a + b * b) + delta);
+      const T delta = T(1e-7);
+      return A::div(a * (b * b + delta) + b * (a * a + delta), (a * a + b * b) + delta);
     }
     case LIM_UMIST:
       s = sign1(a);
-      c = 0.25 * a + 0.75 * b;
-      d = 0.75 * a + 0.25 * b;
-      return s * dmax(0.0, dmin(dmin(dmin(2.0 * __builtin_fabs(a), (2.0 * s) * b), s * c), s * d));
+      c = T(0.25) * a + T(0.75) * b;
+      d = T(0.75) * a + T(0.25) * b;
+      return s * dmax(T(0), dmin(dmin(dmin(T(2) * tabs(a), (T(2) * s) * b), s * c), s * d));
     case LIM_WOODWARD:
       s = sign1(a);
-      c = 0.5 * (a + b);
-      return s * dmax(0.0, dmin(dmin(2.0 * __builtin_fabs(a), (2.0 * s) * b), s * c));
+      c = T(0.5) * (a + b);
+      return s * dmax(T(0), dmin(dmin(T(2) * tabs(a), (T(2) * s) * b), s * c));
     case LIM_SUPERBEE: {
       s = sign1(b);
-      const double av1 = dmin(2.0 * __builtin_fabs(b), s * a);
-      const double av2 = dmin(__builtin_fabs(b), (2.0 * s) * a);
-      return s * dmax(dmax(0.0, av1), av2);
+      const T av1 = dmin(T(2) * tabs(b), s * a);
+      const T av2 = dmin(tabs(b), (T(2) * s) * a);
+      return s * dmax(dmax(T(0), av1), av2);
     }
     default:  // LIM_NO_AVERAGE
-      return 0.0;
+      return T(0);
   }
 }
 
 // limiter, src/hydro_core.f90:712-731 (in place on pl, pr)
-__device__ __forceinline__ void limiter(int lim, const double pll[8], double pl[8], double pr[8],
-                                        const double prr[8]) {
+template <typename T>
+__device__ __forceinline__ void limiter(int lim, const T pll[8], T pl[8], T pr[8], const T prr[8]) {
 #pragma unroll
   for (int q = 0; q < 8; q++) {
-    const double dl = pl[q] - pll[q];
-    const double dm = pr[q] - pl[q];
-    const double dr = prr[q] - pr[q];
-    const double al = lim_average(lim, dl, dm);
-    const double ar = lim_average(lim, dm, dr);
-    pl[q] = pl[q] + al * 0.5;
-    pr[q] = pr[q] - ar * 0.5;
+    const T dl = pl[q] - pll[q];
+    const T dm = pr[q] - pl[q];
+    const T dr = prr[q] - pr[q];
+    const T al = lim_average(lim, dl, dm);
+    const T ar = lim_average(lim, dm, dr);
+    pl[q] = pl[q] + al * T(0.5);
+    pr[q] = pr[q] - ar * T(0.5);
   }
 }
 
 // Star-state transverse velocity and field of one side, hlld.f90:116-133
 // (L) and :164-181 (R); den == 0 takes the degenerate values.
-__device__ __forceinline__ void hlld_star_side(const double s[8], double bx, double smu, double smsM,
-                                               double sMmu, double& vst, double& wst, double& byst,
-                                               double& bzst) {
-  const double den = (s[0] * smu) * smsM - bx * bx;
-  if (den == 0.0) {
+template <typename T>
+__device__ __forceinline__ void hlld_star_side(const T s[8], T bx, T smu, T smsM, T sMmu, T& vst, T& wst,
+                                               T& byst, T& bzst) {
+  using A = Ar<T>;
+  const T den = (s[0] * smu) * smsM - bx * bx;
+  if (den == T(0)) {
     vst = s[2];
     wst = s[3];
-    byst = 0.0;
-    bzst = 0.0;
+    byst = T(0);
+    bzst = T(0);
   } else {
-    vst = s[2] - ((bx * s[6]) * sMmu) / den;
-    wst = s[3] - ((bx * s[7]) * sMmu) / den;
-    byst = (s[6] * (s[0] * (smu * smu) - bx * bx)) / den;
-    bzst = (s[7] * (s[0] * (smu * smu) - bx * bx)) / den;
+    vst = s[2] - A::div((bx * s[6]) * sMmu, den);
+    wst = s[3] - A::div((bx * s[7]) * sMmu, den);
+    byst = A::div(s[6] * (s[0] * (smu * smu) - bx * bx), den);
+    bzst = A::div(s[7] * (s[0] * (smu * smu) - bx * bx), den);
   }
 }
 
 // Total energy of one side, hlld.f90:113-114 / :161-162
-__device__ __forceinline__ double hlld_energy(double cv, const double s[8], double bx) {
-  return ((0.5 * s[0]) * ((s[1] * s[1] + s[2] * s[2]) + s[3] * s[3]) + cv * s[4])
-         + 0.5 * ((bx * bx + s[6] * s[6]) + s[7] * s[7]);
+template <typename T>
+__device__ __forceinline__ T hlld_energy(T cv, const T s[8], T bx) {
+  return ((T(0.5) * s[0]) * ((s[1] * s[1] + s[2] * s[2]) + s[3] * s[3]) + cv * s[4])
+         + T(0.5) * ((bx * bx + s[6] * s[6]) + s[7] * s[7]);
 }
 
 // prim2fhlld, src/hlld.f90:48-319.  Returns false on the fall-through
@@ -215,64 +249,66 @@ __device__ __forceinline__ double hlld_energy(double cv, const double s[8], doub
 // flux is the same expression, on the same operands, as the reference's
 // branch for that region: UL* :111-156, UR* :159-204, UL** :255-283,
 // UR** :286-314.
-__device__ __forceinline__ bool hlld(const Phys& ph, const double pl[8], const double pr[8], double ff[8]) {
-  const double cv = ph.cv;
-  const double csl = cfastX(ph, pl), csr = cfastX(ph, pr);
-  const double sr = dmax(pl[1] + csl, pr[1] + csr);
-  const double sl = dmin(pl[1] - csl, pr[1] - csr);
-  if (sl > 0) { prim2f(ph, pl, ff); return true; }
-  if (sr < 0) { prim2f(ph, pr, ff); return true; }
+template <typename T>
+__device__ __forceinline__ bool hlld(const Phys& ph, const T pl[8], const T pr[8], T ff[8]) {
+  using A = Ar<T>;
+  const T cv = (T)ph.cv;
+  const T csl = cfastX(ph, pl), csr = cfastX(ph, pr);
+  const T sr = dmax(pl[1] + csl, pr[1] + csr);
+  const T sl = dmin(pl[1] - csl, pr[1] - csr);
+  if (sl > T(0)) { prim2f(ph, pl, ff); return true; }
+  if (sr < T(0)) { prim2f(ph, pr, ff); return true; }
 
-  const double bx = 0.5 * (pl[5] + pr[5]);
-  const double signbx = sign1(bx);
-  const double ptl = pl[4] + 0.5 * ((bx * bx + pl[6] * pl[6]) + pl[7] * pl[7]);
-  const double ptr = pr[4] + 0.5 * ((bx * bx + pr[6] * pr[6]) + pr[7] * pr[7]);
-  const double slmul = sl - pl[1], srmur = sr - pr[1];
-  const double rholul = pl[0] * pl[1], rhorur = pr[0] * pr[1];
-  const double den_m = srmur * pr[0] - slmul * pl[0];
-  const double sM = (((srmur * rhorur - slmul * rholul) - ptr) + ptl) / den_m;
-  const double srmsM = sr - sM, slmsM = sl - sM;
-  const double rhostl = (pl[0] * slmul) / slmsM;
-  const double rhostr = (pr[0] * srmur) / srmsM;
-  const double sqrl = __builtin_sqrt(rhostl), sqrr = __builtin_sqrt(rhostr);
-  const double sstl = sM - __builtin_fabs(bx) / sqrl;
-  const double sstr = sM + __builtin_fabs(bx) / sqrr;
-  const double pst = (((srmur * pr[0]) * ptl - (slmul * pl[0]) * ptr)
-                      + (((pl[0] * pr[0]) * srmur) * slmul) * (pr[1] - pl[1])) / den_m;
+  const T bx = T(0.5) * (pl[5] + pr[5]);
+  const T signbx = sign1(bx);
+  const T ptl = pl[4] + T(0.5) * ((bx * bx + pl[6] * pl[6]) + pl[7] * pl[7]);
+  const T ptr = pr[4] + T(0.5) * ((bx * bx + pr[6] * pr[6]) + pr[7] * pr[7]);
+  const T slmul = sl - pl[1], srmur = sr - pr[1];
+  const T rholul = pl[0] * pl[1], rhorur = pr[0] * pr[1];
+  const T den_m = srmur * pr[0] - slmul * pl[0];
+  const T sM = A::div(((srmur * rhorur - slmul * rholul) - ptr) + ptl, den_m);
+  const T srmsM = sr - sM, slmsM = sl - sM;
+  const T rhostl = A::div(pl[0] * slmul, slmsM);
+  const T rhostr = A::div(pr[0] * srmur, srmsM);
+  const T sqrl = A::sqrt(rhostl), sqrr = A::sqrt(rhostr);
+  const T sstl = sM - A::div(tabs(bx), sqrl);
+  const T sstr = sM + A::div(tabs(bx), sqrr);
+  const T pst = A::div(((srmur * pr[0]) * ptl - (slmul * pl[0]) * ptr)
+                       + (((pl[0] * pr[0]) * srmur) * slmul) * (pr[1] - pl[1]), den_m);
 
-  double vstl, wstl, bystl, bzstl, vstr, wstr, bystr, bzstr;
+  T vstl, wstl, bystl, bzstl, vstr, wstr, bystr, bzstr;
   hlld_star_side(pl, bx, slmul, slmsM, sM - pl[1], vstl, wstl, bystl, bzstl);
   hlld_star_side(pr, bx, srmur, srmsM, sM - pr[1], vstr, wstr, bystr, bzstr);
 
-  const bool single = (sstl >= 0) || (sstr <= 0);
-  if (!single && !(sM >= 0) && !(sM <= 0)) return false;
-  const bool left = single ? (sstl >= 0) : (sM >= 0);
-  double s[8];
+  const bool single = (sstl >= T(0)) || (sstr <= T(0));
+  if (!single && !(sM >= T(0)) && !(sM <= T(0))) return false;
+  const bool left = single ? (sstl >= T(0)) : (sM >= T(0));
+  T s[8];
 #pragma unroll
   for (int q = 0; q < 8; q++) s[q] = left ? pl[q] : pr[q];
-  const double smu = left ? slmul : srmur;
-  const double smsM = left ? slmsM : srmsM;
-  const double rhost = left ? rhostl : rhostr;
-  const double sqr = left ? sqrl : sqrr;
-  const double pt = left ? ptl : ptr;
-  const double vst = left ? vstl : vstr, wst = left ? wstl : wstr;
-  const double byst = left ? bystl : bystr, bzst = left ? bzstl : bzstr;
-  const double e = hlld_energy(cv, s, bx);
-  const double vdotb = (s[1] * bx + s[2] * s[6]) + s[3] * s[7];
-  const double vstdotbst = (sM * bx + vst * byst) + wst * bzst;
-  const double est = (((smu * e - pt * s[1]) + pst * sM) + bx * (vdotb - vstdotbst)) / smsM;
+  const T smu = left ? slmul : srmur;
+  const T smsM = left ? slmsM : srmsM;
+  const T rhost = left ? rhostl : rhostr;
+  const T sqr = left ? sqrl : sqrr;
+  const T pt = left ? ptl : ptr;
+  const T vst = left ? vstl : vstr, wst = left ? wstl : wstr;
+  const T byst = left ? bystl : bystr, bzst = left ? bzstl : bzstr;
+  const T e = hlld_energy(cv, s, bx);
+  const T vdotb = (s[1] * bx + s[2] * s[6]) + s[3] * s[7];
+  const T vstdotbst = (sM * bx + vst * byst) + wst * bzst;
+  const T est = A::div(((smu * e - pt * s[1]) + pst * sM) + bx * (vdotb - vstdotbst), smsM);
 
-  double vt = vst, wt = wst, byt = byst, bzt = bzst, et = est, vdb = vstdotbst;
+  T vt = vst, wt = wst, byt = byst, bzt = bzst, et = est, vdb = vstdotbst;
   if (!single) {
-    const double dd = sqrl + sqrr;
-    const double vstst = ((sqrl * vstl + sqrr * vstr) + (bystr - bystl) * signbx) / dd;
-    const double wstst = ((sqrl * wstl + sqrr * wstr) + (bzstr - bzstl) * signbx) / dd;
-    const double sqlr = __builtin_sqrt(rhostl * rhostr);
-    const double bystst = ((sqrl * bystr + sqrr * bystl) + (sqlr * (vstr - vstl)) * signbx) / dd;
-    const double bzstst = ((sqrl * bzstr + sqrr * bzstl) + (sqlr * (wstr - wstl)) * signbx) / dd;
-    const double vststdotbstst = (sM * bx + vstst * bystst) + wstst * bzstst;
+    const T dd = sqrl + sqrr;
+    const T vstst = A::div((sqrl * vstl + sqrr * vstr) + (bystr - bystl) * signbx, dd);
+    const T wstst = A::div((sqrl * wstl + sqrr * wstr) + (bzstr - bzstl) * signbx, dd);
+    const T sqlr = A::sqrt(rhostl * rhostr);
+    const T bystst = A::div((sqrl * bystr + sqrr * bystl) + (sqlr * (vstr - vstl)) * signbx, dd);
+    const T bzstst = A::div((sqrl * bzstr + sqrr * bzstl) + (sqlr * (wstr - wstl)) * signbx, dd);
+    const T vststdotbstst = (sM * bx + vstst * bystst) + wstst * bzstst;
     // eL** = eL* - sqrt(rhoL*)(...)signBx ; eR** = eR* + sqrt(rhoR*)(...)signBx
-    const double corr = (sqr * (vstdotbst - vststdotbstst)) * signbx;
+    const T corr = (sqr * (vstdotbst - vststdotbstst)) * signbx;
     et = left ? est - corr : est + corr;
     vt = vstst;
     wt = wstst;
@@ -285,73 +321,77 @@ __device__ __forceinline__ bool hlld(const Phys& ph, const double pl[8], const d
   ff[2] = (rhost * sM) * vt - bx * byt;
   ff[3] = (rhost * sM) * wt - bx * bzt;
   ff[4] = sM * (et + pst) - bx * vdb;
-  ff[5] = 0.0;
+  ff[5] = T(0);
   ff[6] = byt * sM - bx * vt;
   ff[7] = bzt * sM - bx * wt;
   return true;
 }
 
 // HLL average of prim2fhlle (src/hlle.f90:74-79) and prim2fhll (src/hll.f90:73-78)
-__device__ __forceinline__ void hll_average(const Phys& ph, const double pl[8], const double pr[8],
-                                            double sl, double sr, double ff[8]) {
-  double fl[8], fr[8], ul[8], ur[8];
+template <typename T>
+__device__ __forceinline__ void hll_average(const Phys& ph, const T pl[8], const T pr[8], T sl, T sr, T ff[8]) {
+  T fl[8], fr[8], ul[8], ur[8];
   prim2f(ph, pl, fl);
   prim2f(ph, pr, fr);
   prim2u(ph, pl, ul);
   prim2u(ph, pr, ur);
 #pragma unroll
-  for (int q = 0; q < 8; q++) ff[q] = ((sr * fl[q] - sl * fr[q]) + (sl * sr) * (ur[q] - ul[q])) / (sr - sl);
+  for (int q = 0; q < 8; q++) ff[q] = Ar<T>::div((sr * fl[q] - sl * fr[q]) + (sl * sr) * (ur[q] - ul[q]), sr - sl);
 }
 
 // prim2fhlle, src/hlle.f90:48-83
-__device__ __forceinline__ bool hlle(const Phys& ph, const double pl[8], const double pr[8], double ff[8]) {
-  const double csl = cfastX(ph, pl), csr = cfastX(ph, pr);
-  const double sr = dmax(pl[1] + csl, pr[1] + csr);
-  const double sl = dmin(pl[1] - csl, pr[1] - csr);
-  if (sl > 0) { prim2f(ph, pl, ff); return true; }
-  if (sr < 0) { prim2f(ph, pr, ff); return true; }
+template <typename T>
+__device__ __forceinline__ bool hlle(const Phys& ph, const T pl[8], const T pr[8], T ff[8]) {
+  const T csl = cfastX(ph, pl), csr = cfastX(ph, pr);
+  const T sr = dmax(pl[1] + csl, pr[1] + csr);
+  const T sl = dmin(pl[1] - csl, pr[1] - csr);
+  if (sl > T(0)) { prim2f(ph, pl, ff); return true; }
+  if (sr < T(0)) { prim2f(ph, pr, ff); return true; }
   hll_average(ph, pl, pr, sl, sr, ff);
   return true;
 }
 
 // prim2fhll, src/hll.f90:47-82
-__device__ __forceinline__ bool hll(const Phys& ph, const double pl[8], const double pr[8], double ff[8]) {
-  const double csl = csound(ph, pl[4], pl[0]), csr = csound(ph, pr[4], pr[0]);
-  const double sr = dmax(pl[1] + csl, pr[1] + csr);
-  const double sl = dmin(pl[1] - csl, pr[1] - csr);
-  if (sl > 0) { prim2f(ph, pl, ff); return true; }
-  if (sr < 0) { prim2f(ph, pr, ff); return true; }
+template <typename T>
+__device__ __forceinline__ bool hll(const Phys& ph, const T pl[8], const T pr[8], T ff[8]) {
+  const T csl = csound(ph, pl[4], pl[0]), csr = csound(ph, pr[4], pr[0]);
+  const T sr = dmax(pl[1] + csl, pr[1] + csr);
+  const T sl = dmin(pl[1] - csl, pr[1] - csr);
+  if (sl > T(0)) { prim2f(ph, pl, ff); return true; }
+  if (sr < T(0)) { prim2f(ph, pr, ff); return true; }
   hll_average(ph, pl, pr, sl, sr, ff);
   return true;
 }
 
 // prim2fhllc, src/hllc.f90:44-140 (passive B)
-__device__ __forceinline__ bool hllc(const Phys& ph, const double pl[8], const double pr[8], double ff[8]) {
-  const double csl = csound(ph, pl[4], pl[0]), csr = csound(ph, pr[4], pr[0]);
-  const double sr = dmax(pl[1] + csl, pr[1] + csr);
-  const double sl = dmin(pl[1] - csl, pr[1] - csr);
-  if (sl > 0) { prim2f(ph, pl, ff); return true; }
-  if (sr < 0) { prim2f(ph, pr, ff); return true; }
-  const double slmul = sl - pl[1], srmur = sr - pr[1];
-  const double rholul = pl[0] * pl[1], rhorur = pr[0] * pr[1];
-  const double sst = (((srmur * rhorur - slmul * rholul) - pr[4]) + pl[4]) / (srmur * pr[0] - slmul * pl[0]);
-  if (!(sst >= 0.0) && !(sst <= 0.0)) return false;
-  const bool left = (sst >= 0.0);
-  double s[8];
+template <typename T>
+__device__ __forceinline__ bool hllc(const Phys& ph, const T pl[8], const T pr[8], T ff[8]) {
+  using A = Ar<T>;
+  const T csl = csound(ph, pl[4], pl[0]), csr = csound(ph, pr[4], pr[0]);
+  const T sr = dmax(pl[1] + csl, pr[1] + csr);
+  const T sl = dmin(pl[1] - csl, pr[1] - csr);
+  if (sl > T(0)) { prim2f(ph, pl, ff); return true; }
+  if (sr < T(0)) { prim2f(ph, pr, ff); return true; }
+  const T slmul = sl - pl[1], srmur = sr - pr[1];
+  const T rholul = pl[0] * pl[1], rhorur = pr[0] * pr[1];
+  const T sst = A::div(((srmur * rhorur - slmul * rholul) - pr[4]) + pl[4], srmur * pr[0] - slmul * pl[0]);
+  if (!(sst >= T(0)) && !(sst <= T(0))) return false;
+  const bool left = (sst >= T(0));
+  T s[8];
 #pragma unroll
   for (int q = 0; q < 8; q++) s[q] = left ? pl[q] : pr[q];
-  const double smu = left ? slmul : srmur;
-  const double sw = left ? sl : sr;
-  const double rhost = (s[0] * smu) / (sw - sst);
-  const double ek = (0.5 * s[0]) * ((s[1] * s[1] + s[2] * s[2]) + s[3] * s[3]) + ph.cv * s[4];
-  double uuk[8], uu[8];
+  const T smu = left ? slmul : srmur;
+  const T sw = left ? sl : sr;
+  const T rhost = A::div(s[0] * smu, sw - sst);
+  const T ek = (T(0.5) * s[0]) * ((s[1] * s[1] + s[2] * s[2]) + s[3] * s[3]) + (T)ph.cv * s[4];
+  T uuk[8], uu[8];
   uuk[0] = rhost;
   uuk[1] = rhost * sst;
   uuk[2] = rhost * s[2];
   uuk[3] = rhost * s[3];
-  uuk[4] = rhost * (ek / s[0] + (sst - s[1]) * (sst + s[4] / (s[0] * smu)));
+  uuk[4] = rhost * (A::div(ek, s[0]) + (sst - s[1]) * (sst + A::div(s[4], s[0] * smu)));
 #pragma unroll
-  for (int q = 5; q < 8; q++) uuk[q] = ph.pmhd ? (rhost * s[q]) / s[0] : 0.0;
+  for (int q = 5; q < 8; q++) uuk[q] = ph.pmhd ? A::div(rhost * s[q], s[0]) : T(0);
   prim2f(ph, s, ff);
   prim2u(ph, s, uu);
 #pragma unroll
@@ -359,8 +399,8 @@ __device__ __forceinline__ bool hllc(const Phys& ph, const double pl[8], const d
   return true;
 }
 
-template <int SOLVER>
-__device__ __forceinline__ bool riemann(const Phys& ph, const double pl[8], const double pr[8], double ff[8]) {
+template <int SOLVER, typename T>
+__device__ __forceinline__ bool riemann(const Phys& ph, const T pl[8], const T pr[8], T ff[8]) {
   if constexpr (SOLVER == SOLVER_HLLD) return hlld(ph, pl, pr, ff);
   else if constexpr (SOLVER == SOLVER_HLLE) return hlle(ph, pl, pr, ff);
   else if constexpr (SOLVER == SOLVER_HLL) return hll(ph, pl, pr, ff);

@@ -57,52 +57,93 @@ def orszag_tang(nx: int, ny: int, nz: int, cv: float = 1.5) -> Tuple[np.ndarray,
 
 
 def _random_field(rng, n: int, kmax: int = 8, slope: float = -2.0) -> np.ndarray:
-    """Real periodic field on n^3 cells: random-phase Gaussian modes with
-    integer wavevectors 1 <= |k| <= kmax and amplitude |k|^slope."""
-    k = np.fft.fftfreq(n, d=1.0 / n)
-    kx, ky, kz = np.meshgrid(k, k, k, indexing="ij")
-    kk = np.sqrt(kx * kx + ky * ky + kz * kz)
-    amp = np.zeros_like(kk)
-    m = (kk >= 1.0) & (kk <= kmax)
-    amp[m] = kk[m] ** slope
-    coef = (rng.standard_normal(kk.shape) + 1j * rng.standard_normal(kk.shape)) * amp
-    return np.real(np.fft.ifftn(coef))
+    """Real periodic field on n^3 cells, indexed [x, y, z] in Fortran order:
+    the real part of a sum of random-phase Gaussian modes exp(2 pi i k.x)
+    over integer wavevectors 1 <= |k| <= kmax with amplitude |k|^slope, x
+    on the cell grid j/n.
+
+    Only (2 kmax + 1)^3 modes are non-zero, so the sum is evaluated as three
+    separable partial transforms (small matrix products) instead of a full
+    n^3 FFT."""
+    k = np.arange(-kmax, kmax + 1)
+    kk = np.sqrt(k[:, None, None] ** 2 + k[None, :, None] ** 2 + k[None, None, :] ** 2)
+    amp = np.where((kk >= 1.0) & (kk <= kmax), np.maximum(kk, 1.0) ** slope, 0.0)
+    coef = (rng.standard_normal(kk.shape) + 1j * rng.standard_normal(kk.shape)) * amp   # [kx, ky, kz]
+    ph = np.exp(2j * np.pi * np.outer(k, np.arange(n) / n))            # (K, n)
+    a = np.tensordot(coef, ph, axes=([2], [0]))                         # [kx, ky, z]
+    b = np.einsum("abz,by->azy", a, ph)                                 # [kx, z, y]
+    b = b.reshape(len(k), n * n)
+    br = np.ascontiguousarray(b.real)
+    bi = np.ascontiguousarray(b.imag)
+    f = np.ascontiguousarray(br.T) @ np.ascontiguousarray(ph.real)      # [(z, y), x]
+    f -= np.ascontiguousarray(bi.T) @ np.ascontiguousarray(ph.imag)
+    return f.reshape(n, n, n).transpose(2, 1, 0)                        # [x, y, z], Fortran order
 
 
-def _fill_periodic_ghosts(interior: np.ndarray) -> np.ndarray:
-    """(8, n, n, n) interior -> (8, n+4, n+4, n+4) with periodic ghosts."""
-    return np.asfortranarray(np.pad(interior, ((0, 0), (2, 2), (2, 2), (2, 2)), mode="wrap"))
+def _periodic_state(fields) -> np.ndarray:
+    """Eight [x, y, z] interior fields -> u(8, n+4, n+4, n+4) (Fortran order)
+    with two periodic ghost layers on every face."""
+    n = fields[0].shape[0]
+    u = np.empty((8, n + 4, n + 4, n + 4), dtype=np.float64, order="F")
+    for q, f in enumerate(fields):
+        v = u[q]
+        v[2:-2, 2:-2, 2:-2] = f
+        v[:2, 2:-2, 2:-2] = v[n:n + 2, 2:-2, 2:-2]
+        v[-2:, 2:-2, 2:-2] = v[2:4, 2:-2, 2:-2]
+        v[:, :2, 2:-2] = v[:, n:n + 2, 2:-2]
+        v[:, -2:, 2:-2] = v[:, 2:4, 2:-2]
+        v[:, :, :2] = v[:, :, n:n + 2]
+        v[:, :, -2:] = v[:, :, 2:4]
+    return u
+
+
+def _unit_gaussian(f: np.ndarray) -> np.ndarray:
+    f = f - f.mean()
+    return f / math.sqrt(np.mean(f * f))
 
 
 def turbulence(n: int, seed: int = 2024, cv: float = 1.5, kmax: int = 8,
-               beta: float = 1.0) -> Tuple[np.ndarray, float, float, float]:
+               beta: float = 1.0, white_noise: bool = False) -> Tuple[np.ndarray, float, float, float]:
     """Seeded MHD turbulence box on [0,1]^3, periodic, gamma = 5/3.
 
-    rho = exp(0.5 g - 0.125), p = exp(0.3 g' - 0.045) with g, g' ~ N(0,1)
-    per cell (means 1); v: random-phase modes 1 <= |k| <= 8 with amplitude
-    |k|^-2, scaled to rms |v| = sqrt(gamma) (rms Mach 1 at <c_s> = sqrt(gamma));
-    B = curl A with cell-centred central differences of a vector potential
-    of the same spectrum (central-difference div B = 0 to round-off),
-    scaled so <B^2> = 2<p>/beta.
+    rho = exp(0.5 g - 0.125), p = exp(0.3 g' - 0.045) (lognormal, sigma 0.5
+    and 0.3, means 1), where g, g' are unit-variance Gaussian random fields
+    on the band 1 <= |k| <= kmax (or, with white_noise=True, independent
+    N(0,1) per cell -- grid-scale contrasts that drive the scheme to its
+    pressure floor at 512^3); v: random-phase modes 1 <= |k| <= 8 with
+    amplitude |k|^-2, scaled to rms |v| = sqrt(gamma) (rms Mach 1 at
+    <c_s> = sqrt(gamma)); B = curl A with cell-centred central differences
+    of a vector potential of the same spectrum (central-difference div B = 0
+    to round-off), scaled so <B^2> = 2<p>/beta.
     """
     rng = np.random.default_rng(seed)
     gamma = (cv + 1.0) / cv
-    g1 = rng.standard_normal((n, n, n))
-    g2 = rng.standard_normal((n, n, n))
+    if white_noise:
+        g1 = rng.standard_normal((n, n, n)).transpose(2, 1, 0)
+        g2 = rng.standard_normal((n, n, n)).transpose(2, 1, 0)
+    else:
+        g1 = _unit_gaussian(_random_field(rng, n, kmax, slope=-1.0))
+        g2 = _unit_gaussian(_random_field(rng, n, kmax, slope=-1.0))
     rho = np.exp(0.5 * g1 - 0.125)
     p = np.exp(0.3 * g2 - 0.045)
-    v = np.stack([_random_field(rng, n, kmax) for _ in range(3)])
-    v *= math.sqrt(gamma) / math.sqrt(np.mean(np.sum(v * v, axis=0)))
-    A = np.stack([_random_field(rng, n, kmax) for _ in range(3)])
+    del g1, g2
+    v = [_random_field(rng, n, kmax) for _ in range(3)]
+    vs = math.sqrt(gamma) / math.sqrt(np.mean(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]))
+    v = [c * vs for c in v]
+    A = [_random_field(rng, n, kmax) for _ in range(3)]
     h = 1.0 / n
 
     def d(f, ax):  # central difference, periodic
         return (np.roll(f, -1, axis=ax) - np.roll(f, 1, axis=ax)) / (2.0 * h)
 
-    B = np.stack([d(A[2], 1) - d(A[1], 2), d(A[0], 2) - d(A[2], 0), d(A[1], 0) - d(A[0], 1)])
-    B *= math.sqrt(2.0 * np.mean(p) / beta / np.mean(np.sum(B * B, axis=0)))
-    u = prim_to_cons(rho, v[0], v[1], v[2], p, B[0], B[1], B[2], cv)
-    return _fill_periodic_ghosts(u), h, h, h
+    B = [d(A[2], 1) - d(A[1], 2), d(A[0], 2) - d(A[2], 0), d(A[1], 0) - d(A[0], 1)]
+    del A
+    bs = math.sqrt(2.0 * np.mean(p) / beta / np.mean(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]))
+    B = [c * bs for c in B]
+    e = ((0.5 * rho) * ((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]) + cv * p) \
+        + 0.5 * ((B[0] * B[0] + B[1] * B[1]) + B[2] * B[2])
+    u = _periodic_state([rho, rho * v[0], rho * v[1], rho * v[2], e, B[0], B[1], B[2]])
+    return u, h, h, h
 
 
 def blast(n: int, seed: int = 7, cv: float = 1.5) -> Tuple[np.ndarray, float, float, float]:
