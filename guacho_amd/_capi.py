@@ -14,7 +14,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libguacho_amd.so")
 
-GM_OK, GM_ERR_CONFIG, GM_ERR_HIP, GM_ERR_NONFINITE, GM_ERR_OOM, GM_ERR_ARG = 0, 1, 2, 3, 4, 5
+GM_OK, GM_ERR_CONFIG, GM_ERR_HIP, GM_ERR_NONFINITE, GM_ERR_OOM, GM_ERR_ARG, GM_ERR_BUSY = 0, 1, 2, 3, 4, 5, 6
 GM_PRECISION_FP64, GM_PRECISION_MIXED = 0, 1
 GM_NKCLASS = 8
 
@@ -23,6 +23,9 @@ EXPORTS = (
     "gm_create", "gm_destroy", "gm_last_error", "gm_upload_u", "gm_download_u",
     "gm_download_primit", "gm_boundary_u", "gm_cfl_dtp", "gm_tstep", "gm_sync",
     "gm_download_divb", "gm_set_timing", "gm_kernel_times", "gm_device_bytes",
+    "gm_batch_create", "gm_batch_destroy", "gm_batch_last_error", "gm_batch_box_bytes",
+    "gm_batch_free_bytes", "gm_batch_add", "gm_batch_remove", "gm_batch_advance", "gm_batch_sync",
+    "gm_batch_clock", "gm_batch_download_u", "gm_batch_tuning", "gm_batch_active",
 )
 
 
@@ -44,6 +47,14 @@ class GmParams(C.Structure):
         ("precision", C.c_int32), ("device", C.c_int32), ("reserved", C.c_int32),
         ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
         ("cv", C.c_double), ("gamma", C.c_double), ("eta", C.c_double),
+    ]
+
+
+class GmBatchParams(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int32), ("riemann_solver", C.c_int32), ("enable_flux_cd", C.c_int32),
+        ("device", C.c_int32), ("max_boxes", C.c_int32), ("steps_per_launch", C.c_int32),
+        ("autotune", C.c_int32), ("reserved", C.c_int32), ("memory_budget", C.c_int64),
     ]
 
 
@@ -76,8 +87,24 @@ def lib() -> C.CDLL:
     L.gm_set_timing.argtypes = [vp, C.c_int]
     L.gm_kernel_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_char_p)]
     L.gm_device_bytes.argtypes = [vp, C.POINTER(C.c_int64)]
+    P = C.POINTER(GmParams)
+    L.gm_batch_create.argtypes = [C.POINTER(GmBatchParams), C.POINTER(vp)]
+    L.gm_batch_destroy.argtypes = [vp]
+    L.gm_batch_last_error.argtypes = [vp]
+    L.gm_batch_last_error.restype = C.c_char_p
+    L.gm_batch_box_bytes.argtypes = [vp, P, C.POINTER(C.c_int64)]
+    L.gm_batch_free_bytes.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.gm_batch_add.argtypes = [vp, P, dp, C.c_double, C.c_double, C.c_int64, C.POINTER(C.c_int32)]
+    L.gm_batch_remove.argtypes = [vp, C.c_int32]
+    L.gm_batch_advance.argtypes = [vp, C.c_int32]
+    L.gm_batch_sync.argtypes = [vp]
+    L.gm_batch_clock.argtypes = [vp, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                                 C.POINTER(C.c_int32)]
+    L.gm_batch_download_u.argtypes = [vp, C.c_int32, dp]
+    L.gm_batch_tuning.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+    L.gm_batch_active.argtypes = [vp, C.POINTER(C.c_int32)]
     for name in EXPORTS:
-        if name != "gm_last_error":
+        if name not in ("gm_last_error", "gm_batch_last_error"):
             getattr(L, name).restype = C.c_int
     _LIB = L
     return L
@@ -188,4 +215,96 @@ class Subdomain:
     def device_bytes(self) -> int:
         v = C.c_int64()
         self._check(self.L.gm_device_bytes(self.h, C.byref(v)))
+        return v.value
+
+
+class Batch:
+    """A batch of independent boxes advanced together (gm_batch_*)."""
+
+    def __init__(self, precision: int = GM_PRECISION_MIXED, riemann_solver: int = 4, enable_flux_cd: bool = True,
+                 max_boxes: int = 256, steps_per_launch: int = 1, autotune: bool = False,
+                 memory_budget: int = 0, device: int = 0):
+        self.L = lib()
+        bp = GmBatchParams()
+        bp.precision, bp.riemann_solver, bp.enable_flux_cd = precision, riemann_solver, int(enable_flux_cd)
+        bp.device, bp.max_boxes, bp.steps_per_launch = device, max_boxes, steps_per_launch
+        bp.autotune, bp.reserved, bp.memory_budget = int(autotune), 0, memory_budget
+        self.bp = bp
+        h = C.c_void_p()
+        rc = self.L.gm_batch_create(C.byref(bp), C.byref(h))
+        if rc != GM_OK:
+            raise GmError(rc, self.L.gm_batch_last_error(None).decode())
+        self.h = h
+        self.shapes = {}
+
+    def _check(self, rc: int):
+        if rc != GM_OK:
+            raise GmError(rc, self.L.gm_batch_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.gm_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def box_bytes(self, params: GmParams) -> int:
+        v = C.c_int64()
+        self._check(self.L.gm_batch_box_bytes(self.h, C.byref(params), C.byref(v)))
+        return v.value
+
+    def free_bytes(self) -> int:
+        v = C.c_int64()
+        self._check(self.L.gm_batch_free_bytes(self.h, C.byref(v)))
+        return v.value
+
+    def add(self, params: GmParams, u: np.ndarray, cfl: float = 0.4, t_end: float = float("inf"),
+            max_steps: int = 0) -> int:
+        u = np.asfortranarray(u, dtype=np.float64)
+        shape = (8, params.nx + 4, params.ny + 4, params.nz + 4)
+        if u.shape != shape:
+            raise ValueError(f"u has shape {u.shape}, expected {shape}")
+        i = C.c_int32()
+        self._check(self.L.gm_batch_add(self.h, C.byref(params), u, cfl, t_end, max_steps, C.byref(i)))
+        self.shapes[i.value] = shape
+        return i.value
+
+    def remove(self, box: int):
+        self._check(self.L.gm_batch_remove(self.h, box))
+        self.shapes.pop(box, None)
+
+    def advance(self, nlaunch: int = 1):
+        self._check(self.L.gm_batch_advance(self.h, nlaunch))
+
+    def sync(self):
+        self._check(self.L.gm_batch_sync(self.h))
+
+    def clock(self, box: int):
+        t, n, d = C.c_double(), C.c_int64(), C.c_int32()
+        self._check(self.L.gm_batch_clock(self.h, box, C.byref(t), C.byref(n), C.byref(d)))
+        return t.value, n.value, bool(d.value)
+
+    def download_u(self, box: int) -> np.ndarray:
+        u = np.zeros(self.shapes[box], dtype=np.float64, order="F")
+        self._check(self.L.gm_batch_download_u(self.h, box, u))
+        return u
+
+    def tuning(self):
+        b, k, ms = C.c_int32(), C.c_int32(), C.c_double()
+        self._check(self.L.gm_batch_tuning(self.h, C.byref(b), C.byref(k), C.byref(ms)))
+        return b.value, k.value, ms.value
+
+    def active(self) -> int:
+        v = C.c_int32()
+        self._check(self.L.gm_batch_active(self.h, C.byref(v)))
         return v.value

@@ -1,0 +1,542 @@
+// gm_batch.hip -- batches of independent boxes behind the gm_batch_* C ABI.
+//
+// Each box owns its device buffers (admitted against a memory budget).  A
+// step of the batch is the same kernel sequence as one box's tstep, but
+// every launch covers the tiles of all active boxes: a device table of box
+// views plus per-kernel-class prefix sums of their tile counts lets each
+// workgroup find its box.  Per-box clocks on the device hold time,
+// currentIteration and the CFL minimum, so the ramp and dump clamp of
+// get_timestep (src/hydro_core.f90:677-694) run there and a step needs no
+// host round trip; steps_per_launch steps are captured once in a HIP graph
+// (kernel nodes only) and replayed.  The tables and the graph are rebuilt
+// when boxes join or leave.
+//
+// The autotuner times each candidate launch shape (tile width x z-pencil
+// length) on the admitted boxes with dt = 0 steps: every flux is computed
+// in full while the state keeps its values.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gm_kernels.hpp"
+#include "guacho_amd.h"
+
+namespace {
+
+struct Slot {
+  bool used = false;
+  gm_params p{};
+  void* U = nullptr;
+  void* Us = nullptr;
+  void* P = nullptr;
+  void* Fb = nullptr;
+  void* E = nullptr;
+  int64_t bytes = 0;
+  gm::BoxClock clock{};  // host copy as of the last sync
+};
+
+constexpr int kMaxBoxes = 4096;
+
+}  // namespace
+
+struct gm_batch {
+  gm_batch_params bp{};
+  size_t esize = 8;
+  hipStream_t stream = nullptr;
+  std::vector<Slot> slots;
+  int64_t budget = 0;
+  int64_t used = 0;
+  void* dboxes = nullptr;          // Box<S>[max_boxes], active boxes packed
+  unsigned* dprefix = nullptr;     // KC_COUNT x (max_boxes + 1)
+  gm::BoxClock* dclk = nullptr;    // one clock per slot
+  gm::BoxClock* hclk = nullptr;    // pinned mirror
+  int* derr = nullptr;
+  int* herr = nullptr;
+  std::vector<int> order;          // slots of the packed table
+  unsigned tiles[gm::KC_COUNT] = {0};
+  bool tables_dirty = true;
+  bool graph_dirty = true;
+  gm::Shape shape = {256, 16};
+  bool tuned = false;
+  double tuned_ms = 0.0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::string msg;
+};
+
+namespace {
+
+thread_local std::string g_batch_create_error;
+
+int bfail(gm_batch* b, int code, const std::string& m) {
+  if (b) b->msg = m;
+  else g_batch_create_error = m;
+  return code;
+}
+
+int bhip(gm_batch* b, hipError_t e, const char* what) {
+  if (e == hipSuccess) return GM_OK;
+  return bfail(b, GM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define BHIP(b, call)                                 \
+  do {                                                \
+    int _rc = bhip((b), (call), #call);               \
+    if (_rc != GM_OK) return _rc;                     \
+  } while (0)
+
+int64_t box_bytes_of(const gm_params& p, size_t esize) {
+  const int64_t ncell = (int64_t)(p.nx + 4) * (p.ny + 4) * (p.nz + 4);
+  return ncell * (8 + 8 + 8 + 24 + 3) * (int64_t)esize;
+}
+
+gm::Geom geom_of(const gm_params& p) {
+  gm::Geom g;
+  g.nx = p.nx; g.ny = p.ny; g.nz = p.nz;
+  g.NX = p.nx + 4; g.NY = p.ny + 4; g.NZ = p.nz + 4;
+  g.ncell = (long long)g.NX * g.NY * g.NZ;
+  for (int k = 0; k < 6; k++) g.bc[k] = p.bc[k];
+  return g;
+}
+
+template <typename S>
+gm::Box<S> box_view(const Slot& s, int slot) {
+  gm::Box<S> b;
+  b.g = geom_of(s.p);
+  b.ph.cv = s.p.cv;
+  b.ph.gamma = s.p.gamma;
+  b.ph.mhd = s.p.mhd ? 1 : 0;
+  b.ph.pmhd = s.p.pmhd ? 1 : 0;
+  b.ph.limiter = s.p.slope_limiter;
+  b.U = (S*)s.U;
+  b.Us = (S*)s.Us;
+  b.P = (S*)s.P;
+  b.F = (S*)s.Fb;
+  b.E = (S*)s.E;
+  b.dx = s.p.dx;
+  b.dy = s.p.dy;
+  b.dz = s.p.dz;
+  b.slot = slot;
+  return b;
+}
+
+void free_slot(Slot& s) {
+  void* bufs[] = {s.U, s.Us, s.P, s.Fb, s.E};
+  for (void* p : bufs)
+    if (p) (void)hipFree(p);
+  s = Slot();
+}
+
+void drop_graph(gm_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->graph_dirty = true;
+}
+
+// Pack the active boxes and their per-class tile prefix tables for `sh`.
+template <typename S>
+int rebuild_tables_t(gm_batch* b, const gm::Shape& sh) {
+  const int cap = b->bp.max_boxes;
+  b->order.clear();
+  for (int i = 0; i < cap; i++)
+    if (b->slots[i].used) b->order.push_back(i);
+  const int n = (int)b->order.size();
+  std::vector<gm::Box<S>> boxes(n);
+  std::vector<unsigned> prefix((size_t)gm::KC_COUNT * (cap + 1), 0u);
+  for (int k = 0; k < gm::KC_COUNT; k++) {
+    unsigned acc = 0;
+    for (int i = 0; i < n; i++) {
+      prefix[(size_t)k * (cap + 1) + i] = acc;
+      const gm::Geom g = geom_of(b->slots[b->order[i]].p);
+      const unsigned t = gm::box_tiles(g, k, sh);
+      if ((unsigned long long)acc + t > 0xFFFFFFFFull / 2) return bfail(b, GM_ERR_BUSY, "batch exceeds the tile index range");
+      acc += t;
+    }
+    prefix[(size_t)k * (cap + 1) + n] = acc;
+    b->tiles[k] = acc;
+  }
+  for (int i = 0; i < n; i++) boxes[i] = box_view<S>(b->slots[b->order[i]], b->order[i]);
+  BHIP(b, hipStreamSynchronize(b->stream));
+  if (n) BHIP(b, hipMemcpy(b->dboxes, boxes.data(), sizeof(gm::Box<S>) * n, hipMemcpyHostToDevice));
+  BHIP(b, hipMemcpy(b->dprefix, prefix.data(), sizeof(unsigned) * prefix.size(), hipMemcpyHostToDevice));
+  b->tables_dirty = false;
+  return GM_OK;
+}
+
+int rebuild_tables(gm_batch* b, const gm::Shape& sh) {
+  return b->esize == 8 ? rebuild_tables_t<double>(b, sh) : rebuild_tables_t<float>(b, sh);
+}
+
+// Queue `nsteps` batched steps (tuning: dt = 0, clocks untouched).
+template <typename S>
+void enqueue_steps_t(gm_batch* b, int nsteps, int tuning) {
+  const int n = (int)b->order.size();
+  if (n == 0) return;
+  const int cap = b->bp.max_boxes;
+  const gm::Box<S>* boxes = (const gm::Box<S>*)b->dboxes;
+  auto pre = [&](int k) { return b->dprefix + (size_t)k * (cap + 1); };
+  hipStream_t s = b->stream;
+  const gm::Shape& sh = b->shape;
+  const int solver = b->bp.riemann_solver, fcd = b->bp.enable_flux_cd;
+  for (int st = 0; st < nsteps; st++) {
+    gm::launch_clock_begin(s, b->dclk, cap);
+    gm::launch_cfl_batch(s, boxes, pre(gm::KC_CFLRED), n, b->tiles[gm::KC_CFLRED], b->dclk, b->derr, sh);
+    gm::launch_clock_dt(s, b->dclk, cap, tuning);
+    for (int stage = 1; stage <= 2; stage++) {
+      gm::launch_prim_batch(s, boxes, pre(stage == 1 ? gm::KC_PRIM1 : gm::KC_PRIM2), n,
+                            b->tiles[stage == 1 ? gm::KC_PRIM1 : gm::KC_PRIM2], stage, sh);
+      for (int a = 0; a < 3; a++)
+        gm::launch_flux_batch(s, boxes, pre(gm::KC_FLUXX + a), n, b->tiles[gm::KC_FLUXX + a], solver, stage, a,
+                              b->derr, sh);
+      if (fcd) gm::launch_efield_batch(s, boxes, pre(gm::KC_EFIELD), n, b->tiles[gm::KC_EFIELD], sh);
+      gm::launch_update_batch(s, boxes, pre(gm::KC_UPDATE), n, b->tiles[gm::KC_UPDATE], b->dclk, stage, fcd, sh);
+    }
+  }
+}
+
+void enqueue_steps(gm_batch* b, int nsteps, int tuning) {
+  if (b->esize == 8) enqueue_steps_t<double>(b, nsteps, tuning);
+  else enqueue_steps_t<float>(b, nsteps, tuning);
+}
+
+int capture_graph(gm_batch* b) {
+  drop_graph(b);
+  BHIP(b, hipStreamSynchronize(b->stream));
+  BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  enqueue_steps(b, b->bp.steps_per_launch, 0);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (e != hipSuccess) return bhip(b, e, "hipStreamEndCapture");
+  b->graph = g;
+  BHIP(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  b->graph_dirty = false;
+  return GM_OK;
+}
+
+// Time candidate shapes on the admitted boxes; keep the fastest.
+int autotune(gm_batch* b) {
+  static const int blocks[] = {64, 128, 256};
+  static const int chunks[] = {8, 16, 32};
+  hipEvent_t e0, e1;
+  BHIP(b, hipEventCreate(&e0));
+  BHIP(b, hipEventCreate(&e1));
+  double best = 1e300;
+  gm::Shape best_sh = b->shape;
+  for (int blk : blocks)
+    for (int kc : chunks) {
+      const gm::Shape sh = {blk, kc};
+      b->shape = sh;
+      int rc = rebuild_tables(b, sh);
+      if (rc != GM_OK) return rc;
+      enqueue_steps(b, 1, 1);  // warm-up
+      (void)hipEventRecord(e0, b->stream);
+      enqueue_steps(b, 2, 1);
+      (void)hipEventRecord(e1, b->stream);
+      BHIP(b, hipEventSynchronize(e1));
+      float ms = 0.f;
+      BHIP(b, hipEventElapsedTime(&ms, e0, e1));
+      if (ms / 2.0 < best) {
+        best = ms / 2.0;
+        best_sh = sh;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->shape = best_sh;
+  b->tuned = true;
+  b->tuned_ms = best;
+  b->tables_dirty = true;
+  b->graph_dirty = true;
+  return GM_OK;
+}
+
+int read_back(gm_batch* b) {
+  BHIP(b, hipStreamSynchronize(b->stream));
+  BHIP(b, hipMemcpy(b->hclk, b->dclk, sizeof(gm::BoxClock) * b->bp.max_boxes, hipMemcpyDeviceToHost));
+  BHIP(b, hipMemcpy(b->herr, b->derr, sizeof(int), hipMemcpyDeviceToHost));
+  for (int i = 0; i < b->bp.max_boxes; i++)
+    if (b->slots[i].used) b->slots[i].clock = b->hclk[i];
+  if (*b->herr) {
+    const int e = *b->herr;
+    BHIP(b, hipMemset(b->derr, 0, sizeof(int)));
+    return bfail(b, GM_ERR_NONFINITE,
+                 (e & 1) ? "Riemann solver fell through on a non-finite state (hlld.f90:316-317)"
+                         : "non-finite value in a box's conserved state");
+  }
+  return GM_OK;
+}
+
+bool valid_id(const gm_batch* b, int32_t id) {
+  return id >= 0 && id < b->bp.max_boxes && b->slots[id].used;
+}
+
+template <typename S>
+void ghost_fill_t(gm_batch* b, const Slot& s) {
+  const gm::Geom g = geom_of(s.p);
+  S* U = (S*)s.U;
+  for (int a = 0; a < 3; a++)
+    if (g.bc[2 * a] == GM_BC_PERIODIC) {
+      gm::launch_ghost1(b->stream, g, U, a, 0, GM_BC_PERIODIC);
+      gm::launch_ghost1(b->stream, g, U, a, 1, GM_BC_PERIODIC);
+    }
+  for (int kind : {GM_BC_CLOSED, GM_BC_OUTFLOW})
+    for (int f = 0; f < 6; f++)
+      if (g.bc[f] == kind) gm::launch_ghost1(b->stream, g, U, f / 2, f % 2, kind);
+}
+
+}  // namespace
+
+extern "C" {
+
+int gm_batch_create(const gm_batch_params* bp, gm_batch** out) {
+  if (!out) return bfail(nullptr, GM_ERR_ARG, "batch out-pointer is NULL");
+  *out = nullptr;
+  if (!bp) return bfail(nullptr, GM_ERR_ARG, "params is NULL");
+  if (bp->precision != GM_PRECISION_FP64 && bp->precision != GM_PRECISION_MIXED)
+    return bfail(nullptr, GM_ERR_CONFIG, "precision must be GM_PRECISION_FP64 or GM_PRECISION_MIXED");
+  if (bp->riemann_solver < GM_SOLVER_HLL || bp->riemann_solver > GM_SOLVER_HLLD)
+    return bfail(nullptr, GM_ERR_CONFIG, "riemann_solver out of range");
+  if (bp->max_boxes < 1 || bp->max_boxes > kMaxBoxes)
+    return bfail(nullptr, GM_ERR_CONFIG, "max_boxes must be in 1..4096");
+  if (bp->steps_per_launch < 1 || bp->steps_per_launch > 1000)
+    return bfail(nullptr, GM_ERR_CONFIG, "steps_per_launch must be in 1..1000");
+  if (bp->reserved != 0 || bp->memory_budget < 0) return bfail(nullptr, GM_ERR_CONFIG, "bad reserved / budget field");
+  gm_batch* b = new gm_batch();
+  b->bp = *bp;
+  b->esize = bp->precision == GM_PRECISION_MIXED ? 4 : 8;
+  b->slots.resize(bp->max_boxes);
+  auto bail = [&](int code) {
+    g_batch_create_error = b->msg;
+    gm_batch_destroy(b);
+    return code;
+  };
+  int rc;
+  if ((rc = bhip(b, hipSetDevice(bp->device), "hipSetDevice")) != GM_OK) return bail(rc);
+  if ((rc = bhip(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate")) != GM_OK)
+    return bail(rc);
+  size_t freeb = 0, totalb = 0;
+  if ((rc = bhip(b, hipMemGetInfo(&freeb, &totalb), "hipMemGetInfo")) != GM_OK) return bail(rc);
+  b->budget = bp->memory_budget > 0 ? bp->memory_budget : (int64_t)(0.8 * (double)freeb);
+  const int cap = bp->max_boxes;
+  if ((rc = bhip(b, hipMalloc(&b->dboxes, sizeof(gm::Box<double>) * cap), "hipMalloc")) != GM_OK) return bail(rc);
+  if ((rc = bhip(b, hipMalloc((void**)&b->dprefix, sizeof(unsigned) * gm::KC_COUNT * (cap + 1)), "hipMalloc")) !=
+      GM_OK)
+    return bail(rc);
+  if ((rc = bhip(b, hipMalloc((void**)&b->dclk, sizeof(gm::BoxClock) * cap), "hipMalloc")) != GM_OK) return bail(rc);
+  if ((rc = bhip(b, hipMalloc((void**)&b->derr, sizeof(int)), "hipMalloc")) != GM_OK) return bail(rc);
+  if ((rc = bhip(b, hipHostMalloc((void**)&b->hclk, sizeof(gm::BoxClock) * cap, hipHostMallocDefault),
+                 "hipHostMalloc")) != GM_OK)
+    return bail(rc);
+  if ((rc = bhip(b, hipHostMalloc((void**)&b->herr, sizeof(int), hipHostMallocDefault), "hipHostMalloc")) != GM_OK)
+    return bail(rc);
+  // every slot starts as a completed, empty clock
+  for (int i = 0; i < cap; i++) {
+    b->hclk[i] = gm::BoxClock();
+    b->hclk[i].done = 1;
+  }
+  if ((rc = bhip(b, hipMemcpy(b->dclk, b->hclk, sizeof(gm::BoxClock) * cap, hipMemcpyHostToDevice), "hipMemcpy")) !=
+      GM_OK)
+    return bail(rc);
+  if ((rc = bhip(b, hipMemset(b->derr, 0, sizeof(int)), "hipMemset")) != GM_OK) return bail(rc);
+  *out = b;
+  return GM_OK;
+}
+
+int gm_batch_destroy(gm_batch* b) {
+  if (!b) return GM_ERR_ARG;
+  (void)hipSetDevice(b->bp.device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  drop_graph(b);
+  for (auto& s : b->slots)
+    if (s.used) free_slot(s);
+  if (b->dboxes) (void)hipFree(b->dboxes);
+  if (b->dprefix) (void)hipFree(b->dprefix);
+  if (b->dclk) (void)hipFree(b->dclk);
+  if (b->derr) (void)hipFree(b->derr);
+  if (b->hclk) (void)hipHostFree(b->hclk);
+  if (b->herr) (void)hipHostFree(b->herr);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return GM_OK;
+}
+
+const char* gm_batch_last_error(const gm_batch* b) { return b ? b->msg.c_str() : g_batch_create_error.c_str(); }
+
+int gm_batch_box_bytes(const gm_batch* b, const gm_params* p, int64_t* bytes) {
+  if (!b || !p || !bytes) return GM_ERR_ARG;
+  *bytes = box_bytes_of(*p, b->esize);
+  return GM_OK;
+}
+
+int gm_batch_free_bytes(const gm_batch* b, int64_t* bytes) {
+  if (!b || !bytes) return GM_ERR_ARG;
+  *bytes = b->budget - b->used;
+  return GM_OK;
+}
+
+int gm_batch_add(gm_batch* b, const gm_params* p, const double* u, double cfl, double t_end, int64_t max_steps,
+                 int32_t* id) {
+  if (!b) return GM_ERR_ARG;
+  if (!p || !u || !id) return bfail(b, GM_ERR_ARG, "params, u or id is NULL");
+  std::string m;
+  int rc = gm::check_params(p, m);
+  if (rc != GM_OK) return bfail(b, rc, m);
+  if (p->precision != b->bp.precision || p->riemann_solver != b->bp.riemann_solver ||
+      (p->enable_flux_cd != 0) != (b->bp.enable_flux_cd != 0))
+    return bfail(b, GM_ERR_CONFIG, "box precision, solver and flux-CD must match the batch");
+  if (!(cfl > 0) || !std::isfinite(cfl)) return bfail(b, GM_ERR_ARG, "cfl must be positive");
+  if (!(t_end > 0)) return bfail(b, GM_ERR_ARG, "t_end must be positive (use +inf for no time limit)");
+  if (max_steps < 0) return bfail(b, GM_ERR_ARG, "max_steps must be >= 0");
+  if (std::isinf(t_end) && max_steps == 0) return bfail(b, GM_ERR_ARG, "a box needs t_end or max_steps");
+  int slot = -1;
+  for (int i = 0; i < b->bp.max_boxes; i++)
+    if (!b->slots[i].used) {
+      slot = i;
+      break;
+    }
+  if (slot < 0) return bfail(b, GM_ERR_BUSY, "box table full");
+  const int64_t need = box_bytes_of(*p, b->esize);
+  if (b->used + need > b->budget) return bfail(b, GM_ERR_BUSY, "memory budget exhausted");
+  BHIP(b, hipSetDevice(b->bp.device));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  Slot s;
+  s.p = *p;
+  s.p.device = b->bp.device;
+  const gm::Geom g = geom_of(*p);
+  const size_t n8 = (size_t)g.ncell * 8 * b->esize, n3 = (size_t)g.ncell * 3 * b->esize;
+  struct { void** ptr; size_t n; } al[] = {{&s.U, n8}, {&s.Us, n8}, {&s.P, n8}, {&s.Fb, 3 * n8}, {&s.E, n3}};
+  for (auto& a : al) {
+    if (hipMalloc(a.ptr, a.n) != hipSuccess) {
+      (void)hipGetLastError();
+      free_slot(s);
+      return bfail(b, GM_ERR_OOM, "device allocation failed");
+    }
+    (void)hipMemsetAsync(*a.ptr, 0, a.n, b->stream);
+  }
+  s.bytes = need;
+  // upload: host AoS doubles -> staging (flux block) -> SoA state
+  BHIP(b, hipMemcpyAsync(s.Fb, u, (size_t)g.ncell * 8 * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  if (b->esize == 8) gm::launch_aos_to_soa(b->stream, g.ncell, (const double*)s.Fb, (double*)s.U);
+  else gm::launch_aos_to_soa(b->stream, g.ncell, (const double*)s.Fb, (float*)s.U);
+  BHIP(b, hipGetLastError());
+  gm::BoxClock c{};
+  c.dtmin = 0;
+  c.dt = 0.0;
+  c.time = 0.0;
+  c.t_end = t_end;
+  c.cfl = cfl;
+  c.it = 1;
+  c.n_iter = 10;
+  c.steps = 0;
+  c.max_steps = (int)max_steps;
+  c.done = 0;
+  b->hclk[slot] = c;
+  BHIP(b, hipMemcpyAsync(b->dclk + slot, b->hclk + slot, sizeof(gm::BoxClock), hipMemcpyHostToDevice, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  s.used = true;
+  s.clock = c;
+  b->slots[slot] = s;
+  b->used += need;
+  b->tables_dirty = true;
+  b->graph_dirty = true;
+  *id = slot;
+  return GM_OK;
+}
+
+int gm_batch_remove(gm_batch* b, int32_t id) {
+  if (!b) return GM_ERR_ARG;
+  if (!valid_id(b, id)) return bfail(b, GM_ERR_ARG, "no such box");
+  BHIP(b, hipSetDevice(b->bp.device));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  gm::BoxClock c{};
+  c.done = 1;
+  b->hclk[id] = c;
+  BHIP(b, hipMemcpy(b->dclk + id, b->hclk + id, sizeof(gm::BoxClock), hipMemcpyHostToDevice));
+  b->used -= b->slots[id].bytes;
+  free_slot(b->slots[id]);
+  b->tables_dirty = true;
+  b->graph_dirty = true;
+  return GM_OK;
+}
+
+int gm_batch_advance(gm_batch* b, int32_t nlaunch) {
+  if (!b) return GM_ERR_ARG;
+  if (nlaunch < 1) return bfail(b, GM_ERR_ARG, "nlaunch must be >= 1");
+  BHIP(b, hipSetDevice(b->bp.device));
+  int active = 0;
+  for (auto& s : b->slots) active += s.used ? 1 : 0;
+  if (active == 0) return GM_OK;
+  int rc;
+  if (b->bp.autotune && !b->tuned) {
+    if ((rc = autotune(b)) != GM_OK) return rc;
+  }
+  if (b->tables_dirty) {
+    if ((rc = rebuild_tables(b, b->shape)) != GM_OK) return rc;
+    b->graph_dirty = true;
+  }
+  if (b->graph_dirty) {
+    if ((rc = capture_graph(b)) != GM_OK) return rc;
+  }
+  for (int i = 0; i < nlaunch; i++) BHIP(b, hipGraphLaunch(b->exec, b->stream));
+  return GM_OK;
+}
+
+int gm_batch_sync(gm_batch* b) {
+  if (!b) return GM_ERR_ARG;
+  BHIP(b, hipSetDevice(b->bp.device));
+  return read_back(b);
+}
+
+int gm_batch_clock(gm_batch* b, int32_t id, double* time, int64_t* steps, int32_t* done) {
+  if (!b) return GM_ERR_ARG;
+  if (!valid_id(b, id)) return bfail(b, GM_ERR_ARG, "no such box");
+  const gm::BoxClock& c = b->slots[id].clock;
+  if (time) *time = c.time;
+  if (steps) *steps = c.steps;
+  if (done) *done = c.done;
+  return GM_OK;
+}
+
+int gm_batch_download_u(gm_batch* b, int32_t id, double* u) {
+  if (!b) return GM_ERR_ARG;
+  if (!valid_id(b, id)) return bfail(b, GM_ERR_ARG, "no such box");
+  if (!u) return bfail(b, GM_ERR_ARG, "u is NULL");
+  BHIP(b, hipSetDevice(b->bp.device));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  const Slot& s = b->slots[id];
+  const gm::Geom g = geom_of(s.p);
+  if (b->esize == 8) {
+    ghost_fill_t<double>(b, s);
+    gm::launch_soa_to_aos(b->stream, g.ncell, (const double*)s.U, (double*)s.Fb);
+  } else {
+    ghost_fill_t<float>(b, s);
+    gm::launch_soa_to_aos(b->stream, g.ncell, (const float*)s.U, (double*)s.Fb);
+  }
+  BHIP(b, hipGetLastError());
+  BHIP(b, hipMemcpyAsync(u, s.Fb, (size_t)g.ncell * 8 * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  return GM_OK;
+}
+
+int gm_batch_tuning(const gm_batch* b, int32_t* block, int32_t* kchunk, double* ms_per_step) {
+  if (!b) return GM_ERR_ARG;
+  if (block) *block = b->shape.block;
+  if (kchunk) *kchunk = b->shape.kchunk;
+  if (ms_per_step) *ms_per_step = b->tuned ? b->tuned_ms : 0.0;
+  return GM_OK;
+}
+
+int gm_batch_active(const gm_batch* b, int32_t* n) {
+  if (!b || !n) return GM_ERR_ARG;
+  int a = 0;
+  for (auto& s : b->slots) a += s.used ? 1 : 0;
+  *n = a;
+  return GM_OK;
+}
+
+}  // extern "C"

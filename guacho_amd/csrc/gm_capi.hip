@@ -56,6 +56,7 @@ struct gm_ctx {
   Scalars* dscal = nullptr;  // device scalars
   Scalars* hscal = nullptr;  // pinned host mirror
   bool ghosts_dirty = true;
+  gm::Shape shape = {256, 16};  // tile width, z-pencil length
   int timing = 0;
   struct Rec { int cls; hipEvent_t a, b; };
   std::vector<Rec> recs;
@@ -114,6 +115,22 @@ void timed(gm_ctx* c, int cls, Fn&& fn) {
 }
 
 int validate(const gm_params* p) {
+  std::string msg;
+  const int rc = gm::check_params(p, msg);
+  if (rc != GM_OK) return fail(nullptr, rc, msg);
+  return GM_OK;
+}
+
+}  // namespace
+
+namespace gm {
+
+// Configuration check shared by gm_create and gm_batch_add.
+int check_params(const gm_params* p, std::string& msg) {
+  auto fail = [&msg](gm_ctx*, int code, const std::string& m) {
+    msg = m;
+    return code;
+  };
   if (!p) return fail(nullptr, GM_ERR_ARG, "params is NULL");
   if (p->neq != 8) return fail(nullptr, GM_ERR_CONFIG, "neq must be 8 (BFIELD, no passive scalars)");
   if (p->nghost != 2) return fail(nullptr, GM_ERR_CONFIG, "nghost must be 2");
@@ -152,6 +169,10 @@ int validate(const gm_params* p) {
   if (p->reserved != 0) return fail(nullptr, GM_ERR_CONFIG, "reserved must be 0");
   return GM_OK;
 }
+
+}  // namespace gm
+
+namespace {
 
 void release(gm_ctx* c) {
   if (!c) return;
@@ -196,40 +217,52 @@ void fill_ghost1(gm_ctx* c) {
   else fill_ghost1_t<float>(c);
 }
 
+// this context's box, as the kernels see it
+template <typename S>
+gm::Box<S> box_of(const gm_ctx* c) {
+  gm::Box<S> b;
+  b.g = c->g;
+  b.ph = c->ph;
+  b.U = (S*)c->U;
+  b.Us = (S*)c->Us;
+  b.P = (S*)c->P;
+  b.F = (S*)c->Fb;
+  b.E = (S*)c->E;
+  b.dx = c->p.dx;
+  b.dy = c->p.dy;
+  b.dz = c->p.dz;
+  b.slot = 0;
+  return b;
+}
+
 // the step of one precision: tstep (src/hydro_solver.f90:134-229)
 template <typename S>
 void tstep_t(gm_ctx* c, double dt) {
-  const gm::Geom& g = c->g;
   const gm_params& p = c->p;
   hipStream_t s = c->stream;
   int* err = &c->dscal->err;
-  S* U = (S*)c->U;
-  S* Us = (S*)c->Us;
-  S* P = (S*)c->P;
-  S* F[3] = {(S*)c->Fb, (S*)c->Fb + 8 * g.ncell, (S*)c->Fb + 16 * g.ncell};
-  S* E = (S*)c->E;
+  const gm::Box<S> b = box_of<S>(c);
+  const gm::Shape& sh = c->shape;
   const double dtm = dt / 2.0;  // hydro_solver.f90:152
   // predictor: first-order fluxes of U^n, U* = U^n - dt/2 div F
-  timed(c, KC_PRIM, [&] { gm::launch_prim(s, g, c->ph, (const S*)U, P, 0); });
+  timed(c, KC_PRIM, [&] { gm::launch_prim(s, b, 1, sh); });
   for (int a = 0; a < 3; a++)
-    timed(c, KC_FLUX1, [&] { gm::launch_flux(s, g, c->ph, p.riemann_solver, 1, a, (const S*)P, F[a], err); });
-  if (p.enable_flux_cd)
-    timed(c, KC_EFIELD, [&] { gm::launch_efield(s, g, (const S*)F[0], (const S*)F[1], (const S*)F[2], E); });
-  timed(c, KC_UPDATE, [&] {
-    gm::launch_update(s, g, (const S*)U, Us, (const S*)F[0], (const S*)F[1], (const S*)F[2], (const S*)E, dtm,
-                      p.dx, p.dy, p.dz, p.enable_flux_cd);
-  });
+    timed(c, KC_FLUX1, [&] { gm::launch_flux(s, b, p.riemann_solver, 1, a, err, sh); });
+  if (p.enable_flux_cd) timed(c, KC_EFIELD, [&] { gm::launch_efield(s, b, sh); });
+  timed(c, KC_UPDATE, [&] { gm::launch_update(s, b, dtm, 1, p.enable_flux_cd, sh); });
   // corrector: limited fluxes of U*, U^{n+1} = U^n - dt div F (in place;
   // viscous_copy with eta = 0 is the identity on values)
-  timed(c, KC_PRIM, [&] { gm::launch_prim(s, g, c->ph, (const S*)Us, P, -1); });
+  timed(c, KC_PRIM, [&] { gm::launch_prim(s, b, 2, sh); });
   for (int a = 0; a < 3; a++)
-    timed(c, KC_FLUX2, [&] { gm::launch_flux(s, g, c->ph, p.riemann_solver, 2, a, (const S*)P, F[a], err); });
-  if (p.enable_flux_cd)
-    timed(c, KC_EFIELD, [&] { gm::launch_efield(s, g, (const S*)F[0], (const S*)F[1], (const S*)F[2], E); });
-  timed(c, KC_UPDATE, [&] {
-    gm::launch_update(s, g, (const S*)U, U, (const S*)F[0], (const S*)F[1], (const S*)F[2], (const S*)E, dt, p.dx,
-                      p.dy, p.dz, p.enable_flux_cd);
-  });
+    timed(c, KC_FLUX2, [&] { gm::launch_flux(s, b, p.riemann_solver, 2, a, err, sh); });
+  if (p.enable_flux_cd) timed(c, KC_EFIELD, [&] { gm::launch_efield(s, b, sh); });
+  timed(c, KC_UPDATE, [&] { gm::launch_update(s, b, dt, 2, p.enable_flux_cd, sh); });
+}
+
+template <typename S>
+void cfl_t(gm_ctx* c) {
+  const gm::Box<S> b = box_of<S>(c);
+  timed(c, KC_CFL, [&] { gm::launch_cfl(c->stream, b, &c->dscal->dtmin, &c->dscal->err, c->shape); });
 }
 
 int read_errors(gm_ctx* c) {
@@ -391,14 +424,8 @@ int gm_cfl_dtp(gm_ctx* c, double* dtp) {
   int rc = set_device(c);
   if (rc) return rc;
   hipLaunchKernelGGL(k_init_dtmin, dim3(1), dim3(1), 0, c->stream, c->dscal);
-  timed(c, KC_CFL, [&] {
-    if (c->esize == 8)
-      gm::launch_cfl(c->stream, c->g, c->ph, (const double*)c->U, c->p.dx, c->p.dy, c->p.dz, &c->dscal->dtmin,
-                     &c->dscal->err);
-    else
-      gm::launch_cfl(c->stream, c->g, c->ph, (const float*)c->U, c->p.dx, c->p.dy, c->p.dz, &c->dscal->dtmin,
-                     &c->dscal->err);
-  });
+  if (c->esize == 8) cfl_t<double>(c);
+  else cfl_t<float>(c);
   GM_HIP(c, hipGetLastError());
   GM_HIP(c, hipMemcpyAsync(c->hscal, c->dscal, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
   GM_HIP(c, hipStreamSynchronize(c->stream));

@@ -34,12 +34,27 @@ __device__ __forceinline__ unsigned xcd_logical_block() {
 }
 
 // Logical tile -> (flat index within a plane of `plane_n` items, plane).
-// Tiles of 256 items; `bpp` tiles per plane.  Returns false past the end.
-__device__ __forceinline__ bool tile_item(unsigned bpp, unsigned plane_n, int& t, int& plane) {
-  const unsigned lb = xcd_logical_block();
+// Tiles of blockDim.x items; `bpp` tiles per plane.  False past the end.
+__device__ __forceinline__ bool tile_item(unsigned lb, unsigned bpp, unsigned plane_n, int& t, int& plane) {
   plane = (int)(lb / bpp);
-  t = (int)((lb - (unsigned)plane * bpp) * 256u + threadIdx.x);
+  t = (int)((lb - (unsigned)plane * bpp) * blockDim.x + threadIdx.x);
   return (unsigned)t < plane_n;
+}
+
+__device__ __forceinline__ unsigned tiles_of(unsigned items) { return (items + blockDim.x - 1) / blockDim.x; }
+
+// Block of a batched launch -> (box, logical tile within the box), from the
+// prefix sums of the boxes' tile counts (prefix[0] = 0).
+__device__ __forceinline__ void batch_locate(const unsigned* __restrict__ prefix, int nbox, unsigned lb,
+                                             int& box, unsigned& local) {
+  int lo = 0, hi = nbox;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= lb) lo = mid;
+    else hi = mid;
+  }
+  box = lo;
+  local = lb - prefix[lo];
 }
 
 // Interior source of ghost index i on an axis of n cells; flips when the
@@ -76,48 +91,54 @@ __device__ __forceinline__ void load_cons(const Geom& g, const S* __restrict__ U
 }
 
 // ---------------------------------------------------------------------------
-// primitives: calcprim over the cells the flux stencils read
-// (src/hydro_core.f90:245-320), ghost values by boundary mapping.
+// Kernel bodies.  Each works on one logical tile `lb` of one box; the
+// __global__ wrappers below either launch one box (grid = its tiles, XCD
+// order) or a batch of boxes (grid = all tiles of all boxes, each block
+// finding its box in a prefix table).  Stage 1 is the predictor (reads U^n,
+// writes U*), stage 2 the corrector (reads U*, updates U^n in place).
 // ---------------------------------------------------------------------------
+
+// primitives: calcprim over the cells the flux stencils read
+// (src/hydro_core.f90:245-320), ghost values by boundary mapping; FP64
+// pressure recovery for either storage type.
 template <typename S>
-__global__ void __launch_bounds__(256) k_prim(Geom g, Phys ph, const S* __restrict__ U, S* __restrict__ P,
-                                              int lo, unsigned bpp) {
-  // cells lo..n+1-lo on each axis (lo = 0: one ghost layer, lo = -1: two)
+__device__ __forceinline__ void prim_body(const Box<S>& b, int stage, unsigned lb) {
+  const Geom& g = b.g;
+  const int lo = stage == 1 ? 0 : -1;  // one ghost layer for first order, two for the limiter
   const int ex = g.nx + 2 - 2 * lo, ey = g.ny + 2 - 2 * lo;
   int t, kk;
-  if (!tile_item(bpp, (unsigned)(ex * ey), t, kk)) return;
+  if (!tile_item(lb, tiles_of((unsigned)(ex * ey)), (unsigned)(ex * ey), t, kk)) return;
   const int i = t % ex + lo, j = t / ex + lo, k = kk + lo;
   // only face ghosts are read downstream; skip edges and corners
   const int out = (i < 1 || i > g.nx) + (j < 1 || j > g.ny) + (k < 1 || k > g.nz);
   if (out > 1) return;
   double u[8], p[8];
-  load_cons(g, U, i, j, k, u);
-  u2prim(ph, u, p);
+  load_cons(g, stage == 1 ? b.U : b.Us, i, j, k, u);
+  u2prim(b.ph, u, p);
   const long long c = cidx(g, i, j, k);
 #pragma unroll
-  for (int q = 0; q < 8; q++) P[q * g.ncell + c] = (S)p[q];
+  for (int q = 0; q < 8; q++) b.P[q * g.ncell + c] = (S)p[q];
 }
 
-// ---------------------------------------------------------------------------
 // interface fluxes: the sweep of hlldfluxes / hllefluxes / hllfluxes /
 // hllcfluxes (e.g. src/hlld.f90:331-432) for one axis.  Face f(:,i,j,k)
 // lies between cells i and i+1 (i = 0..n along the axis); only faces with
 // interior transverse indices feed the update.
-// ---------------------------------------------------------------------------
 template <int SOLVER, int ORDER, int axis, typename S>
-__global__ void __launch_bounds__(256) k_flux(Geom g, Phys ph, const S* __restrict__ P, S* __restrict__ F,
-                                              int* __restrict__ err, unsigned bpp) {
-  // face grid: along-axis index 0..n, transverse indices interior
+__device__ __forceinline__ void flux_body(const Box<S>& b, unsigned lb, int* __restrict__ err) {
+  const Geom& g = b.g;
   const int e0 = axis == 0 ? g.nx + 1 : g.nx;
   const int e1 = axis == 1 ? g.ny + 1 : g.ny;
   int t, k;
-  if (!tile_item(bpp, (unsigned)(e0 * e1), t, k)) return;
+  if (!tile_item(lb, tiles_of((unsigned)(e0 * e1)), (unsigned)(e0 * e1), t, k)) return;
   int i = t % e0, j = t / e0;
   if (axis != 0) i += 1;
   if (axis != 1) j += 1;
   if (axis != 2) k += 1;
   const long long stride = axis == 0 ? 1 : axis == 1 ? (long long)g.NX : (long long)g.NX * g.NY;
   const long long c = cidx(g, i, j, k);
+  const S* __restrict__ P = b.P;
+  S* __restrict__ F = b.F + axis * 8 * g.ncell;
   S pl[8], pr[8], ff[8];
 #pragma unroll
   for (int q = 0; q < 8; q++) {
@@ -131,30 +152,32 @@ __global__ void __launch_bounds__(256) k_flux(Geom g, Phys ph, const S* __restri
       pll[q] = P[q * g.ncell + c - stride];
       prr[q] = P[q * g.ncell + c + 2 * stride];
     }
-    limiter(ph.limiter, pll, pl, pr, prr);
+    limiter(b.ph.limiter, pll, pl, pr, prr);
   }
   axis_perm<axis>(pl);
   axis_perm<axis>(pr);
-  const bool ok = riemann<SOLVER>(ph, pl, pr, ff);
+  const bool ok = riemann<SOLVER>(b.ph, pl, pr, ff);
   if (!ok) atomicOr(err, 1);
   axis_perm<axis>(ff);
 #pragma unroll
   for (int q = 0; q < 8; q++) F[q * g.ncell + c] = ff[q];
 }
 
-// ---------------------------------------------------------------------------
 // cell-centred electric field, get_efield (src/flux_cd_module.f90:245-273)
-// ---------------------------------------------------------------------------
 template <typename S>
-__global__ void __launch_bounds__(256) k_efield(Geom g, const S* __restrict__ Fx, const S* __restrict__ Fy,
-                                                const S* __restrict__ Fz, S* __restrict__ E, unsigned bpp) {
+__device__ __forceinline__ void efield_body(const Box<S>& b, unsigned lb) {
+  const Geom& g = b.g;
   int t, k;
-  if (!tile_item(bpp, (unsigned)(g.nx * g.ny), t, k)) return;
+  if (!tile_item(lb, tiles_of((unsigned)(g.nx * g.ny)), (unsigned)(g.nx * g.ny), t, k)) return;
   const int i = t % g.nx + 1, j = t / g.nx + 1;
   k += 1;
   const long long sx = 1, sy = g.NX, sz = (long long)g.NX * g.NY;
   const long long N = g.ncell;
   const long long c = cidx(g, i, j, k);
+  const S* __restrict__ Fx = b.F;
+  const S* __restrict__ Fy = b.F + 8 * N;
+  const S* __restrict__ Fz = b.F + 16 * N;
+  S* __restrict__ E = b.E;
   auto ld = [](const S* a, long long o) { return (double)a[o]; };
   E[0 * N + c] = (S)(0.25 * (((-ld(Fy, 7 * N + c - sy) - ld(Fy, 7 * N + c)) + ld(Fz, 6 * N + c - sz)) + ld(Fz, 6 * N + c)));
   E[1 * N + c] = (S)(0.25 * (((+ld(Fx, 7 * N + c - sx) + ld(Fx, 7 * N + c)) - ld(Fz, 5 * N + c - sz)) - ld(Fz, 5 * N + c)));
@@ -163,18 +186,19 @@ __global__ void __launch_bounds__(256) k_efield(Geom g, const S* __restrict__ Fx
 
 // ---------------------------------------------------------------------------
 // upwind update, step(dt) (src/hydro_solver.f90:77-127) with flux_cd_update
-// (src/flux_cd_module.f90:285-323).  Each thread marches up a z-column
-// segment of KCHUNK cells, carrying the z-face flux below the cell and E
-// one plane below and at the cell, so every z neighbour is read once.
+// (src/flux_cd_module.f90:285-323), accumulated in FP64.  Each thread
+// marches up a z-column segment of `kchunk` cells, carrying the z-face flux
+// below the cell and E one plane below and at the cell, so every z
+// neighbour is read once.
 //
 // E ghosts (boundaryI_ef): the update reads e2, e3 at x neighbours, e1, e3
 // at y neighbours and e1, e2 at z neighbours.  A closed boundary negates e1
 // on x ghosts and e2 on y ghosts only (flux_cd_module.f90:145-176), never a
 // component read there, so every E ghost resolves to a fixed interior
 // neighbour offset: periodic wraps, closed and outflow copy the edge cell.
-// Uout may alias Uin (each cell reads only its own conserved state).
+// Stage 2 writes U^{n+1} over U^n in place (each cell reads only its own
+// conserved state).
 // ---------------------------------------------------------------------------
-constexpr int KCHUNK = 16;
 
 // offset (in cells along the axis) of the E ghost-resolved neighbour
 __device__ __forceinline__ int e_nbr(int i, int n, int d, int bc) {
@@ -185,19 +209,23 @@ __device__ __forceinline__ int e_nbr(int i, int n, int d, int bc) {
 }
 
 template <int FLUXCD, typename S>
-__global__ void __launch_bounds__(256) k_update(Geom g, const S* Uin, S* Uout, const S* __restrict__ Fx,
-                                                const S* __restrict__ Fy, const S* __restrict__ Fz,
-                                                const S* __restrict__ E, double dt, double dx, double dy,
-                                                double dz, unsigned bpp) {
+__device__ __forceinline__ void update_body(const Box<S>& b, unsigned lb, double dt, int stage, int kchunk) {
   constexpr int NH = FLUXCD ? 5 : 8;
+  const Geom& g = b.g;
   int t, kc;
-  if (!tile_item(bpp, (unsigned)(g.nx * g.ny), t, kc)) return;
+  if (!tile_item(lb, tiles_of((unsigned)(g.nx * g.ny)), (unsigned)(g.nx * g.ny), t, kc)) return;
   const int i = t % g.nx + 1, j = t / g.nx + 1;
-  const int k0 = kc * KCHUNK + 1;
-  const int k1 = min(k0 + KCHUNK - 1, g.nz);
+  const int k0 = kc * kchunk + 1;
+  const int k1 = min(k0 + kchunk - 1, g.nz);
   const long long sx = 1, sy = g.NX, sz = (long long)g.NX * g.NY;
   const long long N = g.ncell;
-  const double dtdx = dt / dx, dtdy = dt / dy, dtdz = dt / dz;
+  const double dtdx = dt / b.dx, dtdy = dt / b.dy, dtdz = dt / b.dz;
+  const S* Uin = b.U;
+  S* Uout = stage == 1 ? b.Us : b.U;
+  const S* __restrict__ Fx = b.F;
+  const S* __restrict__ Fy = b.F + 8 * N;
+  const S* __restrict__ Fz = b.F + 16 * N;
+  const S* __restrict__ E = b.E;
   long long c = cidx(g, i, j, k0);
   double fzm[NH];
 #pragma unroll
@@ -262,23 +290,24 @@ __device__ __forceinline__ double wave_min(double v) {
 }
 
 template <typename S>
-__global__ void __launch_bounds__(256) k_cfl(Geom g, Phys ph, const S* __restrict__ U, double dx,
-                                             double dy, double dz, unsigned long long* __restrict__ dtmin,
-                                             int* __restrict__ err, unsigned bpp) {
+__device__ __forceinline__ void cfl_body(const Box<S>& b, unsigned lb, int kchunk,
+                                         unsigned long long* __restrict__ dtmin, int* __restrict__ err) {
   __shared__ double red[4];
+  const Geom& g = b.g;
+  const Phys& ph = b.ph;
   double dtp = 1e30;
   bool bad = false;
   int t, kc;
-  if (tile_item(bpp, (unsigned)(g.nx * g.ny), t, kc)) {
+  if (tile_item(lb, tiles_of((unsigned)(g.nx * g.ny)), (unsigned)(g.nx * g.ny), t, kc)) {
     const int i = t % g.nx + 1, j = t / g.nx + 1;
-    const int k0 = kc * KCHUNK + 1;
-    const int k1 = min(k0 + KCHUNK - 1, g.nz);
+    const int k0 = kc * kchunk + 1;
+    const int k1 = min(k0 + kchunk - 1, g.nz);
     for (int k = k0; k <= k1; k++) {
       const long long c = cidx(g, i, j, k);
       double u[8], p[8];
 #pragma unroll
       for (int q = 0; q < 8; q++) {
-        u[q] = (double)U[q * g.ncell + c];
+        u[q] = (double)b.U[q * g.ncell + c];
         bad |= !__builtin_isfinite(u[q]);
       }
       u2prim(ph, u, p);
@@ -287,14 +316,14 @@ __global__ void __launch_bounds__(256) k_cfl(Geom g, Phys ph, const S* __restric
         const double cx = cfast1(ph.gamma, p[4], p[0], b2, p[5]);
         const double cy = cfast1(ph.gamma, p[4], p[0], b2, p[6]);
         const double cz = cfast1(ph.gamma, p[4], p[0], b2, p[7]);
-        dtp = dmin(dtp, dx / (__builtin_fabs(p[1]) + cx));
-        dtp = dmin(dtp, dy / (__builtin_fabs(p[2]) + cy));
-        dtp = dmin(dtp, dz / (__builtin_fabs(p[3]) + cz));
+        dtp = dmin(dtp, b.dx / (__builtin_fabs(p[1]) + cx));
+        dtp = dmin(dtp, b.dy / (__builtin_fabs(p[2]) + cy));
+        dtp = dmin(dtp, b.dz / (__builtin_fabs(p[3]) + cz));
       } else {
         const double cs = csound(ph, p[4], p[0]);
-        dtp = dmin(dtp, dx / (__builtin_fabs(p[1]) + cs));
-        dtp = dmin(dtp, dy / (__builtin_fabs(p[2]) + cs));
-        dtp = dmin(dtp, dz / (__builtin_fabs(p[3]) + cs));
+        dtp = dmin(dtp, b.dx / (__builtin_fabs(p[1]) + cs));
+        dtp = dmin(dtp, b.dy / (__builtin_fabs(p[2]) + cs));
+        dtp = dmin(dtp, b.dz / (__builtin_fabs(p[3]) + cs));
       }
     }
   }
@@ -310,6 +339,116 @@ __global__ void __launch_bounds__(256) k_cfl(Geom g, Phys ph, const S* __restric
     for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = dmin(m, red[w]);
     atomicMin(dtmin, (unsigned long long)__double_as_longlong(m));
   }
+}
+
+// ---------------------------------------------------------------------------
+// __global__ wrappers: one box ...
+// ---------------------------------------------------------------------------
+template <typename S>
+__global__ void __launch_bounds__(256) k_prim(Box<S> b, int stage) { prim_body(b, stage, xcd_logical_block()); }
+
+template <int SOLVER, int ORDER, int AXIS, typename S>
+__global__ void __launch_bounds__(256) k_flux(Box<S> b, int* err) {
+  flux_body<SOLVER, ORDER, AXIS>(b, xcd_logical_block(), err);
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_efield(Box<S> b) { efield_body(b, xcd_logical_block()); }
+
+template <int FLUXCD, typename S>
+__global__ void __launch_bounds__(256) k_update(Box<S> b, double dt, int stage, int kchunk) {
+  update_body<FLUXCD>(b, xcd_logical_block(), dt, stage, kchunk);
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_cfl(Box<S> b, int kchunk, unsigned long long* dtmin, int* err) {
+  cfl_body(b, xcd_logical_block(), kchunk, dtmin, err);
+}
+
+// ... and a batch of boxes
+template <typename S>
+__global__ void __launch_bounds__(256) kb_prim(const Box<S>* __restrict__ boxes, const unsigned* __restrict__ prefix,
+                                               int nbox, int stage) {
+  int bx;
+  unsigned lb;
+  batch_locate(prefix, nbox, xcd_logical_block(), bx, lb);
+  prim_body(boxes[bx], stage, lb);
+}
+
+template <int SOLVER, int ORDER, int AXIS, typename S>
+__global__ void __launch_bounds__(256) kb_flux(const Box<S>* __restrict__ boxes, const unsigned* __restrict__ prefix,
+                                               int nbox, int* err) {
+  int bx;
+  unsigned lb;
+  batch_locate(prefix, nbox, xcd_logical_block(), bx, lb);
+  flux_body<SOLVER, ORDER, AXIS>(boxes[bx], lb, err);
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) kb_efield(const Box<S>* __restrict__ boxes, const unsigned* __restrict__ prefix,
+                                                 int nbox) {
+  int bx;
+  unsigned lb;
+  batch_locate(prefix, nbox, xcd_logical_block(), bx, lb);
+  efield_body(boxes[bx], lb);
+}
+
+// dt of each box from the device clock table; the predictor uses dt/2
+// (hydro_solver.f90:152)
+template <int FLUXCD, typename S>
+__global__ void __launch_bounds__(256) kb_update(const Box<S>* __restrict__ boxes, const unsigned* __restrict__ prefix,
+                                                 int nbox, const BoxClock* __restrict__ clk, int stage, int kchunk) {
+  int bx;
+  unsigned lb;
+  batch_locate(prefix, nbox, xcd_logical_block(), bx, lb);
+  const Box<S>& b = boxes[bx];
+  const double dt = clk[b.slot].dt;
+  update_body<FLUXCD>(b, lb, stage == 1 ? dt / 2.0 : dt, stage, kchunk);
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) kb_cfl(const Box<S>* __restrict__ boxes, const unsigned* __restrict__ prefix,
+                                              int nbox, int kchunk, BoxClock* clk, int* err) {
+  int bx;
+  unsigned lb;
+  batch_locate(prefix, nbox, xcd_logical_block(), bx, lb);
+  cfl_body(boxes[bx], lb, kchunk, &clk[boxes[bx].slot].dtmin, err);
+}
+
+// Start of a batched step: reset every box's CFL minimum.
+__global__ void kb_clock_begin(BoxClock* clk, int nbox) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nbox) clk[b].dtmin = (unsigned long long)__double_as_longlong(1e30);
+}
+
+// get_timestep's ramp and clamp on the device (src/hydro_core.f90:677-694):
+// dt = cfl * 2^-(n_iter+1-it) * dtp for it <= n_iter, cfl * dtp after;
+// clamped to land on t_end, which completes the box.  A box that is done,
+// or a timing-only launch (tuning != 0), steps with dt = 0: the fluxes are
+// computed in full and the state keeps its values.
+__global__ void kb_clock_dt(BoxClock* clk, int nbox, int tuning) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbox) return;
+  BoxClock c = clk[b];
+  if (c.done || tuning) {
+    clk[b].dt = 0.0;
+    return;
+  }
+  double dtp = __longlong_as_double((long long)c.dtmin);
+  if (c.it <= c.n_iter) dtp = (c.cfl * __builtin_ldexp(1.0, -(c.n_iter + 1 - c.it))) * dtp;
+  else dtp = c.cfl * dtp;
+  double dt = dtp;
+  bool finish = false;
+  if ((c.time + dt) >= c.t_end) {
+    dt = c.t_end - c.time;
+    finish = true;
+  }
+  c.dt = dt;
+  c.time = c.time + dt;
+  c.it += 1;
+  c.steps += 1;
+  if (finish || (c.max_steps > 0 && c.steps >= c.max_steps)) c.done = 1;
+  clk[b] = c;
 }
 
 // ---------------------------------------------------------------------------
@@ -418,74 +557,126 @@ static inline unsigned grid_for(long long n, int block = 256) {
   return (unsigned)b;
 }
 
-static inline unsigned tiles(long long items) { return (unsigned)((items + 255) / 256); }
+static inline unsigned ceil_div(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+
+unsigned box_tiles(const Geom& g, int kclass, const Shape& sh) {
+  const long long B = sh.block;
+  const unsigned nchunk = ceil_div(g.nz, sh.kchunk);
+  switch (kclass) {
+    case KC_PRIM1: return ceil_div((long long)(g.nx + 2) * (g.ny + 2), B) * (unsigned)(g.nz + 2);
+    case KC_PRIM2: return ceil_div((long long)(g.nx + 4) * (g.ny + 4), B) * (unsigned)(g.nz + 4);
+    case KC_FLUXX: return ceil_div((long long)(g.nx + 1) * g.ny, B) * (unsigned)g.nz;
+    case KC_FLUXY: return ceil_div((long long)g.nx * (g.ny + 1), B) * (unsigned)g.nz;
+    case KC_FLUXZ: return ceil_div((long long)g.nx * g.ny, B) * (unsigned)(g.nz + 1);
+    case KC_EFIELD: return ceil_div((long long)g.nx * g.ny, B) * (unsigned)g.nz;
+    case KC_UPDATE:
+    case KC_CFLRED: return ceil_div((long long)g.nx * g.ny, B) * nchunk;
+  }
+  return 0;
+}
 
 template <typename S>
-void launch_prim(hipStream_t s, const Geom& g, const Phys& ph, const S* U, S* P, int lo) {
-  const int ex = g.nx + 2 - 2 * lo, ey = g.ny + 2 - 2 * lo, ez = g.nz + 2 - 2 * lo;
-  const unsigned bpp = tiles((long long)ex * ey);
-  hipLaunchKernelGGL(k_prim<S>, dim3(bpp * ez), dim3(256), 0, s, g, ph, U, P, lo, bpp);
+void launch_prim(hipStream_t s, const Box<S>& b, int stage, const Shape& sh) {
+  const unsigned n = box_tiles(b.g, stage == 1 ? KC_PRIM1 : KC_PRIM2, sh);
+  hipLaunchKernelGGL(k_prim<S>, dim3(n), dim3(sh.block), 0, s, b, stage);
 }
 
 template <int SOLVER, int ORDER, int AXIS, typename S>
-static void launch_flux_a(hipStream_t s, const Geom& g, const Phys& ph, const S* P, S* F, int* err) {
-  const long long e0 = AXIS == 0 ? g.nx + 1 : g.nx;
-  const long long e1 = AXIS == 1 ? g.ny + 1 : g.ny;
-  const long long e2 = AXIS == 2 ? g.nz + 1 : g.nz;
-  const unsigned bpp = tiles(e0 * e1);
-  hipLaunchKernelGGL((k_flux<SOLVER, ORDER, AXIS, S>), dim3(bpp * (unsigned)e2), dim3(256), 0, s, g, ph, P, F,
-                     err, bpp);
+static void launch_flux_a(hipStream_t s, const Box<S>& b, int* err, const Shape& sh) {
+  const unsigned n = box_tiles(b.g, KC_FLUXX + AXIS, sh);
+  hipLaunchKernelGGL((k_flux<SOLVER, ORDER, AXIS, S>), dim3(n), dim3(sh.block), 0, s, b, err);
 }
 
-template <int SOLVER, int ORDER, typename S>
-static void launch_flux_t(hipStream_t s, const Geom& g, const Phys& ph, int axis, const S* P, S* F, int* err) {
-  if (axis == 0) launch_flux_a<SOLVER, ORDER, 0>(s, g, ph, P, F, err);
-  else if (axis == 1) launch_flux_a<SOLVER, ORDER, 1>(s, g, ph, P, F, err);
-  else launch_flux_a<SOLVER, ORDER, 2>(s, g, ph, P, F, err);
+template <int SOLVER, int ORDER, int AXIS, typename S>
+static void launch_flux_ab(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
+                           int* err, const Shape& sh) {
+  hipLaunchKernelGGL((kb_flux<SOLVER, ORDER, AXIS, S>), dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, err);
 }
 
-template <typename S>
-void launch_flux(hipStream_t s, const Geom& g, const Phys& ph, int solver, int order, int axis, const S* P,
-                 S* F, int* err) {
-#define GM_FLUX_CASE(SV)                                                     \
-  case SV:                                                                   \
-    if (order == 1) launch_flux_t<SV, 1>(s, g, ph, axis, P, F, err);         \
-    else launch_flux_t<SV, 2>(s, g, ph, axis, P, F, err);                    \
-    break;
-  switch (solver) {
-    GM_FLUX_CASE(SOLVER_HLLD)
-    GM_FLUX_CASE(SOLVER_HLLE)
-    GM_FLUX_CASE(SOLVER_HLL)
-    GM_FLUX_CASE(SOLVER_HLLC)
+// solver x order x axis dispatch for both launch forms
+#define GM_FLUX_DISPATCH(CALL)                                                       \
+  switch (solver) {                                                                  \
+    case SOLVER_HLLD: GM_FLUX_OA(SOLVER_HLLD, CALL) break;                            \
+    case SOLVER_HLLE: GM_FLUX_OA(SOLVER_HLLE, CALL) break;                            \
+    case SOLVER_HLL: GM_FLUX_OA(SOLVER_HLL, CALL) break;                              \
+    default: GM_FLUX_OA(SOLVER_HLLC, CALL) break;                                     \
   }
-#undef GM_FLUX_CASE
+#define GM_FLUX_OA(SV, CALL)                                                         \
+  if (order == 1) {                                                                  \
+    if (axis == 0) CALL(SV, 1, 0) else if (axis == 1) CALL(SV, 1, 1) else CALL(SV, 1, 2) \
+  } else {                                                                           \
+    if (axis == 0) CALL(SV, 2, 0) else if (axis == 1) CALL(SV, 2, 1) else CALL(SV, 2, 2) \
+  }
+
+template <typename S>
+void launch_flux(hipStream_t s, const Box<S>& b, int solver, int order, int axis, int* err, const Shape& sh) {
+#define GM_CALL1(SV, O, A) { launch_flux_a<SV, O, A, S>(s, b, err, sh); }
+  GM_FLUX_DISPATCH(GM_CALL1)
+#undef GM_CALL1
 }
 
 template <typename S>
-void launch_efield(hipStream_t s, const Geom& g, const S* Fx, const S* Fy, const S* Fz, S* E) {
-  const unsigned bpp = tiles((long long)g.nx * g.ny);
-  hipLaunchKernelGGL(k_efield<S>, dim3(bpp * g.nz), dim3(256), 0, s, g, Fx, Fy, Fz, E, bpp);
+void launch_flux_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
+                       int solver, int order, int axis, int* err, const Shape& sh) {
+#define GM_CALLB(SV, O, A) { launch_flux_ab<SV, O, A, S>(s, boxes, prefix, nbox, tiles, err, sh); }
+  GM_FLUX_DISPATCH(GM_CALLB)
+#undef GM_CALLB
+}
+#undef GM_FLUX_OA
+#undef GM_FLUX_DISPATCH
+
+template <typename S>
+void launch_efield(hipStream_t s, const Box<S>& b, const Shape& sh) {
+  hipLaunchKernelGGL(k_efield<S>, dim3(box_tiles(b.g, KC_EFIELD, sh)), dim3(sh.block), 0, s, b);
 }
 
 template <typename S>
-void launch_update(hipStream_t s, const Geom& g, const S* Uin, S* Uout, const S* Fx, const S* Fy, const S* Fz,
-                   const S* E, double dt, double dx, double dy, double dz, int flux_cd) {
-  const unsigned bpp = tiles((long long)g.nx * g.ny);
-  const unsigned nchunk = (unsigned)((g.nz + KCHUNK - 1) / KCHUNK);
+void launch_update(hipStream_t s, const Box<S>& b, double dt, int stage, int flux_cd, const Shape& sh) {
+  const unsigned n = box_tiles(b.g, KC_UPDATE, sh);
+  if (flux_cd) hipLaunchKernelGGL((k_update<1, S>), dim3(n), dim3(sh.block), 0, s, b, dt, stage, sh.kchunk);
+  else hipLaunchKernelGGL((k_update<0, S>), dim3(n), dim3(sh.block), 0, s, b, dt, stage, sh.kchunk);
+}
+
+template <typename S>
+void launch_cfl(hipStream_t s, const Box<S>& b, unsigned long long* dtmin, int* err, const Shape& sh) {
+  hipLaunchKernelGGL(k_cfl<S>, dim3(box_tiles(b.g, KC_CFLRED, sh)), dim3(sh.block), 0, s, b, sh.kchunk, dtmin, err);
+}
+
+template <typename S>
+void launch_prim_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
+                       int stage, const Shape& sh) {
+  hipLaunchKernelGGL(kb_prim<S>, dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, stage);
+}
+
+template <typename S>
+void launch_efield_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
+                         const Shape& sh) {
+  hipLaunchKernelGGL(kb_efield<S>, dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox);
+}
+
+template <typename S>
+void launch_update_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
+                         const BoxClock* clk, int stage, int flux_cd, const Shape& sh) {
   if (flux_cd)
-    hipLaunchKernelGGL((k_update<1, S>), dim3(bpp * nchunk), dim3(256), 0, s, g, Uin, Uout, Fx, Fy, Fz, E, dt,
-                       dx, dy, dz, bpp);
+    hipLaunchKernelGGL((kb_update<1, S>), dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, clk, stage,
+                       sh.kchunk);
   else
-    hipLaunchKernelGGL((k_update<0, S>), dim3(bpp * nchunk), dim3(256), 0, s, g, Uin, Uout, Fx, Fy, Fz, E, dt,
-                       dx, dy, dz, bpp);
+    hipLaunchKernelGGL((kb_update<0, S>), dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, clk, stage,
+                       sh.kchunk);
 }
 
 template <typename S>
-void launch_cfl(hipStream_t s, const Geom& g, const Phys& ph, const S* U, double dx, double dy, double dz,
-                unsigned long long* dtmin, int* err) {
-  const unsigned bpp = tiles((long long)g.nx * g.ny);
-  const unsigned nchunk = (unsigned)((g.nz + KCHUNK - 1) / KCHUNK);
-  hipLaunchKernelGGL(k_cfl<S>, dim3(bpp * nchunk), dim3(256), 0, s, g, ph, U, dx, dy, dz, dtmin, err, bpp);
+void launch_cfl_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
+                      BoxClock* clk, int* err, const Shape& sh) {
+  hipLaunchKernelGGL(kb_cfl<S>, dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, sh.kchunk, clk, err);
+}
+
+void launch_clock_begin(hipStream_t s, BoxClock* clk, int nbox) {
+  hipLaunchKernelGGL(kb_clock_begin, dim3(ceil_div(nbox, 64)), dim3(64), 0, s, clk, nbox);
+}
+
+void launch_clock_dt(hipStream_t s, BoxClock* clk, int nbox, int tuning) {
+  hipLaunchKernelGGL(kb_clock_dt, dim3(ceil_div(nbox, 64)), dim3(64), 0, s, clk, nbox, tuning);
 }
 
 template <typename S>
@@ -517,18 +708,25 @@ void launch_divb(hipStream_t s, const Geom& g, const S* U, double dx, double dy,
   hipLaunchKernelGGL(k_divb<S>, dim3(grid_for(n)), dim3(256), 0, s, g, U, dx, dy, dz, D);
 }
 
-#define GM_INSTANTIATE(S)                                                                                       \
-  template void launch_prim<S>(hipStream_t, const Geom&, const Phys&, const S*, S*, int);                        \
-  template void launch_flux<S>(hipStream_t, const Geom&, const Phys&, int, int, int, const S*, S*, int*);        \
-  template void launch_efield<S>(hipStream_t, const Geom&, const S*, const S*, const S*, S*);                    \
-  template void launch_update<S>(hipStream_t, const Geom&, const S*, S*, const S*, const S*, const S*, const S*, \
-                                 double, double, double, double, int);                                           \
-  template void launch_cfl<S>(hipStream_t, const Geom&, const Phys&, const S*, double, double, double,           \
-                              unsigned long long*, int*);                                                        \
-  template void launch_ghost1<S>(hipStream_t, const Geom&, S*, int, int, int);                                   \
-  template void launch_aos_to_soa<S>(hipStream_t, long long, const double*, S*);                                 \
-  template void launch_soa_to_aos<S>(hipStream_t, long long, const S*, double*);                                 \
-  template void launch_prim_all<S>(hipStream_t, long long, const Phys&, const S*, double*, double*);             \
+#define GM_INSTANTIATE(S)                                                                                        \
+  template void launch_prim<S>(hipStream_t, const Box<S>&, int, const Shape&);                                    \
+  template void launch_flux<S>(hipStream_t, const Box<S>&, int, int, int, int*, const Shape&);                    \
+  template void launch_efield<S>(hipStream_t, const Box<S>&, const Shape&);                                       \
+  template void launch_update<S>(hipStream_t, const Box<S>&, double, int, int, const Shape&);                     \
+  template void launch_cfl<S>(hipStream_t, const Box<S>&, unsigned long long*, int*, const Shape&);               \
+  template void launch_prim_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, int,             \
+                                     const Shape&);                                                               \
+  template void launch_flux_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, int, int, int,   \
+                                     int*, const Shape&);                                                         \
+  template void launch_efield_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, const Shape&); \
+  template void launch_update_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned,                \
+                                       const BoxClock*, int, int, const Shape&);                                  \
+  template void launch_cfl_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, BoxClock*, int*,  \
+                                    const Shape&);                                                                \
+  template void launch_ghost1<S>(hipStream_t, const Geom&, S*, int, int, int);                                    \
+  template void launch_aos_to_soa<S>(hipStream_t, long long, const double*, S*);                                  \
+  template void launch_soa_to_aos<S>(hipStream_t, long long, const S*, double*);                                  \
+  template void launch_prim_all<S>(hipStream_t, long long, const Phys&, const S*, double*, double*);              \
   template void launch_divb<S>(hipStream_t, const Geom&, const S*, double, double, double, double*);
 GM_INSTANTIATE(double)
 GM_INSTANTIATE(float)

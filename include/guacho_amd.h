@@ -146,6 +146,59 @@ int gm_kernel_times(gm_ctx *ctx, double *ms, int64_t *launches, const char **nam
 /* Device memory held by the context, in bytes. */
 int gm_device_bytes(const gm_ctx *ctx, int64_t *bytes);
 
+/* ---- batches of independent boxes ----------------------------------- */
+/* No counterpart in the reference (one domain per run there).  A batch
+ * advances many independent subdomains ("boxes") of uneven size together:
+ * every kernel of a step covers the tiles of all boxes in one launch, each
+ * box keeps its own CFL dt (ramp and clamp of hydro_core.f90:677-694 on the
+ * device), and the steps are captured in a HIP graph replayed per advance.
+ * A box's arithmetic does not depend on the other boxes: results are the
+ * same alone and in any mix. */
+#define GM_ERR_BUSY 6      /* no room: memory budget or box table full */
+
+typedef struct gm_batch_params {
+  int32_t precision;        /* GM_PRECISION_*, for every box */
+  int32_t riemann_solver;   /* one solver per batch */
+  int32_t enable_flux_cd;   /* one setting per batch */
+  int32_t device;
+  int32_t max_boxes;        /* capacity of the box table (<= 4096) */
+  int32_t steps_per_launch; /* steps captured in one graph launch (>= 1) */
+  int32_t autotune;         /* 1: time tile width x z-pencil candidates on the first advance */
+  int32_t reserved;         /* must be 0 */
+  int64_t memory_budget;    /* device bytes the boxes may hold; 0: 80% of free memory */
+} gm_batch_params;
+
+typedef struct gm_batch gm_batch;
+
+int gm_batch_create(const gm_batch_params *params, gm_batch **batch);
+int gm_batch_destroy(gm_batch *batch);
+const char *gm_batch_last_error(const gm_batch *batch);
+/* device bytes one box of this configuration holds */
+int gm_batch_box_bytes(const gm_batch *batch, const gm_params *box, int64_t *bytes);
+/* budget left for new boxes */
+int gm_batch_free_bytes(const gm_batch *batch, int64_t *bytes);
+/* Admit a box: uploads u (the reference layout, ghosts included) and
+ * starts its clock at time 0, currentIteration 1.  It completes when its
+ * dt clamps to t_end (the dump clamp of hydro_core.f90:691-694) or after
+ * max_steps steps (0: no limit).  GM_ERR_BUSY when it does not fit. */
+int gm_batch_add(gm_batch *batch, const gm_params *box, const double *u, double cfl, double t_end,
+                 int64_t max_steps, int32_t *box_id);
+/* Release a box and its memory. */
+int gm_batch_remove(gm_batch *batch, int32_t box_id);
+/* Queue nlaunch graph launches of steps_per_launch steps for every active
+ * box (completed boxes step with dt = 0).  Asynchronous. */
+int gm_batch_advance(gm_batch *batch, int32_t nlaunch);
+/* Wait; report deferred device errors (GM_ERR_NONFINITE). */
+int gm_batch_sync(gm_batch *batch);
+/* Clock of a box after the last sync: time, steps taken, completion. */
+int gm_batch_clock(gm_batch *batch, int32_t box_id, double *time, int64_t *steps, int32_t *done);
+/* The box's conserved array after boundaryI's ghost fill (synchronizes). */
+int gm_batch_download_u(gm_batch *batch, int32_t box_id, double *u);
+/* Launch shape in use and the autotuner's measured step time (0 if not tuned). */
+int gm_batch_tuning(const gm_batch *batch, int32_t *block, int32_t *kchunk, double *ms_per_step);
+/* Number of boxes in the batch. */
+int gm_batch_active(const gm_batch *batch, int32_t *n);
+
 #ifdef __cplusplus
 }
 #endif

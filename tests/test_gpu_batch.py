@@ -1,0 +1,145 @@
+"""Batches of independent boxes (gm_batch_*): each box advanced in a batch
+equals, bit for bit, the same box advanced alone through the single-box
+path with the reference's host time loop -- for FP64 (hence equal to the
+restatement of the reference) and for the mixed form; a box's result does
+not depend on the other boxes in the batch, on the graph length or on the
+autotuned launch shape."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from guacho_amd import _capi, problems
+from guacho_amd.host import timestep_clamp
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def single(params, u0, nsteps=None, t_end=math.inf, cfl=0.4):
+    """The single-box path: host ramp/clamp; stop after nsteps or at the
+    dump clamp onto t_end.  Returns (u, steps, time)."""
+    with _capi.Subdomain(params) as dev:
+        dev.upload_u(u0)
+        t, it = 0.0, 1
+        while True:
+            dt, dump = timestep_clamp(it, 10, cfl, dev.cfl_dtp(), t, t_end)
+            dev.tstep(dt)
+            t = t + dt
+            if (nsteps is not None and it >= nsteps) or dump:
+                break
+            it += 1
+        return dev.download_u(), it, t
+
+
+def boxes(prec):
+    """A small uneven mix: blasts (outflow), OT (periodic), a random closed box."""
+    out = []
+    for n, seed in ((12, 1), (20, 2), (16, 3)):
+        u, dx, dy, dz = problems.blast(n, seed=seed)
+        out.append((_capi.make_params(n, n, n, dx, dy, dz, bc=(1,) * 6, precision=prec), u))
+    u, dx, dy, dz = problems.orszag_tang(24, 24, 4)
+    out.append((_capi.make_params(24, 24, 4, dx, dy, dz, precision=prec), u))
+    u, dx, dy, dz = problems.turbulence(16, seed=4)
+    out.append((_capi.make_params(16, 16, 16, dx, dy, dz, bc=(2, 2, 1, 1, 3, 3), precision=prec), u))
+    return out
+
+
+@pytest.mark.parametrize("prec", [_capi.GM_PRECISION_FP64, _capi.GM_PRECISION_MIXED], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("spl", [1, 4])
+def test_batch_equals_single_box_path(prec, spl):
+    say(f"case: batch of 5 uneven boxes, precision {prec}, steps_per_launch {spl}, 8 steps")
+    bx = boxes(prec)
+    with _capi.Batch(precision=prec, max_boxes=8, steps_per_launch=spl) as B:
+        ids = [B.add(p, u, max_steps=8) for p, u in bx]
+        B.advance(8 // spl)
+        B.sync()
+        got = [B.download_u(i) for i in ids]
+        clocks = [B.clock(i) for i in ids]
+    for (p, u), g, (t, n, done) in zip(bx, got, clocks):
+        ref, steps, tref = single(p, u, nsteps=8)
+        np.testing.assert_array_equal(g, ref)
+        assert (n, done) == (8, True) and t == tref
+
+
+def test_alone_equals_in_mix_mixed():
+    say("case: one blast box alone vs among four others (mixed), 10 steps")
+    bx = boxes(_capi.GM_PRECISION_MIXED)
+    p, u = bx[1]
+    with _capi.Batch(precision=_capi.GM_PRECISION_MIXED, max_boxes=8) as B:
+        i = B.add(p, u, max_steps=10)
+        B.advance(10)
+        B.sync()
+        alone = B.download_u(i)
+    with _capi.Batch(precision=_capi.GM_PRECISION_MIXED, max_boxes=8) as B:
+        ids = [B.add(q, v, max_steps=10 + k) for k, (q, v) in enumerate(bx)]
+        B.advance(10)
+        B.sync()
+        mixed = B.download_u(ids[1])
+    np.testing.assert_array_equal(alone, mixed)
+
+
+def test_t_end_completion_matches_host_loop():
+    say("case: OT 32x32x4 to t_end = 0.05 in a batch vs the host loop")
+    u, dx, dy, dz = problems.orszag_tang(32, 32, 4)
+    p = _capi.make_params(32, 32, 4, dx, dy, dz)
+    ref, steps, tref = single(p, u, t_end=0.05)
+    with _capi.Batch(precision=_capi.GM_PRECISION_FP64, max_boxes=4) as B:
+        i = B.add(p, u, t_end=0.05)
+        while True:
+            B.advance(1)
+            B.sync()
+            t, n, done = B.clock(i)
+            if done:
+                break
+            assert n < 10000
+        got = B.download_u(i)
+        # completed boxes stay put on further advances
+        B.advance(3)
+        B.sync()
+        assert B.clock(i) == (t, n, True)
+        np.testing.assert_array_equal(B.download_u(i), got)
+    assert (n, t) == (steps, tref) and t == 0.05
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_autotune_keeps_results_and_picks_a_shape():
+    say("case: autotuned batch (dt = 0 timing steps) vs single path, 6 steps")
+    bx = boxes(_capi.GM_PRECISION_FP64)
+    with _capi.Batch(precision=_capi.GM_PRECISION_FP64, max_boxes=8, autotune=True) as B:
+        ids = [B.add(p, u, max_steps=6) for p, u in bx]
+        B.advance(6)
+        B.sync()
+        block, kchunk, ms = B.tuning()
+        got = [B.download_u(i) for i in ids]
+    assert block in (64, 128, 256) and kchunk in (8, 16, 32) and ms > 0
+    for (p, u), g in zip(bx, got):
+        np.testing.assert_array_equal(g, single(p, u, nsteps=6)[0])
+
+
+def test_budget_and_removal():
+    say("case: memory budget refuses a box, removal frees room")
+    u, dx, dy, dz = problems.blast(16, seed=1)
+    p = _capi.make_params(16, 16, 16, dx, dy, dz, bc=(1,) * 6, precision=_capi.GM_PRECISION_MIXED)
+    with _capi.Batch(precision=_capi.GM_PRECISION_MIXED, max_boxes=4, memory_budget=1) as B0:
+        with pytest.raises(_capi.GmError) as ei:
+            B0.add(p, u, max_steps=1)
+        assert ei.value.code == _capi.GM_ERR_BUSY
+    with _capi.Batch(precision=_capi.GM_PRECISION_MIXED, max_boxes=4) as B:
+        need = B.box_bytes(p)
+    with _capi.Batch(precision=_capi.GM_PRECISION_MIXED, max_boxes=4, memory_budget=need + need // 2) as B:
+        a = B.add(p, u, max_steps=2)
+        with pytest.raises(_capi.GmError) as ei:
+            B.add(p, u, max_steps=2)
+        assert ei.value.code == _capi.GM_ERR_BUSY
+        B.remove(a)
+        b = B.add(p, u, max_steps=2)
+        B.advance(2)
+        B.sync()
+        assert B.clock(b)[2]
+        assert B.active() == 1
