@@ -35,6 +35,11 @@ def parse():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision section")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-boxes section")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving section")
+    ap.add_argument("--serving-requests", type=int, default=24)
+    ap.add_argument("--batch-boxes", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--mixed-n", type=int, default=512)
     ap.add_argument("--mixed-steps", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -184,6 +189,91 @@ def mixed_section(n: int, steps: int, seed: int):
     }
 
 
+def batch_section(nbox: int, steps: int, seed: int):
+    """64 independent MHD blasts of uneven cubic size n ~ U{32..128}, outflow,
+    HLLD, mixed precision, `steps` steps each: all boxes in one batch whose
+    steps are captured in one HIP graph launch, launch shape autotuned;
+    against the same boxes solved one at a time with the plain FP64
+    single-box path."""
+    import numpy as np
+    from guacho_amd import _capi, problems
+    from guacho_amd.host import timestep_clamp
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(32, 129, size=nbox)
+    log(f"batch section: {nbox} blasts, sizes {sizes.min()}..{sizes.max()}, {steps} steps")
+    boxes = []
+    for b, n in enumerate(sizes):
+        u, dx, dy, dz = problems.blast(int(n), seed=seed * 1000 + b)
+        boxes.append((int(n), u, dx))
+    zones = float(sum(n ** 3 for n, _, _ in boxes)) * steps
+    mp = _capi.GM_PRECISION_MIXED
+
+    def admit(B):
+        return [B.add(_capi.make_params(n, n, n, h, h, h, bc=(1,) * 6, precision=mp), u, max_steps=steps)
+                for n, u, h in boxes]
+
+    with _capi.Batch(precision=mp, max_boxes=nbox, steps_per_launch=steps, autotune=True) as B:
+        ids = admit(B)
+        B.advance(1)            # first use: autotune (dt = 0 steps), tables, graph capture, the run
+        B.sync()
+        block, kchunk, tuned_ms = B.tuning()
+        u_first = B.download_u(ids[0])
+        for i in ids:
+            B.remove(i)
+        ids = admit(B)          # same boxes again: same geometry, the captured graph is reused
+        B.sync()
+        t0 = time.perf_counter()
+        B.advance(1)            # all 64 boxes x `steps` steps in one graph launch
+        B.sync()
+        el = time.perf_counter() - t0
+        assert all(B.clock(i) == (B.clock(i)[0], steps, True) for i in ids)
+        same = bool(np.array_equal(B.download_u(ids[0]), u_first))
+    log("batch section: one box at a time, plain FP64")
+    t0 = time.perf_counter()
+    for n, u, h in boxes:
+        with _capi.Subdomain(_capi.make_params(n, n, n, h, h, h, bc=(1,) * 6)) as dev:
+            dev.upload_u(u)
+            t = 0.0
+            for it in range(1, steps + 1):
+                dt, _ = timestep_clamp(it, 10, 0.4, dev.cfl_dtp(), t, 1e30)
+                dev.tstep(dt)
+                t += dt
+            dev.sync()
+    seq = time.perf_counter() - t0
+    return {"workload": f"{nbox} MHD blasts n~U{{32..128}}^3, outflow, HLLD, mixed, {steps} steps, one graph launch",
+            "value": round(zones / el, 1), "unit": "zone-updates/s", "seconds": round(el, 4),
+            "zone_updates": zones, "tuned_shape": {"block": block, "kchunk": kchunk, "ms_per_step": round(tuned_ms, 4)},
+            "sequential_fp64_value": round(zones / seq, 1), "sequential_fp64_seconds": round(seq, 3),
+            "speedup_vs_sequential_fp64": round(seq / el, 3), "repeat_bitwise_equal": same}
+
+
+def serving_section(nreq: int, seed: int, fractions=(0.5, 1.0, 2.0)):
+    """Continuous batching of a Poisson stream of mixed solves (Brio-Wu
+    1024x4x4, Orszag-Tang 256^2, blasts 64^3-192^3) against serving the same
+    stream one request at a time with the plain FP64 path; arrival rate
+    swept as fractions of the sequential FP64 service rate."""
+    from guacho_amd import serving
+    log(f"serving section: {nreq} requests, sequential FP64 service rate")
+    base = serving.make_stream(nreq, rate=1e9, seed=seed)        # all present at t ~ 0
+    seq0 = serving.serve_sequential_fp64(base)
+    mu = nreq / seq0["makespan_s"]
+    sweep = []
+    for f in list(fractions) + [None]:
+        lam = f * mu if f is not None else 1e9          # None: saturated, every request present at t ~ 0
+        log(f"serving section: rate {'saturated' if f is None else f'{f:.2f} x sequential'} ({lam:.2f}/s)")
+        cb = serving.serve(serving.make_stream(nreq, rate=lam, seed=seed + 1), steps_per_launch=4)
+        sq = serving.serve_sequential_fp64(serving.make_stream(nreq, rate=lam, seed=seed + 1))
+        sweep.append({"rate_fraction": f if f is not None else "saturated", "rate_per_s": round(lam, 3),
+                      "batched_mixed": {k: round(v, 4) for k, v in cb.items()},
+                      "sequential_fp64": {k: round(v, 4) for k, v in sq.items()},
+                      "throughput_ratio": round(cb["zone_updates_per_s"] / sq["zone_updates_per_s"], 3)})
+    return {"workload": f"{nreq} Poisson requests: Brio-Wu 1024x4x4 to t=0.1, OT 256^2x4 to t=0.05, "
+                        f"blasts n^3 n~U{{64..192}} 40 steps; HLLD minmod flux-CD",
+            "sequential_fp64_service_rate_per_s": round(mu, 3),
+            "sequential_fp64_zone_updates_per_s": round(seq0["zone_updates_per_s"], 1),
+            "sweep": sweep}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -243,6 +333,10 @@ def main():
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_ach / HBM_PEAK_GBS, 4)},
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items() if v[1]},
     }
+    if not args.no_serving:
+        out["serving"] = serving_section(args.serving_requests, args.seed)
+    if not args.no_batch:
+        out["batch"] = batch_section(args.batch_boxes, args.batch_steps, args.seed)
     if not args.no_mixed:
         out["mixed"] = mixed_section(args.mixed_n, args.mixed_steps, args.seed)
     if not args.no_cpu_baseline:

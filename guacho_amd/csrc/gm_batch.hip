@@ -64,6 +64,12 @@ struct gm_batch {
   double tuned_ms = 0.0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // launch geometry the graph was captured with: its nodes read the box and
+  // prefix tables through stable device pointers, so new table contents with
+  // the same geometry reuse it
+  int cap_nbox = -1;
+  unsigned cap_tiles[gm::KC_COUNT] = {0};
+  gm::Shape cap_shape = {0, 0};
   std::string msg;
 };
 
@@ -215,7 +221,15 @@ int capture_graph(gm_batch* b) {
   b->graph = g;
   BHIP(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
   b->graph_dirty = false;
+  b->cap_nbox = (int)b->order.size();
+  std::memcpy(b->cap_tiles, b->tiles, sizeof b->tiles);
+  b->cap_shape = b->shape;
   return GM_OK;
+}
+
+bool graph_matches(const gm_batch* b) {
+  return b->exec && b->cap_nbox == (int)b->order.size() && b->cap_shape.block == b->shape.block &&
+         b->cap_shape.kchunk == b->shape.kchunk && std::memcmp(b->cap_tiles, b->tiles, sizeof b->tiles) == 0;
 }
 
 // Time candidate shapes on the admitted boxes; keep the fastest.
@@ -477,7 +491,7 @@ int gm_batch_advance(gm_batch* b, int32_t nlaunch) {
   }
   if (b->tables_dirty) {
     if ((rc = rebuild_tables(b, b->shape)) != GM_OK) return rc;
-    b->graph_dirty = true;
+    b->graph_dirty = !graph_matches(b);
   }
   if (b->graph_dirty) {
     if ((rc = capture_graph(b)) != GM_OK) return rc;
