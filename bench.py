@@ -78,20 +78,20 @@ def cpu_baseline(seconds: float):
                       f"turbulence {n}^3 FP64 HLLD minmod flux-CD, {steps} steps in {el:.2f} s"}
 
 
-# Compulsory bytes one launch of each kernel class moves (FP64), per the
-# step's decomposition: N interior cells, faces ~ (n+1) n^2 per axis launch.
+# Compulsory bytes one launch of each kernel class moves (FP64, flux-CD),
+# per interior cell: every array the kernel must read or write, once.
 def algorithmic_bytes(cls: str, n: int) -> float:
     N = float(n) ** 3
     return {
-        # read the 8 conserved values of the cells the sweeps read, write 8 primitives
-        "prim": 128.0 * (n + 4) ** 2 * (n + 4) * (1.0 - 12.0 / (n + 4) ** 2),
-        # read 8 primitives per cell once, write 8 flux components per face
-        "flux_predictor": 128.0 * (n + 1) * n * n,
-        "flux_corrector": 128.0 * (n + 1) * n * n,
-        # read the 6 field fluxes, write 3 E components
-        "efield": 72.0 * N,
-        # read U (8) + 5 hydro fluxes x 3 axes + 3 E, write U (8)
-        "update": 272.0 * N,
+        # read U^n (8); write the x/y partials (5 hydro + 2 E) and E3
+        "xy_predictor": 128.0 * N,
+        # read U* (8) and the hydro part of U^n (5); write partials (7) and E3
+        "xy_corrector": 168.0 * N,
+        # read U (8) and the partials (7); write the hydro update (5), E1, E2
+        "z_predictor": 176.0 * N,
+        "z_corrector": 176.0 * N,
+        # read B (3) and E (3); write B (3)
+        "bfield": 72.0 * N,
         # read U (8)
         "cfl_dt": 64.0 * N,
     }.get(cls, 0.0)

@@ -31,8 +31,7 @@ struct Slot {
   gm_params p{};
   void* U = nullptr;
   void* Us = nullptr;
-  void* P = nullptr;
-  void* Fb = nullptr;
+  void* W = nullptr;  // stage partials, 8 doubles per cell; also upload/download staging
   void* E = nullptr;
   int64_t bytes = 0;
   gm::BoxClock clock{};  // host copy as of the last sync
@@ -96,7 +95,7 @@ int bhip(gm_batch* b, hipError_t e, const char* what) {
 
 int64_t box_bytes_of(const gm_params& p, size_t esize) {
   const int64_t ncell = (int64_t)(p.nx + 4) * (p.ny + 4) * (p.nz + 4);
-  return ncell * (8 + 8 + 8 + 24 + 3) * (int64_t)esize;
+  return ncell * ((8 + 8 + 3) * (int64_t)esize + 8 * (int64_t)sizeof(double));
 }
 
 gm::Geom geom_of(const gm_params& p) {
@@ -119,8 +118,7 @@ gm::Box<S> box_view(const Slot& s, int slot) {
   b.ph.limiter = s.p.slope_limiter;
   b.U = (S*)s.U;
   b.Us = (S*)s.Us;
-  b.P = (S*)s.P;
-  b.F = (S*)s.Fb;
+  b.W = (double*)s.W;
   b.E = (S*)s.E;
   b.dx = s.p.dx;
   b.dy = s.p.dy;
@@ -130,7 +128,7 @@ gm::Box<S> box_view(const Slot& s, int slot) {
 }
 
 void free_slot(Slot& s) {
-  void* bufs[] = {s.U, s.Us, s.P, s.Fb, s.E};
+  void* bufs[] = {s.U, s.Us, s.W, s.E};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   s = Slot();
@@ -189,18 +187,15 @@ void enqueue_steps_t(gm_batch* b, int nsteps, int tuning) {
   hipStream_t s = b->stream;
   const gm::Shape& sh = b->shape;
   const int solver = b->bp.riemann_solver, fcd = b->bp.enable_flux_cd;
+  const unsigned txy = b->tiles[gm::KC_XY], tcol = b->tiles[gm::KC_COL];
   for (int st = 0; st < nsteps; st++) {
     gm::launch_clock_begin(s, b->dclk, cap);
-    gm::launch_cfl_batch(s, boxes, pre(gm::KC_CFLRED), n, b->tiles[gm::KC_CFLRED], b->dclk, b->derr, sh);
+    gm::launch_cfl_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, b->derr, sh);
     gm::launch_clock_dt(s, b->dclk, cap, tuning);
     for (int stage = 1; stage <= 2; stage++) {
-      gm::launch_prim_batch(s, boxes, pre(stage == 1 ? gm::KC_PRIM1 : gm::KC_PRIM2), n,
-                            b->tiles[stage == 1 ? gm::KC_PRIM1 : gm::KC_PRIM2], stage, sh);
-      for (int a = 0; a < 3; a++)
-        gm::launch_flux_batch(s, boxes, pre(gm::KC_FLUXX + a), n, b->tiles[gm::KC_FLUXX + a], solver, stage, a,
-                              b->derr, sh);
-      if (fcd) gm::launch_efield_batch(s, boxes, pre(gm::KC_EFIELD), n, b->tiles[gm::KC_EFIELD], sh);
-      gm::launch_update_batch(s, boxes, pre(gm::KC_UPDATE), n, b->tiles[gm::KC_UPDATE], b->dclk, stage, fcd, sh);
+      gm::launch_xy_batch(s, boxes, pre(gm::KC_XY), n, txy, b->dclk, stage, solver, fcd, b->derr);
+      gm::launch_z_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, stage, solver, fcd, b->derr, sh);
+      if (fcd) gm::launch_bfield_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, stage, sh);
     }
   }
 }
@@ -423,7 +418,8 @@ int gm_batch_add(gm_batch* b, const gm_params* p, const double* u, double cfl, d
   s.p.device = b->bp.device;
   const gm::Geom g = geom_of(*p);
   const size_t n8 = (size_t)g.ncell * 8 * b->esize, n3 = (size_t)g.ncell * 3 * b->esize;
-  struct { void** ptr; size_t n; } al[] = {{&s.U, n8}, {&s.Us, n8}, {&s.P, n8}, {&s.Fb, 3 * n8}, {&s.E, n3}};
+  const size_t nw = (size_t)g.ncell * 8 * sizeof(double);
+  struct { void** ptr; size_t n; } al[] = {{&s.U, n8}, {&s.Us, n8}, {&s.W, nw}, {&s.E, n3}};
   for (auto& a : al) {
     if (hipMalloc(a.ptr, a.n) != hipSuccess) {
       (void)hipGetLastError();
@@ -433,10 +429,10 @@ int gm_batch_add(gm_batch* b, const gm_params* p, const double* u, double cfl, d
     (void)hipMemsetAsync(*a.ptr, 0, a.n, b->stream);
   }
   s.bytes = need;
-  // upload: host AoS doubles -> staging (flux block) -> SoA state
-  BHIP(b, hipMemcpyAsync(s.Fb, u, (size_t)g.ncell * 8 * sizeof(double), hipMemcpyHostToDevice, b->stream));
-  if (b->esize == 8) gm::launch_aos_to_soa(b->stream, g.ncell, (const double*)s.Fb, (double*)s.U);
-  else gm::launch_aos_to_soa(b->stream, g.ncell, (const double*)s.Fb, (float*)s.U);
+  // upload: host AoS doubles -> staging (partials block) -> SoA state
+  BHIP(b, hipMemcpyAsync(s.W, u, (size_t)g.ncell * 8 * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  if (b->esize == 8) gm::launch_aos_to_soa(b->stream, g.ncell, (const double*)s.W, (double*)s.U);
+  else gm::launch_aos_to_soa(b->stream, g.ncell, (const double*)s.W, (float*)s.U);
   BHIP(b, hipGetLastError());
   gm::BoxClock c{};
   c.dtmin = 0;
@@ -526,13 +522,13 @@ int gm_batch_download_u(gm_batch* b, int32_t id, double* u) {
   const gm::Geom g = geom_of(s.p);
   if (b->esize == 8) {
     ghost_fill_t<double>(b, s);
-    gm::launch_soa_to_aos(b->stream, g.ncell, (const double*)s.U, (double*)s.Fb);
+    gm::launch_soa_to_aos(b->stream, g.ncell, (const double*)s.U, (double*)s.W);
   } else {
     ghost_fill_t<float>(b, s);
-    gm::launch_soa_to_aos(b->stream, g.ncell, (const float*)s.U, (double*)s.Fb);
+    gm::launch_soa_to_aos(b->stream, g.ncell, (const float*)s.U, (double*)s.W);
   }
   BHIP(b, hipGetLastError());
-  BHIP(b, hipMemcpyAsync(u, s.Fb, (size_t)g.ncell * 8 * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  BHIP(b, hipMemcpyAsync(u, s.W, (size_t)g.ncell * 8 * sizeof(double), hipMemcpyDeviceToHost, b->stream));
   BHIP(b, hipStreamSynchronize(b->stream));
   return GM_OK;
 }

@@ -2,8 +2,9 @@
 //
 // A context owns one subdomain's device state and one HIP stream.  The
 // step (gm_tstep) follows tstep (src/hydro_solver.f90:134-229):
-//   predictor: primitives of U^n -> first-order fluxes -> E -> U* = U^n - dt/2 div F
-//   corrector: primitives of U*  -> limited fluxes     -> E -> U^{n+1} = U^n - dt div F
+//   predictor: first-order fluxes of U^n, U* = U^n - dt/2 div F
+//   corrector: limited fluxes of U*,      U^{n+1} = U^n - dt div F
+// each stage as three kernels (xy faces, z faces, flux-CD field update)
 // with ghosts resolved by boundary mapping inside the kernels, so neither
 // boundaryII nor boundaryI runs on the step path; gm_download_u and
 // gm_boundary_u materialise boundaryI's ghost layer when a caller needs it.
@@ -19,9 +20,9 @@
 
 namespace {
 
-const char* kClassNames[GM_NKCLASS] = {"prim", "flux_predictor", "flux_corrector", "efield",
-                                       "update", "cfl_dt", "ghost", "layout"};
-enum { KC_PRIM, KC_FLUX1, KC_FLUX2, KC_EFIELD, KC_UPDATE, KC_CFL, KC_GHOST, KC_LAYOUT };
+const char* kClassNames[GM_NKCLASS] = {"xy_predictor", "xy_corrector", "z_predictor", "z_corrector",
+                                       "bfield", "cfl_dt", "ghost", "layout"};
+enum { KC_XY1, KC_XY2, KC_Z1, KC_Z2, KC_BFIELD, KC_CFL, KC_GHOST, KC_LAYOUT };
 
 thread_local std::string g_create_error;
 
@@ -49,9 +50,8 @@ struct gm_ctx {
   size_t esize = 8;      // bytes per stored value: 8 (FP64) or 4 (mixed, FP32 storage)
   void* U = nullptr;     // state U^n (SoA, 8 vars, ghosts as uploaded / filled)
   void* Us = nullptr;    // U* after the predictor
-  void* P = nullptr;     // primitives
-  void* Fb = nullptr;    // fluxes of the three axes (3 x 8 vars); also the staging
-                         // buffer for layout conversion (>= 8 doubles per cell)
+  void* W = nullptr;     // stage partials (8 doubles per cell); also the staging
+                         // buffer for layout conversion
   void* E = nullptr;     // cell-centred E (3 vars); also staging for Temp / divB
   Scalars* dscal = nullptr;  // device scalars
   Scalars* hscal = nullptr;  // pinned host mirror
@@ -182,7 +182,7 @@ void release(gm_ctx* c) {
     (void)hipEventDestroy(r.b);
   }
   for (auto e : c->pool) (void)hipEventDestroy(e);
-  void* bufs[] = {c->U, c->Us, c->P, c->Fb, c->E};
+  void* bufs[] = {c->U, c->Us, c->W, c->E};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->dscal) (void)hipFree(c->dscal);
@@ -225,8 +225,7 @@ gm::Box<S> box_of(const gm_ctx* c) {
   b.ph = c->ph;
   b.U = (S*)c->U;
   b.Us = (S*)c->Us;
-  b.P = (S*)c->P;
-  b.F = (S*)c->Fb;
+  b.W = (double*)c->W;
   b.E = (S*)c->E;
   b.dx = c->p.dx;
   b.dy = c->p.dy;
@@ -243,20 +242,17 @@ void tstep_t(gm_ctx* c, double dt) {
   int* err = &c->dscal->err;
   const gm::Box<S> b = box_of<S>(c);
   const gm::Shape& sh = c->shape;
+  const int solver = p.riemann_solver, fcd = p.enable_flux_cd ? 1 : 0;
   const double dtm = dt / 2.0;  // hydro_solver.f90:152
   // predictor: first-order fluxes of U^n, U* = U^n - dt/2 div F
-  timed(c, KC_PRIM, [&] { gm::launch_prim(s, b, 1, sh); });
-  for (int a = 0; a < 3; a++)
-    timed(c, KC_FLUX1, [&] { gm::launch_flux(s, b, p.riemann_solver, 1, a, err, sh); });
-  if (p.enable_flux_cd) timed(c, KC_EFIELD, [&] { gm::launch_efield(s, b, sh); });
-  timed(c, KC_UPDATE, [&] { gm::launch_update(s, b, dtm, 1, p.enable_flux_cd, sh); });
+  timed(c, KC_XY1, [&] { gm::launch_xy(s, b, dtm, 1, solver, fcd, err); });
+  timed(c, KC_Z1, [&] { gm::launch_z(s, b, dtm, 1, solver, fcd, err, sh); });
+  if (fcd) timed(c, KC_BFIELD, [&] { gm::launch_bfield(s, b, dtm, 1, sh); });
   // corrector: limited fluxes of U*, U^{n+1} = U^n - dt div F (in place;
   // viscous_copy with eta = 0 is the identity on values)
-  timed(c, KC_PRIM, [&] { gm::launch_prim(s, b, 2, sh); });
-  for (int a = 0; a < 3; a++)
-    timed(c, KC_FLUX2, [&] { gm::launch_flux(s, b, p.riemann_solver, 2, a, err, sh); });
-  if (p.enable_flux_cd) timed(c, KC_EFIELD, [&] { gm::launch_efield(s, b, sh); });
-  timed(c, KC_UPDATE, [&] { gm::launch_update(s, b, dt, 2, p.enable_flux_cd, sh); });
+  timed(c, KC_XY2, [&] { gm::launch_xy(s, b, dt, 2, solver, fcd, err); });
+  timed(c, KC_Z2, [&] { gm::launch_z(s, b, dt, 2, solver, fcd, err, sh); });
+  if (fcd) timed(c, KC_BFIELD, [&] { gm::launch_bfield(s, b, dt, 2, sh); });
 }
 
 template <typename S>
@@ -312,7 +308,8 @@ int gm_create(const gm_params* params, gm_ctx** out) {
   c->esize = params->precision == GM_PRECISION_MIXED ? 4 : 8;
   const size_t n8 = (size_t)g.ncell * 8 * c->esize;
   const size_t n3 = (size_t)g.ncell * 3 * c->esize;
-  struct { void** p; size_t n; } allocs[] = {{&c->U, n8}, {&c->Us, n8}, {&c->P, n8}, {&c->Fb, 3 * n8}, {&c->E, n3}};
+  const size_t nw = (size_t)g.ncell * 8 * sizeof(double);
+  struct { void** p; size_t n; } allocs[] = {{&c->U, n8}, {&c->Us, n8}, {&c->W, nw}, {&c->E, n3}};
   for (auto& a : allocs) {
     if (hipMalloc(a.p, a.n) != hipSuccess) {
       (void)hipGetLastError();
@@ -348,10 +345,10 @@ int gm_upload_u(gm_ctx* c, const double* u) {
   int rc = set_device(c);
   if (rc) return rc;
   const size_t n8 = (size_t)c->g.ncell * 8 * sizeof(double);
-  GM_HIP(c, hipMemcpyAsync(c->Fb, u, n8, hipMemcpyHostToDevice, c->stream));
+  GM_HIP(c, hipMemcpyAsync(c->W, u, n8, hipMemcpyHostToDevice, c->stream));
   timed(c, KC_LAYOUT, [&] {
-    if (c->esize == 8) gm::launch_aos_to_soa(c->stream, c->g.ncell, (const double*)c->Fb, (double*)c->U);
-    else gm::launch_aos_to_soa(c->stream, c->g.ncell, (const double*)c->Fb, (float*)c->U);
+    if (c->esize == 8) gm::launch_aos_to_soa(c->stream, c->g.ncell, (const double*)c->W, (double*)c->U);
+    else gm::launch_aos_to_soa(c->stream, c->g.ncell, (const double*)c->W, (float*)c->U);
   });
   GM_HIP(c, hipGetLastError());
   GM_HIP(c, hipStreamSynchronize(c->stream));
@@ -375,11 +372,11 @@ int gm_download_u(gm_ctx* c, double* u) {
   if (rc) return rc;
   if (c->ghosts_dirty) fill_ghost1(c);
   timed(c, KC_LAYOUT, [&] {
-    if (c->esize == 8) gm::launch_soa_to_aos(c->stream, c->g.ncell, (const double*)c->U, (double*)c->Fb);
-    else gm::launch_soa_to_aos(c->stream, c->g.ncell, (const float*)c->U, (double*)c->Fb);
+    if (c->esize == 8) gm::launch_soa_to_aos(c->stream, c->g.ncell, (const double*)c->U, (double*)c->W);
+    else gm::launch_soa_to_aos(c->stream, c->g.ncell, (const float*)c->U, (double*)c->W);
   });
   GM_HIP(c, hipGetLastError());
-  GM_HIP(c, hipMemcpyAsync(u, c->Fb, (size_t)c->g.ncell * 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  GM_HIP(c, hipMemcpyAsync(u, c->W, (size_t)c->g.ncell * 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   GM_HIP(c, hipStreamSynchronize(c->stream));
   return GM_OK;
 }
@@ -391,12 +388,12 @@ int gm_download_primit(gm_ctx* c, double* primit, double* temp) {
   if (rc) return rc;
   if (c->ghosts_dirty) fill_ghost1(c);
   double* T = temp ? (double*)c->E : nullptr;
-  timed(c, KC_PRIM, [&] {
-    if (c->esize == 8) gm::launch_prim_all(c->stream, c->g.ncell, c->ph, (const double*)c->U, (double*)c->Fb, T);
-    else gm::launch_prim_all(c->stream, c->g.ncell, c->ph, (const float*)c->U, (double*)c->Fb, T);
+  timed(c, KC_LAYOUT, [&] {
+    if (c->esize == 8) gm::launch_prim_all(c->stream, c->g.ncell, c->ph, (const double*)c->U, (double*)c->W, T);
+    else gm::launch_prim_all(c->stream, c->g.ncell, c->ph, (const float*)c->U, (double*)c->W, T);
   });
   GM_HIP(c, hipGetLastError());
-  GM_HIP(c, hipMemcpyAsync(primit, c->Fb, (size_t)c->g.ncell * 8 * sizeof(double), hipMemcpyDeviceToHost,
+  GM_HIP(c, hipMemcpyAsync(primit, c->W, (size_t)c->g.ncell * 8 * sizeof(double), hipMemcpyDeviceToHost,
                            c->stream));
   if (temp)
     GM_HIP(c, hipMemcpyAsync(temp, c->E, (size_t)c->g.ncell * sizeof(double), hipMemcpyDeviceToHost, c->stream));

@@ -96,108 +96,303 @@ __device__ __forceinline__ void load_cons(const Geom& g, const S* __restrict__ U
 // order) or a batch of boxes (grid = all tiles of all boxes, each block
 // finding its box in a prefix table).  Stage 1 is the predictor (reads U^n,
 // writes U*), stage 2 the corrector (reads U*, updates U^n in place).
+//
+// One stage of tstep (src/hydro_solver.f90:134-229) is three kernels:
+//   xy:     primitives of a 64 x 8 plane tile and its halo into LDS
+//           (calcprim, src/hydro_core.f90:245-320), the x and y interface
+//           fluxes (e.g. hlldfluxes, src/hlld.f90:331-432), the x and y terms
+//           of the update, E3 and the x/y parts of E1, E2;
+//   z:      per column, a march up a z-pencil: primitives, the z interface
+//           fluxes, the z term of the update and E1, E2
+//           (get_efield, src/flux_cd_module.f90:245-273);
+//   bfield: the flux-CD field update (flux_cd_update,
+//           src/flux_cd_module.f90:285-323) from the cell-centred E.
+// Neither primitives nor fluxes reach HBM.  Every sum keeps the reference's
+// evaluation order (step, src/hydro_solver.f90:77-127: the x, y, z terms
+// subtracted from U in that order; get_efield's four-term sums), with the
+// partial sums carried between kernels in FP64, so the split is exact.
 // ---------------------------------------------------------------------------
 
-// primitives: calcprim over the cells the flux stencils read
-// (src/hydro_core.f90:245-320), ghost values by boundary mapping; FP64
-// pressure recovery for either storage type.
-template <typename S>
-__device__ __forceinline__ void prim_body(const Box<S>& b, int stage, unsigned lb) {
-  const Geom& g = b.g;
-  const int lo = stage == 1 ? 0 : -1;  // one ghost layer for first order, two for the limiter
-  const int ex = g.nx + 2 - 2 * lo, ey = g.ny + 2 - 2 * lo;
-  int t, kk;
-  if (!tile_item(lb, tiles_of((unsigned)(ex * ey)), (unsigned)(ex * ey), t, kk)) return;
-  const int i = t % ex + lo, j = t / ex + lo, k = kk + lo;
-  // only face ghosts are read downstream; skip edges and corners
-  const int out = (i < 1 || i > g.nx) + (j < 1 || j > g.ny) + (k < 1 || k > g.nz);
-  if (out > 1) return;
-  double u[8], p[8];
-  load_cons(g, stage == 1 ? b.U : b.Us, i, j, k, u);
-  u2prim(b.ph, u, p);
-  const long long c = cidx(g, i, j, k);
-#pragma unroll
-  for (int q = 0; q < 8; q++) b.P[q * g.ncell + c] = (S)p[q];
-}
+// Plane tile of the xy kernel: TX x TY interior cells of one z-plane, one
+// wavefront per row.
+constexpr int TX = 64, TY = 8, XY_THREADS = TX * TY;
 
-// interface fluxes: the sweep of hlldfluxes / hllefluxes / hllfluxes /
-// hllcfluxes (e.g. src/hlld.f90:331-432) for one axis.  Face f(:,i,j,k)
-// lies between cells i and i+1 (i = 0..n along the axis); only faces with
-// interior transverse indices feed the update.
-template <int SOLVER, int ORDER, int axis, typename S>
-__device__ __forceinline__ void flux_body(const Box<S>& b, unsigned lb, int* __restrict__ err) {
-  const Geom& g = b.g;
-  const int e0 = axis == 0 ? g.nx + 1 : g.nx;
-  const int e1 = axis == 1 ? g.ny + 1 : g.ny;
-  int t, k;
-  if (!tile_item(lb, tiles_of((unsigned)(e0 * e1)), (unsigned)(e0 * e1), t, k)) return;
-  int i = t % e0, j = t / e0;
-  if (axis != 0) i += 1;
-  if (axis != 1) j += 1;
-  if (axis != 2) k += 1;
-  const long long stride = axis == 0 ? 1 : axis == 1 ? (long long)g.NX : (long long)g.NX * g.NY;
-  const long long c = cidx(g, i, j, k);
-  const S* __restrict__ P = b.P;
-  S* __restrict__ F = b.F + axis * 8 * g.ncell;
-  S pl[8], pr[8], ff[8];
+// LDS of the xy kernel for reconstruction order ORDER (halo H = ORDER):
+// primitives [8][RY][RX] of the tile and its halo (corners unused), then
+// the fluxes of the tile's closing faces -- y-face TY of each column
+// [8][TX] and x-face TX of each row [8][TY].  After the face solves, the
+// y-face fluxes [8][TY][TX] reuse the primitive area.
+template <int ORDER>
+struct XYLds {
+  static constexpr int H = ORDER;
+  static constexpr int RX = TX + 2 * H, RY = TY + 2 * H;
+  static constexpr int PS = RX * RY;  // component stride of the primitives
+  static constexpr int PRIM = 8 * PS;
+  static constexpr int TOPY = 8 * TX;
+  static constexpr int XEND = 8 * TY;
+  static constexpr int SIZE = PRIM + TOPY + XEND;
+  static_assert(8 * TY * TX <= PRIM, "y-face fluxes must fit the primitive area");
+};
+
+// Flux of one face from the primitives in LDS: `at` indexes (component 0)
+// the cell on the face's high side, `stride` steps across the face.  The
+// reconstruction is limiter() (src/hydro_core.f90:712-731) on the four-cell
+// stencil; y faces are permuted into the face-normal frame and back.
+template <int SOLVER, int ORDER, int AXIS, typename S>
+__device__ __forceinline__ bool lds_face_flux(const Phys& ph, const S* __restrict__ lp, int at, int stride,
+                                              S ff[8]) {
+  constexpr int PS = XYLds<ORDER>::PS;
+  S pl[8], pr[8];
 #pragma unroll
   for (int q = 0; q < 8; q++) {
-    pl[q] = P[q * g.ncell + c];
-    pr[q] = P[q * g.ncell + c + stride];
+    pl[q] = lp[q * PS + at - stride];
+    pr[q] = lp[q * PS + at];
   }
   if constexpr (ORDER == 2) {
     S pll[8], prr[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-      pll[q] = P[q * g.ncell + c - stride];
-      prr[q] = P[q * g.ncell + c + 2 * stride];
+      pll[q] = lp[q * PS + at - 2 * stride];
+      prr[q] = lp[q * PS + at + stride];
     }
-    limiter(b.ph.limiter, pll, pl, pr, prr);
+    limiter(ph.limiter, pll, pl, pr, prr);
   }
-  axis_perm<axis>(pl);
-  axis_perm<axis>(pr);
-  const bool ok = riemann<SOLVER>(b.ph, pl, pr, ff);
-  if (!ok) atomicOr(err, 1);
-  axis_perm<axis>(ff);
-#pragma unroll
-  for (int q = 0; q < 8; q++) F[q * g.ncell + c] = ff[q];
+  axis_perm<AXIS>(pl);
+  axis_perm<AXIS>(pr);
+  const bool ok = riemann<SOLVER>(ph, pl, pr, ff);
+  axis_perm<AXIS>(ff);
+  return ok;
 }
 
-// cell-centred electric field, get_efield (src/flux_cd_module.f90:245-273)
-template <typename S>
-__device__ __forceinline__ void efield_body(const Box<S>& b, unsigned lb) {
+template <typename T>
+__device__ __forceinline__ T lane_down1(T v) {
+  return __shfl_down(v, 1, 64);
+}
+
+// xy kernel.  Block of 512 threads: wave w, lane l.  Lane l of wave w
+// solves x-face l of row w (between tile cells l-1 and l) and y-face w of
+// column l (between rows w-1 and w) -- the low faces of cell (l, w); its
+// high x face is lane l+1's (x-face TX from LDS for the last lane), its
+// high y face wave w+1's (y-face TY from LDS for the last row).
+template <int SOLVER, int ORDER, int FLUXCD, typename S>
+__device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt, int* __restrict__ err,
+                                        S* __restrict__ lds) {
+  using L = XYLds<ORDER>;
+  constexpr int H = L::H, RX = L::RX, PS = L::PS;
+  constexpr int NH = FLUXCD ? 5 : 8;
   const Geom& g = b.g;
-  int t, k;
-  if (!tile_item(lb, tiles_of((unsigned)(g.nx * g.ny)), (unsigned)(g.nx * g.ny), t, k)) return;
-  const int i = t % g.nx + 1, j = t / g.nx + 1;
-  k += 1;
-  const long long sx = 1, sy = g.NX, sz = (long long)g.NX * g.NY;
+  const int ntx = (g.nx + TX - 1) / TX, nty = (g.ny + TY - 1) / TY;
+  const unsigned per_plane = (unsigned)(ntx * nty);
+  const int k = (int)(lb / per_plane) + 1;
+  const int rem = (int)(lb - (unsigned)(k - 1) * per_plane);
+  const int i0 = (rem % ntx) * TX + 1, j0 = (rem / ntx) * TY + 1;
+  const int wx = min(TX, g.nx - i0 + 1), wy = min(TY, g.ny - j0 + 1);
+  const int tid = (int)threadIdx.x, w = tid >> 6, l = tid & 63;
+  const S* __restrict__ Usrc = ORDER == 1 ? b.U : b.Us;
+  S* __restrict__ lp = lds;
+  S* __restrict__ ltop = lds + L::PRIM;
+  S* __restrict__ lend = ltop + L::TOPY;
+
+  // primitives of the tile, its x halo on the tile's rows and its y halo
+  // on the tile's columns; ghosts by boundary mapping
+  for (int idx = tid; idx < PS; idx += XY_THREADS) {
+    const int rx = idx % RX, ry = idx / RX;
+    if (rx >= wx + 2 * H || ry >= wy + 2 * H) continue;
+    const bool inx = rx >= H && rx < H + wx, iny = ry >= H && ry < H + wy;
+    if (!inx && !iny) continue;
+    double u[8], p[8];
+    load_cons(g, Usrc, i0 - H + rx, j0 - H + ry, k, u);
+    u2prim(b.ph, u, p);
+#pragma unroll
+    for (int q = 0; q < 8; q++) lp[q * PS + idx] = (S)p[q];
+  }
+  __syncthreads();
+
+  bool ok = true;
+  // closing faces of a full tile, straight to LDS outside the primitive area
+  if (w == 1 && wx == TX && l < wy) {
+    S ff[8];
+    ok &= lds_face_flux<SOLVER, ORDER, 0>(b.ph, lp, (l + H) * RX + TX + H, 1, ff);
+#pragma unroll
+    for (int q = 0; q < 8; q++) lend[q * TY + l] = ff[q];
+  }
+  if (w == 0 && wy == TY && l < wx) {
+    S ff[8];
+    ok &= lds_face_flux<SOLVER, ORDER, 1>(b.ph, lp, (TY + H) * RX + l + H, RX, ff);
+#pragma unroll
+    for (int q = 0; q < 8; q++) ltop[q * TX + l] = ff[q];
+  }
+  // the low faces of cell (l, w); a partial tile's closing faces fall in here
+  S fx[8], fy[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) fx[q] = fy[q] = S(0);
+  const bool vx = w < wy && l <= wx;
+  const bool vy = l < wx && w <= wy;
+  if (vx) ok &= lds_face_flux<SOLVER, ORDER, 0>(b.ph, lp, (w + H) * RX + l + H, 1, fx);
+  if (vy) ok &= lds_face_flux<SOLVER, ORDER, 1>(b.ph, lp, (w + H) * RX + l + H, RX, fy);
+  if (!ok) atomicOr(err, 1);
+  __syncthreads();  // every read of the primitives is done
+  S* __restrict__ lfy = lds;
+  if (vy) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) lfy[(q * TY + w) * TX + l] = fy[q];
+  }
+  __syncthreads();
+
+  S fxr[8], fyu[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const S v = lane_down1(fx[q]);
+    fxr[q] = l == TX - 1 ? lend[q * TY + w] : v;
+  }
+  if (l >= wx || w >= wy) return;
+#pragma unroll
+  for (int q = 0; q < 8; q++) fyu[q] = w == TY - 1 ? ltop[q * TX + l] : lfy[(q * TY + w + 1) * TX + l];
   const long long N = g.ncell;
-  const long long c = cidx(g, i, j, k);
-  const S* __restrict__ Fx = b.F;
-  const S* __restrict__ Fy = b.F + 8 * N;
-  const S* __restrict__ Fz = b.F + 16 * N;
-  S* __restrict__ E = b.E;
-  auto ld = [](const S* a, long long o) { return (double)a[o]; };
-  E[0 * N + c] = (S)(0.25 * (((-ld(Fy, 7 * N + c - sy) - ld(Fy, 7 * N + c)) + ld(Fz, 6 * N + c - sz)) + ld(Fz, 6 * N + c)));
-  E[1 * N + c] = (S)(0.25 * (((+ld(Fx, 7 * N + c - sx) + ld(Fx, 7 * N + c)) - ld(Fz, 5 * N + c - sz)) - ld(Fz, 5 * N + c)));
-  E[2 * N + c] = (S)(0.25 * (((-ld(Fx, 6 * N + c - sx) - ld(Fx, 6 * N + c)) + ld(Fy, 5 * N + c - sy)) + ld(Fy, 5 * N + c)));
+  const long long c = cidx(g, i0 + l, j0 + w, k);
+  const double dtdx = dt / b.dx, dtdy = dt / b.dy;
+  double* __restrict__ W = b.W;
+#pragma unroll
+  for (int q = 0; q < NH; q++)
+    W[q * N + c] = ((double)b.U[q * N + c] - dtdx * ((double)fxr[q] - (double)fx[q]))
+                   - dtdy * ((double)fyu[q] - (double)fy[q]);
+  if constexpr (FLUXCD) {
+    W[5 * N + c] = (-(double)fy[7]) - (double)fyu[7];
+    W[6 * N + c] = (+(double)fx[7]) + (double)fxr[7];
+    b.E[2 * N + c] = (S)(0.25 * ((((-(double)fx[6]) - (double)fxr[6]) + (double)fy[5]) + (double)fyu[5]));
+  }
+}
+
+// z kernel.  A thread marches up the pencil k0..k1 of column (i, j),
+// solving z-faces k0-1 .. k1 (face k between cells k and k+1).  Primitives
+// enter a sliding window one cell ahead of the faces; the raw state of the
+// next cell is fetched an iteration early.  The second-order window holds
+// P(k), P(k+1) and the upper-face state of cell k: limiter()'s
+// reconstruction of a cell depends on the cell and its two neighbours
+// only, so each cell's slope is formed once and serves both its faces
+// (pl + al/2 for the face above, pr - ar/2 for the face below).
+template <int SOLVER, int ORDER, int FLUXCD, typename S>
+__device__ __forceinline__ void z_body(const Box<S>& b, unsigned lb, double dt, int kchunk, int* __restrict__ err) {
+  constexpr int NH = FLUXCD ? 5 : 8;
+  const Geom& g = b.g;
+  int t, kc;
+  if (!tile_item(lb, tiles_of((unsigned)(g.nx * g.ny)), (unsigned)(g.nx * g.ny), t, kc)) return;
+  const int i = t % g.nx + 1, j = t / g.nx + 1;
+  const int k0 = kc * kchunk + 1;
+  const int k1 = min(k0 + kchunk - 1, g.nz);
+  const int klast = k1 + ORDER;  // last cell the window takes in
+  const long long N = g.ncell;
+  const long long col = cidx(g, i, j, -1);  // cell k of the column at col + (k+1)*sz
+  const long long sz = (long long)g.NX * g.NY;
+  const double dtdz = dt / b.dz;
+  const S* __restrict__ Usrc = ORDER == 1 ? b.U : b.Us;
+  S* __restrict__ Uout = ORDER == 1 ? b.Us : b.U;
+  const double* __restrict__ W = b.W;
+  const int lim = b.ph.limiter;
+
+  S ur[8];
+  bool uflip = false;
+  auto fetch = [&](int k) {
+    bool f = false;
+    const int sk = map_ghost(k, g.nz, g.bc[4], g.bc[5], f);
+    const long long cc = col + (long long)(sk + 1) * sz;
+#pragma unroll
+    for (int q = 0; q < 8; q++) ur[q] = Usrc[q * N + cc];
+    uflip = f;
+  };
+  auto take = [&](S p[8]) {
+    double u[8], pd[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) u[q] = (double)ur[q];
+    if (uflip) u[3] = -u[3];
+    u2prim(b.ph, u, pd);
+#pragma unroll
+    for (int q = 0; q < 8; q++) p[q] = (S)pd[q];
+  };
+
+  S pm[8], pc[8], pu[8];  // P(kf), P(kf+1), upper-face state of cell kf
+  if constexpr (ORDER == 2) {
+    S p0[8];
+    fetch(k0 - 2);
+    take(p0);
+    fetch(k0 - 1);
+    take(pm);
+    fetch(k0);
+    take(pc);
+#pragma unroll
+    for (int q = 0; q < 8; q++) pu[q] = pm[q] + lim_average(lim, pm[q] - p0[q], pc[q] - pm[q]) * S(0.5);
+    fetch(k0 + 1);
+  } else {
+    fetch(k0 - 1);
+    take(pc);
+    fetch(k0);
+  }
+  S fl[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) fl[q] = S(0);
+#pragma unroll 1
+  for (int kf = k0 - 1; kf <= k1; kf++) {
+    S pn[8];
+    take(pn);  // P(kf+ORDER)
+    if (kf + ORDER + 1 <= klast) fetch(kf + ORDER + 1);
+    // partials of the cell this face closes, loaded ahead of the solve
+    const int kc0 = max(kf, k0);
+    const long long c = col + (long long)(kc0 + 1) * sz;
+    double wv[8];
+#pragma unroll
+    for (int q = 0; q < NH; q++) wv[q] = W[q * N + c];
+    if constexpr (FLUXCD) {
+      wv[5] = W[5 * N + c];
+      wv[6] = W[6 * N + c];
+    }
+    S left[8], right[8];
+    if constexpr (ORDER == 2) {
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const S s = lim_average(lim, pc[q] - pm[q], pn[q] - pc[q]);
+        left[q] = pu[q];
+        right[q] = pc[q] - s * S(0.5);
+        pu[q] = pc[q] + s * S(0.5);
+        pm[q] = pc[q];
+        pc[q] = pn[q];
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        left[q] = pc[q];
+        right[q] = pn[q];
+        pc[q] = pn[q];
+      }
+    }
+    axis_perm<2>(left);
+    axis_perm<2>(right);
+    S ff[8];
+    if (!riemann<SOLVER>(b.ph, left, right, ff)) atomicOr(err, 1);
+    axis_perm<2>(ff);
+    if (kf >= k0) {
+#pragma unroll
+      for (int q = 0; q < NH; q++) Uout[q * N + c] = (S)(wv[q] - dtdz * ((double)ff[q] - (double)fl[q]));
+      if constexpr (FLUXCD) {
+        b.E[0 * N + c] = (S)(0.25 * ((wv[5] + (double)fl[6]) + (double)ff[6]));
+        b.E[1 * N + c] = (S)(0.25 * ((wv[6] - (double)fl[5]) - (double)ff[5]));
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) fl[q] = ff[q];
+  }
 }
 
 // ---------------------------------------------------------------------------
-// upwind update, step(dt) (src/hydro_solver.f90:77-127) with flux_cd_update
-// (src/flux_cd_module.f90:285-323), accumulated in FP64.  Each thread
-// marches up a z-column segment of `kchunk` cells, carrying the z-face flux
-// below the cell and E one plane below and at the cell, so every z
-// neighbour is read once.
+// flux-CD field update (src/flux_cd_module.f90:285-323, with step's
+// ordering of terms), accumulated in FP64.  Each thread marches up a
+// z-pencil carrying E1, E2 one plane below and at the cell.
 //
 // E ghosts (boundaryI_ef): the update reads e2, e3 at x neighbours, e1, e3
 // at y neighbours and e1, e2 at z neighbours.  A closed boundary negates e1
 // on x ghosts and e2 on y ghosts only (flux_cd_module.f90:145-176), never a
 // component read there, so every E ghost resolves to a fixed interior
 // neighbour offset: periodic wraps, closed and outflow copy the edge cell.
-// Stage 2 writes U^{n+1} over U^n in place (each cell reads only its own
-// conserved state).
+// Stage 2 writes over U^n in place (each cell reads only its own field).
 // ---------------------------------------------------------------------------
 
 // offset (in cells along the axis) of the E ghost-resolved neighbour
@@ -208,9 +403,8 @@ __device__ __forceinline__ int e_nbr(int i, int n, int d, int bc) {
   return 0;  // closed / outflow: ghost takes the edge cell's value
 }
 
-template <int FLUXCD, typename S>
-__device__ __forceinline__ void update_body(const Box<S>& b, unsigned lb, double dt, int stage, int kchunk) {
-  constexpr int NH = FLUXCD ? 5 : 8;
+template <typename S>
+__device__ __forceinline__ void bfield_body(const Box<S>& b, unsigned lb, double dt, int stage, int kchunk) {
   const Geom& g = b.g;
   int t, kc;
   if (!tile_item(lb, tiles_of((unsigned)(g.nx * g.ny)), (unsigned)(g.nx * g.ny), t, kc)) return;
@@ -222,57 +416,33 @@ __device__ __forceinline__ void update_body(const Box<S>& b, unsigned lb, double
   const double dtdx = dt / b.dx, dtdy = dt / b.dy, dtdz = dt / b.dz;
   const S* Uin = b.U;
   S* Uout = stage == 1 ? b.Us : b.U;
-  const S* __restrict__ Fx = b.F;
-  const S* __restrict__ Fy = b.F + 8 * N;
-  const S* __restrict__ Fz = b.F + 16 * N;
-  const S* __restrict__ E = b.E;
+  const S* __restrict__ E1 = b.E;
+  const S* __restrict__ E2 = b.E + N;
+  const S* __restrict__ E3 = b.E + 2 * N;
+  auto d = [](S v) { return (double)v; };
   long long c = cidx(g, i, j, k0);
-  double fzm[NH];
-#pragma unroll
-  for (int q = 0; q < NH; q++) fzm[q] = (double)Fz[q * N + c - sz];
-  long long oxm = 0, oxp = 0, oym = 0, oyp = 0;
-  double e1m = 0, e2m = 0, e1c = 0, e2c = 0;
-  if constexpr (FLUXCD) {
-    oxm = e_nbr(i, g.nx, -1, g.bc[0]) * sx;
-    oxp = e_nbr(i, g.nx, +1, g.bc[1]) * sx;
-    oym = e_nbr(j, g.ny, -1, g.bc[2]) * sy;
-    oyp = e_nbr(j, g.ny, +1, g.bc[3]) * sy;
-    const long long ozm = e_nbr(k0, g.nz, -1, g.bc[4]) * sz;
-    e1m = (double)E[0 * N + c + ozm];
-    e2m = (double)E[1 * N + c + ozm];
-    e1c = (double)E[0 * N + c];
-    e2c = (double)E[1 * N + c];
-  }
+  const long long oxm = e_nbr(i, g.nx, -1, g.bc[0]) * sx;
+  const long long oxp = e_nbr(i, g.nx, +1, g.bc[1]) * sx;
+  const long long oym = e_nbr(j, g.ny, -1, g.bc[2]) * sy;
+  const long long oyp = e_nbr(j, g.ny, +1, g.bc[3]) * sy;
+  const long long ozm = e_nbr(k0, g.nz, -1, g.bc[4]) * sz;
+  double e1m = d(E1[c + ozm]), e2m = d(E2[c + ozm]);
+  double e1c = d(E1[c]), e2c = d(E2[c]);
 #pragma unroll 1
   for (int k = k0; k <= k1; k++, c += sz) {
-    double out[8];
-#pragma unroll
-    for (int q = 0; q < NH; q++) {
-      const long long o = q * N + c;
-      const double fz = (double)Fz[o];
-      out[q] = (((double)Uin[o] - dtdx * ((double)Fx[o] - (double)Fx[o - sx]))
-                - dtdy * ((double)Fy[o] - (double)Fy[o - sy]))
-               - dtdz * (fz - fzm[q]);
-      fzm[q] = fz;
-    }
-    if constexpr (FLUXCD) {
-      const long long ozp = e_nbr(k, g.nz, +1, g.bc[5]) * sz;
-      const double e1p = (double)E[0 * N + c + ozp];
-      const double e2p = (double)E[1 * N + c + ozp];
-      const S* E1 = E;
-      const S* E2 = E + N;
-      const S* E3 = E + 2 * N;
-      auto d = [](S v) { return (double)v; };
-      out[5] = (d(Uin[5 * N + c]) - (0.5 * dtdy) * (d(E3[c + oyp]) - d(E3[c + oym]))) + (0.5 * dtdz) * (e2p - e2m);
-      out[6] = (d(Uin[6 * N + c]) + (0.5 * dtdx) * (d(E3[c + oxp]) - d(E3[c + oxm]))) - (0.5 * dtdz) * (e1p - e1m);
-      out[7] = (d(Uin[7 * N + c]) - (0.5 * dtdx) * (d(E2[c + oxp]) - d(E2[c + oxm]))) + (0.5 * dtdy) * (d(E1[c + oyp]) - d(E1[c + oym]));
-      e1m = e1c;
-      e2m = e2c;
-      e1c = e1p;
-      e2c = e2p;
-    }
-#pragma unroll
-    for (int q = 0; q < 8; q++) Uout[q * N + c] = (S)out[q];
+    const long long ozp = e_nbr(k, g.nz, +1, g.bc[5]) * sz;
+    const double e1p = d(E1[c + ozp]);
+    const double e2p = d(E2[c + ozp]);
+    const double b5 = (d(Uin[5 * N + c]) - (0.5 * dtdy) * (d(E3[c + oyp]) - d(E3[c + oym]))) + (0.5 * dtdz) * (e2p - e2m);
+    const double b6 = (d(Uin[6 * N + c]) + (0.5 * dtdx) * (d(E3[c + oxp]) - d(E3[c + oxm]))) - (0.5 * dtdz) * (e1p - e1m);
+    const double b7 = (d(Uin[7 * N + c]) - (0.5 * dtdx) * (d(E2[c + oxp]) - d(E2[c + oxm]))) + (0.5 * dtdy) * (d(E1[c + oyp]) - d(E1[c + oym]));
+    Uout[5 * N + c] = (S)b5;
+    Uout[6 * N + c] = (S)b6;
+    Uout[7 * N + c] = (S)b7;
+    e1m = e1c;
+    e2m = e2c;
+    e1c = e1p;
+    e2c = e2p;
   }
 }
 
@@ -344,20 +514,20 @@ __device__ __forceinline__ void cfl_body(const Box<S>& b, unsigned lb, int kchun
 // ---------------------------------------------------------------------------
 // __global__ wrappers: one box ...
 // ---------------------------------------------------------------------------
-template <typename S>
-__global__ void __launch_bounds__(256) k_prim(Box<S> b, int stage) { prim_body(b, stage, xcd_logical_block()); }
+template <int SOLVER, int ORDER, int FLUXCD, typename S>
+__global__ void __launch_bounds__(XY_THREADS) k_xy(Box<S> b, double dt, int* err) {
+  __shared__ S lds[XYLds<ORDER>::SIZE];
+  xy_body<SOLVER, ORDER, FLUXCD>(b, xcd_logical_block(), dt, err, lds);
+}
 
-template <int SOLVER, int ORDER, int AXIS, typename S>
-__global__ void __launch_bounds__(256) k_flux(Box<S> b, int* err) {
-  flux_body<SOLVER, ORDER, AXIS>(b, xcd_logical_block(), err);
+template <int SOLVER, int ORDER, int FLUXCD, typename S>
+__global__ void __launch_bounds__(256) k_z(Box<S> b, double dt, int kchunk, int* err) {
+  z_body<SOLVER, ORDER, FLUXCD>(b, xcd_logical_block(), dt, kchunk, err);
 }
 
 template <typename S>
-__global__ void __launch_bounds__(256) k_efield(Box<S> b) { efield_body(b, xcd_logical_block()); }
-
-template <int FLUXCD, typename S>
-__global__ void __launch_bounds__(256) k_update(Box<S> b, double dt, int stage, int kchunk) {
-  update_body<FLUXCD>(b, xcd_logical_block(), dt, stage, kchunk);
+__global__ void __launch_bounds__(256) k_bfield(Box<S> b, double dt, int stage, int kchunk) {
+  bfield_body(b, xcd_logical_block(), dt, stage, kchunk);
 }
 
 template <typename S>
@@ -365,45 +535,44 @@ __global__ void __launch_bounds__(256) k_cfl(Box<S> b, int kchunk, unsigned long
   cfl_body(b, xcd_logical_block(), kchunk, dtmin, err);
 }
 
-// ... and a batch of boxes
-template <typename S>
-__global__ void __launch_bounds__(256) kb_prim(const Box<S>* __restrict__ boxes, const unsigned* __restrict__ prefix,
-                                               int nbox, int stage) {
-  int bx;
-  unsigned lb;
-  batch_locate(prefix, nbox, xcd_logical_block(), bx, lb);
-  prim_body(boxes[bx], stage, lb);
+// ... and a batch of boxes.  dt of each box from the device clock table;
+// the predictor uses dt/2 (hydro_solver.f90:152).
+__device__ __forceinline__ double stage_dt(const BoxClock* __restrict__ clk, int slot, int stage) {
+  const double dt = clk[slot].dt;
+  return stage == 1 ? dt / 2.0 : dt;
 }
 
-template <int SOLVER, int ORDER, int AXIS, typename S>
-__global__ void __launch_bounds__(256) kb_flux(const Box<S>* __restrict__ boxes, const unsigned* __restrict__ prefix,
-                                               int nbox, int* err) {
-  int bx;
-  unsigned lb;
-  batch_locate(prefix, nbox, xcd_logical_block(), bx, lb);
-  flux_body<SOLVER, ORDER, AXIS>(boxes[bx], lb, err);
-}
-
-template <typename S>
-__global__ void __launch_bounds__(256) kb_efield(const Box<S>* __restrict__ boxes, const unsigned* __restrict__ prefix,
-                                                 int nbox) {
-  int bx;
-  unsigned lb;
-  batch_locate(prefix, nbox, xcd_logical_block(), bx, lb);
-  efield_body(boxes[bx], lb);
-}
-
-// dt of each box from the device clock table; the predictor uses dt/2
-// (hydro_solver.f90:152)
-template <int FLUXCD, typename S>
-__global__ void __launch_bounds__(256) kb_update(const Box<S>* __restrict__ boxes, const unsigned* __restrict__ prefix,
-                                                 int nbox, const BoxClock* __restrict__ clk, int stage, int kchunk) {
+template <int SOLVER, int ORDER, int FLUXCD, typename S>
+__global__ void __launch_bounds__(XY_THREADS) kb_xy(const Box<S>* __restrict__ boxes,
+                                                    const unsigned* __restrict__ prefix, int nbox,
+                                                    const BoxClock* __restrict__ clk, int* err) {
+  __shared__ S lds[XYLds<ORDER>::SIZE];
   int bx;
   unsigned lb;
   batch_locate(prefix, nbox, xcd_logical_block(), bx, lb);
   const Box<S>& b = boxes[bx];
-  const double dt = clk[b.slot].dt;
-  update_body<FLUXCD>(b, lb, stage == 1 ? dt / 2.0 : dt, stage, kchunk);
+  xy_body<SOLVER, ORDER, FLUXCD>(b, lb, stage_dt(clk, b.slot, ORDER), err, lds);
+}
+
+template <int SOLVER, int ORDER, int FLUXCD, typename S>
+__global__ void __launch_bounds__(256) kb_z(const Box<S>* __restrict__ boxes, const unsigned* __restrict__ prefix,
+                                            int nbox, const BoxClock* __restrict__ clk, int kchunk, int* err) {
+  int bx;
+  unsigned lb;
+  batch_locate(prefix, nbox, xcd_logical_block(), bx, lb);
+  const Box<S>& b = boxes[bx];
+  z_body<SOLVER, ORDER, FLUXCD>(b, lb, stage_dt(clk, b.slot, ORDER), kchunk, err);
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) kb_bfield(const Box<S>* __restrict__ boxes,
+                                                 const unsigned* __restrict__ prefix, int nbox,
+                                                 const BoxClock* __restrict__ clk, int stage, int kchunk) {
+  int bx;
+  unsigned lb;
+  batch_locate(prefix, nbox, xcd_logical_block(), bx, lb);
+  const Box<S>& b = boxes[bx];
+  bfield_body(b, lb, stage_dt(clk, b.slot, stage), stage, kchunk);
 }
 
 template <typename S>
@@ -560,109 +729,83 @@ static inline unsigned grid_for(long long n, int block = 256) {
 static inline unsigned ceil_div(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
 
 unsigned box_tiles(const Geom& g, int kclass, const Shape& sh) {
-  const long long B = sh.block;
-  const unsigned nchunk = ceil_div(g.nz, sh.kchunk);
   switch (kclass) {
-    case KC_PRIM1: return ceil_div((long long)(g.nx + 2) * (g.ny + 2), B) * (unsigned)(g.nz + 2);
-    case KC_PRIM2: return ceil_div((long long)(g.nx + 4) * (g.ny + 4), B) * (unsigned)(g.nz + 4);
-    case KC_FLUXX: return ceil_div((long long)(g.nx + 1) * g.ny, B) * (unsigned)g.nz;
-    case KC_FLUXY: return ceil_div((long long)g.nx * (g.ny + 1), B) * (unsigned)g.nz;
-    case KC_FLUXZ: return ceil_div((long long)g.nx * g.ny, B) * (unsigned)(g.nz + 1);
-    case KC_EFIELD: return ceil_div((long long)g.nx * g.ny, B) * (unsigned)g.nz;
-    case KC_UPDATE:
-    case KC_CFLRED: return ceil_div((long long)g.nx * g.ny, B) * nchunk;
+    case KC_XY: return ceil_div(g.nx, TX) * ceil_div(g.ny, TY) * (unsigned)g.nz;
+    case KC_COL: return ceil_div((long long)g.nx * g.ny, sh.block) * ceil_div(g.nz, sh.kchunk);
   }
   return 0;
 }
 
-template <typename S>
-void launch_prim(hipStream_t s, const Box<S>& b, int stage, const Shape& sh) {
-  const unsigned n = box_tiles(b.g, stage == 1 ? KC_PRIM1 : KC_PRIM2, sh);
-  hipLaunchKernelGGL(k_prim<S>, dim3(n), dim3(sh.block), 0, s, b, stage);
-}
-
-template <int SOLVER, int ORDER, int AXIS, typename S>
-static void launch_flux_a(hipStream_t s, const Box<S>& b, int* err, const Shape& sh) {
-  const unsigned n = box_tiles(b.g, KC_FLUXX + AXIS, sh);
-  hipLaunchKernelGGL((k_flux<SOLVER, ORDER, AXIS, S>), dim3(n), dim3(sh.block), 0, s, b, err);
-}
-
-template <int SOLVER, int ORDER, int AXIS, typename S>
-static void launch_flux_ab(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
-                           int* err, const Shape& sh) {
-  hipLaunchKernelGGL((kb_flux<SOLVER, ORDER, AXIS, S>), dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, err);
-}
-
-// solver x order x axis dispatch for both launch forms
-#define GM_FLUX_DISPATCH(CALL)                                                       \
+// solver x order x flux-CD dispatch; CALL(SOLVER, ORDER, FLUXCD)
+#define GM_STAGE_DISPATCH(CALL)                                                      \
   switch (solver) {                                                                  \
-    case SOLVER_HLLD: GM_FLUX_OA(SOLVER_HLLD, CALL) break;                            \
-    case SOLVER_HLLE: GM_FLUX_OA(SOLVER_HLLE, CALL) break;                            \
-    case SOLVER_HLL: GM_FLUX_OA(SOLVER_HLL, CALL) break;                              \
-    default: GM_FLUX_OA(SOLVER_HLLC, CALL) break;                                     \
+    case SOLVER_HLLD: GM_STAGE_OF(SOLVER_HLLD, CALL) break;                           \
+    case SOLVER_HLLE: GM_STAGE_OF(SOLVER_HLLE, CALL) break;                           \
+    case SOLVER_HLL: GM_STAGE_OF(SOLVER_HLL, CALL) break;                             \
+    default: GM_STAGE_OF(SOLVER_HLLC, CALL) break;                                    \
   }
-#define GM_FLUX_OA(SV, CALL)                                                         \
-  if (order == 1) {                                                                  \
-    if (axis == 0) CALL(SV, 1, 0) else if (axis == 1) CALL(SV, 1, 1) else CALL(SV, 1, 2) \
+#define GM_STAGE_OF(SV, CALL)                                                        \
+  if (stage == 1) {                                                                  \
+    if (flux_cd) CALL(SV, 1, 1) else CALL(SV, 1, 0)                                  \
   } else {                                                                           \
-    if (axis == 0) CALL(SV, 2, 0) else if (axis == 1) CALL(SV, 2, 1) else CALL(SV, 2, 2) \
+    if (flux_cd) CALL(SV, 2, 1) else CALL(SV, 2, 0)                                  \
   }
 
 template <typename S>
-void launch_flux(hipStream_t s, const Box<S>& b, int solver, int order, int axis, int* err, const Shape& sh) {
-#define GM_CALL1(SV, O, A) { launch_flux_a<SV, O, A, S>(s, b, err, sh); }
-  GM_FLUX_DISPATCH(GM_CALL1)
-#undef GM_CALL1
+void launch_xy(hipStream_t s, const Box<S>& b, double dt, int stage, int solver, int flux_cd, int* err) {
+  const unsigned n = box_tiles(b.g, KC_XY, Shape{XY_THREADS, 1});
+#define GM_CALL(SV, O, F) { hipLaunchKernelGGL((k_xy<SV, O, F, S>), dim3(n), dim3(XY_THREADS), 0, s, b, dt, err); }
+  GM_STAGE_DISPATCH(GM_CALL)
+#undef GM_CALL
 }
 
 template <typename S>
-void launch_flux_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
-                       int solver, int order, int axis, int* err, const Shape& sh) {
-#define GM_CALLB(SV, O, A) { launch_flux_ab<SV, O, A, S>(s, boxes, prefix, nbox, tiles, err, sh); }
-  GM_FLUX_DISPATCH(GM_CALLB)
-#undef GM_CALLB
-}
-#undef GM_FLUX_OA
-#undef GM_FLUX_DISPATCH
-
-template <typename S>
-void launch_efield(hipStream_t s, const Box<S>& b, const Shape& sh) {
-  hipLaunchKernelGGL(k_efield<S>, dim3(box_tiles(b.g, KC_EFIELD, sh)), dim3(sh.block), 0, s, b);
+void launch_z(hipStream_t s, const Box<S>& b, double dt, int stage, int solver, int flux_cd, int* err,
+              const Shape& sh) {
+  const unsigned n = box_tiles(b.g, KC_COL, sh);
+#define GM_CALL(SV, O, F) \
+  { hipLaunchKernelGGL((k_z<SV, O, F, S>), dim3(n), dim3(sh.block), 0, s, b, dt, sh.kchunk, err); }
+  GM_STAGE_DISPATCH(GM_CALL)
+#undef GM_CALL
 }
 
 template <typename S>
-void launch_update(hipStream_t s, const Box<S>& b, double dt, int stage, int flux_cd, const Shape& sh) {
-  const unsigned n = box_tiles(b.g, KC_UPDATE, sh);
-  if (flux_cd) hipLaunchKernelGGL((k_update<1, S>), dim3(n), dim3(sh.block), 0, s, b, dt, stage, sh.kchunk);
-  else hipLaunchKernelGGL((k_update<0, S>), dim3(n), dim3(sh.block), 0, s, b, dt, stage, sh.kchunk);
+void launch_bfield(hipStream_t s, const Box<S>& b, double dt, int stage, const Shape& sh) {
+  hipLaunchKernelGGL(k_bfield<S>, dim3(box_tiles(b.g, KC_COL, sh)), dim3(sh.block), 0, s, b, dt, stage, sh.kchunk);
 }
 
 template <typename S>
 void launch_cfl(hipStream_t s, const Box<S>& b, unsigned long long* dtmin, int* err, const Shape& sh) {
-  hipLaunchKernelGGL(k_cfl<S>, dim3(box_tiles(b.g, KC_CFLRED, sh)), dim3(sh.block), 0, s, b, sh.kchunk, dtmin, err);
+  hipLaunchKernelGGL(k_cfl<S>, dim3(box_tiles(b.g, KC_COL, sh)), dim3(sh.block), 0, s, b, sh.kchunk, dtmin, err);
 }
 
 template <typename S>
-void launch_prim_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
-                       int stage, const Shape& sh) {
-  hipLaunchKernelGGL(kb_prim<S>, dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, stage);
+void launch_xy_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
+                     const BoxClock* clk, int stage, int solver, int flux_cd, int* err) {
+#define GM_CALL(SV, O, F) \
+  { hipLaunchKernelGGL((kb_xy<SV, O, F, S>), dim3(tiles), dim3(XY_THREADS), 0, s, boxes, prefix, nbox, clk, err); }
+  GM_STAGE_DISPATCH(GM_CALL)
+#undef GM_CALL
 }
 
 template <typename S>
-void launch_efield_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
-                         const Shape& sh) {
-  hipLaunchKernelGGL(kb_efield<S>, dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox);
+void launch_z_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
+                    const BoxClock* clk, int stage, int solver, int flux_cd, int* err, const Shape& sh) {
+#define GM_CALL(SV, O, F)                                                                                   \
+  {                                                                                                         \
+    hipLaunchKernelGGL((kb_z<SV, O, F, S>), dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, clk,     \
+                       sh.kchunk, err);                                                                     \
+  }
+  GM_STAGE_DISPATCH(GM_CALL)
+#undef GM_CALL
 }
+#undef GM_STAGE_OF
+#undef GM_STAGE_DISPATCH
 
 template <typename S>
-void launch_update_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
-                         const BoxClock* clk, int stage, int flux_cd, const Shape& sh) {
-  if (flux_cd)
-    hipLaunchKernelGGL((kb_update<1, S>), dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, clk, stage,
-                       sh.kchunk);
-  else
-    hipLaunchKernelGGL((kb_update<0, S>), dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, clk, stage,
-                       sh.kchunk);
+void launch_bfield_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
+                         const BoxClock* clk, int stage, const Shape& sh) {
+  hipLaunchKernelGGL(kb_bfield<S>, dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, clk, stage, sh.kchunk);
 }
 
 template <typename S>
@@ -709,18 +852,16 @@ void launch_divb(hipStream_t s, const Geom& g, const S* U, double dx, double dy,
 }
 
 #define GM_INSTANTIATE(S)                                                                                        \
-  template void launch_prim<S>(hipStream_t, const Box<S>&, int, const Shape&);                                    \
-  template void launch_flux<S>(hipStream_t, const Box<S>&, int, int, int, int*, const Shape&);                    \
-  template void launch_efield<S>(hipStream_t, const Box<S>&, const Shape&);                                       \
-  template void launch_update<S>(hipStream_t, const Box<S>&, double, int, int, const Shape&);                     \
+  template void launch_xy<S>(hipStream_t, const Box<S>&, double, int, int, int, int*);                            \
+  template void launch_z<S>(hipStream_t, const Box<S>&, double, int, int, int, int*, const Shape&);               \
+  template void launch_bfield<S>(hipStream_t, const Box<S>&, double, int, const Shape&);                          \
   template void launch_cfl<S>(hipStream_t, const Box<S>&, unsigned long long*, int*, const Shape&);               \
-  template void launch_prim_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, int,             \
-                                     const Shape&);                                                               \
-  template void launch_flux_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, int, int, int,   \
-                                     int*, const Shape&);                                                         \
-  template void launch_efield_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, const Shape&); \
-  template void launch_update_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned,                \
-                                       const BoxClock*, int, int, const Shape&);                                  \
+  template void launch_xy_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, const BoxClock*,   \
+                                   int, int, int, int*);                                                          \
+  template void launch_z_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, const BoxClock*,    \
+                                  int, int, int, int*, const Shape&);                                             \
+  template void launch_bfield_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned,                \
+                                       const BoxClock*, int, const Shape&);                                       \
   template void launch_cfl_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, BoxClock*, int*,  \
                                     const Shape&);                                                                \
   template void launch_ghost1<S>(hipStream_t, const Geom&, S*, int, int, int);                                    \

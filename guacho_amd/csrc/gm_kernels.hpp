@@ -19,17 +19,19 @@ struct Geom {
 };
 
 // One box (subdomain) as the kernels see it: geometry, physics and its
-// device buffers.  F holds the three axes' fluxes back to back (axis a at
-// F + a*8*ncell).
+// device buffers.  W carries a stage's partial results from the xy-face
+// kernel to the z-face kernel, in FP64 for either storage type: the
+// conserved update after its x and y terms (components 0..4 with flux-CD,
+// 0..7 without) and, with flux-CD, the x/y parts of E1 (component 5) and
+// E2 (component 6).
 template <typename S>
 struct Box {
   Geom g;
   Phys ph;
-  S* U;   // U^n, updated in place by the corrector
-  S* Us;  // U* (predictor result)
-  S* P;   // primitives
-  S* F;   // fluxes, 3 x 8 components
-  S* E;   // cell-centred E, 3 components
+  S* U;       // U^n, updated in place by the corrector
+  S* Us;      // U* (predictor result)
+  double* W;  // stage partials, 8 x ncell doubles
+  S* E;       // cell-centred E, 3 components
   double dx, dy, dz;
   int slot;  // index of the box's clock in a batch's clock table
 };
@@ -51,8 +53,9 @@ struct BoxClock {
   int pad;
 };
 
-// Launch shape: threads per tile (64, 128 or 256) and z-pencil length of
-// the column-marching kernels.
+// Launch shape of the column-marching kernels (z faces, field update, CFL):
+// threads per tile (64, 128 or 256) and z-pencil length.  The xy-face
+// kernel has a fixed 64 x 8 plane tile.
 struct Shape {
   int block;
   int kchunk;
@@ -61,8 +64,10 @@ struct Shape {
 // Validates a subdomain configuration; GM_OK or a GM_ERR_* code with msg.
 int check_params(const gm_params* p, std::string& msg);
 
-// Kernel classes of one step, for per-class tile tables.
-enum KClass { KC_PRIM1 = 0, KC_PRIM2, KC_FLUXX, KC_FLUXY, KC_FLUXZ, KC_EFIELD, KC_UPDATE, KC_CFLRED, KC_COUNT };
+// Tile classes of one step, for per-class tile tables: 64 x 8 plane tiles
+// of the xy-face kernel, and column pencils of the z-face, field-update and
+// CFL kernels.
+enum KClass { KC_XY = 0, KC_COL, KC_COUNT };
 
 // Tiles (workgroups) one box needs for a kernel class under a shape.
 unsigned box_tiles(const Geom& g, int kclass, const Shape& sh);
@@ -70,31 +75,31 @@ unsigned box_tiles(const Geom& g, int kclass, const Shape& sh);
 // Launchers, instantiated for S = double (the FP64 drop-in) and S = float
 // (the mixed form's storage).  Kernels compute primitives, the CFL
 // candidates and the conserved update in FP64 for either storage type.
+// A stage is one call of each of launch_xy, launch_z and (with flux-CD)
+// launch_bfield; stage 1 is the predictor (first order, dt/2 passed in
+// `dt`), stage 2 the corrector.
 template <typename S>
-void launch_prim(hipStream_t s, const Box<S>& b, int stage, const Shape& sh);
+void launch_xy(hipStream_t s, const Box<S>& b, double dt, int stage, int solver, int flux_cd, int* err);
 template <typename S>
-void launch_flux(hipStream_t s, const Box<S>& b, int solver, int order, int axis, int* err, const Shape& sh);
+void launch_z(hipStream_t s, const Box<S>& b, double dt, int stage, int solver, int flux_cd, int* err,
+              const Shape& sh);
 template <typename S>
-void launch_efield(hipStream_t s, const Box<S>& b, const Shape& sh);
-template <typename S>
-void launch_update(hipStream_t s, const Box<S>& b, double dt, int stage, int flux_cd, const Shape& sh);
+void launch_bfield(hipStream_t s, const Box<S>& b, double dt, int stage, const Shape& sh);
 template <typename S>
 void launch_cfl(hipStream_t s, const Box<S>& b, unsigned long long* dtmin, int* err, const Shape& sh);
 
 // Batched launchers: `tiles` is the total tile count of the class over all
-// boxes and `prefix` (device, nbox+1 entries) its prefix sums.
+// boxes and `prefix` (device, nbox+1 entries) its prefix sums.  Each box's
+// dt comes from its clock.
 template <typename S>
-void launch_prim_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
-                       int stage, const Shape& sh);
+void launch_xy_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
+                     const BoxClock* clk, int stage, int solver, int flux_cd, int* err);
 template <typename S>
-void launch_flux_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
-                       int solver, int order, int axis, int* err, const Shape& sh);
+void launch_z_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
+                    const BoxClock* clk, int stage, int solver, int flux_cd, int* err, const Shape& sh);
 template <typename S>
-void launch_efield_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
-                         const Shape& sh);
-template <typename S>
-void launch_update_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
-                         const BoxClock* clk, int stage, int flux_cd, const Shape& sh);
+void launch_bfield_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
+                         const BoxClock* clk, int stage, const Shape& sh);
 template <typename S>
 void launch_cfl_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
                       BoxClock* clk, int* err, const Shape& sh);
