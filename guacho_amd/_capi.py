@@ -16,13 +16,14 @@ LIB_PATH = os.path.join(_HERE, "libguacho_amd.so")
 
 GM_OK, GM_ERR_CONFIG, GM_ERR_HIP, GM_ERR_NONFINITE, GM_ERR_OOM, GM_ERR_ARG, GM_ERR_BUSY = 0, 1, 2, 3, 4, 5, 6
 GM_PRECISION_FP64, GM_PRECISION_MIXED = 0, 1
-GM_NKCLASS = 8
+GM_NKCLASS = 9
 
 # exported entry points, in the order include/guacho_amd.h declares them
 EXPORTS = (
     "gm_create", "gm_destroy", "gm_last_error", "gm_upload_u", "gm_download_u",
     "gm_download_primit", "gm_boundary_u", "gm_cfl_dtp", "gm_tstep", "gm_sync",
     "gm_download_divb", "gm_set_timing", "gm_kernel_times", "gm_device_bytes",
+    "gm_set_time", "gm_set_user_sources",
     "gm_batch_create", "gm_batch_destroy", "gm_batch_last_error", "gm_batch_box_bytes",
     "gm_batch_free_bytes", "gm_batch_add", "gm_batch_remove", "gm_batch_advance", "gm_batch_sync",
     "gm_batch_clock", "gm_batch_download_u", "gm_batch_tuning", "gm_batch_active",
@@ -87,6 +88,8 @@ def lib() -> C.CDLL:
     L.gm_set_timing.argtypes = [vp, C.c_int]
     L.gm_kernel_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_char_p)]
     L.gm_device_bytes.argtypes = [vp, C.POINTER(C.c_int64)]
+    L.gm_set_time.argtypes = [vp, C.c_double]
+    L.gm_set_user_sources.argtypes = [vp, C.c_char_p, C.c_char_p]
     P = C.POINTER(GmParams)
     L.gm_batch_create.argtypes = [C.POINTER(GmBatchParams), C.POINTER(vp)]
     L.gm_batch_destroy.argtypes = [vp]
@@ -204,6 +207,17 @@ class Subdomain:
 
     def set_timing(self, enable: bool):
         self._check(self.L.gm_set_timing(self.h, int(enable)))
+
+    def set_time(self, time: float) -> None:
+        """Simulation time the user plugins see (globals::time)."""
+        self._check(self.L.gm_set_time(self.h, float(time)))
+
+    def set_user_sources(self, code_object: Optional[str], kernel: Optional[str] = None) -> None:
+        """Load the user's source-term kernel from a gfx950 code object
+        (None removes it)."""
+        co = code_object.encode() if code_object else None
+        kn = kernel.encode() if kernel else None
+        self._check(self.L.gm_set_user_sources(self.h, co, kn))
 
     def kernel_times(self) -> Dict[str, tuple]:
         ms = (C.c_double * GM_NKCLASS)()

@@ -55,6 +55,7 @@ struct gm_batch {
   int* derr = nullptr;
   int* herr = nullptr;
   std::vector<int> order;          // slots of the packed table
+  bool any_sources = false;        // some active box has 8-wave source terms
   unsigned tiles[gm::KC_COUNT] = {0};
   bool tables_dirty = true;
   bool graph_dirty = true;
@@ -69,6 +70,7 @@ struct gm_batch {
   int cap_nbox = -1;
   unsigned cap_tiles[gm::KC_COUNT] = {0};
   gm::Shape cap_shape = {0, 0};
+  bool cap_sources = false;
   std::string msg;
 };
 
@@ -124,6 +126,7 @@ gm::Box<S> box_view(const Slot& s, int slot) {
   b.dy = s.p.dy;
   b.dz = s.p.dz;
   b.slot = slot;
+  b.src = s.p.eight_wave ? gm::SRC_EIGHT_WAVE : 0;
   return b;
 }
 
@@ -164,7 +167,11 @@ int rebuild_tables_t(gm_batch* b, const gm::Shape& sh) {
     prefix[(size_t)k * (cap + 1) + n] = acc;
     b->tiles[k] = acc;
   }
-  for (int i = 0; i < n; i++) boxes[i] = box_view<S>(b->slots[b->order[i]], b->order[i]);
+  b->any_sources = false;
+  for (int i = 0; i < n; i++) {
+    boxes[i] = box_view<S>(b->slots[b->order[i]], b->order[i]);
+    b->any_sources = b->any_sources || boxes[i].src != 0;
+  }
   BHIP(b, hipStreamSynchronize(b->stream));
   if (n) BHIP(b, hipMemcpy(b->dboxes, boxes.data(), sizeof(gm::Box<S>) * n, hipMemcpyHostToDevice));
   BHIP(b, hipMemcpy(b->dprefix, prefix.data(), sizeof(unsigned) * prefix.size(), hipMemcpyHostToDevice));
@@ -196,6 +203,7 @@ void enqueue_steps_t(gm_batch* b, int nsteps, int tuning) {
       gm::launch_xy_batch(s, boxes, pre(gm::KC_XY), n, txy, b->dclk, stage, solver, fcd, b->derr);
       gm::launch_z_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, stage, solver, fcd, b->derr, sh);
       if (fcd) gm::launch_bfield_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, stage, sh);
+      if (b->any_sources) gm::launch_sources_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, stage, sh);
     }
   }
 }
@@ -219,11 +227,13 @@ int capture_graph(gm_batch* b) {
   b->cap_nbox = (int)b->order.size();
   std::memcpy(b->cap_tiles, b->tiles, sizeof b->tiles);
   b->cap_shape = b->shape;
+  b->cap_sources = b->any_sources;
   return GM_OK;
 }
 
 bool graph_matches(const gm_batch* b) {
-  return b->exec && b->cap_nbox == (int)b->order.size() && b->cap_shape.block == b->shape.block &&
+  return b->exec && b->cap_nbox == (int)b->order.size() && b->cap_sources == b->any_sources &&
+         b->cap_shape.block == b->shape.block &&
          b->cap_shape.kchunk == b->shape.kchunk && std::memcmp(b->cap_tiles, b->tiles, sizeof b->tiles) == 0;
 }
 

@@ -17,12 +17,13 @@
 
 #include "gm_kernels.hpp"
 #include "guacho_amd.h"
+#include "guacho_amd_user.h"
 
 namespace {
 
 const char* kClassNames[GM_NKCLASS] = {"xy_predictor", "xy_corrector", "z_predictor", "z_corrector",
-                                       "bfield", "cfl_dt", "ghost", "layout"};
-enum { KC_XY1, KC_XY2, KC_Z1, KC_Z2, KC_BFIELD, KC_CFL, KC_GHOST, KC_LAYOUT };
+                                       "bfield", "cfl_dt", "ghost", "layout", "sources"};
+enum { KC_XY1, KC_XY2, KC_Z1, KC_Z2, KC_BFIELD, KC_CFL, KC_GHOST, KC_LAYOUT, KC_SOURCES };
 
 thread_local std::string g_create_error;
 
@@ -56,6 +57,9 @@ struct gm_ctx {
   Scalars* dscal = nullptr;  // device scalars
   Scalars* hscal = nullptr;  // pinned host mirror
   bool ghosts_dirty = true;
+  double time = 0.0;                    // simulation time the user hooks see
+  hipModule_t usrc_mod = nullptr;       // user source plugin
+  hipFunction_t usrc_fn = nullptr;
   gm::Shape shape = {256, 16};  // tile width, z-pencil length
   int timing = 0;
   struct Rec { int cls; hipEvent_t a, b; };
@@ -157,7 +161,6 @@ int check_params(const gm_params* p, std::string& msg) {
   for (int a = 0; a < 3; a++)
     if ((p->bc[2 * a] == GM_BC_PERIODIC) != (p->bc[2 * a + 1] == GM_BC_PERIODIC))
       return fail(nullptr, GM_ERR_CONFIG, "a periodic boundary must pair with a periodic boundary on the same axis");
-  if (p->eight_wave) return fail(nullptr, GM_ERR_CONFIG, "eight_wave (Powell sources) is not built");
   if (p->eta != 0.0) return fail(nullptr, GM_ERR_CONFIG, "eta (artificial viscosity) must be 0");
   if (p->precision != GM_PRECISION_FP64 && p->precision != GM_PRECISION_MIXED)
     return fail(nullptr, GM_ERR_CONFIG, "precision must be GM_PRECISION_FP64 or GM_PRECISION_MIXED");
@@ -185,6 +188,7 @@ void release(gm_ctx* c) {
   void* bufs[] = {c->U, c->Us, c->W, c->E};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
+  if (c->usrc_mod) (void)hipModuleUnload(c->usrc_mod);
   if (c->dscal) (void)hipFree(c->dscal);
   if (c->hscal) (void)hipHostFree(c->hscal);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -231,7 +235,42 @@ gm::Box<S> box_of(const gm_ctx* c) {
   b.dy = c->p.dy;
   b.dz = c->p.dz;
   b.slot = 0;
+  b.src = (c->p.eight_wave ? gm::SRC_EIGHT_WAVE : 0) | (c->usrc_fn ? gm::SRC_USER : 0);
   return b;
+}
+
+// The user's source kernel over the interior cells of one stage, on a
+// zeroed s (the W block, free once the stage's z kernel has run).
+template <typename S>
+void user_sources_t(gm_ctx* c, const gm::Box<S>& b, double dt, int stage) {
+  gm_user_args a{};
+  a.nx = c->g.nx; a.ny = c->g.ny; a.nz = c->g.nz;
+  a.NX = c->g.NX; a.NY = c->g.NY; a.NZ = c->g.NZ;
+  a.ncell = c->g.ncell;
+  a.dx = c->p.dx; a.dy = c->p.dy; a.dz = c->p.dz;
+  a.cv = c->p.cv; a.gamma = c->p.gamma;
+  a.time = c->time;
+  a.dt = dt;
+  a.stage = stage;
+  a.precision = c->esize == 8 ? GM_PRECISION_FP64 : GM_PRECISION_MIXED;
+  a.mhd = c->p.mhd; a.pmhd = c->p.pmhd;
+  for (int f = 0; f < 6; f++) a.bc[f] = c->p.bc[f];
+  a.u = stage == 1 ? (void*)b.U : (void*)b.Us;
+  a.s = b.W;
+  (void)hipMemsetAsync(b.W, 0, (size_t)c->g.ncell * 8 * sizeof(double), c->stream);
+  const long long n = (long long)a.nx * a.ny * a.nz;
+  void* params[] = {&a};
+  (void)hipModuleLaunchKernel(c->usrc_fn, (unsigned)((n + 255) / 256), 1, 1, 256, 1, 1, 0, c->stream, params,
+                              nullptr);
+}
+
+template <typename S>
+void stage_sources_t(gm_ctx* c, const gm::Box<S>& b, double dt, int stage) {
+  if (!b.src) return;
+  timed(c, KC_SOURCES, [&] {
+    if (b.src & gm::SRC_USER) user_sources_t(c, b, dt, stage);
+    gm::launch_sources(c->stream, b, dt, stage, c->shape);
+  });
 }
 
 // the step of one precision: tstep (src/hydro_solver.f90:134-229)
@@ -244,15 +283,17 @@ void tstep_t(gm_ctx* c, double dt) {
   const gm::Shape& sh = c->shape;
   const int solver = p.riemann_solver, fcd = p.enable_flux_cd ? 1 : 0;
   const double dtm = dt / 2.0;  // hydro_solver.f90:152
-  // predictor: first-order fluxes of U^n, U* = U^n - dt/2 div F
+  // predictor: first-order fluxes of U^n, U* = U^n - dt/2 div F (+ dt/2 s)
   timed(c, KC_XY1, [&] { gm::launch_xy(s, b, dtm, 1, solver, fcd, err); });
   timed(c, KC_Z1, [&] { gm::launch_z(s, b, dtm, 1, solver, fcd, err, sh); });
   if (fcd) timed(c, KC_BFIELD, [&] { gm::launch_bfield(s, b, dtm, 1, sh); });
-  // corrector: limited fluxes of U*, U^{n+1} = U^n - dt div F (in place;
-  // viscous_copy with eta = 0 is the identity on values)
+  stage_sources_t(c, b, dtm, 1);
+  // corrector: limited fluxes of U*, U^{n+1} = U^n - dt div F (+ dt s), in
+  // place; viscous_copy with eta = 0 is the identity on values
   timed(c, KC_XY2, [&] { gm::launch_xy(s, b, dt, 2, solver, fcd, err); });
   timed(c, KC_Z2, [&] { gm::launch_z(s, b, dt, 2, solver, fcd, err, sh); });
   if (fcd) timed(c, KC_BFIELD, [&] { gm::launch_bfield(s, b, dt, 2, sh); });
+  stage_sources_t(c, b, dt, 2);
 }
 
 template <typename S>
@@ -480,6 +521,39 @@ int gm_kernel_times(gm_ctx* c, double* ms, int64_t* launches, const char** names
     c->ms[k] = 0;
     c->launches[k] = 0;
   }
+  return GM_OK;
+}
+
+int gm_set_time(gm_ctx* c, double time) {
+  if (!c) return GM_ERR_ARG;
+  if (!std::isfinite(time)) return fail(c, GM_ERR_ARG, "time is not finite");
+  c->time = time;
+  return GM_OK;
+}
+
+int gm_set_user_sources(gm_ctx* c, const char* code_object, const char* kernel) {
+  if (!c) return GM_ERR_ARG;
+  int rc = set_device(c);
+  if (rc) return rc;
+  GM_HIP(c, hipStreamSynchronize(c->stream));
+  if (c->usrc_mod) (void)hipModuleUnload(c->usrc_mod);
+  c->usrc_mod = nullptr;
+  c->usrc_fn = nullptr;
+  if (!code_object) return GM_OK;
+  if (!kernel) return fail(c, GM_ERR_ARG, "kernel name is NULL");
+  hipModule_t m = nullptr;
+  if (hipModuleLoad(&m, code_object) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(c, GM_ERR_CONFIG, std::string("cannot load code object ") + code_object);
+  }
+  hipFunction_t f = nullptr;
+  if (hipModuleGetFunction(&f, m, kernel) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipModuleUnload(m);
+    return fail(c, GM_ERR_CONFIG, std::string("no kernel ") + kernel + " in " + code_object);
+  }
+  c->usrc_mod = m;
+  c->usrc_fn = f;
   return GM_OK;
 }
 

@@ -447,6 +447,60 @@ __device__ __forceinline__ void bfield_body(const Box<S>& b, unsigned lb, double
 }
 
 // ---------------------------------------------------------------------------
+// Source terms of step() (src/hydro_solver.f90:115-121): after the stage's
+// update, up = up + dt*s with s from source() (src/sources.f90:188-220):
+// zero, plus the user's terms (already in W, written by the user plugin on
+// a zeroed W), then divbcorr_8w_source (:137-168) from the primitives of the
+// stage's input state -- U^n for the predictor, U* for the corrector, ghosts
+// by boundary mapping (the field is never flipped).  Column pencils.
+// ---------------------------------------------------------------------------
+template <typename S>
+__device__ __forceinline__ double field_at(const Geom& g, const S* __restrict__ U, int q, int i, int j, int k) {
+  bool f = false;
+  const int si = map_ghost(i, g.nx, g.bc[0], g.bc[1], f);
+  const int sj = map_ghost(j, g.ny, g.bc[2], g.bc[3], f);
+  const int sk = map_ghost(k, g.nz, g.bc[4], g.bc[5], f);
+  return (double)U[q * g.ncell + cidx(g, si, sj, sk)];
+}
+
+template <typename S>
+__device__ __forceinline__ void sources_body(const Box<S>& b, unsigned lb, double dt, int stage, int kchunk) {
+  const Geom& g = b.g;
+  if (!b.src) return;
+  int t, kc;
+  if (!tile_item(lb, tiles_of((unsigned)(g.nx * g.ny)), (unsigned)(g.nx * g.ny), t, kc)) return;
+  const int i = t % g.nx + 1, j = t / g.nx + 1;
+  const int k0 = kc * kchunk + 1;
+  const int k1 = min(k0 + kchunk - 1, g.nz);
+  const long long N = g.ncell;
+  const S* __restrict__ Uin = stage == 1 ? b.U : b.Us;
+  S* __restrict__ Uout = stage == 1 ? b.Us : b.U;
+  for (int k = k0; k <= k1; k++) {
+    const long long c = cidx(g, i, j, k);
+    double u[8], pp[8], sv[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) u[q] = (double)Uin[q * N + c];
+    u2prim(b.ph, u, pp);
+#pragma unroll
+    for (int q = 0; q < 8; q++) sv[q] = (b.src & SRC_USER) ? b.W[q * N + c] : 0.0;
+    if (b.src & SRC_EIGHT_WAVE) {
+      const double divb = ((field_at(g, Uin, 5, i + 1, j, k) - field_at(g, Uin, 5, i - 1, j, k)) / (2.0 * b.dx)
+                           + (field_at(g, Uin, 6, i, j + 1, k) - field_at(g, Uin, 6, i, j - 1, k)) / (2.0 * b.dy))
+                          + (field_at(g, Uin, 7, i, j, k + 1) - field_at(g, Uin, 7, i, j, k - 1)) / (2.0 * b.dz);
+      sv[1] = sv[1] - divb * pp[5];
+      sv[2] = sv[2] - divb * pp[6];
+      sv[3] = sv[3] - divb * pp[7];
+      sv[4] = sv[4] - divb * ((pp[1] * pp[5] + pp[2] * pp[6]) + pp[3] * pp[7]);
+      sv[5] = sv[5] - divb * pp[1];
+      sv[6] = sv[6] - divb * pp[2];
+      sv[7] = sv[7] - divb * pp[3];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) Uout[q * N + c] = (S)((double)Uout[q * N + c] + dt * sv[q]);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // CFL candidate reduction: the min loop of get_timestep
 // (src/hydro_core.f90:642-675) on the primitives of the interior.  Exact
 // (min is order-free); block minima meet in one 64-bit atomicMin on the
@@ -535,6 +589,11 @@ __global__ void __launch_bounds__(256) k_cfl(Box<S> b, int kchunk, unsigned long
   cfl_body(b, xcd_logical_block(), kchunk, dtmin, err);
 }
 
+template <typename S>
+__global__ void __launch_bounds__(256) k_sources(Box<S> b, double dt, int stage, int kchunk) {
+  sources_body(b, xcd_logical_block(), dt, stage, kchunk);
+}
+
 // ... and a batch of boxes.  dt of each box from the device clock table;
 // the predictor uses dt/2 (hydro_solver.f90:152).
 __device__ __forceinline__ double stage_dt(const BoxClock* __restrict__ clk, int slot, int stage) {
@@ -582,6 +641,17 @@ __global__ void __launch_bounds__(256) kb_cfl(const Box<S>* __restrict__ boxes, 
   unsigned lb;
   batch_locate(prefix, nbox, xcd_logical_block(), bx, lb);
   cfl_body(boxes[bx], lb, kchunk, &clk[boxes[bx].slot].dtmin, err);
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) kb_sources(const Box<S>* __restrict__ boxes,
+                                                  const unsigned* __restrict__ prefix, int nbox,
+                                                  const BoxClock* __restrict__ clk, int stage, int kchunk) {
+  int bx;
+  unsigned lb;
+  batch_locate(prefix, nbox, xcd_logical_block(), bx, lb);
+  const Box<S>& b = boxes[bx];
+  sources_body(b, lb, stage_dt(clk, b.slot, stage), stage, kchunk);
 }
 
 // Start of a batched step: reset every box's CFL minimum.
@@ -780,6 +850,17 @@ void launch_cfl(hipStream_t s, const Box<S>& b, unsigned long long* dtmin, int* 
 }
 
 template <typename S>
+void launch_sources(hipStream_t s, const Box<S>& b, double dt, int stage, const Shape& sh) {
+  hipLaunchKernelGGL(k_sources<S>, dim3(box_tiles(b.g, KC_COL, sh)), dim3(sh.block), 0, s, b, dt, stage, sh.kchunk);
+}
+
+template <typename S>
+void launch_sources_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
+                          const BoxClock* clk, int stage, const Shape& sh) {
+  hipLaunchKernelGGL(kb_sources<S>, dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, clk, stage, sh.kchunk);
+}
+
+template <typename S>
 void launch_xy_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
                      const BoxClock* clk, int stage, int solver, int flux_cd, int* err) {
 #define GM_CALL(SV, O, F) \
@@ -856,6 +937,9 @@ void launch_divb(hipStream_t s, const Geom& g, const S* U, double dx, double dy,
   template void launch_z<S>(hipStream_t, const Box<S>&, double, int, int, int, int*, const Shape&);               \
   template void launch_bfield<S>(hipStream_t, const Box<S>&, double, int, const Shape&);                          \
   template void launch_cfl<S>(hipStream_t, const Box<S>&, unsigned long long*, int*, const Shape&);               \
+  template void launch_sources<S>(hipStream_t, const Box<S>&, double, int, const Shape&);                         \
+  template void launch_sources_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned,               \
+                                        const BoxClock*, int, const Shape&);                                      \
   template void launch_xy_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, const BoxClock*,   \
                                    int, int, int, int*);                                                          \
   template void launch_z_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, const BoxClock*,    \

@@ -34,7 +34,10 @@ struct Box {
   S* E;       // cell-centred E, 3 components
   double dx, dy, dz;
   int slot;  // index of the box's clock in a batch's clock table
+  int src;   // source terms: SRC_EIGHT_WAVE | SRC_USER (W holds the user's s)
 };
+
+enum : int { SRC_EIGHT_WAVE = 1, SRC_USER = 2 };
 
 // Per-box clock of a batch: the reference's time-loop state (time,
 // currentIteration) plus the CFL minimum of the current step and the stop
@@ -87,6 +90,10 @@ template <typename S>
 void launch_bfield(hipStream_t s, const Box<S>& b, double dt, int stage, const Shape& sh);
 template <typename S>
 void launch_cfl(hipStream_t s, const Box<S>& b, unsigned long long* dtmin, int* err, const Shape& sh);
+// source terms of step() (src/hydro_solver.f90:115-121): up += dt * s after
+// the stage's update, s from the stage's input state
+template <typename S>
+void launch_sources(hipStream_t s, const Box<S>& b, double dt, int stage, const Shape& sh);
 
 // Batched launchers: `tiles` is the total tile count of the class over all
 // boxes and `prefix` (device, nbox+1 entries) its prefix sums.  Each box's
@@ -103,6 +110,9 @@ void launch_bfield_batch(hipStream_t s, const Box<S>* boxes, const unsigned* pre
 template <typename S>
 void launch_cfl_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
                       BoxClock* clk, int* err, const Shape& sh);
+template <typename S>
+void launch_sources_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
+                          const BoxClock* clk, int stage, const Shape& sh);
 void launch_clock_begin(hipStream_t s, BoxClock* clk, int nbox);
 void launch_clock_dt(hipStream_t s, BoxClock* clk, int nbox, int tuning);
 

@@ -15,6 +15,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "guacho_amd_user.h"
+
 namespace gm {
 
 // Arithmetic of one precision.  FP64 uses IEEE division and sqrt (the
@@ -73,23 +75,10 @@ __device__ __forceinline__ void axis_perm(T v[8]) {
 
 // u2prim, src/hydro_core.f90:46-129 (EOS_ADIABATIC).  Always FP64: this is
 // the pressure recovery E - rho v^2/2 - B^2/2, whose cancellation at low
-// plasma beta the mixed form keeps in double.
+// plasma beta the mixed form keeps in double.  One implementation, shared
+// with user plugins (gm_u2prim in guacho_amd_user.h).
 __device__ __forceinline__ void u2prim(const Phys& ph, const double u[8], double p[8]) {
-  const double r = dmax(u[0], 1e-15);
-  p[0] = r;
-  p[1] = u[1] / r;
-  p[2] = u[2] / r;
-  p[3] = u[3] / r;
-  double pr;
-  if (ph.mhd)
-    pr = ((u[4] - (0.5 * r) * ((p[1] * p[1] + p[2] * p[2]) + p[3] * p[3]))
-          - 0.5 * ((u[5] * u[5] + u[6] * u[6]) + u[7] * u[7])) / ph.cv;
-  else
-    pr = (u[4] - (0.5 * r) * ((p[1] * p[1] + p[2] * p[2]) + p[3] * p[3])) / ph.cv;
-  p[4] = dmax(pr, 1e-16);
-  p[5] = u[5];
-  p[6] = u[6];
-  p[7] = u[7];
+  gm_u2prim(ph.cv, ph.mhd, u, p);
 }
 
 // prim2u, src/hydro_core.f90:331-383

@@ -18,7 +18,7 @@ All arithmetic of the step runs on the GPU through libguacho_amd.so.
 from __future__ import annotations
 
 from dataclasses import dataclass, field
-from typing import Callable, List, Optional, Sequence
+from typing import Callable, List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -52,6 +52,7 @@ class Parameters:
     pmhd: bool = False
     enable_flux_cd: bool = True
     eight_wave: bool = False
+    user_source_terms: bool = False
     cfl: float = 0.4
     tmax: float = 0.5
     dtprint: float = 0.1
@@ -132,15 +133,22 @@ def get_timestep(dev: _capi.Subdomain, prm: Parameters, g: Globals, n_iter: int 
 
 def run(prm: Parameters, g: Globals, device: int = 0,
         write_output: Optional[Callable[[int, Globals, _capi.Subdomain], None]] = None,
-        max_steps: Optional[int] = None, log: Optional[Callable[[str], None]] = None) -> Globals:
+        max_steps: Optional[int] = None, log: Optional[Callable[[str], None]] = None,
+        user_sources: Optional[Tuple[str, str]] = None) -> Globals:
     """The time loop of src/main.f90:70-125 with the step on the GPU.
 
     ``g.u`` holds the initial conditions (ghosts included).  boundaryI and
     calcprim at start-up run on the device.  ``write_output(itprint, g, dev)``
     is called at t=0 and at every dump, as write_output is in the reference.
+    ``user_sources`` = (code object, kernel) is the device form of the
+    problem's get_user_source_terms (needed when user_source_terms is set).
     Returns ``g`` with ``u`` downloaded at the end.
     """
+    if prm.user_source_terms and user_sources is None:
+        raise ValueError("user_source_terms needs the device plugin: pass user_sources=(code_object, kernel)")
     with _capi.Subdomain(device_params(prm, g, device)) as dev:
+        if prm.user_source_terms:
+            dev.set_user_sources(*user_sources)
         dev.upload_u(g.u)
         dev.boundary_u()                                     # main.f90:76
         if write_output is not None:                         # main.f90:84-87
@@ -152,6 +160,7 @@ def run(prm: Parameters, g: Globals, device: int = 0,
             g.dt_CFL = dt
             if log is not None:
                 log(f"Iteration {g.currentIteration} | time: {g.time:12.3e} | dt: {dt:12.3e} | tprint: {g.tprint:12.3e}")
+            dev.set_time(g.time)                             # globals::time during tstep
             dev.tstep(dt)                                    # main.f90:106
             g.time = g.time + dt                             # main.f90:108
             if dump:                                         # main.f90:110-121

@@ -48,7 +48,7 @@ contains
     p%mhd = merge(1, 0, mhd)
     p%pmhd = merge(1, 0, pmhd)
     p%enable_flux_cd = merge(1, 0, enable_flux_cd)
-    p%eight_wave = merge(1, 0, eight_wave .or. user_source_terms .or. radiation_pressure)
+    p%eight_wave = merge(1, 0, eight_wave)
     p%precision = GM_PRECISION_FP64
     p%device = dev
     p%reserved = 0
@@ -60,9 +60,30 @@ contains
       print '(a)', 'guacho_amd: cooling, conduction, diffuse radiation and non-adiabatic EOS are not built'
       stop 1
     end if
+    if (radiation_pressure) then
+      print '(a)', 'guacho_amd: radiation pressure (a PASSIVES source term) is not built'
+      stop 1
+    end if
     call gm_check(gm_create(p, gm_ctx), 'gm_create')
+    if (user_source_terms) call load_user_sources()
     call gm_check(gm_upload_u(gm_ctx, u), 'gm_upload_u')
   end subroutine gm_device_init
+
+  !> user_source_terms: get_user_source_terms runs on the device as a
+  !> plugin kernel; GUACHO_USER_SOURCES names it as <code object>:<kernel>
+  subroutine load_user_sources()
+    character(len=1024) :: spec
+    integer :: n, st, colon
+    call get_environment_variable('GUACHO_USER_SOURCES', spec, n, st)
+    colon = index(spec(1:max(n, 0)), ':', back=.true.)
+    if (st /= 0 .or. n == 0 .or. colon < 2 .or. colon == n) then
+      print '(a)', 'guacho_amd: user_source_terms needs the device plugin: '// &
+                   'set GUACHO_USER_SOURCES=<code object>:<kernel>'
+      stop 1
+    end if
+    call gm_check(gm_set_user_sources(gm_ctx, spec(1:colon-1)//c_null_char, spec(colon+1:n)//c_null_char), &
+                  'gm_set_user_sources')
+  end subroutine load_user_sources
 
   subroutine gm_device_finalize()
     integer(c_int) :: rc
@@ -104,6 +125,7 @@ contains
   end subroutine get_timestep
 
   subroutine tstep()
+    call gm_check(gm_set_time(gm_ctx, time), 'gm_set_time')
     call gm_check(gm_tstep(gm_ctx, dt_CFL), 'gm_tstep')
   end subroutine tstep
 

@@ -95,6 +95,19 @@ module guacho_capi
       type(c_ptr), value :: ctx
       real(c_double), intent(out) :: divb(*)
     end function gm_download_divb
+
+    integer(c_int) function gm_set_time(ctx, time) bind(C, name='gm_set_time')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: time
+    end function gm_set_time
+
+    ! code_object and kernel: NUL-terminated strings
+    integer(c_int) function gm_set_user_sources(ctx, code_object, kernel) bind(C, name='gm_set_user_sources')
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: code_object(*), kernel(*)
+    end function gm_set_user_sources
   end interface
 
 contains

@@ -71,7 +71,7 @@ typedef struct gm_params {
   int32_t mhd;            /* full MHD */
   int32_t pmhd;           /* passive B (HLLC) */
   int32_t enable_flux_cd; /* flux-CD div B control */
-  int32_t eight_wave;     /* must be 0 (Powell sources not built) */
+  int32_t eight_wave;     /* Powell 8-wave div B source terms (src/sources.f90:124-175) */
   int32_t precision;      /* GM_PRECISION_* */
   int32_t device;         /* HIP device ordinal */
   int32_t reserved;       /* must be 0 */
@@ -139,12 +139,27 @@ int gm_download_divb(gm_ctx *ctx, double *divb);
  * kernel on the context's stream.  gm_kernel_times returns, per kernel
  * class, the accumulated milliseconds and launch count since the last
  * reset, and the kernel names (static strings). */
-#define GM_NKCLASS 8
+#define GM_NKCLASS 9
 int gm_set_timing(gm_ctx *ctx, int enable);
 int gm_kernel_times(gm_ctx *ctx, double *ms, int64_t *launches, const char **names);
 
 /* Device memory held by the context, in bytes. */
 int gm_device_bytes(const gm_ctx *ctx, int64_t *bytes);
+
+/* ---- user plugins (guacho_amd_user.h) -------------------------------- */
+/* The simulation time the user hooks see (the reference's globals::time,
+ * which its driver advances after each tstep, src/main.f90:94-125). */
+int gm_set_time(gm_ctx *ctx, double time);
+
+/* User source terms: replaces get_user_source_terms (called from source(),
+ * src/sources.f90:206, with user_source_terms = .true.).  Loads the gfx950
+ * code object at `code_object` and its kernel `kernel` (signature in
+ * guacho_amd_user.h); each stage of gm_tstep zeroes s, launches the kernel
+ * over the interior cells, adds the 8-wave terms when eight_wave is set and
+ * applies up = up + dt*s (src/hydro_solver.f90:115-121).  code_object NULL
+ * removes the plugin.  GM_ERR_CONFIG if the file or kernel cannot be
+ * loaded. */
+int gm_set_user_sources(gm_ctx *ctx, const char *code_object, const char *kernel);
 
 /* ---- batches of independent boxes ----------------------------------- */
 /* No counterpart in the reference (one domain per run there).  A batch

@@ -1,0 +1,113 @@
+/* guacho_amd_user.h -- device-side interface of user plugins.
+ *
+ * The reference's user extension points are Fortran routines in the
+ * problem's user_mod (OT/user_mod.f90:77-122, EXO/user_mod.f90:131-206):
+ *   get_user_source_terms(pp, s, i, j, k)  -- called per interior cell from
+ *       source() (src/sources.f90:206) inside step(), s accumulated on zero;
+ *   impose_user_bc(u, order)               -- called at the end of boundaryI
+ *       (order 1, on u) and boundaryII (order 2, on up)
+ *       (src/boundaries.f90:245, 508).
+ * Here each is a HIP kernel in a code object the user builds for gfx950
+ * (hipcc --genco --offload-arch=gfx950 -I include plugin.hip -o plugin.hsaco)
+ * and registers with gm_set_user_sources / gm_set_user_bc.  The library
+ * launches it on its own stream with one argument, a gm_user_args by value,
+ * over ceil(nx*ny*nz / 256) workgroups of 256 threads:
+ *
+ *   extern "C" __global__ void my_sources(gm_user_args a) {
+ *     int i, j, k;
+ *     if (!gm_user_cell(a, blockIdx.x * 256 + threadIdx.x, i, j, k)) return;
+ *     const int64_t c = gm_user_index(a, i, j, k);
+ *     double u[8], pp[8];
+ *     gm_user_load(a, c, u);
+ *     gm_user_u2prim(a, u, pp);              // primit(:, i, j, k)
+ *     a.s[2 * a.ncell + c] -= pp[0] * g;      // s(3) = s(3) - rho g
+ *   }
+ *
+ * Indices are the reference's: interior cells i = 1..nx (likewise j, k),
+ * two ghost layers (-1..n+2).  Variables are 0-based here: q = 0..7 holds
+ * the reference's 1..8 (rho, rho u, rho v, rho w, E, Bx, By, Bz).
+ */
+#ifndef GUACHO_AMD_USER_H
+#define GUACHO_AMD_USER_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef struct gm_user_args {
+  int32_t nx, ny, nz;      /* interior cells */
+  int32_t NX, NY, NZ;      /* padded extents, n + 4 */
+  int64_t ncell;           /* NX * NY * NZ */
+  double dx, dy, dz;
+  double cv, gamma;
+  double time;             /* simulation time of the step (gm_set_time) */
+  double dt;               /* sources: dt of the stage (predictor dt/2); hooks: 0 */
+  int32_t stage;           /* sources: 1 predictor, 2 corrector; hooks: order 1 or 2 */
+  int32_t precision;       /* 0 (FP64): u is double*; 1 (mixed): u is float* */
+  int32_t mhd, pmhd;
+  int32_t bc[6];
+  void *u;                 /* sources: the stage's input state (read only);
+                              hooks: the state whose ghosts were just filled */
+  double *s;               /* sources: 8 x ncell, zero on entry (interior cells
+                              are read back); hooks: NULL */
+  int64_t reserved[4];
+} gm_user_args;
+
+#ifdef __HIPCC__
+
+/* Interior cell of linear index t (x fastest); false past the last cell. */
+__device__ __forceinline__ bool gm_user_cell(const gm_user_args &a, int64_t t, int &i, int &j, int &k) {
+  const int64_t n = (int64_t)a.nx * a.ny * a.nz;
+  if (t < 0 || t >= n) return false;
+  i = (int)(t % a.nx) + 1;
+  const int64_t r = t / a.nx;
+  j = (int)(r % a.ny) + 1;
+  k = (int)(r / a.ny) + 1;
+  return true;
+}
+
+/* Storage index of cell (i, j, k), ghosts included (i in -1..nx+2). */
+__device__ __forceinline__ int64_t gm_user_index(const gm_user_args &a, int i, int j, int k) {
+  return ((int64_t)(k + 1) * a.NY + (j + 1)) * a.NX + (i + 1);
+}
+
+/* Variable q of cell c as stored (structure of arrays: u + q * ncell). */
+__device__ __forceinline__ double gm_user_get(const gm_user_args &a, int64_t c, int q) {
+  return a.precision ? (double)((const float *)a.u)[q * a.ncell + c] : ((const double *)a.u)[q * a.ncell + c];
+}
+
+__device__ __forceinline__ void gm_user_set(const gm_user_args &a, int64_t c, int q, double v) {
+  if (a.precision) ((float *)a.u)[q * a.ncell + c] = (float)v;
+  else ((double *)a.u)[q * a.ncell + c] = v;
+}
+
+__device__ __forceinline__ void gm_user_load(const gm_user_args &a, int64_t c, double u[8]) {
+  for (int q = 0; q < 8; q++) u[q] = gm_user_get(a, c, q);
+}
+
+/* u2prim of the reference (src/hydro_core.f90:46-129, EOS_ADIABATIC):
+ * density floor 1e-15, pressure floor 1e-16; the library's own primitive
+ * recovery is this function. */
+__device__ __forceinline__ void gm_u2prim(double cv, int mhd, const double u[8], double p[8]) {
+  const double r = (u[0] > 1e-15) ? u[0] : 1e-15;
+  p[0] = r;
+  p[1] = u[1] / r;
+  p[2] = u[2] / r;
+  p[3] = u[3] / r;
+  double pr;
+  if (mhd)
+    pr = ((u[4] - (0.5 * r) * ((p[1] * p[1] + p[2] * p[2]) + p[3] * p[3]))
+          - 0.5 * ((u[5] * u[5] + u[6] * u[6]) + u[7] * u[7])) / cv;
+  else
+    pr = (u[4] - (0.5 * r) * ((p[1] * p[1] + p[2] * p[2]) + p[3] * p[3])) / cv;
+  p[4] = (pr > 1e-16) ? pr : 1e-16;
+  p[5] = u[5];
+  p[6] = u[6];
+  p[7] = u[7];
+}
+
+__device__ __forceinline__ void gm_user_u2prim(const gm_user_args &a, const double u[8], double p[8]) {
+  gm_u2prim(a.cv, a.mhd, u, p);
+}
+
+#endif /* __HIPCC__ */
+#endif /* GUACHO_AMD_USER_H */

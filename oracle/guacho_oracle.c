@@ -840,7 +840,58 @@ void orc_step(orc_state *s, double dt) {
                                    - (0.5 * dtdx) * (EAT(s, 1, i + 1, j, k) - EAT(s, 1, i - 1, j, k)))
                                   + (0.5 * dtdy) * (EAT(s, 0, i, j + 1, k) - EAT(s, 0, i, j - 1, k));
         }
+        if (p->eight_wave || p->user_src) {                          /* :115-121 */
+          double sv[NEQ];
+          orc_source(s, i, j, k, &AT(s->primit, s, 0, i, j, k), sv);
+          for (int q = 0; q < NEQ; q++) AT(up, s, q, i, j, k) = AT(up, s, q, i, j, k) + dt * sv[q];
+        }
       }
+}
+
+/* divergence_B, sources.f90:112-122: central differences of the
+ * primitives' field, ((x + y) + z) */
+static double divergence_b(const orc_state *s, int i, int j, int k) {
+  const orc_params *p = &s->p;
+  const double *pm = s->primit;
+  return ((AT(pm, s, 5, i + 1, j, k) - AT(pm, s, 5, i - 1, j, k)) / (2.0 * p->dx)
+          + (AT(pm, s, 6, i, j + 1, k) - AT(pm, s, 6, i, j - 1, k)) / (2.0 * p->dy))
+         + (AT(pm, s, 7, i, j, k + 1) - AT(pm, s, 7, i, j, k - 1)) / (2.0 * p->dz);
+}
+
+/* The checker's get_user_source_terms (oracle/ref/user_mod.f90): uniform
+ * gravity along -y and a softened point mass at the box centre, in the
+ * Fortran's evaluation order.  Test infrastructure: a user's source terms
+ * are user code, supplied to the library as a device plugin. */
+static void user_gravity(const orc_state *s, int i, int j, int k, const double *pp, double *sv) {
+  const orc_params *p = &s->p;
+  const double g0 = 0.5, gm = 0.02, soft = 0.01;
+  const double xc = ((double)i - 0.5) * p->dx - 0.5;
+  const double yc = ((double)j - 0.5) * p->dy - 0.5;
+  const double zc = ((double)k - 0.5) * p->dz - 0.5;
+  const double r2 = ((xc * xc + yc * yc) + zc * zc) + soft;
+  const double r3 = r2 * sqrt(r2);
+  sv[2] = sv[2] - pp[0] * g0;
+  sv[4] = sv[4] - (pp[0] * pp[2]) * g0;
+  sv[1] = sv[1] - ((pp[0] * gm) * xc) / r3;
+  sv[2] = sv[2] - ((pp[0] * gm) * yc) / r3;
+  sv[3] = sv[3] - ((pp[0] * gm) * zc) / r3;
+  sv[4] = sv[4] - ((pp[0] * gm) * ((pp[1] * xc + pp[2] * yc) + pp[3] * zc)) / r3;
+}
+
+/* source, sources.f90:188-220: user terms (:206), then
+ * divbcorr_8w_source (:137-168); no radiation pressure */
+void orc_source(const orc_state *s, int i, int j, int k, const double *pp, double *sv) {
+  for (int q = 0; q < NEQ; q++) sv[q] = 0.;                          /* :200 */
+  if (s->p.user_src) user_gravity(s, i, j, k, pp, sv);
+  if (!s->p.eight_wave) return;
+  const double divb = divergence_b(s, i, j, k);                      /* :149 */
+  sv[1] = sv[1] - divb * pp[5];                                      /* :153-155 */
+  sv[2] = sv[2] - divb * pp[6];
+  sv[3] = sv[3] - divb * pp[7];
+  sv[4] = sv[4] - divb * ((pp[1] * pp[5] + pp[2] * pp[6]) + pp[3] * pp[7]);  /* :158 */
+  sv[5] = sv[5] - divb * pp[1];                                      /* :161-163 */
+  sv[6] = sv[6] - divb * pp[2];
+  sv[7] = sv[7] - divb * pp[3];
 }
 
 /* viscous_copy, hydro_solver.f90:47-65 */

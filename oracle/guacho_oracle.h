@@ -38,6 +38,8 @@ typedef struct orc_params {
   int bc[6];              /* left, right, bottom, top, out, in */
   int mhd, pmhd;          /* physics flags (parameters.f90:48-49) */
   int flux_cd;            /* enable_flux_cd (parameters.f90:63) */
+  int eight_wave;         /* Powell 8-wave source terms (parameters.f90:61) */
+  int user_src;           /* the checker's user source (oracle/ref/user_mod.f90) */
   int nthreads;           /* OpenMP threads for the sweeps (<=0: runtime default) */
 } orc_params;
 
@@ -67,6 +69,10 @@ void orc_step(orc_state *s, double dt);
 void orc_viscous_copy(orc_state *s);
 int  orc_tstep(orc_state *s, double dt);      /* one full tstep() */
 double orc_cfl_dtp(const orc_state *s);       /* min loop of get_timestep */
+/* source(i,j,k,prim,s) of sources.f90:188-220: s = 0, the checker's user
+ * source when user_src, then divbcorr_8w_source from the state's primit
+ * when eight_wave */
+void orc_source(const orc_state *s, int i, int j, int k, const double *prim, double *sv);
 void orc_divb(const orc_state *s, double *divb);   /* divB(nx,ny,nz) of u */
 
 /* get_timestep host logic, hydro_core.f90:677-694 */

@@ -42,7 +42,7 @@ class OrcParams(C.Structure):
         ("cv", C.c_double), ("gamma", C.c_double), ("eta", C.c_double),
         ("solver", C.c_int), ("limiter", C.c_int), ("bc", C.c_int * 6),
         ("mhd", C.c_int), ("pmhd", C.c_int), ("flux_cd", C.c_int),
-        ("nthreads", C.c_int),
+        ("eight_wave", C.c_int), ("user_src", C.c_int), ("nthreads", C.c_int),
     ]
 
 
@@ -63,6 +63,8 @@ class Config:
     mhd: bool = True
     pmhd: bool = False
     enable_flux_cd: bool = True
+    eight_wave: bool = False
+    user_src: bool = False          # the checker's user gravity (oracle/ref/user_mod.f90)
     cfl: float = 0.4
 
     @property
@@ -78,6 +80,8 @@ class Config:
         for b in range(6):
             p.bc[b] = int(self.bc[b])
         p.mhd, p.pmhd, p.flux_cd = int(self.mhd), int(self.pmhd), int(self.enable_flux_cd)
+        p.eight_wave = int(self.eight_wave)
+        p.user_src = int(self.user_src)
         p.nthreads = nthreads
         return p
 
@@ -109,6 +113,7 @@ def _load_oracle() -> C.CDLL:
     L.orc_tstep.argtypes = [C.c_void_p, C.c_double]
     L.orc_cfl_dtp.argtypes = [C.c_void_p]
     L.orc_cfl_dtp.restype = C.c_double
+    L.orc_source.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _dp]
     L.orc_timestep_finish.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                       C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.orc_u2prim.argtypes = [P, _dp, _dp, C.POINTER(C.c_double)]
@@ -227,6 +232,12 @@ class Oracle:
     def cfl_dtp(self) -> float:
         return self.L.orc_cfl_dtp(self.h)
 
+    def source(self, i: int, j: int, k: int, prim) -> np.ndarray:
+        """source(i,j,k,prim,s) with the 8-wave term, from the state's primit."""
+        sv = np.zeros(NEQ)
+        self.L.orc_source(self.h, i, j, k, np.ascontiguousarray(prim, dtype=np.float64), sv)
+        return sv
+
 
 # ---------------------------------------------------------------------------
 # per-function helpers (8-vectors), used by the vector parity tests
@@ -262,6 +273,13 @@ class OracleFunctions:
         pl = np.array(pl, dtype=np.float64); pr = np.array(pr, dtype=np.float64)
         self.L.orc_limiter(C.byref(self.p), np.ascontiguousarray(pll), pl, pr, np.ascontiguousarray(prr))
         return pl, pr
+
+    def source(self, i, j, k, prim):
+        """source(i,j,k,prim,s) of this build (sources.f90:188-220), from the
+        primit array last set."""
+        sv = np.zeros(NEQ)
+        self.L.ref_source(i, j, k, np.ascontiguousarray(prim, dtype=np.float64), sv)
+        return sv
 
     def riemann(self, pl, pr):
         ff = np.zeros(NEQ)
@@ -319,6 +337,10 @@ class RefLib:
         L.ref_fluxes.argtypes = [C.c_int]
         L.ref_get_timestep.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double,
                                        C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.ref_source.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp]
+        sw = (C.c_int * 2)()
+        L.ref_switches(sw)
+        self.eight_wave, self.user_source_terms = bool(sw[0]), bool(sw[1])
 
     def config(self, dx=None, dy=None, dz=None) -> Config:
         dx = dx if dx is not None else 1.0 / self.nx
@@ -326,7 +348,8 @@ class RefLib:
         dz = dz if dz is not None else 1.0 / self.nz
         return Config(nx=self.nx, ny=self.ny, nz=self.nz, dx=dx, dy=dy, dz=dz, cv=self.cv,
                       riemann_solver=self.riemann_solver, slope_limiter=self.slope_limiter, bc=list(self.bc),
-                      mhd=self.mhd, pmhd=self.pmhd, enable_flux_cd=True, cfl=self.cfl)
+                      mhd=self.mhd, pmhd=self.pmhd, enable_flux_cd=True, cfl=self.cfl,
+                      eight_wave=self.eight_wave, user_src=self.user_source_terms)
 
     def init(self, dx, dy, dz):
         self.L.ref_init(dx, dy, dz)
@@ -379,6 +402,13 @@ class RefLib:
 
     def swap(self, axis, v):
         v = np.array(v, dtype=np.float64); self.L.ref_swap(axis, v); return v
+
+    def source(self, i, j, k, prim):
+        """source(i,j,k,prim,s) of this build (sources.f90:188-220), from the
+        primit array last set."""
+        sv = np.zeros(NEQ)
+        self.L.ref_source(i, j, k, np.ascontiguousarray(prim, dtype=np.float64), sv)
+        return sv
 
     def riemann(self, pl, pr):
         ff = np.zeros(NEQ)

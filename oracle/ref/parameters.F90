@@ -18,7 +18,7 @@ module parameters
   logical, parameter :: pmhd = (REF_PMHD /= 0)
   logical, parameter :: mhd  = (REF_MHD /= 0)
   integer, parameter :: riemann_solver = REF_SOLVER
-  logical, parameter :: eight_wave = .false.
+  logical, parameter :: eight_wave = (REF_EIGHT_WAVE /= 0)
   logical, parameter :: enable_flux_cd = .true.
   logical, parameter :: dump_divb = .false.
   logical, parameter :: out_bin = .false.
@@ -38,7 +38,7 @@ module parameters
   integer, parameter :: th_cond = TC_OFF
   logical, parameter :: tc_saturation = .false.
   logical, parameter :: dif_rad = .false.
-  logical, parameter :: user_source_terms = .false.
+  logical, parameter :: user_source_terms = (REF_USER_SRC /= 0)
   logical, parameter :: radiation_pressure = .false.
 
   integer, parameter :: npas = 0

@@ -20,6 +20,7 @@ module ref_kat
   use hll,  only : prim2fhll, hllfluxes
   use hllc, only : prim2fhllc, hllcfluxes
   use boundaries, only : boundaryI, boundaryII
+  use sources, only : source
   implicit none
 
 contains
@@ -169,5 +170,19 @@ contains
     dt = dtl
     dump = merge(1, 0, dflag)
   end subroutine ref_get_timestep
+
+  ! source(i,j,k,prim,s) (sources.f90:188-220) from the current primit
+  subroutine ref_source(i, j, k, prim, sv) bind(C, name='ref_source')
+    integer(c_int), value :: i, j, k
+    real(c_double), intent(in)  :: prim(neq)
+    real(c_double), intent(out) :: sv(neq)
+    call source(i, j, k, prim, sv)
+  end subroutine ref_source
+
+  ! logical switches of this build: eight_wave, user_source_terms
+  subroutine ref_switches(sw) bind(C, name='ref_switches')
+    integer(c_int), intent(out) :: sw(2)
+    sw = (/ merge(1, 0, eight_wave), merge(1, 0, user_source_terms) /)
+  end subroutine ref_switches
 
 end module ref_kat

@@ -8,11 +8,12 @@ from guacho_amd.host import timestep_clamp
 from oracle.oracle import Config, Oracle
 
 
-def oracle_config(params: _capi.GmParams, cfl: float = 0.4) -> Config:
+def oracle_config(params: _capi.GmParams, cfl: float = 0.4, user_src: bool = False) -> Config:
     return Config(nx=params.nx, ny=params.ny, nz=params.nz, dx=params.dx, dy=params.dy, dz=params.dz,
                   cv=params.cv, riemann_solver=params.riemann_solver, slope_limiter=params.slope_limiter,
                   bc=list(params.bc), mhd=bool(params.mhd), pmhd=bool(params.pmhd),
-                  enable_flux_cd=bool(params.enable_flux_cd), cfl=cfl)
+                  enable_flux_cd=bool(params.enable_flux_cd), cfl=cfl,
+                  eight_wave=bool(params.eight_wave), user_src=user_src)
 
 
 def random_state(rng, nx, ny, nz, cv=1.5, contrast=0.3, vmax=1.5, bmax=1.0):
@@ -29,16 +30,20 @@ def random_state(rng, nx, ny, nz, cv=1.5, contrast=0.3, vmax=1.5, bmax=1.0):
     return u
 
 
-def run_pair(params: _capi.GmParams, u0: np.ndarray, nsteps: int, cfl: float = 0.4, check_each=False):
+def run_pair(params: _capi.GmParams, u0: np.ndarray, nsteps: int, cfl: float = 0.4, check_each=False,
+             user_sources=None):
     """Advance u0 nsteps on both paths with dt from each path's own CFL
-    reduction; asserts the reductions agree bitwise every step.  Returns
-    (u_device, u_oracle, dts)."""
-    O = Oracle(oracle_config(params, cfl))
+    reduction; asserts the reductions agree bitwise every step.
+    user_sources: (code object, kernel) of a plugin whose terms equal the
+    oracle's user source.  Returns (u_device, u_oracle, dts)."""
+    O = Oracle(oracle_config(params, cfl, user_src=user_sources is not None))
     O.set_u(u0)
     O.boundaryI()
     O.calcprim_u()
     dts = []
     with _capi.Subdomain(params) as dev:
+        if user_sources is not None:
+            dev.set_user_sources(*user_sources)
         dev.upload_u(u0)
         dev.boundary_u()
         t = 0.0

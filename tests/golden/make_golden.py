@@ -7,12 +7,16 @@ inputs and the reference's outputs for
   - the slope limiter, u2prim, prim2u, prim2f, cfast, cfastX,
   - one boundaryI + calcprim + first-order sweep on a random state,
   - one boundaryII + calcprim + limited sweep on a random state,
-  - get_timestep's ramp and clamp.
+  - get_timestep's ramp and clamp,
+  - source(i,j,k,prim,s) of sources.f90 at every interior cell of a random
+    primitive array (zero, 8-wave, the checker's user gravity, or both,
+    per the variant's switches).
 The fixtures are data (inputs and expected outputs); tests replay the
 inputs through the C restatement and require identical bits.
 """
 import os
 import sys
+import zlib
 
 import numpy as np
 
@@ -21,6 +25,16 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 from oracle.oracle import RefLib, ref_variants  # noqa: E402
 
 NVEC = 600
+
+# seeds of the fixtures already committed (sorted variant order of the
+# first build); later variants take a seed from their name
+SEEDS = {"hll_umist_closed": 1000, "hllc_superbee_pmhd": 1001, "hlld_minmod_periodic": 1002,
+         "hlld_nolimit_periodic": 1003, "hlld_vanalbada_mixed": 1004, "hlld_woodward_mixed": 1005,
+         "hlle_noaverage_closed": 1006, "hlle_vanleer_outflow": 1007}
+
+
+def seed_of(variant):
+    return SEEDS.get(variant, 2000 + zlib.crc32(variant.encode()) % 100000)
 
 
 def random_prims(rng, n, pmhd=False):
@@ -101,6 +115,18 @@ def make(variant, seed):
             dt, dump = R.get_timestep(it, 10, t, tp)
             ts.append([it, t, tp, dt, float(dump)])
     out["timestep"] = np.array(ts)
+    if R.eight_wave or R.user_source_terms:
+        prim = np.asfortranarray(rng.uniform(0.5, 1.5, R.shape()))
+        prim[1:4] -= 1.0
+        prim[5:8] -= 1.0
+        R.set_array(2, prim)
+        src = np.zeros((R.nx, R.ny, R.nz, 8))
+        for k in range(1, R.nz + 1):
+            for j in range(1, R.ny + 1):
+                for i in range(1, R.nx + 1):
+                    src[i - 1, j - 1, k - 1] = R.source(i, j, k, prim[:, i + 1, j + 1, k + 1])
+        out["src_primit"], out["src_out"] = prim, src
+    out["switches"] = np.array([int(R.eight_wave), int(R.user_source_terms)])
     out["dxyz"] = np.array([dx, dy, dz])
     cfg = np.array([R.nx, R.ny, R.nz, R.riemann_solver, R.slope_limiter, *R.bc, int(R.mhd), int(R.pmhd)])
     out["config_int"] = cfg
@@ -112,8 +138,12 @@ def main():
     vs = ref_variants()
     if not vs:
         raise SystemExit("oracle/_ref is not built (make -C oracle ref)")
-    for n, v in enumerate(vs):
-        np.savez_compressed(os.path.join(HERE, f"ref_{v}.npz"), **make(v, 1000 + n))
+    rewrite = "--all" in sys.argv
+    for v in vs:
+        path = os.path.join(HERE, f"ref_{v}.npz")
+        if os.path.exists(path) and not rewrite:
+            continue
+        np.savez_compressed(path, **make(v, seed_of(v)))
         print("wrote", v)
 
 

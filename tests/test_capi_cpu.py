@@ -51,7 +51,6 @@ def _create(**kw):
     (dict(slope_limiter=9), "slope_limiter"),
     (dict(bc=(4, 4, 3, 3, 3, 3)), "BC_OTHER"),
     (dict(bc=(3, 1, 3, 3, 3, 3)), "periodic"),
-    (dict(eight_wave=True), "eight_wave"),
     (dict(eta=0.1), "eta"),
     (dict(nz=1), "at least 2"),
     (dict(dx=0.0), "dx"),

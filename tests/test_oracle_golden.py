@@ -20,9 +20,11 @@ def _cfg(z):
     ci = z["config_int"]
     cr = z["config_real"]
     dx, dy, dz = z["dxyz"]
+    sw = z["switches"] if "switches" in z else np.zeros(2, dtype=int)
     return Config(nx=int(ci[0]), ny=int(ci[1]), nz=int(ci[2]), dx=dx, dy=dy, dz=dz, cv=float(cr[0]),
                   riemann_solver=int(ci[3]), slope_limiter=int(ci[4]), bc=[int(v) for v in ci[5:11]],
-                  mhd=bool(ci[11]), pmhd=bool(ci[12]), cfl=float(cr[2]))
+                  mhd=bool(ci[11]), pmhd=bool(ci[12]), cfl=float(cr[2]),
+                  eight_wave=bool(sw[0]), user_src=bool(sw[1]))
 
 
 def test_fixtures_present():
@@ -80,6 +82,55 @@ def test_sweeps_match_reference(path):
 
 
 REF = ref_variants()
+
+
+SRC_GOLDEN = [p for p in GOLDEN if "src_out" in np.load(p)]
+
+
+def test_source_fixtures_present():
+    assert len(SRC_GOLDEN) >= 3
+
+
+@pytest.mark.parametrize("path", SRC_GOLDEN, ids=lambda p: os.path.basename(p)[4:-4])
+def test_sources_match_reference(path):
+    """source(i,j,k,prim,s) (sources.f90:188-220): Powell 8-wave terms
+    (:137-168) and the checker's user gravity, at every interior cell."""
+    z = np.load(path)
+    cfg = _cfg(z)
+    O = Oracle(cfg)
+    prim = z["src_primit"]
+    O.set_primit(prim)
+    want = z["src_out"]
+    for k in range(cfg.nz):
+        for j in range(cfg.ny):
+            for i in range(cfg.nx):
+                got = O.source(i + 1, j + 1, k + 1, prim[:, i + 2, j + 2, k + 2])
+                assert np.array_equal(got, want[i, j, k]), (i, j, k, got, want[i, j, k])
+    O.close()
+
+
+def test_eight_wave_closed_form():
+    """The Powell terms against their closed form: B = (x, 2y, -z) has
+    central-difference div B = 1 + 2 - 1 = 2 exactly on a uniform grid, so
+    s = -2 (0, Bx, By, Bz, v.B, vx, vy, vz)."""
+    n = 6
+    cfg = Config(nx=n, ny=n, nz=n, dx=0.25, dy=0.25, dz=0.25, eight_wave=True)
+    O = Oracle(cfg)
+    prim = np.zeros((8, n + 4, n + 4, n + 4), order="F")
+    x = (np.arange(n + 4) - 1.5) * 0.25
+    prim[0] = 1.0
+    prim[1], prim[2], prim[3] = 0.5, -0.25, 0.75
+    prim[4] = 1.0
+    prim[5] = x[:, None, None]
+    prim[6] = 2.0 * x[None, :, None]
+    prim[7] = -x[None, None, :]
+    O.set_primit(prim)
+    for (i, j, k) in ((1, 1, 1), (3, 4, 2), (6, 6, 6)):
+        pp = prim[:, i + 1, j + 1, k + 1]
+        v, B = pp[1:4], pp[5:8]
+        want = -2.0 * np.array([0.0, *B, v @ B, *v])
+        np.testing.assert_allclose(O.source(i, j, k, pp), want, rtol=1e-15, atol=1e-15)
+    O.close()
 
 
 @pytest.mark.skipif(not REF, reason="oracle/_ref not built here (the reference tree is absent)")
