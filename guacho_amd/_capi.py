@@ -23,7 +23,7 @@ EXPORTS = (
     "gm_create", "gm_destroy", "gm_last_error", "gm_upload_u", "gm_download_u",
     "gm_download_primit", "gm_boundary_u", "gm_cfl_dtp", "gm_tstep", "gm_sync",
     "gm_download_divb", "gm_set_timing", "gm_kernel_times", "gm_device_bytes",
-    "gm_set_time", "gm_set_user_sources",
+    "gm_set_time", "gm_set_user_sources", "gm_set_user_bc",
     "gm_batch_create", "gm_batch_destroy", "gm_batch_last_error", "gm_batch_box_bytes",
     "gm_batch_free_bytes", "gm_batch_add", "gm_batch_remove", "gm_batch_advance", "gm_batch_sync",
     "gm_batch_clock", "gm_batch_download_u", "gm_batch_tuning", "gm_batch_active",
@@ -90,6 +90,7 @@ def lib() -> C.CDLL:
     L.gm_device_bytes.argtypes = [vp, C.POINTER(C.c_int64)]
     L.gm_set_time.argtypes = [vp, C.c_double]
     L.gm_set_user_sources.argtypes = [vp, C.c_char_p, C.c_char_p]
+    L.gm_set_user_bc.argtypes = [vp, C.c_char_p, C.c_char_p]
     P = C.POINTER(GmParams)
     L.gm_batch_create.argtypes = [C.POINTER(GmBatchParams), C.POINTER(vp)]
     L.gm_batch_destroy.argtypes = [vp]
@@ -218,6 +219,13 @@ class Subdomain:
         co = code_object.encode() if code_object else None
         kn = kernel.encode() if kernel else None
         self._check(self.L.gm_set_user_sources(self.h, co, kn))
+
+    def set_user_bc(self, code_object: Optional[str], kernel: Optional[str] = None) -> None:
+        """Load the user's boundary hook (impose_user_bc) from a gfx950 code
+        object (None removes it); ghosts then live in memory."""
+        co = code_object.encode() if code_object else None
+        kn = kernel.encode() if kernel else None
+        self._check(self.L.gm_set_user_bc(self.h, co, kn))
 
     def kernel_times(self) -> Dict[str, tuple]:
         ms = (C.c_double * GM_NKCLASS)()

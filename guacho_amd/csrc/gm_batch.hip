@@ -106,6 +106,7 @@ gm::Geom geom_of(const gm_params& p) {
   g.NX = p.nx + 4; g.NY = p.ny + 4; g.NZ = p.nz + 4;
   g.ncell = (long long)g.NX * g.NY * g.NZ;
   for (int k = 0; k < 6; k++) g.bc[k] = p.bc[k];
+  g.gmem = 0;
   return g;
 }
 
@@ -300,12 +301,12 @@ void ghost_fill_t(gm_batch* b, const Slot& s) {
   S* U = (S*)s.U;
   for (int a = 0; a < 3; a++)
     if (g.bc[2 * a] == GM_BC_PERIODIC) {
-      gm::launch_ghost1(b->stream, g, U, a, 0, GM_BC_PERIODIC);
-      gm::launch_ghost1(b->stream, g, U, a, 1, GM_BC_PERIODIC);
+      gm::launch_ghost(b->stream, g, U, a, 0, GM_BC_PERIODIC, 1);
+      gm::launch_ghost(b->stream, g, U, a, 1, GM_BC_PERIODIC, 1);
     }
   for (int kind : {GM_BC_CLOSED, GM_BC_OUTFLOW})
     for (int f = 0; f < 6; f++)
-      if (g.bc[f] == kind) gm::launch_ghost1(b->stream, g, U, f / 2, f % 2, kind);
+      if (g.bc[f] == kind) gm::launch_ghost(b->stream, g, U, f / 2, f % 2, kind, 1);
 }
 
 }  // namespace
@@ -405,6 +406,9 @@ int gm_batch_add(gm_batch* b, const gm_params* p, const double* u, double cfl, d
   std::string m;
   int rc = gm::check_params(p, m);
   if (rc != GM_OK) return bfail(b, rc, m);
+  for (int f = 0; f < 6; f++)
+    if (p->bc[f] == GM_BC_OTHER)
+      return bfail(b, GM_ERR_CONFIG, "BC_OTHER (user boundary hooks) is for single-box contexts (gm_set_user_bc)");
   if (p->precision != b->bp.precision || p->riemann_solver != b->bp.riemann_solver ||
       (p->enable_flux_cd != 0) != (b->bp.enable_flux_cd != 0))
     return bfail(b, GM_ERR_CONFIG, "box precision, solver and flux-CD must match the batch");

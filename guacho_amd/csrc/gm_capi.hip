@@ -60,6 +60,8 @@ struct gm_ctx {
   double time = 0.0;                    // simulation time the user hooks see
   hipModule_t usrc_mod = nullptr;       // user source plugin
   hipFunction_t usrc_fn = nullptr;
+  hipModule_t ubc_mod = nullptr;        // user boundary plugin (ghosts then live in memory)
+  hipFunction_t ubc_fn = nullptr;
   gm::Shape shape = {256, 16};  // tile width, z-pencil length
   int timing = 0;
   struct Rec { int cls; hipEvent_t a, b; };
@@ -153,9 +155,7 @@ int check_params(const gm_params* p, std::string& msg) {
   if (p->slope_limiter < GM_LIMITER_NO_AVERAGE || p->slope_limiter > GM_LIMITER_SUPERBEE)
     return fail(nullptr, GM_ERR_CONFIG, "slope_limiter out of range");
   for (int b = 0; b < 6; b++) {
-    if (p->bc[b] == GM_BC_OTHER)
-      return fail(nullptr, GM_ERR_CONFIG, "BC_OTHER / bc_user needs the user's impose_user_bc hook on the host; not supported on device");
-    if (p->bc[b] < GM_BC_OUTFLOW || p->bc[b] > GM_BC_PERIODIC)
+    if (p->bc[b] < GM_BC_OUTFLOW || p->bc[b] > GM_BC_OTHER)
       return fail(nullptr, GM_ERR_CONFIG, "boundary kind out of range");
   }
   for (int a = 0; a < 3; a++)
@@ -189,6 +189,7 @@ void release(gm_ctx* c) {
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->usrc_mod) (void)hipModuleUnload(c->usrc_mod);
+  if (c->ubc_mod) (void)hipModuleUnload(c->ubc_mod);
   if (c->dscal) (void)hipFree(c->dscal);
   if (c->hscal) (void)hipHostFree(c->hscal);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -197,28 +198,58 @@ void release(gm_ctx* c) {
 
 int set_device(gm_ctx* c) { return hip_check(c, hipSetDevice(c->p.device), "hipSetDevice"); }
 
+// Arguments of a user plugin launch.
+gm_user_args user_args(const gm_ctx* c, void* u, double* sv, double dt, int stage) {
+  gm_user_args a{};
+  a.nx = c->g.nx; a.ny = c->g.ny; a.nz = c->g.nz;
+  a.NX = c->g.NX; a.NY = c->g.NY; a.NZ = c->g.NZ;
+  a.ncell = c->g.ncell;
+  a.dx = c->p.dx; a.dy = c->p.dy; a.dz = c->p.dz;
+  a.cv = c->p.cv; a.gamma = c->p.gamma;
+  a.time = c->time;
+  a.dt = dt;
+  a.stage = stage;
+  a.precision = c->esize == 8 ? GM_PRECISION_FP64 : GM_PRECISION_MIXED;
+  a.mhd = c->p.mhd; a.pmhd = c->p.pmhd;
+  for (int f = 0; f < 6; f++) a.bc[f] = c->p.bc[f];
+  a.u = u;
+  a.s = sv;
+  return a;
+}
+
+void launch_plugin(gm_ctx* c, hipFunction_t fn, gm_user_args a, long long items) {
+  void* params[] = {&a};
+  (void)hipModuleLaunchKernel(fn, (unsigned)((items + 255) / 256), 1, 1, 256, 1, 1, 0, c->stream, params, nullptr);
+}
+
+// The user's impose_user_bc(u, order) over every stored cell of u.
+void user_bc(gm_ctx* c, void* u, int order) {
+  if (c->ubc_fn) launch_plugin(c, c->ubc_fn, user_args(c, u, nullptr, 0.0, order), c->g.ncell);
+}
+
 // boundaryI ghost layer on U, in the reference's serial order: periodic
-// x, y, z; then closed left..in; then outflow left..in.
+// x, y, z; then closed left..in; then outflow left..in; then the user hook
+// (src/boundaries.f90:245).
 template <typename S>
-void fill_ghost1_t(gm_ctx* c) {
+void fill_ghost_t(gm_ctx* c, S* U, int layers) {
   const gm::Geom& g = c->g;
-  S* U = (S*)c->U;
   timed(c, KC_GHOST, [&] {
     for (int a = 0; a < 3; a++)
       if (g.bc[2 * a] == GM_BC_PERIODIC) {
-        gm::launch_ghost1(c->stream, g, U, a, 0, GM_BC_PERIODIC);
-        gm::launch_ghost1(c->stream, g, U, a, 1, GM_BC_PERIODIC);
+        gm::launch_ghost(c->stream, g, U, a, 0, GM_BC_PERIODIC, layers);
+        gm::launch_ghost(c->stream, g, U, a, 1, GM_BC_PERIODIC, layers);
       }
     for (int kind : {GM_BC_CLOSED, GM_BC_OUTFLOW})
       for (int f = 0; f < 6; f++)
-        if (g.bc[f] == kind) gm::launch_ghost1(c->stream, g, U, f / 2, f % 2, kind);
+        if (g.bc[f] == kind) gm::launch_ghost(c->stream, g, U, f / 2, f % 2, kind, layers);
+    user_bc(c, U, layers);
   });
-  c->ghosts_dirty = false;
 }
 
 void fill_ghost1(gm_ctx* c) {
-  if (c->esize == 8) fill_ghost1_t<double>(c);
-  else fill_ghost1_t<float>(c);
+  if (c->esize == 8) fill_ghost_t<double>(c, (double*)c->U, 1);
+  else fill_ghost_t<float>(c, (float*)c->U, 1);
+  c->ghosts_dirty = false;
 }
 
 // this context's box, as the kernels see it
@@ -236,6 +267,7 @@ gm::Box<S> box_of(const gm_ctx* c) {
   b.dz = c->p.dz;
   b.slot = 0;
   b.src = (c->p.eight_wave ? gm::SRC_EIGHT_WAVE : 0) | (c->usrc_fn ? gm::SRC_USER : 0);
+  b.g.gmem = c->ubc_fn ? 1 : 0;
   return b;
 }
 
@@ -243,25 +275,9 @@ gm::Box<S> box_of(const gm_ctx* c) {
 // zeroed s (the W block, free once the stage's z kernel has run).
 template <typename S>
 void user_sources_t(gm_ctx* c, const gm::Box<S>& b, double dt, int stage) {
-  gm_user_args a{};
-  a.nx = c->g.nx; a.ny = c->g.ny; a.nz = c->g.nz;
-  a.NX = c->g.NX; a.NY = c->g.NY; a.NZ = c->g.NZ;
-  a.ncell = c->g.ncell;
-  a.dx = c->p.dx; a.dy = c->p.dy; a.dz = c->p.dz;
-  a.cv = c->p.cv; a.gamma = c->p.gamma;
-  a.time = c->time;
-  a.dt = dt;
-  a.stage = stage;
-  a.precision = c->esize == 8 ? GM_PRECISION_FP64 : GM_PRECISION_MIXED;
-  a.mhd = c->p.mhd; a.pmhd = c->p.pmhd;
-  for (int f = 0; f < 6; f++) a.bc[f] = c->p.bc[f];
-  a.u = stage == 1 ? (void*)b.U : (void*)b.Us;
-  a.s = b.W;
   (void)hipMemsetAsync(b.W, 0, (size_t)c->g.ncell * 8 * sizeof(double), c->stream);
-  const long long n = (long long)a.nx * a.ny * a.nz;
-  void* params[] = {&a};
-  (void)hipModuleLaunchKernel(c->usrc_fn, (unsigned)((n + 255) / 256), 1, 1, 256, 1, 1, 0, c->stream, params,
-                              nullptr);
+  void* u = stage == 1 ? (void*)b.U : (void*)b.Us;
+  launch_plugin(c, c->usrc_fn, user_args(c, u, b.W, dt, stage), (long long)c->g.nx * c->g.ny * c->g.nz);
 }
 
 template <typename S>
@@ -290,10 +306,14 @@ void tstep_t(gm_ctx* c, double dt) {
   stage_sources_t(c, b, dtm, 1);
   // corrector: limited fluxes of U*, U^{n+1} = U^n - dt div F (+ dt s), in
   // place; viscous_copy with eta = 0 is the identity on values
+  // boundaryII on U* (src/hydro_solver.f90:169) when ghosts live in memory
+  if (b.g.gmem) fill_ghost_t<S>(c, b.Us, 2);
   timed(c, KC_XY2, [&] { gm::launch_xy(s, b, dt, 2, solver, fcd, err); });
   timed(c, KC_Z2, [&] { gm::launch_z(s, b, dt, 2, solver, fcd, err, sh); });
   if (fcd) timed(c, KC_BFIELD, [&] { gm::launch_bfield(s, b, dt, 2, sh); });
   stage_sources_t(c, b, dt, 2);
+  // boundaryI on U^{n+1} (src/hydro_solver.f90:216) when ghosts live in memory
+  if (b.g.gmem) fill_ghost_t<S>(c, b.U, 1);
 }
 
 template <typename S>
@@ -332,6 +352,7 @@ int gm_create(const gm_params* params, gm_ctx** out) {
   g.NX = g.nx + 4; g.NY = g.ny + 4; g.NZ = g.nz + 4;
   g.ncell = (long long)g.NX * g.NY * g.NZ;
   for (int b = 0; b < 6; b++) g.bc[b] = params->bc[b];
+  g.gmem = 0;
   c->ph.cv = params->cv;
   c->ph.gamma = params->gamma;
   c->ph.mhd = params->mhd ? 1 : 0;
@@ -478,9 +499,13 @@ int gm_tstep(gm_ctx* c, double dt) {
   if (!std::isfinite(dt)) return fail(c, GM_ERR_ARG, "dt is not finite");
   int rc = set_device(c);
   if (rc) return rc;
+  if (!c->ubc_fn)
+    for (int f = 0; f < 6; f++)
+      if (c->p.bc[f] == GM_BC_OTHER)
+        return fail(c, GM_ERR_CONFIG, "BC_OTHER faces need a user boundary plugin (gm_set_user_bc)");
   if (c->esize == 8) tstep_t<double>(c, dt);
   else tstep_t<float>(c, dt);
-  c->ghosts_dirty = true;
+  c->ghosts_dirty = c->ubc_fn == nullptr;  // with a user hook the step ends with boundaryI
   GM_HIP(c, hipGetLastError());
   return GM_OK;
 }
@@ -537,6 +562,7 @@ int gm_set_user_sources(gm_ctx* c, const char* code_object, const char* kernel) 
   if (rc) return rc;
   GM_HIP(c, hipStreamSynchronize(c->stream));
   if (c->usrc_mod) (void)hipModuleUnload(c->usrc_mod);
+  if (c->ubc_mod) (void)hipModuleUnload(c->ubc_mod);
   c->usrc_mod = nullptr;
   c->usrc_fn = nullptr;
   if (!code_object) return GM_OK;
@@ -554,6 +580,35 @@ int gm_set_user_sources(gm_ctx* c, const char* code_object, const char* kernel) 
   }
   c->usrc_mod = m;
   c->usrc_fn = f;
+  return GM_OK;
+}
+
+int gm_set_user_bc(gm_ctx* c, const char* code_object, const char* kernel) {
+  if (!c) return GM_ERR_ARG;
+  int rc = set_device(c);
+  if (rc) return rc;
+  GM_HIP(c, hipStreamSynchronize(c->stream));
+  if (c->ubc_mod) (void)hipModuleUnload(c->ubc_mod);
+  c->ubc_mod = nullptr;
+  c->ubc_fn = nullptr;
+  if (!code_object) return GM_OK;
+  if (!kernel) return fail(c, GM_ERR_ARG, "kernel name is NULL");
+  hipModule_t m = nullptr;
+  if (hipModuleLoad(&m, code_object) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(c, GM_ERR_CONFIG, std::string("cannot load code object ") + code_object);
+  }
+  hipFunction_t f = nullptr;
+  if (hipModuleGetFunction(&f, m, kernel) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipModuleUnload(m);
+    return fail(c, GM_ERR_CONFIG, std::string("no kernel ") + kernel + " in " + code_object);
+  }
+  c->ubc_mod = m;
+  c->ubc_fn = f;
+  // from here on the kernels read U's ghosts as stored: materialise them
+  if (c->ghosts_dirty) fill_ghost1(c);
+  GM_HIP(c, hipGetLastError());
   return GM_OK;
 }
 

@@ -73,15 +73,22 @@ __device__ __forceinline__ int map_ghost(int i, int n, int lo, int hi, bool& fli
   return i;
 }
 
+// Source index of ghost index i: the boundary mapping, or i itself when
+// the ghosts are materialised in memory (user boundary hooks).
+__device__ __forceinline__ int ghost_src(const Geom& g, int i, int n, int lo, int hi, bool& flip) {
+  if (g.gmem) return i;
+  return map_ghost(i, n, lo, hi, flip);
+}
+
 // Conserved 8-vector of cell (i,j,k) in FP64, ghosts resolved through the
 // boundary conditions.
 template <typename S>
 __device__ __forceinline__ void load_cons(const Geom& g, const S* __restrict__ U, int i, int j, int k,
                                           double u[8]) {
   bool fx = false, fy = false, fz = false;
-  const int si = map_ghost(i, g.nx, g.bc[0], g.bc[1], fx);
-  const int sj = map_ghost(j, g.ny, g.bc[2], g.bc[3], fy);
-  const int sk = map_ghost(k, g.nz, g.bc[4], g.bc[5], fz);
+  const int si = ghost_src(g, i, g.nx, g.bc[0], g.bc[1], fx);
+  const int sj = ghost_src(g, j, g.ny, g.bc[2], g.bc[3], fy);
+  const int sk = ghost_src(g, k, g.nz, g.bc[4], g.bc[5], fz);
   const long long c = cidx(g, si, sj, sk);
 #pragma unroll
   for (int q = 0; q < 8; q++) u[q] = (double)U[q * g.ncell + c];
@@ -294,7 +301,7 @@ __device__ __forceinline__ void z_body(const Box<S>& b, unsigned lb, double dt, 
   bool uflip = false;
   auto fetch = [&](int k) {
     bool f = false;
-    const int sk = map_ghost(k, g.nz, g.bc[4], g.bc[5], f);
+    const int sk = ghost_src(g, k, g.nz, g.bc[4], g.bc[5], f);
     const long long cc = col + (long long)(sk + 1) * sz;
 #pragma unroll
     for (int q = 0; q < 8; q++) ur[q] = Usrc[q * N + cc];
@@ -395,10 +402,13 @@ __device__ __forceinline__ void z_body(const Box<S>& b, unsigned lb, double dt, 
 // Stage 2 writes over U^n in place (each cell reads only its own field).
 // ---------------------------------------------------------------------------
 
-// offset (in cells along the axis) of the E ghost-resolved neighbour
+// offset (in cells along the axis) of the E ghost-resolved neighbour; a
+// BC_OTHER face's E ghosts are never filled (boundaryI_ef has no branch for
+// it) and are read as stored
 __device__ __forceinline__ int e_nbr(int i, int n, int d, int bc) {
   const int t = i + d;
   if (t >= 1 && t <= n) return d;
+  if (bc == BC_OTHER) return d;
   if (bc == BC_PERIODIC) return (t < 1 ? t + n : t - n) - i;
   return 0;  // closed / outflow: ghost takes the edge cell's value
 }
@@ -457,9 +467,9 @@ __device__ __forceinline__ void bfield_body(const Box<S>& b, unsigned lb, double
 template <typename S>
 __device__ __forceinline__ double field_at(const Geom& g, const S* __restrict__ U, int q, int i, int j, int k) {
   bool f = false;
-  const int si = map_ghost(i, g.nx, g.bc[0], g.bc[1], f);
-  const int sj = map_ghost(j, g.ny, g.bc[2], g.bc[3], f);
-  const int sk = map_ghost(k, g.nz, g.bc[4], g.bc[5], f);
+  const int si = ghost_src(g, i, g.nx, g.bc[0], g.bc[1], f);
+  const int sj = ghost_src(g, j, g.ny, g.bc[2], g.bc[3], f);
+  const int sk = ghost_src(g, k, g.nz, g.bc[4], g.bc[5], f);
   return (double)U[q * g.ncell + cidx(g, si, sj, sk)];
 }
 
@@ -691,26 +701,33 @@ __global__ void kb_clock_dt(BoxClock* clk, int nbox, int tuning) {
 }
 
 // ---------------------------------------------------------------------------
-// ghost fill of one face of the conserved state, boundaryI
-// (src/boundaries.f90:44-247, serial branch): periodic copies over the full
-// transverse range, closed/outflow over 0..n+1.
+// ghost fill of one face of the conserved state, serial branches:
+//   layers 1, boundaryI (src/boundaries.f90:44-247): ghost 0 (n+1); periodic
+//     copies over the full transverse range, closed/outflow over 0..n+1;
+//   layers 2, boundaryII (:256-512): ghosts -1, 0 (n+1, n+2) over the full
+//     transverse range; periodic from n-1, n (1, 2), closed/outflow mirrored
+//     (ghost 0 <- cell 1, ghost -1 <- cell 2).
+// Closed walls negate the normal momentum.
 // ---------------------------------------------------------------------------
 template <typename S>
-__global__ void __launch_bounds__(256) k_ghost1(Geom g, S* __restrict__ U, int axis, int side, int kind) {
+__global__ void __launch_bounds__(256) k_ghost(Geom g, S* __restrict__ U, int axis, int side, int kind, int layers) {
   const int n0 = axis == 0 ? g.ny : g.nx;  // transverse extents (interior)
   const int n1 = axis == 2 ? g.ny : g.nz;
   const int na = axis == 0 ? g.nx : axis == 1 ? g.ny : g.nz;
-  const int lo = kind == BC_PERIODIC ? -1 : 0;
+  const int lo = (kind == BC_PERIODIC || layers == 2) ? -1 : 0;
   const int e0 = n0 + 2 - 2 * lo, e1 = n1 + 2 - 2 * lo;
-  const long long n = (long long)e0 * e1;
+  const long long n = (long long)e0 * e1 * layers;
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
        t += (long long)gridDim.x * blockDim.x) {
-    const int a = (int)(t % e0) + lo;
-    const int b = (int)(t / e0) + lo;
-    const int dst = side == 0 ? 0 : na + 1;
+    const long long plane = (long long)e0 * e1;
+    const int layer = (int)(t / plane);  // 0: the layer next to the domain
+    const long long r = t - (long long)layer * plane;
+    const int a = (int)(r % e0) + lo;
+    const int b = (int)(r / e0) + lo;
+    const int dst = side == 0 ? -layer : na + 1 + layer;
     int src;
-    if (kind == BC_PERIODIC) src = side == 0 ? na : 1;
-    else src = side == 0 ? 1 : na;
+    if (kind == BC_PERIODIC) src = side == 0 ? dst + na : dst - na;
+    else src = side == 0 ? 1 - dst : 2 * na + 1 - dst;
     int di, dj, dk, si, sj, sk;
     if (axis == 0) { di = dst; si = src; dj = sj = a; dk = sk = b; }
     else if (axis == 1) { dj = dst; sj = src; di = si = a; dk = sk = b; }
@@ -904,11 +921,11 @@ void launch_clock_dt(hipStream_t s, BoxClock* clk, int nbox, int tuning) {
 }
 
 template <typename S>
-void launch_ghost1(hipStream_t s, const Geom& g, S* U, int axis, int side, int kind) {
+void launch_ghost(hipStream_t s, const Geom& g, S* U, int axis, int side, int kind, int layers) {
   const int n0 = axis == 0 ? g.ny : g.nx;
   const int n1 = axis == 2 ? g.ny : g.nz;
-  const long long n = (long long)(n0 + 4) * (n1 + 4);
-  hipLaunchKernelGGL(k_ghost1<S>, dim3(grid_for(n)), dim3(256), 0, s, g, U, axis, side, kind);
+  const long long n = (long long)(n0 + 4) * (n1 + 4) * layers;
+  hipLaunchKernelGGL(k_ghost<S>, dim3(grid_for(n)), dim3(256), 0, s, g, U, axis, side, kind, layers);
 }
 
 template <typename S>
@@ -948,7 +965,7 @@ void launch_divb(hipStream_t s, const Geom& g, const S* U, double dx, double dy,
                                        const BoxClock*, int, const Shape&);                                       \
   template void launch_cfl_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, BoxClock*, int*,  \
                                     const Shape&);                                                                \
-  template void launch_ghost1<S>(hipStream_t, const Geom&, S*, int, int, int);                                    \
+  template void launch_ghost<S>(hipStream_t, const Geom&, S*, int, int, int, int);                               \
   template void launch_aos_to_soa<S>(hipStream_t, long long, const double*, S*);                                  \
   template void launch_soa_to_aos<S>(hipStream_t, long long, const S*, double*);                                  \
   template void launch_prim_all<S>(hipStream_t, long long, const Phys&, const S*, double*, double*);              \

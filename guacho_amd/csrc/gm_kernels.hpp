@@ -16,6 +16,8 @@ struct Geom {
   int NX, NY, NZ;     // extents with two ghost layers: n + 4
   long long ncell;    // NX * NY * NZ
   int bc[6];
+  int gmem;           // 1: ghosts are materialised in memory (user boundary
+                      // hooks); 0: resolved by boundary mapping
 };
 
 // One box (subdomain) as the kernels see it: geometry, physics and its
@@ -117,8 +119,10 @@ void launch_clock_begin(hipStream_t s, BoxClock* clk, int nbox);
 void launch_clock_dt(hipStream_t s, BoxClock* clk, int nbox, int tuning);
 
 // Layout conversion, ghost fill and diagnostics of one box's state.
+// layers 1: one face of boundaryI (src/boundaries.f90:44-247); layers 2:
+// one face of boundaryII (:256-512), two mirrored layers, full transverse range
 template <typename S>
-void launch_ghost1(hipStream_t s, const Geom& g, S* U, int axis, int side, int kind);
+void launch_ghost(hipStream_t s, const Geom& g, S* U, int axis, int side, int kind, int layers);
 template <typename S>
 void launch_aos_to_soa(hipStream_t s, long long ncell, const double* A, S* D);
 template <typename S>

@@ -53,6 +53,7 @@ class Parameters:
     enable_flux_cd: bool = True
     eight_wave: bool = False
     user_source_terms: bool = False
+    bc_user: bool = False
     cfl: float = 0.4
     tmax: float = 0.5
     dtprint: float = 0.1
@@ -134,22 +135,29 @@ def get_timestep(dev: _capi.Subdomain, prm: Parameters, g: Globals, n_iter: int 
 def run(prm: Parameters, g: Globals, device: int = 0,
         write_output: Optional[Callable[[int, Globals, _capi.Subdomain], None]] = None,
         max_steps: Optional[int] = None, log: Optional[Callable[[str], None]] = None,
-        user_sources: Optional[Tuple[str, str]] = None) -> Globals:
+        user_sources: Optional[Tuple[str, str]] = None,
+        user_bc: Optional[Tuple[str, str]] = None) -> Globals:
     """The time loop of src/main.f90:70-125 with the step on the GPU.
 
     ``g.u`` holds the initial conditions (ghosts included).  boundaryI and
     calcprim at start-up run on the device.  ``write_output(itprint, g, dev)``
     is called at t=0 and at every dump, as write_output is in the reference.
     ``user_sources`` = (code object, kernel) is the device form of the
-    problem's get_user_source_terms (needed when user_source_terms is set).
+    problem's get_user_source_terms (needed when user_source_terms is set),
+    ``user_bc`` that of impose_user_bc (needed when bc_user is set).
     Returns ``g`` with ``u`` downloaded at the end.
     """
     if prm.user_source_terms and user_sources is None:
         raise ValueError("user_source_terms needs the device plugin: pass user_sources=(code_object, kernel)")
+    if prm.bc_user and user_bc is None:
+        raise ValueError("bc_user needs the device plugin: pass user_bc=(code_object, kernel)")
     with _capi.Subdomain(device_params(prm, g, device)) as dev:
         if prm.user_source_terms:
             dev.set_user_sources(*user_sources)
+        if prm.bc_user:
+            dev.set_user_bc(*user_bc)
         dev.upload_u(g.u)
+        dev.set_time(g.time)
         dev.boundary_u()                                     # main.f90:76
         if write_output is not None:                         # main.f90:84-87
             write_output(g.itprint, g, dev)

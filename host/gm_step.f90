@@ -44,7 +44,6 @@ contains
     p%riemann_solver = riemann_solver
     p%slope_limiter = slope_limiter
     p%bc = (/ bc_left, bc_right, bc_bottom, bc_top, bc_out, bc_in /)
-    if (bc_user) p%bc = BC_OTHER
     p%mhd = merge(1, 0, mhd)
     p%pmhd = merge(1, 0, pmhd)
     p%enable_flux_cd = merge(1, 0, enable_flux_cd)
@@ -66,6 +65,7 @@ contains
     end if
     call gm_check(gm_create(p, gm_ctx), 'gm_create')
     if (user_source_terms) call load_user_sources()
+    if (bc_user) call load_user_bc()
     call gm_check(gm_upload_u(gm_ctx, u), 'gm_upload_u')
   end subroutine gm_device_init
 
@@ -85,6 +85,21 @@ contains
                   'gm_set_user_sources')
   end subroutine load_user_sources
 
+  !> bc_user: impose_user_bc runs on the device as a plugin kernel;
+  !> GUACHO_USER_BC names it as <code object>:<kernel>
+  subroutine load_user_bc()
+    character(len=1024) :: spec
+    integer :: n, st, colon
+    call get_environment_variable('GUACHO_USER_BC', spec, n, st)
+    colon = index(spec(1:max(n, 0)), ':', back=.true.)
+    if (st /= 0 .or. n == 0 .or. colon < 2 .or. colon == n) then
+      print '(a)', 'guacho_amd: bc_user needs the device plugin: set GUACHO_USER_BC=<code object>:<kernel>'
+      stop 1
+    end if
+    call gm_check(gm_set_user_bc(gm_ctx, spec(1:colon-1)//c_null_char, spec(colon+1:n)//c_null_char), &
+                  'gm_set_user_bc')
+  end subroutine load_user_bc
+
   subroutine gm_device_finalize()
     integer(c_int) :: rc
     if (c_associated(gm_ctx)) rc = gm_destroy(gm_ctx)
@@ -97,6 +112,7 @@ contains
   end subroutine gm_download
 
   subroutine boundaryI()
+    call gm_check(gm_set_time(gm_ctx, time), 'gm_set_time')
     call gm_check(gm_boundary_u(gm_ctx), 'gm_boundary_u')
   end subroutine boundaryI
 

@@ -108,6 +108,12 @@ module guacho_capi
       type(c_ptr), value :: ctx
       character(kind=c_char), intent(in) :: code_object(*), kernel(*)
     end function gm_set_user_sources
+
+    integer(c_int) function gm_set_user_bc(ctx, code_object, kernel) bind(C, name='gm_set_user_bc')
+      import :: c_ptr, c_int, c_char
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: code_object(*), kernel(*)
+    end function gm_set_user_bc
   end interface
 
 contains

@@ -67,7 +67,8 @@ typedef struct gm_params {
   int32_t nghost;         /* must be 2 */
   int32_t riemann_solver; /* GM_SOLVER_* */
   int32_t slope_limiter;  /* GM_LIMITER_* */
-  int32_t bc[6];          /* bc_left, bc_right, bc_bottom, bc_top, bc_out, bc_in */
+  int32_t bc[6];          /* bc_left, bc_right, bc_bottom, bc_top, bc_out, bc_in
+                             (GM_BC_OTHER: the user boundary hook's face) */
   int32_t mhd;            /* full MHD */
   int32_t pmhd;           /* passive B (HLLC) */
   int32_t enable_flux_cd; /* flux-CD div B control */
@@ -160,6 +161,21 @@ int gm_set_time(gm_ctx *ctx, double time);
  * removes the plugin.  GM_ERR_CONFIG if the file or kernel cannot be
  * loaded. */
 int gm_set_user_sources(gm_ctx *ctx, const char *code_object, const char *kernel);
+
+/* User boundary hook: replaces impose_user_bc(u, order) (bc_user = .true.;
+ * called at the end of boundaryI with order 1 and of boundaryII with order
+ * 2, src/boundaries.f90:245, 508).  Loads `kernel` from the gfx950 code
+ * object (signature in guacho_amd_user.h; launched over every stored cell,
+ * ghosts included).  With a hook installed the ghost layers live in memory,
+ * as in the reference: gm_boundary_u runs boundaryI then the hook (order 1);
+ * each gm_tstep runs boundaryII then the hook (order 2) on U* between its
+ * stages and boundaryI then the hook (order 1) on U at its end
+ * (src/hydro_solver.f90:169, 216).  Faces of kind GM_BC_OTHER are left to
+ * the hook: no standard fill touches them, and their E ghosts are read as
+ * stored (boundaryI_ef has no branch for them).  A step with a GM_BC_OTHER
+ * face and no hook fails with GM_ERR_CONFIG.  code_object NULL removes the
+ * hook.  Hooks are not available in batches. */
+int gm_set_user_bc(gm_ctx *ctx, const char *code_object, const char *kernel);
 
 /* ---- batches of independent boxes ----------------------------------- */
 /* No counterpart in the reference (one domain per run there).  A batch

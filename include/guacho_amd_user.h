@@ -8,10 +8,15 @@
  *       (order 1, on u) and boundaryII (order 2, on up)
  *       (src/boundaries.f90:245, 508).
  * Here each is a HIP kernel in a code object the user builds for gfx950
- * (hipcc --genco --offload-arch=gfx950 -I include plugin.hip -o plugin.hsaco)
- * and registers with gm_set_user_sources / gm_set_user_bc.  The library
- * launches it on its own stream with one argument, a gm_user_args by value,
- * over ceil(nx*ny*nz / 256) workgroups of 256 threads:
+ * (hipcc --genco --offload-arch=gfx950 -ffp-contract=off -I include
+ * plugin.hip -o plugin.hsaco) and registers with gm_set_user_sources /
+ * gm_set_user_bc.  The library launches it on its own stream with one
+ * argument, a gm_user_args by value, in workgroups of 256 threads: over
+ * the interior cells (ceil(nx*ny*nz / 256) workgroups, gm_user_cell) for
+ * sources, over every stored cell (ceil(ncell / 256), gm_user_any_cell) for
+ * boundary hooks.  A hook runs as a parallel loop: a thread may read any
+ * cell, but a cell another thread writes in the same call must not feed
+ * the value it writes.  A source kernel, for example:
  *
  *   extern "C" __global__ void my_sources(gm_user_args a) {
  *     int i, j, k;
@@ -62,6 +67,17 @@ __device__ __forceinline__ bool gm_user_cell(const gm_user_args &a, int64_t t, i
   const int64_t r = t / a.nx;
   j = (int)(r % a.ny) + 1;
   k = (int)(r / a.ny) + 1;
+  return true;
+}
+
+/* Any stored cell of linear index t (x fastest), ghosts included:
+ * i in -1..nx+2 (likewise j, k); false past the last cell. */
+__device__ __forceinline__ bool gm_user_any_cell(const gm_user_args &a, int64_t t, int &i, int &j, int &k) {
+  if (t < 0 || t >= a.ncell) return false;
+  i = (int)(t % a.NX) - 1;
+  const int64_t r = t / a.NX;
+  j = (int)(r % a.NY) - 1;
+  k = (int)(r / a.NY) - 1;
   return true;
 }
 

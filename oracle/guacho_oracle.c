@@ -42,6 +42,7 @@ struct orc_state {
   orc_params p;
   int NX, NY, NZ;   /* extents incl. ghosts: n+4 */
   long ncell;
+  double time;      /* globals::time */
   double *u, *up, *primit, *f, *g, *h, *e, *temp;
 };
 
@@ -480,8 +481,8 @@ int orc_create(const orc_params *p, orc_state **out) {
   if (p->solver < 1 || p->solver > 4) return 1;
   if (p->limiter < -1 || p->limiter > 6) return 1;
   if (p->solver == ORC_SOLVER_HLLC && p->mhd) return 1;
-  for (int b = 0; b < 6; b++)
-    if (p->bc[b] < 1 || p->bc[b] > 3) return 1;
+  for (int b = 0; b < 6; b++)   /* BC_OTHER (4): no fill, the user hook's face */
+    if (p->bc[b] < 1 || p->bc[b] > 4) return 1;
   /* periodic must pair on an axis, as the reference's tests do (boundaries.f90:111,122,133) */
   for (int a = 0; a < 3; a++)
     if ((p->bc[2 * a] == ORC_BC_PERIODIC) != (p->bc[2 * a + 1] == ORC_BC_PERIODIC)) return 1;
@@ -542,6 +543,51 @@ static inline void copy_cell(orc_state *s, double *arr, int di, int dj, int dk,
   for (int q = 0; q < NEQ; q++) d[q] = (q == q_flip) ? -src[q] : src[q];
 }
 
+void orc_set_time(orc_state *s, double time) { s->time = time; }
+
+/* The checker's impose_user_bc (oracle/ref/user_mod.f90), in the Fortran's
+ * evaluation order: an inflow jet through the left x face (the ghost layers
+ * 1-order..0, full transverse range, where (y-1/2)^2 + (z-1/2)^2 < 1/16)
+ * with vx = 1 + time, and an obstacle of interior cells within 0.15 of
+ * (0.7, 0.5, 0.5) reset to rest at density 5, pressure 1, keeping B.
+ * Test infrastructure: a user's hook is user code, supplied to the library
+ * as a device plugin. */
+static void user_jet(const orc_state *s, double *u, int order) {
+  const orc_params *p = &s->p;
+  const double t = s->time;
+  for (int k = -1; k <= p->nz + 2; k++)
+    for (int j = -1; j <= p->ny + 2; j++) {
+      const double yc = ((double)j - 0.5) * p->dy - 0.5;
+      const double zc = ((double)k - 0.5) * p->dz - 0.5;
+      if (!(yc * yc + zc * zc < 0.0625)) continue;
+      for (int i = 1 - order; i <= 0; i++) {
+        const double rho = 2.0, vx = 1.0 + t, pr = 1.0, bx = 0.5;
+        AT(u, s, 0, i, j, k) = rho;
+        AT(u, s, 1, i, j, k) = rho * vx;
+        AT(u, s, 2, i, j, k) = 0.0;
+        AT(u, s, 3, i, j, k) = 0.0;
+        AT(u, s, 4, i, j, k) = ((0.5 * rho) * (vx * vx) + p->cv * pr) + 0.5 * (bx * bx);
+        AT(u, s, 5, i, j, k) = bx;
+        AT(u, s, 6, i, j, k) = 0.0;
+        AT(u, s, 7, i, j, k) = 0.0;
+      }
+    }
+  for (int k = 1; k <= p->nz; k++)
+    for (int j = 1; j <= p->ny; j++)
+      for (int i = 1; i <= p->nx; i++) {
+        const double xc = ((double)i - 0.5) * p->dx - 0.7;
+        const double yc = ((double)j - 0.5) * p->dy - 0.5;
+        const double zc = ((double)k - 0.5) * p->dz - 0.5;
+        if (!((xc * xc + yc * yc) + zc * zc < 0.0225)) continue;
+        const double bx = AT(u, s, 5, i, j, k), by = AT(u, s, 6, i, j, k), bz = AT(u, s, 7, i, j, k);
+        AT(u, s, 0, i, j, k) = 5.0;
+        AT(u, s, 1, i, j, k) = 0.0;
+        AT(u, s, 2, i, j, k) = 0.0;
+        AT(u, s, 3, i, j, k) = 0.0;
+        AT(u, s, 4, i, j, k) = p->cv * 1.0 + 0.5 * ((bx * bx + by * by) + bz * bz);
+      }
+}
+
 /* boundaryI, src/boundaries.f90:44-247, serial branch (:108-144) for
  * periodic; closed (:146-199) and outflow (:201-242) on ranges 0:n+1 */
 void orc_boundaryI(orc_state *s) {
@@ -589,6 +635,7 @@ void orc_boundaryI(orc_state *s) {
       for (int j = 0; j <= ny + 1; j++)
         for (int i = 0; i <= nx + 1; i++) copy_cell(s, u, i, j, nz + 1, i, j, nz, flip_z);
   }
+  if (p->user_bc) user_jet(s, u, 1);                                 /* :245 */
 }
 
 /* boundaryII, src/boundaries.f90:256-512, serial branch (:323-359) for
@@ -674,6 +721,7 @@ void orc_boundaryII(orc_state *s) {
           copy_cell(s, u, i, j, nz + 2, i, j, nz - 1, fz);
         }
   }
+  if (p->user_bc) user_jet(s, u, 2);                                 /* :508 */
 }
 
 /* calcprim over the entire domain, hydro_core.f90:304-316 */

@@ -40,6 +40,7 @@ typedef struct orc_params {
   int flux_cd;            /* enable_flux_cd (parameters.f90:63) */
   int eight_wave;         /* Powell 8-wave source terms (parameters.f90:61) */
   int user_src;           /* the checker's user source (oracle/ref/user_mod.f90) */
+  int user_bc;            /* bc_user: the checker's impose_user_bc after boundaryI/II */
   int nthreads;           /* OpenMP threads for the sweeps (<=0: runtime default) */
 } orc_params;
 
@@ -59,6 +60,7 @@ void orc_get_temp(const orc_state *s, double *temp);
 void orc_get_flux(const orc_state *s, int axis, double *fgh);   /* axis 0:f 1:g 2:h */
 void orc_get_e(const orc_state *s, double *e);                  /* e(3, ...) */
 
+void orc_set_time(orc_state *s, double time);   /* globals::time, seen by the user hook */
 void orc_boundaryI(orc_state *s);
 void orc_boundaryII(orc_state *s);
 void orc_calcprim_u(orc_state *s);

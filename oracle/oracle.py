@@ -42,7 +42,7 @@ class OrcParams(C.Structure):
         ("cv", C.c_double), ("gamma", C.c_double), ("eta", C.c_double),
         ("solver", C.c_int), ("limiter", C.c_int), ("bc", C.c_int * 6),
         ("mhd", C.c_int), ("pmhd", C.c_int), ("flux_cd", C.c_int),
-        ("eight_wave", C.c_int), ("user_src", C.c_int), ("nthreads", C.c_int),
+        ("eight_wave", C.c_int), ("user_src", C.c_int), ("user_bc", C.c_int), ("nthreads", C.c_int),
     ]
 
 
@@ -65,6 +65,7 @@ class Config:
     enable_flux_cd: bool = True
     eight_wave: bool = False
     user_src: bool = False          # the checker's user gravity (oracle/ref/user_mod.f90)
+    user_bc: bool = False           # the checker's impose_user_bc (oracle/ref/user_mod.f90)
     cfl: float = 0.4
 
     @property
@@ -82,6 +83,7 @@ class Config:
         p.mhd, p.pmhd, p.flux_cd = int(self.mhd), int(self.pmhd), int(self.enable_flux_cd)
         p.eight_wave = int(self.eight_wave)
         p.user_src = int(self.user_src)
+        p.user_bc = int(self.user_bc)
         p.nthreads = nthreads
         return p
 
@@ -114,6 +116,7 @@ def _load_oracle() -> C.CDLL:
     L.orc_cfl_dtp.argtypes = [C.c_void_p]
     L.orc_cfl_dtp.restype = C.c_double
     L.orc_source.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _dp]
+    L.orc_set_time.argtypes = [C.c_void_p, C.c_double]
     L.orc_timestep_finish.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                       C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.orc_u2prim.argtypes = [P, _dp, _dp, C.POINTER(C.c_double)]
@@ -232,6 +235,10 @@ class Oracle:
     def cfl_dtp(self) -> float:
         return self.L.orc_cfl_dtp(self.h)
 
+    def set_time(self, time: float) -> None:
+        """globals::time, as the user hook sees it."""
+        self.L.orc_set_time(self.h, float(time))
+
     def source(self, i: int, j: int, k: int, prim) -> np.ndarray:
         """source(i,j,k,prim,s) with the 8-wave term, from the state's primit."""
         sv = np.zeros(NEQ)
@@ -338,9 +345,10 @@ class RefLib:
         L.ref_get_timestep.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double,
                                        C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.ref_source.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp]
-        sw = (C.c_int * 2)()
+        L.ref_set_time.argtypes = [C.c_double]
+        sw = (C.c_int * 3)()
         L.ref_switches(sw)
-        self.eight_wave, self.user_source_terms = bool(sw[0]), bool(sw[1])
+        self.eight_wave, self.user_source_terms, self.bc_user = bool(sw[0]), bool(sw[1]), bool(sw[2])
 
     def config(self, dx=None, dy=None, dz=None) -> Config:
         dx = dx if dx is not None else 1.0 / self.nx
@@ -349,7 +357,7 @@ class RefLib:
         return Config(nx=self.nx, ny=self.ny, nz=self.nz, dx=dx, dy=dy, dz=dz, cv=self.cv,
                       riemann_solver=self.riemann_solver, slope_limiter=self.slope_limiter, bc=list(self.bc),
                       mhd=self.mhd, pmhd=self.pmhd, enable_flux_cd=True, cfl=self.cfl,
-                      eight_wave=self.eight_wave, user_src=self.user_source_terms)
+                      eight_wave=self.eight_wave, user_src=self.user_source_terms, user_bc=self.bc_user)
 
     def init(self, dx, dy, dz):
         self.L.ref_init(dx, dy, dz)
@@ -366,6 +374,7 @@ class RefLib:
     def get_temp(self):
         a = np.zeros(self.shape()[1:], order="F"); self.L.ref_get_temp(a); return a
 
+    def set_time(self, t): self.L.ref_set_time(float(t))
     def boundary(self, order): self.L.ref_boundary(order)
     def calcprim(self, which): self.L.ref_calcprim(which)
     def fluxes(self, choice): self.L.ref_fluxes(choice)

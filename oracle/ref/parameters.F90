@@ -32,7 +32,7 @@ module parameters
   integer, parameter :: bc_top    = REF_BC_TOP
   integer, parameter :: bc_out    = REF_BC_OUT
   integer, parameter :: bc_in     = REF_BC_IN
-  logical, parameter :: bc_user   = .false.
+  logical, parameter :: bc_user   = (REF_USER_BC /= 0)
 
   integer, parameter :: slope_limiter = REF_LIMITER
   integer, parameter :: th_cond = TC_OFF

@@ -179,10 +179,16 @@ contains
     call source(i, j, k, prim, sv)
   end subroutine ref_source
 
-  ! logical switches of this build: eight_wave, user_source_terms
+  ! logical switches of this build: eight_wave, user_source_terms, bc_user
   subroutine ref_switches(sw) bind(C, name='ref_switches')
-    integer(c_int), intent(out) :: sw(2)
-    sw = (/ merge(1, 0, eight_wave), merge(1, 0, user_source_terms) /)
+    integer(c_int), intent(out) :: sw(3)
+    sw = (/ merge(1, 0, eight_wave), merge(1, 0, user_source_terms), merge(1, 0, bc_user) /)
   end subroutine ref_switches
+
+  ! globals::time, which the user hooks read
+  subroutine ref_set_time(t) bind(C, name='ref_set_time')
+    real(c_double), value :: t
+    time = t
+  end subroutine ref_set_time
 
 end module ref_kat

@@ -8,12 +8,13 @@ from guacho_amd.host import timestep_clamp
 from oracle.oracle import Config, Oracle
 
 
-def oracle_config(params: _capi.GmParams, cfl: float = 0.4, user_src: bool = False) -> Config:
+def oracle_config(params: _capi.GmParams, cfl: float = 0.4, user_src: bool = False,
+                  user_bc: bool = False) -> Config:
     return Config(nx=params.nx, ny=params.ny, nz=params.nz, dx=params.dx, dy=params.dy, dz=params.dz,
                   cv=params.cv, riemann_solver=params.riemann_solver, slope_limiter=params.slope_limiter,
                   bc=list(params.bc), mhd=bool(params.mhd), pmhd=bool(params.pmhd),
                   enable_flux_cd=bool(params.enable_flux_cd), cfl=cfl,
-                  eight_wave=bool(params.eight_wave), user_src=user_src)
+                  eight_wave=bool(params.eight_wave), user_src=user_src, user_bc=user_bc)
 
 
 def random_state(rng, nx, ny, nz, cv=1.5, contrast=0.3, vmax=1.5, bmax=1.0):
@@ -31,12 +32,13 @@ def random_state(rng, nx, ny, nz, cv=1.5, contrast=0.3, vmax=1.5, bmax=1.0):
 
 
 def run_pair(params: _capi.GmParams, u0: np.ndarray, nsteps: int, cfl: float = 0.4, check_each=False,
-             user_sources=None):
+             user_sources=None, user_bc=None):
     """Advance u0 nsteps on both paths with dt from each path's own CFL
     reduction; asserts the reductions agree bitwise every step.
-    user_sources: (code object, kernel) of a plugin whose terms equal the
-    oracle's user source.  Returns (u_device, u_oracle, dts)."""
-    O = Oracle(oracle_config(params, cfl, user_src=user_sources is not None))
+    user_sources / user_bc: (code object, kernel) of plugins equal to the
+    oracle's user source / user boundary hook; both sides see the same
+    globals::time.  Returns (u_device, u_oracle, dts)."""
+    O = Oracle(oracle_config(params, cfl, user_src=user_sources is not None, user_bc=user_bc is not None))
     O.set_u(u0)
     O.boundaryI()
     O.calcprim_u()
@@ -44,6 +46,8 @@ def run_pair(params: _capi.GmParams, u0: np.ndarray, nsteps: int, cfl: float = 0
     with _capi.Subdomain(params) as dev:
         if user_sources is not None:
             dev.set_user_sources(*user_sources)
+        if user_bc is not None:
+            dev.set_user_bc(*user_bc)
         dev.upload_u(u0)
         dev.boundary_u()
         t = 0.0
@@ -52,6 +56,8 @@ def run_pair(params: _capi.GmParams, u0: np.ndarray, nsteps: int, cfl: float = 0
             dtp_d = dev.cfl_dtp()
             assert dtp_o == dtp_d, f"step {it}: CFL reduction differs {dtp_o!r} vs {dtp_d!r}"
             dt, _ = timestep_clamp(it, 10, cfl, dtp_o, t, 1e30)
+            O.set_time(t)
+            dev.set_time(t)
             O.tstep(dt)
             dev.tstep(dt)
             t += dt

@@ -7,7 +7,8 @@ inputs and the reference's outputs for
   - the slope limiter, u2prim, prim2u, prim2f, cfast, cfastX,
   - one boundaryI + calcprim + first-order sweep on a random state,
   - one boundaryII + calcprim + limited sweep on a random state,
-  - get_timestep's ramp and clamp,
+  - get_timestep's ramp and clamp (the boundary fills include the
+    checker's impose_user_bc in bc_user builds, at globals::time = 0.3),
   - source(i,j,k,prim,s) of sources.f90 at every interior cell of a random
     primitive array (zero, 8-wave, the checker's user gravity, or both,
     per the variant's switches).
@@ -91,6 +92,8 @@ def make(variant, seed):
     # sweeps on a random state
     dx, dy, dz = 0.1, 0.07, 0.13
     R.init(dx, dy, dz)
+    hook_time = 0.3 if R.bc_user else 0.0  # globals::time the user boundary hook reads
+    R.set_time(hook_time)
     u = random_state(rng, R.shape(), R.cv)
     out["sweep_u_in"] = u
     R.set_array(0, u)
@@ -126,7 +129,8 @@ def make(variant, seed):
                 for i in range(1, R.nx + 1):
                     src[i - 1, j - 1, k - 1] = R.source(i, j, k, prim[:, i + 1, j + 1, k + 1])
         out["src_primit"], out["src_out"] = prim, src
-    out["switches"] = np.array([int(R.eight_wave), int(R.user_source_terms)])
+    out["switches"] = np.array([int(R.eight_wave), int(R.user_source_terms), int(R.bc_user)])
+    out["hook_time"] = np.array([hook_time])
     out["dxyz"] = np.array([dx, dy, dz])
     cfg = np.array([R.nx, R.ny, R.nz, R.riemann_solver, R.slope_limiter, *R.bc, int(R.mhd), int(R.pmhd)])
     out["config_int"] = cfg

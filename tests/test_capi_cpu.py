@@ -49,7 +49,7 @@ def _create(**kw):
     (dict(riemann_solver=2, mhd=True), "HLLC"),
     (dict(mhd=False, pmhd=False), "hydro"),
     (dict(slope_limiter=9), "slope_limiter"),
-    (dict(bc=(4, 4, 3, 3, 3, 3)), "BC_OTHER"),
+    (dict(bc=(5, 1, 3, 3, 3, 3)), "boundary kind"),
     (dict(bc=(3, 1, 3, 3, 3, 3)), "periodic"),
     (dict(eta=0.1), "eta"),
     (dict(nz=1), "at least 2"),

@@ -20,11 +20,12 @@ def _cfg(z):
     ci = z["config_int"]
     cr = z["config_real"]
     dx, dy, dz = z["dxyz"]
-    sw = z["switches"] if "switches" in z else np.zeros(2, dtype=int)
+    sw = list(z["switches"]) if "switches" in z else []
+    sw = sw + [0] * (3 - len(sw))
     return Config(nx=int(ci[0]), ny=int(ci[1]), nz=int(ci[2]), dx=dx, dy=dy, dz=dz, cv=float(cr[0]),
                   riemann_solver=int(ci[3]), slope_limiter=int(ci[4]), bc=[int(v) for v in ci[5:11]],
                   mhd=bool(ci[11]), pmhd=bool(ci[12]), cfl=float(cr[2]),
-                  eight_wave=bool(sw[0]), user_src=bool(sw[1]))
+                  eight_wave=bool(sw[0]), user_src=bool(sw[1]), user_bc=bool(sw[2]))
 
 
 def test_fixtures_present():
@@ -59,6 +60,8 @@ def test_sweeps_match_reference(path):
     z = np.load(path)
     cfg = _cfg(z)
     O = Oracle(cfg)
+    if "hook_time" in z:
+        O.set_time(float(z["hook_time"][0]))
     O.set_u(z["sweep_u_in"])
     O.boundaryI()
     np.testing.assert_array_equal(O.get_u(), z["sweep_u_bc1"])
