@@ -10,8 +10,9 @@ Prints one JSON line: zone-updates/s over the timed steps, the HBM roofline
 of the dominant kernel class from HIP events on the library's stream, the
 step-level roofline at the 128 B/zone-update compulsory traffic, the
 mixed-precision form on the 512^3 box against the FP64 path (speed, per-field
-error, drift, div B), and the C restatement of the reference's CPU path timed
-on a bounded sample on the host cores.  Progress goes to stderr.
+error, drift, div B), the optional terms (8-wave sources, user plugins), the
+batch and serving forms, and the C restatement of the reference's CPU path
+timed on a bounded sample on the host cores.  Progress goes to stderr.
 """
 import argparse
 import json
@@ -37,6 +38,8 @@ def parse():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision section")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-boxes section")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving section")
+    ap.add_argument("--no-extensions", action="store_true",
+                    help="skip the 8-wave / user-plugin section")
     ap.add_argument("--serving-requests", type=int, default=24)
     ap.add_argument("--batch-boxes", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=100)
@@ -247,6 +250,64 @@ def batch_section(nbox: int, steps: int, seed: int):
             "speedup_vs_sequential_fp64": round(seq / el, 3), "repeat_bitwise_equal": same}
 
 
+def extensions_section(u0, dx, dy, dz, n: int, steps: int):
+    """The step with the reference's optional terms, FP64 on the seeded
+    turbulence box, against the same box without them: Powell 8-wave
+    source terms (no flux-CD, as the 8-wave scheme is run), and the user
+    plugins -- a source-term kernel (uniform + point-mass gravity) and a
+    boundary hook (inflow jet + interior obstacle; ghosts then live in
+    memory: boundaryII + hook between the stages, boundaryI + hook after)
+    -- on outflow faces.  Per variant: zone-updates/s and the per-class
+    kernel times."""
+    from guacho_amd import _capi
+    from guacho_amd.host import timestep_clamp
+    plugins = os.path.join(ROOT, "tests", "plugins")
+    gravity = (os.path.join(plugins, "gravity.hsaco"), "gm_test_gravity")
+    jet = (os.path.join(plugins, "jet.hsaco"), "gm_test_jet")
+    variants = (
+        ("no_flux_cd", dict(enable_flux_cd=False), None, None),
+        ("eight_wave", dict(enable_flux_cd=False, eight_wave=True), None, None),
+        ("outflow", dict(bc=(1,) * 6), None, None),
+        ("outflow_user_plugins", dict(bc=(1,) * 6), gravity, jet),
+    )
+    res = {}
+    for name, kw, src, hook in variants:
+        log(f"extensions section: {name}, {steps} steps")
+        prm = _capi.make_params(n, n, n, dx, dy, dz, **kw)
+        with _capi.Subdomain(prm) as dev:
+            if src:
+                dev.set_user_sources(*src)
+            if hook:
+                dev.set_user_bc(*hook)
+            run_steps(dev, u0, 2, warmup=0)
+            dev.upload_u(u0)
+            dev.boundary_u()
+            dev.sync()
+            dev.kernel_times()
+            dev.set_timing(True)
+            t = 0.0
+            t0 = time.perf_counter()
+            for it in range(1, steps + 1):
+                dt, _ = timestep_clamp(it, 10, 0.4, dev.cfl_dtp(), t, 1e30)
+                dev.set_time(t)
+                dev.tstep(dt)
+                t += dt
+            dev.sync()
+            el = time.perf_counter() - t0
+            kt = dev.kernel_times()
+        res[name] = {"value": round(n ** 3 * steps / el, 1), "ms_per_step": round(1e3 * el / steps, 4),
+                     "kernel_ms_per_step": {k: round(v[0] / steps, 4) for k, v in kt.items() if v[1]}}
+    src_ms = res["eight_wave"]["kernel_ms_per_step"].get("sources", 0.0)
+    # the source pass per launch: read the stage input (8) and the updated state (8), write it (8)
+    src_gbs = 192.0 * n ** 3 / (src_ms / 2 / 1e3) / 1e9 if src_ms else None
+    return {"workload": f"turbulence-{n}^3-fp64-hlld-minmod, {steps} steps per variant", "unit": "zone-updates/s",
+            "variants": res,
+            "eight_wave_overhead": round(res["eight_wave"]["ms_per_step"] / res["no_flux_cd"]["ms_per_step"] - 1, 4),
+            "sources_pass_achieved_GBs": round(src_gbs, 1) if src_gbs else None,
+            "user_plugins_overhead": round(res["outflow_user_plugins"]["ms_per_step"]
+                                           / res["outflow"]["ms_per_step"] - 1, 4)}
+
+
 def serving_section(nreq: int, seed: int, fractions=(0.5, 1.0, 2.0)):
     """Continuous batching of a Poisson stream of mixed solves (Brio-Wu
     1024x4x4, Orszag-Tang 256^2, blasts 64^3-192^3) against serving the same
@@ -333,6 +394,8 @@ def main():
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_ach / HBM_PEAK_GBS, 4)},
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items() if v[1]},
     }
+    if not args.no_extensions:
+        out["extensions"] = extensions_section(u0, dx, dy, dz, n, 10)
     if not args.no_serving:
         out["serving"] = serving_section(args.serving_requests, args.seed)
     if not args.no_batch:
