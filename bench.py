@@ -253,22 +253,25 @@ def batch_section(nbox: int, steps: int, seed: int):
 def extensions_section(u0, dx, dy, dz, n: int, steps: int):
     """The step with the reference's optional terms, FP64 on the seeded
     turbulence box, against the same box without them: Powell 8-wave
-    source terms (no flux-CD, as the 8-wave scheme is run), and the user
+    source terms (no flux-CD, as the 8-wave scheme is run), the user
     plugins -- a source-term kernel (uniform + point-mass gravity) and a
     boundary hook (inflow jet + interior obstacle; ghosts then live in
     memory: boundaryII + hook between the stages, boundaryI + hook after)
-    -- on outflow faces.  Per variant: zone-updates/s and the per-class
-    kernel times."""
+    -- on outflow faces, and DMC table cooling (T ~ 1e6 K).  Per variant:
+    zone-updates/s and the per-class kernel times."""
     from guacho_amd import _capi
     from guacho_amd.host import timestep_clamp
     plugins = os.path.join(ROOT, "tests", "plugins")
     gravity = (os.path.join(plugins, "gravity.hsaco"), "gm_test_gravity")
     jet = (os.path.join(plugins, "jet.hsaco"), "gm_test_jet")
+    import numpy as np
+    dmc = np.load(os.path.join(ROOT, "tests", "golden", "ref_cooling.npz"))["dmc_table"]
     variants = (
         ("no_flux_cd", dict(enable_flux_cd=False), None, None),
         ("eight_wave", dict(enable_flux_cd=False, eight_wave=True), None, None),
         ("outflow", dict(bc=(1,) * 6), None, None),
         ("outflow_user_plugins", dict(bc=(1,) * 6), gravity, jet),
+        ("cooling_dmc", dict(), None, None),
     )
     res = {}
     for name, kw, src, hook in variants:
@@ -279,6 +282,8 @@ def extensions_section(u0, dx, dy, dz, n: int, steps: int):
                 dev.set_user_sources(*src)
             if hook:
                 dev.set_user_bc(*hook)
+            if name == "cooling_dmc":
+                dev.set_cooling(_capi.GM_COOL_DMC, dmc, 1e6, 1e-22, 10.0)
             run_steps(dev, u0, 2, warmup=0)
             dev.upload_u(u0)
             dev.boundary_u()
@@ -305,7 +310,8 @@ def extensions_section(u0, dx, dy, dz, n: int, steps: int):
             "eight_wave_overhead": round(res["eight_wave"]["ms_per_step"] / res["no_flux_cd"]["ms_per_step"] - 1, 4),
             "sources_pass_achieved_GBs": round(src_gbs, 1) if src_gbs else None,
             "user_plugins_overhead": round(res["outflow_user_plugins"]["ms_per_step"]
-                                           / res["outflow"]["ms_per_step"] - 1, 4)}
+                                           / res["outflow"]["ms_per_step"] - 1, 4),
+            "cooling_ms_per_step": res["cooling_dmc"]["kernel_ms_per_step"].get("cooling")}
 
 
 def serving_section(nreq: int, seed: int, fractions=(0.5, 1.0, 2.0)):

@@ -16,14 +16,15 @@ LIB_PATH = os.path.join(_HERE, "libguacho_amd.so")
 
 GM_OK, GM_ERR_CONFIG, GM_ERR_HIP, GM_ERR_NONFINITE, GM_ERR_OOM, GM_ERR_ARG, GM_ERR_BUSY = 0, 1, 2, 3, 4, 5, 6
 GM_PRECISION_FP64, GM_PRECISION_MIXED = 0, 1
-GM_NKCLASS = 9
+GM_COOL_NONE, GM_COOL_DMC, GM_COOL_CHI = 0, 3, 4
+GM_NKCLASS = 10
 
 # exported entry points, in the order include/guacho_amd.h declares them
 EXPORTS = (
     "gm_create", "gm_destroy", "gm_last_error", "gm_upload_u", "gm_download_u",
     "gm_download_primit", "gm_boundary_u", "gm_cfl_dtp", "gm_tstep", "gm_sync",
     "gm_download_divb", "gm_set_timing", "gm_kernel_times", "gm_device_bytes",
-    "gm_set_time", "gm_set_user_sources", "gm_set_user_bc",
+    "gm_set_time", "gm_set_user_sources", "gm_set_user_bc", "gm_set_cooling",
     "gm_batch_create", "gm_batch_destroy", "gm_batch_last_error", "gm_batch_box_bytes",
     "gm_batch_free_bytes", "gm_batch_add", "gm_batch_remove", "gm_batch_advance", "gm_batch_sync",
     "gm_batch_clock", "gm_batch_download_u", "gm_batch_tuning", "gm_batch_active",
@@ -91,6 +92,7 @@ def lib() -> C.CDLL:
     L.gm_set_time.argtypes = [vp, C.c_double]
     L.gm_set_user_sources.argtypes = [vp, C.c_char_p, C.c_char_p]
     L.gm_set_user_bc.argtypes = [vp, C.c_char_p, C.c_char_p]
+    L.gm_set_cooling.argtypes = [vp, C.c_int32, C.c_void_p, C.c_double, C.c_double, C.c_double]
     P = C.POINTER(GmParams)
     L.gm_batch_create.argtypes = [C.POINTER(GmBatchParams), C.POINTER(vp)]
     L.gm_batch_destroy.argtypes = [vp]
@@ -226,6 +228,17 @@ class Subdomain:
         co = code_object.encode() if code_object else None
         kn = kernel.encode() if kernel else None
         self._check(self.L.gm_set_user_bc(self.h, co, kn))
+
+    def set_cooling(self, kind: int, cooltab=None, tempsc: float = 1.0, psc: float = 1.0, tsc: float = 1.0) -> None:
+        """Table cooling after each step (GM_COOL_DMC / GM_COOL_CHI; 0 off).
+        cooltab: the reference's cooltab(2, 41)."""
+        if kind == GM_COOL_NONE:
+            self._check(self.L.gm_set_cooling(self.h, 0, None, tempsc, psc, tsc))
+            return
+        tab = np.asfortranarray(cooltab, dtype=np.float64)
+        if tab.shape != (2, 41):
+            raise ValueError("cooltab must be (2, 41)")
+        self._check(self.L.gm_set_cooling(self.h, int(kind), tab.ctypes.data_as(C.c_void_p), tempsc, psc, tsc))
 
     def kernel_times(self) -> Dict[str, tuple]:
         ms = (C.c_double * GM_NKCLASS)()

@@ -22,8 +22,8 @@
 namespace {
 
 const char* kClassNames[GM_NKCLASS] = {"xy_predictor", "xy_corrector", "z_predictor", "z_corrector",
-                                       "bfield", "cfl_dt", "ghost", "layout", "sources"};
-enum { KC_XY1, KC_XY2, KC_Z1, KC_Z2, KC_BFIELD, KC_CFL, KC_GHOST, KC_LAYOUT, KC_SOURCES };
+                                       "bfield", "cfl_dt", "ghost", "layout", "sources", "cooling"};
+enum { KC_XY1, KC_XY2, KC_Z1, KC_Z2, KC_BFIELD, KC_CFL, KC_GHOST, KC_LAYOUT, KC_SOURCES, KC_COOLING };
 
 thread_local std::string g_create_error;
 
@@ -62,6 +62,7 @@ struct gm_ctx {
   hipFunction_t usrc_fn = nullptr;
   hipModule_t ubc_mod = nullptr;        // user boundary plugin (ghosts then live in memory)
   hipFunction_t ubc_fn = nullptr;
+  gm::Cooling cool{};                   // kind 0: no cooling
   gm::Shape shape = {256, 16};  // tile width, z-pencil length
   int timing = 0;
   struct Rec { int cls; hipEvent_t a, b; };
@@ -312,6 +313,8 @@ void tstep_t(gm_ctx* c, double dt) {
   timed(c, KC_Z2, [&] { gm::launch_z(s, b, dt, 2, solver, fcd, err, sh); });
   if (fcd) timed(c, KC_BFIELD, [&] { gm::launch_bfield(s, b, dt, 2, sh); });
   stage_sources_t(c, b, dt, 2);
+  // cooling (src/hydro_solver.f90:206-210)
+  if (c->cool.kind) timed(c, KC_COOLING, [&] { gm::launch_cooling(s, b, c->cool, dt); });
   // boundaryI on U^{n+1} (src/hydro_solver.f90:216) when ghosts live in memory
   if (b.g.gmem) fill_ghost_t<S>(c, b.U, 1);
 }
@@ -609,6 +612,31 @@ int gm_set_user_bc(gm_ctx* c, const char* code_object, const char* kernel) {
   // from here on the kernels read U's ghosts as stored: materialise them
   if (c->ghosts_dirty) fill_ghost1(c);
   GM_HIP(c, hipGetLastError());
+  return GM_OK;
+}
+
+int gm_set_cooling(gm_ctx* c, int32_t kind, const double* cooltab, double tempsc, double psc, double tsc) {
+  if (!c) return GM_ERR_ARG;
+  if (kind == GM_COOL_NONE) {
+    c->cool.kind = 0;
+    return GM_OK;
+  }
+  if (kind != GM_COOL_DMC && kind != GM_COOL_CHI)
+    return fail(c, GM_ERR_CONFIG, "cooling kind must be GM_COOL_NONE, GM_COOL_DMC or GM_COOL_CHI");
+  if (!cooltab) return fail(c, GM_ERR_ARG, "cooltab is NULL");
+  if (!(tempsc > 0) || !(psc > 0) || !(tsc > 0) || !std::isfinite(tempsc) || !std::isfinite(psc) ||
+      !std::isfinite(tsc))
+    return fail(c, GM_ERR_CONFIG, "tempsc, psc and tsc must be positive and finite");
+  for (int i = 0; i < 82; i++)
+    if (!std::isfinite(cooltab[i])) return fail(c, GM_ERR_CONFIG, "cooltab holds a non-finite value");
+  gm::Cooling cl{};
+  cl.kind = kind;
+  cl.tempsc = tempsc;
+  cl.psc = psc;
+  cl.tsc = tsc;
+  cl.hicoef = kind == GM_COOL_DMC ? 0.27e-26 : 0.21e-26;  // cooling_dmc.f90:88, cooling_chi.f90
+  std::memcpy(cl.tab, cooltab, sizeof cl.tab);
+  c->cool = cl;
   return GM_OK;
 }
 

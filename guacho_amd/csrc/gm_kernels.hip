@@ -511,6 +511,48 @@ __device__ __forceinline__ void sources_body(const Box<S>& b, unsigned lb, doubl
 }
 
 // ---------------------------------------------------------------------------
+// Table cooling, coolingdmc (src/cooling_dmc.f90:115-156) / coolingchi
+// (src/cooling_chi.f90): every interior cell above Tmin = 1e4 K loses
+// thermal energy, p -> p exp(-Ce dt tsc), Ce = Lambda(T) rho^2 / (cv p Psc),
+// Lambda from cooldmc / coolchi (sqrt law above 1e8 K, else linear
+// interpolation at table index int(10 log10 T) - 39).  E changes by the
+// thermal energy lost.  exp and log10 are the device's (within 1 ulp of the
+// host's libm).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double cool_lambda(const Cooling& cl, double T) {
+  if (T > 1e8) return cl.hicoef * __builtin_sqrt(T);
+  int if1 = (int)(log10(T) * 10) - 39;
+  if1 = if1 < 1 ? 1 : (if1 > 40 ? 40 : if1);  // T = 1e8 exactly would read past the 41-point table
+  const double T0 = cl.tab[2 * (if1 - 1)], c0 = cl.tab[2 * (if1 - 1) + 1];
+  const double T1 = cl.tab[2 * if1], c1 = cl.tab[2 * if1 + 1];
+  return ((c1 - c0) * (T - T0)) / (T1 - T0) + c0;
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_cooling(Box<S> b, Cooling cl, double dt) {
+  const Geom& g = b.g;
+  const long long n = (long long)g.nx * g.ny * g.nz;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % g.nx) + 1;
+  const long long r = t / g.nx;
+  const int j = (int)(r % g.ny) + 1, k = (int)(r / g.ny) + 1;
+  const long long c = cidx(g, i, j, k), N = g.ncell;
+  double u[8], p[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) u[q] = (double)b.U[q * N + c];
+  u2prim(b.ph, u, p);
+  const double T = (p[4] / p[0]) * cl.tempsc;
+  if (!(T > 10000.)) return;
+  const double eth0 = b.ph.cv * p[4];
+  const double aloss = cool_lambda(cl, T);
+  const double dens = p[0];
+  const double ce = (aloss * (dens * dens)) / (eth0 * cl.psc);
+  const double pn = p[4] * exp(-(ce * (dt * cl.tsc)));
+  b.U[4 * N + c] = (S)((u[4] - eth0) + b.ph.cv * pn);
+}
+
+// ---------------------------------------------------------------------------
 // CFL candidate reduction: the min loop of get_timestep
 // (src/hydro_core.f90:642-675) on the primitives of the interior.  Exact
 // (min is order-free); block minima meet in one 64-bit atomicMin on the
@@ -867,6 +909,12 @@ void launch_cfl(hipStream_t s, const Box<S>& b, unsigned long long* dtmin, int* 
 }
 
 template <typename S>
+void launch_cooling(hipStream_t s, const Box<S>& b, const Cooling& cl, double dt) {
+  const long long n = (long long)b.g.nx * b.g.ny * b.g.nz;
+  hipLaunchKernelGGL(k_cooling<S>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, b, cl, dt);
+}
+
+template <typename S>
 void launch_sources(hipStream_t s, const Box<S>& b, double dt, int stage, const Shape& sh) {
   hipLaunchKernelGGL(k_sources<S>, dim3(box_tiles(b.g, KC_COL, sh)), dim3(sh.block), 0, s, b, dt, stage, sh.kchunk);
 }
@@ -955,6 +1003,7 @@ void launch_divb(hipStream_t s, const Geom& g, const S* U, double dx, double dy,
   template void launch_bfield<S>(hipStream_t, const Box<S>&, double, int, const Shape&);                          \
   template void launch_cfl<S>(hipStream_t, const Box<S>&, unsigned long long*, int*, const Shape&);               \
   template void launch_sources<S>(hipStream_t, const Box<S>&, double, int, const Shape&);                         \
+  template void launch_cooling<S>(hipStream_t, const Box<S>&, const Cooling&, double);                            \
   template void launch_sources_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned,               \
                                         const BoxClock*, int, const Shape&);                                      \
   template void launch_xy_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, const BoxClock*,   \

@@ -41,6 +41,15 @@ struct Box {
 
 enum : int { SRC_EIGHT_WAVE = 1, SRC_USER = 2 };
 
+// Table cooling after the step (coolingdmc / coolingchi): the reference's
+// cooltab(2, 41) (T_i, Lambda_i pairs), the sqrt-law coefficient above
+// 1e8 K and the parameters module's Tempsc, Psc, tsc.
+struct Cooling {
+  int kind;            // 3 (DMC) or 4 (CHIANTI)
+  double tempsc, psc, tsc, hicoef;
+  double tab[82];
+};
+
 // Per-box clock of a batch: the reference's time-loop state (time,
 // currentIteration) plus the CFL minimum of the current step and the stop
 // condition (t_end reached by the dump clamp, or max_steps taken).
@@ -92,6 +101,9 @@ template <typename S>
 void launch_bfield(hipStream_t s, const Box<S>& b, double dt, int stage, const Shape& sh);
 template <typename S>
 void launch_cfl(hipStream_t s, const Box<S>& b, unsigned long long* dtmin, int* err, const Shape& sh);
+// cooling pass on U^{n+1} (src/hydro_solver.f90:206-210) with the step's dt
+template <typename S>
+void launch_cooling(hipStream_t s, const Box<S>& b, const Cooling& cl, double dt);
 // source terms of step() (src/hydro_solver.f90:115-121): up += dt * s after
 // the stage's update, s from the stage's input state
 template <typename S>

@@ -54,6 +54,10 @@ class Parameters:
     eight_wave: bool = False
     user_source_terms: bool = False
     bc_user: bool = False
+    cooling: int = _capi.GM_COOL_NONE                # COOL_NONE / COOL_DMC / COOL_CHI
+    Tempsc: float = 1.0                              # parameters module scalings
+    Psc: float = 1.0
+    tsc: float = 1.0
     cfl: float = 0.4
     tmax: float = 0.5
     dtprint: float = 0.1
@@ -132,11 +136,26 @@ def get_timestep(dev: _capi.Subdomain, prm: Parameters, g: Globals, n_iter: int 
     return timestep_clamp(g.currentIteration, n_iter, prm.cfl, dtp, g.time, g.tprint)
 
 
+def read_cooling_table(path: str, kind: int):
+    """cooltab(2, 41) as read_table_dmc / read_table_chianti build it from
+    their .tab files (src/cooling_dmc.f90:57-66): DMC rows (a, b) become
+    (10**a, 10**(-b)); CHIANTI rows are taken as they are."""
+    import numpy as np
+    rows = np.loadtxt(path, converters=lambda s: float(s.replace("d", "e").replace("D", "e")))[:41]
+    tab = np.empty((2, 41), order="F")
+    if kind == _capi.GM_COOL_DMC:
+        tab[0] = [10.0 ** a for a in rows[:, 0]]
+        tab[1] = [10.0 ** (-b) for b in rows[:, 1]]
+    else:
+        tab[0], tab[1] = rows[:, 0], rows[:, 1]
+    return tab
+
+
 def run(prm: Parameters, g: Globals, device: int = 0,
         write_output: Optional[Callable[[int, Globals, _capi.Subdomain], None]] = None,
         max_steps: Optional[int] = None, log: Optional[Callable[[str], None]] = None,
         user_sources: Optional[Tuple[str, str]] = None,
-        user_bc: Optional[Tuple[str, str]] = None) -> Globals:
+        user_bc: Optional[Tuple[str, str]] = None, cooltab=None) -> Globals:
     """The time loop of src/main.f90:70-125 with the step on the GPU.
 
     ``g.u`` holds the initial conditions (ghosts included).  boundaryI and
@@ -144,7 +163,9 @@ def run(prm: Parameters, g: Globals, device: int = 0,
     is called at t=0 and at every dump, as write_output is in the reference.
     ``user_sources`` = (code object, kernel) is the device form of the
     problem's get_user_source_terms (needed when user_source_terms is set),
-    ``user_bc`` that of impose_user_bc (needed when bc_user is set).
+    ``user_bc`` that of impose_user_bc (needed when bc_user is set);
+    ``cooltab`` is cooltab(2, 41) for DMC / CHIANTI cooling
+    (read_cooling_table).
     Returns ``g`` with ``u`` downloaded at the end.
     """
     if prm.user_source_terms and user_sources is None:
@@ -156,6 +177,10 @@ def run(prm: Parameters, g: Globals, device: int = 0,
             dev.set_user_sources(*user_sources)
         if prm.bc_user:
             dev.set_user_bc(*user_bc)
+        if prm.cooling != _capi.GM_COOL_NONE:
+            if cooltab is None:
+                raise ValueError("cooling needs its table: pass cooltab (read_cooling_table)")
+            dev.set_cooling(prm.cooling, cooltab, prm.Tempsc, prm.Psc, prm.tsc)
         dev.upload_u(g.u)
         dev.set_time(g.time)
         dev.boundary_u()                                     # main.f90:76

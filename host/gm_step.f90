@@ -55,8 +55,10 @@ contains
     p%cv = cv
     p%gamma = gamma
     p%eta = eta
-    if (cooling /= COOL_NONE .or. th_cond /= TC_OFF .or. dif_rad .or. eq_of_state /= EOS_ADIABATIC) then
-      print '(a)', 'guacho_amd: cooling, conduction, diffuse radiation and non-adiabatic EOS are not built'
+    if ((cooling /= COOL_NONE .and. cooling /= COOL_DMC .and. cooling /= COOL_CHI) .or. th_cond /= TC_OFF &
+        .or. dif_rad .or. eq_of_state /= EOS_ADIABATIC) then
+      print '(a)', 'guacho_amd: cooling other than DMC/CHIANTI tables, conduction, diffuse radiation and '// &
+                   'non-adiabatic EOS are not built'
       stop 1
     end if
     if (radiation_pressure) then
@@ -64,10 +66,38 @@ contains
       stop 1
     end if
     call gm_check(gm_create(p, gm_ctx), 'gm_create')
+    if (cooling == COOL_DMC .or. cooling == COOL_CHI) call load_cooling()
     if (user_source_terms) call load_user_sources()
     if (bc_user) call load_user_bc()
     call gm_check(gm_upload_u(gm_ctx, u), 'gm_upload_u')
   end subroutine gm_device_init
+
+  !> the cooling table as read_table_dmc / read_table_chianti read it
+  !> (src/cooling_dmc.f90:57-66: 10**a, 10**(-b) for DMC; raw for CHIANTI)
+  subroutine load_cooling()
+    real(c_double) :: tab(2, 41)
+    real(kind=8) :: a, b
+    integer :: i
+    character(len=256) :: path
+    if (cooling == COOL_DMC) then
+      path = trim(workdir)//'../src/DMClib/coolingDMC.tab'
+    else
+      path = trim(workdir)//'../src/CHIANTIlib/coolingCHIANTI.tab'
+    end if
+    open(unit=10, file=trim(path), status='old')
+    do i = 1, 41
+      read(10, *) a, b
+      if (cooling == COOL_DMC) then
+        tab(1, i) = 10.e0**(a)
+        tab(2, i) = 10.e0**(-b)
+      else
+        tab(1, i) = a
+        tab(2, i) = b
+      end if
+    end do
+    close(unit=10)
+    call gm_check(gm_set_cooling(gm_ctx, int(cooling, c_int), tab, Tempsc, Psc, tsc), 'gm_set_cooling')
+  end subroutine load_cooling
 
   !> user_source_terms: get_user_source_terms runs on the device as a
   !> plugin kernel; GUACHO_USER_SOURCES names it as <code object>:<kernel>

@@ -114,6 +114,14 @@ module guacho_capi
       type(c_ptr), value :: ctx
       character(kind=c_char), intent(in) :: code_object(*), kernel(*)
     end function gm_set_user_bc
+
+    integer(c_int) function gm_set_cooling(ctx, kind, cooltab, tempsc, psc, tsc) bind(C, name='gm_set_cooling')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: kind
+      real(c_double), intent(in) :: cooltab(*)
+      real(c_double), value :: tempsc, psc, tsc
+    end function gm_set_cooling
   end interface
 
 contains

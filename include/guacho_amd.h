@@ -140,7 +140,7 @@ int gm_download_divb(gm_ctx *ctx, double *divb);
  * kernel on the context's stream.  gm_kernel_times returns, per kernel
  * class, the accumulated milliseconds and launch count since the last
  * reset, and the kernel names (static strings). */
-#define GM_NKCLASS 9
+#define GM_NKCLASS 10
 int gm_set_timing(gm_ctx *ctx, int enable);
 int gm_kernel_times(gm_ctx *ctx, double *ms, int64_t *launches, const char **names);
 
@@ -176,6 +176,24 @@ int gm_set_user_sources(gm_ctx *ctx, const char *code_object, const char *kernel
  * face and no hook fails with GM_ERR_CONFIG.  code_object NULL removes the
  * hook.  Hooks are not available in batches. */
 int gm_set_user_bc(gm_ctx *ctx, const char *code_object, const char *kernel);
+
+/* ---- post-step physics ------------------------------------------------ */
+#define GM_COOL_NONE 0   /* constants.f90:72-77 */
+#define GM_COOL_DMC  3
+#define GM_COOL_CHI  4
+
+/* Table cooling at the end of every gm_tstep, after the corrector and before
+ * the boundary fill: coolingdmc (src/cooling_dmc.f90:115-156) for
+ * GM_COOL_DMC, coolingchi (src/cooling_chi.f90) for GM_COOL_CHI.  cooltab
+ * holds the reference's cooltab(2, 41) as its read_table leaves it (column
+ * major: T_1, Lambda_1, T_2, Lambda_2, ...; for DMC already 10**a,
+ * 10**(-b)); tempsc, psc, tsc are the parameters module's Tempsc, Psc and
+ * tsc.  GM_COOL_NONE turns cooling off (cooltab may then be NULL).  The
+ * reference keeps the cooled pressure in primit for the next step; the
+ * device recovers it from E, which differs at the rounding level, and its
+ * exp/log10 are within 1 ulp of the host's: cooled runs match the reference
+ * to a tolerance, not bit for bit. */
+int gm_set_cooling(gm_ctx *ctx, int32_t kind, const double *cooltab, double tempsc, double psc, double tsc);
 
 /* ---- batches of independent boxes ----------------------------------- */
 /* No counterpart in the reference (one domain per run there).  A batch

@@ -43,6 +43,8 @@ struct orc_state {
   int NX, NY, NZ;   /* extents incl. ghosts: n+4 */
   long ncell;
   double time;      /* globals::time */
+  int cool_kind;    /* 0, COOL_DMC (3) or COOL_CHI (4) */
+  double cool_tab[82], cool_tempsc, cool_psc, cool_tsc;
   double *u, *up, *primit, *f, *g, *h, *e, *temp;
 };
 
@@ -732,6 +734,62 @@ static void calcprim_arr(orc_state *s, const double *src) {
   for (long c = 0; c < n; c++) orc_u2prim(p, &src[c * NEQ], &s->primit[c * NEQ], &s->temp[c]);
 }
 void orc_calcprim_u(orc_state *s) { calcprim_arr(s, s->u); }
+
+/* calcprim(u, primit, only_ghost=.true.): every cell outside the interior */
+static void calcprim_ghosts(orc_state *s) {
+  const orc_params *p = &s->p;
+  for (int k = -1; k <= p->nz + 2; k++)
+    for (int j = -1; j <= p->ny + 2; j++)
+      for (int i = -1; i <= p->nx + 2; i++) {
+        if (i >= 1 && i <= p->nx && j >= 1 && j <= p->ny && k >= 1 && k <= p->nz) continue;
+        const long c = cidx(s, i, j, k);
+        orc_u2prim(p, &s->u[c * NEQ], &s->primit[c * NEQ], &s->temp[c]);
+      }
+}
+
+void orc_set_cooling(orc_state *s, int kind, const double *cooltab, double tempsc, double psc, double tsc) {
+  s->cool_kind = kind;
+  if (cooltab) memcpy(s->cool_tab, cooltab, sizeof s->cool_tab);
+  s->cool_tempsc = tempsc;
+  s->cool_psc = psc;
+  s->cool_tsc = tsc;
+}
+
+/* cooldmc / coolchi (cooling_dmc.f90:79-104): sqrt law above 1e8 K, else
+ * linear interpolation in the 41-point table at index int(10 log10 T) - 39 */
+static double cool_lambda(const orc_state *s, double T) {
+  if (T > 1e8) return (s->cool_kind == 3 ? 0.27e-26 : 0.21e-26) * sqrt(T);
+  const int if1 = (int)(log10(T) * 10) - 39;
+  const double *t = s->cool_tab;
+  const double T0 = t[2 * (if1 - 1)], c0 = t[2 * (if1 - 1) + 1];
+  const double T1 = t[2 * if1], c1 = t[2 * if1 + 1];
+  return ((c1 - c0) * (T - T0)) / (T1 - T0) + c0;
+}
+
+/* coolingdmc / coolingchi (cooling_dmc.f90:115-156): for every interior cell
+ * above Tmin = 1e4 K, p *= exp(-Ce dt), Ce = Lambda rho^2 / (cv p Psc), and
+ * E changes by the same thermal energy; primit keeps the cooled pressure */
+void orc_cooling(orc_state *s, double dt) {
+  const orc_params *p = &s->p;
+  const double dt_seconds = dt * s->cool_tsc;
+  for (int k = 1; k <= p->nz; k++)
+    for (int j = 1; j <= p->ny; j++)
+      for (int i = 1; i <= p->nx; i++) {
+        double *uu = &AT(s->u, s, 0, i, j, k);
+        double *pm = &AT(s->primit, s, 0, i, j, k);
+        double tdum;
+        orc_u2prim(p, uu, pm, &tdum);
+        const double T = (pm[4] / pm[0]) * s->cool_tempsc;            /* hydro_core.f90:94 */
+        if (T > 10000.) {
+          const double eth0 = p->cv * pm[4];
+          const double aloss = cool_lambda(s, T);
+          const double dens = pm[0];
+          const double ce = (aloss * (dens * dens)) / (eth0 * s->cool_psc);
+          pm[4] = pm[4] * exp(-(ce * dt_seconds));
+          uu[4] = (uu[4] - eth0) + p->cv * pm[4];
+        }
+      }
+}
 void orc_calcprim_up(orc_state *s) { calcprim_arr(s, s->up); }
 
 /* hlldfluxes / hllefluxes / hllfluxes / hllcfluxes, e.g. hlld.f90:331-432:
@@ -972,8 +1030,10 @@ int orc_tstep(orc_state *s, double dt) {
   bad |= orc_fluxes(s, 2);                                           /* :177 */
   orc_step(s, dt);                                                   /* :184 */
   orc_viscous_copy(s);                                               /* :188 */
+  if (s->cool_kind) orc_cooling(s, dt);                              /* :206-210 */
   orc_boundaryI(s);                                                  /* :216 */
-  orc_calcprim_u(s);                                                 /* :220 */
+  if (s->cool_kind) calcprim_ghosts(s);                              /* :222-223 */
+  else orc_calcprim_u(s);                                            /* :220 */
   return bad;
 }
 

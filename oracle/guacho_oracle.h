@@ -61,6 +61,13 @@ void orc_get_flux(const orc_state *s, int axis, double *fgh);   /* axis 0:f 1:g 
 void orc_get_e(const orc_state *s, double *e);                  /* e(3, ...) */
 
 void orc_set_time(orc_state *s, double time);   /* globals::time, seen by the user hook */
+/* Table cooling after each step (kind 3: coolingdmc, cooling_dmc.f90:115-156;
+ * kind 4: coolingchi, cooling_chi.f90), cooltab(2, 41) as the reference reads
+ * it; kind 0 turns it off.  tempsc, psc, tsc: Tempsc, Psc, tsc of the
+ * parameters module. */
+void orc_set_cooling(orc_state *s, int kind, const double *cooltab, double tempsc, double psc, double tsc);
+/* one cooling pass on u and primit (interior) with dt_CFL = dt */
+void orc_cooling(orc_state *s, double dt);
 void orc_boundaryI(orc_state *s);
 void orc_boundaryII(orc_state *s);
 void orc_calcprim_u(orc_state *s);

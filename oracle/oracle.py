@@ -117,6 +117,8 @@ def _load_oracle() -> C.CDLL:
     L.orc_cfl_dtp.restype = C.c_double
     L.orc_source.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, _dp]
     L.orc_set_time.argtypes = [C.c_void_p, C.c_double]
+    L.orc_set_cooling.argtypes = [C.c_void_p, C.c_int, _dpF, C.c_double, C.c_double, C.c_double]
+    L.orc_cooling.argtypes = [C.c_void_p, C.c_double]
     L.orc_timestep_finish.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                       C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.orc_u2prim.argtypes = [P, _dp, _dp, C.POINTER(C.c_double)]
@@ -235,6 +237,14 @@ class Oracle:
     def cfl_dtp(self) -> float:
         return self.L.orc_cfl_dtp(self.h)
 
+    def set_cooling(self, kind: int, cooltab, tempsc: float = 1.0, psc: float = 1.0, tsc: float = 1.0) -> None:
+        """Table cooling (3: DMC, 4: CHIANTI) after each step; cooltab is (2, 41)."""
+        tab = np.asfortranarray(cooltab if cooltab is not None else np.zeros((2, 41)), dtype=np.float64)
+        self.L.orc_set_cooling(self.h, int(kind), tab, tempsc, psc, tsc)
+
+    def cooling(self, dt: float) -> None:
+        self.L.orc_cooling(self.h, float(dt))
+
     def set_time(self, time: float) -> None:
         """globals::time, as the user hook sees it."""
         self.L.orc_set_time(self.h, float(time))
@@ -280,6 +290,13 @@ class OracleFunctions:
         pl = np.array(pl, dtype=np.float64); pr = np.array(pr, dtype=np.float64)
         self.L.orc_limiter(C.byref(self.p), np.ascontiguousarray(pll), pl, pr, np.ascontiguousarray(prr))
         return pl, pr
+
+    def cooling_table(self, kind):
+        """cooltab(2, 41) as the reference reads it (3: DMC, 4: CHIANTI)."""
+        t = np.zeros((2, 41), order="F"); self.L.ref_cooling_table(kind, t); return t
+
+    def cooling(self, kind, dt):
+        self.L.ref_cooling(kind, dt)
 
     def source(self, i, j, k, prim):
         """source(i,j,k,prim,s) of this build (sources.f90:188-220), from the
@@ -346,6 +363,8 @@ class RefLib:
                                        C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.ref_source.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp]
         L.ref_set_time.argtypes = [C.c_double]
+        L.ref_cooling_table.argtypes = [C.c_int, _dpF]
+        L.ref_cooling.argtypes = [C.c_int, C.c_double]
         sw = (C.c_int * 3)()
         L.ref_switches(sw)
         self.eight_wave, self.user_source_terms, self.bc_user = bool(sw[0]), bool(sw[1]), bool(sw[2])
@@ -411,6 +430,13 @@ class RefLib:
 
     def swap(self, axis, v):
         v = np.array(v, dtype=np.float64); self.L.ref_swap(axis, v); return v
+
+    def cooling_table(self, kind):
+        """cooltab(2, 41) as the reference reads it (3: DMC, 4: CHIANTI)."""
+        t = np.zeros((2, 41), order="F"); self.L.ref_cooling_table(kind, t); return t
+
+    def cooling(self, kind, dt):
+        self.L.ref_cooling(kind, dt)
 
     def source(self, i, j, k, prim):
         """source(i,j,k,prim,s) of this build (sources.f90:188-220), from the

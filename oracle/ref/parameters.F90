@@ -77,6 +77,9 @@ module parameters
   integer, parameter :: nymin = 1 - nghost, nymax = ny + nghost
   integer, parameter :: nzmin = 1 - nghost, nzmax = nz + nghost
   integer, parameter :: master = 0
+  ! the cooling modules read src/<lib>/<table> as workdir//'../src/...';
+  ! the build writes workdir (the reference's src/) into refpaths.inc
+#include "refpaths.inc"
   integer, parameter :: mpi_real_kind = 0
 
 end module parameters

@@ -21,6 +21,8 @@ module ref_kat
   use hllc, only : prim2fhllc, hllcfluxes
   use boundaries, only : boundaryI, boundaryII
   use sources, only : source
+  use cooling_dmc, only : init_cooling_dmc, coolingdmc, cooltab_dmc
+  use cooling_chi, only : init_cooling_chianti, coolingchi, cooltab_chianti
   implicit none
 
 contains
@@ -190,5 +192,34 @@ contains
     real(c_double), value :: t
     time = t
   end subroutine ref_set_time
+
+  ! cooling tables as the reference reads them (kind 3: DMC, 4: CHIANTI),
+  ! cooltab(2, 41)
+  subroutine ref_cooling_table(kind, tab) bind(C, name='ref_cooling_table')
+    integer(c_int), value :: kind
+    real(c_double), intent(out) :: tab(2, 41)
+    if (kind == COOL_DMC) then
+      if (.not. allocated(cooltab_dmc)) call init_cooling_dmc()
+      tab = cooltab_dmc
+    else
+      if (.not. allocated(cooltab_chianti)) call init_cooling_chianti()
+      tab = cooltab_chianti
+    end if
+  end subroutine ref_cooling_table
+
+  ! one cooling pass (coolingdmc / coolingchi) on globals u, primit with
+  ! dt_CFL = dt
+  subroutine ref_cooling(kind, dt) bind(C, name='ref_cooling')
+    integer(c_int), value :: kind
+    real(c_double), value :: dt
+    dt_CFL = dt
+    if (kind == COOL_DMC) then
+      if (.not. allocated(cooltab_dmc)) call init_cooling_dmc()
+      call coolingdmc()
+    else
+      if (.not. allocated(cooltab_chianti)) call init_cooling_chianti()
+      call coolingchi()
+    end if
+  end subroutine ref_cooling
 
 end module ref_kat

@@ -9,6 +9,7 @@ inputs and the reference's outputs for
   - one boundaryII + calcprim + limited sweep on a random state,
   - get_timestep's ramp and clamp (the boundary fills include the
     checker's impose_user_bc in bc_user builds, at globals::time = 0.3),
+  - (ref_cooling.npz) coolingdmc and coolingchi with the reference's tables,
   - source(i,j,k,prim,s) of sources.f90 at every interior cell of a random
     primitive array (zero, 8-wave, the checker's user gravity, or both,
     per the variant's switches).
@@ -138,6 +139,37 @@ def make(variant, seed):
     return out
 
 
+def make_cooling(seed=77):
+    """coolingdmc / coolingchi (cooling_dmc.f90:115-156, cooling_chi.f90) of
+    the reference on a random state whose temperatures span the table, the
+    sqrt law above 1e8 K and the uncooled range below 1e4 K; the tables as
+    the reference reads them from its own src/DMClib and src/CHIANTIlib."""
+    R = RefLib("hlld_minmod_periodic")
+    dx, dy, dz = 0.1, 0.07, 0.13
+    R.init(dx, dy, dz)
+    rng = np.random.default_rng(seed)
+    sh = R.shape()
+    rho = rng.uniform(0.5, 2.0, sh[1:])
+    v = rng.uniform(-1.0, 1.0, (3,) + sh[1:])
+    B = rng.uniform(-1.0, 1.0, (3,) + sh[1:])
+    pr = rho * 10.0 ** rng.uniform(3.5, 8.5, sh[1:])
+    u = np.zeros(sh, order="F")
+    u[0] = rho
+    u[1:4] = rho * v
+    u[4] = 0.5 * rho * (v ** 2).sum(0) + R.cv * pr + 0.5 * (B ** 2).sum(0)
+    u[5:8] = B
+    out = {"u_in": u, "dt": np.array([1e26]), "dxyz": np.array([dx, dy, dz]),
+           "config_real": np.array([R.cv, R.gamma, R.cfl])}
+    for kind, name in ((3, "dmc"), (4, "chi")):
+        out[f"{name}_table"] = R.cooling_table(kind)
+        R.set_array(0, u)
+        R.calcprim(0)
+        R.cooling(kind, 1e26)
+        out[f"{name}_u"] = R.get_array(0)
+        out[f"{name}_primit"] = R.get_array(2)
+    return out
+
+
 def main():
     vs = ref_variants()
     if not vs:
@@ -149,6 +181,10 @@ def main():
             continue
         np.savez_compressed(path, **make(v, seed_of(v)))
         print("wrote", v)
+    path = os.path.join(HERE, "ref_cooling.npz")
+    if rewrite or not os.path.exists(path):
+        np.savez_compressed(path, **make_cooling())
+        print("wrote cooling")
 
 
 if __name__ == "__main__":

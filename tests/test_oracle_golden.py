@@ -13,7 +13,9 @@ import pytest
 
 from oracle.oracle import Config, Oracle, OracleFunctions, RefLib, ref_variants, timestep_finish
 
-GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "ref_*.npz")))
+GOLDEN = sorted(p for p in glob.glob(os.path.join(os.path.dirname(__file__), "golden", "ref_*.npz"))
+                if not p.endswith("ref_cooling.npz"))
+COOLING = os.path.join(os.path.dirname(__file__), "golden", "ref_cooling.npz")
 
 
 def _cfg(z):
@@ -110,6 +112,39 @@ def test_sources_match_reference(path):
                 got = O.source(i + 1, j + 1, k + 1, prim[:, i + 2, j + 2, k + 2])
                 assert np.array_equal(got, want[i, j, k]), (i, j, k, got, want[i, j, k])
     O.close()
+
+
+@pytest.mark.parametrize("name,kind", [("dmc", 3), ("chi", 4)])
+def test_cooling_matches_reference(name, kind):
+    """coolingdmc / coolingchi of the reference (cooling_dmc.f90:115-156):
+    the restatement's u and interior primitives, bit for bit."""
+    z = np.load(COOLING)
+    dx, dy, dz = z["dxyz"]
+    u = z["u_in"]
+    cfg = Config(nx=u.shape[1] - 4, ny=u.shape[2] - 4, nz=u.shape[3] - 4, dx=dx, dy=dy, dz=dz,
+                 cv=float(z["config_real"][0]))
+    O = Oracle(cfg)
+    O.set_u(u)
+    O.calcprim_u()
+    O.set_cooling(kind, z[f"{name}_table"])
+    O.cooling(float(z["dt"][0]))
+    np.testing.assert_array_equal(O.get_u(), z[f"{name}_u"])
+    np.testing.assert_array_equal(O.get_primit()[:, 2:-2, 2:-2, 2:-2], z[f"{name}_primit"][:, 2:-2, 2:-2, 2:-2])
+    changed = (O.get_u()[4] != u[4]).mean()
+    assert 0.05 < changed < 0.2, changed         # the interior cells above 1e4 K
+    O.close()
+
+
+REF_TABLES = {3: "/root/reference/src/DMClib/coolingDMC.tab", 4: "/root/reference/src/CHIANTIlib/coolingCHIANTI.tab"}
+
+
+@pytest.mark.skipif(not os.path.exists(REF_TABLES[3]), reason="the reference tree is absent")
+@pytest.mark.parametrize("name,kind", [("dmc", 3), ("chi", 4)])
+def test_cooling_table_reader(name, kind):
+    """host.read_cooling_table builds cooltab exactly as read_table_dmc /
+    read_table_chianti do (cooling_dmc.f90:57-66)."""
+    from guacho_amd.host import read_cooling_table
+    np.testing.assert_array_equal(read_cooling_table(REF_TABLES[kind], kind), np.load(COOLING)[f"{name}_table"])
 
 
 def test_eight_wave_closed_form():
