@@ -29,6 +29,7 @@
  */
 #include "guacho_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -44,6 +45,9 @@ struct orc_state {
   long ncell;
   double time;      /* globals::time */
   int cool_kind;    /* 0, COOL_DMC (3) or COOL_CHI (4) */
+  int tc_kind, tc_sat;  /* thermal conduction: 0 off, 1 isotropic, 2 anisotropic */
+  double tc_tempsc, tc_rsc, tc_rhosc, tc_mu, tc_psc, tc_tsc, tc_vsc2, tc_bsc;
+  double *tcf;      /* heat fluxes f(5), g(5), h(5) per cell (3 x ncell) */
   double cool_tab[82], cool_tempsc, cool_psc, cool_tsc;
   double *u, *up, *primit, *f, *g, *h, *e, *temp;
 };
@@ -516,6 +520,7 @@ int orc_create(const orc_params *p, orc_state **out) {
 }
 
 void orc_destroy(orc_state *s) {
+  if (s && s->tcf) free(s->tcf);
   if (!s) return;
   free(s->u); free(s->up); free(s->primit); free(s->f); free(s->g); free(s->h);
   free(s->e); free(s->temp);
@@ -1034,6 +1039,7 @@ int orc_tstep(orc_state *s, double dt) {
   orc_boundaryI(s);                                                  /* :216 */
   if (s->cool_kind) calcprim_ghosts(s);                              /* :222-223 */
   else orc_calcprim_u(s);                                            /* :220 */
+  if (s->tc_kind) orc_thermal(s, dt);                                /* :227 */
   return bad;
 }
 
@@ -1089,4 +1095,243 @@ void orc_divb(const orc_state *s, double *divb) {
             ((AT(u, s, 5, i + 1, j, k) - AT(u, s, 5, i - 1, j, k)) / (2.0 * p->dx)
              + (AT(u, s, 6, i, j + 1, k) - AT(u, s, 6, i, j - 1, k)) / (2.0 * p->dy))
             + (AT(u, s, 7, i, j, k + 1) - AT(u, s, 7, i, j, k - 1)) / (2.0 * p->dz);
+}
+
+/* ===================================================================== */
+/* Thermal conduction, src/thermal_cond.f90 (serial branches)             */
+/* ===================================================================== */
+
+void orc_set_conduction(orc_state *s, int kind, int sat, double tempsc, double rsc, double rhosc, double mu,
+                        double psc, double tsc, double vsc2, double bsc) {
+  s->tc_kind = kind;
+  s->tc_sat = sat;
+  s->tc_tempsc = tempsc;
+  s->tc_rsc = rsc;
+  s->tc_rhosc = rhosc;
+  s->tc_mu = mu;
+  s->tc_psc = psc;
+  s->tc_tsc = tsc;
+  s->tc_vsc2 = vsc2;
+  s->tc_bsc = bsc;
+  if (kind && !s->tcf) s->tcf = (double *)calloc((size_t)s->ncell * 3, sizeof(double));
+}
+
+/* the compiler's real**integer: right-to-left binary exponentiation */
+static double powi_rl(double x, int n) {
+  double r = 1.0, b = x;
+  while (n) {
+    if (n & 1) r = r * b;
+    b = b * b;
+    n >>= 1;
+  }
+  return r;
+}
+
+static double ksp(double t) { return 6.e-7 * pow(t, 2.5); }                       /* :142-149 */
+static double ksp_parl(double t) { return 9.2181e-7 * pow(t, 2.5); }              /* :157-164 */
+static double ksp_perp(double t, double d, double b2) {                            /* :172-178 */
+  return ((0.30089e+33 * d) / (b2 * sqrt(t))) * d;
+}
+
+/* superstep / substep / ST_steps, :625-681 (nu = 0.01, snu = sqrt(nu)) */
+static double superstep(int n, double snu) {
+  const double a = powi_rl(1 + snu, 2 * n), b = powi_rl(1 - snu, 2 * n);
+  return (((double)n / (2. * snu)) * (a - b)) / (a + b);
+}
+static double substep(int j, int n, double nu) {
+  const double pi = acos(-1.);
+  return 1. / (((nu - 1.) * cos((pi * (double)(2 * j - 1)) / (2. * (double)n)) + nu) + 1.);
+}
+static void st_steps(double fs, int *ns, double *fstep) {
+  const double snu = sqrt(0.01);
+  int j;
+  for (j = 1; j <= 199; j++)
+    if (superstep(j, snu) > fs) break;
+  *ns = j;
+  *fstep = fs / superstep(*ns, snu);
+}
+
+/* primitives of every cell and Temp = (p / rho) Tempsc (calcprim) */
+static void tc_calcprim(orc_state *s) {
+  calcprim_arr(s, s->u);
+  for (long c = 0; c < s->ncell; c++) s->temp[c] = (s->primit[c * NEQ + 4] / s->primit[c * NEQ]) * s->tc_tempsc;
+}
+
+#define TEMP(i, j, k) (s->temp[cidx(s, (i), (j), (k))])
+#define PRIM(q, i, j, k) AT(s->primit, s, q, i, j, k)
+
+/* get_dt_cond, :78-110 */
+double orc_dt_cond(orc_state *s) {
+  const orc_params *p = &s->p;
+  double dtp = DBL_MAX;
+  double ddx = fmin2(p->dx, p->dy);
+  ddx = fmin2(ddx, p->dz);
+  for (int k = 1; k <= p->nz; k++)
+    for (int j = 1; j <= p->ny; j++)
+      for (int i = 1; i <= p->nx; i++)
+        dtp = fmin2(dtp, PRIM(0, i, j, k) / ksp((PRIM(4, i, j, k) / PRIM(0, i, j, k)) * s->tc_tempsc));
+  const double x = ddx * s->tc_rsc;
+  return ((((((0.25 * 0.5) * (x * x)) * p->cv) * 8.3145e7) * dtp) * s->tc_rhosc) / s->tc_mu;
+}
+
+/* one face of heatfluxes (:189-267), a = cell (i,j,k), b = its neighbour,
+ * h = the spacing; isotropic */
+static double iso_face(const orc_state *s, int ia, int ja, int ka, int ib, int jb, int kb, double h) {
+  const double ta = s->temp[cidx(s, ia, ja, ka)], tb = s->temp[cidx(s, ib, jb, kb)];
+  if (ta == tb) return 0.;
+  const double *pa = &AT(s->primit, s, 0, ia, ja, ka), *pb = &AT(s->primit, s, 0, ib, jb, kb);
+  const double meanp = 0.5 * (pa[4] + pb[4]);
+  const double meand = 0.5 * (pa[0] + pb[0]);
+  const double meant = 0.5 * (ta + tb);
+  const double dt = (tb - ta) / (h * s->tc_rsc);
+  double coef;
+  if (s->tc_sat) {
+    double cs = orc_csound(&s->p, meanp, meand);
+    cs = fmin2(cs * sqrt(s->tc_vsc2), 3.e10);
+    coef = fmin2(ksp(meant), ((((5. * 0.4) * cs) * meanp) * s->tc_psc) / fabs(dt));
+  } else {
+    coef = ksp(meant);
+  }
+  return -((coef * dt) * 1.);
+}
+
+/* MHD_heatfluxes (:277-487) at cell (i,j,k): f, g, h of the cell's faces */
+static void mhd_faces(const orc_state *s, int i, int j, int k, double *fx, double *fy, double *fz) {
+  const orc_params *p = &s->p;
+  const double alpha = 5.0 * 0.3;
+  double bx = PRIM(5, i, j, k), by = PRIM(6, i, j, k), bz = PRIM(7, i, j, k);
+  const double b2 = (bx * bx + by * by) + bz * bz;
+  const double modb = sqrt(b2);
+  bx = bx / modb;
+  by = by / modb;
+  bz = bz / modb;
+  const double bsc2 = s->tc_bsc * s->tc_bsc;
+  const int nb[3][3] = {{i + 1, j, k}, {i, j + 1, k}, {i, j, k + 1}};
+  const double hh[3] = {p->dx, p->dy, p->dz};
+  double grad[3], kpar[3], kperp[3], csat[3];
+  const double t0 = TEMP(i, j, k);
+  for (int a = 0; a < 3; a++) {
+    const double t1 = TEMP(nb[a][0], nb[a][1], nb[a][2]);
+    if (fabs(t0 - t1) < 1.0e-14) {
+      grad[a] = 0.0;
+      kpar[a] = 0.0;
+      kperp[a] = 0.0;
+      csat[a] = 0.0;
+      continue;
+    }
+    const double meand = 0.5 * (PRIM(0, i, j, k) + PRIM(0, nb[a][0], nb[a][1], nb[a][2]));
+    const double meant = 0.5 * (t0 + t1);
+    if (s->tc_sat) {
+      const double meanp = 0.5 * (PRIM(4, i, j, k) + PRIM(4, nb[a][0], nb[a][1], nb[a][2]));
+      double cs = orc_csound(p, meanp, meand);
+      cs = fmin2(cs * sqrt(s->tc_vsc2), 3.e10);
+      csat[a] = (alpha * meand) * powi_rl(cs, 3);
+    } else {
+      csat[a] = 0.0;
+    }
+    grad[a] = (t1 - t0) / (hh[a] * s->tc_rsc);
+    kpar[a] = ksp_parl(meant);
+    kperp[a] = ksp_perp(meant, meand * s->tc_rhosc, b2 * bsc2);
+  }
+  const double bgradt = (bx * grad[0] + by * grad[1]) + bz * grad[2];
+  const double gpar[3] = {bgradt * bx, bgradt * by, bgradt * bz};
+  const double gperp[3] = {grad[0] - gpar[0], grad[1] - gpar[1], grad[2] - gpar[2]};
+  double out[3];
+  if (!s->tc_sat) {
+    for (int a = 0; a < 3; a++) out[a] = (-(kpar[a] * gpar[a])) - kperp[a] * gperp[a];
+  } else {
+    const double gperpm = sqrt((gperp[0] * gperp[0] + gperp[1] * gperp[1]) + gperp[2] * gperp[2]);
+    for (int a = 0; a < 3; a++)
+      out[a] = (-((1. / (1. / (kpar[a] + 1.e-14) + bgradt / (csat[a] + 1.e-14))) * gpar[a]))
+               - (1. / (1. / (kperp[a] + 1.e-14) + gperpm / (csat[a] + 1.e-14))) * gperp[a];
+  }
+  *fx = out[0];
+  *fy = out[1];
+  *fz = out[2];
+}
+
+/* thermal_bounds, :496-616 (serial): periodic copies of E on full ranges,
+ * then every face copies its edge cell over 0..n+1 */
+static void thermal_bounds(orc_state *s) {
+  const orc_params *p = &s->p;
+  const int nx = p->nx, ny = p->ny, nz = p->nz;
+  double *u = s->u;
+  if (p->bc[0] == ORC_BC_PERIODIC && p->bc[1] == ORC_BC_PERIODIC)
+    for (int k = -1; k <= nz + 2; k++)
+      for (int j = -1; j <= ny + 2; j++) {
+        AT(u, s, 4, 0, j, k) = AT(u, s, 4, nx, j, k);
+        AT(u, s, 4, nx + 1, j, k) = AT(u, s, 4, 1, j, k);
+      }
+  if (p->bc[2] == ORC_BC_PERIODIC && p->bc[3] == ORC_BC_PERIODIC)
+    for (int k = -1; k <= nz + 2; k++)
+      for (int i = -1; i <= nx + 2; i++) {
+        AT(u, s, 4, i, 0, k) = AT(u, s, 4, i, ny, k);
+        AT(u, s, 4, i, ny + 1, k) = AT(u, s, 4, i, 1, k);
+      }
+  if (p->bc[4] == ORC_BC_PERIODIC && p->bc[5] == ORC_BC_PERIODIC)
+    for (int j = -1; j <= ny + 2; j++)
+      for (int i = -1; i <= nx + 2; i++) {
+        AT(u, s, 4, i, j, 0) = AT(u, s, 4, i, j, nz);
+        AT(u, s, 4, i, j, nz + 1) = AT(u, s, 4, i, j, 1);
+      }
+  for (int k = 0; k <= nz + 1; k++)
+    for (int j = 0; j <= ny + 1; j++) AT(u, s, 4, 0, j, k) = AT(u, s, 4, 1, j, k);
+  for (int k = 0; k <= nz + 1; k++)
+    for (int j = 0; j <= ny + 1; j++) AT(u, s, 4, nx + 1, j, k) = AT(u, s, 4, nx, j, k);
+  for (int k = 0; k <= nz + 1; k++)
+    for (int i = 0; i <= nx + 1; i++) AT(u, s, 4, i, 0, k) = AT(u, s, 4, i, 1, k);
+  for (int k = 0; k <= nz + 1; k++)
+    for (int i = 0; i <= nx + 1; i++) AT(u, s, 4, i, ny + 1, k) = AT(u, s, 4, i, ny, k);
+  for (int j = 0; j <= ny + 1; j++)
+    for (int i = 0; i <= nx + 1; i++) AT(u, s, 4, i, j, 0) = AT(u, s, 4, i, j, 1);
+  for (int j = 0; j <= ny + 1; j++)
+    for (int i = 0; i <= nx + 1; i++) AT(u, s, 4, i, j, nz + 1) = AT(u, s, 4, i, j, nz);
+}
+
+/* thermal_conduction, :690-768 */
+int orc_thermal(orc_state *s, double dt) {
+  const orc_params *p = &s->p;
+  const int nx = p->nx, ny = p->ny, nz = p->nz;
+  const double nu = 0.01;
+  for (long c = 0; c < s->ncell; c++) s->temp[c] = (s->primit[c * NEQ + 4] / s->primit[c * NEQ]) * s->tc_tempsc;
+  const double dt_hydro = dt * s->tc_tsc;
+  const double dt_cond = orc_dt_cond(s);
+  int superstep_on = 1, nsteps;
+  double fstep;
+  if (dt_cond < dt_hydro) {
+    st_steps(dt_hydro / dt_cond, &nsteps, &fstep);
+  } else {
+    superstep_on = 0;
+    fstep = dt_hydro / dt_cond;
+    nsteps = 1;
+  }
+  double *F = s->tcf, *G = s->tcf + s->ncell, *H = s->tcf + 2 * s->ncell;
+  for (int n = 1; n <= nsteps; n++) {
+    const double dts = superstep_on ? (((dt_cond * fstep) * substep(n, nsteps, nu)) / s->tc_psc) / s->tc_rsc
+                                    : ((dt_hydro / (double)nsteps) / s->tc_psc) / s->tc_rsc;
+    for (int k = 0; k <= nz; k++)
+      for (int j = 0; j <= ny; j++)
+        for (int i = 0; i <= nx; i++) {
+          const long c = cidx(s, i, j, k);
+          if (s->tc_kind == 2) {
+            mhd_faces(s, i, j, k, &F[c], &G[c], &H[c]);
+          } else {
+            F[c] = iso_face(s, i, j, k, i + 1, j, k, p->dx);
+            G[c] = iso_face(s, i, j, k, i, j + 1, k, p->dy);
+            H[c] = iso_face(s, i, j, k, i, j, k + 1, p->dz);
+          }
+        }
+    for (int k = 1; k <= nz; k++)
+      for (int j = 1; j <= ny; j++)
+        for (int i = 1; i <= nx; i++) {
+          const long c = cidx(s, i, j, k);
+          AT(s->u, s, 4, i, j, k) = AT(s->u, s, 4, i, j, k)
+                                    - dts * (((F[c] - F[cidx(s, i - 1, j, k)]) / p->dx
+                                              + (G[c] - G[cidx(s, i, j - 1, k)]) / p->dy)
+                                             + (H[c] - H[cidx(s, i, j, k - 1)]) / p->dz);
+        }
+    thermal_bounds(s);
+    tc_calcprim(s);
+  }
+  return nsteps;
 }

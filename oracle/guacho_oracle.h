@@ -68,6 +68,15 @@ void orc_set_time(orc_state *s, double time);   /* globals::time, seen by the us
 void orc_set_cooling(orc_state *s, int kind, const double *cooltab, double tempsc, double psc, double tsc);
 /* one cooling pass on u and primit (interior) with dt_CFL = dt */
 void orc_cooling(orc_state *s, double dt);
+/* Thermal conduction after each step (thermal_cond.f90:690-768): kind 1
+ * isotropic, 2 anisotropic (0 off); sat: tc_saturation; the parameters
+ * module's Tempsc, rsc, rhosc, mu, Psc, tsc, vsc2, bsc. */
+void orc_set_conduction(orc_state *s, int kind, int sat, double tempsc, double rsc, double rhosc, double mu,
+                        double psc, double tsc, double vsc2, double bsc);
+/* thermal_conduction with dt_CFL = dt on the current u, primit (primit and
+ * its temperature as tstep leaves them); returns the sub-step count */
+int orc_thermal(orc_state *s, double dt);
+double orc_dt_cond(orc_state *s);   /* get_dt_cond on the current primit */
 void orc_boundaryI(orc_state *s);
 void orc_boundaryII(orc_state *s);
 void orc_calcprim_u(orc_state *s);

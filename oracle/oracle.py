@@ -119,6 +119,11 @@ def _load_oracle() -> C.CDLL:
     L.orc_set_time.argtypes = [C.c_void_p, C.c_double]
     L.orc_set_cooling.argtypes = [C.c_void_p, C.c_int, _dpF, C.c_double, C.c_double, C.c_double]
     L.orc_cooling.argtypes = [C.c_void_p, C.c_double]
+    L.orc_set_conduction.argtypes = [C.c_void_p, C.c_int, C.c_int] + [C.c_double] * 8
+    L.orc_thermal.argtypes = [C.c_void_p, C.c_double]
+    L.orc_thermal.restype = C.c_int
+    L.orc_dt_cond.argtypes = [C.c_void_p]
+    L.orc_dt_cond.restype = C.c_double
     L.orc_timestep_finish.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                       C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.orc_u2prim.argtypes = [P, _dp, _dp, C.POINTER(C.c_double)]
@@ -245,6 +250,19 @@ class Oracle:
     def cooling(self, dt: float) -> None:
         self.L.orc_cooling(self.h, float(dt))
 
+    def set_conduction(self, kind: int, saturation: bool = False, tempsc: float = 1.0, rsc: float = 1.0,
+                       rhosc: float = 1.0, mu: float = 1.0, psc: float = 1.0, tsc: float = 1.0,
+                       vsc2: float = 1.0, bsc: float = 1.0) -> None:
+        """Thermal conduction after each step (1 isotropic, 2 anisotropic, 0 off)."""
+        self.L.orc_set_conduction(self.h, int(kind), int(saturation), tempsc, rsc, rhosc, mu, psc, tsc, vsc2, bsc)
+
+    def thermal(self, dt: float) -> int:
+        """thermal_conduction with dt_CFL = dt; returns the sub-step count."""
+        return self.L.orc_thermal(self.h, float(dt))
+
+    def dt_cond(self) -> float:
+        return self.L.orc_dt_cond(self.h)
+
     def set_time(self, time: float) -> None:
         """globals::time, as the user hook sees it."""
         self.L.orc_set_time(self.h, float(time))
@@ -297,6 +315,13 @@ class OracleFunctions:
 
     def cooling(self, kind, dt):
         self.L.ref_cooling(kind, dt)
+
+    def thermal(self, dt):
+        """thermal_conduction with dt_CFL = dt on the current globals."""
+        self.L.ref_thermal(float(dt))
+
+    def dt_cond(self):
+        d = C.c_double(); self.L.ref_dt_cond(C.byref(d)); return d.value
 
     def source(self, i, j, k, prim):
         """source(i,j,k,prim,s) of this build (sources.f90:188-220), from the
@@ -365,6 +390,11 @@ class RefLib:
         L.ref_set_time.argtypes = [C.c_double]
         L.ref_cooling_table.argtypes = [C.c_int, _dpF]
         L.ref_cooling.argtypes = [C.c_int, C.c_double]
+        L.ref_thermal.argtypes = [C.c_double]
+        L.ref_dt_cond.argtypes = [C.POINTER(C.c_double)]
+        tk, ts, tt = C.c_int(), C.c_int(), C.c_double()
+        L.ref_conduction_config(C.byref(tk), C.byref(ts), C.byref(tt))
+        self.th_cond, self.tc_saturation, self.tempsc = tk.value, bool(ts.value), tt.value
         sw = (C.c_int * 3)()
         L.ref_switches(sw)
         self.eight_wave, self.user_source_terms, self.bc_user = bool(sw[0]), bool(sw[1]), bool(sw[2])
@@ -437,6 +467,13 @@ class RefLib:
 
     def cooling(self, kind, dt):
         self.L.ref_cooling(kind, dt)
+
+    def thermal(self, dt):
+        """thermal_conduction with dt_CFL = dt on the current globals."""
+        self.L.ref_thermal(float(dt))
+
+    def dt_cond(self):
+        d = C.c_double(); self.L.ref_dt_cond(C.byref(d)); return d.value
 
     def source(self, i, j, k, prim):
         """source(i,j,k,prim,s) of this build (sources.f90:188-220), from the

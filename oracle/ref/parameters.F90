@@ -35,8 +35,8 @@ module parameters
   logical, parameter :: bc_user   = (REF_USER_BC /= 0)
 
   integer, parameter :: slope_limiter = REF_LIMITER
-  integer, parameter :: th_cond = TC_OFF
-  logical, parameter :: tc_saturation = .false.
+  integer, parameter :: th_cond = REF_TH_COND
+  logical, parameter :: tc_saturation = (REF_TC_SAT /= 0)
   logical, parameter :: dif_rad = .false.
   logical, parameter :: user_source_terms = (REF_USER_SRC /= 0)
   logical, parameter :: radiation_pressure = .false.
@@ -51,7 +51,7 @@ module parameters
   real, parameter :: mu = 1.
   real, parameter :: rsc = xphys / xmax
   real, parameter :: rhosc = 1.
-  real, parameter :: Tempsc = 1.
+  real, parameter :: Tempsc = REF_TEMPSC
   real, parameter :: vsc2 = 1.
   real, parameter :: vsc = sqrt(vsc2)
   real, parameter :: Psc = rhosc * vsc2

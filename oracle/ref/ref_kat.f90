@@ -23,6 +23,7 @@ module ref_kat
   use sources, only : source
   use cooling_dmc, only : init_cooling_dmc, coolingdmc, cooltab_dmc
   use cooling_chi, only : init_cooling_chianti, coolingchi, cooltab_chianti
+  use thermal_cond, only : thermal_conduction, get_dt_cond
   implicit none
 
 contains
@@ -221,5 +222,29 @@ contains
       call coolingchi()
     end if
   end subroutine ref_cooling
+
+  ! thermal_conduction (thermal_cond.f90:690-768) with dt_CFL = dt, on the
+  ! globals as tstep leaves them (the caller ran boundaryI and calcprim)
+  subroutine ref_thermal(dt) bind(C, name='ref_thermal')
+    real(c_double), value :: dt
+    dt_CFL = dt
+    call thermal_conduction()
+  end subroutine ref_thermal
+
+  subroutine ref_dt_cond(dt) bind(C, name='ref_dt_cond')
+    real(c_double), intent(out) :: dt
+    real :: d
+    call get_dt_cond(d)
+    dt = d
+  end subroutine ref_dt_cond
+
+  ! th_cond, tc_saturation, Tempsc of this build
+  subroutine ref_conduction_config(kind, sat, tempsc_out) bind(C, name='ref_conduction_config')
+    integer(c_int), intent(out) :: kind, sat
+    real(c_double), intent(out) :: tempsc_out
+    kind = th_cond
+    sat = merge(1, 0, tc_saturation)
+    tempsc_out = Tempsc
+  end subroutine ref_conduction_config
 
 end module ref_kat

@@ -10,6 +10,8 @@ inputs and the reference's outputs for
   - get_timestep's ramp and clamp (the boundary fills include the
     checker's impose_user_bc in bc_user builds, at globals::time = 0.3),
   - (ref_cooling.npz) coolingdmc and coolingchi with the reference's tables,
+  - thermal_conduction (isotropic / anisotropic, with and without
+    saturation) on a static state with a random temperature field,
   - source(i,j,k,prim,s) of sources.f90 at every interior cell of a random
     primitive array (zero, 8-wave, the checker's user gravity, or both,
     per the variant's switches).
@@ -119,6 +121,32 @@ def make(variant, seed):
             dt, dump = R.get_timestep(it, 10, t, tp)
             ts.append([it, t, tp, dt, float(dump)])
     out["timestep"] = np.array(ts)
+    if R.th_cond:
+        # thermal_conduction after boundaryI + calcprim, on a static state
+        # with a random temperature field (T ~ 10^5.5..10^6.5 K), at 0.5, 3
+        # and 20 conduction timesteps (1, 2 and 6 super-steps)
+        sh = R.shape()
+        rho = rng.uniform(0.5, 2.0, sh[1:])
+        temp = 10.0 ** rng.uniform(5.5, 6.5, sh[1:])
+        tu = np.zeros(sh, order="F")
+        tu[0] = rho
+        tu[5:8] = np.array([0.3, 0.2, 0.1])[:, None, None, None] + rng.uniform(-0.1, 0.1, (3,) + sh[1:])
+        tu[4] = R.cv * rho * temp / R.tempsc + 0.5 * (tu[5:8] ** 2).sum(0)
+        out["tc_u_in"] = tu
+        R.set_array(0, tu)
+        R.boundary(1)
+        R.calcprim(0)
+        dtc = R.dt_cond()
+        facs = (0.5, 3.0) if R.th_cond == 2 and not R.tc_saturation else (0.5, 3.0, 20.0)
+        out["tc_dt"] = np.array([f * dtc for f in facs])
+        for n, f in enumerate(facs):
+            R.set_array(0, tu)
+            R.boundary(1)
+            R.calcprim(0)
+            R.thermal(f * dtc)
+            out[f"tc_u_{n}"] = R.get_array(0)
+            out[f"tc_primit_{n}"] = R.get_array(2)
+        out["tc_config"] = np.array([R.th_cond, int(R.tc_saturation), R.tempsc])
     if R.eight_wave or R.user_source_terms:
         prim = np.asfortranarray(rng.uniform(0.5, 1.5, R.shape()))
         prim[1:4] -= 1.0

@@ -30,6 +30,10 @@ def _cfg(z):
                   eight_wave=bool(sw[0]), user_src=bool(sw[1]), user_bc=bool(sw[2]))
 
 
+def _tempsc(z):
+    return float(z["tc_config"][2]) if "tc_config" in z else 1.0
+
+
 def test_fixtures_present():
     assert len(GOLDEN) >= 8
 
@@ -50,6 +54,7 @@ def test_interface_functions_match_reference(path):
     lim = np.array([np.concatenate(O.limiter(a, b, c, d)) for a, b, c, d in zip(pll, pl, pr, prr)])
     np.testing.assert_array_equal(lim, z["lim_out"])
     u2p = np.array([np.concatenate([O.u2prim(a)[0], [O.u2prim(a)[1]]]) for a in z["u2p_in"]])
+    u2p[:, 8] *= _tempsc(z)   # T = (p / rho) Tempsc; the restatement's u2prim has Tempsc = 1
     np.testing.assert_array_equal(u2p, z["u2p_out"])
     np.testing.assert_array_equal(np.array([O.prim2u(a) for a in pl]), z["p2u_out"])
     np.testing.assert_array_equal(np.array([O.prim2f(a) for a in pl]), z["p2f_out"])
@@ -69,7 +74,7 @@ def test_sweeps_match_reference(path):
     np.testing.assert_array_equal(O.get_u(), z["sweep_u_bc1"])
     O.calcprim_u()
     np.testing.assert_array_equal(O.get_primit(), z["sweep_primit1"])
-    np.testing.assert_array_equal(O.get_temp(), z["sweep_temp1"])
+    np.testing.assert_array_equal(O.get_temp() * _tempsc(z), z["sweep_temp1"])
     O.fluxes(1)
     for a, k in enumerate("fgh"):
         np.testing.assert_array_equal(O.get_flux(a), z[f"sweep_{k}1"], err_msg=f"{k} first order")
@@ -133,6 +138,35 @@ def test_cooling_matches_reference(name, kind):
     changed = (O.get_u()[4] != u[4]).mean()
     assert 0.05 < changed < 0.2, changed         # the interior cells above 1e4 K
     O.close()
+
+
+TC_GOLDEN = [p for p in GOLDEN if "tc_u_in" in np.load(p)]
+
+
+def test_conduction_fixtures_present():
+    assert len(TC_GOLDEN) >= 4
+
+
+@pytest.mark.parametrize("path", TC_GOLDEN, ids=lambda p: os.path.basename(p)[4:-4])
+def test_conduction_matches_reference(path):
+    """thermal_conduction (thermal_cond.f90:690-768): get_dt_cond,
+    super-stepping, isotropic / anisotropic heat fluxes with and without
+    saturation, thermal_bounds and calcprim -- u and primitives bit for bit
+    (NaN where the reference's anisotropic sub-steps diverge too)."""
+    import math
+    z = np.load(path)
+    cfg = _cfg(z)
+    kind, sat, tempsc = z["tc_config"]
+    for n, dt in enumerate(z["tc_dt"]):
+        O = Oracle(cfg)
+        O.set_conduction(int(kind), bool(sat), tempsc=float(tempsc), bsc=math.sqrt(4.0 * math.pi))
+        O.set_u(z["tc_u_in"])
+        O.boundaryI()
+        O.calcprim_u()
+        O.thermal(float(dt))
+        np.testing.assert_array_equal(O.get_u(), z[f"tc_u_{n}"])
+        np.testing.assert_array_equal(O.get_primit(), z[f"tc_primit_{n}"])
+        O.close()
 
 
 REF_TABLES = {3: "/root/reference/src/DMClib/coolingDMC.tab", 4: "/root/reference/src/CHIANTIlib/coolingCHIANTI.tab"}
