@@ -257,8 +257,9 @@ def extensions_section(u0, dx, dy, dz, n: int, steps: int):
     plugins -- a source-term kernel (uniform + point-mass gravity) and a
     boundary hook (inflow jet + interior obstacle; ghosts then live in
     memory: boundaryII + hook between the stages, boundaryI + hook after)
-    -- on outflow faces, and DMC table cooling (T ~ 1e6 K).  Per variant:
-    zone-updates/s and the per-class kernel times."""
+    -- on outflow faces, DMC table cooling (T ~ 1e6 K), and isotropic
+    thermal conduction (one sub-step per step at these scalings).  Per
+    variant: zone-updates/s and the per-class kernel times."""
     from guacho_amd import _capi
     from guacho_amd.host import timestep_clamp
     plugins = os.path.join(ROOT, "tests", "plugins")
@@ -272,6 +273,7 @@ def extensions_section(u0, dx, dy, dz, n: int, steps: int):
         ("outflow", dict(bc=(1,) * 6), None, None),
         ("outflow_user_plugins", dict(bc=(1,) * 6), gravity, jet),
         ("cooling_dmc", dict(), None, None),
+        ("conduction_iso", dict(), None, None),
     )
     res = {}
     for name, kw, src, hook in variants:
@@ -284,6 +286,8 @@ def extensions_section(u0, dx, dy, dz, n: int, steps: int):
                 dev.set_user_bc(*hook)
             if name == "cooling_dmc":
                 dev.set_cooling(_capi.GM_COOL_DMC, dmc, 1e6, 1e-22, 10.0)
+            if name == "conduction_iso":
+                dev.set_conduction(_capi.GM_TC_ISOTROPIC)
             run_steps(dev, u0, 2, warmup=0)
             dev.upload_u(u0)
             dev.boundary_u()
@@ -311,7 +315,8 @@ def extensions_section(u0, dx, dy, dz, n: int, steps: int):
             "sources_pass_achieved_GBs": round(src_gbs, 1) if src_gbs else None,
             "user_plugins_overhead": round(res["outflow_user_plugins"]["ms_per_step"]
                                            / res["outflow"]["ms_per_step"] - 1, 4),
-            "cooling_ms_per_step": res["cooling_dmc"]["kernel_ms_per_step"].get("cooling")}
+            "cooling_ms_per_step": res["cooling_dmc"]["kernel_ms_per_step"].get("cooling"),
+            "conduction_ms_per_step": res["conduction_iso"]["kernel_ms_per_step"].get("conduction")}
 
 
 def serving_section(nreq: int, seed: int, fractions=(0.5, 1.0, 2.0)):

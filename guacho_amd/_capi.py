@@ -17,14 +17,16 @@ LIB_PATH = os.path.join(_HERE, "libguacho_amd.so")
 GM_OK, GM_ERR_CONFIG, GM_ERR_HIP, GM_ERR_NONFINITE, GM_ERR_OOM, GM_ERR_ARG, GM_ERR_BUSY = 0, 1, 2, 3, 4, 5, 6
 GM_PRECISION_FP64, GM_PRECISION_MIXED = 0, 1
 GM_COOL_NONE, GM_COOL_DMC, GM_COOL_CHI = 0, 3, 4
-GM_NKCLASS = 10
+GM_TC_OFF, GM_TC_ISOTROPIC, GM_TC_ANISOTROPIC = 0, 1, 2
+GM_NKCLASS = 11
 
 # exported entry points, in the order include/guacho_amd.h declares them
 EXPORTS = (
     "gm_create", "gm_destroy", "gm_last_error", "gm_upload_u", "gm_download_u",
     "gm_download_primit", "gm_boundary_u", "gm_cfl_dtp", "gm_tstep", "gm_sync",
     "gm_download_divb", "gm_set_timing", "gm_kernel_times", "gm_device_bytes",
-    "gm_set_time", "gm_set_user_sources", "gm_set_user_bc", "gm_set_cooling",
+    "gm_set_time", "gm_set_user_sources", "gm_set_user_bc", "gm_set_cooling", "gm_set_conduction",
+    "gm_conduction_substeps",
     "gm_batch_create", "gm_batch_destroy", "gm_batch_last_error", "gm_batch_box_bytes",
     "gm_batch_free_bytes", "gm_batch_add", "gm_batch_remove", "gm_batch_advance", "gm_batch_sync",
     "gm_batch_clock", "gm_batch_download_u", "gm_batch_tuning", "gm_batch_active",
@@ -50,6 +52,12 @@ class GmParams(C.Structure):
         ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double),
         ("cv", C.c_double), ("gamma", C.c_double), ("eta", C.c_double),
     ]
+
+
+class GmConductionParams(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("saturation", C.c_int32),
+                ("tempsc", C.c_double), ("rsc", C.c_double), ("rhosc", C.c_double), ("mu", C.c_double),
+                ("psc", C.c_double), ("tsc", C.c_double), ("vsc2", C.c_double), ("bsc", C.c_double)]
 
 
 class GmBatchParams(C.Structure):
@@ -93,6 +101,8 @@ def lib() -> C.CDLL:
     L.gm_set_user_sources.argtypes = [vp, C.c_char_p, C.c_char_p]
     L.gm_set_user_bc.argtypes = [vp, C.c_char_p, C.c_char_p]
     L.gm_set_cooling.argtypes = [vp, C.c_int32, C.c_void_p, C.c_double, C.c_double, C.c_double]
+    L.gm_set_conduction.argtypes = [vp, C.POINTER(GmConductionParams)]
+    L.gm_conduction_substeps.argtypes = [vp, C.POINTER(C.c_int32)]
     P = C.POINTER(GmParams)
     L.gm_batch_create.argtypes = [C.POINTER(GmBatchParams), C.POINTER(vp)]
     L.gm_batch_destroy.argtypes = [vp]
@@ -239,6 +249,18 @@ class Subdomain:
         if tab.shape != (2, 41):
             raise ValueError("cooltab must be (2, 41)")
         self._check(self.L.gm_set_cooling(self.h, int(kind), tab.ctypes.data_as(C.c_void_p), tempsc, psc, tsc))
+
+    def set_conduction(self, kind: int, saturation: bool = False, tempsc: float = 1.0, rsc: float = 1.0,
+                       rhosc: float = 1.0, mu: float = 1.0, psc: float = 1.0, tsc: float = 1.0,
+                       vsc2: float = 1.0, bsc: float = 1.0) -> None:
+        """Thermal conduction after each step (GM_TC_ISOTROPIC / GM_TC_ANISOTROPIC; 0 off)."""
+        tp = GmConductionParams(int(kind), int(bool(saturation)), tempsc, rsc, rhosc, mu, psc, tsc, vsc2, bsc)
+        self._check(self.L.gm_set_conduction(self.h, C.byref(tp)))
+
+    def conduction_substeps(self) -> int:
+        n = C.c_int32()
+        self._check(self.L.gm_conduction_substeps(self.h, C.byref(n)))
+        return n.value
 
     def kernel_times(self) -> Dict[str, tuple]:
         ms = (C.c_double * GM_NKCLASS)()

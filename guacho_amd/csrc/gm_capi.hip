@@ -22,19 +22,25 @@
 namespace {
 
 const char* kClassNames[GM_NKCLASS] = {"xy_predictor", "xy_corrector", "z_predictor", "z_corrector",
-                                       "bfield", "cfl_dt", "ghost", "layout", "sources", "cooling"};
-enum { KC_XY1, KC_XY2, KC_Z1, KC_Z2, KC_BFIELD, KC_CFL, KC_GHOST, KC_LAYOUT, KC_SOURCES, KC_COOLING };
+                                       "bfield", "cfl_dt", "ghost", "layout", "sources", "cooling",
+                                       "conduction"};
+enum { KC_XY1, KC_XY2, KC_Z1, KC_Z2, KC_BFIELD, KC_CFL, KC_GHOST, KC_LAYOUT, KC_SOURCES, KC_COOLING, KC_CONDUCTION };
 
 thread_local std::string g_create_error;
 
 struct Scalars {
   unsigned long long dtmin;
+  unsigned long long tcmin;  // conduction: min rho / Ksp(T) (bits of a positive double)
   int err;
   int pad;
 };
 
 __global__ void k_init_dtmin(Scalars* s) {
   if (threadIdx.x == 0 && blockIdx.x == 0) s->dtmin = (unsigned long long)__double_as_longlong(1e30);
+}
+
+__global__ void k_init_tcmin(Scalars* s) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) s->tcmin = (unsigned long long)__double_as_longlong(1.79769313486231570815e+308);
 }
 
 __global__ void k_clear_err(Scalars* s) {
@@ -63,6 +69,8 @@ struct gm_ctx {
   hipModule_t ubc_mod = nullptr;        // user boundary plugin (ghosts then live in memory)
   hipFunction_t ubc_fn = nullptr;
   gm::Cooling cool{};                   // kind 0: no cooling
+  gm::Conduction cond{};                // kind 0: no thermal conduction
+  int tc_substeps = 0;                  // sub-steps of the last step's conduction
   gm::Shape shape = {256, 16};  // tile width, z-pencil length
   int timing = 0;
   struct Rec { int cls; hipEvent_t a, b; };
@@ -268,7 +276,7 @@ gm::Box<S> box_of(const gm_ctx* c) {
   b.dz = c->p.dz;
   b.slot = 0;
   b.src = (c->p.eight_wave ? gm::SRC_EIGHT_WAVE : 0) | (c->usrc_fn ? gm::SRC_USER : 0);
-  b.g.gmem = c->ubc_fn ? 1 : 0;
+  b.g.gmem = (c->ubc_fn || c->cond.kind) ? 1 : 0;
   return b;
 }
 
@@ -287,6 +295,71 @@ void stage_sources_t(gm_ctx* c, const gm::Box<S>& b, double dt, int stage) {
   timed(c, KC_SOURCES, [&] {
     if (b.src & gm::SRC_USER) user_sources_t(c, b, dt, stage);
     gm::launch_sources(c->stream, b, dt, stage, c->shape);
+  });
+}
+
+// superstep / substep / ST_steps (src/thermal_cond.f90:625-681), in the
+// reference's arithmetic: real**integer as the compiler's right-to-left
+// binary exponentiation, cos from libm
+double powi_rl(double x, int n) {
+  double r = 1.0, bb = x;
+  while (n) {
+    if (n & 1) r = r * bb;
+    bb = bb * bb;
+    n >>= 1;
+  }
+  return r;
+}
+double tc_superstep(int n, double snu) {
+  const double a = powi_rl(1 + snu, 2 * n), bq = powi_rl(1 - snu, 2 * n);
+  return (((double)n / (2. * snu)) * (a - bq)) / (a + bq);
+}
+double tc_substep(int j, int n, double nu) {
+  const double pi = std::acos(-1.);
+  return 1. / (((nu - 1.) * std::cos((pi * (double)(2 * j - 1)) / (2. * (double)n)) + nu) + 1.);
+}
+
+// thermal_conduction (src/thermal_cond.f90:690-768): dt_cond from the
+// device minimum, the super-step count on the host, then the sub-steps
+template <typename S>
+void conduction_t(gm_ctx* c, const gm::Box<S>& b, double dt) {
+  const gm::Conduction& tc = c->cond;
+  timed(c, KC_CONDUCTION, [&] {
+    hipLaunchKernelGGL(k_init_tcmin, dim3(1), dim3(1), 0, c->stream, c->dscal);
+    gm::launch_tc_dtmin(c->stream, b, tc, &c->dscal->tcmin);
+  });
+  (void)hipMemcpyAsync(&c->hscal->tcmin, &c->dscal->tcmin, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                       c->stream);
+  (void)hipStreamSynchronize(c->stream);
+  double dtp;
+  std::memcpy(&dtp, &c->hscal->tcmin, sizeof dtp);
+  double ddx = c->p.dx < c->p.dy ? c->p.dx : c->p.dy;
+  ddx = ddx < c->p.dz ? ddx : c->p.dz;
+  const double x = ddx * tc.rsc;
+  const double dt_cond = ((((((0.25 * 0.5) * (x * x)) * c->p.cv) * 8.3145e7) * dtp) * tc.rhosc) / tc.mu;
+  const double dt_hydro = dt * tc.tsc;
+  const double nu = 0.01, snu = std::sqrt(0.01);
+  bool super = true;
+  int nsteps;
+  double fstep;
+  if (dt_cond < dt_hydro) {
+    int j;
+    for (j = 1; j <= 199; j++)
+      if (tc_superstep(j, snu) > dt_hydro / dt_cond) break;
+    nsteps = j;
+    fstep = (dt_hydro / dt_cond) / tc_superstep(nsteps, snu);
+  } else {
+    super = false;
+    fstep = dt_hydro / dt_cond;
+    nsteps = 1;
+  }
+  c->tc_substeps = nsteps;
+  timed(c, KC_CONDUCTION, [&] {
+    for (int n = 1; n <= nsteps; n++) {
+      const double dts = super ? (((dt_cond * fstep) * tc_substep(n, nsteps, nu)) / tc.psc) / tc.rsc
+                               : ((dt_hydro / (double)nsteps) / tc.psc) / tc.rsc;
+      gm::launch_tc_substep(c->stream, b, tc, dts);
+    }
   });
 }
 
@@ -317,6 +390,8 @@ void tstep_t(gm_ctx* c, double dt) {
   if (c->cool.kind) timed(c, KC_COOLING, [&] { gm::launch_cooling(s, b, c->cool, dt); });
   // boundaryI on U^{n+1} (src/hydro_solver.f90:216) when ghosts live in memory
   if (b.g.gmem) fill_ghost_t<S>(c, b.U, 1);
+  // thermal conduction (src/hydro_solver.f90:227)
+  if (c->cond.kind) conduction_t<S>(c, b, dt);
 }
 
 template <typename S>
@@ -508,7 +583,7 @@ int gm_tstep(gm_ctx* c, double dt) {
         return fail(c, GM_ERR_CONFIG, "BC_OTHER faces need a user boundary plugin (gm_set_user_bc)");
   if (c->esize == 8) tstep_t<double>(c, dt);
   else tstep_t<float>(c, dt);
-  c->ghosts_dirty = c->ubc_fn == nullptr;  // with a user hook the step ends with boundaryI
+  c->ghosts_dirty = c->ubc_fn == nullptr && !c->cond.kind;  // else the step ends with boundaryI
   GM_HIP(c, hipGetLastError());
   return GM_OK;
 }
@@ -637,6 +712,38 @@ int gm_set_cooling(gm_ctx* c, int32_t kind, const double* cooltab, double tempsc
   cl.hicoef = kind == GM_COOL_DMC ? 0.27e-26 : 0.21e-26;  // cooling_dmc.f90:88, cooling_chi.f90
   std::memcpy(cl.tab, cooltab, sizeof cl.tab);
   c->cool = cl;
+  return GM_OK;
+}
+
+int gm_set_conduction(gm_ctx* c, const gm_conduction_params* tp) {
+  if (!c) return GM_ERR_ARG;
+  if (!tp) return fail(c, GM_ERR_ARG, "conduction params is NULL");
+  if (tp->kind == GM_TC_OFF) {
+    c->cond.kind = 0;
+    return GM_OK;
+  }
+  if (tp->kind != GM_TC_ISOTROPIC && tp->kind != GM_TC_ANISOTROPIC)
+    return fail(c, GM_ERR_CONFIG, "conduction kind must be GM_TC_OFF, GM_TC_ISOTROPIC or GM_TC_ANISOTROPIC");
+  const double v[] = {tp->tempsc, tp->rsc, tp->rhosc, tp->mu, tp->psc, tp->tsc, tp->vsc2, tp->bsc};
+  for (double x : v)
+    if (!(x > 0) || !std::isfinite(x)) return fail(c, GM_ERR_CONFIG, "conduction scalings must be positive and finite");
+  int rc = set_device(c);
+  if (rc) return rc;
+  gm::Conduction t{};
+  t.kind = tp->kind;
+  t.sat = tp->saturation ? 1 : 0;
+  t.tempsc = tp->tempsc; t.rsc = tp->rsc; t.rhosc = tp->rhosc; t.mu = tp->mu;
+  t.psc = tp->psc; t.tsc = tp->tsc; t.vsc2 = tp->vsc2; t.bsc = tp->bsc;
+  c->cond = t;
+  // from here on the kernels read U's ghosts as stored: materialise them
+  if (c->ghosts_dirty) fill_ghost1(c);
+  GM_HIP(c, hipGetLastError());
+  return GM_OK;
+}
+
+int gm_conduction_substeps(const gm_ctx* c, int32_t* n) {
+  if (!c || !n) return GM_ERR_ARG;
+  *n = c->tc_substeps;
   return GM_OK;
 }
 

@@ -553,6 +553,203 @@ __global__ void __launch_bounds__(256) k_cooling(Box<S> b, Cooling cl, double dt
 }
 
 // ---------------------------------------------------------------------------
+// Thermal conduction, thermal_conduction (src/thermal_cond.f90:690-768),
+// sub-step kernels.  Temperatures and primitives come from the stored state
+// (ghosts in memory), T = (p / rho) Tempsc as calcprim gives them.
+// Spitzer coefficients Ksp = 6e-7 T^2.5 (:142-149), Ksp_parl = 9.2181e-7
+// T^2.5, Ksp_perp = 0.30089e33 rho / (B^2 sqrt(T)) rho (:157-178); pow and
+// sqrt are the device's.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double tc_ksp(double t) { return 6.e-7 * pow(t, 2.5); }
+__device__ __forceinline__ double tc_ksp_parl(double t) { return 9.2181e-7 * pow(t, 2.5); }
+__device__ __forceinline__ double tc_ksp_perp(double t, double d, double b2) {
+  return ((0.30089e+33 * d) / (b2 * __builtin_sqrt(t))) * d;
+}
+__device__ __forceinline__ double tc_min(double a, double b) { return (a < b) ? a : b; }
+
+struct TcCell {
+  double rho, p, t, b[3];
+};
+
+template <typename S>
+__device__ __forceinline__ TcCell tc_cell(const Box<S>& b, const Conduction& tc, int i, int j, int k) {
+  const long long c = cidx(b.g, i, j, k), N = b.g.ncell;
+  double u[8], p[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) u[q] = (double)b.U[q * N + c];
+  u2prim(b.ph, u, p);
+  TcCell r;
+  r.rho = p[0];
+  r.p = p[4];
+  r.t = (p[4] / p[0]) * tc.tempsc;
+  r.b[0] = p[5];
+  r.b[1] = p[6];
+  r.b[2] = p[7];
+  return r;
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_tc_dtmin(Box<S> b, Conduction tc, unsigned long long* out) {
+  __shared__ double red[4];
+  const Geom& g = b.g;
+  const long long n = (long long)g.nx * g.ny * g.nz;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 1.79769313486231570815e+308;  // huge(1.)
+  if (t < n) {
+    const int i = (int)(t % g.nx) + 1;
+    const long long r = t / g.nx;
+    const TcCell cl = tc_cell(b, tc, i, (int)(r % g.ny) + 1, (int)(r / g.ny) + 1);
+    v = cl.rho / tc_ksp(cl.t);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = tc_min(v, __shfl_xor(v, off, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = tc_min(m, red[w]);
+    atomicMin(out, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// heatfluxes (:189-267) / MHD_heatfluxes (:277-487): the three forward faces
+// of every cell (i, j, k) in 0..n, written to W (f, g, h at components 0..2)
+template <typename S>
+__global__ void __launch_bounds__(256) k_tc_flux(Box<S> b, Conduction tc) {
+  const Geom& g = b.g;
+  const int ex = g.nx + 1, ey = g.ny + 1;
+  const long long n = (long long)ex * ey * (g.nz + 1);
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % ex);
+  const long long r = t / ex;
+  const int j = (int)(r % ey), k = (int)(r / ey);
+  const double hh[3] = {b.dx, b.dy, b.dz};
+  const TcCell c0 = tc_cell(b, tc, i, j, k);
+  TcCell nb[3];
+  nb[0] = tc_cell(b, tc, i + 1, j, k);
+  nb[1] = tc_cell(b, tc, i, j + 1, k);
+  nb[2] = tc_cell(b, tc, i, j, k + 1);
+  double out[3];
+  if (tc.kind == 1) {
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const TcCell& c1 = nb[a];
+      if (c0.t == c1.t) {
+        out[a] = 0.;
+        continue;
+      }
+      const double meanp = 0.5 * (c0.p + c1.p);
+      const double meand = 0.5 * (c0.rho + c1.rho);
+      const double meant = 0.5 * (c0.t + c1.t);
+      const double dtg = (c1.t - c0.t) / (hh[a] * tc.rsc);
+      double coef;
+      if (tc.sat) {
+        double cs = __builtin_sqrt((b.ph.gamma * meanp) / meand);
+        cs = tc_min(cs * __builtin_sqrt(tc.vsc2), 3.e10);
+        coef = tc_min(tc_ksp(meant), ((((5. * 0.4) * cs) * meanp) * tc.psc) / __builtin_fabs(dtg));
+      } else {
+        coef = tc_ksp(meant);
+      }
+      out[a] = -((coef * dtg) * 1.);
+    }
+  } else {
+    const double alpha = 5.0 * 0.3;
+    double bx = c0.b[0], by = c0.b[1], bz = c0.b[2];
+    const double b2 = (bx * bx + by * by) + bz * bz;
+    const double modb = __builtin_sqrt(b2);
+    bx = bx / modb;
+    by = by / modb;
+    bz = bz / modb;
+    const double bsc2 = tc.bsc * tc.bsc;
+    double grad[3], kpar[3], kperp[3], csat[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const TcCell& c1 = nb[a];
+      if (__builtin_fabs(c0.t - c1.t) < 1.0e-14) {
+        grad[a] = 0.0;
+        kpar[a] = 0.0;
+        kperp[a] = 0.0;
+        csat[a] = 0.0;
+        continue;
+      }
+      const double meand = 0.5 * (c0.rho + c1.rho);
+      const double meant = 0.5 * (c0.t + c1.t);
+      if (tc.sat) {
+        const double meanp = 0.5 * (c0.p + c1.p);
+        double cs = __builtin_sqrt((b.ph.gamma * meanp) / meand);
+        cs = tc_min(cs * __builtin_sqrt(tc.vsc2), 3.e10);
+        csat[a] = (alpha * meand) * (cs * (cs * cs));  // cs**3 as the compiler's powi forms it
+      } else {
+        csat[a] = 0.0;
+      }
+      grad[a] = (c1.t - c0.t) / (hh[a] * tc.rsc);
+      kpar[a] = tc_ksp_parl(meant);
+      kperp[a] = tc_ksp_perp(meant, meand * tc.rhosc, b2 * bsc2);
+    }
+    const double bgradt = (bx * grad[0] + by * grad[1]) + bz * grad[2];
+    const double gpar[3] = {bgradt * bx, bgradt * by, bgradt * bz};
+    const double gperp[3] = {grad[0] - gpar[0], grad[1] - gpar[1], grad[2] - gpar[2]};
+    if (!tc.sat) {
+#pragma unroll
+      for (int a = 0; a < 3; a++) out[a] = (-(kpar[a] * gpar[a])) - kperp[a] * gperp[a];
+    } else {
+      const double gperpm = __builtin_sqrt((gperp[0] * gperp[0] + gperp[1] * gperp[1]) + gperp[2] * gperp[2]);
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+        out[a] = (-((1. / (1. / (kpar[a] + 1.e-14) + bgradt / (csat[a] + 1.e-14))) * gpar[a]))
+                 - (1. / (1. / (kperp[a] + 1.e-14) + gperpm / (csat[a] + 1.e-14))) * gperp[a];
+    }
+  }
+  const long long c = cidx(g, i, j, k), N = g.ncell;
+  b.W[c] = out[0];
+  b.W[N + c] = out[1];
+  b.W[2 * N + c] = out[2];
+}
+
+// u(5) = u(5) - dts (df/dx + dg/dy + dh/dz) on the interior (:752-758)
+template <typename S>
+__global__ void __launch_bounds__(256) k_tc_update(Box<S> b, double dts) {
+  const Geom& g = b.g;
+  const long long n = (long long)g.nx * g.ny * g.nz;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % g.nx) + 1;
+  const long long r = t / g.nx;
+  const int j = (int)(r % g.ny) + 1, k = (int)(r / g.ny) + 1;
+  const long long c = cidx(g, i, j, k), N = g.ncell;
+  const long long sy = g.NX, sz = (long long)g.NX * g.NY;
+  const double* F = b.W;
+  const double* G = b.W + N;
+  const double* H = b.W + 2 * N;
+  const double e = (double)b.U[4 * N + c];
+  b.U[4 * N + c] = (S)(e - dts * (((F[c] - F[c - 1]) / b.dx + (G[c] - G[c - sy]) / b.dy) + (H[c] - H[c - sz]) / b.dz));
+}
+
+// thermal_bounds (:496-616) on E, one face: periodic copies over the full
+// transverse range (periodic != 0), else the edge-cell copy over 0..n+1
+template <typename S>
+__global__ void __launch_bounds__(256) k_tc_bound(Geom g, S* __restrict__ U, int axis, int side, int periodic) {
+  const int n0 = axis == 0 ? g.ny : g.nx;
+  const int n1 = axis == 2 ? g.ny : g.nz;
+  const int na = axis == 0 ? g.nx : axis == 1 ? g.ny : g.nz;
+  const int lo = periodic ? -1 : 0;
+  const int e0 = n0 + 2 - 2 * lo, e1 = n1 + 2 - 2 * lo;
+  const long long n = (long long)e0 * e1;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int a = (int)(t % e0) + lo, bq = (int)(t / e0) + lo;
+  const int dst = side == 0 ? 0 : na + 1;
+  const int src = periodic ? (side == 0 ? na : 1) : (side == 0 ? 1 : na);
+  int di, dj, dk, si, sj, sk;
+  if (axis == 0) { di = dst; si = src; dj = sj = a; dk = sk = bq; }
+  else if (axis == 1) { dj = dst; sj = src; di = si = a; dk = sk = bq; }
+  else { dk = dst; sk = src; di = si = a; dj = sj = bq; }
+  U[4 * g.ncell + cidx(g, di, dj, dk)] = U[4 * g.ncell + cidx(g, si, sj, sk)];
+}
+
+// ---------------------------------------------------------------------------
 // CFL candidate reduction: the min loop of get_timestep
 // (src/hydro_core.f90:642-675) on the primitives of the interior.  Exact
 // (min is order-free); block minima meet in one 64-bit atomicMin on the
@@ -915,6 +1112,37 @@ void launch_cooling(hipStream_t s, const Box<S>& b, const Cooling& cl, double dt
 }
 
 template <typename S>
+void launch_tc_dtmin(hipStream_t s, const Box<S>& b, const Conduction& tc, unsigned long long* out) {
+  const long long n = (long long)b.g.nx * b.g.ny * b.g.nz;
+  hipLaunchKernelGGL(k_tc_dtmin<S>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, b, tc, out);
+}
+
+template <typename S>
+void launch_tc_substep(hipStream_t s, const Box<S>& b, const Conduction& tc, double dts) {
+  const Geom& g = b.g;
+  const long long nf = (long long)(g.nx + 1) * (g.ny + 1) * (g.nz + 1);
+  hipLaunchKernelGGL(k_tc_flux<S>, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, b, tc);
+  const long long ni = (long long)g.nx * g.ny * g.nz;
+  hipLaunchKernelGGL(k_tc_update<S>, dim3((unsigned)((ni + 255) / 256)), dim3(256), 0, s, b, dts);
+  auto bound = [&](int axis, int side, int periodic) {
+    const int n0 = axis == 0 ? g.ny : g.nx;
+    const int n1 = axis == 2 ? g.ny : g.nz;
+    const long long n = (long long)(n0 + 4) * (n1 + 4);
+    hipLaunchKernelGGL(k_tc_bound<S>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g, b.U, axis, side,
+                       periodic);
+  };
+  for (int a = 0; a < 3; a++)
+    if (g.bc[2 * a] == BC_PERIODIC && g.bc[2 * a + 1] == BC_PERIODIC) {
+      bound(a, 0, 1);
+      bound(a, 1, 1);
+    }
+  for (int a = 0; a < 3; a++) {
+    bound(a, 0, 0);
+    bound(a, 1, 0);
+  }
+}
+
+template <typename S>
 void launch_sources(hipStream_t s, const Box<S>& b, double dt, int stage, const Shape& sh) {
   hipLaunchKernelGGL(k_sources<S>, dim3(box_tiles(b.g, KC_COL, sh)), dim3(sh.block), 0, s, b, dt, stage, sh.kchunk);
 }
@@ -1004,6 +1232,8 @@ void launch_divb(hipStream_t s, const Geom& g, const S* U, double dx, double dy,
   template void launch_cfl<S>(hipStream_t, const Box<S>&, unsigned long long*, int*, const Shape&);               \
   template void launch_sources<S>(hipStream_t, const Box<S>&, double, int, const Shape&);                         \
   template void launch_cooling<S>(hipStream_t, const Box<S>&, const Cooling&, double);                            \
+  template void launch_tc_dtmin<S>(hipStream_t, const Box<S>&, const Conduction&, unsigned long long*);           \
+  template void launch_tc_substep<S>(hipStream_t, const Box<S>&, const Conduction&, double);                      \
   template void launch_sources_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned,               \
                                         const BoxClock*, int, const Shape&);                                      \
   template void launch_xy_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, const BoxClock*,   \

@@ -104,6 +104,21 @@ void launch_cfl(hipStream_t s, const Box<S>& b, unsigned long long* dtmin, int* 
 // cooling pass on U^{n+1} (src/hydro_solver.f90:206-210) with the step's dt
 template <typename S>
 void launch_cooling(hipStream_t s, const Box<S>& b, const Cooling& cl, double dt);
+
+// Thermal conduction (thermal_cond.f90): kind 1 isotropic, 2 anisotropic;
+// the parameters module's scalings.  Ghosts live in memory while it is on.
+struct Conduction {
+  int kind, sat;
+  double tempsc, rsc, rhosc, mu, psc, tsc, vsc2, bsc;
+};
+// min over interior cells of rho / Ksp(T), as the bit pattern of a positive
+// double atomically reduced into *out (get_dt_cond's loop, :78-110)
+template <typename S>
+void launch_tc_dtmin(hipStream_t s, const Box<S>& b, const Conduction& tc, unsigned long long* out);
+// one sub-step: heat fluxes of cells 0..n (into W), the E update of the
+// interior with dts, thermal_bounds on E
+template <typename S>
+void launch_tc_substep(hipStream_t s, const Box<S>& b, const Conduction& tc, double dts);
 // source terms of step() (src/hydro_solver.f90:115-121): up += dt * s after
 // the stage's update, s from the stage's input state
 template <typename S>

@@ -55,9 +55,16 @@ class Parameters:
     user_source_terms: bool = False
     bc_user: bool = False
     cooling: int = _capi.GM_COOL_NONE                # COOL_NONE / COOL_DMC / COOL_CHI
+    th_cond: int = _capi.GM_TC_OFF                   # TC_OFF / TC_ISOTROPIC / TC_ANISOTROPIC
+    tc_saturation: bool = False
     Tempsc: float = 1.0                              # parameters module scalings
     Psc: float = 1.0
     tsc: float = 1.0
+    rsc: float = 1.0
+    rhosc: float = 1.0
+    mu: float = 1.0
+    vsc2: float = 1.0
+    bsc: float = 3.5449077018110318                  # sqrt(4 pi Psc)
     cfl: float = 0.4
     tmax: float = 0.5
     dtprint: float = 0.1
@@ -181,6 +188,9 @@ def run(prm: Parameters, g: Globals, device: int = 0,
             if cooltab is None:
                 raise ValueError("cooling needs its table: pass cooltab (read_cooling_table)")
             dev.set_cooling(prm.cooling, cooltab, prm.Tempsc, prm.Psc, prm.tsc)
+        if prm.th_cond != _capi.GM_TC_OFF:
+            dev.set_conduction(prm.th_cond, prm.tc_saturation, tempsc=prm.Tempsc, rsc=prm.rsc, rhosc=prm.rhosc,
+                               mu=prm.mu, psc=prm.Psc, tsc=prm.tsc, vsc2=prm.vsc2, bsc=prm.bsc)
         dev.upload_u(g.u)
         dev.set_time(g.time)
         dev.boundary_u()                                     # main.f90:76

@@ -32,6 +32,7 @@ contains
   !> create the device subdomain and upload u (after initflow)
   subroutine gm_device_init()
     type(gm_params) :: p
+    type(gm_conduction_params) :: tp
     integer :: dev
     character(len=16) :: env
     integer :: envlen, envstat
@@ -55,9 +56,9 @@ contains
     p%cv = cv
     p%gamma = gamma
     p%eta = eta
-    if ((cooling /= COOL_NONE .and. cooling /= COOL_DMC .and. cooling /= COOL_CHI) .or. th_cond /= TC_OFF &
+    if ((cooling /= COOL_NONE .and. cooling /= COOL_DMC .and. cooling /= COOL_CHI) &
         .or. dif_rad .or. eq_of_state /= EOS_ADIABATIC) then
-      print '(a)', 'guacho_amd: cooling other than DMC/CHIANTI tables, conduction, diffuse radiation and '// &
+      print '(a)', 'guacho_amd: cooling other than DMC/CHIANTI tables, diffuse radiation and '// &
                    'non-adiabatic EOS are not built'
       stop 1
     end if
@@ -67,6 +68,13 @@ contains
     end if
     call gm_check(gm_create(p, gm_ctx), 'gm_create')
     if (cooling == COOL_DMC .or. cooling == COOL_CHI) call load_cooling()
+    if (th_cond /= TC_OFF) then
+      tp%kind = th_cond
+      tp%saturation = merge(1, 0, tc_saturation)
+      tp%tempsc = Tempsc; tp%rsc = rsc; tp%rhosc = rhosc; tp%mu = mu
+      tp%psc = Psc; tp%tsc = tsc; tp%vsc2 = vsc2; tp%bsc = bsc
+      call gm_check(gm_set_conduction(gm_ctx, tp), 'gm_set_conduction')
+    end if
     if (user_source_terms) call load_user_sources()
     if (bc_user) call load_user_bc()
     call gm_check(gm_upload_u(gm_ctx, u), 'gm_upload_u')

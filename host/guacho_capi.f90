@@ -32,6 +32,12 @@ module guacho_capi
     real(c_double) :: eta
   end type gm_params
 
+  ! thermal conduction settings (gm_conduction_params of include/guacho_amd.h)
+  type, bind(C) :: gm_conduction_params
+    integer(c_int32_t) :: kind, saturation
+    real(c_double) :: tempsc, rsc, rhosc, mu, psc, tsc, vsc2, bsc
+  end type gm_conduction_params
+
   interface
     integer(c_int) function gm_create(params, ctx) bind(C, name='gm_create')
       import :: gm_params, c_ptr, c_int
@@ -122,6 +128,12 @@ module guacho_capi
       real(c_double), intent(in) :: cooltab(*)
       real(c_double), value :: tempsc, psc, tsc
     end function gm_set_cooling
+
+    integer(c_int) function gm_set_conduction(ctx, params) bind(C, name='gm_set_conduction')
+      import :: c_ptr, c_int, gm_conduction_params
+      type(c_ptr), value :: ctx
+      type(gm_conduction_params), intent(in) :: params
+    end function gm_set_conduction
   end interface
 
 contains

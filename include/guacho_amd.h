@@ -140,7 +140,7 @@ int gm_download_divb(gm_ctx *ctx, double *divb);
  * kernel on the context's stream.  gm_kernel_times returns, per kernel
  * class, the accumulated milliseconds and launch count since the last
  * reset, and the kernel names (static strings). */
-#define GM_NKCLASS 10
+#define GM_NKCLASS 11
 int gm_set_timing(gm_ctx *ctx, int enable);
 int gm_kernel_times(gm_ctx *ctx, double *ms, int64_t *launches, const char **names);
 
@@ -194,6 +194,32 @@ int gm_set_user_bc(gm_ctx *ctx, const char *code_object, const char *kernel);
  * exp/log10 are within 1 ulp of the host's: cooled runs match the reference
  * to a tolerance, not bit for bit. */
 int gm_set_cooling(gm_ctx *ctx, int32_t kind, const double *cooltab, double tempsc, double psc, double tsc);
+
+#define GM_TC_OFF         0   /* constants.f90:96-98 */
+#define GM_TC_ISOTROPIC   1
+#define GM_TC_ANISOTROPIC 2
+
+/* Thermal conduction: th_cond and tc_saturation of the parameters module,
+ * and its scalings (Tempsc, rsc, rhosc, mu, Psc, tsc, vsc2, bsc). */
+typedef struct gm_conduction_params {
+  int32_t kind;        /* GM_TC_* */
+  int32_t saturation;  /* tc_saturation */
+  double tempsc, rsc, rhosc, mu, psc, tsc, vsc2, bsc;
+} gm_conduction_params;
+
+/* Thermal conduction at the end of every gm_tstep, after the boundary fill
+ * (thermal_conduction, src/thermal_cond.f90:690-768): dt_cond from the
+ * Spitzer coefficient (:78-110), super-time-stepping when it is shorter than
+ * the step (:625-681), per sub-step the isotropic (:189-267) or
+ * field-aligned (:277-487) heat fluxes, optionally saturated, the update of
+ * E, and thermal_bounds' E ghosts (:496-616).  With conduction on, ghosts
+ * live in memory (as with a user boundary hook) and gm_tstep waits for the
+ * dt_cond reduction.  pow and sqrt are the device's: conducting runs match
+ * the reference to a tolerance.  kind GM_TC_OFF turns it off. */
+int gm_set_conduction(gm_ctx *ctx, const gm_conduction_params *params);
+
+/* Sub-steps the last gm_tstep's conduction took. */
+int gm_conduction_substeps(const gm_ctx *ctx, int32_t *nsteps);
 
 /* ---- batches of independent boxes ----------------------------------- */
 /* No counterpart in the reference (one domain per run there).  A batch
