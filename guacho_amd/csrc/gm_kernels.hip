@@ -105,7 +105,7 @@ __device__ __forceinline__ void load_cons(const Geom& g, const S* __restrict__ U
 // writes U*), stage 2 the corrector (reads U*, updates U^n in place).
 //
 // One stage of tstep (src/hydro_solver.f90:134-229) is three kernels:
-//   xy:     primitives of a 64 x 8 plane tile and its halo into LDS
+//   xy:     primitives of a 32 x 16 plane tile and its halo into LDS
 //           (calcprim, src/hydro_core.f90:245-320), the x and y interface
 //           fluxes (e.g. hlldfluxes, src/hlld.f90:331-432), the x and y terms
 //           of the update, E3 and the x/y parts of E1, E2;
@@ -121,66 +121,95 @@ __device__ __forceinline__ void load_cons(const Geom& g, const S* __restrict__ U
 // ---------------------------------------------------------------------------
 
 // Plane tile of the xy kernel: TX x TY interior cells of one z-plane, one
-// wavefront per row.
-constexpr int TX = 64, TY = 8, XY_THREADS = TX * TY;
+// thread per cell.  Wave w holds rows 2w (lanes 0-31) and 2w+1 (lanes
+// 32-63), so a row is half a wavefront and x neighbours are adjacent lanes.
+constexpr int TX = 32, TY = 16, XY_THREADS = TX * TY;
+static_assert(TX == 32 && XY_THREADS == 512, "two tile rows per 64-lane wavefront");
 
-// LDS of the xy kernel for reconstruction order ORDER (halo H = ORDER):
-// primitives [8][RY][RX] of the tile and its halo (corners unused), then
-// the fluxes of the tile's closing faces -- y-face TY of each column
-// [8][TX] and x-face TX of each row [8][TY].  After the face solves, the
-// y-face fluxes [8][TY][TX] reuse the primitive area.
+// LDS of the xy kernel for reconstruction order ORDER (halo H = ORDER).
+// Primitives [8][RY][RX] of the tile and its x and y halos (corners are
+// not loaded); after the face solves that area holds the y-face fluxes
+// [8][TY][TX].  Then the fluxes of the tile's closing faces: y-face TY of
+// each column [8][TX] and x-face TX of each row [8][TY].  Second order adds
+// reconstructed face states: the upper x state of the halo cell (-1, r),
+// of the tile's last cell (TX-1, r) and the lower x state of the halo cell
+// (TX, r) [8][TY] each; the lower y state of the halo cell (c, TY) [8][TX];
+// and the upper y states of rows -1, 1, 3, .., TY-1 [8][TY/2+1][TX] (the
+// rows an even row, held in the other wavefront, reads below itself).
 template <int ORDER>
 struct XYLds {
   static constexpr int H = ORDER;
   static constexpr int RX = TX + 2 * H, RY = TY + 2 * H;
   static constexpr int PS = RX * RY;  // component stride of the primitives
   static constexpr int PRIM = 8 * PS;
-  static constexpr int TOPY = 8 * TX;
-  static constexpr int XEND = 8 * TY;
-  static constexpr int SIZE = PRIM + TOPY + XEND;
+  static constexpr int NLOAD = 2 * H * TX + TY * RX;  // primitives loaded
+  static constexpr int TOP = PRIM;
+  static constexpr int XEND = TOP + 8 * TX;
+  static constexpr int XL = XEND + 8 * TY;
+  static constexpr int XU = XL + 8 * TY;
+  static constexpr int XR = XU + 8 * TY;
+  static constexpr int YD = XR + 8 * TY;
+  static constexpr int YU = YD + 8 * TX;
+  static constexpr int NYU = TY / 2 + 1;
+  static constexpr int SIZE = ORDER == 1 ? XL : YU + 8 * NYU * TX;
   static_assert(8 * TY * TX <= PRIM, "y-face fluxes must fit the primitive area");
 };
 
-// Flux of one face from the primitives in LDS: `at` indexes (component 0)
-// the cell on the face's high side, `stride` steps across the face.  The
-// reconstruction is limiter() (src/hydro_core.f90:712-731) on the four-cell
-// stencil; y faces are permuted into the face-normal frame and back.
-template <int SOLVER, int ORDER, int AXIS, typename S>
-__device__ __forceinline__ bool lds_face_flux(const Phys& ph, const S* __restrict__ lp, int at, int stride,
-                                              S ff[8]) {
-  constexpr int PS = XYLds<ORDER>::PS;
-  S pl[8], pr[8];
+// Reconstructed states of one cell along one axis, from the primitives in
+// LDS (`at` the cell, `stride` one step along the axis): up is the left
+// state of the face above the cell, dn the right state of the face below.
+// limiter() (src/hydro_core.f90:712-731) forms both from the one slope
+// average(P - P-, P+ - P) of the cell, so each cell's slope serves both of
+// its faces.
+template <typename S>
+__device__ __forceinline__ void cell_states(int lim, const S* __restrict__ lp, int ps, int at, int stride, S up[8],
+                                            S dn[8]) {
 #pragma unroll
   for (int q = 0; q < 8; q++) {
-    pl[q] = lp[q * PS + at - stride];
-    pr[q] = lp[q * PS + at];
+    const S p = lp[q * ps + at];
+    const S s = lim_average(lim, p - lp[q * ps + at - stride], lp[q * ps + at + stride] - p);
+    up[q] = p + s * S(0.5);
+    dn[q] = p - s * S(0.5);
   }
-  if constexpr (ORDER == 2) {
-    S pll[8], prr[8];
+}
+
+// One of the two: the upper state when `up` holds, else the lower.
+template <typename S>
+__device__ __forceinline__ void cell_state(int lim, const S* __restrict__ lp, int ps, int at, int stride, bool up,
+                                           S v[8]) {
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-      pll[q] = lp[q * PS + at - 2 * stride];
-      prr[q] = lp[q * PS + at + stride];
-    }
-    limiter(ph.limiter, pll, pl, pr, prr);
+  for (int q = 0; q < 8; q++) {
+    const S p = lp[q * ps + at];
+    const S s = lim_average(lim, p - lp[q * ps + at - stride], lp[q * ps + at + stride] - p);
+    const S hs = s * S(0.5);
+    v[q] = up ? p + hs : p - hs;
   }
-  axis_perm<AXIS>(pl);
-  axis_perm<AXIS>(pr);
-  const bool ok = riemann<SOLVER>(ph, pl, pr, ff);
-  axis_perm<AXIS>(ff);
-  return ok;
 }
 
+// y faces exchange slots (1,2) and (5,6) when `ydir` holds (axis_perm<1>
+// with a run-time axis, by value selects).
 template <typename T>
-__device__ __forceinline__ T lane_down1(T v) {
-  return __shfl_down(v, 1, 64);
+__device__ __forceinline__ void perm_y_if(bool ydir, T v[8]) {
+  const T a = ydir ? v[2] : v[1], b = ydir ? v[1] : v[2];
+  const T e = ydir ? v[6] : v[5], f = ydir ? v[5] : v[6];
+  v[1] = a;
+  v[2] = b;
+  v[5] = e;
+  v[6] = f;
 }
 
-// xy kernel.  Block of 512 threads: wave w, lane l.  Lane l of wave w
-// solves x-face l of row w (between tile cells l-1 and l) and y-face w of
-// column l (between rows w-1 and w) -- the low faces of cell (l, w); its
-// high x face is lane l+1's (x-face TX from LDS for the last lane), its
-// high y face wave w+1's (y-face TY from LDS for the last row).
+// xy kernel.  Thread (c, r) -- lane c + 32 (r & 1) of wave r/2 -- solves the
+// low faces of cell (c, r): x-face c of row r (between cells c-1 and c)
+// and y-face r of column c.  Its high x face is lane c+1's, its high y face
+// row r+1's; the closing faces of a full tile (x-face TX of each row, y-face
+// TY of each column: TY + TX = 48 faces) are solved together by wave 0.
+// In a partial tile the closing faces are the low faces of the halo cells
+// just past it, solved by the threads of those cells.
+//
+// Second order: every thread forms the x and y states of its own cell
+// (cell_states), waves 6 and 7 those of the 2 (TX + TY) halo cells next to
+// the tile.  The x faces are solved before the y states are formed, which
+// keeps the live register set of one axis at a time.
 template <int SOLVER, int ORDER, int FLUXCD, typename S>
 __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt, int* __restrict__ err,
                                         S* __restrict__ lds) {
@@ -195,78 +224,230 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
   const int i0 = (rem % ntx) * TX + 1, j0 = (rem / ntx) * TY + 1;
   const int wx = min(TX, g.nx - i0 + 1), wy = min(TY, g.ny - j0 + 1);
   const int tid = (int)threadIdx.x, w = tid >> 6, l = tid & 63;
+  const int c = l & (TX - 1), r = 2 * w + (l >> 5);
   const S* __restrict__ Usrc = ORDER == 1 ? b.U : b.Us;
   S* __restrict__ lp = lds;
-  S* __restrict__ ltop = lds + L::PRIM;
-  S* __restrict__ lend = ltop + L::TOPY;
+  S* __restrict__ ltop = lds + L::TOP;
+  S* __restrict__ lend = lds + L::XEND;
 
   // primitives of the tile, its x halo on the tile's rows and its y halo
-  // on the tile's columns; ghosts by boundary mapping
-  for (int idx = tid; idx < PS; idx += XY_THREADS) {
-    const int rx = idx % RX, ry = idx / RX;
-    if (rx >= wx + 2 * H || ry >= wy + 2 * H) continue;
-    const bool inx = rx >= H && rx < H + wx, iny = ry >= H && ry < H + wy;
-    if (!inx && !iny) continue;
-    double u[8], p[8];
-    load_cons(g, Usrc, i0 - H + rx, j0 - H + ry, k, u);
-    u2prim(b.ph, u, p);
+  // on the tile's columns (H rows below, TY rows of RX, H rows above);
+  // ghosts by boundary mapping
+  // (at most two cells a thread; both cells' loads are issued before either
+  // recovery, so their latencies overlap)
+  static_assert(L::NLOAD <= 2 * XY_THREADS, "two primitive loads per thread");
+  {
+    double u[2][8];
+    int dst[2];
 #pragma unroll
-    for (int q = 0; q < 8; q++) lp[q * PS + idx] = (S)p[q];
+    for (int it = 0; it < 2; it++) {
+      const int idx = tid + it * XY_THREADS;
+      int rx, ry;
+      if (idx < H * TX) {
+        ry = idx / TX;
+        rx = H + idx % TX;
+      } else if (idx < H * TX + TY * RX) {
+        const int t = idx - H * TX;
+        ry = H + t / RX;
+        rx = t % RX;
+      } else {
+        const int t = idx - H * TX - TY * RX;
+        ry = H + TY + t / TX;
+        rx = H + t % TX;
+      }
+      dst[it] = -1;
+      if (idx < L::NLOAD && rx < wx + 2 * H && ry < wy + 2 * H) {
+        dst[it] = ry * RX + rx;
+        load_cons(g, Usrc, i0 - H + rx, j0 - H + ry, k, u[it]);
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < 2; it++) {
+      if (dst[it] < 0) continue;
+      double p[8];
+      u2prim(b.ph, u[it], p);
+#pragma unroll
+      for (int q = 0; q < 8; q++) lp[q * PS + dst[it]] = (S)p[q];
+    }
   }
   __syncthreads();
 
+  const int at = (r + H) * RX + c + H;  // own cell in the primitive area
+  const bool vx = r < wy && c <= wx;    // x-face c of row r is the tile's
+  const bool vy = c < wx && r <= wy;    // y-face r of column c is the tile's
+  const bool cx = w == 0 && l < TY;                   // closing x-face of row l
+  const bool cy = w == 0 && l >= TY && l < TY + TX;   // closing y-face of column l - TY
+  const int crow = cx ? l : 0, ccol = cy ? l - TY : 0;
   bool ok = true;
-  // closing faces of a full tile, straight to LDS outside the primitive area
-  if (w == 1 && wx == TX && l < wy) {
-    S ff[8];
-    ok &= lds_face_flux<SOLVER, ORDER, 0>(b.ph, lp, (l + H) * RX + TX + H, 1, ff);
-#pragma unroll
-    for (int q = 0; q < 8; q++) lend[q * TY + l] = ff[q];
-  }
-  if (w == 0 && wy == TY && l < wx) {
-    S ff[8];
-    ok &= lds_face_flux<SOLVER, ORDER, 1>(b.ph, lp, (TY + H) * RX + l + H, RX, ff);
-#pragma unroll
-    for (int q = 0; q < 8; q++) ltop[q * TX + l] = ff[q];
-  }
-  // the low faces of cell (l, w); a partial tile's closing faces fall in here
   S fx[8], fy[8];
 #pragma unroll
   for (int q = 0; q < 8; q++) fx[q] = fy[q] = S(0);
-  const bool vx = w < wy && l <= wx;
-  const bool vy = l < wx && w <= wy;
-  if (vx) ok &= lds_face_flux<SOLVER, ORDER, 0>(b.ph, lp, (w + H) * RX + l + H, 1, fx);
-  if (vy) ok &= lds_face_flux<SOLVER, ORDER, 1>(b.ph, lp, (w + H) * RX + l + H, RX, fy);
+
+  if constexpr (ORDER == 1) {
+    S pl[8], pr[8];
+    if (vx) {
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        pl[q] = lp[q * PS + at - 1];
+        pr[q] = lp[q * PS + at];
+      }
+      ok &= riemann<SOLVER>(b.ph, pl, pr, fx);
+    }
+    if (vy) {
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        pl[q] = lp[q * PS + at - RX];
+        pr[q] = lp[q * PS + at];
+      }
+      axis_perm<1>(pl);
+      axis_perm<1>(pr);
+      ok &= riemann<SOLVER>(b.ph, pl, pr, fy);
+      axis_perm<1>(fy);
+    }
+    if ((cx && wx == TX && crow < wy) || (cy && wy == TY && ccol < wx)) {
+      const int hi = cy ? (TY + H) * RX + ccol + H : (crow + H) * RX + TX + H;
+      const int st = cy ? RX : 1;
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        pl[q] = lp[q * PS + hi - st];
+        pr[q] = lp[q * PS + hi];
+      }
+      S ff[8];
+      perm_y_if(cy, pl);
+      perm_y_if(cy, pr);
+      ok &= riemann<SOLVER>(b.ph, pl, pr, ff);
+      perm_y_if(cy, ff);
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        if (cy) ltop[q * TX + ccol] = ff[q];
+        else lend[q * TY + crow] = ff[q];
+      }
+    }
+  } else {
+    S* __restrict__ xl = lds + L::XL;
+    S* __restrict__ xu = lds + L::XU;
+    S* __restrict__ xr = lds + L::XR;
+    S* __restrict__ yd = lds + L::YD;
+    S* __restrict__ yu = lds + L::YU;
+    const int lim = b.ph.limiter;
+    // halo cells, into LDS: wave 6 the x states of (-1, r) and (TX, r),
+    // wave 7 the y states of (c, -1) and (c, TY)
+    if (w >= 6) {
+      const int h = (w - 6) * 64 + l;
+      if (h < 2 * TY + 2 * TX) {
+        int hat, hst, hdst, hqs;
+        bool hup;
+        if (h < TY) {
+          hat = (h + H) * RX + H - 1, hst = 1, hup = true, hdst = L::XL + h, hqs = TY;
+        } else if (h < 2 * TY) {
+          hat = (h - TY + H) * RX + H + TX, hst = 1, hup = false, hdst = L::XR + h - TY, hqs = TY;
+        } else if (h < 2 * TY + TX) {
+          hat = (H - 1) * RX + h - 2 * TY + H, hst = RX, hup = true, hdst = L::YU + h - 2 * TY, hqs = L::NYU * TX;
+        } else {
+          hat = (H + TY) * RX + h - 2 * TY - TX + H, hst = RX, hup = false, hdst = L::YD + h - 2 * TY - TX, hqs = TX;
+        }
+        S hs[8];
+        cell_state(lim, lp, PS, hat, hst, hup, hs);
+#pragma unroll
+        for (int q = 0; q < 8; q++) lds[hdst + q * hqs] = hs[q];
+      }
+    }
+    // x: the upper state passes one lane up (lane c-1 holds cell c-1)
+    S pl[8], pr[8];
+    {
+      S pxu[8];
+      cell_states(lim, lp, PS, at, 1, pxu, pr);
+      if (c == TX - 1) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) xu[q * TY + r] = pxu[q];
+      }
+#pragma unroll
+      for (int q = 0; q < 8; q++) pl[q] = __shfl_up(pxu[q], 1, 64);
+    }
+    __syncthreads();  // halo states in LDS
+    if (c == 0) {
+#pragma unroll
+      for (int q = 0; q < 8; q++) pl[q] = xl[q * TY + r];
+    }
+    if (vx) ok &= riemann<SOLVER>(b.ph, pl, pr, fx);
+    // y: odd rows hand their upper state to the even row above them (the
+    // next wavefront) through LDS, even rows theirs to the odd row above
+    // (lanes 32 up in the same wavefront)
+    {
+      S pyu[8];
+      cell_states(lim, lp, PS, at, RX, pyu, pr);
+      if (r & 1) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) yu[(q * L::NYU + (r + 1) / 2) * TX + c] = pyu[q];
+      }
+#pragma unroll
+      for (int q = 0; q < 8; q++) pl[q] = __shfl_up(pyu[q], 32, 64);
+    }
+    __syncthreads();  // every read of the primitives is done; y states in LDS
+    if (!(r & 1)) {
+#pragma unroll
+      for (int q = 0; q < 8; q++) pl[q] = yu[(q * L::NYU + r / 2) * TX + c];
+    }
+    if (vy) {
+      axis_perm<1>(pl);
+      axis_perm<1>(pr);
+      ok &= riemann<SOLVER>(b.ph, pl, pr, fy);
+      axis_perm<1>(fy);
+    }
+    if ((cx && wx == TX && crow < wy) || (cy && wy == TY && ccol < wx)) {
+      S ff[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        pl[q] = cy ? yu[(q * L::NYU + TY / 2) * TX + ccol] : xu[q * TY + crow];
+        pr[q] = cy ? yd[q * TX + ccol] : xr[q * TY + crow];
+      }
+      perm_y_if(cy, pl);
+      perm_y_if(cy, pr);
+      ok &= riemann<SOLVER>(b.ph, pl, pr, ff);
+      perm_y_if(cy, ff);
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        if (cy) ltop[q * TX + ccol] = ff[q];
+        else lend[q * TY + crow] = ff[q];
+      }
+    }
+  }
   if (!ok) atomicOr(err, 1);
-  __syncthreads();  // every read of the primitives is done
+  // the cell's state for the update, fetched ahead of the flux exchange
+  const bool own = c < wx && r < wy;
+  const long long N = g.ncell;
+  const long long cc = cidx(g, i0 + c, j0 + r, k);
+  double u0[NH];
+  if (own) {
+#pragma unroll
+    for (int q = 0; q < NH; q++) u0[q] = (double)b.U[q * N + cc];
+  }
+  if constexpr (ORDER == 1) __syncthreads();  // every read of the primitives is done
   S* __restrict__ lfy = lds;
   if (vy) {
 #pragma unroll
-    for (int q = 0; q < 8; q++) lfy[(q * TY + w) * TX + l] = fy[q];
+    for (int q = 0; q < 8; q++) lfy[(q * TY + r) * TX + c] = fy[q];
   }
   __syncthreads();
 
   S fxr[8], fyu[8];
 #pragma unroll
   for (int q = 0; q < 8; q++) {
-    const S v = lane_down1(fx[q]);
-    fxr[q] = l == TX - 1 ? lend[q * TY + w] : v;
+    const S v = __shfl_down(fx[q], 1, 64);
+    fxr[q] = c == TX - 1 ? lend[q * TY + r] : v;
   }
-  if (l >= wx || w >= wy) return;
+  if (!own) return;
 #pragma unroll
-  for (int q = 0; q < 8; q++) fyu[q] = w == TY - 1 ? ltop[q * TX + l] : lfy[(q * TY + w + 1) * TX + l];
-  const long long N = g.ncell;
-  const long long c = cidx(g, i0 + l, j0 + w, k);
+  for (int q = 0; q < 8; q++) fyu[q] = r == TY - 1 ? ltop[q * TX + c] : lfy[(q * TY + r + 1) * TX + c];
   const double dtdx = dt / b.dx, dtdy = dt / b.dy;
   double* __restrict__ W = b.W;
 #pragma unroll
   for (int q = 0; q < NH; q++)
-    W[q * N + c] = ((double)b.U[q * N + c] - dtdx * ((double)fxr[q] - (double)fx[q]))
-                   - dtdy * ((double)fyu[q] - (double)fy[q]);
+    W[q * N + cc] = (u0[q] - dtdx * ((double)fxr[q] - (double)fx[q])) - dtdy * ((double)fyu[q] - (double)fy[q]);
   if constexpr (FLUXCD) {
-    W[5 * N + c] = (-(double)fy[7]) - (double)fyu[7];
-    W[6 * N + c] = (+(double)fx[7]) + (double)fxr[7];
-    b.E[2 * N + c] = (S)(0.25 * ((((-(double)fx[6]) - (double)fxr[6]) + (double)fy[5]) + (double)fyu[5]));
+    W[5 * N + cc] = (-(double)fy[7]) - (double)fyu[7];
+    W[6 * N + cc] = (+(double)fx[7]) + (double)fxr[7];
+    b.E[2 * N + cc] = (S)(0.25 * ((((-(double)fx[6]) - (double)fxr[6]) + (double)fy[5]) + (double)fyu[5]));
   }
 }
 
@@ -818,7 +999,7 @@ __device__ __forceinline__ void cfl_body(const Box<S>& b, unsigned lb, int kchun
 // __global__ wrappers: one box ...
 // ---------------------------------------------------------------------------
 template <int SOLVER, int ORDER, int FLUXCD, typename S>
-__global__ void __launch_bounds__(XY_THREADS) k_xy(Box<S> b, double dt, int* err) {
+__global__ void __launch_bounds__(XY_THREADS) __attribute__((amdgpu_waves_per_eu(4))) k_xy(Box<S> b, double dt, int* err) {
   __shared__ S lds[XYLds<ORDER>::SIZE];
   xy_body<SOLVER, ORDER, FLUXCD>(b, xcd_logical_block(), dt, err, lds);
 }
@@ -851,7 +1032,7 @@ __device__ __forceinline__ double stage_dt(const BoxClock* __restrict__ clk, int
 }
 
 template <int SOLVER, int ORDER, int FLUXCD, typename S>
-__global__ void __launch_bounds__(XY_THREADS) kb_xy(const Box<S>* __restrict__ boxes,
+__global__ void __launch_bounds__(XY_THREADS) __attribute__((amdgpu_waves_per_eu(4))) kb_xy(const Box<S>* __restrict__ boxes,
                                                     const unsigned* __restrict__ prefix, int nbox,
                                                     const BoxClock* __restrict__ clk, int* err) {
   __shared__ S lds[XYLds<ORDER>::SIZE];

@@ -69,7 +69,7 @@ struct BoxClock {
 
 // Launch shape of the column-marching kernels (z faces, field update, CFL):
 // threads per tile (64, 128 or 256) and z-pencil length.  The xy-face
-// kernel has a fixed 64 x 8 plane tile.
+// kernel has a fixed 32 x 16 plane tile.
 struct Shape {
   int block;
   int kchunk;
@@ -78,7 +78,7 @@ struct Shape {
 // Validates a subdomain configuration; GM_OK or a GM_ERR_* code with msg.
 int check_params(const gm_params* p, std::string& msg);
 
-// Tile classes of one step, for per-class tile tables: 64 x 8 plane tiles
+// Tile classes of one step, for per-class tile tables: 32 x 16 plane tiles
 // of the xy-face kernel, and column pencils of the z-face, field-update and
 // CFL kernels.
 enum KClass { KC_XY = 0, KC_COL, KC_COUNT };
