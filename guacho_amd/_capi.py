@@ -18,7 +18,7 @@ GM_OK, GM_ERR_CONFIG, GM_ERR_HIP, GM_ERR_NONFINITE, GM_ERR_OOM, GM_ERR_ARG, GM_E
 GM_PRECISION_FP64, GM_PRECISION_MIXED = 0, 1
 GM_COOL_NONE, GM_COOL_DMC, GM_COOL_CHI = 0, 3, 4
 GM_TC_OFF, GM_TC_ISOTROPIC, GM_TC_ANISOTROPIC = 0, 1, 2
-GM_NKCLASS = 11
+GM_NKCLASS = 12
 
 # exported entry points, in the order include/guacho_amd.h declares them
 EXPORTS = (

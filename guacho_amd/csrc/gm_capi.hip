@@ -23,8 +23,8 @@ namespace {
 
 const char* kClassNames[GM_NKCLASS] = {"xy_predictor", "xy_corrector", "z_predictor", "z_corrector",
                                        "bfield", "cfl_dt", "ghost", "layout", "sources", "cooling",
-                                       "conduction"};
-enum { KC_XY1, KC_XY2, KC_Z1, KC_Z2, KC_BFIELD, KC_CFL, KC_GHOST, KC_LAYOUT, KC_SOURCES, KC_COOLING, KC_CONDUCTION };
+                                       "conduction", "viscous"};
+enum { KC_XY1, KC_XY2, KC_Z1, KC_Z2, KC_BFIELD, KC_CFL, KC_GHOST, KC_LAYOUT, KC_SOURCES, KC_COOLING, KC_CONDUCTION, KC_VISCOUS };
 
 thread_local std::string g_create_error;
 
@@ -170,7 +170,7 @@ int check_params(const gm_params* p, std::string& msg) {
   for (int a = 0; a < 3; a++)
     if ((p->bc[2 * a] == GM_BC_PERIODIC) != (p->bc[2 * a + 1] == GM_BC_PERIODIC))
       return fail(nullptr, GM_ERR_CONFIG, "a periodic boundary must pair with a periodic boundary on the same axis");
-  if (p->eta != 0.0) return fail(nullptr, GM_ERR_CONFIG, "eta (artificial viscosity) must be 0");
+  if (!std::isfinite(p->eta)) return fail(nullptr, GM_ERR_CONFIG, "eta (artificial viscosity) must be finite");
   if (p->precision != GM_PRECISION_FP64 && p->precision != GM_PRECISION_MIXED)
     return fail(nullptr, GM_ERR_CONFIG, "precision must be GM_PRECISION_FP64 or GM_PRECISION_MIXED");
   if (!(p->dx > 0) || !(p->dy > 0) || !(p->dz > 0) || !std::isfinite(p->dx) || !std::isfinite(p->dy) ||
@@ -379,13 +379,16 @@ void tstep_t(gm_ctx* c, double dt) {
   if (fcd) timed(c, KC_BFIELD, [&] { gm::launch_bfield(s, b, dtm, 1, sh); });
   stage_sources_t(c, b, dtm, 1);
   // corrector: limited fluxes of U*, U^{n+1} = U^n - dt div F (+ dt s), in
-  // place; viscous_copy with eta = 0 is the identity on values
+  // place
   // boundaryII on U* (src/hydro_solver.f90:169) when ghosts live in memory
   if (b.g.gmem) fill_ghost_t<S>(c, b.Us, 2);
   timed(c, KC_XY2, [&] { gm::launch_xy(s, b, dt, 2, solver, fcd, err); });
   timed(c, KC_Z2, [&] { gm::launch_z(s, b, dt, 2, solver, fcd, err, sh); });
   if (fcd) timed(c, KC_BFIELD, [&] { gm::launch_bfield(s, b, dt, 2, sh); });
   stage_sources_t(c, b, dt, 2);
+  // viscous_copy (src/hydro_solver.f90:188); with eta = 0 the identity on
+  // values, so no pass
+  if (p.eta != 0.0) timed(c, KC_VISCOUS, [&] { gm::launch_viscous(s, b, p.eta); });
   // cooling (src/hydro_solver.f90:206-210)
   if (c->cool.kind) timed(c, KC_COOLING, [&] { gm::launch_cooling(s, b, c->cool, dt); });
   // boundaryI on U^{n+1} (src/hydro_solver.f90:216) when ghosts live in memory

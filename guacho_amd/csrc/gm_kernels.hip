@@ -734,6 +734,82 @@ __global__ void __launch_bounds__(256) k_cooling(Box<S> b, Cooling cl, double dt
 }
 
 // ---------------------------------------------------------------------------
+// Artificial viscosity, viscous_copy (src/hydro_solver.f90:47-65):
+//   u = up + eta ((((((up(i+1) + up(i-1)) + up(j+1)) + up(j-1)) + up(k+1))
+//                   + up(k-1)) - 6 up)
+// on interior cells.  up is the corrector's result (U, updated in place);
+// at that point up's ghosts still hold boundaryII's fill of the predictor
+// state U* (step() writes interior cells only), so a neighbour outside the
+// domain is U* through the boundary mapping (or U*'s stored ghost when
+// ghosts live in memory).  The result goes to W in FP64 and a second pass
+// stores it, so no cell is overwritten while a neighbour still reads it.
+// ---------------------------------------------------------------------------
+struct ViscNbr {
+  long long c;  // storage index
+  int flip;     // momentum component negated by a closed face (-1: none)
+  bool star;    // read from U* (outside the domain)
+};
+
+__device__ __forceinline__ ViscNbr visc_nbr(const Geom& g, int i, int j, int k) {
+  ViscNbr n;
+  n.star = i < 1 || i > g.nx || j < 1 || j > g.ny || k < 1 || k > g.nz;
+  n.flip = -1;
+  if (!n.star) {
+    n.c = cidx(g, i, j, k);
+    return n;
+  }
+  bool fx = false, fy = false, fz = false;
+  const int si = ghost_src(g, i, g.nx, g.bc[0], g.bc[1], fx);
+  const int sj = ghost_src(g, j, g.ny, g.bc[2], g.bc[3], fy);
+  const int sk = ghost_src(g, k, g.nz, g.bc[4], g.bc[5], fz);
+  n.c = cidx(g, si, sj, sk);
+  n.flip = fx ? 1 : fy ? 2 : fz ? 3 : -1;
+  return n;
+}
+
+template <typename S>
+__device__ __forceinline__ double visc_val(const Box<S>& b, const ViscNbr& n, int q) {
+  const double v = (double)(n.star ? b.Us : b.U)[q * b.g.ncell + n.c];
+  return q == n.flip ? -v : v;
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_viscous(Box<S> b, double eta) {
+  const Geom& g = b.g;
+  const long long n = (long long)g.nx * g.ny * g.nz;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % g.nx) + 1;
+  const long long r = t / g.nx;
+  const int j = (int)(r % g.ny) + 1, k = (int)(r / g.ny) + 1;
+  const long long c = cidx(g, i, j, k), N = g.ncell;
+  const ViscNbr xp = visc_nbr(g, i + 1, j, k), xm = visc_nbr(g, i - 1, j, k);
+  const ViscNbr yp = visc_nbr(g, i, j + 1, k), ym = visc_nbr(g, i, j - 1, k);
+  const ViscNbr zp = visc_nbr(g, i, j, k + 1), zm = visc_nbr(g, i, j, k - 1);
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const double up = (double)b.U[q * N + c];
+    const double lap = (((((visc_val(b, xp, q) + visc_val(b, xm, q)) + visc_val(b, yp, q)) + visc_val(b, ym, q))
+                         + visc_val(b, zp, q)) + visc_val(b, zm, q)) - 6.0 * up;
+    b.W[q * N + c] = up + eta * lap;
+  }
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_visc_store(Box<S> b) {
+  const Geom& g = b.g;
+  const long long n = (long long)g.nx * g.ny * g.nz;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % g.nx) + 1;
+  const long long r = t / g.nx;
+  const int j = (int)(r % g.ny) + 1, k = (int)(r / g.ny) + 1;
+  const long long c = cidx(g, i, j, k), N = g.ncell;
+#pragma unroll
+  for (int q = 0; q < 8; q++) b.U[q * N + c] = (S)b.W[q * N + c];
+}
+
+// ---------------------------------------------------------------------------
 // Thermal conduction, thermal_conduction (src/thermal_cond.f90:690-768),
 // sub-step kernels.  Temperatures and primitives come from the stored state
 // (ghosts in memory), T = (p / rho) Tempsc as calcprim gives them.
@@ -1287,6 +1363,14 @@ void launch_cfl(hipStream_t s, const Box<S>& b, unsigned long long* dtmin, int* 
 }
 
 template <typename S>
+void launch_viscous(hipStream_t s, const Box<S>& b, double eta) {
+  const long long n = (long long)b.g.nx * b.g.ny * b.g.nz;
+  const dim3 grid((unsigned)((n + 255) / 256));
+  hipLaunchKernelGGL(k_viscous<S>, grid, dim3(256), 0, s, b, eta);
+  hipLaunchKernelGGL(k_visc_store<S>, grid, dim3(256), 0, s, b);
+}
+
+template <typename S>
 void launch_cooling(hipStream_t s, const Box<S>& b, const Cooling& cl, double dt) {
   const long long n = (long long)b.g.nx * b.g.ny * b.g.nz;
   hipLaunchKernelGGL(k_cooling<S>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, b, cl, dt);
@@ -1413,6 +1497,7 @@ void launch_divb(hipStream_t s, const Geom& g, const S* U, double dx, double dy,
   template void launch_cfl<S>(hipStream_t, const Box<S>&, unsigned long long*, int*, const Shape&);               \
   template void launch_sources<S>(hipStream_t, const Box<S>&, double, int, const Shape&);                         \
   template void launch_cooling<S>(hipStream_t, const Box<S>&, const Cooling&, double);                            \
+  template void launch_viscous<S>(hipStream_t, const Box<S>&, double);                                            \
   template void launch_tc_dtmin<S>(hipStream_t, const Box<S>&, const Conduction&, unsigned long long*);           \
   template void launch_tc_substep<S>(hipStream_t, const Box<S>&, const Conduction&, double);                      \
   template void launch_sources_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned,               \

@@ -102,6 +102,9 @@ void launch_bfield(hipStream_t s, const Box<S>& b, double dt, int stage, const S
 template <typename S>
 void launch_cfl(hipStream_t s, const Box<S>& b, unsigned long long* dtmin, int* err, const Shape& sh);
 // cooling pass on U^{n+1} (src/hydro_solver.f90:206-210) with the step's dt
+// viscous_copy with eta != 0 (W as scratch), after the corrector
+template <typename S>
+void launch_viscous(hipStream_t s, const Box<S>& b, double eta);
 template <typename S>
 void launch_cooling(hipStream_t s, const Box<S>& b, const Cooling& cl, double dt);
 

@@ -79,7 +79,9 @@ typedef struct gm_params {
   double dx, dy, dz;
   double cv;              /* specific heat at constant volume */
   double gamma;           /* (cv+1)/cv, passed as the caller computed it */
-  double eta;             /* artificial viscosity; must be 0 */
+  double eta;             /* artificial viscosity of viscous_copy
+                             (src/hydro_solver.f90:47-65); finite; batches
+                             take 0 only */
 } gm_params;
 
 typedef struct gm_ctx gm_ctx;
@@ -140,7 +142,7 @@ int gm_download_divb(gm_ctx *ctx, double *divb);
  * kernel on the context's stream.  gm_kernel_times returns, per kernel
  * class, the accumulated milliseconds and launch count since the last
  * reset, and the kernel names (static strings). */
-#define GM_NKCLASS 11
+#define GM_NKCLASS 12
 int gm_set_timing(gm_ctx *ctx, int enable);
 int gm_kernel_times(gm_ctx *ctx, double *ms, int64_t *launches, const char **names);
 

@@ -14,7 +14,7 @@ def oracle_config(params: _capi.GmParams, cfl: float = 0.4, user_src: bool = Fal
                   cv=params.cv, riemann_solver=params.riemann_solver, slope_limiter=params.slope_limiter,
                   bc=list(params.bc), mhd=bool(params.mhd), pmhd=bool(params.pmhd),
                   enable_flux_cd=bool(params.enable_flux_cd), cfl=cfl,
-                  eight_wave=bool(params.eight_wave), user_src=user_src, user_bc=user_bc)
+                  eight_wave=bool(params.eight_wave), user_src=user_src, user_bc=user_bc, eta=params.eta)
 
 
 def random_state(rng, nx, ny, nz, cv=1.5, contrast=0.3, vmax=1.5, bmax=1.0):

@@ -51,7 +51,7 @@ def _create(**kw):
     (dict(slope_limiter=9), "slope_limiter"),
     (dict(bc=(5, 1, 3, 3, 3, 3)), "boundary kind"),
     (dict(bc=(3, 1, 3, 3, 3, 3)), "periodic"),
-    (dict(eta=0.1), "eta"),
+    (dict(eta=float("nan")), "eta"),
     (dict(nz=1), "at least 2"),
     (dict(dx=0.0), "dx"),
     (dict(precision=7), "PRECISION"),
