@@ -18,6 +18,7 @@ GM_OK, GM_ERR_CONFIG, GM_ERR_HIP, GM_ERR_NONFINITE, GM_ERR_OOM, GM_ERR_ARG, GM_E
 GM_PRECISION_FP64, GM_PRECISION_MIXED = 0, 1
 GM_COOL_NONE, GM_COOL_DMC, GM_COOL_CHI = 0, 3, 4
 GM_TC_OFF, GM_TC_ISOTROPIC, GM_TC_ANISOTROPIC = 0, 1, 2
+GM_EOS_ADIABATIC, GM_EOS_SINGLE_SPECIE = 1, 2
 GM_NKCLASS = 12
 
 # exported entry points, in the order include/guacho_amd.h declares them
@@ -26,7 +27,7 @@ EXPORTS = (
     "gm_download_primit", "gm_boundary_u", "gm_cfl_dtp", "gm_tstep", "gm_sync",
     "gm_download_divb", "gm_set_timing", "gm_kernel_times", "gm_device_bytes",
     "gm_set_time", "gm_set_user_sources", "gm_set_user_bc", "gm_set_cooling", "gm_set_conduction",
-    "gm_conduction_substeps",
+    "gm_conduction_substeps", "gm_set_eos",
     "gm_batch_create", "gm_batch_destroy", "gm_batch_last_error", "gm_batch_box_bytes",
     "gm_batch_free_bytes", "gm_batch_add", "gm_batch_remove", "gm_batch_advance", "gm_batch_sync",
     "gm_batch_clock", "gm_batch_download_u", "gm_batch_tuning", "gm_batch_active",
@@ -103,6 +104,7 @@ def lib() -> C.CDLL:
     L.gm_set_cooling.argtypes = [vp, C.c_int32, C.c_void_p, C.c_double, C.c_double, C.c_double]
     L.gm_set_conduction.argtypes = [vp, C.POINTER(GmConductionParams)]
     L.gm_conduction_substeps.argtypes = [vp, C.POINTER(C.c_int32)]
+    L.gm_set_eos.argtypes = [vp, C.c_int32, C.c_double]
     P = C.POINTER(GmParams)
     L.gm_batch_create.argtypes = [C.POINTER(GmBatchParams), C.POINTER(vp)]
     L.gm_batch_destroy.argtypes = [vp]
@@ -261,6 +263,10 @@ class Subdomain:
         n = C.c_int32()
         self._check(self.L.gm_conduction_substeps(self.h, C.byref(n)))
         return n.value
+
+    def set_eos(self, eq_of_state: int, tempsc: float = 1.0) -> None:
+        """Equation of state of u2prim (GM_EOS_ADIABATIC / GM_EOS_SINGLE_SPECIE) and its Tempsc."""
+        self._check(self.L.gm_set_eos(self.h, int(eq_of_state), float(tempsc)))
 
     def kernel_times(self) -> Dict[str, tuple]:
         ms = (C.c_double * GM_NKCLASS)()

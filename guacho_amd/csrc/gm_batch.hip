@@ -119,6 +119,8 @@ gm::Box<S> box_view(const Slot& s, int slot) {
   b.ph.mhd = s.p.mhd ? 1 : 0;
   b.ph.pmhd = s.p.pmhd ? 1 : 0;
   b.ph.limiter = s.p.slope_limiter;
+  b.ph.eos = GM_EOS_ADIABATIC;
+  b.ph.tempsc = 1.0;
   b.U = (S*)s.U;
   b.Us = (S*)s.Us;
   b.W = (double*)s.W;

@@ -223,6 +223,8 @@ gm_user_args user_args(const gm_ctx* c, void* u, double* sv, double dt, int stag
   for (int f = 0; f < 6; f++) a.bc[f] = c->p.bc[f];
   a.u = u;
   a.s = sv;
+  a.eos = c->ph.eos;
+  a.tempsc = c->ph.tempsc;
   return a;
 }
 
@@ -439,6 +441,8 @@ int gm_create(const gm_params* params, gm_ctx** out) {
   c->ph.mhd = params->mhd ? 1 : 0;
   c->ph.pmhd = params->pmhd ? 1 : 0;
   c->ph.limiter = params->slope_limiter;
+  c->ph.eos = GM_EOS_ADIABATIC;
+  c->ph.tempsc = 1.0;
 
   auto bail = [&](int code) {
     g_create_error = c->msg;
@@ -741,6 +745,17 @@ int gm_set_conduction(gm_ctx* c, const gm_conduction_params* tp) {
   // from here on the kernels read U's ghosts as stored: materialise them
   if (c->ghosts_dirty) fill_ghost1(c);
   GM_HIP(c, hipGetLastError());
+  return GM_OK;
+}
+
+int gm_set_eos(gm_ctx* c, int32_t eos, double tempsc) {
+  if (!c) return GM_ERR_ARG;
+  if (eos != GM_EOS_ADIABATIC && eos != GM_EOS_SINGLE_SPECIE)
+    return fail(c, GM_ERR_CONFIG, "eq_of_state must be GM_EOS_ADIABATIC or GM_EOS_SINGLE_SPECIE "
+                                  "(the others need passive scalars)");
+  if (!(tempsc > 0) || !std::isfinite(tempsc)) return fail(c, GM_ERR_CONFIG, "Tempsc must be positive and finite");
+  c->ph.eos = eos;
+  c->ph.tempsc = tempsc;
   return GM_OK;
 }
 

@@ -722,8 +722,7 @@ __global__ void __launch_bounds__(256) k_cooling(Box<S> b, Cooling cl, double dt
   double u[8], p[8];
 #pragma unroll
   for (int q = 0; q < 8; q++) u[q] = (double)b.U[q * N + c];
-  u2prim(b.ph, u, p);
-  const double T = (p[4] / p[0]) * cl.tempsc;
+  const double T = module_temp(b.ph, u2prim_t(b.ph, u, p), p, cl.tempsc);
   if (!(T > 10000.)) return;
   const double eth0 = b.ph.cv * p[4];
   const double aloss = cool_lambda(cl, T);
@@ -834,11 +833,11 @@ __device__ __forceinline__ TcCell tc_cell(const Box<S>& b, const Conduction& tc,
   double u[8], p[8];
 #pragma unroll
   for (int q = 0; q < 8; q++) u[q] = (double)b.U[q * N + c];
-  u2prim(b.ph, u, p);
+  const double te = u2prim_t(b.ph, u, p);
   TcCell r;
   r.rho = p[0];
   r.p = p[4];
-  r.t = (p[4] / p[0]) * tc.tempsc;
+  r.t = module_temp(b.ph, te, p, tc.tempsc);
   r.b[0] = p[5];
   r.b[1] = p[6];
   r.b[2] = p[7];
@@ -1270,10 +1269,10 @@ __global__ void __launch_bounds__(256) k_prim_all(long long ncell, Phys ph, cons
     double u[8], p[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) u[q] = (double)U[q * ncell + c];
-    u2prim(ph, u, p);
+    const double te = u2prim_t(ph, u, p);
 #pragma unroll
     for (int q = 0; q < 8; q++) Paos[c * 8 + q] = p[q];
-    if (T) T[c] = (p[4] / p[0]) * 1.0;
+    if (T) T[c] = te;
   }
 }
 

@@ -15,6 +15,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "guacho_amd.h"
 #include "guacho_amd_user.h"
 
 namespace gm {
@@ -49,6 +50,8 @@ struct Phys {
   double cv, gamma;
   int mhd, pmhd;   // full MHD / passive B
   int limiter;
+  int eos;         // GM_EOS_ADIABATIC or GM_EOS_SINGLE_SPECIE
+  double tempsc;   // Tempsc of the equation of state
 };
 
 template <typename T> __device__ __forceinline__ T dmax(T a, T b) { return (a > b) ? a : b; }
@@ -73,12 +76,27 @@ __device__ __forceinline__ void axis_perm(T v[8]) {
   }
 }
 
-// u2prim, src/hydro_core.f90:46-129 (EOS_ADIABATIC).  Always FP64: this is
-// the pressure recovery E - rho v^2/2 - B^2/2, whose cancellation at low
-// plasma beta the mixed form keeps in double.  One implementation, shared
-// with user plugins (gm_u2prim in guacho_amd_user.h).
+// u2prim, src/hydro_core.f90:46-129, with the equation of state's pressure
+// floor (EOS_SINGLE_SPECIE, :94-99).  Always FP64: this is the pressure
+// recovery E - rho v^2/2 - B^2/2, whose cancellation at low plasma beta the
+// mixed form keeps in double.  One implementation, shared with user plugins
+// (gm_u2prim, gm_eos in guacho_amd_user.h).
 __device__ __forceinline__ void u2prim(const Phys& ph, const double u[8], double p[8]) {
   gm_u2prim(ph.cv, ph.mhd, u, p);
+  if (ph.eos == GM_EOS_SINGLE_SPECIE) (void)gm_eos(ph.eos, ph.tempsc, p);
+}
+
+// ... and u2prim's temperature T (:90-99).
+__device__ __forceinline__ double u2prim_t(const Phys& ph, const double u[8], double p[8]) {
+  gm_u2prim(ph.cv, ph.mhd, u, p);
+  return gm_eos(ph.eos, ph.tempsc, p);
+}
+
+// The temperature a post-step module uses: its own Tempsc scaling of p/rho
+// under the adiabatic EOS, u2prim's floored T under EOS_SINGLE_SPECIE (one
+// Tempsc for both: the parameters module's).
+__device__ __forceinline__ double module_temp(const Phys& ph, double t_eos, const double p[8], double tempsc) {
+  return ph.eos == GM_EOS_SINGLE_SPECIE ? t_eos : (p[4] / p[0]) * tempsc;
 }
 
 // prim2u, src/hydro_core.f90:331-383

@@ -56,6 +56,7 @@ class Parameters:
     bc_user: bool = False
     cooling: int = _capi.GM_COOL_NONE                # COOL_NONE / COOL_DMC / COOL_CHI
     th_cond: int = _capi.GM_TC_OFF                   # TC_OFF / TC_ISOTROPIC / TC_ANISOTROPIC
+    eq_of_state: int = _capi.GM_EOS_ADIABATIC        # EOS_ADIABATIC / EOS_SINGLE_SPECIE
     tc_saturation: bool = False
     Tempsc: float = 1.0                              # parameters module scalings
     Psc: float = 1.0
@@ -180,6 +181,7 @@ def run(prm: Parameters, g: Globals, device: int = 0,
     if prm.bc_user and user_bc is None:
         raise ValueError("bc_user needs the device plugin: pass user_bc=(code_object, kernel)")
     with _capi.Subdomain(device_params(prm, g, device)) as dev:
+        dev.set_eos(prm.eq_of_state, prm.Tempsc)
         if prm.user_source_terms:
             dev.set_user_sources(*user_sources)
         if prm.bc_user:

@@ -56,10 +56,10 @@ contains
     p%cv = cv
     p%gamma = gamma
     p%eta = eta
-    if ((cooling /= COOL_NONE .and. cooling /= COOL_DMC .and. cooling /= COOL_CHI) &
-        .or. dif_rad .or. eq_of_state /= EOS_ADIABATIC) then
+    if ((cooling /= COOL_NONE .and. cooling /= COOL_DMC .and. cooling /= COOL_CHI) .or. dif_rad &
+        .or. (eq_of_state /= EOS_ADIABATIC .and. eq_of_state /= EOS_SINGLE_SPECIE)) then
       print '(a)', 'guacho_amd: cooling other than DMC/CHIANTI tables, diffuse radiation and '// &
-                   'non-adiabatic EOS are not built'
+                   'the EOS that need passive scalars are not built'
       stop 1
     end if
     if (radiation_pressure) then
@@ -67,6 +67,7 @@ contains
       stop 1
     end if
     call gm_check(gm_create(p, gm_ctx), 'gm_create')
+    call gm_check(gm_set_eos(gm_ctx, int(eq_of_state, c_int), real(Tempsc, c_double)), 'gm_set_eos')
     if (cooling == COOL_DMC .or. cooling == COOL_CHI) call load_cooling()
     if (th_cond /= TC_OFF) then
       tp%kind = th_cond

@@ -134,6 +134,12 @@ module guacho_capi
       type(c_ptr), value :: ctx
       type(gm_conduction_params), intent(in) :: params
     end function gm_set_conduction
+    integer(c_int) function gm_set_eos(ctx, eos, tempsc) bind(C, name='gm_set_eos')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: eos
+      real(c_double), value :: tempsc
+    end function gm_set_eos
   end interface
 
 contains

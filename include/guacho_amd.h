@@ -223,6 +223,20 @@ int gm_set_conduction(gm_ctx *ctx, const gm_conduction_params *params);
 /* Sub-steps the last gm_tstep's conduction took. */
 int gm_conduction_substeps(const gm_ctx *ctx, int32_t *nsteps);
 
+#define GM_EOS_ADIABATIC     1   /* constants.f90 EOS_ADIABATIC */
+#define GM_EOS_SINGLE_SPECIE 2   /* constants.f90 EOS_SINGLE_SPECIE */
+
+/* Equation of state of every primitive recovery, u2prim
+ * (src/hydro_core.f90:46-129) with eq_of_state and Tempsc of the parameters
+ * module.  GM_EOS_ADIABATIC (the default): T = (p / rho) Tempsc (:90-92).
+ * GM_EOS_SINGLE_SPECIE (:94-99): T = max(1, (p / rho) Tempsc) and the
+ * pressure becomes rho T / Tempsc -- a temperature floor of 1 K that the
+ * step, the CFL reduction, the source terms, cooling, conduction,
+ * gm_download_primit's temperature and plugins (gm_user_u2prim) all see.
+ * The other EOS need passive scalars and are not built (GM_ERR_CONFIG).
+ * Before the first call Tempsc is 1. */
+int gm_set_eos(gm_ctx *ctx, int32_t eq_of_state, double tempsc);
+
 /* ---- batches of independent boxes ----------------------------------- */
 /* No counterpart in the reference (one domain per run there).  A batch
  * advances many independent subdomains ("boxes") of uneven size together:

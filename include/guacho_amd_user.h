@@ -54,7 +54,10 @@ typedef struct gm_user_args {
                               hooks: the state whose ghosts were just filled */
   double *s;               /* sources: 8 x ncell, zero on entry (interior cells
                               are read back); hooks: NULL */
-  int64_t reserved[4];
+  int32_t eos;             /* eq_of_state (gm_set_eos); 0 reads as adiabatic */
+  int32_t pad;
+  double tempsc;           /* Tempsc of the equation of state */
+  int64_t reserved[2];
 } gm_user_args;
 
 #ifdef __HIPCC__
@@ -121,8 +124,22 @@ __device__ __forceinline__ void gm_u2prim(double cv, int mhd, const double u[8],
   p[7] = u[7];
 }
 
+/* The equation of state's temperature and pressure (src/hydro_core.f90:
+ * 90-99): eos 2 (EOS_SINGLE_SPECIE) floors T at 1 and sets p = rho T /
+ * Tempsc; any other eos is adiabatic, T = (p / rho) Tempsc.  Returns T. */
+__device__ __forceinline__ double gm_eos(int eos, double tempsc, double p[8]) {
+  if (eos == 2) {
+    const double x = (p[4] / p[0]) * tempsc;
+    const double t = (1.0 > x) ? 1.0 : x;
+    p[4] = (p[0] * t) / tempsc;
+    return t;
+  }
+  return (p[4] / p[0]) * tempsc;
+}
+
 __device__ __forceinline__ void gm_user_u2prim(const gm_user_args &a, const double u[8], double p[8]) {
   gm_u2prim(a.cv, a.mhd, u, p);
+  if (a.eos == 2) (void)gm_eos(a.eos, a.tempsc, p);
 }
 
 #endif /* __HIPCC__ */

@@ -69,7 +69,8 @@ static inline long cidx(const orc_state *s, int i, int j, int k) {
 /* per-cell physics: src/hydro_core.f90                                  */
 /* ===================================================================== */
 
-/* u2prim, hydro_core.f90:46-129 (EOS_ADIABATIC, no passives) */
+/* u2prim, hydro_core.f90:46-129 (EOS_ADIABATIC and EOS_SINGLE_SPECIE, no
+ * passives).  The adiabatic T is returned with Tempsc = 1; callers scale it. */
 void orc_u2prim(const orc_params *p, const double *uu, double *prim, double *T) {
   double r = fmax2(uu[0], 1e-15);                                   /* :62 */
   prim[0] = r;
@@ -92,7 +93,14 @@ void orc_u2prim(const orc_params *p, const double *uu, double *prim, double *T) 
     prim[6] = uu[6];
     prim[7] = uu[7];
   }
-  if (T) *T = (prim[4] / r) * 1.0;                                   /* :94, Tempsc=1 */
+  if (p->eos == 2) {                                                 /* :97-102 */
+    r = fmax2(r, 1e-15);
+    const double t = fmax2(1.0, (prim[4] / r) * p->tempsc);
+    prim[4] = (r * t) / p->tempsc;
+    if (T) *T = t;
+  } else if (T) {
+    *T = (prim[4] / r) * 1.0;                                        /* :94, Tempsc=1 */
+  }
 }
 
 /* prim2u, hydro_core.f90:331-383 (non-split branch) */
@@ -784,7 +792,7 @@ void orc_cooling(orc_state *s, double dt) {
         double *pm = &AT(s->primit, s, 0, i, j, k);
         double tdum;
         orc_u2prim(p, uu, pm, &tdum);
-        const double T = (pm[4] / pm[0]) * s->cool_tempsc;            /* hydro_core.f90:94 */
+        const double T = p->eos == 2 ? tdum : (pm[4] / pm[0]) * s->cool_tempsc;  /* hydro_core.f90:94-101 */
         if (T > 10000.) {
           const double eth0 = p->cv * pm[4];
           const double aloss = cool_lambda(s, T);
@@ -1151,10 +1159,17 @@ static void st_steps(double fs, int *ns, double *fstep) {
   *fstep = fs / superstep(*ns, snu);
 }
 
-/* primitives of every cell and Temp = (p / rho) Tempsc (calcprim) */
+/* Temp of calcprim from the primitives: (p / rho) Tempsc; under
+ * EOS_SINGLE_SPECIE u2prim's floored T, already in temp */
+static void tc_temp(orc_state *s) {
+  if (s->p.eos == 2) return;
+  for (long c = 0; c < s->ncell; c++) s->temp[c] = (s->primit[c * NEQ + 4] / s->primit[c * NEQ]) * s->tc_tempsc;
+}
+
+/* primitives of every cell and Temp (calcprim) */
 static void tc_calcprim(orc_state *s) {
   calcprim_arr(s, s->u);
-  for (long c = 0; c < s->ncell; c++) s->temp[c] = (s->primit[c * NEQ + 4] / s->primit[c * NEQ]) * s->tc_tempsc;
+  tc_temp(s);
 }
 
 #define TEMP(i, j, k) (s->temp[cidx(s, (i), (j), (k))])
@@ -1169,7 +1184,8 @@ double orc_dt_cond(orc_state *s) {
   for (int k = 1; k <= p->nz; k++)
     for (int j = 1; j <= p->ny; j++)
       for (int i = 1; i <= p->nx; i++)
-        dtp = fmin2(dtp, PRIM(0, i, j, k) / ksp((PRIM(4, i, j, k) / PRIM(0, i, j, k)) * s->tc_tempsc));
+        dtp = fmin2(dtp, PRIM(0, i, j, k) / ksp(p->eos == 2 ? TEMP(i, j, k)
+                                                 : (PRIM(4, i, j, k) / PRIM(0, i, j, k)) * s->tc_tempsc));
   const double x = ddx * s->tc_rsc;
   return ((((((0.25 * 0.5) * (x * x)) * p->cv) * 8.3145e7) * dtp) * s->tc_rhosc) / s->tc_mu;
 }
@@ -1293,7 +1309,7 @@ int orc_thermal(orc_state *s, double dt) {
   const orc_params *p = &s->p;
   const int nx = p->nx, ny = p->ny, nz = p->nz;
   const double nu = 0.01;
-  for (long c = 0; c < s->ncell; c++) s->temp[c] = (s->primit[c * NEQ + 4] / s->primit[c * NEQ]) * s->tc_tempsc;
+  tc_temp(s);
   const double dt_hydro = dt * s->tc_tsc;
   const double dt_cond = orc_dt_cond(s);
   int superstep_on = 1, nsteps;

@@ -43,6 +43,7 @@ class OrcParams(C.Structure):
         ("solver", C.c_int), ("limiter", C.c_int), ("bc", C.c_int * 6),
         ("mhd", C.c_int), ("pmhd", C.c_int), ("flux_cd", C.c_int),
         ("eight_wave", C.c_int), ("user_src", C.c_int), ("user_bc", C.c_int), ("nthreads", C.c_int),
+        ("eos", C.c_int), ("tempsc", C.c_double),
     ]
 
 
@@ -67,6 +68,8 @@ class Config:
     user_src: bool = False          # the checker's user gravity (oracle/ref/user_mod.f90)
     user_bc: bool = False           # the checker's impose_user_bc (oracle/ref/user_mod.f90)
     cfl: float = 0.4
+    eq_of_state: int = 1            # 2: EOS_SINGLE_SPECIE
+    tempsc: float = 1.0             # Tempsc of the EOS_SINGLE_SPECIE floor
 
     @property
     def gamma(self) -> float:
@@ -85,6 +88,8 @@ class Config:
         p.user_src = int(self.user_src)
         p.user_bc = int(self.user_bc)
         p.nthreads = nthreads
+        p.eos = int(self.eq_of_state)
+        p.tempsc = float(self.tempsc)
         return p
 
     def shape(self):
@@ -398,6 +403,9 @@ class RefLib:
         sw = (C.c_int * 3)()
         L.ref_switches(sw)
         self.eight_wave, self.user_source_terms, self.bc_user = bool(sw[0]), bool(sw[1]), bool(sw[2])
+        ev = C.c_int()
+        L.ref_eos(C.byref(ev))
+        self.eq_of_state = ev.value
 
     def config(self, dx=None, dy=None, dz=None) -> Config:
         dx = dx if dx is not None else 1.0 / self.nx
@@ -406,7 +414,8 @@ class RefLib:
         return Config(nx=self.nx, ny=self.ny, nz=self.nz, dx=dx, dy=dy, dz=dz, cv=self.cv,
                       riemann_solver=self.riemann_solver, slope_limiter=self.slope_limiter, bc=list(self.bc),
                       mhd=self.mhd, pmhd=self.pmhd, enable_flux_cd=True, cfl=self.cfl,
-                      eight_wave=self.eight_wave, user_src=self.user_source_terms, user_bc=self.bc_user)
+                      eight_wave=self.eight_wave, user_src=self.user_source_terms, user_bc=self.bc_user,
+                      eq_of_state=self.eq_of_state, tempsc=self.tempsc)
 
     def init(self, dx, dy, dz):
         self.L.ref_init(dx, dy, dz)

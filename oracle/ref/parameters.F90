@@ -23,7 +23,7 @@ module parameters
   logical, parameter :: dump_divb = .false.
   logical, parameter :: out_bin = .false.
   logical, parameter :: out_vtk = .false.
-  integer, parameter :: eq_of_state = EOS_ADIABATIC
+  integer, parameter :: eq_of_state = REF_EOS
   integer, parameter :: cooling = COOL_NONE
 
   integer, parameter :: bc_left   = REF_BC_LEFT

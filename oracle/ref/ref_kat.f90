@@ -247,4 +247,10 @@ contains
     tempsc_out = Tempsc
   end subroutine ref_conduction_config
 
+  ! eq_of_state of this build
+  subroutine ref_eos(eos) bind(C, name='ref_eos')
+    integer(c_int), intent(out) :: eos
+    eos = eq_of_state
+  end subroutine ref_eos
+
 end module ref_kat

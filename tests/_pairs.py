@@ -9,12 +9,13 @@ from oracle.oracle import Config, Oracle
 
 
 def oracle_config(params: _capi.GmParams, cfl: float = 0.4, user_src: bool = False,
-                  user_bc: bool = False) -> Config:
+                  user_bc: bool = False, eos=(1, 1.0)) -> Config:
     return Config(nx=params.nx, ny=params.ny, nz=params.nz, dx=params.dx, dy=params.dy, dz=params.dz,
                   cv=params.cv, riemann_solver=params.riemann_solver, slope_limiter=params.slope_limiter,
                   bc=list(params.bc), mhd=bool(params.mhd), pmhd=bool(params.pmhd),
                   enable_flux_cd=bool(params.enable_flux_cd), cfl=cfl,
-                  eight_wave=bool(params.eight_wave), user_src=user_src, user_bc=user_bc, eta=params.eta)
+                  eight_wave=bool(params.eight_wave), user_src=user_src, user_bc=user_bc, eta=params.eta,
+                  eq_of_state=eos[0], tempsc=eos[1])
 
 
 def random_state(rng, nx, ny, nz, cv=1.5, contrast=0.3, vmax=1.5, bmax=1.0):
@@ -32,18 +33,20 @@ def random_state(rng, nx, ny, nz, cv=1.5, contrast=0.3, vmax=1.5, bmax=1.0):
 
 
 def run_pair(params: _capi.GmParams, u0: np.ndarray, nsteps: int, cfl: float = 0.4, check_each=False,
-             user_sources=None, user_bc=None):
+             user_sources=None, user_bc=None, eos=(1, 1.0)):
     """Advance u0 nsteps on both paths with dt from each path's own CFL
     reduction; asserts the reductions agree bitwise every step.
     user_sources / user_bc: (code object, kernel) of plugins equal to the
     oracle's user source / user boundary hook; both sides see the same
-    globals::time.  Returns (u_device, u_oracle, dts)."""
-    O = Oracle(oracle_config(params, cfl, user_src=user_sources is not None, user_bc=user_bc is not None))
+    globals::time.  eos: (eq_of_state, Tempsc) of u2prim on both paths.
+    Returns (u_device, u_oracle, dts)."""
+    O = Oracle(oracle_config(params, cfl, user_src=user_sources is not None, user_bc=user_bc is not None, eos=eos))
     O.set_u(u0)
     O.boundaryI()
     O.calcprim_u()
     dts = []
     with _capi.Subdomain(params) as dev:
+        dev.set_eos(*eos)
         if user_sources is not None:
             dev.set_user_sources(*user_sources)
         if user_bc is not None:

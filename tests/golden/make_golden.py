@@ -159,6 +159,8 @@ def make(variant, seed):
                     src[i - 1, j - 1, k - 1] = R.source(i, j, k, prim[:, i + 1, j + 1, k + 1])
         out["src_primit"], out["src_out"] = prim, src
     out["switches"] = np.array([int(R.eight_wave), int(R.user_source_terms), int(R.bc_user)])
+    if R.eq_of_state != 1:
+        out["eos_config"] = np.array([R.eq_of_state, R.tempsc])
     out["hook_time"] = np.array([hook_time])
     out["dxyz"] = np.array([dx, dy, dz])
     cfg = np.array([R.nx, R.ny, R.nz, R.riemann_solver, R.slope_limiter, *R.bc, int(R.mhd), int(R.pmhd)])

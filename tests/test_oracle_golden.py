@@ -24,13 +24,20 @@ def _cfg(z):
     dx, dy, dz = z["dxyz"]
     sw = list(z["switches"]) if "switches" in z else []
     sw = sw + [0] * (3 - len(sw))
+    eos, etempsc = (int(z["eos_config"][0]), float(z["eos_config"][1])) if "eos_config" in z else (1, 1.0)
     return Config(nx=int(ci[0]), ny=int(ci[1]), nz=int(ci[2]), dx=dx, dy=dy, dz=dz, cv=float(cr[0]),
                   riemann_solver=int(ci[3]), slope_limiter=int(ci[4]), bc=[int(v) for v in ci[5:11]],
                   mhd=bool(ci[11]), pmhd=bool(ci[12]), cfl=float(cr[2]),
-                  eight_wave=bool(sw[0]), user_src=bool(sw[1]), user_bc=bool(sw[2]))
+                  eight_wave=bool(sw[0]), user_src=bool(sw[1]), user_bc=bool(sw[2]),
+                  eq_of_state=eos, tempsc=etempsc)
 
 
 def _tempsc(z):
+    """Tempsc the fixture's adiabatic T carries (the restatement's adiabatic
+    u2prim returns T with Tempsc = 1; under EOS_SINGLE_SPECIE its T is the
+    floored temperature itself)."""
+    if "eos_config" in z:
+        return 1.0
     return float(z["tc_config"][2]) if "tc_config" in z else 1.0
 
 
