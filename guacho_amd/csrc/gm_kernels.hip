@@ -123,7 +123,7 @@ __device__ __forceinline__ void load_cons(const Geom& g, const S* __restrict__ U
 // Plane tile of the xy kernel: TX x TY interior cells of one z-plane, one
 // thread per cell.  Wave w holds rows 2w (lanes 0-31) and 2w+1 (lanes
 // 32-63), so a row is half a wavefront and x neighbours are adjacent lanes.
-constexpr int TX = 32, TY = 16, XY_THREADS = TX * TY;
+constexpr int TX = 32, TY = 16, XY_THREADS = TX * TY, XY_WAVES = XY_THREADS / 64;
 static_assert(TX == 32 && XY_THREADS == 512, "two tile rows per 64-lane wavefront");
 
 // LDS of the xy kernel for reconstruction order ORDER (halo H = ORDER).
@@ -236,6 +236,11 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
   // (at most two cells a thread; both cells' loads are issued before either
   // recovery, so their latencies overlap)
   static_assert(L::NLOAD <= 2 * XY_THREADS, "two primitive loads per thread");
+  // Wave priorities: a new workgroup issues its tile loads ahead of the
+  // older workgroup's arithmetic (issue is oldest-first otherwise), and the
+  // waves with extra work before a barrier (halo states, closing faces) run
+  // ahead of the rest.  Priority changes issue order only.
+  __builtin_amdgcn_s_setprio(3);
   {
     double u[2][8];
     int dst[2];
@@ -261,6 +266,7 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
         load_cons(g, Usrc, i0 - H + rx, j0 - H + ry, k, u[it]);
       }
     }
+    __builtin_amdgcn_s_setprio(0);
 #pragma unroll
     for (int it = 0; it < 2; it++) {
       if (dst[it] < 0) continue;
@@ -330,10 +336,11 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
     S* __restrict__ yd = lds + L::YD;
     S* __restrict__ yu = lds + L::YU;
     const int lim = b.ph.limiter;
-    // halo cells, into LDS: wave 6 the x states of (-1, r) and (TX, r),
-    // wave 7 the y states of (c, -1) and (c, TY)
-    if (w >= 6) {
-      const int h = (w - 6) * 64 + l;
+    // halo cells, into LDS, by the last two waves (h = 0..95): the x
+    // states of (-1, r) and (TX, r), the y states of (c, -1) and (c, TY)
+    if (w >= XY_WAVES - 2) __builtin_amdgcn_s_setprio(2);
+    if (w >= XY_WAVES - 2) {
+      const int h = (w - (XY_WAVES - 2)) * 64 + l;
       if (h < 2 * TY + 2 * TX) {
         int hat, hst, hdst, hqs;
         bool hup;
@@ -352,6 +359,7 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
         for (int q = 0; q < 8; q++) lds[hdst + q * hqs] = hs[q];
       }
     }
+    if (w >= XY_WAVES - 2) __builtin_amdgcn_s_setprio(0);
     // x: the upper state passes one lane up (lane c-1 holds cell c-1)
     S pl[8], pr[8];
     {
@@ -384,6 +392,7 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
       for (int q = 0; q < 8; q++) pl[q] = __shfl_up(pyu[q], 32, 64);
     }
     __syncthreads();  // every read of the primitives is done; y states in LDS
+    if (w == 0) __builtin_amdgcn_s_setprio(2);
     if (!(r & 1)) {
 #pragma unroll
       for (int q = 0; q < 8; q++) pl[q] = yu[(q * L::NYU + r / 2) * TX + c];
@@ -412,6 +421,7 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
       }
     }
   }
+  if (w == 0) __builtin_amdgcn_s_setprio(0);
   if (!ok) atomicOr(err, 1);
   // the cell's state for the update, fetched ahead of the flux exchange
   const bool own = c < wx && r < wy;
