@@ -31,20 +31,25 @@
 
 #include <float.h>
 #include <math.h>
+#include <quadmath.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
 
-#define NEQ 8
+#define NEQ 8          /* MHD variables (neqdyn) */
+#define NEQ_MAX 10     /* with the largest passive count */
+#define NEQP(p) (NEQ + (p)->npas)
+/* an EOS whose u2prim returns its own temperature (SINGLE_SPECIE, H_RATE) */
+#define EOS_OWN_T(p) ((p)->eos == 2 || (p)->eos == 3)
 
 struct orc_state {
   orc_params p;
   int NX, NY, NZ;   /* extents incl. ghosts: n+4 */
   long ncell;
   double time;      /* globals::time */
-  int cool_kind;    /* 0, COOL_DMC (3) or COOL_CHI (4) */
+  int cool_kind;    /* 0, COOL_H (1), COOL_DMC (3) or COOL_CHI (4) */
   int tc_kind, tc_sat;  /* thermal conduction: 0 off, 1 isotropic, 2 anisotropic */
   double tc_tempsc, tc_rsc, tc_rhosc, tc_mu, tc_psc, tc_tsc, tc_vsc2, tc_bsc;
   double *tcf;      /* heat fluxes f(5), g(5), h(5) per cell (3 x ncell) */
@@ -62,7 +67,7 @@ static inline double sign1(double x) { return copysign(1.0, x); }
 static inline long cidx(const orc_state *s, int i, int j, int k) {
   return ((long)(k + 1) * s->NY + (j + 1)) * s->NX + (i + 1);
 }
-#define AT(arr, s, q, i, j, k) ((arr)[cidx((s), (i), (j), (k)) * NEQ + (q)])
+#define AT(arr, s, q, i, j, k) ((arr)[cidx((s), (i), (j), (k)) * NEQP(&(s)->p) + (q)])
 #define EAT(s, q, i, j, k) ((s)->e[cidx((s), (i), (j), (k)) * 3 + (q)])
 
 /* ===================================================================== */
@@ -93,10 +98,17 @@ void orc_u2prim(const orc_params *p, const double *uu, double *prim, double *T) 
     prim[6] = uu[6];
     prim[7] = uu[7];
   }
+  for (int a = 0; a < p->npas; a++) prim[NEQ + a] = uu[NEQ + a];     /* :86-90 */
   if (p->eos == 2) {                                                 /* :97-102 */
     r = fmax2(r, 1e-15);
     const double t = fmax2(1.0, (prim[4] / r) * p->tempsc);
     prim[4] = (r * t) / p->tempsc;
+    if (T) *T = t;
+  } else if (p->eos == 3) {                                          /* EOS_H_RATE, :107-112 */
+    double dentot = 2.0 * r - prim[NEQ];
+    dentot = fmax2(dentot, 1e-15);
+    const double t = fmax2(1.0, (prim[4] / dentot) * p->tempsc);
+    prim[4] = (dentot * t) / p->tempsc;
     if (T) *T = t;
   } else if (T) {
     *T = (prim[4] / r) * 1.0;                                        /* :94, Tempsc=1 */
@@ -120,6 +132,7 @@ void orc_prim2u(const orc_params *p, const double *prim, double *uu) {
   } else {
     uu[5] = uu[6] = uu[7] = 0.0;
   }
+  for (int a = 0; a < p->npas; a++) uu[NEQ + a] = prim[NEQ + a];     /* :377-379 */
 }
 
 /* prim2f, hydro_core.f90:395-476 (non-split branches) */
@@ -151,6 +164,7 @@ void orc_prim2f(const orc_params *p, const double *prim, double *ff) {
   } else {
     ff[5] = ff[6] = ff[7] = 0.0;
   }
+  for (int a = 0; a < p->npas; a++) ff[NEQ + a] = prim[NEQ + a] * prim[1];  /* :470-472 */
 }
 
 /* swapy / swapz, hydro_core.f90:485-534 */
@@ -232,7 +246,7 @@ static double lim_average(int lim, double a, double b) {
 
 /* limiter, hydro_core.f90:712-731 */
 void orc_limiter(const orc_params *p, const double *pll, double *pl, double *pr, const double *prr) {
-  for (int q = 0; q < NEQ; q++) {
+  for (int q = 0; q < NEQP(p); q++) {
     double dl = pl[q] - pll[q];
     double dm = pr[q] - pl[q];
     double dr = prr[q] - pr[q];
@@ -300,6 +314,7 @@ int orc_hlld(const orc_params *p, const double *pl, const double *pr, double *ff
     ff[5] = 0.0;
     ff[6] = bystl * sM - bx * vstl;
     ff[7] = bzstl * sM - bx * wstl;
+    for (int a = 0; a < p->npas; a++) ff[NEQ + a] = ((sM * pl[NEQ + a]) * slmul) / slmsM;  /* :149-151 */
     return 0;
   }
 
@@ -328,6 +343,7 @@ int orc_hlld(const orc_params *p, const double *pl, const double *pr, double *ff
     ff[5] = 0.0;
     ff[6] = bystr * sM - bx * vstr;
     ff[7] = bzstr * sM - bx * wstr;
+    for (int a = 0; a < p->npas; a++) ff[NEQ + a] = ((sM * pr[NEQ + a]) * srmur) / srmsM;  /* :197-199 */
     return 0;
   }
 
@@ -375,6 +391,7 @@ int orc_hlld(const orc_params *p, const double *pl, const double *pr, double *ff
     ff[5] = 0.0;
     ff[6] = bystst * sM - bx * vstst;
     ff[7] = bzstst * sM - bx * wstst;
+    for (int a = 0; a < p->npas; a++) ff[NEQ + a] = ((sM * pl[NEQ + a]) * slmul) / slmsM;  /* :276-278 */
     return 0;
   }
 
@@ -393,6 +410,7 @@ int orc_hlld(const orc_params *p, const double *pl, const double *pr, double *ff
     ff[5] = 0.0;
     ff[6] = bystst * sM - bx * vstst;
     ff[7] = bzstst * sM - bx * wstst;
+    for (int a = 0; a < p->npas; a++) ff[NEQ + a] = ((sM * pr[NEQ + a]) * srmur) / srmsM;  /* :307-309 */
     return 0;
   }
   return 1;                                                          /* :316-317 */
@@ -401,12 +419,12 @@ int orc_hlld(const orc_params *p, const double *pl, const double *pr, double *ff
 /* HLL average shared by prim2fhlle (hlle.f90:74-79) and prim2fhll (hll.f90:73-78) */
 static void hll_average(const orc_params *p, const double *pl, const double *pr,
                         double sl, double sr, double *ff) {
-  double fl[NEQ], fr[NEQ], ul[NEQ], ur[NEQ];
+  double fl[NEQ_MAX], fr[NEQ_MAX], ul[NEQ_MAX], ur[NEQ_MAX];
   orc_prim2f(p, pl, fl);
   orc_prim2f(p, pr, fr);
   orc_prim2u(p, pl, ul);
   orc_prim2u(p, pr, ur);
-  for (int q = 0; q < NEQ; q++)
+  for (int q = 0; q < NEQP(p); q++)
     ff[q] = ((sr * fl[q] - sl * fr[q]) + (sl * sr) * (ur[q] - ul[q])) / (sr - sl);
 }
 
@@ -444,7 +462,7 @@ int orc_hllc(const orc_params *p, const double *pl, const double *pr, double *ff
   double slmul = sl - pl[1], srmur = sr - pr[1];
   double rholul = pl[0] * pl[1], rhorur = pr[0] * pr[1];
   double sst = (((srmur * rhorur - slmul * rholul) - pr[4]) + pl[4]) / (srmur * pr[0] - slmul * pl[0]);
-  double uuk[NEQ], uu[NEQ];
+  double uuk[NEQ_MAX], uu[NEQ_MAX];
   if (sst >= 0.0) {                                                  /* :79-104 */
     double rhost = (pl[0] * slmul) / (sl - sst);
     double ek = (0.5 * pl[0]) * ((pl[1] * pl[1] + pl[2] * pl[2]) + pl[3] * pl[3]) + cv * pl[4];
@@ -454,9 +472,10 @@ int orc_hllc(const orc_params *p, const double *pl, const double *pr, double *ff
     uuk[3] = rhost * pl[3];
     uuk[4] = rhost * (ek / pl[0] + (sst - pl[1]) * (sst + pl[4] / (pl[0] * slmul)));
     for (int q = 5; q < 8; q++) uuk[q] = p->pmhd ? (rhost * pl[q]) / pl[0] : 0.0;
+    for (int a = 0; a < p->npas; a++) uuk[NEQ + a] = (rhost * pl[NEQ + a]) / pl[0];  /* :94-97 */
     orc_prim2f(p, pl, ff);
     orc_prim2u(p, pl, uu);
-    for (int q = 0; q < NEQ; q++) ff[q] = ff[q] + sl * (uuk[q] - uu[q]);
+    for (int q = 0; q < NEQP(p); q++) ff[q] = ff[q] + sl * (uuk[q] - uu[q]);
     return 0;
   }
   if (sst <= 0.0) {                                                  /* :106-133 */
@@ -468,9 +487,10 @@ int orc_hllc(const orc_params *p, const double *pl, const double *pr, double *ff
     uuk[3] = rhost * pr[3];
     uuk[4] = rhost * (ek / pr[0] + (sst - pr[1]) * (sst + pr[4] / (pr[0] * srmur)));
     for (int q = 5; q < 8; q++) uuk[q] = p->pmhd ? (rhost * pr[q]) / pr[0] : 0.0;
+    for (int a = 0; a < p->npas; a++) uuk[NEQ + a] = (rhost * pr[NEQ + a]) / pr[0];  /* :123-126 */
     orc_prim2f(p, pr, ff);
     orc_prim2u(p, pr, uu);
-    for (int q = 0; q < NEQ; q++) ff[q] = ff[q] + sr * (uuk[q] - uu[q]);
+    for (int q = 0; q < NEQP(p); q++) ff[q] = ff[q] + sr * (uuk[q] - uu[q]);
     return 0;
   }
   return 1;                                                          /* :135-138 */
@@ -507,7 +527,11 @@ int orc_create(const orc_params *p, orc_state **out) {
   s->NY = p->ny + 4;
   s->NZ = p->nz + 4;
   s->ncell = (long)s->NX * s->NY * s->NZ;
-  size_t n8 = (size_t)s->ncell * NEQ * sizeof(double);
+  if (p->npas < 0 || p->npas > NEQ_MAX - NEQ) {
+    free(s);
+    return 2;
+  }
+  size_t n8 = (size_t)s->ncell * NEQP(p) * sizeof(double);
   s->u = (double *)calloc(1, n8);
   s->up = (double *)calloc(1, n8);
   s->primit = (double *)calloc(1, n8);
@@ -536,26 +560,26 @@ void orc_destroy(orc_state *s) {
 }
 
 long orc_cells(const orc_state *s) { return s->ncell; }
-void orc_set_u(orc_state *s, const double *u) { memcpy(s->u, u, (size_t)s->ncell * NEQ * sizeof(double)); }
-void orc_get_u(const orc_state *s, double *u) { memcpy(u, s->u, (size_t)s->ncell * NEQ * sizeof(double)); }
-void orc_set_up(orc_state *s, const double *u) { memcpy(s->up, u, (size_t)s->ncell * NEQ * sizeof(double)); }
-void orc_get_up(const orc_state *s, double *u) { memcpy(u, s->up, (size_t)s->ncell * NEQ * sizeof(double)); }
-void orc_set_primit(orc_state *s, const double *u) { memcpy(s->primit, u, (size_t)s->ncell * NEQ * sizeof(double)); }
-void orc_get_primit(const orc_state *s, double *u) { memcpy(u, s->primit, (size_t)s->ncell * NEQ * sizeof(double)); }
+void orc_set_u(orc_state *s, const double *u) { memcpy(s->u, u, (size_t)s->ncell * NEQP(&s->p) * sizeof(double)); }
+void orc_get_u(const orc_state *s, double *u) { memcpy(u, s->u, (size_t)s->ncell * NEQP(&s->p) * sizeof(double)); }
+void orc_set_up(orc_state *s, const double *u) { memcpy(s->up, u, (size_t)s->ncell * NEQP(&s->p) * sizeof(double)); }
+void orc_get_up(const orc_state *s, double *u) { memcpy(u, s->up, (size_t)s->ncell * NEQP(&s->p) * sizeof(double)); }
+void orc_set_primit(orc_state *s, const double *u) { memcpy(s->primit, u, (size_t)s->ncell * NEQP(&s->p) * sizeof(double)); }
+void orc_get_primit(const orc_state *s, double *u) { memcpy(u, s->primit, (size_t)s->ncell * NEQP(&s->p) * sizeof(double)); }
 void orc_get_temp(const orc_state *s, double *t) { memcpy(t, s->temp, (size_t)s->ncell * sizeof(double)); }
 void orc_get_flux(const orc_state *s, int axis, double *out) {
   const double *src = axis == 0 ? s->f : axis == 1 ? s->g : s->h;
-  memcpy(out, src, (size_t)s->ncell * NEQ * sizeof(double));
+  memcpy(out, src, (size_t)s->ncell * NEQP(&s->p) * sizeof(double));
 }
 void orc_get_e(const orc_state *s, double *e) { memcpy(e, s->e, (size_t)s->ncell * 3 * sizeof(double)); }
 
-/* copy the 8-vector of cell (si,sj,sk) of arr into cell (di,dj,dk), optionally
+/* copy the variables of cell (si,sj,sk) of arr into cell (di,dj,dk), optionally
  * negating component q_flip (0-based; -1 for none) */
 static inline void copy_cell(orc_state *s, double *arr, int di, int dj, int dk,
                              int si, int sj, int sk, int q_flip) {
   double *d = &AT(arr, s, 0, di, dj, dk);
   const double *src = &AT(arr, s, 0, si, sj, sk);
-  for (int q = 0; q < NEQ; q++) d[q] = (q == q_flip) ? -src[q] : src[q];
+  for (int q = 0; q < NEQP(&s->p); q++) d[q] = (q == q_flip) ? -src[q] : src[q];
 }
 
 void orc_set_time(orc_state *s, double time) { s->time = time; }
@@ -743,8 +767,9 @@ void orc_boundaryII(orc_state *s) {
 static void calcprim_arr(orc_state *s, const double *src) {
   const orc_params *p = &s->p;
   long n = s->ncell;
+  const int nq = NEQP(p);
 #pragma omp parallel for schedule(static)
-  for (long c = 0; c < n; c++) orc_u2prim(p, &src[c * NEQ], &s->primit[c * NEQ], &s->temp[c]);
+  for (long c = 0; c < n; c++) orc_u2prim(p, &src[c * nq], &s->primit[c * nq], &s->temp[c]);
 }
 void orc_calcprim_u(orc_state *s) { calcprim_arr(s, s->u); }
 
@@ -756,7 +781,7 @@ static void calcprim_ghosts(orc_state *s) {
       for (int i = -1; i <= p->nx + 2; i++) {
         if (i >= 1 && i <= p->nx && j >= 1 && j <= p->ny && k >= 1 && k <= p->nz) continue;
         const long c = cidx(s, i, j, k);
-        orc_u2prim(p, &s->u[c * NEQ], &s->primit[c * NEQ], &s->temp[c]);
+        orc_u2prim(p, &s->u[c * NEQP(p)], &s->primit[c * NEQP(p)], &s->temp[c]);
       }
 }
 
@@ -782,9 +807,120 @@ static double cool_lambda(const orc_state *s, double T) {
 /* coolingdmc / coolingchi (cooling_dmc.f90:115-156): for every interior cell
  * above Tmin = 1e4 K, p *= exp(-Ce dt), Ce = Lambda rho^2 / (cv p Psc), and
  * E changes by the same thermal energy; primit keeps the cooled pressure */
+/* cooling_h.f90 (PASSIVES): rates of atomic(), :80-139.  The reference is
+ * built with -fdefault-real-8 alone (OT/Makefile:121), under which
+ * double-precision constants (1.d4, 2.55d-13, 5.83d-11, 1.133D-24) are
+ * REAL(16): the operations that touch them run in quad precision and the
+ * REAL(8) function result is their rounding.  __float128 reproduces that. */
+typedef __float128 quad;
+static double hr_alpha(double t) {                                                   /* :80-88 */
+  return (double)(2.55e-13Q * powq(1.e4Q / (quad)t, (quad)0.79));
+}
+static double hr_colf(double t) {                                                    /* :108-116 */
+  return (double)((5.83e-11Q * (quad)sqrt(t)) * (quad)exp(-(157828. / t)));
+}
+static double hr_betah(double t) {                                                   /* :124-137 */
+  const double a = 157890. / t;
+  const double s = (-0.0713 + 0.5 * log(a)) + 0.640 * pow(a, -0.33333);
+  return (double)((1.133e-24Q / (quad)sqrt(a)) * (quad)s);
+}
+
+/* ALOSS, cooling_h.f90:141-218 */
+static double hr_aloss(double x1, double x2, double dt, double den, double dh0, double te0) {
+  const double XION = 2.179e-11, XO = 1.e-3;
+  const double C0 = 0.5732, C1 = 1.8288e-5, C2 = -1.15822e-10, C3 = 9.4288e-16;
+  const double D0 = 0.5856, D1 = 1.55083e-5, D2 = -9.669e-12, D3 = 5.716e-19;
+  const double ENK = 118409., EN = 1.634E-11;
+  const double te = fmax2(te0, 10.);
+  const double dh = den;
+  const double dhp = (1. - x1) * dh;
+  const double de = dhp + 1.e-4 * dh;
+  const double doi = XO * dh0;
+  const double doii = XO * dhp;
+  (void)x2;
+  (void)dt;                                   /* SHP, :166, is never used */
+  if (te <= 1e4) return 0.;
+  double omega = 0.;
+  if (te <= 55000.) omega = C0 + te * (C1 + te * (C2 + te * C3));
+  if (te >= 72000.) omega = D0 + te * (D1 + te * (D2 + te * D3));
+  if (te > 55000. && te < 72000.) {
+    const double omegal = C0 + te * (C1 + te * (C2 + te * C3));
+    const double omegah = D0 + te * (D1 + te * (D2 + te * D3));
+    const double frac = (te - 55000.) / 17000.;
+    omega = (1. - frac) * omegal + frac * omegah;
+  }
+  const double qla = ((8.6287E-6 / (2. * sqrt(te))) * omega) * exp(-(ENK / te));
+  double ecoll = ((de * dh0) * qla) * EN;
+  ecoll = fmax2(ecoll, 0.);
+  const double cion = (5.834E-11 * sqrt(te)) * exp(-(1.579E5 / te));
+  const double eion = ((de * dh0) * cion) * XION;
+  double erec = (de * dhp) * hr_betah(te);
+  erec = fmax2(erec, 0.);
+  const double tm = 1. / te;
+  const double t2 = tm * tm;
+  double eoi = (de * doi) * pow(10., ((1381465. * t2) - (12328.69 * tm)) - 19.82621);
+  double eoii = (de * doii) * pow(10., (-(2061075. * t2) - (14596.24 * tm)) - 19.01402);
+  eoi = fmax2(eoi, 0.);
+  eoii = fmax2(eoii, 0.);
+  const double betaf = (1.3 * 1.42E-27) * pow(te, 0.5);
+  const double hiicool = (de * dhp) * betaf;
+  const double equil = (((1.0455E-18 / pow(te, 0.63)) * (1. - exp(-pow(te * 1.E-5, 1.63)))) * de) * den + hiicool;
+  double fr = 0.;
+  if (te <= 44770.) fr = 0.;
+  if (te >= 54770.) fr = 1.;
+  if (te > 44770. && te < 54770.) {
+    const double ex2 = exp(-((2. * (te - 49770.)) / 500.));
+    const double th = (1. - ex2) / (1. + ex2);
+    fr = 0.5 * (1. + th);
+  }
+  return ((ecoll + eion) + ((erec + 7.033 * (eoi + eoii)) * (1. - fr))) + equil * fr;
+}
+
+/* atomic(dt, uu, 1., 1.) without diffuse radiation, cooling_h.f90:226-374:
+ * the neutral fraction's rate equation and the cooled temperature; the
+ * neutral density is the first passive scalar */
+static void hr_atomic(const orc_state *s, double dt, double *uu) {
+  const orc_params *p = &s->p;
+  const double xi = (double)1.e-4Q, boltzm = (double)1.3807e-16Q;  /* REAL(8) parameters of REAL(16) constants */
+  double prim[NEQ_MAX], tdum;
+  orc_u2prim(p, uu, prim, &tdum);
+  const double t = EOS_OWN_T(p) ? tdum : (prim[4] / prim[0]) * s->cool_tempsc;
+  const double col = hr_colf(t);
+  const double rec = hr_alpha(t);
+  const double y0 = uu[NEQ] / uu[0];
+  const double dh = uu[0];
+  const double a = rec + col;
+  const double b = -((2. + xi) * rec + (1. + xi) * col);
+  const double c = (1. + xi) * rec;
+  const double d = sqrt(b * b - (4. * a) * c);
+  const double g0 = ((((2. * a) * y0) + b) + d) / ((((2. * a) * y0) + b) - d);
+  const double e = exp(-((d * dh) * dt));
+  double y1 = ((-b) - ((d * (1. + g0 * e)) / (1. - g0 * e))) / (2. * a);
+  y1 = fmin2(y1, 0.9999);
+  y1 = fmax2(y1, 0.);
+  const double dh0 = uu[NEQ];
+  const double al = hr_aloss(y0, y1, dt, dh, dh0, t) / (dh * dh);
+  const double tprime = 10.;
+  const double ce = ((2. * dh) * al) / ((3. * boltzm) * t);
+  double t1 = tprime + (t - tprime) * exp(-(ce * dt));
+  t1 = fmax2(t1, 0.1 * t);
+  t1 = fmin2(t1, 10. * t);
+  uu[NEQ] = y1 * uu[0];
+  double e5 = (((p->cv * ((2. * uu[0]) - uu[NEQ])) * t1) / s->cool_tempsc)
+              + (0.5 * prim[0]) * ((prim[1] * prim[1] + prim[2] * prim[2]) + prim[3] * prim[3]);
+  if (p->mhd) e5 = e5 + 0.5 * ((prim[5] * prim[5] + prim[6] * prim[6]) + prim[7] * prim[7]);
+  uu[4] = e5;
+}
+
 void orc_cooling(orc_state *s, double dt) {
   const orc_params *p = &s->p;
   const double dt_seconds = dt * s->cool_tsc;
+  if (s->cool_kind == 1) {                                           /* coolingh, :41-67 */
+    for (int k = 1; k <= p->nz; k++)
+      for (int j = 1; j <= p->ny; j++)
+        for (int i = 1; i <= p->nx; i++) hr_atomic(s, dt_seconds, &AT(s->u, s, 0, i, j, k));
+    return;
+  }
   for (int k = 1; k <= p->nz; k++)
     for (int j = 1; j <= p->ny; j++)
       for (int i = 1; i <= p->nx; i++) {
@@ -792,7 +928,7 @@ void orc_cooling(orc_state *s, double dt) {
         double *pm = &AT(s->primit, s, 0, i, j, k);
         double tdum;
         orc_u2prim(p, uu, pm, &tdum);
-        const double T = p->eos == 2 ? tdum : (pm[4] / pm[0]) * s->cool_tempsc;  /* hydro_core.f90:94-101 */
+        const double T = EOS_OWN_T(p) ? tdum : (pm[4] / pm[0]) * s->cool_tempsc;  /* hydro_core.f90:94-112 */
         if (T > 10000.) {
           const double eth0 = p->cv * pm[4];
           const double aloss = cool_lambda(s, T);
@@ -816,14 +952,15 @@ int orc_fluxes(orc_state *s, int choice) {
   for (int k = 0; k <= nz; k++)
     for (int j = 0; j <= ny; j++)
       for (int i = 0; i <= nx; i++) {
-        double pl[NEQ], pr[NEQ], pll[NEQ], prr[NEQ], ff[NEQ];
+        double pl[NEQ_MAX], pr[NEQ_MAX], pll[NEQ_MAX], prr[NEQ_MAX], ff[NEQ_MAX];
+        const size_t vb = (size_t)NEQP(p) * sizeof(double);
         for (int axis = 0; axis < 3; axis++) {
           int di = axis == 0, dj = axis == 1, dk = axis == 2;
-          memcpy(pl, &AT(s->primit, s, 0, i, j, k), sizeof pl);
-          memcpy(pr, &AT(s->primit, s, 0, i + di, j + dj, k + dk), sizeof pr);
+          memcpy(pl, &AT(s->primit, s, 0, i, j, k), vb);
+          memcpy(pr, &AT(s->primit, s, 0, i + di, j + dj, k + dk), vb);
           if (choice == 2) {
-            memcpy(pll, &AT(s->primit, s, 0, i - di, j - dj, k - dk), sizeof pll);
-            memcpy(prr, &AT(s->primit, s, 0, i + 2 * di, j + 2 * dj, k + 2 * dk), sizeof prr);
+            memcpy(pll, &AT(s->primit, s, 0, i - di, j - dj, k - dk), vb);
+            memcpy(prr, &AT(s->primit, s, 0, i + 2 * di, j + 2 * dj, k + 2 * dk), vb);
           }
           if (axis == 1) {
             orc_swapy(pl); orc_swapy(pr);
@@ -837,7 +974,7 @@ int orc_fluxes(orc_state *s, int choice) {
           if (axis == 1) orc_swapy(ff);
           else if (axis == 2) orc_swapz(ff);
           double *dst = axis == 0 ? s->f : axis == 1 ? s->g : s->h;
-          memcpy(&dst[cidx(s, i, j, k) * NEQ], ff, sizeof ff);
+          memcpy(&dst[cidx(s, i, j, k) * NEQP(p)], ff, vb);
         }
       }
   return bad ? 1 : 0;
@@ -938,13 +1075,18 @@ void orc_step(orc_state *s, double dt) {
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx; i++) {
         if (!p->flux_cd) {
-          for (int q = 0; q < NEQ; q++)                              /* :105-107 */
+          for (int q = 0; q < NEQP(p); q++)                          /* :105-107 */
             AT(up, s, q, i, j, k) = ((AT(u, s, q, i, j, k)
                                       - dtdx * (AT(f, s, q, i, j, k) - AT(f, s, q, i - 1, j, k)))
                                      - dtdy * (AT(g, s, q, i, j, k) - AT(g, s, q, i, j - 1, k)))
                                     - dtdz * (AT(h, s, q, i, j, k) - AT(h, s, q, i, j, k - 1));
         } else {
           for (int q = 0; q < 5; q++)                                /* cd :299-301 */
+            AT(up, s, q, i, j, k) = ((AT(u, s, q, i, j, k)
+                                      - dtdx * (AT(f, s, q, i, j, k) - AT(f, s, q, i - 1, j, k)))
+                                     - dtdy * (AT(g, s, q, i, j, k) - AT(g, s, q, i, j - 1, k)))
+                                    - dtdz * (AT(h, s, q, i, j, k) - AT(h, s, q, i, j, k - 1));
+          for (int q = NEQ; q < NEQP(p); q++)                        /* cd :303-308 */
             AT(up, s, q, i, j, k) = ((AT(u, s, q, i, j, k)
                                       - dtdx * (AT(f, s, q, i, j, k) - AT(f, s, q, i - 1, j, k)))
                                      - dtdy * (AT(g, s, q, i, j, k) - AT(g, s, q, i, j - 1, k)))
@@ -960,9 +1102,9 @@ void orc_step(orc_state *s, double dt) {
                                   + (0.5 * dtdy) * (EAT(s, 0, i, j + 1, k) - EAT(s, 0, i, j - 1, k));
         }
         if (p->eight_wave || p->user_src) {                          /* :115-121 */
-          double sv[NEQ];
+          double sv[NEQ_MAX];
           orc_source(s, i, j, k, &AT(s->primit, s, 0, i, j, k), sv);
-          for (int q = 0; q < NEQ; q++) AT(up, s, q, i, j, k) = AT(up, s, q, i, j, k) + dt * sv[q];
+          for (int q = 0; q < NEQP(p); q++) AT(up, s, q, i, j, k) = AT(up, s, q, i, j, k) + dt * sv[q];
         }
       }
 }
@@ -1000,7 +1142,7 @@ static void user_gravity(const orc_state *s, int i, int j, int k, const double *
 /* source, sources.f90:188-220: user terms (:206), then
  * divbcorr_8w_source (:137-168); no radiation pressure */
 void orc_source(const orc_state *s, int i, int j, int k, const double *pp, double *sv) {
-  for (int q = 0; q < NEQ; q++) sv[q] = 0.;                          /* :200 */
+  for (int q = 0; q < NEQP(&s->p); q++) sv[q] = 0.;                  /* :200 */
   if (s->p.user_src) user_gravity(s, i, j, k, pp, sv);
   if (!s->p.eight_wave) return;
   const double divb = divergence_b(s, i, j, k);                      /* :149 */
@@ -1024,7 +1166,7 @@ void orc_viscous_copy(orc_state *s) {
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx; i++)
-        for (int q = 0; q < NEQ; q++)
+        for (int q = 0; q < NEQP(p); q++)
           AT(u, s, q, i, j, k) = AT(up, s, q, i, j, k)
               + eta * ((((((AT(up, s, q, i + 1, j, k) + AT(up, s, q, i - 1, j, k))
                            + AT(up, s, q, i, j + 1, k)) + AT(up, s, q, i, j - 1, k))
@@ -1045,8 +1187,8 @@ int orc_tstep(orc_state *s, double dt) {
   orc_viscous_copy(s);                                               /* :188 */
   if (s->cool_kind) orc_cooling(s, dt);                              /* :206-210 */
   orc_boundaryI(s);                                                  /* :216 */
-  if (s->cool_kind) calcprim_ghosts(s);                              /* :222-223 */
-  else orc_calcprim_u(s);                                            /* :220 */
+  if (s->cool_kind && s->cool_kind != 1) calcprim_ghosts(s);         /* :222-223 */
+  else orc_calcprim_u(s);                                            /* :218-220 */
   if (s->tc_kind) orc_thermal(s, dt);                                /* :227 */
   return bad;
 }
@@ -1060,7 +1202,7 @@ double orc_cfl_dtp(const orc_state *s) {
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx; i++) {
-        const double *pm = &s->primit[cidx(s, i, j, k) * NEQ];
+        const double *pm = &AT(s->primit, s, 0, i, j, k);
         if (p->mhd) {
           double cx, cy, cz;
           orc_cfast(p, pm[4], pm[0], pm[5], pm[6], pm[7], &cx, &cy, &cz);
@@ -1160,10 +1302,11 @@ static void st_steps(double fs, int *ns, double *fstep) {
 }
 
 /* Temp of calcprim from the primitives: (p / rho) Tempsc; under
- * EOS_SINGLE_SPECIE u2prim's floored T, already in temp */
+ * EOS_SINGLE_SPECIE and EOS_H_RATE u2prim's own T, already in temp */
 static void tc_temp(orc_state *s) {
-  if (s->p.eos == 2) return;
-  for (long c = 0; c < s->ncell; c++) s->temp[c] = (s->primit[c * NEQ + 4] / s->primit[c * NEQ]) * s->tc_tempsc;
+  if (EOS_OWN_T(&s->p)) return;
+  const int nq = NEQP(&s->p);
+  for (long c = 0; c < s->ncell; c++) s->temp[c] = (s->primit[c * nq + 4] / s->primit[c * nq]) * s->tc_tempsc;
 }
 
 /* primitives of every cell and Temp (calcprim) */
@@ -1184,7 +1327,7 @@ double orc_dt_cond(orc_state *s) {
   for (int k = 1; k <= p->nz; k++)
     for (int j = 1; j <= p->ny; j++)
       for (int i = 1; i <= p->nx; i++)
-        dtp = fmin2(dtp, PRIM(0, i, j, k) / ksp(p->eos == 2 ? TEMP(i, j, k)
+        dtp = fmin2(dtp, PRIM(0, i, j, k) / ksp(EOS_OWN_T(p) ? TEMP(i, j, k)
                                                  : (PRIM(4, i, j, k) / PRIM(0, i, j, k)) * s->tc_tempsc));
   const double x = ddx * s->tc_rsc;
   return ((((((0.25 * 0.5) * (x * x)) * p->cv) * 8.3145e7) * dtp) * s->tc_rhosc) / s->tc_mu;

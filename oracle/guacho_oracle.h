@@ -42,8 +42,9 @@ typedef struct orc_params {
   int user_src;           /* the checker's user source (oracle/ref/user_mod.f90) */
   int user_bc;            /* bc_user: the checker's impose_user_bc after boundaryI/II */
   int nthreads;           /* OpenMP threads for the sweeps (<=0: runtime default) */
-  int eos;                /* eq_of_state: 2 EOS_SINGLE_SPECIE, else adiabatic */
-  double tempsc;          /* Tempsc of the EOS_SINGLE_SPECIE temperature floor */
+  int eos;                /* eq_of_state: 2 EOS_SINGLE_SPECIE, 3 EOS_H_RATE, else adiabatic */
+  double tempsc;          /* Tempsc of the EOS_SINGLE_SPECIE / EOS_H_RATE temperature */
+  int npas;               /* passive scalars after the 8 MHD variables (0..2) */
 } orc_params;
 
 typedef struct orc_state orc_state;

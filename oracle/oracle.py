@@ -43,7 +43,7 @@ class OrcParams(C.Structure):
         ("solver", C.c_int), ("limiter", C.c_int), ("bc", C.c_int * 6),
         ("mhd", C.c_int), ("pmhd", C.c_int), ("flux_cd", C.c_int),
         ("eight_wave", C.c_int), ("user_src", C.c_int), ("user_bc", C.c_int), ("nthreads", C.c_int),
-        ("eos", C.c_int), ("tempsc", C.c_double),
+        ("eos", C.c_int), ("tempsc", C.c_double), ("npas", C.c_int),
     ]
 
 
@@ -68,8 +68,13 @@ class Config:
     user_src: bool = False          # the checker's user gravity (oracle/ref/user_mod.f90)
     user_bc: bool = False           # the checker's impose_user_bc (oracle/ref/user_mod.f90)
     cfl: float = 0.4
-    eq_of_state: int = 1            # 2: EOS_SINGLE_SPECIE
-    tempsc: float = 1.0             # Tempsc of the EOS_SINGLE_SPECIE floor
+    eq_of_state: int = 1            # 2: EOS_SINGLE_SPECIE, 3: EOS_H_RATE
+    tempsc: float = 1.0             # Tempsc of the EOS_SINGLE_SPECIE / EOS_H_RATE temperature
+    npas: int = 0                   # passive scalars after the 8 MHD variables
+
+    @property
+    def neq(self) -> int:
+        return NEQ + self.npas
 
     @property
     def gamma(self) -> float:
@@ -90,10 +95,11 @@ class Config:
         p.nthreads = nthreads
         p.eos = int(self.eq_of_state)
         p.tempsc = float(self.tempsc)
+        p.npas = int(self.npas)
         return p
 
     def shape(self):
-        return (NEQ, self.nx + 4, self.ny + 4, self.nz + 4)
+        return (self.neq, self.nx + 4, self.ny + 4, self.nz + 4)
 
 
 def build_oracle() -> None:
@@ -190,9 +196,9 @@ class Oracle:
         except Exception:
             pass
 
-    def _new(self, nvar=NEQ):
+    def _new(self, nvar=None):
         sh = self.cfg.shape()
-        return np.zeros((nvar,) + sh[1:], dtype=np.float64, order="F")
+        return np.zeros((nvar or sh[0],) + sh[1:], dtype=np.float64, order="F")
 
     def set_u(self, u):
         self.L.orc_set_u(self.h, np.asfortranarray(u, dtype=np.float64))
@@ -274,7 +280,7 @@ class Oracle:
 
     def source(self, i: int, j: int, k: int, prim) -> np.ndarray:
         """source(i,j,k,prim,s) with the 8-wave term, from the state's primit."""
-        sv = np.zeros(NEQ)
+        sv = np.zeros(self.cfg.neq)
         self.L.orc_source(self.h, i, j, k, np.ascontiguousarray(prim, dtype=np.float64), sv)
         return sv
 
@@ -286,17 +292,18 @@ class OracleFunctions:
     def __init__(self, cfg: Config):
         self.L = orc_lib()
         self.p = cfg.to_orc()
+        self.neq = cfg.neq
 
     def u2prim(self, uu):
-        prim = np.zeros(NEQ); t = C.c_double()
+        prim = np.zeros(self.neq); t = C.c_double()
         self.L.orc_u2prim(C.byref(self.p), np.ascontiguousarray(uu, dtype=np.float64), prim, C.byref(t))
         return prim, t.value
 
     def prim2u(self, prim):
-        out = np.zeros(NEQ); self.L.orc_prim2u(C.byref(self.p), np.ascontiguousarray(prim), out); return out
+        out = np.zeros(self.neq); self.L.orc_prim2u(C.byref(self.p), np.ascontiguousarray(prim), out); return out
 
     def prim2f(self, prim):
-        out = np.zeros(NEQ); self.L.orc_prim2f(C.byref(self.p), np.ascontiguousarray(prim), out); return out
+        out = np.zeros(self.neq); self.L.orc_prim2f(C.byref(self.p), np.ascontiguousarray(prim), out); return out
 
     def cfast(self, p, d, bx, by, bz):
         a, b, c = C.c_double(), C.c_double(), C.c_double()
@@ -331,12 +338,12 @@ class OracleFunctions:
     def source(self, i, j, k, prim):
         """source(i,j,k,prim,s) of this build (sources.f90:188-220), from the
         primit array last set."""
-        sv = np.zeros(NEQ)
+        sv = np.zeros(self.neq)
         self.L.ref_source(i, j, k, np.ascontiguousarray(prim, dtype=np.float64), sv)
         return sv
 
     def riemann(self, pl, pr):
-        ff = np.zeros(NEQ)
+        ff = np.zeros(self.neq)
         rc = self.L.orc_riemann(C.byref(self.p), np.ascontiguousarray(pl), np.ascontiguousarray(pr), ff)
         return ff, rc
 
@@ -368,6 +375,7 @@ class RefLib:
         L.ref_config(cfg, r)
         c = list(cfg)
         self.nx, self.ny, self.nz, self.neq = c[0:4]
+        self.npas = self.neq - NEQ
         self.riemann_solver, self.slope_limiter = c[4], c[5]
         self.bc = c[6:12]
         self.mhd, self.pmhd = bool(c[12]), bool(c[13])
@@ -415,13 +423,13 @@ class RefLib:
                       riemann_solver=self.riemann_solver, slope_limiter=self.slope_limiter, bc=list(self.bc),
                       mhd=self.mhd, pmhd=self.pmhd, enable_flux_cd=True, cfl=self.cfl,
                       eight_wave=self.eight_wave, user_src=self.user_source_terms, user_bc=self.bc_user,
-                      eq_of_state=self.eq_of_state, tempsc=self.tempsc)
+                      eq_of_state=self.eq_of_state, tempsc=self.tempsc, npas=self.npas)
 
     def init(self, dx, dy, dz):
         self.L.ref_init(dx, dy, dz)
 
     def shape(self):
-        return (NEQ, self.nx + 4, self.ny + 4, self.nz + 4)
+        return (self.neq, self.nx + 4, self.ny + 4, self.nz + 4)
 
     def set_array(self, which, a):
         self.L.ref_set_array(which, np.asfortranarray(a, dtype=np.float64))
@@ -443,15 +451,15 @@ class RefLib:
         return dt.value, bool(d.value)
 
     def u2prim(self, uu):
-        prim = np.zeros(NEQ); t = C.c_double()
+        prim = np.zeros(self.neq); t = C.c_double()
         self.L.ref_u2prim(np.ascontiguousarray(uu, dtype=np.float64), prim, C.byref(t))
         return prim, t.value
 
     def prim2u(self, prim):
-        out = np.zeros(NEQ); self.L.ref_prim2u(np.ascontiguousarray(prim), out); return out
+        out = np.zeros(self.neq); self.L.ref_prim2u(np.ascontiguousarray(prim), out); return out
 
     def prim2f(self, prim):
-        out = np.zeros(NEQ); self.L.ref_prim2f(np.ascontiguousarray(prim), out); return out
+        out = np.zeros(self.neq); self.L.ref_prim2f(np.ascontiguousarray(prim), out); return out
 
     def cfast(self, p, d, bx, by, bz):
         out = np.zeros(3); self.L.ref_cfast(p, d, bx, by, bz, out); return out
@@ -487,11 +495,11 @@ class RefLib:
     def source(self, i, j, k, prim):
         """source(i,j,k,prim,s) of this build (sources.f90:188-220), from the
         primit array last set."""
-        sv = np.zeros(NEQ)
+        sv = np.zeros(self.neq)
         self.L.ref_source(i, j, k, np.ascontiguousarray(prim, dtype=np.float64), sv)
         return sv
 
     def riemann(self, pl, pr):
-        ff = np.zeros(NEQ)
+        ff = np.zeros(self.neq)
         self.L.ref_riemann(np.ascontiguousarray(pl), np.ascontiguousarray(pr), ff)
         return ff

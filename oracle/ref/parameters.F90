@@ -41,7 +41,7 @@ module parameters
   logical, parameter :: user_source_terms = (REF_USER_SRC /= 0)
   logical, parameter :: radiation_pressure = .false.
 
-  integer, parameter :: npas = 0
+  integer, parameter :: npas = REF_NPAS
   integer, parameter :: nxtot = REF_NX, nytot = REF_NY, nztot = REF_NZ
 
   real, parameter :: xmax = 1.0, ymax = 1.0, zmax = 1.0
@@ -64,7 +64,7 @@ module parameters
   logical, parameter :: iwarm = .false.
   integer :: itprint0 = 0
 
-  logical, parameter :: passives = .false.
+  logical, parameter :: passives = (REF_NPAS > 0)
   integer, parameter :: ndim = 3
   integer, parameter :: nghost = 2
   integer, parameter :: neqdyn = 8

@@ -23,6 +23,9 @@ module ref_kat
   use sources, only : source
   use cooling_dmc, only : init_cooling_dmc, coolingdmc, cooltab_dmc
   use cooling_chi, only : init_cooling_chianti, coolingchi, cooltab_chianti
+#ifdef PASSIVES
+  use cooling_H, only : coolingh
+#endif
   use thermal_cond, only : thermal_conduction, get_dt_cond
   implicit none
 
@@ -214,7 +217,11 @@ contains
     integer(c_int), value :: kind
     real(c_double), value :: dt
     dt_CFL = dt
-    if (kind == COOL_DMC) then
+    if (kind == COOL_H) then
+#ifdef PASSIVES
+      call coolingh()
+#endif
+    else if (kind == COOL_DMC) then
       if (.not. allocated(cooltab_dmc)) call init_cooling_dmc()
       call coolingdmc()
     else

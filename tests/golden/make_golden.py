@@ -10,6 +10,8 @@ inputs and the reference's outputs for
   - get_timestep's ramp and clamp (the boundary fills include the
     checker's impose_user_bc in bc_user builds, at globals::time = 0.3),
   - (ref_cooling.npz) coolingdmc and coolingchi with the reference's tables,
+  - (ref_cooling_h.npz) coolingh, the H-rate cooling with the neutral
+    density as the first passive scalar, under EOS_H_RATE,
   - thermal_conduction (isotropic / anisotropic, with and without
     saturation) on a static state with a random temperature field,
   - source(i,j,k,prim,s) of sources.f90 at every interior cell of a random
@@ -41,12 +43,15 @@ def seed_of(variant):
     return SEEDS.get(variant, 2000 + zlib.crc32(variant.encode()) % 100000)
 
 
-def random_prims(rng, n, pmhd=False):
-    p = np.empty((n, 8))
+def random_prims(rng, n, npas=0):
+    p = np.empty((n, 8 + npas))
     p[:, 0] = rng.uniform(0.1, 2.0, n)
     p[:, 1:4] = rng.uniform(-3.0, 3.0, (n, 3))
     p[:, 4] = rng.uniform(0.02, 2.0, n)
     p[:, 5:8] = rng.uniform(-2.0, 2.0, (n, 3))
+    if npas:
+        # passive densities below rho (a neutral density under EOS_H_RATE)
+        p[:, 8:] = rng.uniform(0.05, 0.95, (n, npas)) * p[:, :1]
     return p
 
 
@@ -63,6 +68,8 @@ def random_state(rng, shape, cv):
     u[1:4] = rho * v
     u[4] = 0.5 * rho * (v ** 2).sum(0) + cv * p + 0.5 * (B ** 2).sum(0)
     u[5:8] = B
+    if shape[0] > 8:
+        u[8:] = rng.uniform(0.05, 0.95, (shape[0] - 8, NX, NY, NZ)) * rho
     return u
 
 
@@ -70,8 +77,8 @@ def make(variant, seed):
     R = RefLib(variant)
     rng = np.random.default_rng(seed)
     out = {}
-    pl = random_prims(rng, NVEC)
-    pr = random_prims(rng, NVEC)
+    pl = random_prims(rng, NVEC, R.npas)
+    pr = random_prims(rng, NVEC, R.npas)
     pr[::7, 5] = pl[::7, 5]            # continuous normal field
     pl[::11, 5] = 0.0
     pr[::11, 5] = 0.0                  # Bx = 0: single-star regions only
@@ -83,7 +90,7 @@ def make(variant, seed):
     pll[::5] = pl[::5]                 # flat left slope
     out["lim_in"] = np.stack([pll, pl, pr, prr])
     out["lim_out"] = np.array([np.concatenate(R.limiter(a, b, c, d)) for a, b, c, d in zip(pll, pl, pr, prr)])
-    uu = np.array([np.concatenate([[a[0]], a[0] * a[1:4], [a[4] * 2.0], a[5:8]]) for a in pl])
+    uu = np.array([np.concatenate([[a[0]], a[0] * a[1:4], [a[4] * 2.0], a[5:8], a[8:]]) for a in pl])
     uu[::17, 0] = 1e-20                # density floor
     uu[::19, 4] = -1.0                 # pressure floor
     out["u2p_in"] = uu
@@ -161,6 +168,8 @@ def make(variant, seed):
     out["switches"] = np.array([int(R.eight_wave), int(R.user_source_terms), int(R.bc_user)])
     if R.eq_of_state != 1:
         out["eos_config"] = np.array([R.eq_of_state, R.tempsc])
+    if R.npas:
+        out["npas"] = np.array([R.npas])
     out["hook_time"] = np.array([hook_time])
     out["dxyz"] = np.array([dx, dy, dz])
     cfg = np.array([R.nx, R.ny, R.nz, R.riemann_solver, R.slope_limiter, *R.bc, int(R.mhd), int(R.pmhd)])
@@ -200,6 +209,37 @@ def make_cooling(seed=77):
     return out
 
 
+def make_cooling_h(seed=78):
+    """coolingh (cooling_h.f90:41-67: atomic and ALOSS per cell) of the
+    reference under EOS_H_RATE with two passive scalars, the first the
+    neutral density: temperatures 5e3..2e5 K span ALOSS's no-loss, blended
+    and high-temperature branches; dt = 1e10 s (tsc = 1) so the neutral
+    fraction and the temperature move."""
+    R = RefLib("hlld_minmod_pas2_hrate")
+    dx, dy, dz = 0.1, 0.07, 0.13
+    R.init(dx, dy, dz)
+    rng = np.random.default_rng(seed)
+    sh = R.shape()
+    rho = rng.uniform(0.5, 2.0, sh[1:])
+    v = rng.uniform(-1.0, 1.0, (3,) + sh[1:])
+    B = rng.uniform(-1.0, 1.0, (3,) + sh[1:])
+    y0 = rng.uniform(0.02, 0.98, sh[1:])
+    t = 10.0 ** rng.uniform(np.log10(5e3), np.log10(2e5), sh[1:])
+    u = np.zeros(sh, order="F")
+    u[0] = rho
+    u[1:4] = rho * v
+    u[8] = y0 * rho
+    u[9] = rng.uniform(0.0, 1.0, sh[1:]) * rho
+    pr = (2.0 * rho - u[8]) * t / R.tempsc
+    u[4] = 0.5 * rho * (v ** 2).sum(0) + R.cv * pr + 0.5 * (B ** 2).sum(0)
+    u[5:8] = B
+    R.set_array(0, u)
+    R.cooling(1, 1e10)
+    return {"u_in": u, "u_out": R.get_array(0), "dt": np.array([1e10]), "tempsc": np.array([R.tempsc]),
+            "tsc": np.array([1.0]), "dxyz": np.array([dx, dy, dz]),
+            "config_real": np.array([R.cv, R.gamma, R.cfl])}
+
+
 def main():
     vs = ref_variants()
     if not vs:
@@ -215,6 +255,10 @@ def main():
     if rewrite or not os.path.exists(path):
         np.savez_compressed(path, **make_cooling())
         print("wrote cooling")
+    path = os.path.join(HERE, "ref_cooling_h.npz")
+    if rewrite or not os.path.exists(path):
+        np.savez_compressed(path, **make_cooling_h())
+        print("wrote cooling_h")
 
 
 if __name__ == "__main__":
