@@ -18,7 +18,8 @@ GM_OK, GM_ERR_CONFIG, GM_ERR_HIP, GM_ERR_NONFINITE, GM_ERR_OOM, GM_ERR_ARG, GM_E
 GM_PRECISION_FP64, GM_PRECISION_MIXED = 0, 1
 GM_COOL_NONE, GM_COOL_DMC, GM_COOL_CHI = 0, 3, 4
 GM_TC_OFF, GM_TC_ISOTROPIC, GM_TC_ANISOTROPIC = 0, 1, 2
-GM_EOS_ADIABATIC, GM_EOS_SINGLE_SPECIE = 1, 2
+GM_COOL_H = 1
+GM_EOS_ADIABATIC, GM_EOS_SINGLE_SPECIE, GM_EOS_H_RATE = 1, 2, 3
 GM_NKCLASS = 12
 
 # exported entry points, in the order include/guacho_amd.h declares them
@@ -132,10 +133,11 @@ def make_params(nx: int, ny: int, nz: int, dx: float, dy: float, dz: float, *, c
                 gamma: Optional[float] = None, riemann_solver: int = 4, slope_limiter: int = 1,
                 bc: Sequence[int] = (3, 3, 3, 3, 3, 3), mhd: bool = True, pmhd: bool = False,
                 enable_flux_cd: bool = True, eight_wave: bool = False,
-                precision: int = GM_PRECISION_FP64, device: int = 0, eta: float = 0.0) -> GmParams:
+                precision: int = GM_PRECISION_FP64, device: int = 0, eta: float = 0.0,
+                npas: int = 0) -> GmParams:
     p = GmParams()
     p.nx, p.ny, p.nz = nx, ny, nz
-    p.neq, p.nghost = 8, 2
+    p.neq, p.nghost = 8 + npas, 2
     p.riemann_solver, p.slope_limiter = riemann_solver, slope_limiter
     for b in range(6):
         p.bc[b] = int(bc[b])
@@ -160,7 +162,7 @@ class Subdomain:
         if rc != GM_OK:
             raise GmError(rc, self.L.gm_last_error(None).decode())
         self.h = h
-        self.shape = (8, params.nx + 4, params.ny + 4, params.nz + 4)
+        self.shape = (params.neq, params.nx + 4, params.ny + 4, params.nz + 4)
 
     def _check(self, rc: int):
         if rc != GM_OK:
@@ -242,10 +244,10 @@ class Subdomain:
         self._check(self.L.gm_set_user_bc(self.h, co, kn))
 
     def set_cooling(self, kind: int, cooltab=None, tempsc: float = 1.0, psc: float = 1.0, tsc: float = 1.0) -> None:
-        """Table cooling after each step (GM_COOL_DMC / GM_COOL_CHI; 0 off).
-        cooltab: the reference's cooltab(2, 41)."""
-        if kind == GM_COOL_NONE:
-            self._check(self.L.gm_set_cooling(self.h, 0, None, tempsc, psc, tsc))
+        """Cooling after each step (GM_COOL_DMC / GM_COOL_CHI with cooltab, the
+        reference's cooltab(2, 41); GM_COOL_H without a table; 0 off)."""
+        if kind in (GM_COOL_NONE, GM_COOL_H):
+            self._check(self.L.gm_set_cooling(self.h, int(kind), None, tempsc, psc, tsc))
             return
         tab = np.asfortranarray(cooltab, dtype=np.float64)
         if tab.shape != (2, 41):
@@ -334,7 +336,7 @@ class Batch:
     def add(self, params: GmParams, u: np.ndarray, cfl: float = 0.4, t_end: float = float("inf"),
             max_steps: int = 0) -> int:
         u = np.asfortranarray(u, dtype=np.float64)
-        shape = (8, params.nx + 4, params.ny + 4, params.nz + 4)
+        shape = (params.neq, params.nx + 4, params.ny + 4, params.nz + 4)
         if u.shape != shape:
             raise ValueError(f"u has shape {u.shape}, expected {shape}")
         i = C.c_int32()

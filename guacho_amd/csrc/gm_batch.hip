@@ -107,6 +107,7 @@ gm::Geom geom_of(const gm_params& p) {
   g.ncell = (long long)g.NX * g.NY * g.NZ;
   for (int k = 0; k < 6; k++) g.bc[k] = p.bc[k];
   g.gmem = 0;
+  g.npas = 0;
   return g;
 }
 
@@ -411,6 +412,7 @@ int gm_batch_add(gm_batch* b, const gm_params* p, const double* u, double cfl, d
   for (int f = 0; f < 6; f++)
     if (p->bc[f] == GM_BC_OTHER)
       return bfail(b, GM_ERR_CONFIG, "BC_OTHER (user boundary hooks) is for single-box contexts (gm_set_user_bc)");
+  if (p->neq != 8) return bfail(b, GM_ERR_CONFIG, "passive scalars (neq > 8) are for single-box contexts");
   if (p->eta != 0.0) return bfail(b, GM_ERR_CONFIG, "artificial viscosity (eta != 0) is for single-box contexts");
   if (p->precision != b->bp.precision || p->riemann_solver != b->bp.riemann_solver ||
       (p->enable_flux_cd != 0) != (b->bp.enable_flux_cd != 0))

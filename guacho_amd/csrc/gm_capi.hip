@@ -55,9 +55,9 @@ struct gm_ctx {
   gm::Phys ph;
   hipStream_t stream = nullptr;
   size_t esize = 8;      // bytes per stored value: 8 (FP64) or 4 (mixed, FP32 storage)
-  void* U = nullptr;     // state U^n (SoA, 8 vars, ghosts as uploaded / filled)
+  void* U = nullptr;     // state U^n (SoA, neq vars, ghosts as uploaded / filled)
   void* Us = nullptr;    // U* after the predictor
-  void* W = nullptr;     // stage partials (8 doubles per cell); also the staging
+  void* W = nullptr;     // stage partials (neq doubles per cell); also the staging
                          // buffer for layout conversion
   void* E = nullptr;     // cell-centred E (3 vars); also staging for Temp / divB
   Scalars* dscal = nullptr;  // device scalars
@@ -147,7 +147,8 @@ int check_params(const gm_params* p, std::string& msg) {
     return code;
   };
   if (!p) return fail(nullptr, GM_ERR_ARG, "params is NULL");
-  if (p->neq != 8) return fail(nullptr, GM_ERR_CONFIG, "neq must be 8 (BFIELD, no passive scalars)");
+  if (p->neq < 8 || p->neq > 10)
+    return fail(nullptr, GM_ERR_CONFIG, "neq must be 8 (BFIELD) plus 0, 1 or 2 passive scalars");
   if (p->nghost != 2) return fail(nullptr, GM_ERR_CONFIG, "nghost must be 2");
   if (p->nx < 2 || p->ny < 2 || p->nz < 2)
     return fail(nullptr, GM_ERR_CONFIG, "nx, ny, nz must each be at least 2 (two-layer ghost mirror)");
@@ -224,6 +225,7 @@ gm_user_args user_args(const gm_ctx* c, void* u, double* sv, double dt, int stag
   a.u = u;
   a.s = sv;
   a.eos = c->ph.eos;
+  a.npas = c->g.npas;
   a.tempsc = c->ph.tempsc;
   return a;
 }
@@ -286,7 +288,7 @@ gm::Box<S> box_of(const gm_ctx* c) {
 // zeroed s (the W block, free once the stage's z kernel has run).
 template <typename S>
 void user_sources_t(gm_ctx* c, const gm::Box<S>& b, double dt, int stage) {
-  (void)hipMemsetAsync(b.W, 0, (size_t)c->g.ncell * 8 * sizeof(double), c->stream);
+  (void)hipMemsetAsync(b.W, 0, (size_t)c->g.ncell * c->p.neq * sizeof(double), c->stream);
   void* u = stage == 1 ? (void*)b.U : (void*)b.Us;
   launch_plugin(c, c->usrc_fn, user_args(c, u, b.W, dt, stage), (long long)c->g.nx * c->g.ny * c->g.nz);
 }
@@ -436,6 +438,7 @@ int gm_create(const gm_params* params, gm_ctx** out) {
   g.ncell = (long long)g.NX * g.NY * g.NZ;
   for (int b = 0; b < 6; b++) g.bc[b] = params->bc[b];
   g.gmem = 0;
+  g.npas = params->neq - 8;
   c->ph.cv = params->cv;
   c->ph.gamma = params->gamma;
   c->ph.mhd = params->mhd ? 1 : 0;
@@ -453,9 +456,10 @@ int gm_create(const gm_params* params, gm_ctx** out) {
   if ((rc = hip_check(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate")) != GM_OK)
     return bail(rc);
   c->esize = params->precision == GM_PRECISION_MIXED ? 4 : 8;
-  const size_t n8 = (size_t)g.ncell * 8 * c->esize;
+  const size_t nq = (size_t)params->neq;
+  const size_t n8 = (size_t)g.ncell * nq * c->esize;
   const size_t n3 = (size_t)g.ncell * 3 * c->esize;
-  const size_t nw = (size_t)g.ncell * 8 * sizeof(double);
+  const size_t nw = (size_t)g.ncell * nq * sizeof(double);
   struct { void** p; size_t n; } allocs[] = {{&c->U, n8}, {&c->Us, n8}, {&c->W, nw}, {&c->E, n3}};
   for (auto& a : allocs) {
     if (hipMalloc(a.p, a.n) != hipSuccess) {
@@ -491,11 +495,12 @@ int gm_upload_u(gm_ctx* c, const double* u) {
   if (!u) return fail(c, GM_ERR_ARG, "u is NULL");
   int rc = set_device(c);
   if (rc) return rc;
-  const size_t n8 = (size_t)c->g.ncell * 8 * sizeof(double);
+  const int nq = c->p.neq;
+  const size_t n8 = (size_t)c->g.ncell * nq * sizeof(double);
   GM_HIP(c, hipMemcpyAsync(c->W, u, n8, hipMemcpyHostToDevice, c->stream));
   timed(c, KC_LAYOUT, [&] {
-    if (c->esize == 8) gm::launch_aos_to_soa(c->stream, c->g.ncell, (const double*)c->W, (double*)c->U);
-    else gm::launch_aos_to_soa(c->stream, c->g.ncell, (const double*)c->W, (float*)c->U);
+    if (c->esize == 8) gm::launch_aos_to_soa(c->stream, c->g.ncell, (const double*)c->W, (double*)c->U, nq);
+    else gm::launch_aos_to_soa(c->stream, c->g.ncell, (const double*)c->W, (float*)c->U, nq);
   });
   GM_HIP(c, hipGetLastError());
   GM_HIP(c, hipStreamSynchronize(c->stream));
@@ -518,12 +523,13 @@ int gm_download_u(gm_ctx* c, double* u) {
   int rc = set_device(c);
   if (rc) return rc;
   if (c->ghosts_dirty) fill_ghost1(c);
+  const int nq = c->p.neq;
   timed(c, KC_LAYOUT, [&] {
-    if (c->esize == 8) gm::launch_soa_to_aos(c->stream, c->g.ncell, (const double*)c->U, (double*)c->W);
-    else gm::launch_soa_to_aos(c->stream, c->g.ncell, (const float*)c->U, (double*)c->W);
+    if (c->esize == 8) gm::launch_soa_to_aos(c->stream, c->g.ncell, (const double*)c->U, (double*)c->W, nq);
+    else gm::launch_soa_to_aos(c->stream, c->g.ncell, (const float*)c->U, (double*)c->W, nq);
   });
   GM_HIP(c, hipGetLastError());
-  GM_HIP(c, hipMemcpyAsync(u, c->W, (size_t)c->g.ncell * 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  GM_HIP(c, hipMemcpyAsync(u, c->W, (size_t)c->g.ncell * nq * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   GM_HIP(c, hipStreamSynchronize(c->stream));
   return GM_OK;
 }
@@ -535,12 +541,13 @@ int gm_download_primit(gm_ctx* c, double* primit, double* temp) {
   if (rc) return rc;
   if (c->ghosts_dirty) fill_ghost1(c);
   double* T = temp ? (double*)c->E : nullptr;
+  const int nq = c->p.neq;
   timed(c, KC_LAYOUT, [&] {
-    if (c->esize == 8) gm::launch_prim_all(c->stream, c->g.ncell, c->ph, (const double*)c->U, (double*)c->W, T);
-    else gm::launch_prim_all(c->stream, c->g.ncell, c->ph, (const float*)c->U, (double*)c->W, T);
+    if (c->esize == 8) gm::launch_prim_all(c->stream, c->g.ncell, c->ph, (const double*)c->U, (double*)c->W, T, nq);
+    else gm::launch_prim_all(c->stream, c->g.ncell, c->ph, (const float*)c->U, (double*)c->W, T, nq);
   });
   GM_HIP(c, hipGetLastError());
-  GM_HIP(c, hipMemcpyAsync(primit, c->W, (size_t)c->g.ncell * 8 * sizeof(double), hipMemcpyDeviceToHost,
+  GM_HIP(c, hipMemcpyAsync(primit, c->W, (size_t)c->g.ncell * nq * sizeof(double), hipMemcpyDeviceToHost,
                            c->stream));
   if (temp)
     GM_HIP(c, hipMemcpyAsync(temp, c->E, (size_t)c->g.ncell * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -703,21 +710,24 @@ int gm_set_cooling(gm_ctx* c, int32_t kind, const double* cooltab, double tempsc
     c->cool.kind = 0;
     return GM_OK;
   }
-  if (kind != GM_COOL_DMC && kind != GM_COOL_CHI)
-    return fail(c, GM_ERR_CONFIG, "cooling kind must be GM_COOL_NONE, GM_COOL_DMC or GM_COOL_CHI");
-  if (!cooltab) return fail(c, GM_ERR_ARG, "cooltab is NULL");
+  if (kind != GM_COOL_DMC && kind != GM_COOL_CHI && kind != GM_COOL_H)
+    return fail(c, GM_ERR_CONFIG, "cooling kind must be GM_COOL_NONE, GM_COOL_H, GM_COOL_DMC or GM_COOL_CHI");
+  if (kind == GM_COOL_H && c->g.npas < 1)
+    return fail(c, GM_ERR_CONFIG, "GM_COOL_H needs the neutral density as a passive scalar (neq >= 9)");
+  if (!cooltab && kind != GM_COOL_H) return fail(c, GM_ERR_ARG, "cooltab is NULL");
   if (!(tempsc > 0) || !(psc > 0) || !(tsc > 0) || !std::isfinite(tempsc) || !std::isfinite(psc) ||
       !std::isfinite(tsc))
     return fail(c, GM_ERR_CONFIG, "tempsc, psc and tsc must be positive and finite");
-  for (int i = 0; i < 82; i++)
-    if (!std::isfinite(cooltab[i])) return fail(c, GM_ERR_CONFIG, "cooltab holds a non-finite value");
+  if (cooltab)
+    for (int i = 0; i < 82; i++)
+      if (!std::isfinite(cooltab[i])) return fail(c, GM_ERR_CONFIG, "cooltab holds a non-finite value");
   gm::Cooling cl{};
   cl.kind = kind;
   cl.tempsc = tempsc;
   cl.psc = psc;
   cl.tsc = tsc;
   cl.hicoef = kind == GM_COOL_DMC ? 0.27e-26 : 0.21e-26;  // cooling_dmc.f90:88, cooling_chi.f90
-  std::memcpy(cl.tab, cooltab, sizeof cl.tab);
+  if (cooltab) std::memcpy(cl.tab, cooltab, sizeof cl.tab);
   c->cool = cl;
   return GM_OK;
 }
@@ -750,9 +760,11 @@ int gm_set_conduction(gm_ctx* c, const gm_conduction_params* tp) {
 
 int gm_set_eos(gm_ctx* c, int32_t eos, double tempsc) {
   if (!c) return GM_ERR_ARG;
-  if (eos != GM_EOS_ADIABATIC && eos != GM_EOS_SINGLE_SPECIE)
-    return fail(c, GM_ERR_CONFIG, "eq_of_state must be GM_EOS_ADIABATIC or GM_EOS_SINGLE_SPECIE "
-                                  "(the others need passive scalars)");
+  if (eos != GM_EOS_ADIABATIC && eos != GM_EOS_SINGLE_SPECIE && eos != GM_EOS_H_RATE)
+    return fail(c, GM_ERR_CONFIG, "eq_of_state must be GM_EOS_ADIABATIC, GM_EOS_SINGLE_SPECIE or GM_EOS_H_RATE "
+                                  "(EOS_CHEM needs the chemistry network)");
+  if (eos == GM_EOS_H_RATE && c->g.npas < 1)
+    return fail(c, GM_ERR_CONFIG, "GM_EOS_H_RATE needs the neutral density as a passive scalar (neq >= 9)");
   if (!(tempsc > 0) || !std::isfinite(tempsc)) return fail(c, GM_ERR_CONFIG, "Tempsc must be positive and finite");
   c->ph.eos = eos;
   c->ph.tempsc = tempsc;

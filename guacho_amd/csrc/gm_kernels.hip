@@ -80,18 +80,18 @@ __device__ __forceinline__ int ghost_src(const Geom& g, int i, int n, int lo, in
   return map_ghost(i, n, lo, hi, flip);
 }
 
-// Conserved 8-vector of cell (i,j,k) in FP64, ghosts resolved through the
-// boundary conditions.
-template <typename S>
+// Conserved variables (NQ: the 8 MHD ones and the passive scalars) of
+// cell (i,j,k) in FP64, ghosts resolved through the boundary conditions.
+template <int NQ = 8, typename S>
 __device__ __forceinline__ void load_cons(const Geom& g, const S* __restrict__ U, int i, int j, int k,
-                                          double u[8]) {
+                                          double u[NQ]) {
   bool fx = false, fy = false, fz = false;
   const int si = ghost_src(g, i, g.nx, g.bc[0], g.bc[1], fx);
   const int sj = ghost_src(g, j, g.ny, g.bc[2], g.bc[3], fy);
   const int sk = ghost_src(g, k, g.nz, g.bc[4], g.bc[5], fz);
   const long long c = cidx(g, si, sj, sk);
 #pragma unroll
-  for (int q = 0; q < 8; q++) u[q] = (double)U[q * g.ncell + c];
+  for (int q = 0; q < NQ; q++) u[q] = (double)U[q * g.ncell + c];
   if (fx) u[1] = -u[1];
   if (fy) u[2] = -u[2];
   if (fz) u[3] = -u[3];
@@ -136,23 +136,23 @@ static_assert(TX == 32 && XY_THREADS == 512, "two tile rows per 64-lane wavefron
 // (TX, r) [8][TY] each; the lower y state of the halo cell (c, TY) [8][TX];
 // and the upper y states of rows -1, 1, 3, .., TY-1 [8][TY/2+1][TX] (the
 // rows an even row, held in the other wavefront, reads below itself).
-template <int ORDER>
+template <int ORDER, int NQ = 8>
 struct XYLds {
   static constexpr int H = ORDER;
   static constexpr int RX = TX + 2 * H, RY = TY + 2 * H;
   static constexpr int PS = RX * RY;  // component stride of the primitives
-  static constexpr int PRIM = 8 * PS;
+  static constexpr int PRIM = NQ * PS;
   static constexpr int NLOAD = 2 * H * TX + TY * RX;  // primitives loaded
   static constexpr int TOP = PRIM;
-  static constexpr int XEND = TOP + 8 * TX;
-  static constexpr int XL = XEND + 8 * TY;
-  static constexpr int XU = XL + 8 * TY;
-  static constexpr int XR = XU + 8 * TY;
-  static constexpr int YD = XR + 8 * TY;
-  static constexpr int YU = YD + 8 * TX;
+  static constexpr int XEND = TOP + NQ * TX;
+  static constexpr int XL = XEND + NQ * TY;
+  static constexpr int XU = XL + NQ * TY;
+  static constexpr int XR = XU + NQ * TY;
+  static constexpr int YD = XR + NQ * TY;
+  static constexpr int YU = YD + NQ * TX;
   static constexpr int NYU = TY / 2 + 1;
-  static constexpr int SIZE = ORDER == 1 ? XL : YU + 8 * NYU * TX;
-  static_assert(8 * TY * TX <= PRIM, "y-face fluxes must fit the primitive area");
+  static constexpr int SIZE = ORDER == 1 ? XL : YU + NQ * NYU * TX;
+  static_assert(NQ * TY * TX <= PRIM, "y-face fluxes must fit the primitive area");
 };
 
 // Reconstructed states of one cell along one axis, from the primitives in
@@ -161,11 +161,11 @@ struct XYLds {
 // limiter() (src/hydro_core.f90:712-731) forms both from the one slope
 // average(P - P-, P+ - P) of the cell, so each cell's slope serves both of
 // its faces.
-template <typename S>
-__device__ __forceinline__ void cell_states(int lim, const S* __restrict__ lp, int ps, int at, int stride, S up[8],
-                                            S dn[8]) {
+template <int NQ, typename S>
+__device__ __forceinline__ void cell_states(int lim, const S* __restrict__ lp, int ps, int at, int stride, S up[NQ],
+                                            S dn[NQ]) {
 #pragma unroll
-  for (int q = 0; q < 8; q++) {
+  for (int q = 0; q < NQ; q++) {
     const S p = lp[q * ps + at];
     const S s = lim_average(lim, p - lp[q * ps + at - stride], lp[q * ps + at + stride] - p);
     up[q] = p + s * S(0.5);
@@ -174,11 +174,11 @@ __device__ __forceinline__ void cell_states(int lim, const S* __restrict__ lp, i
 }
 
 // One of the two: the upper state when `up` holds, else the lower.
-template <typename S>
+template <int NQ, typename S>
 __device__ __forceinline__ void cell_state(int lim, const S* __restrict__ lp, int ps, int at, int stride, bool up,
-                                           S v[8]) {
+                                           S v[NQ]) {
 #pragma unroll
-  for (int q = 0; q < 8; q++) {
+  for (int q = 0; q < NQ; q++) {
     const S p = lp[q * ps + at];
     const S s = lim_average(lim, p - lp[q * ps + at - stride], lp[q * ps + at + stride] - p);
     const S hs = s * S(0.5);
@@ -189,7 +189,7 @@ __device__ __forceinline__ void cell_state(int lim, const S* __restrict__ lp, in
 // y faces exchange slots (1,2) and (5,6) when `ydir` holds (axis_perm<1>
 // with a run-time axis, by value selects).
 template <typename T>
-__device__ __forceinline__ void perm_y_if(bool ydir, T v[8]) {
+__device__ __forceinline__ void perm_y_if(bool ydir, T* v) {
   const T a = ydir ? v[2] : v[1], b = ydir ? v[1] : v[2];
   const T e = ydir ? v[6] : v[5], f = ydir ? v[5] : v[6];
   v[1] = a;
@@ -210,10 +210,11 @@ __device__ __forceinline__ void perm_y_if(bool ydir, T v[8]) {
 // (cell_states), waves 6 and 7 those of the 2 (TX + TY) halo cells next to
 // the tile.  The x faces are solved before the y states are formed, which
 // keeps the live register set of one axis at a time.
-template <int SOLVER, int ORDER, int FLUXCD, typename S>
+template <int SOLVER, int ORDER, int FLUXCD, typename S, int NP = 0>
 __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt, int* __restrict__ err,
                                         S* __restrict__ lds) {
-  using L = XYLds<ORDER>;
+  constexpr int NQ = 8 + NP;  // the MHD variables and the passive scalars
+  using L = XYLds<ORDER, NQ>;
   constexpr int H = L::H, RX = L::RX, PS = L::PS;
   constexpr int NH = FLUXCD ? 5 : 8;
   const Geom& g = b.g;
@@ -242,7 +243,7 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
   // ahead of the rest.  Priority changes issue order only.
   __builtin_amdgcn_s_setprio(3);
   {
-    double u[2][8];
+    double u[2][NQ];
     int dst[2];
 #pragma unroll
     for (int it = 0; it < 2; it++) {
@@ -263,17 +264,17 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
       dst[it] = -1;
       if (idx < L::NLOAD && rx < wx + 2 * H && ry < wy + 2 * H) {
         dst[it] = ry * RX + rx;
-        load_cons(g, Usrc, i0 - H + rx, j0 - H + ry, k, u[it]);
+        load_cons<NQ>(g, Usrc, i0 - H + rx, j0 - H + ry, k, u[it]);
       }
     }
     __builtin_amdgcn_s_setprio(0);
 #pragma unroll
     for (int it = 0; it < 2; it++) {
       if (dst[it] < 0) continue;
-      double p[8];
-      u2prim(b.ph, u[it], p);
+      double p[NQ];
+      u2prim<NQ>(b.ph, u[it], p);
 #pragma unroll
-      for (int q = 0; q < 8; q++) lp[q * PS + dst[it]] = (S)p[q];
+      for (int q = 0; q < NQ; q++) lp[q * PS + dst[it]] = (S)p[q];
     }
   }
   __syncthreads();
@@ -285,46 +286,46 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
   const bool cy = w == 0 && l >= TY && l < TY + TX;   // closing y-face of column l - TY
   const int crow = cx ? l : 0, ccol = cy ? l - TY : 0;
   bool ok = true;
-  S fx[8], fy[8];
+  S fx[NQ], fy[NQ];
 #pragma unroll
-  for (int q = 0; q < 8; q++) fx[q] = fy[q] = S(0);
+  for (int q = 0; q < NQ; q++) fx[q] = fy[q] = S(0);
 
   if constexpr (ORDER == 1) {
-    S pl[8], pr[8];
+    S pl[NQ], pr[NQ];
     if (vx) {
 #pragma unroll
-      for (int q = 0; q < 8; q++) {
+      for (int q = 0; q < NQ; q++) {
         pl[q] = lp[q * PS + at - 1];
         pr[q] = lp[q * PS + at];
       }
-      ok &= riemann<SOLVER>(b.ph, pl, pr, fx);
+      ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, fx);
     }
     if (vy) {
 #pragma unroll
-      for (int q = 0; q < 8; q++) {
+      for (int q = 0; q < NQ; q++) {
         pl[q] = lp[q * PS + at - RX];
         pr[q] = lp[q * PS + at];
       }
       axis_perm<1>(pl);
       axis_perm<1>(pr);
-      ok &= riemann<SOLVER>(b.ph, pl, pr, fy);
+      ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, fy);
       axis_perm<1>(fy);
     }
     if ((cx && wx == TX && crow < wy) || (cy && wy == TY && ccol < wx)) {
       const int hi = cy ? (TY + H) * RX + ccol + H : (crow + H) * RX + TX + H;
       const int st = cy ? RX : 1;
 #pragma unroll
-      for (int q = 0; q < 8; q++) {
+      for (int q = 0; q < NQ; q++) {
         pl[q] = lp[q * PS + hi - st];
         pr[q] = lp[q * PS + hi];
       }
-      S ff[8];
+      S ff[NQ];
       perm_y_if(cy, pl);
       perm_y_if(cy, pr);
-      ok &= riemann<SOLVER>(b.ph, pl, pr, ff);
+      ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, ff);
       perm_y_if(cy, ff);
 #pragma unroll
-      for (int q = 0; q < 8; q++) {
+      for (int q = 0; q < NQ; q++) {
         if (cy) ltop[q * TX + ccol] = ff[q];
         else lend[q * TY + crow] = ff[q];
       }
@@ -353,69 +354,69 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
         } else {
           hat = (H + TY) * RX + h - 2 * TY - TX + H, hst = RX, hup = false, hdst = L::YD + h - 2 * TY - TX, hqs = TX;
         }
-        S hs[8];
-        cell_state(lim, lp, PS, hat, hst, hup, hs);
+        S hs[NQ];
+        cell_state<NQ>(lim, lp, PS, hat, hst, hup, hs);
 #pragma unroll
-        for (int q = 0; q < 8; q++) lds[hdst + q * hqs] = hs[q];
+        for (int q = 0; q < NQ; q++) lds[hdst + q * hqs] = hs[q];
       }
     }
     if (w >= XY_WAVES - 2) __builtin_amdgcn_s_setprio(0);
     // x: the upper state passes one lane up (lane c-1 holds cell c-1)
-    S pl[8], pr[8];
+    S pl[NQ], pr[NQ];
     {
-      S pxu[8];
-      cell_states(lim, lp, PS, at, 1, pxu, pr);
+      S pxu[NQ];
+      cell_states<NQ>(lim, lp, PS, at, 1, pxu, pr);
       if (c == TX - 1) {
 #pragma unroll
-        for (int q = 0; q < 8; q++) xu[q * TY + r] = pxu[q];
+        for (int q = 0; q < NQ; q++) xu[q * TY + r] = pxu[q];
       }
 #pragma unroll
-      for (int q = 0; q < 8; q++) pl[q] = __shfl_up(pxu[q], 1, 64);
+      for (int q = 0; q < NQ; q++) pl[q] = __shfl_up(pxu[q], 1, 64);
     }
     __syncthreads();  // halo states in LDS
     if (c == 0) {
 #pragma unroll
-      for (int q = 0; q < 8; q++) pl[q] = xl[q * TY + r];
+      for (int q = 0; q < NQ; q++) pl[q] = xl[q * TY + r];
     }
-    if (vx) ok &= riemann<SOLVER>(b.ph, pl, pr, fx);
+    if (vx) ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, fx);
     // y: odd rows hand their upper state to the even row above them (the
     // next wavefront) through LDS, even rows theirs to the odd row above
     // (lanes 32 up in the same wavefront)
     {
-      S pyu[8];
-      cell_states(lim, lp, PS, at, RX, pyu, pr);
+      S pyu[NQ];
+      cell_states<NQ>(lim, lp, PS, at, RX, pyu, pr);
       if (r & 1) {
 #pragma unroll
-        for (int q = 0; q < 8; q++) yu[(q * L::NYU + (r + 1) / 2) * TX + c] = pyu[q];
+        for (int q = 0; q < NQ; q++) yu[(q * L::NYU + (r + 1) / 2) * TX + c] = pyu[q];
       }
 #pragma unroll
-      for (int q = 0; q < 8; q++) pl[q] = __shfl_up(pyu[q], 32, 64);
+      for (int q = 0; q < NQ; q++) pl[q] = __shfl_up(pyu[q], 32, 64);
     }
     __syncthreads();  // every read of the primitives is done; y states in LDS
     if (w == 0) __builtin_amdgcn_s_setprio(2);
     if (!(r & 1)) {
 #pragma unroll
-      for (int q = 0; q < 8; q++) pl[q] = yu[(q * L::NYU + r / 2) * TX + c];
+      for (int q = 0; q < NQ; q++) pl[q] = yu[(q * L::NYU + r / 2) * TX + c];
     }
     if (vy) {
       axis_perm<1>(pl);
       axis_perm<1>(pr);
-      ok &= riemann<SOLVER>(b.ph, pl, pr, fy);
+      ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, fy);
       axis_perm<1>(fy);
     }
     if ((cx && wx == TX && crow < wy) || (cy && wy == TY && ccol < wx)) {
-      S ff[8];
+      S ff[NQ];
 #pragma unroll
-      for (int q = 0; q < 8; q++) {
+      for (int q = 0; q < NQ; q++) {
         pl[q] = cy ? yu[(q * L::NYU + TY / 2) * TX + ccol] : xu[q * TY + crow];
         pr[q] = cy ? yd[q * TX + ccol] : xr[q * TY + crow];
       }
       perm_y_if(cy, pl);
       perm_y_if(cy, pr);
-      ok &= riemann<SOLVER>(b.ph, pl, pr, ff);
+      ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, ff);
       perm_y_if(cy, ff);
 #pragma unroll
-      for (int q = 0; q < 8; q++) {
+      for (int q = 0; q < NQ; q++) {
         if (cy) ltop[q * TX + ccol] = ff[q];
         else lend[q * TY + crow] = ff[q];
       }
@@ -427,33 +428,41 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
   const bool own = c < wx && r < wy;
   const long long N = g.ncell;
   const long long cc = cidx(g, i0 + c, j0 + r, k);
-  double u0[NH];
+  double u0[NH], u0p[NP > 0 ? NP : 1];
   if (own) {
 #pragma unroll
     for (int q = 0; q < NH; q++) u0[q] = (double)b.U[q * N + cc];
+#pragma unroll
+    for (int a = 0; a < NP; a++) u0p[a] = (double)b.U[(8 + a) * N + cc];
   }
   if constexpr (ORDER == 1) __syncthreads();  // every read of the primitives is done
   S* __restrict__ lfy = lds;
   if (vy) {
 #pragma unroll
-    for (int q = 0; q < 8; q++) lfy[(q * TY + r) * TX + c] = fy[q];
+    for (int q = 0; q < NQ; q++) lfy[(q * TY + r) * TX + c] = fy[q];
   }
   __syncthreads();
 
-  S fxr[8], fyu[8];
+  S fxr[NQ], fyu[NQ];
 #pragma unroll
-  for (int q = 0; q < 8; q++) {
+  for (int q = 0; q < NQ; q++) {
     const S v = __shfl_down(fx[q], 1, 64);
     fxr[q] = c == TX - 1 ? lend[q * TY + r] : v;
   }
   if (!own) return;
 #pragma unroll
-  for (int q = 0; q < 8; q++) fyu[q] = r == TY - 1 ? ltop[q * TX + c] : lfy[(q * TY + r + 1) * TX + c];
+  for (int q = 0; q < NQ; q++) fyu[q] = r == TY - 1 ? ltop[q * TX + c] : lfy[(q * TY + r + 1) * TX + c];
   const double dtdx = dt / b.dx, dtdy = dt / b.dy;
   double* __restrict__ W = b.W;
 #pragma unroll
   for (int q = 0; q < NH; q++)
     W[q * N + cc] = (u0[q] - dtdx * ((double)fxr[q] - (double)fx[q])) - dtdy * ((double)fyu[q] - (double)fy[q]);
+  // passive scalars: the same partial sums (flux_cd_module.f90:303-308)
+#pragma unroll
+  for (int a = 0; a < NP; a++) {
+    const int q = 8 + a;
+    W[q * N + cc] = (u0p[a] - dtdx * ((double)fxr[q] - (double)fx[q])) - dtdy * ((double)fyu[q] - (double)fy[q]);
+  }
   if constexpr (FLUXCD) {
     W[5 * N + cc] = (-(double)fy[7]) - (double)fyu[7];
     W[6 * N + cc] = (+(double)fx[7]) + (double)fxr[7];
@@ -469,9 +478,10 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
 // reconstruction of a cell depends on the cell and its two neighbours
 // only, so each cell's slope is formed once and serves both its faces
 // (pl + al/2 for the face above, pr - ar/2 for the face below).
-template <int SOLVER, int ORDER, int FLUXCD, typename S>
+template <int SOLVER, int ORDER, int FLUXCD, typename S, int NP = 0>
 __device__ __forceinline__ void z_body(const Box<S>& b, unsigned lb, double dt, int kchunk, int* __restrict__ err) {
   constexpr int NH = FLUXCD ? 5 : 8;
+  constexpr int NQ = 8 + NP;  // the MHD variables and the passive scalars
   const Geom& g = b.g;
   int t, kc;
   if (!tile_item(lb, tiles_of((unsigned)(g.nx * g.ny)), (unsigned)(g.nx * g.ny), t, kc)) return;
@@ -488,29 +498,29 @@ __device__ __forceinline__ void z_body(const Box<S>& b, unsigned lb, double dt, 
   const double* __restrict__ W = b.W;
   const int lim = b.ph.limiter;
 
-  S ur[8];
+  S ur[NQ];
   bool uflip = false;
   auto fetch = [&](int k) {
     bool f = false;
     const int sk = ghost_src(g, k, g.nz, g.bc[4], g.bc[5], f);
     const long long cc = col + (long long)(sk + 1) * sz;
 #pragma unroll
-    for (int q = 0; q < 8; q++) ur[q] = Usrc[q * N + cc];
+    for (int q = 0; q < NQ; q++) ur[q] = Usrc[q * N + cc];
     uflip = f;
   };
-  auto take = [&](S p[8]) {
-    double u[8], pd[8];
+  auto take = [&](S p[NQ]) {
+    double u[NQ], pd[NQ];
 #pragma unroll
-    for (int q = 0; q < 8; q++) u[q] = (double)ur[q];
+    for (int q = 0; q < NQ; q++) u[q] = (double)ur[q];
     if (uflip) u[3] = -u[3];
-    u2prim(b.ph, u, pd);
+    u2prim<NQ>(b.ph, u, pd);
 #pragma unroll
-    for (int q = 0; q < 8; q++) p[q] = (S)pd[q];
+    for (int q = 0; q < NQ; q++) p[q] = (S)pd[q];
   };
 
-  S pm[8], pc[8], pu[8];  // P(kf), P(kf+1), upper-face state of cell kf
+  S pm[NQ], pc[NQ], pu[NQ];  // P(kf), P(kf+1), upper-face state of cell kf
   if constexpr (ORDER == 2) {
-    S p0[8];
+    S p0[NQ];
     fetch(k0 - 2);
     take(p0);
     fetch(k0 - 1);
@@ -518,35 +528,37 @@ __device__ __forceinline__ void z_body(const Box<S>& b, unsigned lb, double dt, 
     fetch(k0);
     take(pc);
 #pragma unroll
-    for (int q = 0; q < 8; q++) pu[q] = pm[q] + lim_average(lim, pm[q] - p0[q], pc[q] - pm[q]) * S(0.5);
+    for (int q = 0; q < NQ; q++) pu[q] = pm[q] + lim_average(lim, pm[q] - p0[q], pc[q] - pm[q]) * S(0.5);
     fetch(k0 + 1);
   } else {
     fetch(k0 - 1);
     take(pc);
     fetch(k0);
   }
-  S fl[8];
+  S fl[NQ];
 #pragma unroll
-  for (int q = 0; q < 8; q++) fl[q] = S(0);
+  for (int q = 0; q < NQ; q++) fl[q] = S(0);
 #pragma unroll 1
   for (int kf = k0 - 1; kf <= k1; kf++) {
-    S pn[8];
+    S pn[NQ];
     take(pn);  // P(kf+ORDER)
     if (kf + ORDER + 1 <= klast) fetch(kf + ORDER + 1);
     // partials of the cell this face closes, loaded ahead of the solve
     const int kc0 = max(kf, k0);
     const long long c = col + (long long)(kc0 + 1) * sz;
-    double wv[8];
+    double wv[NQ];
 #pragma unroll
     for (int q = 0; q < NH; q++) wv[q] = W[q * N + c];
     if constexpr (FLUXCD) {
       wv[5] = W[5 * N + c];
       wv[6] = W[6 * N + c];
     }
-    S left[8], right[8];
+#pragma unroll
+    for (int q = 8; q < NQ; q++) wv[q] = W[q * N + c];
+    S left[NQ], right[NQ];
     if constexpr (ORDER == 2) {
 #pragma unroll
-      for (int q = 0; q < 8; q++) {
+      for (int q = 0; q < NQ; q++) {
         const S s = lim_average(lim, pc[q] - pm[q], pn[q] - pc[q]);
         left[q] = pu[q];
         right[q] = pc[q] - s * S(0.5);
@@ -556,7 +568,7 @@ __device__ __forceinline__ void z_body(const Box<S>& b, unsigned lb, double dt, 
       }
     } else {
 #pragma unroll
-      for (int q = 0; q < 8; q++) {
+      for (int q = 0; q < NQ; q++) {
         left[q] = pc[q];
         right[q] = pn[q];
         pc[q] = pn[q];
@@ -564,19 +576,21 @@ __device__ __forceinline__ void z_body(const Box<S>& b, unsigned lb, double dt, 
     }
     axis_perm<2>(left);
     axis_perm<2>(right);
-    S ff[8];
-    if (!riemann<SOLVER>(b.ph, left, right, ff)) atomicOr(err, 1);
+    S ff[NQ];
+    if (!riemann<SOLVER, NQ>(b.ph, left, right, ff)) atomicOr(err, 1);
     axis_perm<2>(ff);
     if (kf >= k0) {
 #pragma unroll
       for (int q = 0; q < NH; q++) Uout[q * N + c] = (S)(wv[q] - dtdz * ((double)ff[q] - (double)fl[q]));
+#pragma unroll
+      for (int q = 8; q < NQ; q++) Uout[q * N + c] = (S)(wv[q] - dtdz * ((double)ff[q] - (double)fl[q]));
       if constexpr (FLUXCD) {
         b.E[0 * N + c] = (S)(0.25 * ((wv[5] + (double)fl[6]) + (double)ff[6]));
         b.E[1 * N + c] = (S)(0.25 * ((wv[6] - (double)fl[5]) - (double)ff[5]));
       }
     }
 #pragma unroll
-    for (int q = 0; q < 8; q++) fl[q] = ff[q];
+    for (int q = 0; q < NQ; q++) fl[q] = ff[q];
   }
 }
 
@@ -678,12 +692,12 @@ __device__ __forceinline__ void sources_body(const Box<S>& b, unsigned lb, doubl
   S* __restrict__ Uout = stage == 1 ? b.Us : b.U;
   for (int k = k0; k <= k1; k++) {
     const long long c = cidx(g, i, j, k);
-    double u[8], pp[8], sv[8];
+    double u[8], pp[8], sv[10];
 #pragma unroll
     for (int q = 0; q < 8; q++) u[q] = (double)Uin[q * N + c];
-    u2prim(b.ph, u, pp);
+    u2prim(b.ph, u, pp);  // the 8-wave terms read no pressure: the EOS leaves them alone
 #pragma unroll
-    for (int q = 0; q < 8; q++) sv[q] = (b.src & SRC_USER) ? b.W[q * N + c] : 0.0;
+    for (int q = 0; q < 10; q++) sv[q] = (q < 8 + g.npas && (b.src & SRC_USER)) ? b.W[q * N + c] : 0.0;
     if (b.src & SRC_EIGHT_WAVE) {
       const double divb = ((field_at(g, Uin, 5, i + 1, j, k) - field_at(g, Uin, 5, i - 1, j, k)) / (2.0 * b.dx)
                            + (field_at(g, Uin, 6, i, j + 1, k) - field_at(g, Uin, 6, i, j - 1, k)) / (2.0 * b.dy))
@@ -697,7 +711,8 @@ __device__ __forceinline__ void sources_body(const Box<S>& b, unsigned lb, doubl
       sv[7] = sv[7] - divb * pp[3];
     }
 #pragma unroll
-    for (int q = 0; q < 8; q++) Uout[q * N + c] = (S)((double)Uout[q * N + c] + dt * sv[q]);
+    for (int q = 0; q < 10; q++)
+      if (q < 8 + g.npas) Uout[q * N + c] = (S)((double)Uout[q * N + c] + dt * sv[q]);
   }
 }
 
@@ -729,10 +744,10 @@ __global__ void __launch_bounds__(256) k_cooling(Box<S> b, Cooling cl, double dt
   const long long r = t / g.nx;
   const int j = (int)(r % g.ny) + 1, k = (int)(r / g.ny) + 1;
   const long long c = cidx(g, i, j, k), N = g.ncell;
-  double u[8], p[8];
+  double u[10], p[10];
 #pragma unroll
-  for (int q = 0; q < 8; q++) u[q] = (double)b.U[q * N + c];
-  const double T = module_temp(b.ph, u2prim_t(b.ph, u, p), p, cl.tempsc);
+  for (int q = 0; q < 10; q++) u[q] = q < 8 + g.npas ? (double)b.U[q * N + c] : 0.0;
+  const double T = module_temp(b.ph, u2prim_t<10>(b.ph, u, p), p, cl.tempsc);
   if (!(T > 10000.)) return;
   const double eth0 = b.ph.cv * p[4];
   const double aloss = cool_lambda(cl, T);
@@ -740,6 +755,106 @@ __global__ void __launch_bounds__(256) k_cooling(Box<S> b, Cooling cl, double dt
   const double ce = (aloss * (dens * dens)) / (eth0 * cl.psc);
   const double pn = p[4] * exp(-(ce * (dt * cl.tsc)));
   b.U[4 * N + c] = (S)((u[4] - eth0) + b.ph.cv * pn);
+}
+
+// ---------------------------------------------------------------------------
+// H-rate cooling, coolingh (src/cooling_h.f90:41-67): per interior cell,
+// atomic(dt tsc, u, 1, 1) without diffuse radiation (:226-374) -- the
+// neutral fraction's rate equation solved over the step (recombination
+// alpha :80-88, collisional ionisation colf :108-116), the radiative loss
+// ALOSS (:141-218, with betah :124-137) and the relaxed temperature -- then
+// the neutral density (the first passive scalar) and E.  Evaluation order
+// is the reference's; the reference runs alpha, colf and betah with REAL(16)
+// constants, the device in FP64, and exp/log/pow are the device's.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double hr_betah(double t) {
+  const double a = 157890. / t;
+  return (1.133e-24 / __builtin_sqrt(a)) * ((-0.0713 + 0.5 * log(a)) + 0.640 * pow(a, -0.33333));
+}
+
+__device__ double hr_aloss(double x1, double den, double dh0, double te0) {
+  const double XION = 2.179e-11, XO = 1.e-3;
+  const double C0 = 0.5732, C1 = 1.8288e-5, C2 = -1.15822e-10, C3 = 9.4288e-16;
+  const double D0 = 0.5856, D1 = 1.55083e-5, D2 = -9.669e-12, D3 = 5.716e-19;
+  const double ENK = 118409., EN = 1.634E-11;
+  const double te = dmax(te0, 10.);
+  const double dh = den;
+  const double dhp = (1. - x1) * dh;
+  const double de = dhp + 1.e-4 * dh;
+  const double doi = XO * dh0;
+  const double doii = XO * dhp;
+  if (te <= 1e4) return 0.;
+  double omega = 0.;
+  if (te <= 55000.) omega = C0 + te * (C1 + te * (C2 + te * C3));
+  if (te >= 72000.) omega = D0 + te * (D1 + te * (D2 + te * D3));
+  if (te > 55000. && te < 72000.) {
+    const double omegal = C0 + te * (C1 + te * (C2 + te * C3));
+    const double omegah = D0 + te * (D1 + te * (D2 + te * D3));
+    const double frac = (te - 55000.) / 17000.;
+    omega = (1. - frac) * omegal + frac * omegah;
+  }
+  const double qla = ((8.6287E-6 / (2. * __builtin_sqrt(te))) * omega) * exp(-(ENK / te));
+  const double ecoll = dmax(((de * dh0) * qla) * EN, 0.);
+  const double cion = (5.834E-11 * __builtin_sqrt(te)) * exp(-(1.579E5 / te));
+  const double eion = ((de * dh0) * cion) * XION;
+  const double erec = dmax((de * dhp) * hr_betah(te), 0.);
+  const double tm = 1. / te;
+  const double t2 = tm * tm;
+  const double eoi = dmax((de * doi) * pow(10., ((1381465. * t2) - (12328.69 * tm)) - 19.82621), 0.);
+  const double eoii = dmax((de * doii) * pow(10., (-(2061075. * t2) - (14596.24 * tm)) - 19.01402), 0.);
+  const double betaf = (1.3 * 1.42E-27) * pow(te, 0.5);
+  const double hiicool = (de * dhp) * betaf;
+  const double equil = (((1.0455E-18 / pow(te, 0.63)) * (1. - exp(-pow(te * 1.E-5, 1.63)))) * de) * den + hiicool;
+  double fr = 0.;
+  if (te >= 54770.) fr = 1.;
+  if (te > 44770. && te < 54770.) {
+    const double ex2 = exp(-((2. * (te - 49770.)) / 500.));
+    fr = 0.5 * (1. + (1. - ex2) / (1. + ex2));
+  }
+  return ((ecoll + eion) + ((erec + 7.033 * (eoi + eoii)) * (1. - fr))) + equil * fr;
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_cooling_h(Box<S> b, Cooling cl, double dt) {
+  const Geom& g = b.g;
+  const long long n = (long long)g.nx * g.ny * g.nz;
+  const long long tt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tt >= n) return;
+  const int i = (int)(tt % g.nx) + 1;
+  const long long r = tt / g.nx;
+  const int j = (int)(r % g.ny) + 1, k = (int)(r / g.ny) + 1;
+  const long long c = cidx(g, i, j, k), N = g.ncell;
+  double u[10], p[10];
+#pragma unroll
+  for (int q = 0; q < 10; q++) u[q] = q < 8 + g.npas ? (double)b.U[q * N + c] : 0.0;
+  const double t = module_temp(b.ph, u2prim_t<10>(b.ph, u, p), p, cl.tempsc);
+  const double dts = dt * cl.tsc;
+  const double xi = 1.e-4, boltzm = 1.3807e-16;
+  const double col = (5.83e-11 * __builtin_sqrt(t)) * exp(-(157828. / t));
+  const double rec = 2.55e-13 * pow(1.e4 / t, 0.79);
+  const double y0 = u[8] / u[0];
+  const double dh = u[0];
+  const double a = rec + col;
+  const double bq = -((2. + xi) * rec + (1. + xi) * col);
+  const double cq = (1. + xi) * rec;
+  const double d = __builtin_sqrt(bq * bq - (4. * a) * cq);
+  const double g0 = ((((2. * a) * y0) + bq) + d) / ((((2. * a) * y0) + bq) - d);
+  const double e = exp(-((d * dh) * dts));
+  double y1 = ((-bq) - ((d * (1. + g0 * e)) / (1. - g0 * e))) / (2. * a);
+  y1 = dmin(y1, 0.9999);
+  y1 = dmax(y1, 0.);
+  const double al = hr_aloss(y0, dh, u[8], t) / (dh * dh);
+  const double tprime = 10.;
+  const double ce = ((2. * dh) * al) / ((3. * boltzm) * t);
+  double t1 = tprime + (t - tprime) * exp(-(ce * dts));
+  t1 = dmax(t1, 0.1 * t);
+  t1 = dmin(t1, 10. * t);
+  const double u9 = y1 * u[0];
+  double e5 = (((b.ph.cv * ((2. * u[0]) - u9)) * t1) / cl.tempsc)
+              + (0.5 * p[0]) * ((p[1] * p[1] + p[2] * p[2]) + p[3] * p[3]);
+  if (b.ph.mhd) e5 = e5 + 0.5 * ((p[5] * p[5] + p[6] * p[6]) + p[7] * p[7]);
+  b.U[8 * N + c] = (S)u9;
+  b.U[4 * N + c] = (S)e5;
 }
 
 // ---------------------------------------------------------------------------
@@ -795,8 +910,7 @@ __global__ void __launch_bounds__(256) k_viscous(Box<S> b, double eta) {
   const ViscNbr xp = visc_nbr(g, i + 1, j, k), xm = visc_nbr(g, i - 1, j, k);
   const ViscNbr yp = visc_nbr(g, i, j + 1, k), ym = visc_nbr(g, i, j - 1, k);
   const ViscNbr zp = visc_nbr(g, i, j, k + 1), zm = visc_nbr(g, i, j, k - 1);
-#pragma unroll
-  for (int q = 0; q < 8; q++) {
+  for (int q = 0; q < 8 + g.npas; q++) {
     const double up = (double)b.U[q * N + c];
     const double lap = (((((visc_val(b, xp, q) + visc_val(b, xm, q)) + visc_val(b, yp, q)) + visc_val(b, ym, q))
                          + visc_val(b, zp, q)) + visc_val(b, zm, q)) - 6.0 * up;
@@ -814,8 +928,7 @@ __global__ void __launch_bounds__(256) k_visc_store(Box<S> b) {
   const long long r = t / g.nx;
   const int j = (int)(r % g.ny) + 1, k = (int)(r / g.ny) + 1;
   const long long c = cidx(g, i, j, k), N = g.ncell;
-#pragma unroll
-  for (int q = 0; q < 8; q++) b.U[q * N + c] = (S)b.W[q * N + c];
+  for (int q = 0; q < 8 + g.npas; q++) b.U[q * N + c] = (S)b.W[q * N + c];
 }
 
 // ---------------------------------------------------------------------------
@@ -840,10 +953,10 @@ struct TcCell {
 template <typename S>
 __device__ __forceinline__ TcCell tc_cell(const Box<S>& b, const Conduction& tc, int i, int j, int k) {
   const long long c = cidx(b.g, i, j, k), N = b.g.ncell;
-  double u[8], p[8];
+  double u[10], p[10];
 #pragma unroll
-  for (int q = 0; q < 8; q++) u[q] = (double)b.U[q * N + c];
-  const double te = u2prim_t(b.ph, u, p);
+  for (int q = 0; q < 10; q++) u[q] = q < 8 + b.g.npas ? (double)b.U[q * N + c] : 0.0;
+  const double te = u2prim_t<10>(b.ph, u, p);
   TcCell r;
   r.rho = p[0];
   r.p = p[4];
@@ -1028,7 +1141,7 @@ __device__ __forceinline__ double wave_min(double v) {
   return v;
 }
 
-template <typename S>
+template <typename S, int NQ = 8>
 __device__ __forceinline__ void cfl_body(const Box<S>& b, unsigned lb, int kchunk,
                                          unsigned long long* __restrict__ dtmin, int* __restrict__ err) {
   __shared__ double red[4];
@@ -1043,13 +1156,13 @@ __device__ __forceinline__ void cfl_body(const Box<S>& b, unsigned lb, int kchun
     const int k1 = min(k0 + kchunk - 1, g.nz);
     for (int k = k0; k <= k1; k++) {
       const long long c = cidx(g, i, j, k);
-      double u[8], p[8];
+      double u[NQ], p[NQ];
 #pragma unroll
-      for (int q = 0; q < 8; q++) {
-        u[q] = (double)b.U[q * g.ncell + c];
+      for (int q = 0; q < NQ; q++) {
+        u[q] = q < 8 + g.npas ? (double)b.U[q * g.ncell + c] : 0.0;
         bad |= !__builtin_isfinite(u[q]);
       }
-      u2prim(ph, u, p);
+      u2prim<NQ>(ph, u, p);
       if (ph.mhd) {
         const double b2 = (p[5] * p[5] + p[6] * p[6]) + p[7] * p[7];
         const double cx = cfast1(ph.gamma, p[4], p[0], b2, p[5]);
@@ -1083,15 +1196,18 @@ __device__ __forceinline__ void cfl_body(const Box<S>& b, unsigned lb, int kchun
 // ---------------------------------------------------------------------------
 // __global__ wrappers: one box ...
 // ---------------------------------------------------------------------------
-template <int SOLVER, int ORDER, int FLUXCD, typename S>
-__global__ void __launch_bounds__(XY_THREADS) __attribute__((amdgpu_waves_per_eu(4))) k_xy(Box<S> b, double dt, int* err) {
-  __shared__ S lds[XYLds<ORDER>::SIZE];
-  xy_body<SOLVER, ORDER, FLUXCD>(b, xcd_logical_block(), dt, err, lds);
+// NP passive scalars: their larger LDS tile leaves one workgroup per CU, so
+// the register budget is not squeezed for them.
+template <int SOLVER, int ORDER, int FLUXCD, typename S, int NP>
+__global__ void __launch_bounds__(XY_THREADS) __attribute__((amdgpu_waves_per_eu(NP == 0 ? 4 : 2)))
+k_xy(Box<S> b, double dt, int* err) {
+  __shared__ S lds[XYLds<ORDER, 8 + NP>::SIZE];
+  xy_body<SOLVER, ORDER, FLUXCD, S, NP>(b, xcd_logical_block(), dt, err, lds);
 }
 
-template <int SOLVER, int ORDER, int FLUXCD, typename S>
+template <int SOLVER, int ORDER, int FLUXCD, typename S, int NP>
 __global__ void __launch_bounds__(256) k_z(Box<S> b, double dt, int kchunk, int* err) {
-  z_body<SOLVER, ORDER, FLUXCD>(b, xcd_logical_block(), dt, kchunk, err);
+  z_body<SOLVER, ORDER, FLUXCD, S, NP>(b, xcd_logical_block(), dt, kchunk, err);
 }
 
 template <typename S>
@@ -1099,9 +1215,9 @@ __global__ void __launch_bounds__(256) k_bfield(Box<S> b, double dt, int stage, 
   bfield_body(b, xcd_logical_block(), dt, stage, kchunk);
 }
 
-template <typename S>
+template <typename S, int NQ>
 __global__ void __launch_bounds__(256) k_cfl(Box<S> b, int kchunk, unsigned long long* dtmin, int* err) {
-  cfl_body(b, xcd_logical_block(), kchunk, dtmin, err);
+  cfl_body<S, NQ>(b, xcd_logical_block(), kchunk, dtmin, err);
 }
 
 template <typename S>
@@ -1238,8 +1354,7 @@ __global__ void __launch_bounds__(256) k_ghost(Geom g, S* __restrict__ U, int ax
     else if (axis == 1) { dj = dst; sj = src; di = si = a; dk = sk = b; }
     else { dk = dst; sk = src; di = si = a; dj = sj = b; }
     const long long cd = cidx(g, di, dj, dk), cs = cidx(g, si, sj, sk);
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
+    for (int q = 0; q < 8 + g.npas; q++) {
       const S v = U[q * g.ncell + cs];
       U[q * g.ncell + cd] = (kind == BC_CLOSED && q == 1 + axis) ? -v : v;
     }
@@ -1250,38 +1365,35 @@ __global__ void __launch_bounds__(256) k_ghost(Geom g, S* __restrict__ U, int ax
 // layout conversion between the reference's AoS u(neq,i,j,k) and SoA
 // ---------------------------------------------------------------------------
 template <typename S>
-__global__ void __launch_bounds__(256) k_aos_to_soa(long long ncell, const double* __restrict__ A,
+__global__ void __launch_bounds__(256) k_aos_to_soa(long long ncell, int nq, const double* __restrict__ A,
                                                     S* __restrict__ D) {
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < ncell;
        c += (long long)gridDim.x * blockDim.x) {
-#pragma unroll
-    for (int q = 0; q < 8; q++) D[q * ncell + c] = (S)A[c * 8 + q];
+    for (int q = 0; q < nq; q++) D[q * ncell + c] = (S)A[c * nq + q];
   }
 }
 
 template <typename S>
-__global__ void __launch_bounds__(256) k_soa_to_aos(long long ncell, const S* __restrict__ D,
+__global__ void __launch_bounds__(256) k_soa_to_aos(long long ncell, int nq, const S* __restrict__ D,
                                                     double* __restrict__ A) {
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < ncell;
        c += (long long)gridDim.x * blockDim.x) {
-#pragma unroll
-    for (int q = 0; q < 8; q++) A[c * 8 + q] = (double)D[q * ncell + c];
+    for (int q = 0; q < nq; q++) A[c * nq + q] = (double)D[q * ncell + c];
   }
 }
 
 // primitives and temperature of every cell as stored (calcprim over the
 // whole array, hydro_core.f90:304-316), AoS output
 template <typename S>
-__global__ void __launch_bounds__(256) k_prim_all(long long ncell, Phys ph, const S* __restrict__ U,
+__global__ void __launch_bounds__(256) k_prim_all(long long ncell, int nq, Phys ph, const S* __restrict__ U,
                                                   double* __restrict__ Paos, double* __restrict__ T) {
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < ncell;
        c += (long long)gridDim.x * blockDim.x) {
-    double u[8], p[8];
+    double u[10], p[10];
 #pragma unroll
-    for (int q = 0; q < 8; q++) u[q] = (double)U[q * ncell + c];
-    const double te = u2prim_t(ph, u, p);
-#pragma unroll
-    for (int q = 0; q < 8; q++) Paos[c * 8 + q] = p[q];
+    for (int q = 0; q < 10; q++) u[q] = q < nq ? (double)U[q * ncell + c] : 0.0;
+    const double te = u2prim_t<10>(ph, u, p);
+    for (int q = 0; q < nq; q++) Paos[c * nq + q] = p[q];
     if (T) T[c] = te;
   }
 }
@@ -1346,7 +1458,13 @@ unsigned box_tiles(const Geom& g, int kclass, const Shape& sh) {
 template <typename S>
 void launch_xy(hipStream_t s, const Box<S>& b, double dt, int stage, int solver, int flux_cd, int* err) {
   const unsigned n = box_tiles(b.g, KC_XY, Shape{XY_THREADS, 1});
-#define GM_CALL(SV, O, F) { hipLaunchKernelGGL((k_xy<SV, O, F, S>), dim3(n), dim3(XY_THREADS), 0, s, b, dt, err); }
+#define GM_CALL(SV, O, F)                                                                                  \
+  {                                                                                                        \
+    if (b.g.npas == 0) hipLaunchKernelGGL((k_xy<SV, O, F, S, 0>), dim3(n), dim3(XY_THREADS), 0, s, b, dt, err); \
+    else if (b.g.npas == 1)                                                                                \
+      hipLaunchKernelGGL((k_xy<SV, O, F, S, 1>), dim3(n), dim3(XY_THREADS), 0, s, b, dt, err);           \
+    else hipLaunchKernelGGL((k_xy<SV, O, F, S, 2>), dim3(n), dim3(XY_THREADS), 0, s, b, dt, err);       \
+  }
   GM_STAGE_DISPATCH(GM_CALL)
 #undef GM_CALL
 }
@@ -1355,8 +1473,13 @@ template <typename S>
 void launch_z(hipStream_t s, const Box<S>& b, double dt, int stage, int solver, int flux_cd, int* err,
               const Shape& sh) {
   const unsigned n = box_tiles(b.g, KC_COL, sh);
-#define GM_CALL(SV, O, F) \
-  { hipLaunchKernelGGL((k_z<SV, O, F, S>), dim3(n), dim3(sh.block), 0, s, b, dt, sh.kchunk, err); }
+#define GM_CALL(SV, O, F)                                                                                  \
+  {                                                                                                        \
+    if (b.g.npas == 0) hipLaunchKernelGGL((k_z<SV, O, F, S, 0>), dim3(n), dim3(sh.block), 0, s, b, dt, sh.kchunk, err); \
+    else if (b.g.npas == 1)                                                                                \
+      hipLaunchKernelGGL((k_z<SV, O, F, S, 1>), dim3(n), dim3(sh.block), 0, s, b, dt, sh.kchunk, err);    \
+    else hipLaunchKernelGGL((k_z<SV, O, F, S, 2>), dim3(n), dim3(sh.block), 0, s, b, dt, sh.kchunk, err); \
+  }
   GM_STAGE_DISPATCH(GM_CALL)
 #undef GM_CALL
 }
@@ -1368,7 +1491,10 @@ void launch_bfield(hipStream_t s, const Box<S>& b, double dt, int stage, const S
 
 template <typename S>
 void launch_cfl(hipStream_t s, const Box<S>& b, unsigned long long* dtmin, int* err, const Shape& sh) {
-  hipLaunchKernelGGL(k_cfl<S>, dim3(box_tiles(b.g, KC_COL, sh)), dim3(sh.block), 0, s, b, sh.kchunk, dtmin, err);
+  if (b.g.npas == 0)
+    hipLaunchKernelGGL((k_cfl<S, 8>), dim3(box_tiles(b.g, KC_COL, sh)), dim3(sh.block), 0, s, b, sh.kchunk, dtmin, err);
+  else  // EOS_H_RATE reads the first passive scalar
+    hipLaunchKernelGGL((k_cfl<S, 10>), dim3(box_tiles(b.g, KC_COL, sh)), dim3(sh.block), 0, s, b, sh.kchunk, dtmin, err);
 }
 
 template <typename S>
@@ -1382,7 +1508,8 @@ void launch_viscous(hipStream_t s, const Box<S>& b, double eta) {
 template <typename S>
 void launch_cooling(hipStream_t s, const Box<S>& b, const Cooling& cl, double dt) {
   const long long n = (long long)b.g.nx * b.g.ny * b.g.nz;
-  hipLaunchKernelGGL(k_cooling<S>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, b, cl, dt);
+  if (cl.kind == GM_COOL_H) hipLaunchKernelGGL(k_cooling_h<S>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, b, cl, dt);
+  else hipLaunchKernelGGL(k_cooling<S>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, b, cl, dt);
 }
 
 template <typename S>
@@ -1479,18 +1606,18 @@ void launch_ghost(hipStream_t s, const Geom& g, S* U, int axis, int side, int ki
 }
 
 template <typename S>
-void launch_aos_to_soa(hipStream_t s, long long ncell, const double* A, S* D) {
-  hipLaunchKernelGGL(k_aos_to_soa<S>, dim3(grid_for(ncell)), dim3(256), 0, s, ncell, A, D);
+void launch_aos_to_soa(hipStream_t s, long long ncell, const double* A, S* D, int nq) {
+  hipLaunchKernelGGL(k_aos_to_soa<S>, dim3(grid_for(ncell)), dim3(256), 0, s, ncell, nq, A, D);
 }
 
 template <typename S>
-void launch_soa_to_aos(hipStream_t s, long long ncell, const S* D, double* A) {
-  hipLaunchKernelGGL(k_soa_to_aos<S>, dim3(grid_for(ncell)), dim3(256), 0, s, ncell, D, A);
+void launch_soa_to_aos(hipStream_t s, long long ncell, const S* D, double* A, int nq) {
+  hipLaunchKernelGGL(k_soa_to_aos<S>, dim3(grid_for(ncell)), dim3(256), 0, s, ncell, nq, D, A);
 }
 
 template <typename S>
-void launch_prim_all(hipStream_t s, long long ncell, const Phys& ph, const S* U, double* Paos, double* T) {
-  hipLaunchKernelGGL(k_prim_all<S>, dim3(grid_for(ncell)), dim3(256), 0, s, ncell, ph, U, Paos, T);
+void launch_prim_all(hipStream_t s, long long ncell, const Phys& ph, const S* U, double* Paos, double* T, int nq) {
+  hipLaunchKernelGGL(k_prim_all<S>, dim3(grid_for(ncell)), dim3(256), 0, s, ncell, nq, ph, U, Paos, T);
 }
 
 template <typename S>
@@ -1520,9 +1647,9 @@ void launch_divb(hipStream_t s, const Geom& g, const S* U, double dx, double dy,
   template void launch_cfl_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, BoxClock*, int*,  \
                                     const Shape&);                                                                \
   template void launch_ghost<S>(hipStream_t, const Geom&, S*, int, int, int, int);                               \
-  template void launch_aos_to_soa<S>(hipStream_t, long long, const double*, S*);                                  \
-  template void launch_soa_to_aos<S>(hipStream_t, long long, const S*, double*);                                  \
-  template void launch_prim_all<S>(hipStream_t, long long, const Phys&, const S*, double*, double*);              \
+  template void launch_aos_to_soa<S>(hipStream_t, long long, const double*, S*, int);                             \
+  template void launch_soa_to_aos<S>(hipStream_t, long long, const S*, double*, int);                             \
+  template void launch_prim_all<S>(hipStream_t, long long, const Phys&, const S*, double*, double*, int);         \
   template void launch_divb<S>(hipStream_t, const Geom&, const S*, double, double, double, double*);
 GM_INSTANTIATE(double)
 GM_INSTANTIATE(float)

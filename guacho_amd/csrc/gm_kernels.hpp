@@ -18,21 +18,23 @@ struct Geom {
   int bc[6];
   int gmem;           // 1: ghosts are materialised in memory (user boundary
                       // hooks); 0: resolved by boundary mapping
+  int npas;           // passive scalars after the 8 MHD variables (0..2):
+                      // variable q of a cell at q*ncell, q < 8 + npas
 };
 
 // One box (subdomain) as the kernels see it: geometry, physics and its
 // device buffers.  W carries a stage's partial results from the xy-face
 // kernel to the z-face kernel, in FP64 for either storage type: the
 // conserved update after its x and y terms (components 0..4 with flux-CD,
-// 0..7 without) and, with flux-CD, the x/y parts of E1 (component 5) and
-// E2 (component 6).
+// 0..7 without), with flux-CD the x/y parts of E1 (component 5) and E2
+// (component 6), and the passive scalars' partials (components 8..).
 template <typename S>
 struct Box {
   Geom g;
   Phys ph;
   S* U;       // U^n, updated in place by the corrector
   S* Us;      // U* (predictor result)
-  double* W;  // stage partials, 8 x ncell doubles
+  double* W;  // stage partials, (8 + npas) x ncell doubles
   S* E;       // cell-centred E, 3 components
   double dx, dy, dz;
   int slot;  // index of the box's clock in a batch's clock table
@@ -45,7 +47,7 @@ enum : int { SRC_EIGHT_WAVE = 1, SRC_USER = 2 };
 // cooltab(2, 41) (T_i, Lambda_i pairs), the sqrt-law coefficient above
 // 1e8 K and the parameters module's Tempsc, Psc, tsc.
 struct Cooling {
-  int kind;            // 3 (DMC) or 4 (CHIANTI)
+  int kind;            // 1 (H-rate), 3 (DMC) or 4 (CHIANTI)
   double tempsc, psc, tsc, hicoef;
   double tab[82];
 };
@@ -154,11 +156,12 @@ void launch_clock_dt(hipStream_t s, BoxClock* clk, int nbox, int tuning);
 template <typename S>
 void launch_ghost(hipStream_t s, const Geom& g, S* U, int axis, int side, int kind, int layers);
 template <typename S>
-void launch_aos_to_soa(hipStream_t s, long long ncell, const double* A, S* D);
+void launch_aos_to_soa(hipStream_t s, long long ncell, const double* A, S* D, int nq = 8);
 template <typename S>
-void launch_soa_to_aos(hipStream_t s, long long ncell, const S* D, double* A);
+void launch_soa_to_aos(hipStream_t s, long long ncell, const S* D, double* A, int nq = 8);
 template <typename S>
-void launch_prim_all(hipStream_t s, long long ncell, const Phys& ph, const S* U, double* Paos, double* T);
+void launch_prim_all(hipStream_t s, long long ncell, const Phys& ph, const S* U, double* Paos, double* T,
+                     int nq = 8);
 template <typename S>
 void launch_divb(hipStream_t s, const Geom& g, const S* U, double dx, double dy, double dz, double* D);
 

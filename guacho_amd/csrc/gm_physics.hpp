@@ -81,27 +81,35 @@ __device__ __forceinline__ void axis_perm(T v[8]) {
 // recovery E - rho v^2/2 - B^2/2, whose cancellation at low plasma beta the
 // mixed form keeps in double.  One implementation, shared with user plugins
 // (gm_u2prim, gm_eos in guacho_amd_user.h).
-__device__ __forceinline__ void u2prim(const Phys& ph, const double u[8], double p[8]) {
+// NQ = 8 + the passive scalars, whose primitives are their conserved
+// densities (:86-90); EOS_H_RATE reads the first one (the neutral density).
+template <int NQ = 8>
+__device__ __forceinline__ void u2prim(const Phys& ph, const double u[NQ], double p[NQ]) {
   gm_u2prim(ph.cv, ph.mhd, u, p);
-  if (ph.eos == GM_EOS_SINGLE_SPECIE) (void)gm_eos(ph.eos, ph.tempsc, p);
+#pragma unroll
+  for (int a = 8; a < NQ; a++) p[a] = u[a];
+  if (ph.eos == GM_EOS_SINGLE_SPECIE || (NQ > 8 && ph.eos == GM_EOS_H_RATE)) (void)gm_eos(ph.eos, ph.tempsc, p);
 }
 
-// ... and u2prim's temperature T (:90-99).
-__device__ __forceinline__ double u2prim_t(const Phys& ph, const double u[8], double p[8]) {
+// ... and u2prim's temperature T (:90-112).
+template <int NQ = 8>
+__device__ __forceinline__ double u2prim_t(const Phys& ph, const double u[NQ], double p[NQ]) {
   gm_u2prim(ph.cv, ph.mhd, u, p);
-  return gm_eos(ph.eos, ph.tempsc, p);
+#pragma unroll
+  for (int a = 8; a < NQ; a++) p[a] = u[a];
+  return gm_eos(NQ > 8 ? ph.eos : (ph.eos == GM_EOS_H_RATE ? GM_EOS_ADIABATIC : ph.eos), ph.tempsc, p);
 }
 
 // The temperature a post-step module uses: its own Tempsc scaling of p/rho
 // under the adiabatic EOS, u2prim's floored T under EOS_SINGLE_SPECIE (one
 // Tempsc for both: the parameters module's).
 __device__ __forceinline__ double module_temp(const Phys& ph, double t_eos, const double p[8], double tempsc) {
-  return ph.eos == GM_EOS_SINGLE_SPECIE ? t_eos : (p[4] / p[0]) * tempsc;
+  return (ph.eos == GM_EOS_SINGLE_SPECIE || ph.eos == GM_EOS_H_RATE) ? t_eos : (p[4] / p[0]) * tempsc;
 }
 
-// prim2u, src/hydro_core.f90:331-383
-template <typename T>
-__device__ __forceinline__ void prim2u(const Phys& ph, const T p[8], T u[8]) {
+// prim2u, src/hydro_core.f90:331-383 (passive scalars :377-379)
+template <int NQ = 8, typename T>
+__device__ __forceinline__ void prim2u(const Phys& ph, const T p[NQ], T u[NQ]) {
   const T cv = (T)ph.cv;
   u[0] = p[0];
   u[1] = p[0] * p[1];
@@ -113,11 +121,14 @@ __device__ __forceinline__ void prim2u(const Phys& ph, const T p[8], T u[8]) {
   u[5] = p[5];
   u[6] = p[6];
   u[7] = p[7];
+#pragma unroll
+  for (int a = 8; a < NQ; a++) u[a] = p[a];
 }
 
-// prim2f, src/hydro_core.f90:395-476 (x-direction flux)
-template <typename T>
-__device__ __forceinline__ void prim2f(const Phys& ph, const T p[8], T f[8]) {
+// prim2f, src/hydro_core.f90:395-476 (x-direction flux; passive scalars
+// :470-472)
+template <int NQ = 8, typename T>
+__device__ __forceinline__ void prim2f(const Phys& ph, const T p[NQ], T f[NQ]) {
   const T cv = (T)ph.cv;
   const T rv = p[0] * p[1];
   if (ph.mhd) {
@@ -141,6 +152,8 @@ __device__ __forceinline__ void prim2f(const Phys& ph, const T p[8], T f[8]) {
   f[5] = T(0);
   f[6] = p[1] * p[6] - p[5] * p[2];
   f[7] = p[1] * p[7] - p[5] * p[3];
+#pragma unroll
+  for (int a = 8; a < NQ; a++) f[a] = p[a] * p[1];
 }
 
 // csound, src/hydro_core.f90:544-553
@@ -205,10 +218,10 @@ __device__ __forceinline__ T lim_average(int lim, T a, T b) {
 }
 
 // limiter, src/hydro_core.f90:712-731 (in place on pl, pr)
-template <typename T>
-__device__ __forceinline__ void limiter(int lim, const T pll[8], T pl[8], T pr[8], const T prr[8]) {
+template <int NQ = 8, typename T>
+__device__ __forceinline__ void limiter(int lim, const T pll[NQ], T pl[NQ], T pr[NQ], const T prr[NQ]) {
 #pragma unroll
-  for (int q = 0; q < 8; q++) {
+  for (int q = 0; q < NQ; q++) {
     const T dl = pl[q] - pll[q];
     const T dm = pr[q] - pl[q];
     const T dr = prr[q] - pr[q];
@@ -256,15 +269,15 @@ __device__ __forceinline__ T hlld_energy(T cv, const T s[8], T bx) {
 // flux is the same expression, on the same operands, as the reference's
 // branch for that region: UL* :111-156, UR* :159-204, UL** :255-283,
 // UR** :286-314.
-template <typename T>
-__device__ __forceinline__ bool hlld(const Phys& ph, const T pl[8], const T pr[8], T ff[8]) {
+template <int NQ = 8, typename T>
+__device__ __forceinline__ bool hlld(const Phys& ph, const T pl[NQ], const T pr[NQ], T ff[NQ]) {
   using A = Ar<T>;
   const T cv = (T)ph.cv;
   const T csl = cfastX(ph, pl), csr = cfastX(ph, pr);
   const T sr = dmax(pl[1] + csl, pr[1] + csr);
   const T sl = dmin(pl[1] - csl, pr[1] - csr);
-  if (sl > T(0)) { prim2f(ph, pl, ff); return true; }
-  if (sr < T(0)) { prim2f(ph, pr, ff); return true; }
+  if (sl > T(0)) { prim2f<NQ>(ph, pl, ff); return true; }
+  if (sr < T(0)) { prim2f<NQ>(ph, pr, ff); return true; }
 
   const T bx = T(0.5) * (pl[5] + pr[5]);
   const T signbx = sign1(bx);
@@ -290,9 +303,9 @@ __device__ __forceinline__ bool hlld(const Phys& ph, const T pl[8], const T pr[8
   const bool single = (sstl >= T(0)) || (sstr <= T(0));
   if (!single && !(sM >= T(0)) && !(sM <= T(0))) return false;
   const bool left = single ? (sstl >= T(0)) : (sM >= T(0));
-  T s[8];
+  T s[NQ];
 #pragma unroll
-  for (int q = 0; q < 8; q++) s[q] = left ? pl[q] : pr[q];
+  for (int q = 0; q < NQ; q++) s[q] = left ? pl[q] : pr[q];
   const T smu = left ? slmul : srmur;
   const T smsM = left ? slmsM : srmsM;
   const T rhost = left ? rhostl : rhostr;
@@ -331,67 +344,71 @@ __device__ __forceinline__ bool hlld(const Phys& ph, const T pl[8], const T pr[8
   ff[5] = T(0);
   ff[6] = byt * sM - bx * vt;
   ff[7] = bzt * sM - bx * wt;
+  // passive scalars of the flux side, one expression for its star and
+  // double-star regions (hlld.f90:149-151, 197-199, 276-278, 307-309)
+#pragma unroll
+  for (int a = 8; a < NQ; a++) ff[a] = A::div((sM * s[a]) * smu, smsM);
   return true;
 }
 
 // HLL average of prim2fhlle (src/hlle.f90:74-79) and prim2fhll (src/hll.f90:73-78)
-template <typename T>
-__device__ __forceinline__ void hll_average(const Phys& ph, const T pl[8], const T pr[8], T sl, T sr, T ff[8]) {
-  T fl[8], fr[8], ul[8], ur[8];
-  prim2f(ph, pl, fl);
-  prim2f(ph, pr, fr);
-  prim2u(ph, pl, ul);
-  prim2u(ph, pr, ur);
+template <int NQ = 8, typename T>
+__device__ __forceinline__ void hll_average(const Phys& ph, const T pl[NQ], const T pr[NQ], T sl, T sr, T ff[NQ]) {
+  T fl[NQ], fr[NQ], ul[NQ], ur[NQ];
+  prim2f<NQ>(ph, pl, fl);
+  prim2f<NQ>(ph, pr, fr);
+  prim2u<NQ>(ph, pl, ul);
+  prim2u<NQ>(ph, pr, ur);
 #pragma unroll
-  for (int q = 0; q < 8; q++) ff[q] = Ar<T>::div((sr * fl[q] - sl * fr[q]) + (sl * sr) * (ur[q] - ul[q]), sr - sl);
+  for (int q = 0; q < NQ; q++) ff[q] = Ar<T>::div((sr * fl[q] - sl * fr[q]) + (sl * sr) * (ur[q] - ul[q]), sr - sl);
 }
 
 // prim2fhlle, src/hlle.f90:48-83
-template <typename T>
-__device__ __forceinline__ bool hlle(const Phys& ph, const T pl[8], const T pr[8], T ff[8]) {
+template <int NQ = 8, typename T>
+__device__ __forceinline__ bool hlle(const Phys& ph, const T pl[NQ], const T pr[NQ], T ff[NQ]) {
   const T csl = cfastX(ph, pl), csr = cfastX(ph, pr);
   const T sr = dmax(pl[1] + csl, pr[1] + csr);
   const T sl = dmin(pl[1] - csl, pr[1] - csr);
-  if (sl > T(0)) { prim2f(ph, pl, ff); return true; }
-  if (sr < T(0)) { prim2f(ph, pr, ff); return true; }
-  hll_average(ph, pl, pr, sl, sr, ff);
+  if (sl > T(0)) { prim2f<NQ>(ph, pl, ff); return true; }
+  if (sr < T(0)) { prim2f<NQ>(ph, pr, ff); return true; }
+  hll_average<NQ>(ph, pl, pr, sl, sr, ff);
   return true;
 }
 
 // prim2fhll, src/hll.f90:47-82
-template <typename T>
-__device__ __forceinline__ bool hll(const Phys& ph, const T pl[8], const T pr[8], T ff[8]) {
+template <int NQ = 8, typename T>
+__device__ __forceinline__ bool hll(const Phys& ph, const T pl[NQ], const T pr[NQ], T ff[NQ]) {
   const T csl = csound(ph, pl[4], pl[0]), csr = csound(ph, pr[4], pr[0]);
   const T sr = dmax(pl[1] + csl, pr[1] + csr);
   const T sl = dmin(pl[1] - csl, pr[1] - csr);
-  if (sl > T(0)) { prim2f(ph, pl, ff); return true; }
-  if (sr < T(0)) { prim2f(ph, pr, ff); return true; }
-  hll_average(ph, pl, pr, sl, sr, ff);
+  if (sl > T(0)) { prim2f<NQ>(ph, pl, ff); return true; }
+  if (sr < T(0)) { prim2f<NQ>(ph, pr, ff); return true; }
+  hll_average<NQ>(ph, pl, pr, sl, sr, ff);
   return true;
 }
 
 // prim2fhllc, src/hllc.f90:44-140 (passive B)
-template <typename T>
-__device__ __forceinline__ bool hllc(const Phys& ph, const T pl[8], const T pr[8], T ff[8]) {
+template <int NQ = 8, typename T>
+__device__ __forceinline__ bool hllc(const Phys& ph, const T pl[NQ], const T pr[NQ], T ff[NQ]) {
   using A = Ar<T>;
   const T csl = csound(ph, pl[4], pl[0]), csr = csound(ph, pr[4], pr[0]);
   const T sr = dmax(pl[1] + csl, pr[1] + csr);
   const T sl = dmin(pl[1] - csl, pr[1] - csr);
-  if (sl > T(0)) { prim2f(ph, pl, ff); return true; }
-  if (sr < T(0)) { prim2f(ph, pr, ff); return true; }
+  if (sl > T(0)) { prim2f<NQ>(ph, pl, ff); return true; }
+  if (sr < T(0)) { prim2f<NQ>(ph, pr, ff); return true; }
   const T slmul = sl - pl[1], srmur = sr - pr[1];
   const T rholul = pl[0] * pl[1], rhorur = pr[0] * pr[1];
   const T sst = A::div(((srmur * rhorur - slmul * rholul) - pr[4]) + pl[4], srmur * pr[0] - slmul * pl[0]);
   if (!(sst >= T(0)) && !(sst <= T(0))) return false;
   const bool left = (sst >= T(0));
-  T s[8];
+  T s[NQ];
 #pragma unroll
-  for (int q = 0; q < 8; q++) s[q] = left ? pl[q] : pr[q];
+  for (int q = 0; q < NQ; q++) s[q] = left ? pl[q] : pr[q];
   const T smu = left ? slmul : srmur;
   const T sw = left ? sl : sr;
   const T rhost = A::div(s[0] * smu, sw - sst);
   const T ek = (T(0.5) * s[0]) * ((s[1] * s[1] + s[2] * s[2]) + s[3] * s[3]) + (T)ph.cv * s[4];
-  T uuk[8], uu[8];
+  T uuk[NQ], uu[NQ];
   uuk[0] = rhost;
   uuk[1] = rhost * sst;
   uuk[2] = rhost * s[2];
@@ -399,19 +416,21 @@ __device__ __forceinline__ bool hllc(const Phys& ph, const T pl[8], const T pr[8
   uuk[4] = rhost * (A::div(ek, s[0]) + (sst - s[1]) * (sst + A::div(s[4], s[0] * smu)));
 #pragma unroll
   for (int q = 5; q < 8; q++) uuk[q] = ph.pmhd ? A::div(rhost * s[q], s[0]) : T(0);
-  prim2f(ph, s, ff);
-  prim2u(ph, s, uu);
 #pragma unroll
-  for (int q = 0; q < 8; q++) ff[q] = ff[q] + sw * (uuk[q] - uu[q]);
+  for (int a = 8; a < NQ; a++) uuk[a] = A::div(rhost * s[a], s[0]);  // hllc.f90:94-97, 123-126
+  prim2f<NQ>(ph, s, ff);
+  prim2u<NQ>(ph, s, uu);
+#pragma unroll
+  for (int q = 0; q < NQ; q++) ff[q] = ff[q] + sw * (uuk[q] - uu[q]);
   return true;
 }
 
-template <int SOLVER, typename T>
-__device__ __forceinline__ bool riemann(const Phys& ph, const T pl[8], const T pr[8], T ff[8]) {
-  if constexpr (SOLVER == SOLVER_HLLD) return hlld(ph, pl, pr, ff);
-  else if constexpr (SOLVER == SOLVER_HLLE) return hlle(ph, pl, pr, ff);
-  else if constexpr (SOLVER == SOLVER_HLL) return hll(ph, pl, pr, ff);
-  else return hllc(ph, pl, pr, ff);
+template <int SOLVER, int NQ = 8, typename T>
+__device__ __forceinline__ bool riemann(const Phys& ph, const T pl[NQ], const T pr[NQ], T ff[NQ]) {
+  if constexpr (SOLVER == SOLVER_HLLD) return hlld<NQ>(ph, pl, pr, ff);
+  else if constexpr (SOLVER == SOLVER_HLLE) return hlle<NQ>(ph, pl, pr, ff);
+  else if constexpr (SOLVER == SOLVER_HLL) return hll<NQ>(ph, pl, pr, ff);
+  else return hllc<NQ>(ph, pl, pr, ff);
 }
 
 }  // namespace gm

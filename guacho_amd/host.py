@@ -56,7 +56,8 @@ class Parameters:
     bc_user: bool = False
     cooling: int = _capi.GM_COOL_NONE                # COOL_NONE / COOL_DMC / COOL_CHI
     th_cond: int = _capi.GM_TC_OFF                   # TC_OFF / TC_ISOTROPIC / TC_ANISOTROPIC
-    eq_of_state: int = _capi.GM_EOS_ADIABATIC        # EOS_ADIABATIC / EOS_SINGLE_SPECIE
+    eq_of_state: int = _capi.GM_EOS_ADIABATIC        # EOS_ADIABATIC / EOS_SINGLE_SPECIE / EOS_H_RATE
+    npas: int = 0                                    # passive scalars (PASSIVES builds)
     tc_saturation: bool = False
     Tempsc: float = 1.0                              # parameters module scalings
     Psc: float = 1.0
@@ -111,7 +112,7 @@ def initmain(prm: Parameters) -> Globals:
     dx = prm.xmax / prm.nxtot
     dy = prm.ymax / prm.nytot
     dz = prm.zmax / prm.nztot
-    u = np.zeros((8, prm.nxtot + 4, prm.nytot + 4, prm.nztot + 4), dtype=np.float64, order="F")
+    u = np.zeros((8 + prm.npas, prm.nxtot + 4, prm.nytot + 4, prm.nztot + 4), dtype=np.float64, order="F")
     return Globals(u=u, dx=dx, dy=dy, dz=dz, time=0.0, currentIteration=1, tprint=prm.dtprint, itprint=0)
 
 
@@ -120,7 +121,7 @@ def device_params(prm: Parameters, g: Globals, device: int = 0) -> _capi.GmParam
                              riemann_solver=prm.riemann_solver, slope_limiter=prm.slope_limiter,
                              bc=prm.bc, mhd=prm.mhd, pmhd=prm.pmhd, enable_flux_cd=prm.enable_flux_cd,
                              eight_wave=prm.eight_wave, precision=prm.precision, device=device,
-                             eta=prm.eta)
+                             eta=prm.eta, npas=prm.npas)
 
 
 def timestep_clamp(current_iter: int, n_iter: int, cfl: float, dtp: float, current_time: float,
@@ -187,7 +188,7 @@ def run(prm: Parameters, g: Globals, device: int = 0,
         if prm.bc_user:
             dev.set_user_bc(*user_bc)
         if prm.cooling != _capi.GM_COOL_NONE:
-            if cooltab is None:
+            if cooltab is None and prm.cooling != _capi.GM_COOL_H:
                 raise ValueError("cooling needs its table: pass cooltab (read_cooling_table)")
             dev.set_cooling(prm.cooling, cooltab, prm.Tempsc, prm.Psc, prm.tsc)
         if prm.th_cond != _capi.GM_TC_OFF:

@@ -56,10 +56,9 @@ contains
     p%cv = cv
     p%gamma = gamma
     p%eta = eta
-    if ((cooling /= COOL_NONE .and. cooling /= COOL_DMC .and. cooling /= COOL_CHI) .or. dif_rad &
-        .or. (eq_of_state /= EOS_ADIABATIC .and. eq_of_state /= EOS_SINGLE_SPECIE)) then
-      print '(a)', 'guacho_amd: cooling other than DMC/CHIANTI tables, diffuse radiation and '// &
-                   'the EOS that need passive scalars are not built'
+    if ((cooling /= COOL_NONE .and. cooling /= COOL_H .and. cooling /= COOL_DMC .and. cooling /= COOL_CHI) &
+        .or. dif_rad .or. eq_of_state == EOS_CHEM) then
+      print '(a)', 'guacho_amd: BBC and chemistry cooling, diffuse radiation and the chemistry EOS are not built'
       stop 1
     end if
     if (radiation_pressure) then
@@ -69,6 +68,15 @@ contains
     call gm_check(gm_create(p, gm_ctx), 'gm_create')
     call gm_check(gm_set_eos(gm_ctx, int(eq_of_state, c_int), real(Tempsc, c_double)), 'gm_set_eos')
     if (cooling == COOL_DMC .or. cooling == COOL_CHI) call load_cooling()
+    if (cooling == COOL_H) then
+      ! coolingh takes no table; a zero table stands for the unused argument
+      block
+        real(c_double) :: notab(2, 41)
+        notab = 0.0_c_double
+        call gm_check(gm_set_cooling(gm_ctx, int(COOL_H, c_int), notab, real(Tempsc, c_double), &
+                                     real(Psc, c_double), real(tsc, c_double)), 'gm_set_cooling')
+      end block
+    end if
     if (th_cond /= TC_OFF) then
       tp%kind = th_cond
       tp%saturation = merge(1, 0, tc_saturation)

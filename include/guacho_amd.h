@@ -63,7 +63,9 @@ extern "C" {
  * computed by initmain (src/init.f90:119-122). */
 typedef struct gm_params {
   int32_t nx, ny, nz;     /* interior cells of the subdomain */
-  int32_t neq;            /* must be 8 (BFIELD, no passives) */
+  int32_t neq;            /* 8 (BFIELD) plus the passive scalars (PASSIVES
+                             builds: npas = 1 or 2), so 8, 9 or 10;
+                             batches take 8 only */
   int32_t nghost;         /* must be 2 */
   int32_t riemann_solver; /* GM_SOLVER_* */
   int32_t slope_limiter;  /* GM_LIMITER_* */
@@ -181,12 +183,17 @@ int gm_set_user_bc(gm_ctx *ctx, const char *code_object, const char *kernel);
 
 /* ---- post-step physics ------------------------------------------------ */
 #define GM_COOL_NONE 0   /* constants.f90:72-77 */
+#define GM_COOL_H    1   /* H-rate cooling: needs neq >= 9 (the neutral density) */
 #define GM_COOL_DMC  3
 #define GM_COOL_CHI  4
 
-/* Table cooling at the end of every gm_tstep, after the corrector and before
- * the boundary fill: coolingdmc (src/cooling_dmc.f90:115-156) for
- * GM_COOL_DMC, coolingchi (src/cooling_chi.f90) for GM_COOL_CHI.  cooltab
+/* Cooling at the end of every gm_tstep, after the corrector and before the
+ * boundary fill: coolingdmc (src/cooling_dmc.f90:115-156) for GM_COOL_DMC,
+ * coolingchi (src/cooling_chi.f90) for GM_COOL_CHI, coolingh
+ * (src/cooling_h.f90:41-67: the hydrogen rate equation on the neutral
+ * density, the first passive scalar, and the parametrised loss ALOSS) for
+ * GM_COOL_H, which takes no table (tempsc and tsc as below; psc unused) and
+ * needs neq >= 9.  cooltab
  * holds the reference's cooltab(2, 41) as its read_table leaves it (column
  * major: T_1, Lambda_1, T_2, Lambda_2, ...; for DMC already 10**a,
  * 10**(-b)); tempsc, psc, tsc are the parameters module's Tempsc, Psc and
@@ -225,6 +232,7 @@ int gm_conduction_substeps(const gm_ctx *ctx, int32_t *nsteps);
 
 #define GM_EOS_ADIABATIC     1   /* constants.f90 EOS_ADIABATIC */
 #define GM_EOS_SINGLE_SPECIE 2   /* constants.f90 EOS_SINGLE_SPECIE */
+#define GM_EOS_H_RATE        3   /* constants.f90 EOS_H_RATE (needs npas >= 1) */
 
 /* Equation of state of every primitive recovery, u2prim
  * (src/hydro_core.f90:46-129) with eq_of_state and Tempsc of the parameters

@@ -22,7 +22,7 @@
  *     int i, j, k;
  *     if (!gm_user_cell(a, blockIdx.x * 256 + threadIdx.x, i, j, k)) return;
  *     const int64_t c = gm_user_index(a, i, j, k);
- *     double u[8], pp[8];
+ *     double u[GM_USER_MAXVAR], pp[GM_USER_MAXVAR];
  *     gm_user_load(a, c, u);
  *     gm_user_u2prim(a, u, pp);              // primit(:, i, j, k)
  *     a.s[2 * a.ncell + c] -= pp[0] * g;      // s(3) = s(3) - rho g
@@ -30,7 +30,8 @@
  *
  * Indices are the reference's: interior cells i = 1..nx (likewise j, k),
  * two ghost layers (-1..n+2).  Variables are 0-based here: q = 0..7 holds
- * the reference's 1..8 (rho, rho u, rho v, rho w, E, Bx, By, Bz).
+ * the reference's 1..8 (rho, rho u, rho v, rho w, E, Bx, By, Bz), q = 8..
+ * the passive scalars (a.npas of them, the reference's neqdyn+1..neq).
  */
 #ifndef GUACHO_AMD_USER_H
 #define GUACHO_AMD_USER_H
@@ -55,7 +56,7 @@ typedef struct gm_user_args {
   double *s;               /* sources: 8 x ncell, zero on entry (interior cells
                               are read back); hooks: NULL */
   int32_t eos;             /* eq_of_state (gm_set_eos); 0 reads as adiabatic */
-  int32_t pad;
+  int32_t npas;            /* passive scalars after the 8 MHD variables */
   double tempsc;           /* Tempsc of the equation of state */
   int64_t reserved[2];
 } gm_user_args;
@@ -99,8 +100,10 @@ __device__ __forceinline__ void gm_user_set(const gm_user_args &a, int64_t c, in
   else ((double *)a.u)[q * a.ncell + c] = v;
 }
 
-__device__ __forceinline__ void gm_user_load(const gm_user_args &a, int64_t c, double u[8]) {
-  for (int q = 0; q < 8; q++) u[q] = gm_user_get(a, c, q);
+/* The 8 + a.npas variables of cell c (at most GM_USER_MAXVAR). */
+#define GM_USER_MAXVAR 10
+__device__ __forceinline__ void gm_user_load(const gm_user_args &a, int64_t c, double *u) {
+  for (int q = 0; q < 8 + a.npas; q++) u[q] = gm_user_get(a, c, q);
 }
 
 /* u2prim of the reference (src/hydro_core.f90:46-129, EOS_ADIABATIC):
@@ -125,21 +128,32 @@ __device__ __forceinline__ void gm_u2prim(double cv, int mhd, const double u[8],
 }
 
 /* The equation of state's temperature and pressure (src/hydro_core.f90:
- * 90-99): eos 2 (EOS_SINGLE_SPECIE) floors T at 1 and sets p = rho T /
- * Tempsc; any other eos is adiabatic, T = (p / rho) Tempsc.  Returns T. */
-__device__ __forceinline__ double gm_eos(int eos, double tempsc, double p[8]) {
-  if (eos == 2) {
-    const double x = (p[4] / p[0]) * tempsc;
+ * 90-112): eos 2 (EOS_SINGLE_SPECIE) floors T at 1 and sets p = rho T /
+ * Tempsc; eos 3 (EOS_H_RATE, p[8] the neutral density, the first passive
+ * scalar) does the same with the particle density 2 rho - p[8] (floored at
+ * 1e-15) in place of rho; any other eos is adiabatic, T = (p / rho) Tempsc.
+ * Returns T. */
+__device__ __forceinline__ double gm_eos(int eos, double tempsc, double *p) {
+  if (eos == 2 || eos == 3) {
+    double den = p[0];
+    if (eos == 3) {
+      den = 2.0 * p[0] - p[8];
+      den = (den > 1e-15) ? den : 1e-15;
+    }
+    const double x = (p[4] / den) * tempsc;
     const double t = (1.0 > x) ? 1.0 : x;
-    p[4] = (p[0] * t) / tempsc;
+    p[4] = (den * t) / tempsc;
     return t;
   }
   return (p[4] / p[0]) * tempsc;
 }
 
-__device__ __forceinline__ void gm_user_u2prim(const gm_user_args &a, const double u[8], double p[8]) {
+/* u (8 + a.npas values, as gm_user_load gives them) -> primitives p (as
+ * many), passive scalars copied, the equation of state applied. */
+__device__ __forceinline__ void gm_user_u2prim(const gm_user_args &a, const double *u, double *p) {
   gm_u2prim(a.cv, a.mhd, u, p);
-  if (a.eos == 2) (void)gm_eos(a.eos, a.tempsc, p);
+  for (int q = 8; q < 8 + a.npas; q++) p[q] = u[q];
+  if (a.eos == 2 || (a.eos == 3 && a.npas > 0)) (void)gm_eos(a.eos, a.tempsc, p);
 }
 
 #endif /* __HIPCC__ */

@@ -15,20 +15,23 @@ def oracle_config(params: _capi.GmParams, cfl: float = 0.4, user_src: bool = Fal
                   bc=list(params.bc), mhd=bool(params.mhd), pmhd=bool(params.pmhd),
                   enable_flux_cd=bool(params.enable_flux_cd), cfl=cfl,
                   eight_wave=bool(params.eight_wave), user_src=user_src, user_bc=user_bc, eta=params.eta,
-                  eq_of_state=eos[0], tempsc=eos[1])
+                  eq_of_state=eos[0], tempsc=eos[1], npas=params.neq - 8)
 
 
-def random_state(rng, nx, ny, nz, cv=1.5, contrast=0.3, vmax=1.5, bmax=1.0):
+def random_state(rng, nx, ny, nz, cv=1.5, contrast=0.3, vmax=1.5, bmax=1.0, npas=0):
     shape = (nx + 4, ny + 4, nz + 4)
     rho = rng.uniform(1 - contrast, 1 + contrast, shape)
     v = rng.uniform(-vmax, vmax, (3,) + shape)
     B = rng.uniform(-bmax, bmax, (3,) + shape)
     p = rng.uniform(1 - contrast, 1 + contrast, shape)
-    u = np.zeros((8,) + shape, order="F")
+    u = np.zeros((8 + npas,) + shape, order="F")
     u[0] = rho
     u[1:4] = rho * v
     u[4] = 0.5 * rho * (v ** 2).sum(0) + cv * p + 0.5 * (B ** 2).sum(0)
     u[5:8] = B
+    if npas:
+        # passive densities below rho (a neutral density under EOS_H_RATE)
+        u[8:] = rng.uniform(0.05, 0.95, (npas,) + shape) * rho
     return u
 
 

@@ -10,7 +10,7 @@ extern "C" __global__ void __launch_bounds__(256) gm_test_gravity(gm_user_args a
   if (!gm_user_cell(a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
   const int64_t c = gm_user_index(a, i, j, k);
   const int64_t N = a.ncell;
-  double u[8], pp[8];
+  double u[GM_USER_MAXVAR], pp[GM_USER_MAXVAR];
   gm_user_load(a, c, u);
   gm_user_u2prim(a, u, pp);
   const double g0 = 0.5, gm = 0.02, soft = 0.01;
