@@ -95,6 +95,9 @@ def algorithmic_bytes(cls: str, n: int) -> float:
         "z_corrector": 176.0 * N,
         # read B (3) and E (3); write B (3)
         "bfield": 72.0 * N,
+        # the corrector's field update plus the CFL candidates of the new
+        # state: also read its hydro part (5)
+        "bfield_cfl": 112.0 * N,
         # read U (8)
         "cfl_dt": 64.0 * N,
     }.get(cls, 0.0)

@@ -23,8 +23,9 @@ namespace {
 
 const char* kClassNames[GM_NKCLASS] = {"xy_predictor", "xy_corrector", "z_predictor", "z_corrector",
                                        "bfield", "cfl_dt", "ghost", "layout", "sources", "cooling",
-                                       "conduction", "viscous"};
-enum { KC_XY1, KC_XY2, KC_Z1, KC_Z2, KC_BFIELD, KC_CFL, KC_GHOST, KC_LAYOUT, KC_SOURCES, KC_COOLING, KC_CONDUCTION, KC_VISCOUS };
+                                       "conduction", "viscous", "bfield_cfl"};
+enum { KC_XY1, KC_XY2, KC_Z1, KC_Z2, KC_BFIELD, KC_CFL, KC_GHOST, KC_LAYOUT, KC_SOURCES, KC_COOLING, KC_CONDUCTION, KC_VISCOUS,
+       KC_BFIELD_CFL };
 
 thread_local std::string g_create_error;
 
@@ -63,6 +64,9 @@ struct gm_ctx {
   Scalars* dscal = nullptr;  // device scalars
   Scalars* hscal = nullptr;  // pinned host mirror
   bool ghosts_dirty = true;
+  // the last gm_tstep reduced the CFL minimum of its result into hscal and
+  // nothing has changed U's interior or the EOS since
+  bool cfl_ready = false;
   double time = 0.0;                    // simulation time the user hooks see
   hipModule_t usrc_mod = nullptr;       // user source plugin
   hipFunction_t usrc_fn = nullptr;
@@ -260,6 +264,7 @@ void fill_ghost_t(gm_ctx* c, S* U, int layers) {
 }
 
 void fill_ghost1(gm_ctx* c) {
+  if (c->ubc_fn) c->cfl_ready = false;  // the hook may write inside the box
   if (c->esize == 8) fill_ghost_t<double>(c, (double*)c->U, 1);
   else fill_ghost_t<float>(c, (float*)c->U, 1);
   c->ghosts_dirty = false;
@@ -388,7 +393,18 @@ void tstep_t(gm_ctx* c, double dt) {
   if (b.g.gmem) fill_ghost_t<S>(c, b.Us, 2);
   timed(c, KC_XY2, [&] { gm::launch_xy(s, b, dt, 2, solver, fcd, err); });
   timed(c, KC_Z2, [&] { gm::launch_z(s, b, dt, 2, solver, fcd, err, sh); });
-  if (fcd) timed(c, KC_BFIELD, [&] { gm::launch_bfield(s, b, dt, 2, sh); });
+  // when nothing after the field update touches U's interior (no sources,
+  // viscosity, cooling, conduction, or boundary hook that may write inside
+  // the box), the next get_timestep's minimum is reduced in the same pass
+  const bool fuse_cfl = fcd && !b.src && p.eta == 0.0 && !c->cool.kind && !c->cond.kind && !c->ubc_fn;
+  if (fuse_cfl) {
+    hipLaunchKernelGGL(k_init_dtmin, dim3(1), dim3(1), 0, s, c->dscal);
+    timed(c, KC_BFIELD_CFL, [&] { gm::launch_bfield_cfl(s, b, dt, &c->dscal->dtmin, err, sh); });
+    (void)hipMemcpyAsync(c->hscal, c->dscal, sizeof(Scalars), hipMemcpyDeviceToHost, s);
+    c->cfl_ready = true;
+  } else if (fcd) {
+    timed(c, KC_BFIELD, [&] { gm::launch_bfield(s, b, dt, 2, sh); });
+  }
   stage_sources_t(c, b, dt, 2);
   // viscous_copy (src/hydro_solver.f90:188); with eta = 0 the identity on
   // values, so no pass
@@ -495,6 +511,7 @@ int gm_upload_u(gm_ctx* c, const double* u) {
   if (!u) return fail(c, GM_ERR_ARG, "u is NULL");
   int rc = set_device(c);
   if (rc) return rc;
+  c->cfl_ready = false;
   const int nq = c->p.neq;
   const size_t n8 = (size_t)c->g.ncell * nq * sizeof(double);
   GM_HIP(c, hipMemcpyAsync(c->W, u, n8, hipMemcpyHostToDevice, c->stream));
@@ -512,6 +529,7 @@ int gm_boundary_u(gm_ctx* c) {
   if (!c) return GM_ERR_ARG;
   int rc = set_device(c);
   if (rc) return rc;
+  c->cfl_ready = false;  // a boundary hook may write inside the box
   fill_ghost1(c);
   GM_HIP(c, hipGetLastError());
   return GM_OK;
@@ -574,11 +592,14 @@ int gm_cfl_dtp(gm_ctx* c, double* dtp) {
   if (!dtp) return fail(c, GM_ERR_ARG, "dtp is NULL");
   int rc = set_device(c);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_init_dtmin, dim3(1), dim3(1), 0, c->stream, c->dscal);
-  if (c->esize == 8) cfl_t<double>(c);
-  else cfl_t<float>(c);
-  GM_HIP(c, hipGetLastError());
-  GM_HIP(c, hipMemcpyAsync(c->hscal, c->dscal, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
+  if (!c->cfl_ready) {
+    hipLaunchKernelGGL(k_init_dtmin, dim3(1), dim3(1), 0, c->stream, c->dscal);
+    if (c->esize == 8) cfl_t<double>(c);
+    else cfl_t<float>(c);
+    GM_HIP(c, hipGetLastError());
+    GM_HIP(c, hipMemcpyAsync(c->hscal, c->dscal, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
+  }
+  c->cfl_ready = false;
   GM_HIP(c, hipStreamSynchronize(c->stream));
   const unsigned long long bits = c->hscal->dtmin;
   std::memcpy(dtp, &bits, sizeof(double));
@@ -595,6 +616,7 @@ int gm_tstep(gm_ctx* c, double dt) {
     for (int f = 0; f < 6; f++)
       if (c->p.bc[f] == GM_BC_OTHER)
         return fail(c, GM_ERR_CONFIG, "BC_OTHER faces need a user boundary plugin (gm_set_user_bc)");
+  c->cfl_ready = false;
   if (c->esize == 8) tstep_t<double>(c, dt);
   else tstep_t<float>(c, dt);
   c->ghosts_dirty = c->ubc_fn == nullptr && !c->cond.kind;  // else the step ends with boundaryI
@@ -768,6 +790,7 @@ int gm_set_eos(gm_ctx* c, int32_t eos, double tempsc) {
   if (!(tempsc > 0) || !std::isfinite(tempsc)) return fail(c, GM_ERR_CONFIG, "Tempsc must be positive and finite");
   c->ph.eos = eos;
   c->ph.tempsc = tempsc;
+  c->cfl_ready = false;
   return GM_OK;
 }
 

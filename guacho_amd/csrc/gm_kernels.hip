@@ -618,8 +618,15 @@ __device__ __forceinline__ int e_nbr(int i, int n, int d, int bc) {
   return 0;  // closed / outflow: ghost takes the edge cell's value
 }
 
-template <typename S>
-__device__ __forceinline__ void bfield_body(const Box<S>& b, unsigned lb, double dt, int stage, int kchunk) {
+template <int NQ>
+__device__ __forceinline__ double cfl_cell(const Phys& ph, double dx, double dy, double dz, const double* u,
+                                           double dtp);
+
+// CFL = 1 (corrector only): also get_timestep's candidate of each updated
+// cell into *dtp, non-finite values into *bad
+template <typename S, int CFL = 0, int NQ = 8>
+__device__ __forceinline__ void bfield_body(const Box<S>& b, unsigned lb, double dt, int stage, int kchunk,
+                                            double* dtp = nullptr, bool* bad = nullptr) {
   const Geom& g = b.g;
   int t, kc;
   if (!tile_item(lb, tiles_of((unsigned)(g.nx * g.ny)), (unsigned)(g.nx * g.ny), t, kc)) return;
@@ -654,6 +661,20 @@ __device__ __forceinline__ void bfield_body(const Box<S>& b, unsigned lb, double
     Uout[5 * N + c] = (S)b5;
     Uout[6 * N + c] = (S)b6;
     Uout[7 * N + c] = (S)b7;
+    if (CFL) {
+      // this cell of U^{n+1} is final: its hydro part and passives were
+      // written by the z kernel, its field just now (as stored, in S)
+      double u[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        u[q] = q == 5   ? (double)(S)b5
+               : q == 6 ? (double)(S)b6
+               : q == 7 ? (double)(S)b7
+                        : (q < 8 + g.npas ? (double)Uout[q * N + c] : 0.0);
+        *bad |= !__builtin_isfinite(u[q]);
+      }
+      *dtp = cfl_cell<NQ>(b.ph, b.dx, b.dy, b.dz, u, *dtp);
+    }
     e1m = e1c;
     e2m = e2c;
     e1c = e1p;
@@ -1141,44 +1162,37 @@ __device__ __forceinline__ double wave_min(double v) {
   return v;
 }
 
-template <typename S, int NQ = 8>
-__device__ __forceinline__ void cfl_body(const Box<S>& b, unsigned lb, int kchunk,
-                                         unsigned long long* __restrict__ dtmin, int* __restrict__ err) {
-  __shared__ double red[4];
-  const Geom& g = b.g;
-  const Phys& ph = b.ph;
-  double dtp = 1e30;
-  bool bad = false;
-  int t, kc;
-  if (tile_item(lb, tiles_of((unsigned)(g.nx * g.ny)), (unsigned)(g.nx * g.ny), t, kc)) {
-    const int i = t % g.nx + 1, j = t / g.nx + 1;
-    const int k0 = kc * kchunk + 1;
-    const int k1 = min(k0 + kchunk - 1, g.nz);
-    for (int k = k0; k <= k1; k++) {
-      const long long c = cidx(g, i, j, k);
-      double u[NQ], p[NQ];
-#pragma unroll
-      for (int q = 0; q < NQ; q++) {
-        u[q] = q < 8 + g.npas ? (double)b.U[q * g.ncell + c] : 0.0;
-        bad |= !__builtin_isfinite(u[q]);
-      }
-      u2prim<NQ>(ph, u, p);
-      if (ph.mhd) {
-        const double b2 = (p[5] * p[5] + p[6] * p[6]) + p[7] * p[7];
-        const double cx = cfast1(ph.gamma, p[4], p[0], b2, p[5]);
-        const double cy = cfast1(ph.gamma, p[4], p[0], b2, p[6]);
-        const double cz = cfast1(ph.gamma, p[4], p[0], b2, p[7]);
-        dtp = dmin(dtp, b.dx / (__builtin_fabs(p[1]) + cx));
-        dtp = dmin(dtp, b.dy / (__builtin_fabs(p[2]) + cy));
-        dtp = dmin(dtp, b.dz / (__builtin_fabs(p[3]) + cz));
-      } else {
-        const double cs = csound(ph, p[4], p[0]);
-        dtp = dmin(dtp, b.dx / (__builtin_fabs(p[1]) + cs));
-        dtp = dmin(dtp, b.dy / (__builtin_fabs(p[2]) + cs));
-        dtp = dmin(dtp, b.dz / (__builtin_fabs(p[3]) + cs));
-      }
-    }
+// get_timestep's candidate of one interior cell (src/hydro_core.f90:641-672):
+// min over the axes of d / (|v| + c), c the fast speed along the axis (MHD)
+// or the sound speed
+template <int NQ>
+__device__ __forceinline__ double cfl_cell(const Phys& ph, double dx, double dy, double dz, const double* u,
+                                           double dtp) {
+  double p[NQ];
+  u2prim<NQ>(ph, u, p);
+  if (ph.mhd) {
+    const double b2 = (p[5] * p[5] + p[6] * p[6]) + p[7] * p[7];
+    const double cx = cfast1(ph.gamma, p[4], p[0], b2, p[5]);
+    const double cy = cfast1(ph.gamma, p[4], p[0], b2, p[6]);
+    const double cz = cfast1(ph.gamma, p[4], p[0], b2, p[7]);
+    dtp = dmin(dtp, dx / (__builtin_fabs(p[1]) + cx));
+    dtp = dmin(dtp, dy / (__builtin_fabs(p[2]) + cy));
+    dtp = dmin(dtp, dz / (__builtin_fabs(p[3]) + cz));
+  } else {
+    const double cs = csound(ph, p[4], p[0]);
+    dtp = dmin(dtp, dx / (__builtin_fabs(p[1]) + cs));
+    dtp = dmin(dtp, dy / (__builtin_fabs(p[2]) + cs));
+    dtp = dmin(dtp, dz / (__builtin_fabs(p[3]) + cs));
   }
+  return dtp;
+}
+
+// the workgroup's minimum into *dtmin (positive doubles order as their bit
+// patterns); a non-finite conserved value sets bit 2 of *err.  Every thread
+// of the workgroup (at most 256) calls it.
+__device__ __forceinline__ void cfl_reduce(double dtp, bool bad, unsigned long long* __restrict__ dtmin,
+                                           int* __restrict__ err) {
+  __shared__ double red[4];
   if (__any(bad)) {
     if ((threadIdx.x & 63) == 0) atomicOr(err, 2);
   }
@@ -1191,6 +1205,31 @@ __device__ __forceinline__ void cfl_body(const Box<S>& b, unsigned lb, int kchun
     for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = dmin(m, red[w]);
     atomicMin(dtmin, (unsigned long long)__double_as_longlong(m));
   }
+}
+
+template <typename S, int NQ = 8>
+__device__ __forceinline__ void cfl_body(const Box<S>& b, unsigned lb, int kchunk,
+                                         unsigned long long* __restrict__ dtmin, int* __restrict__ err) {
+  const Geom& g = b.g;
+  double dtp = 1e30;
+  bool bad = false;
+  int t, kc;
+  if (tile_item(lb, tiles_of((unsigned)(g.nx * g.ny)), (unsigned)(g.nx * g.ny), t, kc)) {
+    const int i = t % g.nx + 1, j = t / g.nx + 1;
+    const int k0 = kc * kchunk + 1;
+    const int k1 = min(k0 + kchunk - 1, g.nz);
+    for (int k = k0; k <= k1; k++) {
+      const long long c = cidx(g, i, j, k);
+      double u[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        u[q] = q < 8 + g.npas ? (double)b.U[q * g.ncell + c] : 0.0;
+        bad |= !__builtin_isfinite(u[q]);
+      }
+      dtp = cfl_cell<NQ>(b.ph, b.dx, b.dy, b.dz, u, dtp);
+    }
+  }
+  cfl_reduce(dtp, bad, dtmin, err);
 }
 
 // ---------------------------------------------------------------------------
@@ -1213,6 +1252,19 @@ __global__ void __launch_bounds__(256) k_z(Box<S> b, double dt, int kchunk, int*
 template <typename S>
 __global__ void __launch_bounds__(256) k_bfield(Box<S> b, double dt, int stage, int kchunk) {
   bfield_body(b, xcd_logical_block(), dt, stage, kchunk);
+}
+
+// the corrector's field update with the next step's CFL minimum folded in:
+// U^{n+1} is complete cell by cell once its field is written, so when
+// nothing after the field update changes U's interior the reduction needs
+// no pass of its own over U
+template <typename S, int NQ>
+__global__ void __launch_bounds__(256) k_bfield_cfl(Box<S> b, double dt, int kchunk, unsigned long long* dtmin,
+                                                    int* err) {
+  double dtp = 1e30;
+  bool bad = false;
+  bfield_body<S, 1, NQ>(b, xcd_logical_block(), dt, 2, kchunk, &dtp, &bad);
+  cfl_reduce(dtp, bad, dtmin, err);
 }
 
 template <typename S, int NQ>
@@ -1490,6 +1542,17 @@ void launch_bfield(hipStream_t s, const Box<S>& b, double dt, int stage, const S
 }
 
 template <typename S>
+void launch_bfield_cfl(hipStream_t s, const Box<S>& b, double dt, unsigned long long* dtmin, int* err,
+                       const Shape& sh) {
+  if (b.g.npas == 0)
+    hipLaunchKernelGGL((k_bfield_cfl<S, 8>), dim3(box_tiles(b.g, KC_COL, sh)), dim3(sh.block), 0, s, b, dt,
+                       sh.kchunk, dtmin, err);
+  else
+    hipLaunchKernelGGL((k_bfield_cfl<S, 10>), dim3(box_tiles(b.g, KC_COL, sh)), dim3(sh.block), 0, s, b, dt,
+                       sh.kchunk, dtmin, err);
+}
+
+template <typename S>
 void launch_cfl(hipStream_t s, const Box<S>& b, unsigned long long* dtmin, int* err, const Shape& sh) {
   if (b.g.npas == 0)
     hipLaunchKernelGGL((k_cfl<S, 8>), dim3(box_tiles(b.g, KC_COL, sh)), dim3(sh.block), 0, s, b, sh.kchunk, dtmin, err);
@@ -1631,6 +1694,7 @@ void launch_divb(hipStream_t s, const Geom& g, const S* U, double dx, double dy,
   template void launch_z<S>(hipStream_t, const Box<S>&, double, int, int, int, int*, const Shape&);               \
   template void launch_bfield<S>(hipStream_t, const Box<S>&, double, int, const Shape&);                          \
   template void launch_cfl<S>(hipStream_t, const Box<S>&, unsigned long long*, int*, const Shape&);               \
+  template void launch_bfield_cfl<S>(hipStream_t, const Box<S>&, double, unsigned long long*, int*, const Shape&); \
   template void launch_sources<S>(hipStream_t, const Box<S>&, double, int, const Shape&);                         \
   template void launch_cooling<S>(hipStream_t, const Box<S>&, const Cooling&, double);                            \
   template void launch_viscous<S>(hipStream_t, const Box<S>&, double);                                            \

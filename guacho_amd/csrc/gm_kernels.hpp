@@ -103,6 +103,11 @@ template <typename S>
 void launch_bfield(hipStream_t s, const Box<S>& b, double dt, int stage, const Shape& sh);
 template <typename S>
 void launch_cfl(hipStream_t s, const Box<S>& b, unsigned long long* dtmin, int* err, const Shape& sh);
+// the corrector's field update (stage 2 of launch_bfield) plus launch_cfl's
+// reduction over the updated state; *dtmin must hold 1e30 beforehand
+template <typename S>
+void launch_bfield_cfl(hipStream_t s, const Box<S>& b, double dt, unsigned long long* dtmin, int* err,
+                       const Shape& sh);
 // cooling pass on U^{n+1} (src/hydro_solver.f90:206-210) with the step's dt
 // viscous_copy with eta != 0 (W as scratch), after the corrector
 template <typename S>

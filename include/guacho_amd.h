@@ -123,7 +123,11 @@ int gm_boundary_u(gm_ctx *ctx);
  * dz/(|vz|+cfz) (src/hydro_core.f90:642-675; csound for pmhd).  The host
  * applies the CFL ramp, the MIN allreduce and the dump clamp of
  * hydro_core.f90:677-694.  Returns GM_ERR_NONFINITE if any interior cell
- * holds a non-finite value.  Synchronizes. */
+ * holds a non-finite value.  Synchronizes.  Right after a gm_tstep with
+ * flux-CD and no sources, viscosity, cooling, conduction or boundary hook,
+ * the minimum was already reduced by that step's field update and this
+ * call only waits for it; any upload, ghost fill with a hook or EOS change
+ * in between makes it reduce over the stored state again. */
 int gm_cfl_dtp(gm_ctx *ctx, double *dtp);
 
 /* Advance the state by dt: tstep (src/hydro_solver.f90:134-229) with the
@@ -144,7 +148,7 @@ int gm_download_divb(gm_ctx *ctx, double *divb);
  * kernel on the context's stream.  gm_kernel_times returns, per kernel
  * class, the accumulated milliseconds and launch count since the last
  * reset, and the kernel names (static strings). */
-#define GM_NKCLASS 12
+#define GM_NKCLASS 13
 int gm_set_timing(gm_ctx *ctx, int enable);
 int gm_kernel_times(gm_ctx *ctx, double *ms, int64_t *launches, const char **names);
 
