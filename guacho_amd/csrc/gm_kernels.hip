@@ -1175,14 +1175,14 @@ __device__ __forceinline__ double cfl_cell(const Phys& ph, double dx, double dy,
     const double cx = cfast1(ph.gamma, p[4], p[0], b2, p[5]);
     const double cy = cfast1(ph.gamma, p[4], p[0], b2, p[6]);
     const double cz = cfast1(ph.gamma, p[4], p[0], b2, p[7]);
-    dtp = dmin(dtp, dx / (__builtin_fabs(p[1]) + cx));
-    dtp = dmin(dtp, dy / (__builtin_fabs(p[2]) + cy));
-    dtp = dmin(dtp, dz / (__builtin_fabs(p[3]) + cz));
+    dtp = dmin(dtp, gm_div(dx, __builtin_fabs(p[1]) + cx));
+    dtp = dmin(dtp, gm_div(dy, __builtin_fabs(p[2]) + cy));
+    dtp = dmin(dtp, gm_div(dz, __builtin_fabs(p[3]) + cz));
   } else {
     const double cs = csound(ph, p[4], p[0]);
-    dtp = dmin(dtp, dx / (__builtin_fabs(p[1]) + cs));
-    dtp = dmin(dtp, dy / (__builtin_fabs(p[2]) + cs));
-    dtp = dmin(dtp, dz / (__builtin_fabs(p[3]) + cs));
+    dtp = dmin(dtp, gm_div(dx, __builtin_fabs(p[1]) + cs));
+    dtp = dmin(dtp, gm_div(dy, __builtin_fabs(p[2]) + cs));
+    dtp = dmin(dtp, gm_div(dz, __builtin_fabs(p[3]) + cs));
   }
   return dtp;
 }

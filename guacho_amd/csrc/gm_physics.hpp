@@ -20,8 +20,9 @@
 
 namespace gm {
 
-// Arithmetic of one precision.  FP64 uses IEEE division and sqrt (the
-// drop-in's bitwise parity depends on it).  FP32 -- the mixed form's
+// Arithmetic of one precision.  FP64 division and sqrt are correctly rounded
+// (gm_div, gm_sqrt: the IEEE results, on which the drop-in's bitwise parity
+// depends).  FP32 -- the mixed form's
 // reconstruction and fluxes -- uses the hardware reciprocal and square
 // root (v_rcp_f32, v_sqrt_f32, about 1 ulp), since its accuracy is judged
 // against the FP64 path by tolerance.
@@ -29,8 +30,8 @@ template <typename T>
 struct Ar;
 template <>
 struct Ar<double> {
-  static __device__ __forceinline__ double div(double a, double b) { return a / b; }
-  static __device__ __forceinline__ double sqrt(double a) { return __builtin_sqrt(a); }
+  static __device__ __forceinline__ double div(double a, double b) { return gm_div(a, b); }
+  static __device__ __forceinline__ double sqrt(double a) { return gm_sqrt(a); }
 };
 template <>
 struct Ar<float> {
@@ -104,7 +105,7 @@ __device__ __forceinline__ double u2prim_t(const Phys& ph, const double u[NQ], d
 // under the adiabatic EOS, u2prim's floored T under EOS_SINGLE_SPECIE (one
 // Tempsc for both: the parameters module's).
 __device__ __forceinline__ double module_temp(const Phys& ph, double t_eos, const double p[8], double tempsc) {
-  return (ph.eos == GM_EOS_SINGLE_SPECIE || ph.eos == GM_EOS_H_RATE) ? t_eos : (p[4] / p[0]) * tempsc;
+  return (ph.eos == GM_EOS_SINGLE_SPECIE || ph.eos == GM_EOS_H_RATE) ? t_eos : gm_div(p[4], p[0]) * tempsc;
 }
 
 // prim2u, src/hydro_core.f90:331-383 (passive scalars :377-379)
@@ -165,7 +166,7 @@ __device__ __forceinline__ T csound(const Phys& ph, T pr, T d) {
 // one component of cfast, src/hydro_core.f90:576-579 (FP64: the CFL reduction)
 __device__ __forceinline__ double cfast1(double g, double pr, double d, double b2, double bn) {
   const double t = g * pr + b2;
-  return __builtin_sqrt((0.5 * (t + __builtin_sqrt(t * t - (((4.0 * g) * pr) * bn) * bn))) / d);
+  return gm_sqrt(gm_div(0.5 * (t + gm_sqrt(t * t - (((4.0 * g) * pr) * bn) * bn)), d));
 }
 
 // cfastX, src/hydro_core.f90:591-604

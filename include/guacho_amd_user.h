@@ -106,21 +106,59 @@ __device__ __forceinline__ void gm_user_load(const gm_user_args &a, int64_t c, d
   for (int q = 0; q < 8 + a.npas; q++) u[q] = gm_user_get(a, c, q);
 }
 
+/* a / b, correctly rounded: the gfx9 FP64 division sequence (reciprocal,
+ * two Newton-Raphson refinements, quotient, one residual correction,
+ * v_div_fixup) without its operand scaling.  v_div_scale only rescales when
+ * |a| < 2^-969, b or a/b is subnormal, |b| > 2^1021 or exp(a) - exp(b) >=
+ * 768; for all other finite nonzero operands these are the full sequence's
+ * steps, so the quotient is the IEEE one.  v_div_fixup gives zero, infinite
+ * and NaN operands their IEEE results and the quotient its sign.  Divisions
+ * by one denominator share its refined reciprocal. */
+__device__ __forceinline__ double gm_div(double a, double b) {
+  const double r0 = __builtin_amdgcn_rcp(b);
+  const double e0 = __builtin_fma(-b, r0, 1.0);
+  const double r1 = __builtin_fma(r0, e0, r0);
+  const double e1 = __builtin_fma(-b, r1, 1.0);
+  const double r2 = __builtin_fma(r1, e1, r1);
+  const double q0 = a * r2;
+  const double rem = __builtin_fma(-b, q0, a);
+  const double q1 = __builtin_fma(rem, r2, q0);
+  return __builtin_amdgcn_div_fixup(q1, b, a);
+}
+
+/* sqrt(a), correctly rounded: the gfx9 FP64 sequence (reciprocal square
+ * root, Goldschmidt refinement, two residual corrections) without the 2^256
+ * prescaling it applies below 2^-767; zero and +inf return themselves and
+ * negative operands NaN, as IEEE sqrt. */
+__device__ __forceinline__ double gm_sqrt(double a) {
+  const double y = __builtin_amdgcn_rsq(a);
+  const double s0 = a * y;
+  const double h0 = y * 0.5;
+  const double r0 = __builtin_fma(-h0, s0, 0.5);
+  const double h1 = __builtin_fma(h0, r0, h0);
+  const double s1 = __builtin_fma(s0, r0, s0);
+  const double d0 = __builtin_fma(-s1, s1, a);
+  const double s2 = __builtin_fma(d0, h1, s1);
+  const double d1 = __builtin_fma(-s2, s2, a);
+  const double s3 = __builtin_fma(d1, h1, s2);
+  return (a == 0.0 || a == __builtin_inf()) ? a : s3;
+}
+
 /* u2prim of the reference (src/hydro_core.f90:46-129, EOS_ADIABATIC):
  * density floor 1e-15, pressure floor 1e-16; the library's own primitive
  * recovery is this function. */
 __device__ __forceinline__ void gm_u2prim(double cv, int mhd, const double u[8], double p[8]) {
   const double r = (u[0] > 1e-15) ? u[0] : 1e-15;
   p[0] = r;
-  p[1] = u[1] / r;
-  p[2] = u[2] / r;
-  p[3] = u[3] / r;
+  p[1] = gm_div(u[1], r);
+  p[2] = gm_div(u[2], r);
+  p[3] = gm_div(u[3], r);
   double pr;
   if (mhd)
-    pr = ((u[4] - (0.5 * r) * ((p[1] * p[1] + p[2] * p[2]) + p[3] * p[3]))
-          - 0.5 * ((u[5] * u[5] + u[6] * u[6]) + u[7] * u[7])) / cv;
+    pr = gm_div((u[4] - (0.5 * r) * ((p[1] * p[1] + p[2] * p[2]) + p[3] * p[3]))
+                - 0.5 * ((u[5] * u[5] + u[6] * u[6]) + u[7] * u[7]), cv);
   else
-    pr = (u[4] - (0.5 * r) * ((p[1] * p[1] + p[2] * p[2]) + p[3] * p[3])) / cv;
+    pr = gm_div(u[4] - (0.5 * r) * ((p[1] * p[1] + p[2] * p[2]) + p[3] * p[3]), cv);
   p[4] = (pr > 1e-16) ? pr : 1e-16;
   p[5] = u[5];
   p[6] = u[6];
@@ -140,12 +178,12 @@ __device__ __forceinline__ double gm_eos(int eos, double tempsc, double *p) {
       den = 2.0 * p[0] - p[8];
       den = (den > 1e-15) ? den : 1e-15;
     }
-    const double x = (p[4] / den) * tempsc;
+    const double x = gm_div(p[4], den) * tempsc;
     const double t = (1.0 > x) ? 1.0 : x;
-    p[4] = (den * t) / tempsc;
+    p[4] = gm_div(den * t, tempsc);
     return t;
   }
-  return (p[4] / p[0]) * tempsc;
+  return gm_div(p[4], p[0]) * tempsc;
 }
 
 /* u (8 + a.npas values, as gm_user_load gives them) -> primitives p (as

@@ -97,6 +97,33 @@ def test_turbulence_32():
     np.testing.assert_array_equal(ud, uo)
 
 
+@pytest.mark.parametrize("solver,mhd,pmhd,lim", [(4, True, False, 1), (1, True, False, 4), (2, False, True, 6)],
+                         ids=["hlld-minmod", "hll-umist", "hllc-superbee-pmhd"])
+@pytest.mark.parametrize("log2a", [-44, 200])
+def test_scaled_state_bitwise(log2a, solver, mhd, pmhd, lim):
+    """The step is invariant under rho, rho v, E, p -> a rho, ..., B ->
+    sqrt(a) B (velocities and wave speeds unchanged) up to the absolute
+    constants of the reference (u2prim's floors 1e-15 and 1e-16, the
+    limiters' 1e-7).  At a = 2^-44 (the floors still off) and 2^200 the
+    divisions and square roots of the solvers see operands far from unit
+    exponents; the device must still equal the restatement bit for bit, and
+    the first time step, a function of the scaled initial state alone, must
+    not move."""
+    say(f"case: turbulence 16^3 scaled by 2^{log2a}, solver {solver} limiter {lim}, 4 steps")
+    u0, dx, dy, dz = problems.turbulence(16, seed=19)
+    a = 2.0 ** log2a
+    us = u0.copy()
+    us[0:5] *= a
+    us[5:8] *= math.sqrt(a)              # exact: a power of 4
+    inner = (slice(None),) + (slice(2, -2),) * 3
+    assert us[0][inner[1:]].min() > 1e-15 and us[4][inner[1:]].min() > 1e-15
+    p = params(16, 16, 16, dx, dy, dz, riemann_solver=solver, slope_limiter=lim, mhd=mhd, pmhd=pmhd)
+    ud, uo, dts = run_pair(p, us, 4, check_each=True)
+    np.testing.assert_array_equal(ud, uo)
+    _, _, dts0 = run_pair(p, u0, 1)
+    assert dts[0] == dts0[0]
+
+
 def test_nonfinite_state_is_reported():
     say("case: NaN in one interior cell -> GM_ERR_NONFINITE from the CFL reduction")
     u0, dx, dy, dz = problems.orszag_tang(16, 16, 4)
