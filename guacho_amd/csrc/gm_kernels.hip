@@ -428,8 +428,12 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
   const bool own = c < wx && r < wy;
   const long long N = g.ncell;
   const long long cc = cidx(g, i0 + c, j0 + r, k);
+  // FP64 carries the reference's partial sums u - dtx*df - dty*dg to the z
+  // kernel; the mixed form carries the FP32 flux increment -dtx*df - dty*dg
+  // and the z kernel adds it to u in FP64
+  constexpr bool INC = sizeof(S) == 4;
   double u0[NH], u0p[NP > 0 ? NP : 1];
-  if (own) {
+  if (own && !INC) {
 #pragma unroll
     for (int q = 0; q < NH; q++) u0[q] = (double)b.U[q * N + cc];
 #pragma unroll
@@ -453,21 +457,35 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
 #pragma unroll
   for (int q = 0; q < NQ; q++) fyu[q] = r == TY - 1 ? ltop[q * TX + c] : lfy[(q * TY + r + 1) * TX + c];
   const double dtdx = dt / b.dx, dtdy = dt / b.dy;
-  double* __restrict__ W = b.W;
+  if constexpr (INC) {
+    float* __restrict__ Wf = reinterpret_cast<float*>(b.W);
 #pragma unroll
-  for (int q = 0; q < NH; q++)
-    W[q * N + cc] = (u0[q] - dtdx * ((double)fxr[q] - (double)fx[q])) - dtdy * ((double)fyu[q] - (double)fy[q]);
-  // passive scalars: the same partial sums (flux_cd_module.f90:303-308)
+    for (int q = 0; q < NQ; q++) {
+      if (q >= NH && q < 8) continue;
+      Wf[q * N + cc] = (float)((-(dtdx * ((double)fxr[q] - (double)fx[q]))) - dtdy * ((double)fyu[q] - (double)fy[q]));
+    }
+    if constexpr (FLUXCD) {
+      Wf[5 * N + cc] = (float)((-(double)fy[7]) - (double)fyu[7]);
+      Wf[6 * N + cc] = (float)((+(double)fx[7]) + (double)fxr[7]);
+    }
+  } else {
+    double* __restrict__ W = b.W;
 #pragma unroll
-  for (int a = 0; a < NP; a++) {
-    const int q = 8 + a;
-    W[q * N + cc] = (u0p[a] - dtdx * ((double)fxr[q] - (double)fx[q])) - dtdy * ((double)fyu[q] - (double)fy[q]);
+    for (int q = 0; q < NH; q++)
+      W[q * N + cc] = (u0[q] - dtdx * ((double)fxr[q] - (double)fx[q])) - dtdy * ((double)fyu[q] - (double)fy[q]);
+    // passive scalars: the same partial sums (flux_cd_module.f90:303-308)
+#pragma unroll
+    for (int a = 0; a < NP; a++) {
+      const int q = 8 + a;
+      W[q * N + cc] = (u0p[a] - dtdx * ((double)fxr[q] - (double)fx[q])) - dtdy * ((double)fyu[q] - (double)fy[q]);
+    }
+    if constexpr (FLUXCD) {
+      W[5 * N + cc] = (-(double)fy[7]) - (double)fyu[7];
+      W[6 * N + cc] = (+(double)fx[7]) + (double)fxr[7];
+    }
   }
-  if constexpr (FLUXCD) {
-    W[5 * N + cc] = (-(double)fy[7]) - (double)fyu[7];
-    W[6 * N + cc] = (+(double)fx[7]) + (double)fxr[7];
+  if constexpr (FLUXCD)
     b.E[2 * N + cc] = (S)(0.25 * ((((-(double)fx[6]) - (double)fxr[6]) + (double)fy[5]) + (double)fyu[5]));
-  }
 }
 
 // z kernel.  A thread marches up the pencil k0..k1 of column (i, j),
@@ -497,6 +515,7 @@ __device__ __forceinline__ void z_body(const Box<S>& b, unsigned lb, double dt, 
   S* __restrict__ Uout = ORDER == 1 ? b.Us : b.U;
   const double* __restrict__ W = b.W;
   const int lim = b.ph.limiter;
+  constexpr bool INC = sizeof(S) == 4;  // W holds FP32 x/y flux increments
 
   S ur[NQ];
   bool uflip = false;
@@ -547,14 +566,29 @@ __device__ __forceinline__ void z_body(const Box<S>& b, unsigned lb, double dt, 
     const int kc0 = max(kf, k0);
     const long long c = col + (long long)(kc0 + 1) * sz;
     double wv[NQ];
+    if constexpr (INC) {
+      // U^n (the predictor's input, the corrector's output) plus the FP32
+      // x/y flux increment, in FP64
+      const float* __restrict__ Wf = reinterpret_cast<const float*>(W);
 #pragma unroll
-    for (int q = 0; q < NH; q++) wv[q] = W[q * N + c];
-    if constexpr (FLUXCD) {
-      wv[5] = W[5 * N + c];
-      wv[6] = W[6 * N + c];
+      for (int q = 0; q < NQ; q++) {
+        if (q >= NH && q < 8) continue;
+        wv[q] = (double)b.U[q * N + c] + (double)Wf[q * N + c];
+      }
+      if constexpr (FLUXCD) {
+        wv[5] = (double)Wf[5 * N + c];
+        wv[6] = (double)Wf[6 * N + c];
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < NH; q++) wv[q] = W[q * N + c];
+      if constexpr (FLUXCD) {
+        wv[5] = W[5 * N + c];
+        wv[6] = W[6 * N + c];
+      }
+#pragma unroll
+      for (int q = 8; q < NQ; q++) wv[q] = W[q * N + c];
     }
-#pragma unroll
-    for (int q = 8; q < NQ; q++) wv[q] = W[q * N + c];
     S left[NQ], right[NQ];
     if constexpr (ORDER == 2) {
 #pragma unroll
