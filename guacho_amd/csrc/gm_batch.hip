@@ -131,6 +131,7 @@ gm::Box<S> box_view(const Slot& s, int slot) {
   b.dz = s.p.dz;
   b.slot = slot;
   b.src = s.p.eight_wave ? gm::SRC_EIGHT_WAVE : 0;
+  b.w8 = 0;
   return b;
 }
 

@@ -285,6 +285,7 @@ gm::Box<S> box_of(const gm_ctx* c) {
   b.dz = c->p.dz;
   b.slot = 0;
   b.src = (c->p.eight_wave ? gm::SRC_EIGHT_WAVE : 0) | (c->usrc_fn ? gm::SRC_USER : 0);
+  b.w8 = (b.src == gm::SRC_EIGHT_WAVE && c->g.npas == 0) ? 1 : 0;
   b.g.gmem = (c->ubc_fn || c->cond.kind) ? 1 : 0;
   return b;
 }
@@ -300,7 +301,7 @@ void user_sources_t(gm_ctx* c, const gm::Box<S>& b, double dt, int stage) {
 
 template <typename S>
 void stage_sources_t(gm_ctx* c, const gm::Box<S>& b, double dt, int stage) {
-  if (!b.src) return;
+  if (!b.src || b.w8) return;  // b.w8: the stage kernels added the 8-wave terms
   timed(c, KC_SOURCES, [&] {
     if (b.src & gm::SRC_USER) user_sources_t(c, b, dt, stage);
     gm::launch_sources(c->stream, b, dt, stage, c->shape);

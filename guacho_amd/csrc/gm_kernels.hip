@@ -80,6 +80,39 @@ __device__ __forceinline__ int ghost_src(const Geom& g, int i, int n, int lo, in
   return map_ghost(i, n, lo, hi, flip);
 }
 
+// Powell's 8-wave terms of cell (i,j,k) (divbcorr_8w_source,
+// src/sources.f90:137-168) from the stage's input state U, added to sv as
+// source() adds them (sv - divB*..., :188-220): divB by central differences
+// of the neighbours' field (ghosts by boundary mapping; the field is never
+// flipped), v and B of the cell as u2prim gives them.
+template <typename S>
+__device__ __forceinline__ void w8_terms(const Box<S>& b, const S* __restrict__ U, int i, int j, int k, double sv[8]) {
+  const Geom& g = b.g;
+  const long long N = g.ncell, sy = g.NX, sz = (long long)g.NX * g.NY;
+  const long long c = cidx(g, i, j, k);
+  bool f = false;
+  const long long oxm = ghost_src(g, i - 1, g.nx, g.bc[0], g.bc[1], f) - i;
+  const long long oxp = ghost_src(g, i + 1, g.nx, g.bc[0], g.bc[1], f) - i;
+  const long long oym = (ghost_src(g, j - 1, g.ny, g.bc[2], g.bc[3], f) - j) * sy;
+  const long long oyp = (ghost_src(g, j + 1, g.ny, g.bc[2], g.bc[3], f) - j) * sy;
+  const long long ozm = (ghost_src(g, k - 1, g.nz, g.bc[4], g.bc[5], f) - k) * sz;
+  const long long ozp = (ghost_src(g, k + 1, g.nz, g.bc[4], g.bc[5], f) - k) * sz;
+  auto d = [&](int q, long long o) { return (double)U[q * N + c + o]; };
+  const double divb = (gm_div(d(5, oxp) - d(5, oxm), 2.0 * b.dx) + gm_div(d(6, oyp) - d(6, oym), 2.0 * b.dy))
+                      + gm_div(d(7, ozp) - d(7, ozm), 2.0 * b.dz);
+  const double r0 = d(0, 0);
+  const double r = (r0 > 1e-15) ? r0 : 1e-15;
+  const double v1 = gm_div(d(1, 0), r), v2 = gm_div(d(2, 0), r), v3 = gm_div(d(3, 0), r);
+  const double b1 = d(5, 0), b2 = d(6, 0), b3 = d(7, 0);
+  sv[1] = sv[1] - divb * b1;
+  sv[2] = sv[2] - divb * b2;
+  sv[3] = sv[3] - divb * b3;
+  sv[4] = sv[4] - divb * ((v1 * b1 + v2 * b2) + v3 * b3);
+  sv[5] = sv[5] - divb * v1;
+  sv[6] = sv[6] - divb * v2;
+  sv[7] = sv[7] - divb * v3;
+}
+
 // Conserved variables (NQ: the 8 MHD ones and the passive scalars) of
 // cell (i,j,k) in FP64, ghosts resolved through the boundary conditions.
 template <int NQ = 8, typename S>
@@ -496,7 +529,8 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
 // reconstruction of a cell depends on the cell and its two neighbours
 // only, so each cell's slope is formed once and serves both its faces
 // (pl + al/2 for the face above, pr - ar/2 for the face below).
-template <int SOLVER, int ORDER, int FLUXCD, typename S, int NP = 0>
+// W8: Powell's 8-wave terms added to each stored cell (Box::w8).
+template <int SOLVER, int ORDER, int FLUXCD, typename S, int NP = 0, bool W8 = false>
 __device__ __forceinline__ void z_body(const Box<S>& b, unsigned lb, double dt, int kchunk, int* __restrict__ err) {
   constexpr int NH = FLUXCD ? 5 : 8;
   constexpr int NQ = 8 + NP;  // the MHD variables and the passive scalars
@@ -516,6 +550,7 @@ __device__ __forceinline__ void z_body(const Box<S>& b, unsigned lb, double dt, 
   const double* __restrict__ W = b.W;
   const int lim = b.ph.limiter;
   constexpr bool INC = sizeof(S) == 4;  // W holds FP32 x/y flux increments
+  constexpr bool w8 = W8;
 
   S ur[NQ];
   bool uflip = false;
@@ -614,10 +649,24 @@ __device__ __forceinline__ void z_body(const Box<S>& b, unsigned lb, double dt, 
     if (!riemann<SOLVER, NQ>(b.ph, left, right, ff)) atomicOr(err, 1);
     axis_perm<2>(ff);
     if (kf >= k0) {
+      // Powell's 8-wave terms (no user sources): added to the stored update
+      // here rather than in a pass of their own over U
+      double sv[8];
 #pragma unroll
-      for (int q = 0; q < NH; q++) Uout[q * N + c] = (S)(wv[q] - dtdz * ((double)ff[q] - (double)fl[q]));
+      for (int q = 0; q < 8; q++) sv[q] = 0.0;
+      if (w8) w8_terms(b, Usrc, i, j, kf, sv);
 #pragma unroll
-      for (int q = 8; q < NQ; q++) Uout[q * N + c] = (S)(wv[q] - dtdz * ((double)ff[q] - (double)fl[q]));
+      for (int q = 0; q < NH; q++) {
+        S r = (S)(wv[q] - dtdz * ((double)ff[q] - (double)fl[q]));
+        if (w8) r = (S)((double)r + dt * sv[q]);
+        Uout[q * N + c] = r;
+      }
+#pragma unroll
+      for (int q = 8; q < NQ; q++) {
+        S r = (S)(wv[q] - dtdz * ((double)ff[q] - (double)fl[q]));
+        if (w8) r = (S)((double)r + dt * 0.0);
+        Uout[q * N + c] = r;
+      }
       if constexpr (FLUXCD) {
         b.E[0 * N + c] = (S)(0.25 * ((wv[5] + (double)fl[6]) + (double)ff[6]));
         b.E[1 * N + c] = (S)(0.25 * ((wv[6] - (double)fl[5]) - (double)ff[5]));
@@ -692,9 +741,20 @@ __device__ __forceinline__ void bfield_body(const Box<S>& b, unsigned lb, double
     const double b5 = (d(Uin[5 * N + c]) - (0.5 * dtdy) * (d(E3[c + oyp]) - d(E3[c + oym]))) + (0.5 * dtdz) * (e2p - e2m);
     const double b6 = (d(Uin[6 * N + c]) + (0.5 * dtdx) * (d(E3[c + oxp]) - d(E3[c + oxm]))) - (0.5 * dtdz) * (e1p - e1m);
     const double b7 = (d(Uin[7 * N + c]) - (0.5 * dtdx) * (d(E2[c + oxp]) - d(E2[c + oxm]))) + (0.5 * dtdy) * (d(E1[c + oyp]) - d(E1[c + oym]));
-    Uout[5 * N + c] = (S)b5;
-    Uout[6 * N + c] = (S)b6;
-    Uout[7 * N + c] = (S)b7;
+    if (!CFL && b.w8) {  // (the CFL fold runs only without source terms)
+      // Powell's terms of the field, from the stage's input state
+      double sv[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) sv[q] = 0.0;
+      w8_terms(b, stage == 1 ? b.U : b.Us, i, j, k, sv);
+      Uout[5 * N + c] = (S)((double)(S)b5 + dt * sv[5]);
+      Uout[6 * N + c] = (S)((double)(S)b6 + dt * sv[6]);
+      Uout[7 * N + c] = (S)((double)(S)b7 + dt * sv[7]);
+    } else {
+      Uout[5 * N + c] = (S)b5;
+      Uout[6 * N + c] = (S)b6;
+      Uout[7 * N + c] = (S)b7;
+    }
     if (CFL) {
       // this cell of U^{n+1} is final: its hydro part and passives were
       // written by the z kernel, its field just now (as stored, in S)
@@ -720,23 +780,15 @@ __device__ __forceinline__ void bfield_body(const Box<S>& b, unsigned lb, double
 // Source terms of step() (src/hydro_solver.f90:115-121): after the stage's
 // update, up = up + dt*s with s from source() (src/sources.f90:188-220):
 // zero, plus the user's terms (already in W, written by the user plugin on
-// a zeroed W), then divbcorr_8w_source (:137-168) from the primitives of the
-// stage's input state -- U^n for the predictor, U* for the corrector, ghosts
-// by boundary mapping (the field is never flipped).  Column pencils.
+// a zeroed W), then Powell's 8-wave terms (w8_terms) from the stage's input
+// state -- U^n for the predictor, U* for the corrector.  Without user
+// sources and passive scalars (Box::w8) k_z and the field update add the
+// 8-wave terms as they store each cell instead.  Column pencils.
 // ---------------------------------------------------------------------------
-template <typename S>
-__device__ __forceinline__ double field_at(const Geom& g, const S* __restrict__ U, int q, int i, int j, int k) {
-  bool f = false;
-  const int si = ghost_src(g, i, g.nx, g.bc[0], g.bc[1], f);
-  const int sj = ghost_src(g, j, g.ny, g.bc[2], g.bc[3], f);
-  const int sk = ghost_src(g, k, g.nz, g.bc[4], g.bc[5], f);
-  return (double)U[q * g.ncell + cidx(g, si, sj, sk)];
-}
-
 template <typename S>
 __device__ __forceinline__ void sources_body(const Box<S>& b, unsigned lb, double dt, int stage, int kchunk) {
   const Geom& g = b.g;
-  if (!b.src) return;
+  if (!b.src || b.w8) return;  // b.w8: the stage kernels added the 8-wave terms
   int t, kc;
   if (!tile_item(lb, tiles_of((unsigned)(g.nx * g.ny)), (unsigned)(g.nx * g.ny), t, kc)) return;
   const int i = t % g.nx + 1, j = t / g.nx + 1;
@@ -747,24 +799,10 @@ __device__ __forceinline__ void sources_body(const Box<S>& b, unsigned lb, doubl
   S* __restrict__ Uout = stage == 1 ? b.Us : b.U;
   for (int k = k0; k <= k1; k++) {
     const long long c = cidx(g, i, j, k);
-    double u[8], pp[8], sv[10];
-#pragma unroll
-    for (int q = 0; q < 8; q++) u[q] = (double)Uin[q * N + c];
-    u2prim(b.ph, u, pp);  // the 8-wave terms read no pressure: the EOS leaves them alone
+    double sv[10];
 #pragma unroll
     for (int q = 0; q < 10; q++) sv[q] = (q < 8 + g.npas && (b.src & SRC_USER)) ? b.W[q * N + c] : 0.0;
-    if (b.src & SRC_EIGHT_WAVE) {
-      const double divb = ((field_at(g, Uin, 5, i + 1, j, k) - field_at(g, Uin, 5, i - 1, j, k)) / (2.0 * b.dx)
-                           + (field_at(g, Uin, 6, i, j + 1, k) - field_at(g, Uin, 6, i, j - 1, k)) / (2.0 * b.dy))
-                          + (field_at(g, Uin, 7, i, j, k + 1) - field_at(g, Uin, 7, i, j, k - 1)) / (2.0 * b.dz);
-      sv[1] = sv[1] - divb * pp[5];
-      sv[2] = sv[2] - divb * pp[6];
-      sv[3] = sv[3] - divb * pp[7];
-      sv[4] = sv[4] - divb * ((pp[1] * pp[5] + pp[2] * pp[6]) + pp[3] * pp[7]);
-      sv[5] = sv[5] - divb * pp[1];
-      sv[6] = sv[6] - divb * pp[2];
-      sv[7] = sv[7] - divb * pp[3];
-    }
+    if (b.src & SRC_EIGHT_WAVE) w8_terms(b, Uin, i, j, k, sv);
 #pragma unroll
     for (int q = 0; q < 10; q++)
       if (q < 8 + g.npas) Uout[q * N + c] = (S)((double)Uout[q * N + c] + dt * sv[q]);
@@ -1278,9 +1316,9 @@ k_xy(Box<S> b, double dt, int* err) {
   xy_body<SOLVER, ORDER, FLUXCD, S, NP>(b, xcd_logical_block(), dt, err, lds);
 }
 
-template <int SOLVER, int ORDER, int FLUXCD, typename S, int NP>
+template <int SOLVER, int ORDER, int FLUXCD, typename S, int NP, bool W8 = false>
 __global__ void __launch_bounds__(256) k_z(Box<S> b, double dt, int kchunk, int* err) {
-  z_body<SOLVER, ORDER, FLUXCD, S, NP>(b, xcd_logical_block(), dt, kchunk, err);
+  z_body<SOLVER, ORDER, FLUXCD, S, NP, W8>(b, xcd_logical_block(), dt, kchunk, err);
 }
 
 template <typename S>
@@ -1352,15 +1390,6 @@ __global__ void __launch_bounds__(256) kb_bfield(const Box<S>* __restrict__ boxe
 }
 
 template <typename S>
-__global__ void __launch_bounds__(256) kb_cfl(const Box<S>* __restrict__ boxes, const unsigned* __restrict__ prefix,
-                                              int nbox, int kchunk, BoxClock* clk, int* err) {
-  int bx;
-  unsigned lb;
-  batch_locate(prefix, nbox, xcd_logical_block(), bx, lb);
-  cfl_body(boxes[bx], lb, kchunk, &clk[boxes[bx].slot].dtmin, err);
-}
-
-template <typename S>
 __global__ void __launch_bounds__(256) kb_sources(const Box<S>* __restrict__ boxes,
                                                   const unsigned* __restrict__ prefix, int nbox,
                                                   const BoxClock* __restrict__ clk, int stage, int kchunk) {
@@ -1370,6 +1399,16 @@ __global__ void __launch_bounds__(256) kb_sources(const Box<S>* __restrict__ box
   const Box<S>& b = boxes[bx];
   sources_body(b, lb, stage_dt(clk, b.slot, stage), stage, kchunk);
 }
+
+template <typename S>
+__global__ void __launch_bounds__(256) kb_cfl(const Box<S>* __restrict__ boxes, const unsigned* __restrict__ prefix,
+                                              int nbox, int kchunk, BoxClock* clk, int* err) {
+  int bx;
+  unsigned lb;
+  batch_locate(prefix, nbox, xcd_logical_block(), bx, lb);
+  cfl_body(boxes[bx], lb, kchunk, &clk[boxes[bx].slot].dtmin, err);
+}
+
 
 // Start of a batched step: reset every box's CFL minimum.
 __global__ void kb_clock_begin(BoxClock* clk, int nbox) {
@@ -1561,7 +1600,8 @@ void launch_z(hipStream_t s, const Box<S>& b, double dt, int stage, int solver, 
   const unsigned n = box_tiles(b.g, KC_COL, sh);
 #define GM_CALL(SV, O, F)                                                                                  \
   {                                                                                                        \
-    if (b.g.npas == 0) hipLaunchKernelGGL((k_z<SV, O, F, S, 0>), dim3(n), dim3(sh.block), 0, s, b, dt, sh.kchunk, err); \
+    if (b.w8) hipLaunchKernelGGL((k_z<SV, O, F, S, 0, true>), dim3(n), dim3(sh.block), 0, s, b, dt, sh.kchunk, err); \
+    else if (b.g.npas == 0) hipLaunchKernelGGL((k_z<SV, O, F, S, 0>), dim3(n), dim3(sh.block), 0, s, b, dt, sh.kchunk, err); \
     else if (b.g.npas == 1)                                                                                \
       hipLaunchKernelGGL((k_z<SV, O, F, S, 1>), dim3(n), dim3(sh.block), 0, s, b, dt, sh.kchunk, err);    \
     else hipLaunchKernelGGL((k_z<SV, O, F, S, 2>), dim3(n), dim3(sh.block), 0, s, b, dt, sh.kchunk, err); \
