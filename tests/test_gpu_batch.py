@@ -47,13 +47,19 @@ def boxes(prec):
     out.append((_capi.make_params(24, 24, 4, dx, dy, dz, precision=prec), u))
     u, dx, dy, dz = problems.turbulence(16, seed=4)
     out.append((_capi.make_params(16, 16, 16, dx, dy, dz, bc=(2, 2, 1, 1, 3, 3), precision=prec), u))
+    # Powell 8-wave: batches add the terms in a pass of their own, single
+    # boxes in the stage kernels
+    u, dx, dy, dz = problems.blast(14, seed=5)
+    out.append((_capi.make_params(14, 14, 14, dx, dy, dz, bc=(1,) * 6, precision=prec, eight_wave=True), u))
+    u, dx, dy, dz = problems.turbulence(12, seed=6)
+    out.append((_capi.make_params(12, 12, 12, dx, dy, dz, precision=prec, eight_wave=True), u))
     return out
 
 
 @pytest.mark.parametrize("prec", [_capi.GM_PRECISION_FP64, _capi.GM_PRECISION_MIXED], ids=["fp64", "mixed"])
 @pytest.mark.parametrize("spl", [1, 4])
 def test_batch_equals_single_box_path(prec, spl):
-    say(f"case: batch of 5 uneven boxes, precision {prec}, steps_per_launch {spl}, 8 steps")
+    say(f"case: batch of 7 uneven boxes, precision {prec}, steps_per_launch {spl}, 8 steps")
     bx = boxes(prec)
     with _capi.Batch(precision=prec, max_boxes=8, steps_per_launch=spl) as B:
         ids = [B.add(p, u, max_steps=8) for p, u in bx]
@@ -65,6 +71,23 @@ def test_batch_equals_single_box_path(prec, spl):
         ref, steps, tref = single(p, u, nsteps=8)
         np.testing.assert_array_equal(g, ref)
         assert (n, done) == (8, True) and t == tref
+
+
+@pytest.mark.parametrize("prec", [_capi.GM_PRECISION_FP64, _capi.GM_PRECISION_MIXED], ids=["fp64", "mixed"])
+def test_eight_wave_batch_without_flux_cd(prec):
+    say(f"case: batch without flux-CD, 8-wave and plain boxes, precision {prec}, 6 steps")
+    bx = []
+    for n, seed, eight in ((14, 7, True), (10, 8, False), (12, 9, True)):
+        u, dx, dy, dz = problems.blast(n, seed=seed)
+        bx.append((_capi.make_params(n, n, n, dx, dy, dz, bc=(1,) * 6, precision=prec, eight_wave=eight,
+                                     enable_flux_cd=False), u))
+    with _capi.Batch(precision=prec, max_boxes=4, enable_flux_cd=False) as B:
+        ids = [B.add(p, u, max_steps=6) for p, u in bx]
+        B.advance(6)
+        B.sync()
+        got = [B.download_u(i) for i in ids]
+    for (p, u), g in zip(bx, got):
+        np.testing.assert_array_equal(g, single(p, u, nsteps=6)[0])
 
 
 def test_alone_equals_in_mix_mixed():

@@ -95,3 +95,15 @@ def test_mixed_other_solvers_close(solver, mhd, pmhd, bc):
     say(f"relative L1 per field: {e}")
     assert e[0] <= 1e-5, e
     assert np.all(np.isfinite(u32))
+
+
+@pytest.mark.parametrize("fcd", [False, True], ids=["8wave", "8wave-fluxcd"])
+def test_mixed_eight_wave_close(fcd):
+    """The mixed form with Powell's terms added by the stage kernels."""
+    say(f"case: mixed vs FP64, turbulence 32^3 8-wave (flux-CD {fcd}), 12 steps")
+    u0, dx, dy, dz = problems.turbulence(32, seed=13)
+    u64, u32, _, _, _ = both((32, 32, 32), u0, dx, dy, dz, 12, eight_wave=True, enable_flux_cd=fcd)
+    e = rel_l1(u32, u64)
+    say(f"relative L1 per field: {e}")
+    assert e[0] <= 1e-5, e
+    assert np.all(e <= 1e-4), e
