@@ -51,7 +51,7 @@ struct Phys {
   double cv, gamma;
   int mhd, pmhd;   // full MHD / passive B
   int limiter;
-  int eos;         // GM_EOS_ADIABATIC or GM_EOS_SINGLE_SPECIE
+  int eos;         // GM_EOS_ADIABATIC, GM_EOS_SINGLE_SPECIE or GM_EOS_H_RATE
   double tempsc;   // Tempsc of the equation of state
 };
 

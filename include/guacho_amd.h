@@ -10,7 +10,7 @@
  * Conventions
  *  - Host arrays are caller-allocated in the reference's globals layout,
  *    Fortran column-major u(neq, nxmin:nxmax, nymin:nymax, nzmin:nzmax)
- *    with neq = 8 fastest and nghost = 2 (src/globals.f90:33-38,
+ *    with neq (8 plus the passive scalars) fastest and nghost = 2 (src/globals.f90:33-38,
  *    src/init.f90:145-151).  The library stores structure-of-arrays in
  *    device memory and transposes on upload/download.
  *  - Every call returns GM_OK (0) or a positive GM_ERR_* code; the message
