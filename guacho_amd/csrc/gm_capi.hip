@@ -67,6 +67,7 @@ struct gm_ctx {
   // the last gm_tstep reduced the CFL minimum of its result into hscal and
   // nothing has changed U's interior or the EOS since
   bool cfl_ready = false;
+  bool cfl_pending = false;  // this gm_tstep queued the reduction (ready once its launches succeeded)
   double time = 0.0;                    // simulation time the user hooks see
   hipModule_t usrc_mod = nullptr;       // user source plugin
   hipFunction_t usrc_fn = nullptr;
@@ -402,7 +403,7 @@ void tstep_t(gm_ctx* c, double dt) {
     hipLaunchKernelGGL(k_init_dtmin, dim3(1), dim3(1), 0, s, c->dscal);
     timed(c, KC_BFIELD_CFL, [&] { gm::launch_bfield_cfl(s, b, dt, &c->dscal->dtmin, err, sh); });
     (void)hipMemcpyAsync(c->hscal, c->dscal, sizeof(Scalars), hipMemcpyDeviceToHost, s);
-    c->cfl_ready = true;
+    c->cfl_pending = true;
   } else if (fcd) {
     timed(c, KC_BFIELD, [&] { gm::launch_bfield(s, b, dt, 2, sh); });
   }
@@ -617,11 +618,12 @@ int gm_tstep(gm_ctx* c, double dt) {
     for (int f = 0; f < 6; f++)
       if (c->p.bc[f] == GM_BC_OTHER)
         return fail(c, GM_ERR_CONFIG, "BC_OTHER faces need a user boundary plugin (gm_set_user_bc)");
-  c->cfl_ready = false;
+  c->cfl_ready = c->cfl_pending = false;
   if (c->esize == 8) tstep_t<double>(c, dt);
   else tstep_t<float>(c, dt);
   c->ghosts_dirty = c->ubc_fn == nullptr && !c->cond.kind;  // else the step ends with boundaryI
   GM_HIP(c, hipGetLastError());
+  c->cfl_ready = c->cfl_pending;
   return GM_OK;
 }
 
