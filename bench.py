@@ -104,15 +104,17 @@ def algorithmic_bytes(cls: str, n: int) -> float:
 
 
 def measured_traffic(cls: str):
-    """Calibrated HBM bytes per launch of `cls` from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary of this workload, if one exists."""
+    """Calibrated HBM bytes per launch of `cls` and its FP64 VALU issue
+    fraction, from the committed rocprofv3 summary of this workload (FETCH_SIZE,
+    WRITE_SIZE, SQ_INSTS_VALU), if one exists."""
     path = os.path.join(ROOT, "profiles", "traffic_fp64_turb256.json")
     try:
         with open(path) as fh:
             d = json.load(fh)
-        return d["per_launch_bytes"].get(cls), os.path.relpath(path, ROOT)
+        return (d["per_launch_bytes"].get(cls), d.get("valu_issue_frac", {}).get(cls),
+                os.path.relpath(path, ROOT))
     except (OSError, KeyError, ValueError):
-        return None, None
+        return None, None, None
 
 
 def dominant_roofline(kt, n, steps):
@@ -121,11 +123,14 @@ def dominant_roofline(kt, n, steps):
     ms, launches = kt[cls]
     avg_s = ms / launches / 1e3
     ach = algorithmic_bytes(cls, n) / avg_s / 1e9
-    traffic, src = measured_traffic(cls)
+    traffic, valu, src = measured_traffic(cls)
     return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": cls,
             "launches_per_step": launches // steps,
             "avg_launch_ms": round(ms / launches, 4), "algorithmic_bytes_per_launch": algorithmic_bytes(cls, n),
+            # what actually bounds it: FP64 VALU instructions issued per cycle
+            # against the nominal rate (rocprofv3 SQ_INSTS_VALU)
+            "valu_issue_frac": valu,
             "traffic_source": src}
 
 
