@@ -131,7 +131,8 @@ gm::Box<S> box_view(const Slot& s, int slot) {
   b.dz = s.p.dz;
   b.slot = slot;
   b.src = s.p.eight_wave ? gm::SRC_EIGHT_WAVE : 0;
-  b.w8 = 0;
+  b.srcin = 0;
+  b.usrc = nullptr;
   return b;
 }
 

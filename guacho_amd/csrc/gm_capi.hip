@@ -61,6 +61,7 @@ struct gm_ctx {
   void* W = nullptr;     // stage partials (neq doubles per cell); also the staging
                          // buffer for layout conversion
   void* E = nullptr;     // cell-centred E (3 vars); also staging for Temp / divB
+  void* Ssrc = nullptr;  // the user source plugin's s of a stage (neq doubles per cell)
   Scalars* dscal = nullptr;  // device scalars
   Scalars* hscal = nullptr;  // pinned host mirror
   bool ghosts_dirty = true;
@@ -200,7 +201,7 @@ void release(gm_ctx* c) {
     (void)hipEventDestroy(r.b);
   }
   for (auto e : c->pool) (void)hipEventDestroy(e);
-  void* bufs[] = {c->U, c->Us, c->W, c->E};
+  void* bufs[] = {c->U, c->Us, c->W, c->E, c->Ssrc};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->usrc_mod) (void)hipModuleUnload(c->usrc_mod);
@@ -286,7 +287,8 @@ gm::Box<S> box_of(const gm_ctx* c) {
   b.dz = c->p.dz;
   b.slot = 0;
   b.src = (c->p.eight_wave ? gm::SRC_EIGHT_WAVE : 0) | (c->usrc_fn ? gm::SRC_USER : 0);
-  b.w8 = (b.src == gm::SRC_EIGHT_WAVE && c->g.npas == 0) ? 1 : 0;
+  b.srcin = (b.src != 0 && c->g.npas == 0) ? 1 : 0;
+  b.usrc = (b.srcin && c->usrc_fn) ? (const double*)c->Ssrc : nullptr;
   b.g.gmem = (c->ubc_fn || c->cond.kind) ? 1 : 0;
   return b;
 }
@@ -300,9 +302,21 @@ void user_sources_t(gm_ctx* c, const gm::Box<S>& b, double dt, int stage) {
   launch_plugin(c, c->usrc_fn, user_args(c, u, b.W, dt, stage), (long long)c->g.nx * c->g.ny * c->g.nz);
 }
 
+// With the source terms added by the stage kernels (Box::srcin): the user's
+// s of the stage, from its input state, before the stage's kernels run.
+template <typename S>
+void stage_user_s_t(gm_ctx* c, const gm::Box<S>& b, double dt, int stage) {
+  if (!b.srcin || !(b.src & gm::SRC_USER)) return;
+  timed(c, KC_SOURCES, [&] {
+    (void)hipMemsetAsync(c->Ssrc, 0, (size_t)c->g.ncell * c->p.neq * sizeof(double), c->stream);
+    void* u = stage == 1 ? (void*)b.U : (void*)b.Us;
+    launch_plugin(c, c->usrc_fn, user_args(c, u, (double*)c->Ssrc, dt, stage), (long long)c->g.nx * c->g.ny * c->g.nz);
+  });
+}
+
 template <typename S>
 void stage_sources_t(gm_ctx* c, const gm::Box<S>& b, double dt, int stage) {
-  if (!b.src || b.w8) return;  // b.w8: the stage kernels added the 8-wave terms
+  if (!b.src || b.srcin) return;  // b.srcin: the stage kernels added the terms
   timed(c, KC_SOURCES, [&] {
     if (b.src & gm::SRC_USER) user_sources_t(c, b, dt, stage);
     gm::launch_sources(c->stream, b, dt, stage, c->shape);
@@ -385,6 +399,7 @@ void tstep_t(gm_ctx* c, double dt) {
   const int solver = p.riemann_solver, fcd = p.enable_flux_cd ? 1 : 0;
   const double dtm = dt / 2.0;  // hydro_solver.f90:152
   // predictor: first-order fluxes of U^n, U* = U^n - dt/2 div F (+ dt/2 s)
+  stage_user_s_t(c, b, dtm, 1);
   timed(c, KC_XY1, [&] { gm::launch_xy(s, b, dtm, 1, solver, fcd, err); });
   timed(c, KC_Z1, [&] { gm::launch_z(s, b, dtm, 1, solver, fcd, err, sh); });
   if (fcd) timed(c, KC_BFIELD, [&] { gm::launch_bfield(s, b, dtm, 1, sh); });
@@ -393,6 +408,7 @@ void tstep_t(gm_ctx* c, double dt) {
   // place
   // boundaryII on U* (src/hydro_solver.f90:169) when ghosts live in memory
   if (b.g.gmem) fill_ghost_t<S>(c, b.Us, 2);
+  stage_user_s_t(c, b, dt, 2);
   timed(c, KC_XY2, [&] { gm::launch_xy(s, b, dt, 2, solver, fcd, err); });
   timed(c, KC_Z2, [&] { gm::launch_z(s, b, dt, 2, solver, fcd, err, sh); });
   // when nothing after the field update touches U's interior (no sources,
@@ -679,7 +695,6 @@ int gm_set_user_sources(gm_ctx* c, const char* code_object, const char* kernel) 
   if (rc) return rc;
   GM_HIP(c, hipStreamSynchronize(c->stream));
   if (c->usrc_mod) (void)hipModuleUnload(c->usrc_mod);
-  if (c->ubc_mod) (void)hipModuleUnload(c->ubc_mod);
   c->usrc_mod = nullptr;
   c->usrc_fn = nullptr;
   if (!code_object) return GM_OK;
@@ -694,6 +709,16 @@ int gm_set_user_sources(gm_ctx* c, const char* code_object, const char* kernel) 
     (void)hipGetLastError();
     (void)hipModuleUnload(m);
     return fail(c, GM_ERR_CONFIG, std::string("no kernel ") + kernel + " in " + code_object);
+  }
+  if (!c->Ssrc) {
+    const size_t n = (size_t)c->g.ncell * c->p.neq * sizeof(double);
+    if (hipMalloc(&c->Ssrc, n) != hipSuccess) {
+      (void)hipGetLastError();
+      c->Ssrc = nullptr;
+      (void)hipModuleUnload(m);
+      return fail(c, GM_ERR_OOM, "device allocation for the user source terms failed");
+    }
+    c->bytes += (int64_t)n;
   }
   c->usrc_mod = m;
   c->usrc_fn = f;

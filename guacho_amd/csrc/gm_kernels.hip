@@ -529,8 +529,9 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
 // reconstruction of a cell depends on the cell and its two neighbours
 // only, so each cell's slope is formed once and serves both its faces
 // (pl + al/2 for the face above, pr - ar/2 for the face below).
-// W8: Powell's 8-wave terms added to each stored cell (Box::w8).
-template <int SOLVER, int ORDER, int FLUXCD, typename S, int NP = 0, bool W8 = false>
+// SRCIN: the source terms (the user's s, Powell's 8-wave terms) added to
+// each stored cell (Box::srcin).
+template <int SOLVER, int ORDER, int FLUXCD, typename S, int NP = 0, bool SRCIN = false>
 __device__ __forceinline__ void z_body(const Box<S>& b, unsigned lb, double dt, int kchunk, int* __restrict__ err) {
   constexpr int NH = FLUXCD ? 5 : 8;
   constexpr int NQ = 8 + NP;  // the MHD variables and the passive scalars
@@ -550,7 +551,7 @@ __device__ __forceinline__ void z_body(const Box<S>& b, unsigned lb, double dt, 
   const double* __restrict__ W = b.W;
   const int lim = b.ph.limiter;
   constexpr bool INC = sizeof(S) == 4;  // W holds FP32 x/y flux increments
-  constexpr bool w8 = W8;
+  constexpr bool w8 = SRCIN;
 
   S ur[NQ];
   bool uflip = false;
@@ -649,12 +650,12 @@ __device__ __forceinline__ void z_body(const Box<S>& b, unsigned lb, double dt, 
     if (!riemann<SOLVER, NQ>(b.ph, left, right, ff)) atomicOr(err, 1);
     axis_perm<2>(ff);
     if (kf >= k0) {
-      // Powell's 8-wave terms (no user sources): added to the stored update
-      // here rather than in a pass of their own over U
+      // source terms (source(), src/sources.f90:188-220): added to the
+      // stored update here rather than in a pass of their own over U
       double sv[8];
 #pragma unroll
-      for (int q = 0; q < 8; q++) sv[q] = 0.0;
-      if (w8) w8_terms(b, Usrc, i, j, kf, sv);
+      for (int q = 0; q < 8; q++) sv[q] = (w8 && b.usrc) ? b.usrc[q * N + c] : 0.0;
+      if (w8 && (b.src & SRC_EIGHT_WAVE)) w8_terms(b, Usrc, i, j, kf, sv);
 #pragma unroll
       for (int q = 0; q < NH; q++) {
         S r = (S)(wv[q] - dtdz * ((double)ff[q] - (double)fl[q]));
@@ -741,12 +742,12 @@ __device__ __forceinline__ void bfield_body(const Box<S>& b, unsigned lb, double
     const double b5 = (d(Uin[5 * N + c]) - (0.5 * dtdy) * (d(E3[c + oyp]) - d(E3[c + oym]))) + (0.5 * dtdz) * (e2p - e2m);
     const double b6 = (d(Uin[6 * N + c]) + (0.5 * dtdx) * (d(E3[c + oxp]) - d(E3[c + oxm]))) - (0.5 * dtdz) * (e1p - e1m);
     const double b7 = (d(Uin[7 * N + c]) - (0.5 * dtdx) * (d(E2[c + oxp]) - d(E2[c + oxm]))) + (0.5 * dtdy) * (d(E1[c + oyp]) - d(E1[c + oym]));
-    if (!CFL && b.w8) {  // (the CFL fold runs only without source terms)
-      // Powell's terms of the field, from the stage's input state
+    if (!CFL && b.srcin) {  // (the CFL fold runs only without source terms)
+      // the field's source terms, from the stage's input state
       double sv[8];
 #pragma unroll
-      for (int q = 0; q < 8; q++) sv[q] = 0.0;
-      w8_terms(b, stage == 1 ? b.U : b.Us, i, j, k, sv);
+      for (int q = 0; q < 8; q++) sv[q] = b.usrc ? b.usrc[q * N + c] : 0.0;
+      if (b.src & SRC_EIGHT_WAVE) w8_terms(b, stage == 1 ? b.U : b.Us, i, j, k, sv);
       Uout[5 * N + c] = (S)((double)(S)b5 + dt * sv[5]);
       Uout[6 * N + c] = (S)((double)(S)b6 + dt * sv[6]);
       Uout[7 * N + c] = (S)((double)(S)b7 + dt * sv[7]);
@@ -781,14 +782,14 @@ __device__ __forceinline__ void bfield_body(const Box<S>& b, unsigned lb, double
 // update, up = up + dt*s with s from source() (src/sources.f90:188-220):
 // zero, plus the user's terms (already in W, written by the user plugin on
 // a zeroed W), then Powell's 8-wave terms (w8_terms) from the stage's input
-// state -- U^n for the predictor, U* for the corrector.  Without user
-// sources and passive scalars (Box::w8) k_z and the field update add the
-// 8-wave terms as they store each cell instead.  Column pencils.
+// state -- U^n for the predictor, U* for the corrector.  Without passive
+// scalars (Box::srcin) k_z and the field update add the terms as they store
+// each cell instead, the user's s computed before the stage.  Column pencils.
 // ---------------------------------------------------------------------------
 template <typename S>
 __device__ __forceinline__ void sources_body(const Box<S>& b, unsigned lb, double dt, int stage, int kchunk) {
   const Geom& g = b.g;
-  if (!b.src || b.w8) return;  // b.w8: the stage kernels added the 8-wave terms
+  if (!b.src || b.srcin) return;  // b.srcin: the stage kernels added the terms
   int t, kc;
   if (!tile_item(lb, tiles_of((unsigned)(g.nx * g.ny)), (unsigned)(g.nx * g.ny), t, kc)) return;
   const int i = t % g.nx + 1, j = t / g.nx + 1;
@@ -1600,7 +1601,7 @@ void launch_z(hipStream_t s, const Box<S>& b, double dt, int stage, int solver, 
   const unsigned n = box_tiles(b.g, KC_COL, sh);
 #define GM_CALL(SV, O, F)                                                                                  \
   {                                                                                                        \
-    if (b.w8) hipLaunchKernelGGL((k_z<SV, O, F, S, 0, true>), dim3(n), dim3(sh.block), 0, s, b, dt, sh.kchunk, err); \
+    if (b.srcin) hipLaunchKernelGGL((k_z<SV, O, F, S, 0, true>), dim3(n), dim3(sh.block), 0, s, b, dt, sh.kchunk, err); \
     else if (b.g.npas == 0) hipLaunchKernelGGL((k_z<SV, O, F, S, 0>), dim3(n), dim3(sh.block), 0, s, b, dt, sh.kchunk, err); \
     else if (b.g.npas == 1)                                                                                \
       hipLaunchKernelGGL((k_z<SV, O, F, S, 1>), dim3(n), dim3(sh.block), 0, s, b, dt, sh.kchunk, err);    \

@@ -39,8 +39,9 @@ struct Box {
   double dx, dy, dz;
   int slot;  // index of the box's clock in a batch's clock table
   int src;   // source terms: SRC_EIGHT_WAVE | SRC_USER (W holds the user's s)
-  int w8;    // 1: the stage kernels add the 8-wave terms as they store each
-             // cell (8-wave alone, no passive scalars); else the source pass does
+  int srcin;  // 1: the stage kernels add the source terms as they store each
+              // cell (no passive scalars); else the source pass does
+  const double* usrc;  // with srcin: the user's s of the stage (neq x ncell), or null
 };
 
 enum : int { SRC_EIGHT_WAVE = 1, SRC_USER = 2 };

@@ -36,12 +36,13 @@ def random_state(rng, nx, ny, nz, cv=1.5, contrast=0.3, vmax=1.5, bmax=1.0, npas
 
 
 def run_pair(params: _capi.GmParams, u0: np.ndarray, nsteps: int, cfl: float = 0.4, check_each=False,
-             user_sources=None, user_bc=None, eos=(1, 1.0)):
+             user_sources=None, user_bc=None, eos=(1, 1.0), bc_first=False):
     """Advance u0 nsteps on both paths with dt from each path's own CFL
     reduction; asserts the reductions agree bitwise every step.
     user_sources / user_bc: (code object, kernel) of plugins equal to the
     oracle's user source / user boundary hook; both sides see the same
     globals::time.  eos: (eq_of_state, Tempsc) of u2prim on both paths.
+    bc_first: register the boundary plugin before the source plugin.
     Returns (u_device, u_oracle, dts)."""
     O = Oracle(oracle_config(params, cfl, user_src=user_sources is not None, user_bc=user_bc is not None, eos=eos))
     O.set_u(u0)
@@ -50,9 +51,11 @@ def run_pair(params: _capi.GmParams, u0: np.ndarray, nsteps: int, cfl: float = 0
     dts = []
     with _capi.Subdomain(params) as dev:
         dev.set_eos(*eos)
+        if bc_first and user_bc is not None:
+            dev.set_user_bc(*user_bc)
         if user_sources is not None:
             dev.set_user_sources(*user_sources)
-        if user_bc is not None:
+        if user_bc is not None and not bc_first:
             dev.set_user_bc(*user_bc)
         dev.upload_u(u0)
         dev.boundary_u()

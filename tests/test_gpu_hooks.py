@@ -63,6 +63,17 @@ def test_user_bc_and_sources_together():
     np.testing.assert_array_equal(ud, uo)
 
 
+def test_boundary_plugin_registered_first():
+    """Registering the source plugin must leave an earlier boundary plugin
+    loaded (each setter releases only its own module)."""
+    say("case: jet hook registered before the gravity plugin, 14x12x10 outflow, 4 steps")
+    rng = np.random.default_rng(22)
+    u0 = random_state(rng, 14, 12, 10)
+    p = _capi.make_params(14, 12, 10, 1 / 14, 1 / 12, 1 / 10, bc=(4, 1, 1, 1, 1, 1))
+    ud, uo, _ = run_pair(p, u0, 4, check_each=True, user_bc=JET, user_sources=GRAVITY, bc_first=True)
+    np.testing.assert_array_equal(ud, uo)
+
+
 def test_hook_mixed_against_fp64():
     say("case: jet hook mixed vs FP64, 24^3 outflow, 8 steps")
     rng = np.random.default_rng(4)
