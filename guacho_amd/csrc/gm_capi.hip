@@ -60,8 +60,9 @@ struct gm_ctx {
   void* Us = nullptr;    // U* after the predictor
   void* W = nullptr;     // stage partials (neq doubles per cell); also the staging
                          // buffer for layout conversion
-  void* E = nullptr;     // cell-centred E (3 vars); also staging for Temp / divB
+  void* E = nullptr;     // cell-centred E (3 vars)
   void* Ssrc = nullptr;  // the user source plugin's s of a stage (neq doubles per cell)
+  void* Dl = nullptr;    // staging of the Temp / divB downloads (one double per cell), on first use
   Scalars* dscal = nullptr;  // device scalars
   Scalars* hscal = nullptr;  // pinned host mirror
   bool ghosts_dirty = true;
@@ -118,11 +119,30 @@ hipEvent_t take_event(gm_ctx* c) {
   return e;
 }
 
+// The records into the per-class totals, their events back to the pool;
+// the stream has been synchronised.
+void fold_recs(gm_ctx* c) {
+  for (auto& r : c->recs) {
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
+      c->ms[r.cls] += t;
+      c->launches[r.cls] += 1;
+    }
+    c->pool.push_back(r.a);
+    c->pool.push_back(r.b);
+  }
+  c->recs.clear();
+}
+
 template <class Fn>
 void timed(gm_ctx* c, int cls, Fn&& fn) {
   if (!c->timing) {
     fn();
     return;
+  }
+  if (c->recs.size() >= 4096) {  // bounded while nobody reads the times
+    (void)hipStreamSynchronize(c->stream);
+    fold_recs(c);
   }
   hipEvent_t a = take_event(c), b = take_event(c);
   if (!a || !b) {
@@ -201,7 +221,7 @@ void release(gm_ctx* c) {
     (void)hipEventDestroy(r.b);
   }
   for (auto e : c->pool) (void)hipEventDestroy(e);
-  void* bufs[] = {c->U, c->Us, c->W, c->E, c->Ssrc};
+  void* bufs[] = {c->U, c->Us, c->W, c->E, c->Ssrc, c->Dl};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->usrc_mod) (void)hipModuleUnload(c->usrc_mod);
@@ -213,6 +233,23 @@ void release(gm_ctx* c) {
 }
 
 int set_device(gm_ctx* c) { return hip_check(c, hipSetDevice(c->p.device), "hipSetDevice"); }
+
+// Staging of the one-value-per-cell downloads.  Not E or U*: E's ghosts at
+// BC_OTHER faces and U*'s ghosts outside a boundary hook's reach are read
+// as stored, so they carry state from step to step.
+int download_staging(gm_ctx* c, double** out) {
+  if (!c->Dl) {
+    const size_t n = (size_t)c->g.ncell * sizeof(double);
+    if (hipMalloc(&c->Dl, n) != hipSuccess) {
+      (void)hipGetLastError();
+      c->Dl = nullptr;
+      return fail(c, GM_ERR_OOM, "device allocation for the download staging failed");
+    }
+    c->bytes += (int64_t)n;
+  }
+  *out = (double*)c->Dl;
+  return GM_OK;
+}
 
 // Arguments of a user plugin launch.
 gm_user_args user_args(const gm_ctx* c, void* u, double* sv, double dt, int stage) {
@@ -576,7 +613,8 @@ int gm_download_primit(gm_ctx* c, double* primit, double* temp) {
   int rc = set_device(c);
   if (rc) return rc;
   if (c->ghosts_dirty) fill_ghost1(c);
-  double* T = temp ? (double*)c->E : nullptr;
+  double* T = nullptr;
+  if (temp && (rc = download_staging(c, &T)) != GM_OK) return rc;
   const int nq = c->p.neq;
   timed(c, KC_LAYOUT, [&] {
     if (c->esize == 8) gm::launch_prim_all(c->stream, c->g.ncell, c->ph, (const double*)c->U, (double*)c->W, T, nq);
@@ -586,7 +624,7 @@ int gm_download_primit(gm_ctx* c, double* primit, double* temp) {
   GM_HIP(c, hipMemcpyAsync(primit, c->W, (size_t)c->g.ncell * nq * sizeof(double), hipMemcpyDeviceToHost,
                            c->stream));
   if (temp)
-    GM_HIP(c, hipMemcpyAsync(temp, c->E, (size_t)c->g.ncell * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    GM_HIP(c, hipMemcpyAsync(temp, T, (size_t)c->g.ncell * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   GM_HIP(c, hipStreamSynchronize(c->stream));
   return GM_OK;
 }
@@ -596,11 +634,13 @@ int gm_download_divb(gm_ctx* c, double* divb) {
   if (!divb) return fail(c, GM_ERR_ARG, "divb is NULL");
   int rc = set_device(c);
   if (rc) return rc;
-  if (c->esize == 8) gm::launch_divb(c->stream, c->g, (const double*)c->U, c->p.dx, c->p.dy, c->p.dz, (double*)c->E);
-  else gm::launch_divb(c->stream, c->g, (const float*)c->U, c->p.dx, c->p.dy, c->p.dz, (double*)c->E);
+  double* D = nullptr;
+  if ((rc = download_staging(c, &D)) != GM_OK) return rc;
+  if (c->esize == 8) gm::launch_divb(c->stream, c->g, (const double*)c->U, c->p.dx, c->p.dy, c->p.dz, D);
+  else gm::launch_divb(c->stream, c->g, (const float*)c->U, c->p.dx, c->p.dy, c->p.dz, D);
   GM_HIP(c, hipGetLastError());
   const size_t n = (size_t)c->g.nx * c->g.ny * c->g.nz;
-  GM_HIP(c, hipMemcpyAsync(divb, c->E, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  GM_HIP(c, hipMemcpyAsync(divb, D, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   GM_HIP(c, hipStreamSynchronize(c->stream));
   return GM_OK;
 }
@@ -662,16 +702,7 @@ int gm_kernel_times(gm_ctx* c, double* ms, int64_t* launches, const char** names
   int rc = set_device(c);
   if (rc) return rc;
   GM_HIP(c, hipStreamSynchronize(c->stream));
-  for (auto& r : c->recs) {
-    float t = 0.f;
-    if (hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
-      c->ms[r.cls] += t;
-      c->launches[r.cls] += 1;
-    }
-    c->pool.push_back(r.a);
-    c->pool.push_back(r.b);
-  }
-  c->recs.clear();
+  fold_recs(c);
   for (int k = 0; k < GM_NKCLASS; k++) {
     if (ms) ms[k] = c->ms[k];
     if (launches) launches[k] = c->launches[k];

@@ -121,3 +121,31 @@ def test_batch_refuses_bc_other():
         with pytest.raises(_capi.GmError) as ei:
             B.add(p, random_state(rng, 8, 8, 8), max_steps=1)
         assert ei.value.code == _capi.GM_ERR_CONFIG
+
+
+@pytest.mark.parametrize("prec", [_capi.GM_PRECISION_FP64, _capi.GM_PRECISION_MIXED], ids=["fp64", "mixed"])
+def test_downloads_between_steps_change_nothing(prec):
+    """Temperature and div B downloads stage their output in device memory;
+    the E ghosts a BC_OTHER face reads must not be that memory."""
+    say(f"case: jet hook (BC_OTHER x face), downloads between steps, precision {prec}, 12^3, 5 steps")
+    rng = np.random.default_rng(25)
+    u0 = random_state(rng, 12, 12, 12)
+    p = _capi.make_params(12, 12, 12, 1 / 12, 1 / 12, 1 / 12, bc=(4, 1, 1, 1, 1, 1), precision=prec)
+
+    def run(downloads):
+        with _capi.Subdomain(p) as dev:
+            dev.set_user_bc(*JET)
+            dev.upload_u(u0)
+            dev.boundary_u()
+            t = 0.0
+            for it in range(1, 6):
+                dt, _ = timestep_clamp(it, 10, 0.4, dev.cfl_dtp(), t, 1e30)
+                dev.set_time(t)
+                dev.tstep(dt)
+                t += dt
+                if downloads and it in (2, 3):
+                    dev.download_primit(with_temp=True)
+                    dev.divb()
+            return dev.download_u()
+
+    np.testing.assert_array_equal(run(True), run(False))
