@@ -121,3 +121,22 @@ def test_not_fused_with_post_step_physics():
             assert dtp == _fresh_cfl(p, dev.download_u())
             times = dev.kernel_times()
         assert times["bfield_cfl"][1] == 0 and times["cfl_dt"][1] == 2, kw
+
+
+def test_timing_counts_past_the_record_fold():
+    """With timing on, records are folded into the totals every 4096; every
+    launch is still counted once."""
+    say("case: kernel timing over 800 steps of an 8^3 box")
+    rng = np.random.default_rng(59)
+    u0 = random_state(rng, 8, 8, 8, vmax=0.2, bmax=0.2)
+    p = _capi.make_params(8, 8, 8, 1 / 8, 1 / 8, 1 / 8)
+    with _capi.Subdomain(p) as dev:
+        dev.upload_u(u0)
+        dev.set_timing(True)
+        for _ in range(800):
+            dev.tstep(0.2 * dev.cfl_dtp())
+        times = dev.kernel_times()
+        assert all(times[k][1] == 800 for k in ("xy_predictor", "xy_corrector", "z_predictor", "z_corrector",
+                                               "bfield", "bfield_cfl")), times
+        assert times["cfl_dt"][1] == 1 and all(times[k][0] > 0 for k in ("xy_corrector", "z_corrector"))
+        assert dev.kernel_times()["xy_corrector"] == (0.0, 0)   # read and reset
