@@ -702,7 +702,7 @@ __device__ __forceinline__ int e_nbr(int i, int n, int d, int bc) {
   return 0;  // closed / outflow: ghost takes the edge cell's value
 }
 
-template <int NQ>
+template <int NQ, typename C>
 __device__ __forceinline__ double cfl_cell(const Phys& ph, double dx, double dy, double dz, const double* u,
                                            double dtp);
 
@@ -768,7 +768,7 @@ __device__ __forceinline__ void bfield_body(const Box<S>& b, unsigned lb, double
                         : (q < 8 + g.npas ? (double)Uout[q * N + c] : 0.0);
         *bad |= !__builtin_isfinite(u[q]);
       }
-      *dtp = cfl_cell<NQ>(b.ph, b.dx, b.dy, b.dz, u, *dtp);
+      *dtp = cfl_cell<NQ, S>(b.ph, b.dx, b.dy, b.dz, u, *dtp);
     }
     e1m = e1c;
     e2m = e2c;
@@ -1237,25 +1237,31 @@ __device__ __forceinline__ double wave_min(double v) {
 
 // get_timestep's candidate of one interior cell (src/hydro_core.f90:641-672):
 // min over the axes of d / (|v| + c), c the fast speed along the axis (MHD)
-// or the sound speed
-template <int NQ>
+// or the sound speed.  The primitives and the minimum are FP64; the wave
+// speeds and candidates are C (FP32 in the mixed form).
+template <int NQ, typename C>
 __device__ __forceinline__ double cfl_cell(const Phys& ph, double dx, double dy, double dz, const double* u,
                                            double dtp) {
-  double p[NQ];
-  u2prim<NQ>(ph, u, p);
+  using A = Ar<C>;
+  double pd[NQ];
+  u2prim<NQ>(ph, u, pd);
+  C p[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) p[q] = (C)pd[q];
+  const C g = (C)ph.gamma;
   if (ph.mhd) {
-    const double b2 = (p[5] * p[5] + p[6] * p[6]) + p[7] * p[7];
-    const double cx = cfast1(ph.gamma, p[4], p[0], b2, p[5]);
-    const double cy = cfast1(ph.gamma, p[4], p[0], b2, p[6]);
-    const double cz = cfast1(ph.gamma, p[4], p[0], b2, p[7]);
-    dtp = dmin(dtp, gm_div(dx, __builtin_fabs(p[1]) + cx));
-    dtp = dmin(dtp, gm_div(dy, __builtin_fabs(p[2]) + cy));
-    dtp = dmin(dtp, gm_div(dz, __builtin_fabs(p[3]) + cz));
+    const C b2 = (p[5] * p[5] + p[6] * p[6]) + p[7] * p[7];
+    const C cx = cfast1(g, p[4], p[0], b2, p[5]);
+    const C cy = cfast1(g, p[4], p[0], b2, p[6]);
+    const C cz = cfast1(g, p[4], p[0], b2, p[7]);
+    dtp = dmin(dtp, (double)A::div((C)dx, tabs(p[1]) + cx));
+    dtp = dmin(dtp, (double)A::div((C)dy, tabs(p[2]) + cy));
+    dtp = dmin(dtp, (double)A::div((C)dz, tabs(p[3]) + cz));
   } else {
-    const double cs = csound(ph, p[4], p[0]);
-    dtp = dmin(dtp, gm_div(dx, __builtin_fabs(p[1]) + cs));
-    dtp = dmin(dtp, gm_div(dy, __builtin_fabs(p[2]) + cs));
-    dtp = dmin(dtp, gm_div(dz, __builtin_fabs(p[3]) + cs));
+    const C cs = csound(ph, p[4], p[0]);
+    dtp = dmin(dtp, (double)A::div((C)dx, tabs(p[1]) + cs));
+    dtp = dmin(dtp, (double)A::div((C)dy, tabs(p[2]) + cs));
+    dtp = dmin(dtp, (double)A::div((C)dz, tabs(p[3]) + cs));
   }
   return dtp;
 }
@@ -1299,7 +1305,7 @@ __device__ __forceinline__ void cfl_body(const Box<S>& b, unsigned lb, int kchun
         u[q] = q < 8 + g.npas ? (double)b.U[q * g.ncell + c] : 0.0;
         bad |= !__builtin_isfinite(u[q]);
       }
-      dtp = cfl_cell<NQ>(b.ph, b.dx, b.dy, b.dz, u, dtp);
+      dtp = cfl_cell<NQ, S>(b.ph, b.dx, b.dy, b.dz, u, dtp);
     }
   }
   cfl_reduce(dtp, bad, dtmin, err);

@@ -163,10 +163,13 @@ __device__ __forceinline__ T csound(const Phys& ph, T pr, T d) {
   return Ar<T>::sqrt(Ar<T>::div((T)ph.gamma * pr, d));
 }
 
-// one component of cfast, src/hydro_core.f90:576-579 (FP64: the CFL reduction)
-__device__ __forceinline__ double cfast1(double g, double pr, double d, double b2, double bn) {
-  const double t = g * pr + b2;
-  return gm_sqrt(gm_div(0.5 * (t + gm_sqrt(t * t - (((4.0 * g) * pr) * bn) * bn)), d));
+// one component of cfast, src/hydro_core.f90:576-579 (the CFL candidates;
+// FP32 in the mixed form)
+template <typename T>
+__device__ __forceinline__ T cfast1(T g, T pr, T d, T b2, T bn) {
+  using A = Ar<T>;
+  const T t = g * pr + b2;
+  return A::sqrt(A::div(T(0.5) * (t + A::sqrt(t * t - (((T(4) * g) * pr) * bn) * bn)), d));
 }
 
 // cfastX, src/hydro_core.f90:591-604
