@@ -1,7 +1,7 @@
 /*
  * guacho_oracle.c -- CPU restatement of Guacho-3D's Godunov MHD step.
  *
- * TEST INFRASTRUCTURE: this is the checker for the HIP product path, never
+ * TEST INFRASTRUCTURE: this is the oracle for the HIP product path, never
  * the thing measured as the product and never linked into it.  Every
  * function restates the reference routine cited beside it, keeping the
  * reference's array layout (AoS, neq fastest), sweep structure and
@@ -584,7 +584,7 @@ static inline void copy_cell(orc_state *s, double *arr, int di, int dj, int dk,
 
 void orc_set_time(orc_state *s, double time) { s->time = time; }
 
-/* The checker's impose_user_bc (oracle/ref/user_mod.f90), in the Fortran's
+/* The oracle's impose_user_bc (oracle/ref/user_mod.f90), in the Fortran's
  * evaluation order: an inflow jet through the left x face (the ghost layers
  * 1-order..0, full transverse range, where (y-1/2)^2 + (z-1/2)^2 < 1/16)
  * with vx = 1 + time, and an obstacle of interior cells within 0.15 of
@@ -1119,7 +1119,7 @@ static double divergence_b(const orc_state *s, int i, int j, int k) {
          + (AT(pm, s, 7, i, j, k + 1) - AT(pm, s, 7, i, j, k - 1)) / (2.0 * p->dz);
 }
 
-/* The checker's get_user_source_terms (oracle/ref/user_mod.f90): uniform
+/* The oracle's get_user_source_terms (oracle/ref/user_mod.f90): uniform
  * gravity along -y and a softened point mass at the box centre, in the
  * Fortran's evaluation order.  Test infrastructure: a user's source terms
  * are user code, supplied to the library as a device plugin. */

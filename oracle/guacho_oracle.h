@@ -1,7 +1,7 @@
 /*
  * guacho_oracle.h -- CPU restatement of Guacho-3D's Godunov MHD step.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle for the HIP
  * product path: only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg load it.  It is never linked into, or called from,
  * the product library (guacho_amd/libguacho_amd.so).
@@ -39,8 +39,8 @@ typedef struct orc_params {
   int mhd, pmhd;          /* physics flags (parameters.f90:48-49) */
   int flux_cd;            /* enable_flux_cd (parameters.f90:63) */
   int eight_wave;         /* Powell 8-wave source terms (parameters.f90:61) */
-  int user_src;           /* the checker's user source (oracle/ref/user_mod.f90) */
-  int user_bc;            /* bc_user: the checker's impose_user_bc after boundaryI/II */
+  int user_src;           /* the oracle's user source (oracle/ref/user_mod.f90) */
+  int user_bc;            /* bc_user: the oracle's impose_user_bc after boundaryI/II */
   int nthreads;           /* OpenMP threads for the sweeps (<=0: runtime default) */
   int eos;                /* eq_of_state: 2 EOS_SINGLE_SPECIE, 3 EOS_H_RATE, else adiabatic */
   double tempsc;          /* Tempsc of the EOS_SINGLE_SPECIE / EOS_H_RATE temperature */
@@ -90,7 +90,7 @@ void orc_step(orc_state *s, double dt);
 void orc_viscous_copy(orc_state *s);
 int  orc_tstep(orc_state *s, double dt);      /* one full tstep() */
 double orc_cfl_dtp(const orc_state *s);       /* min loop of get_timestep */
-/* source(i,j,k,prim,s) of sources.f90:188-220: s = 0, the checker's user
+/* source(i,j,k,prim,s) of sources.f90:188-220: s = 0, the oracle's user
  * source when user_src, then divbcorr_8w_source from the state's primit
  * when eight_wave */
 void orc_source(const orc_state *s, int i, int j, int k, const double *prim, double *sv);

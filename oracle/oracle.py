@@ -1,4 +1,4 @@
-"""ctypes bindings to the parity checkers.
+"""ctypes bindings to the oracle libraries.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
 bench.py's cpu_baseline leg, never by the product package guacho_amd/.
@@ -65,8 +65,8 @@ class Config:
     pmhd: bool = False
     enable_flux_cd: bool = True
     eight_wave: bool = False
-    user_src: bool = False          # the checker's user gravity (oracle/ref/user_mod.f90)
-    user_bc: bool = False           # the checker's impose_user_bc (oracle/ref/user_mod.f90)
+    user_src: bool = False          # the oracle's user gravity (oracle/ref/user_mod.f90)
+    user_bc: bool = False           # the oracle's impose_user_bc (oracle/ref/user_mod.f90)
     cfl: float = 0.4
     eq_of_state: int = 1            # 2: EOS_SINGLE_SPECIE, 3: EOS_H_RATE
     tempsc: float = 1.0             # Tempsc of the EOS_SINGLE_SPECIE / EOS_H_RATE temperature

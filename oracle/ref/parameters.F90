@@ -1,9 +1,9 @@
 !=======================================================================
 ! Test-problem configuration for building the reference's own physics
 ! modules (src/hydro_core.f90, src/hll*.f90, src/boundaries.f90) into the
-! parity checker oracle/_ref.  Every Guacho problem supplies a module named
+! reference build oracle/_ref.  Every Guacho problem supplies a module named
 ! `parameters` (OT/parameters.f90 and EXO/parameters.f90 are the two the
-! reference ships); this one is the checker's problem, with the
+! reference ships); this one is the oracle's problem, with the
 ! compile-time choices taken from REF_* macros so one source builds each
 ! configuration variant listed in oracle/Makefile.
 !

@@ -1,6 +1,6 @@
 !=======================================================================
 ! C-callable entry points into the reference's own routines, for the
-! parity checker.  Test infrastructure only: oracle/Makefile links this
+! oracle.  Test infrastructure only: oracle/Makefile links this
 ! file with the reference's unmodified sources (compiled where they lie
 ! under the reference tree) into oracle/_ref/<variant>/libguacho_ref.so,
 ! which tests/ load to validate the C restatement (oracle/guacho_oracle.c)

@@ -1,7 +1,7 @@
 !=======================================================================
-! Problem plug-in of the checker's test problem: the four hooks every
+! Problem plug-in of the oracle's test problem: the four hooks every
 ! Guacho problem provides (interface as OT/user_mod.f90:43-122), all empty
-! because the checker sets the state through oracle/ref/ref_kat.f90.
+! because the oracle sets the state through oracle/ref/ref_kat.f90.
 !=======================================================================
 module user_mod
   implicit none
@@ -16,7 +16,7 @@ contains
     u = 0.
   end subroutine initial_conditions
 
-  ! The checker's user boundary (active in builds with bc_user), in the
+  ! The oracle's user boundary (active in builds with bc_user), in the
   ! style of EXO/user_mod.f90:131-146: an inflow jet through the left x face
   ! (ghost layers 1-order..0 where (y-1/2)^2 + (z-1/2)^2 < 1/16) with
   ! vx = 1 + time, and an obstacle of interior cells within 0.15 of
@@ -71,7 +71,7 @@ contains
     end do
   end subroutine impose_user_bc
 
-  ! The checker's user source (active in builds with user_source_terms):
+  ! The oracle's user source (active in builds with user_source_terms):
   ! uniform gravity along -y plus a softened point mass at the box centre,
   ! in the style of EXO/user_mod.f90:159-200 (r^3 written as r2*sqrt(r2)).
   subroutine get_user_source_terms(pp, s, i, j, k)

@@ -8,14 +8,14 @@ inputs and the reference's outputs for
   - one boundaryI + calcprim + first-order sweep on a random state,
   - one boundaryII + calcprim + limited sweep on a random state,
   - get_timestep's ramp and clamp (the boundary fills include the
-    checker's impose_user_bc in bc_user builds, at globals::time = 0.3),
+    oracle's impose_user_bc in bc_user builds, at globals::time = 0.3),
   - (ref_cooling.npz) coolingdmc and coolingchi with the reference's tables,
   - (ref_cooling_h.npz) coolingh, the H-rate cooling with the neutral
     density as the first passive scalar, under EOS_H_RATE,
   - thermal_conduction (isotropic / anisotropic, with and without
     saturation) on a static state with a random temperature field,
   - source(i,j,k,prim,s) of sources.f90 at every interior cell of a random
-    primitive array (zero, 8-wave, the checker's user gravity, or both,
+    primitive array (zero, 8-wave, the oracle's user gravity, or both,
     per the variant's switches).
 The fixtures are data (inputs and expected outputs); tests replay the
 inputs through the C restatement and require identical bits.

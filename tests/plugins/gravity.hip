@@ -1,7 +1,7 @@
-// Test plugin: the checker's user source term as a device kernel -- uniform
+// Test plugin: the oracle's user source term as a device kernel -- uniform
 // gravity along -y plus a softened point mass at the box centre, the same
 // arithmetic, in the same order, as the Fortran get_user_source_terms the
-// reference is built with for the parity checker (oracle/ref/user_mod.f90).
+// reference is built with for the oracle (oracle/ref/user_mod.f90).
 // Built by tests/plugins/Makefile into gravity.hsaco (gfx950 code object).
 #include "guacho_amd_user.h"
 

@@ -1,10 +1,10 @@
-// Test plugin: the checker's user boundary hook as a device kernel -- an
+// Test plugin: the oracle's user boundary hook as a device kernel -- an
 // inflow jet through the left x face (ghost layers 1-order..0 where
 // (y-1/2)^2 + (z-1/2)^2 < 1/16) with vx = 1 + time, and an obstacle of
 // interior cells within 0.15 of (0.7, 0.5, 0.5) reset to rest at density
 // 5, pressure 1, keeping B -- the same arithmetic, in the same order, as
 // the Fortran impose_user_bc the reference is built with for the parity
-// checker (oracle/ref/user_mod.f90).  Every cell is written from its own
+// oracle (oracle/ref/user_mod.f90).  Every cell is written from its own
 // values or constants only, so the parallel loop equals the serial one.
 #include "guacho_amd_user.h"
 

@@ -1,7 +1,7 @@
 """User boundary hooks (impose_user_bc, src/boundaries.f90:245, 508) on the
 device against the C restatement.
 
-The test plugin tests/plugins/jet.hip computes the checker's hook (an
+The test plugin tests/plugins/jet.hip computes the oracle's hook (an
 inflow jet through the left x face whose speed depends on globals::time,
 and an interior obstacle reset every call); the restatement's boundaryI /
 boundaryII with that hook are pinned bit for bit against the reference's

@@ -8,7 +8,7 @@ bit for bit against the reference's own sources.f90 (tests/golden
 fixtures ref_hlld_minmod_8wave, ref_hlle_minmod_usersrc_8wave,
 ref_hlld_minmod_usersrc); here the FP64 device path must equal the
 restatement bit for bit, every step.  The test plugin
-(tests/plugins/gravity.hip) computes the same gravity as the checker's
+(tests/plugins/gravity.hip) computes the same gravity as the oracle's
 Fortran user_mod.
 """
 import math

@@ -114,7 +114,7 @@ def test_source_fixtures_present():
 @pytest.mark.parametrize("path", SRC_GOLDEN, ids=lambda p: os.path.basename(p)[4:-4])
 def test_sources_match_reference(path):
     """source(i,j,k,prim,s) (sources.f90:188-220): Powell 8-wave terms
-    (:137-168) and the checker's user gravity, at every interior cell."""
+    (:137-168) and the oracle's user gravity, at every interior cell."""
     z = np.load(path)
     cfg = _cfg(z)
     O = Oracle(cfg)
