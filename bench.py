@@ -357,7 +357,7 @@ def serving_section(nreq: int, seed: int, fractions=(0.5, 1.0, 2.0)):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build runs one subdomain per GPU and no "
+        print(f"bench.py measures one GPU: the library runs one subdomain per GPU and no "
               f"multi-GPU path exists (asked for --gpus {args.gpus})", file=sys.stderr)
         sys.exit(2)
     from guacho_amd import _capi, problems

@@ -3,7 +3,7 @@
 The step (predictor/corrector with limited reconstruction, HLLD/HLLE/HLL/HLLC
 fluxes, flux-CD, boundaries, CFL reduction) runs as hand-written gfx950 HIP
 kernels in libguacho_amd.so behind the C ABI of include/guacho_amd.h.
-This package is the host side: the ctypes binding (_capi), the driver
+This package is the host side: the ctypes binding (_capi), the time loop
 mirroring the reference's main loop (host), problem set-ups (problems) and
 the BIN dump format (binio).
 """

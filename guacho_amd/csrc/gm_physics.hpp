@@ -24,7 +24,7 @@ namespace gm {
 // (gm_div, gm_sqrt: the IEEE results, on which the drop-in's bitwise parity
 // depends).  FP32 -- the mixed form's
 // reconstruction and fluxes -- uses the hardware reciprocal and square
-// root (v_rcp_f32, v_sqrt_f32, about 1 ulp), since its accuracy is judged
+// root (v_rcp_f32, v_sqrt_f32, about 1 ulp), since its accuracy is measured
 // against the FP64 path by tolerance.
 template <typename T>
 struct Ar;

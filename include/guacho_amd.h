@@ -2,7 +2,7 @@
  * guacho_amd.h -- C ABI of the MI355X Godunov MHD step library
  * (guacho_amd/libguacho_amd.so).
  *
- * The reference has no FFI: its hot path is Fortran called from the driver
+ * The reference has no FFI: its hot path is Fortran called from the time
  * loop src/main.f90:94-125.  Each entry point below replaces one reference
  * routine on that path; a Fortran host binds them with ISO_C_BINDING
  * (host/guacho_capi.f90) and Python with ctypes (guacho_amd/_capi.py).

@@ -336,7 +336,7 @@ class OracleFunctions:
         d = C.c_double(); self.L.ref_dt_cond(C.byref(d)); return d.value
 
     def source(self, i, j, k, prim):
-        """source(i,j,k,prim,s) of this build (sources.f90:188-220), from the
+        """source(i,j,k,prim,s) of this variant (sources.f90:188-220), from the
         primit array last set."""
         sv = np.zeros(self.neq)
         self.L.ref_source(i, j, k, np.ascontiguousarray(prim, dtype=np.float64), sv)
@@ -493,7 +493,7 @@ class RefLib:
         d = C.c_double(); self.L.ref_dt_cond(C.byref(d)); return d.value
 
     def source(self, i, j, k, prim):
-        """source(i,j,k,prim,s) of this build (sources.f90:188-220), from the
+        """source(i,j,k,prim,s) of this variant (sources.f90:188-220), from the
         primit array last set."""
         sv = np.zeros(self.neq)
         self.L.ref_source(i, j, k, np.ascontiguousarray(prim, dtype=np.float64), sv)

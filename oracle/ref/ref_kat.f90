@@ -185,7 +185,7 @@ contains
     call source(i, j, k, prim, sv)
   end subroutine ref_source
 
-  ! logical switches of this build: eight_wave, user_source_terms, bc_user
+  ! logical switches of this variant: eight_wave, user_source_terms, bc_user
   subroutine ref_switches(sw) bind(C, name='ref_switches')
     integer(c_int), intent(out) :: sw(3)
     sw = (/ merge(1, 0, eight_wave), merge(1, 0, user_source_terms), merge(1, 0, bc_user) /)
@@ -245,7 +245,7 @@ contains
     dt = d
   end subroutine ref_dt_cond
 
-  ! th_cond, tc_saturation, Tempsc of this build
+  ! th_cond, tc_saturation, Tempsc of this variant
   subroutine ref_conduction_config(kind, sat, tempsc_out) bind(C, name='ref_conduction_config')
     integer(c_int), intent(out) :: kind, sat
     real(c_double), intent(out) :: tempsc_out
@@ -254,7 +254,7 @@ contains
     tempsc_out = Tempsc
   end subroutine ref_conduction_config
 
-  ! eq_of_state of this build
+  ! eq_of_state of this variant
   subroutine ref_eos(eos) bind(C, name='ref_eos')
     integer(c_int), intent(out) :: eos
     eos = eq_of_state
