@@ -124,6 +124,22 @@ def test_scaled_state_bitwise(log2a, solver, mhd, pmhd, lim):
     assert dts[0] == dts0[0]
 
 
+@pytest.mark.parametrize("shape,bc", [((2, 2, 2), (P,) * 6), ((2, 3, 2), (O_,) * 6), ((3, 2, 5), (C_,) * 6),
+                                      ((33, 17, 3), (P, P, O_, O_, C_, C_)), ((47, 31, 18), (O_, O_, P, P, P, P)),
+                                      ((64, 48, 33), (C_, C_, P, P, O_, O_))],
+                         ids=["2x2x2-per", "2x3x2-out", "3x2x5-closed", "33x17x3", "47x31x18", "64x48x33"])
+def test_minimal_and_ragged_sizes_bitwise(shape, bc):
+    """Grids of 2-3 cells (the two-layer ghost mirror at its limit) and
+    sizes one past the 32 x 16 plane tile and the 16-plane z-pencil."""
+    nx, ny, nz = shape
+    say(f"case: HLLD minmod flux-CD {nx}x{ny}x{nz} bc {bc}, 4 steps, checked each step")
+    rng = np.random.default_rng(nx * 10007 + ny * 101 + nz)
+    u0 = random_state(rng, nx, ny, nz)
+    p = params(nx, ny, nz, 1 / nx, 1 / ny, 1 / nz, bc=bc)
+    ud, uo, _ = run_pair(p, u0, 4, check_each=True)
+    np.testing.assert_array_equal(ud, uo)
+
+
 def test_nonfinite_state_is_reported():
     say("case: NaN in one interior cell -> GM_ERR_NONFINITE from the CFL reduction")
     u0, dx, dy, dz = problems.orszag_tang(16, 16, 4)
