@@ -29,6 +29,14 @@ enum { KC_XY1, KC_XY2, KC_Z1, KC_Z2, KC_BFIELD, KC_CFL, KC_GHOST, KC_LAYOUT, KC_
 
 thread_local std::string g_create_error;
 
+// After table cooling the reference refreshes only the ghost primitives
+// (src/hydro_solver.f90:218-223), so its conduction starts from the
+// temperature of the half-step state in the interior; that combination is
+// not built.  H-rate cooling is followed by a full calcprim and combines.
+const char* const kTableCoolCond =
+    "table cooling (GM_COOL_DMC / GM_COOL_CHI) with thermal conduction is not built: the reference's conduction "
+    "then starts from the half-step temperature of the interior cells";
+
 struct Scalars {
   unsigned long long dtmin;
   unsigned long long tcmin;  // conduction: min rho / Ksp(T) (bits of a positive double)
@@ -796,6 +804,7 @@ int gm_set_cooling(gm_ctx* c, int32_t kind, const double* cooltab, double tempsc
   if (kind == GM_COOL_H && c->g.npas < 1)
     return fail(c, GM_ERR_CONFIG, "GM_COOL_H needs the neutral density as a passive scalar (neq >= 9)");
   if (!cooltab && kind != GM_COOL_H) return fail(c, GM_ERR_ARG, "cooltab is NULL");
+  if (kind != GM_COOL_H && c->cond.kind) return fail(c, GM_ERR_CONFIG, kTableCoolCond);
   if (!(tempsc > 0) || !(psc > 0) || !(tsc > 0) || !std::isfinite(tempsc) || !std::isfinite(psc) ||
       !std::isfinite(tsc))
     return fail(c, GM_ERR_CONFIG, "tempsc, psc and tsc must be positive and finite");
@@ -822,6 +831,7 @@ int gm_set_conduction(gm_ctx* c, const gm_conduction_params* tp) {
   }
   if (tp->kind != GM_TC_ISOTROPIC && tp->kind != GM_TC_ANISOTROPIC)
     return fail(c, GM_ERR_CONFIG, "conduction kind must be GM_TC_OFF, GM_TC_ISOTROPIC or GM_TC_ANISOTROPIC");
+  if (c->cool.kind && c->cool.kind != GM_COOL_H) return fail(c, GM_ERR_CONFIG, kTableCoolCond);
   const double v[] = {tp->tempsc, tp->rsc, tp->rhosc, tp->mu, tp->psc, tp->tsc, tp->vsc2, tp->bsc};
   for (double x : v)
     if (!(x > 0) || !std::isfinite(x)) return fail(c, GM_ERR_CONFIG, "conduction scalings must be positive and finite");

@@ -194,3 +194,58 @@ def test_passive_argument_checks():
         with pytest.raises(_capi.GmError) as ei:
             B.add(p2, random_state(rng, 8, 8, 8, npas=2), max_steps=1)
         assert ei.value.code == _capi.GM_ERR_CONFIG
+
+
+def test_h_rate_cooling_with_conduction():
+    """H-rate cooling is followed by a full calcprim (src/hydro_solver.f90:
+    218-220), so the conduction after it starts from the cooled state's
+    temperature, as on the device.  One step of two conduction sub-steps:
+    fields neither module touches bit for bit, E and the neutral density to
+    a relative 1e-10."""
+    say("case: H-rate cooling + isotropic conduction, EOS_H_RATE, npas=2, 12^3 periodic, 1 step")
+    tempsc, tsc = 1.0e4, 1.0e13
+    n = 12
+    rng = np.random.default_rng(43)
+    u0 = _h_state(rng, n, tempsc)
+    p = _capi.make_params(n, n, n, 1 / n, 1 / n, 1 / n, npas=2)
+    O = Oracle(oracle_config(p, eos=(HR, tempsc)))
+    O.set_cooling(_capi.GM_COOL_H, None, tempsc, 1.0, tsc)
+    O.set_conduction(1, False, tempsc=tempsc, tsc=tsc)
+    O.set_u(u0)
+    O.boundaryI()
+    O.calcprim_u()
+    dt = 3.0 * O.dt_cond() / tsc
+    O.tstep(dt)
+    uo = O.get_u()
+    O.close()
+    with _capi.Subdomain(p) as dev:
+        dev.set_eos(HR, tempsc)
+        dev.set_cooling(_capi.GM_COOL_H, None, tempsc, 1.0, tsc)
+        dev.set_conduction(1, False, tempsc=tempsc, tsc=tsc)
+        dev.upload_u(u0)
+        dev.boundary_u()
+        dev.tstep(dt)
+        ud = dev.download_u()
+        assert dev.conduction_substeps() == 2
+    for q in (0, 1, 2, 3, 5, 6, 7, 9):
+        np.testing.assert_array_equal(ud[q], uo[q], err_msg=f"field {q}")
+    for q in (4, 8):
+        np.testing.assert_allclose(ud[q], uo[q], rtol=1e-10, atol=0, err_msg=f"field {q}")
+    inner = (slice(2, -2),) * 3
+    assert np.abs(ud[4][inner] - u0[4][inner]).max() > 0
+
+
+def test_table_cooling_with_conduction_is_refused():
+    say("case: DMC cooling with conduction refused, either order")
+    tab = np.load(os.path.join(HERE, "golden", "ref_cooling.npz"))["dmc_table"]
+    p = _capi.make_params(8, 8, 8, 1 / 8, 1 / 8, 1 / 8)
+    with _capi.Subdomain(p) as dev:
+        dev.set_cooling(_capi.GM_COOL_DMC, tab, 1e6, 1e-22, 10.0)
+        with pytest.raises(_capi.GmError) as ei:
+            dev.set_conduction(1, False)
+        assert ei.value.code == _capi.GM_ERR_CONFIG
+        dev.set_cooling(_capi.GM_COOL_NONE)
+        dev.set_conduction(1, False)
+        with pytest.raises(_capi.GmError) as ei:
+            dev.set_cooling(_capi.GM_COOL_CHI, tab, 1e6, 1e-22, 10.0)
+        assert ei.value.code == _capi.GM_ERR_CONFIG
