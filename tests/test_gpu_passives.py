@@ -154,6 +154,38 @@ def test_h_rate_cooling(nsteps):
         assert np.max(np.abs(ud - uo) / scale) <= 1e-9
 
 
+def test_h_rate_cooling_adiabatic_eos():
+    """coolingh reads u2prim's temperature, so under EOS_ADIABATIC it cools
+    with T = (p / rho) Tempsc; one step against the oracle."""
+    say("case: H-rate cooling + EOS_ADIABATIC, npas=1, 12^3 periodic, 1 step")
+    tempsc, tsc = 1.0e4, 1.0e13
+    n = 12
+    rng = np.random.default_rng(47)
+    u0 = _h_state(rng, n, tempsc)[:9].copy(order="F")
+    p = _capi.make_params(n, n, n, 1 / n, 1 / n, 1 / n, npas=1)
+    O = Oracle(oracle_config(p, eos=(_capi.GM_EOS_ADIABATIC, tempsc)))
+    O.set_cooling(_capi.GM_COOL_H, None, tempsc, 1.0, tsc)
+    O.set_u(u0)
+    O.boundaryI()
+    O.calcprim_u()
+    dt, _ = timestep_clamp(1, 10, 0.4, O.cfl_dtp(), 0.0, 1e30)
+    O.tstep(dt)
+    uo = O.get_u()
+    O.close()
+    with _capi.Subdomain(p) as dev:
+        dev.set_eos(_capi.GM_EOS_ADIABATIC, tempsc)
+        dev.set_cooling(_capi.GM_COOL_H, None, tempsc, 1.0, tsc)
+        dev.upload_u(u0)
+        dev.boundary_u()
+        dev.tstep(dt)
+        ud = dev.download_u()
+    for q in (0, 1, 2, 3, 5, 6, 7):
+        np.testing.assert_array_equal(ud[q], uo[q], err_msg=f"field {q}")
+    for q in (4, 8):
+        np.testing.assert_allclose(ud[q], uo[q], rtol=1e-10, atol=0, err_msg=f"field {q}")
+    assert not np.array_equal(ud[8], u0[8])
+
+
 def test_passives_mixed_against_fp64():
     say("case: passives npas=2 mixed vs FP64, 24^3 periodic, 8 steps")
     rng = np.random.default_rng(37)
