@@ -107,3 +107,31 @@ def test_mixed_eight_wave_close(fcd):
     say(f"relative L1 per field: {e}")
     assert e[0] <= 1e-5, e
     assert np.all(e <= 1e-4), e
+
+
+def test_mixed_user_sources_close():
+    """The mixed form with a user source plugin (its s added by the stage
+    kernels) against FP64 with the same plugin."""
+    import os
+    plugin = (os.path.join(os.path.dirname(os.path.abspath(__file__)), "plugins", "gravity.hsaco"),
+              "gm_test_gravity")
+    say("case: mixed vs FP64, blast 24^3 outflow + gravity plugin + 8-wave, 10 steps")
+    u0, dx, dy, dz = problems.blast(24, seed=21)
+    out = {}
+    for prec in (_capi.GM_PRECISION_FP64, _capi.GM_PRECISION_MIXED):
+        p = _capi.make_params(24, 24, 24, dx, dy, dz, bc=(1,) * 6, eight_wave=True, precision=prec)
+        with _capi.Subdomain(p) as dev:
+            dev.set_user_sources(*plugin)
+            dev.upload_u(u0)
+            dev.boundary_u()
+            t = 0.0
+            for it in range(1, 11):
+                dt, _ = timestep_clamp(it, 10, 0.4, dev.cfl_dtp(), t, 1e30)
+                dev.set_time(t)
+                dev.tstep(dt)
+                t += dt
+            out[prec] = dev.download_u()
+    e = rel_l1(out[_capi.GM_PRECISION_MIXED], out[_capi.GM_PRECISION_FP64])
+    say(f"relative L1 per field: {e}")
+    assert e[0] <= 1e-5, e
+    assert np.all(e <= 1e-4), e
