@@ -18,7 +18,8 @@ import pytest
 
 from guacho_amd import _capi
 from guacho_amd.host import timestep_clamp
-from tests._pairs import random_state, run_pair
+from oracle.oracle import Oracle
+from tests._pairs import oracle_config, random_state, run_pair
 
 pytestmark = pytest.mark.gpu
 
@@ -110,3 +111,36 @@ def test_batch_refuses_viscosity():
         with pytest.raises(_capi.GmError) as ei:
             B.add(p, random_state(rng, 8, 8, 8), max_steps=1)
         assert ei.value.code == _capi.GM_ERR_CONFIG
+
+
+def test_viscosity_then_table_cooling():
+    """viscous_copy runs before cooling (src/hydro_solver.f90:188, 206-210):
+    one step with both against the oracle; every field but E bit for bit, E
+    to 1e-14 (the device's exp and log10)."""
+    say("case: eta = 0.02 + DMC cooling, 16^3 outflow, 1 step")
+    fixture = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_cooling.npz")
+    tab = np.load(fixture)["dmc_table"]
+    tempsc, psc, tsc = 1e6, 1e-22, 10.0
+    rng = np.random.default_rng(61)
+    n = 16
+    u0 = random_state(rng, n, n, n)
+    p = _capi.make_params(n, n, n, 1 / n, 1 / n, 1 / n, bc=(1,) * 6, eta=0.02)
+    O = Oracle(oracle_config(p))
+    O.set_cooling(_capi.GM_COOL_DMC, tab, tempsc, psc, tsc)
+    O.set_u(u0)
+    O.boundaryI()
+    O.calcprim_u()
+    with _capi.Subdomain(p) as dev:
+        dev.set_cooling(_capi.GM_COOL_DMC, tab, tempsc, psc, tsc)
+        dev.upload_u(u0)
+        dev.boundary_u()
+        dt, _ = timestep_clamp(1, 10, 0.4, O.cfl_dtp(), 0.0, 1e30)
+        assert dev.cfl_dtp() == O.cfl_dtp()
+        O.tstep(dt)
+        dev.tstep(dt)
+        ud = dev.download_u()
+    uo = O.get_u()
+    O.close()
+    for q in (0, 1, 2, 3, 5, 6, 7):
+        np.testing.assert_array_equal(ud[q], uo[q], err_msg=f"field {q}")
+    np.testing.assert_allclose(ud[4], uo[4], rtol=1e-14, atol=0)
