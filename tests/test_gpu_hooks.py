@@ -149,3 +149,15 @@ def test_downloads_between_steps_change_nothing(prec):
             return dev.download_u()
 
     np.testing.assert_array_equal(run(True), run(False))
+
+
+def test_hook_sources_eight_wave_viscosity_inline():
+    """Without passive scalars the source terms are added by the stage
+    kernels: a boundary hook, the gravity plugin, 8-wave and viscosity in
+    one run, bit for bit against the oracle every step."""
+    say("case: jet hook + gravity + 8-wave + eta, npas=0, 14x12x10, 5 steps, checked each step")
+    rng = np.random.default_rng(27)
+    u0 = random_state(rng, 14, 12, 10)
+    p = _capi.make_params(14, 12, 10, 1 / 14, 1 / 12, 1 / 10, bc=(4, 1, 1, 1, 2, 2), eight_wave=True, eta=0.02)
+    ud, uo, _ = run_pair(p, u0, 5, check_each=True, user_bc=JET, user_sources=GRAVITY)
+    np.testing.assert_array_equal(ud, uo)
