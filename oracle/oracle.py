@@ -414,6 +414,16 @@ class RefLib:
         ev = C.c_int()
         L.ref_eos(C.byref(ev))
         self.eq_of_state = ev.value
+        # step variants (oracle/Makefile, flux_cd = 0) also carry the
+        # reference's own tstep / step / viscous_copy
+        self.has_tstep = hasattr(L, "ref_tstep")
+        self.enable_flux_cd, self.eta = True, 0.0
+        if self.has_tstep:
+            fcd, etav = C.c_int(), C.c_double()
+            L.ref_step_config(C.byref(fcd), C.byref(etav))
+            self.enable_flux_cd, self.eta = bool(fcd.value), etav.value
+            L.ref_tstep.argtypes = [C.c_double]
+            L.ref_step.argtypes = [C.c_double]
 
     def config(self, dx=None, dy=None, dz=None) -> Config:
         dx = dx if dx is not None else 1.0 / self.nx
@@ -421,7 +431,8 @@ class RefLib:
         dz = dz if dz is not None else 1.0 / self.nz
         return Config(nx=self.nx, ny=self.ny, nz=self.nz, dx=dx, dy=dy, dz=dz, cv=self.cv,
                       riemann_solver=self.riemann_solver, slope_limiter=self.slope_limiter, bc=list(self.bc),
-                      mhd=self.mhd, pmhd=self.pmhd, enable_flux_cd=True, cfl=self.cfl,
+                      mhd=self.mhd, pmhd=self.pmhd, enable_flux_cd=self.enable_flux_cd, eta=self.eta,
+                      cfl=self.cfl,
                       eight_wave=self.eight_wave, user_src=self.user_source_terms, user_bc=self.bc_user,
                       eq_of_state=self.eq_of_state, tempsc=self.tempsc, npas=self.npas)
 
@@ -441,6 +452,18 @@ class RefLib:
         a = np.zeros(self.shape()[1:], order="F"); self.L.ref_get_temp(a); return a
 
     def set_time(self, t): self.L.ref_set_time(float(t))
+
+    def tstep(self, dt):
+        """tstep() of hydro_solver.f90:134-229 with dt_CFL = dt (step variants)."""
+        self.L.ref_tstep(float(dt))
+
+    def step(self, dt):
+        """step(dt) of hydro_solver.f90:77-127 on the current globals (step variants)."""
+        self.L.ref_step(float(dt))
+
+    def viscous_copy(self):
+        self.L.ref_viscous_copy()
+
     def boundary(self, order): self.L.ref_boundary(order)
     def calcprim(self, which): self.L.ref_calcprim(which)
     def fluxes(self, choice): self.L.ref_fluxes(choice)

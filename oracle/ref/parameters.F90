@@ -10,7 +10,17 @@
 ! Serial build (no MPIP): the reference's hydro_core.f90:625 imports
 ! `mpi_real_kind` from this module unconditionally, so it is declared here
 ! as an unused integer; no MPI routine is called on the serial path.
+!
+! REF_FLUX_CD (default 1) and REF_ETA (default 0.) select enable_flux_cd
+! and the artificial viscosity; the step builds (REF_FLUX_CD = 0) link the
+! reference's hydro_solver.f90 and call its tstep.
 !=======================================================================
+#ifndef REF_FLUX_CD
+#define REF_FLUX_CD 1
+#endif
+#ifndef REF_ETA
+#define REF_ETA 0.0
+#endif
 module parameters
   use constants
   implicit none
@@ -19,7 +29,7 @@ module parameters
   logical, parameter :: mhd  = (REF_MHD /= 0)
   integer, parameter :: riemann_solver = REF_SOLVER
   logical, parameter :: eight_wave = (REF_EIGHT_WAVE /= 0)
-  logical, parameter :: enable_flux_cd = .true.
+  logical, parameter :: enable_flux_cd = (REF_FLUX_CD /= 0)
   logical, parameter :: dump_divb = .false.
   logical, parameter :: out_bin = .false.
   logical, parameter :: out_vtk = .false.
@@ -60,7 +70,7 @@ module parameters
   real, parameter :: tmax = 1.0
   real, parameter :: dtprint = 0.1
   real, parameter :: cfl = REF_CFL
-  real, parameter :: eta = 0.0
+  real, parameter :: eta = REF_ETA
   logical, parameter :: iwarm = .false.
   integer :: itprint0 = 0
 

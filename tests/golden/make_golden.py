@@ -16,7 +16,11 @@ inputs and the reference's outputs for
     saturation) on a static state with a random temperature field,
   - source(i,j,k,prim,s) of sources.f90 at every interior cell of a random
     primitive array (zero, 8-wave, the oracle's user gravity, or both,
-    per the variant's switches).
+    per the variant's switches),
+  - (tstep_<variant>.npz, the step variants) the reference's own main-loop
+    iteration, get_timestep then tstep (hydro_solver.f90:134-229), four
+    times from a random state: the dt sequence, a CRC of u after every
+    step and the final u and primit, ghosts included.
 The fixtures are data (inputs and expected outputs); tests replay the
 inputs through the C restatement and require identical bits.
 """
@@ -240,12 +244,62 @@ def make_cooling_h(seed=78):
             "config_real": np.array([R.cv, R.gamma, R.cfl])}
 
 
+# main-loop iterations of the tstep fixtures: two in the CFL ramp
+# (hydro_core.f90:677-682), two at the full CFL number
+TSTEP_ITERS = (1, 2, 11, 12)
+
+
+def tstep_state(R, rng):
+    """A random state for the tstep fixtures; for conduction builds the
+    thermal energy is raised to ~1e6 K so conduction sub-steps."""
+    u = random_state(rng, R.shape(), R.cv)
+    if R.th_cond:
+        temp = 10.0 ** rng.uniform(5.8, 6.2, R.shape()[1:])
+        u[4] = u[4] + R.cv * u[0] * temp / R.tempsc
+    return u
+
+
+def make_tstep(variant, seed):
+    """get_timestep + tstep of the reference (step variants, flux-CD off),
+    as main.f90:94-125 calls them, on a random state."""
+    R = RefLib(variant)
+    assert R.has_tstep and not R.enable_flux_cd
+    dx, dy, dz = 0.1, 0.07, 0.13
+    R.init(dx, dy, dz)
+    rng = np.random.default_rng(seed)
+    t = 0.3 if R.bc_user else 0.0
+    R.set_time(t)
+    u = tstep_state(rng=rng, R=R)
+    R.set_array(0, u)
+    R.boundary(1)
+    R.calcprim(0)
+    dts, crcs = [], []
+    for it in TSTEP_ITERS:
+        dt, _ = R.get_timestep(it, 10, t, 10.0)
+        R.tstep(dt)
+        dts.append(dt)
+        crcs.append(zlib.crc32(np.ascontiguousarray(R.get_array(0)).tobytes()))
+    cfg = np.array([R.nx, R.ny, R.nz, R.riemann_solver, R.slope_limiter, *R.bc, int(R.mhd), int(R.pmhd)])
+    return {"u_in": u, "iters": np.array(TSTEP_ITERS), "dt": np.array(dts), "u_crc": np.array(crcs, dtype=np.uint32),
+            "u_out": R.get_array(0), "primit_out": R.get_array(2), "time": np.array([t]),
+            "dxyz": np.array([dx, dy, dz]), "config_int": cfg, "config_real": np.array([R.cv, R.gamma, R.cfl]),
+            "switches": np.array([int(R.eight_wave), int(R.user_source_terms), int(R.bc_user)]),
+            "eos_config": np.array([R.eq_of_state, R.tempsc]), "npas": np.array([R.npas]),
+            "eta": np.array([R.eta]), "tc_config": np.array([R.th_cond, int(R.tc_saturation), R.tempsc])}
+
+
 def main():
     vs = ref_variants()
     if not vs:
         raise SystemExit("oracle/_ref is not built (make -C oracle ref)")
     rewrite = "--all" in sys.argv
     for v in vs:
+        if v.startswith("step_"):
+            path = os.path.join(HERE, f"tstep_{v[5:]}.npz")
+            if rewrite or not os.path.exists(path):
+                np.savez_compressed(path, **make_tstep(v, seed_of(v)))
+                print("wrote", v)
+            continue
         path = os.path.join(HERE, f"ref_{v}.npz")
         if os.path.exists(path) and not rewrite:
             continue

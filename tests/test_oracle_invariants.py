@@ -1,7 +1,8 @@
-"""Invariants that pin the restatement's step where the reference cannot be
-compiled here: flux_cd_module.f90 needs MPI (its serial branch at :120 is
-never closed) and hydro_solver.f90 uses it, so get_efield, flux_cd_update,
-step, viscous_copy and tstep have no reference build.  Each check below is
+"""Invariants of the restatement's step.  The reference's own tstep, step and
+viscous_copy pin it bit for bit with flux-CD off (tests/test_oracle_tstep.py);
+flux_cd_module.f90 does not compile serially (its non-MPI branch leaves an
+`if` open at :120), so get_efield, flux_cd_update and boundaryI_ef are
+pinned by tests/test_oracle_fluxcd.py and by the checks below.  Each is
 fixed by the mathematics and fails for a plausible slip (a dropped term, a
 wrong sign or index, a transposed operand):
 
