@@ -421,7 +421,8 @@ int gm_batch_add(gm_batch* b, const gm_params* p, const double* u, double cfl, d
     return bfail(b, GM_ERR_CONFIG, "box precision, solver and flux-CD must match the batch");
   if (!(cfl > 0) || !std::isfinite(cfl)) return bfail(b, GM_ERR_ARG, "cfl must be positive");
   if (!(t_end > 0)) return bfail(b, GM_ERR_ARG, "t_end must be positive (use +inf for no time limit)");
-  if (max_steps < 0) return bfail(b, GM_ERR_ARG, "max_steps must be >= 0");
+  if (max_steps < 0 || max_steps > INT32_MAX)
+    return bfail(b, GM_ERR_ARG, "max_steps must be in 0..2147483647");
   if (std::isinf(t_end) && max_steps == 0) return bfail(b, GM_ERR_ARG, "a box needs t_end or max_steps");
   int slot = -1;
   for (int i = 0; i < b->bp.max_boxes; i++)

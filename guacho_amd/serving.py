@@ -38,6 +38,7 @@ class Request:
     # filled by the server
     admitted: float = -1.0
     finished: float = -1.0
+    rejected: bool = False        # its box exceeds the whole memory budget
     steps: int = 0
     result: Optional[np.ndarray] = field(default=None, repr=False)
 
@@ -79,27 +80,54 @@ def make_stream(n: int, rate: float, seed: int, precision: int = _capi.GM_PRECIS
 
 
 def serve(requests: List[Request], steps_per_launch: int = 4, memory_budget: int = 0, keep_results: bool = False,
-          max_boxes: int = 256, clock: Callable[[], float] = time.perf_counter) -> Dict[str, float]:
-    """Serve the requests with continuous batching (mixed precision)."""
+          max_boxes: int = 256, clock: Callable[[], float] = time.perf_counter,
+          batch_factory: Optional[Callable[..., object]] = None) -> Dict[str, float]:
+    """Serve the requests with continuous batching (mixed precision).
+
+    Admission: every arrived request whose box fits the free memory and a
+    free slot is admitted, in arrival order, and a request that does not fit
+    does not hold back smaller ones behind it.  A request whose box exceeds
+    the whole budget can never run: it is rejected on arrival (r.rejected).
+    """
     pending = sorted(requests, key=lambda r: r.arrival)
     precision = pending[0].params.precision
     active: Dict[int, Request] = {}
-    with _capi.Batch(precision=precision, max_boxes=max_boxes, steps_per_launch=steps_per_launch,
-                     memory_budget=memory_budget) as B:
+    make = batch_factory or _capi.Batch
+    with make(precision=precision, max_boxes=max_boxes, steps_per_launch=steps_per_launch,
+              memory_budget=memory_budget) as B:
+        budget = B.free_bytes()
+        waiting: List[Request] = []
         t0 = clock()
         qi = 0
-        while qi < len(pending) or active:
+        while qi < len(pending) or waiting or active:
             now = clock() - t0
-            # admit every arrived request that fits, in arrival order
             while qi < len(pending) and pending[qi].arrival <= now:
                 r = pending[qi]
-                if B.box_bytes(r.params) > B.free_bytes():
-                    break
-                box = B.add(r.params, r.u0, cfl=r.cfl, t_end=r.t_end, max_steps=r.max_steps)
+                qi += 1
+                if B.box_bytes(r.params) > budget:
+                    r.rejected = True
+                    r.finished = now
+                    continue
+                waiting.append(r)
+            # admit every waiting request that fits, oldest first
+            still: List[Request] = []
+            for r in waiting:
+                if len(active) >= max_boxes or B.box_bytes(r.params) > B.free_bytes():
+                    still.append(r)
+                    continue
+                try:
+                    box = B.add(r.params, r.u0, cfl=r.cfl, t_end=r.t_end, max_steps=r.max_steps)
+                except _capi.GmError as e:
+                    if e.code != _capi.GM_ERR_BUSY:
+                        raise
+                    still.append(r)
+                    continue
                 r.admitted = clock() - t0
                 active[box] = r
-                qi += 1
+            waiting = still
             if not active:
+                if waiting:
+                    raise RuntimeError("waiting requests fit the budget but none can be admitted with no box active")
                 if qi < len(pending):
                     time.sleep(max(0.0, pending[qi].arrival - (clock() - t0)))
                 continue
@@ -151,9 +179,11 @@ def serve_sequential_fp64(requests: List[Request], keep_results: bool = False,
 
 
 def summarize(requests: List[Request], makespan: float) -> Dict[str, float]:
-    tts = np.array([r.finished - r.arrival for r in requests])
-    work = float(sum(r.cells * r.steps for r in requests))
-    return {"requests": len(requests), "makespan_s": makespan, "zone_updates": work,
+    served = [r for r in requests if not r.rejected]
+    tts = np.array([r.finished - r.arrival for r in served]) if served else np.zeros(1)
+    work = float(sum(r.cells * r.steps for r in served))
+    return {"requests": len(served), "rejected": len(requests) - len(served), "makespan_s": makespan,
+            "zone_updates": work,
             "zone_updates_per_s": work / makespan if makespan > 0 else 0.0,
             "p50_s": float(np.percentile(tts, 50)), "p99_s": float(np.percentile(tts, 99)),
             "mean_s": float(tts.mean())}

@@ -283,7 +283,8 @@ int gm_batch_free_bytes(const gm_batch *batch, int64_t *bytes);
 /* Admit a box: uploads u (the reference layout, ghosts included) and
  * starts its clock at time 0, currentIteration 1.  It completes when its
  * dt clamps to t_end (the dump clamp of hydro_core.f90:691-694) or after
- * max_steps steps (0: no limit).  GM_ERR_BUSY when it does not fit. */
+ * max_steps steps (0: no limit; at most 2^31 - 1, else GM_ERR_ARG).
+ * GM_ERR_BUSY when it does not fit. */
 int gm_batch_add(gm_batch *batch, const gm_params *box, const double *u, double cfl, double t_end,
                  int64_t max_steps, int32_t *box_id);
 /* Release a box and its memory. */
