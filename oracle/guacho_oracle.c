@@ -15,14 +15,21 @@
  *     fixtures under tests/golden/ made from it;
  *   - whole sweeps (hlldfluxes/hllefluxes/hllfluxes/hllcfluxes, calcprim,
  *     boundaryI/II, get_timestep) against the same reference build;
- *   - get_efield / flux_cd_update / step / viscous_copy / boundaryI_ef /
- *     tstep: the reference's flux_cd_module.f90 does not compile without
- *     MPI (serial branch at flux_cd_module.f90:120 is never closed), so
- *     these are pinned by invariants: flux-CD keeps the central-difference
- *     div B of boundaryI's dump formula at round-off, periodic boxes
- *     conserve mass/momentum/energy/B to round-off, 180-degree rotation
- *     symmetry of Orszag-Tang, axis-permutation symmetry, and a step
- *     composed from the reference's own sweeps with restated flux-CD.
+ *   - tstep / step / viscous_copy with flux-CD off: bit for bit against the
+ *     reference's own hydro_solver.f90 (compiled without BFIELD, which in
+ *     that file guards only the flux-CD calls), dt sequence and u after
+ *     every step, ghosts included (tests/test_oracle_tstep.py);
+ *   - get_efield / flux_cd_update / boundaryI_ef: the reference's
+ *     flux_cd_module.f90 does not compile without MPI (serial branch at
+ *     flux_cd_module.f90:120 is never closed), so these are pinned by what
+ *     the mathematics fixes (tests/test_oracle_fluxcd.py): the hydro part
+ *     equals the upwind update bit for bit, E = -v x B on a uniform state,
+ *     the field update converges to the upwind one at first order, the
+ *     flux-CD step is second order on an exact Alfven wave, outflow walls
+ *     keep axis-uniform states uniform, closed-wall E ghosts read as
+ *     outflow ones; plus flux-CD's preservation of the dump formula's
+ *     central-difference div B, periodic conservation and axis-permutation
+ *     symmetry (tests/test_oracle_invariants.py);
  *   - UR* degenerate branch (hlld.f90:167-172) : parity unpinned -- the
  *     reference assigns the L variables there and leaves vstr..bzstr
  *     undefined; this restatement mirrors the L logic for R.
