@@ -49,26 +49,42 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(seconds: float):
-    """The C restatement (oracle/) on the host cores: turbulence 128^3, steps
-    until `seconds` of wall time have passed (at most 200)."""
-    import numpy as np  # noqa: F401
-    from guacho_amd import problems
+def host_physical_cores():
+    """Physical cores of the host (distinct core, socket pairs of lscpu -p)."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu", "-p=Core,Socket"], capture_output=True, text=True, timeout=10).stdout
+        pairs = {ln.strip() for ln in out.splitlines() if ln and not ln.startswith("#")}
+        return len(pairs) or None
+    except (OSError, subprocess.SubprocessError):
+        return None
+
+
+def cpu_baseline(u0, dx, dy, dz, n: int, seconds: float):
+    """The C restatement of the reference's CPU path (oracle/) on the host
+    cores, on the headline workload itself (the seeded turbulence box, FP64
+    HLLD + minmod + flux-CD): one untimed step, then steps until `seconds`
+    of wall time have passed (at most 200).  Threads: the host share this
+    process is given (OMP_NUM_THREADS, which the GPU host sets to its CPU
+    share per GPU), else every physical core."""
     from guacho_amd.host import timestep_clamp
     from oracle.oracle import Config, Oracle
 
-    threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")), 16)
-    n = 128
-    u0, dx, dy, dz = problems.turbulence(n, seed=2024)
+    phys = host_physical_cores()
+    env_threads = os.environ.get("OMP_NUM_THREADS")
+    threads = int(env_threads) if env_threads else (phys or os.cpu_count() or 1)
     O = Oracle(Config(nx=n, ny=n, nz=n, dx=dx, dy=dy, dz=dz), nthreads=threads)
     O.set_u(u0)
     O.boundaryI()
     O.calcprim_u()
+    t = 0.0
+    dt, _ = timestep_clamp(1, 10, 0.4, O.cfl_dtp(), t, 1e30)
+    O.tstep(dt)
+    t += dt
     t0 = time.perf_counter()
     steps = 0
-    t = 0.0
     while steps < 200:
-        dt, _ = timestep_clamp(steps + 1, 10, 0.4, O.cfl_dtp(), t, 1e30)
+        dt, _ = timestep_clamp(steps + 2, 10, 0.4, O.cfl_dtp(), t, 1e30)
         O.tstep(dt)
         t += dt
         steps += 1
@@ -76,9 +92,22 @@ def cpu_baseline(seconds: float):
             break
     el = time.perf_counter() - t0
     O.close()
-    return {"value": n ** 3 * steps / el, "unit": "zone-updates/s", "cores": threads, "kind": "port",
-            "sample": f"C restatement (oracle/guacho_oracle.c, gcc -O2, OpenMP {threads} threads), "
-                      f"turbulence {n}^3 FP64 HLLD minmod flux-CD, {steps} steps in {el:.2f} s"}
+    value = n ** 3 * steps / el
+    out = {"value": value, "unit": "zone-updates/s", "cores": threads, "kind": "port",
+           "host_physical_cores": phys,
+           "threads_from": "OMP_NUM_THREADS (the host's CPU share per GPU)" if env_threads else "lscpu physical cores",
+           "sample": f"C restatement (oracle/guacho_oracle.c, gcc -O3, OpenMP {threads} threads) of the headline "
+                     f"workload, turbulence {n}^3 FP64 HLLD minmod flux-CD: {steps} steps in {el:.2f} s after one "
+                     f"untimed step"}
+    try:
+        with open(os.path.join(ROOT, "profiles", "cpu_calibration.json")) as fh:
+            cal = json.load(fh)
+        r = float(cal["ratio_3d"])
+        out["calibration"] = {"restatement_over_reference_1core_3d": r, "source": cal["source"]}
+        out["reference_equivalent_value"] = value * r
+    except (OSError, KeyError, ValueError):
+        pass
+    return out
 
 
 # Compulsory bytes one launch of each kernel class moves (FP64, flux-CD),
@@ -100,37 +129,59 @@ def algorithmic_bytes(cls: str, n: int) -> float:
         "bfield_cfl": 112.0 * N,
         # read U (8)
         "cfl_dt": 64.0 * N,
+        # the 2.5-D stage kernel (xy and z faces in one pass): read U^n (8);
+        # write U*'s hydro part (5) and E (3)
+        "slab_predictor": 128.0 * N,
+        # read U* (8) and U^n's hydro part (5); write the hydro part (5) and E (3)
+        "slab_corrector": 168.0 * N,
     }.get(cls, 0.0)
 
 
-def measured_traffic(cls: str):
-    """Calibrated HBM bytes per launch of `cls` and its FP64 VALU issue
-    fraction, from the committed rocprofv3 summary of this workload (FETCH_SIZE,
+def measured_profile(cls: str):
+    """Calibrated HBM bytes and FP64 VALU instructions per launch of `cls`
+    from the committed rocprofv3 summary of this workload (FETCH_SIZE,
     WRITE_SIZE, SQ_INSTS_VALU), if one exists."""
     path = os.path.join(ROOT, "profiles", "traffic_fp64_turb256.json")
     try:
         with open(path) as fh:
             d = json.load(fh)
-        return (d["per_launch_bytes"].get(cls), d.get("valu_issue_frac", {}).get(cls),
+        return (d["per_launch_bytes"].get(cls), d.get("valu_instr_per_launch", {}).get(cls),
                 os.path.relpath(path, ROOT))
     except (OSError, KeyError, ValueError):
         return None, None, None
 
 
+N_SIMD = 1024        # 256 CUs x 4 SIMDs
+CLOCK_HZ = 2.4e9     # MI355X peak engine clock
+VALU_CYCLES = 4      # a wave64 FP64 VALU instruction occupies a 16-lane SIMD 4 cycles
+
+
 def dominant_roofline(kt, n, steps):
-    """HBM roofline of the kernel class with the largest total time."""
+    """Roofline of the kernel class with the largest total time.  `achieved`
+    is the path's algorithmic bytes, 128 B per zone-update (read and write
+    the 8 conserved doubles), times the zones one launch advances, over the
+    kernel's average launch time from HIP events on the library's stream.
+    The kernel is bounded by FP64 VALU issue, not HBM: `valu_issue_frac`
+    is its VALU instructions per launch (a property of the kernel and the
+    workload, from the committed SQ counters) x 4 cycles over the live
+    launch time x 1024 SIMDs x 2.4 GHz."""
     cls = max((k for k in kt if kt[k][1] and algorithmic_bytes(k, n) > 0), key=lambda k: kt[k][0])
     ms, launches = kt[cls]
     avg_s = ms / launches / 1e3
-    ach = algorithmic_bytes(cls, n) / avg_s / 1e9
-    traffic, valu, src = measured_traffic(cls)
-    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+    zone_bytes = float(BYTES_PER_ZONE_FP64) * n ** 3
+    ach = zone_bytes / avg_s / 1e9
+    io = algorithmic_bytes(cls, n)
+    traffic, valu_instr, src = measured_profile(cls)
+    valu = round(valu_instr * VALU_CYCLES / (avg_s * N_SIMD * CLOCK_HZ), 3) if valu_instr else None
+    return {"bound": "valu", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": cls,
-            "launches_per_step": launches // steps,
-            "avg_launch_ms": round(ms / launches, 4), "algorithmic_bytes_per_launch": algorithmic_bytes(cls, n),
-            # what actually bounds it: FP64 VALU instructions issued per cycle
-            # against the nominal rate (rocprofv3 SQ_INSTS_VALU)
-            "valu_issue_frac": valu,
+            "frac_basis": f"{BYTES_PER_ZONE_FP64} B per zone-update x {n}^3 zones per launch",
+            "launches_per_step": launches // steps, "avg_launch_ms": round(ms / launches, 4),
+            "algorithmic_bytes_per_launch": zone_bytes,
+            # the kernel's own compulsory I/O (every array it reads or writes, once)
+            "kernel_io_bytes_per_launch": io, "kernel_io_achieved": round(io / avg_s / 1e9, 1),
+            "kernel_io_frac": round(io / avg_s / 1e9 / HBM_PEAK_GBS, 4),
+            "valu_issue_frac": valu, "valu_instr_per_launch": valu_instr,
             "traffic_source": src}
 
 
@@ -202,10 +253,12 @@ def mixed_section(n: int, steps: int, seed: int):
 
 def batch_section(nbox: int, steps: int, seed: int):
     """64 independent MHD blasts of uneven cubic size n ~ U{32..128}, outflow,
-    HLLD, mixed precision, `steps` steps each: all boxes in one batch whose
-    steps are captured in one HIP graph launch, launch shape autotuned;
-    against the same boxes solved one at a time with the plain FP64
-    single-box path."""
+    HLLD, `steps` steps each: all boxes in one batch whose steps are captured
+    in one HIP graph launch, launch shape autotuned, in mixed precision (the
+    configuration) and in FP64; against the same boxes solved one at a time
+    with the single-box path in both precisions.  Every leg's timed region
+    starts with the boxes uploaded and ends with their last step done: no
+    leg times an upload or allocation."""
     import numpy as np
     from guacho_amd import _capi, problems
     from guacho_amd.host import timestep_clamp
@@ -217,45 +270,71 @@ def batch_section(nbox: int, steps: int, seed: int):
         u, dx, dy, dz = problems.blast(int(n), seed=seed * 1000 + b)
         boxes.append((int(n), u, dx))
     zones = float(sum(n ** 3 for n, _, _ in boxes)) * steps
-    mp = _capi.GM_PRECISION_MIXED
 
-    def admit(B):
-        return [B.add(_capi.make_params(n, n, n, h, h, h, bc=(1,) * 6, precision=mp), u, max_steps=steps)
-                for n, u, h in boxes]
+    def batched(prec):
+        def admit(B):
+            return [B.add(_capi.make_params(n, n, n, h, h, h, bc=(1,) * 6, precision=prec), u, max_steps=steps)
+                    for n, u, h in boxes]
+        with _capi.Batch(precision=prec, max_boxes=nbox, steps_per_launch=steps, autotune=True) as B:
+            ids = admit(B)
+            B.advance(1)        # first use: autotune (dt = 0 steps), tables, graph capture, the run
+            B.sync()
+            block, kchunk, tuned_ms = B.tuning()
+            u_first = B.download_u(ids[0])
+            for i in ids:
+                B.remove(i)
+            ids = admit(B)      # same boxes again: same geometry, the captured graph is reused
+            B.sync()
+            t0 = time.perf_counter()
+            B.advance(1)        # all boxes x `steps` steps in one graph launch
+            B.sync()
+            el = time.perf_counter() - t0
+            assert all(B.clock(i) == (B.clock(i)[0], steps, True) for i in ids)
+            same = bool(np.array_equal(B.download_u(ids[0]), u_first))
+        return el, (block, kchunk, tuned_ms), same
 
-    with _capi.Batch(precision=mp, max_boxes=nbox, steps_per_launch=steps, autotune=True) as B:
-        ids = admit(B)
-        B.advance(1)            # first use: autotune (dt = 0 steps), tables, graph capture, the run
-        B.sync()
-        block, kchunk, tuned_ms = B.tuning()
-        u_first = B.download_u(ids[0])
-        for i in ids:
-            B.remove(i)
-        ids = admit(B)          # same boxes again: same geometry, the captured graph is reused
-        B.sync()
-        t0 = time.perf_counter()
-        B.advance(1)            # all 64 boxes x `steps` steps in one graph launch
-        B.sync()
-        el = time.perf_counter() - t0
-        assert all(B.clock(i) == (B.clock(i)[0], steps, True) for i in ids)
-        same = bool(np.array_equal(B.download_u(ids[0]), u_first))
-    log("batch section: one box at a time, plain FP64")
-    t0 = time.perf_counter()
-    for n, u, h in boxes:
-        with _capi.Subdomain(_capi.make_params(n, n, n, h, h, h, bc=(1,) * 6)) as dev:
-            dev.upload_u(u)
-            t = 0.0
-            for it in range(1, steps + 1):
-                dt, _ = timestep_clamp(it, 10, 0.4, dev.cfl_dtp(), t, 1e30)
-                dev.tstep(dt)
-                t += dt
-            dev.sync()
-    seq = time.perf_counter() - t0
+    def sequential(prec):
+        devs = []
+        try:
+            for n, u, h in boxes:
+                d = _capi.Subdomain(_capi.make_params(n, n, n, h, h, h, bc=(1,) * 6, precision=prec))
+                devs.append(d)
+                d.upload_u(u)
+                d.sync()
+            t0 = time.perf_counter()
+            for d in devs:
+                t = 0.0
+                for it in range(1, steps + 1):
+                    dt, _ = timestep_clamp(it, 10, 0.4, d.cfl_dtp(), t, 1e30)
+                    d.tstep(dt)
+                    t += dt
+                d.sync()
+            return time.perf_counter() - t0
+        finally:
+            for d in devs:
+                d.close()
+
+    mp, fp = _capi.GM_PRECISION_MIXED, _capi.GM_PRECISION_FP64
+    log("batch section: batched, mixed")
+    el_m, tune_m, same_m = batched(mp)
+    log("batch section: batched, FP64")
+    el_f, tune_f, same_f = batched(fp)
+    log("batch section: one box at a time, mixed")
+    seq_m = sequential(mp)
+    log("batch section: one box at a time, FP64")
+    seq_f = sequential(fp)
     return {"workload": f"{nbox} MHD blasts n~U{{32..128}}^3, outflow, HLLD, mixed, {steps} steps, one graph launch",
-            "value": round(zones / el, 1), "unit": "zone-updates/s", "seconds": round(el, 4),
-            "zone_updates": zones, "tuned_shape": {"block": block, "kchunk": kchunk, "ms_per_step": round(tuned_ms, 4)},
-            "sequential_fp64_value": round(zones / seq, 1), "sequential_fp64_seconds": round(seq, 3),
-            "speedup_vs_sequential_fp64": round(seq / el, 3), "repeat_bitwise_equal": same}
+            "value": round(zones / el_m, 1), "unit": "zone-updates/s", "seconds": round(el_m, 4),
+            "zone_updates": zones,
+            "tuned_shape": {"block": tune_m[0], "kchunk": tune_m[1], "ms_per_step": round(tune_m[2], 4)},
+            "batched_fp64_value": round(zones / el_f, 1), "batched_fp64_seconds": round(el_f, 4),
+            "sequential_mixed_value": round(zones / seq_m, 1), "sequential_mixed_seconds": round(seq_m, 4),
+            "sequential_fp64_value": round(zones / seq_f, 1), "sequential_fp64_seconds": round(seq_f, 4),
+            # batching's own gain, precision held fixed
+            "batching_speedup_mixed": round(seq_m / el_m, 3), "batching_speedup_fp64": round(seq_f / el_f, 3),
+            "speedup_vs_sequential_fp64": round(seq_f / el_m, 3),
+            "timed_region": "boxes resident in HBM; last step done (sync); no uploads or allocations in any leg",
+            "repeat_bitwise_equal": same_m and same_f}
 
 
 def extensions_section(u0, dx, dy, dz, n: int, steps: int):
@@ -423,7 +502,7 @@ def main():
         out["mixed"] = mixed_section(args.mixed_n, args.mixed_steps, args.seed)
     if not args.no_cpu_baseline:
         log("CPU baseline (C restatement on the host cores)")
-        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(u0, dx, dy, dz, n, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 
