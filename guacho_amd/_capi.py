@@ -20,7 +20,7 @@ GM_COOL_NONE, GM_COOL_DMC, GM_COOL_CHI = 0, 3, 4
 GM_TC_OFF, GM_TC_ISOTROPIC, GM_TC_ANISOTROPIC = 0, 1, 2
 GM_COOL_H = 1
 GM_EOS_ADIABATIC, GM_EOS_SINGLE_SPECIE, GM_EOS_H_RATE = 1, 2, 3
-GM_NKCLASS = 13
+GM_NKCLASS = 15
 
 # exported entry points, in the order include/guacho_amd.h declares them
 EXPORTS = (

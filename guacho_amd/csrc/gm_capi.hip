@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -23,9 +24,10 @@ namespace {
 
 const char* kClassNames[GM_NKCLASS] = {"xy_predictor", "xy_corrector", "z_predictor", "z_corrector",
                                        "bfield", "cfl_dt", "ghost", "layout", "sources", "cooling",
-                                       "conduction", "viscous", "bfield_cfl"};
+                                       "conduction", "viscous", "bfield_cfl", "slab_predictor",
+                                       "slab_corrector"};
 enum { KC_XY1, KC_XY2, KC_Z1, KC_Z2, KC_BFIELD, KC_CFL, KC_GHOST, KC_LAYOUT, KC_SOURCES, KC_COOLING, KC_CONDUCTION, KC_VISCOUS,
-       KC_BFIELD_CFL };
+       KC_BFIELD_CFL, KC_SLAB1, KC_SLAB2 };
 
 thread_local std::string g_create_error;
 
@@ -87,6 +89,9 @@ struct gm_ctx {
   gm::Conduction cond{};                // kind 0: no thermal conduction
   int tc_substeps = 0;                  // sub-steps of the last step's conduction
   gm::Shape shape = {256, 16};  // tile width, z-pencil length
+  // planes per slab of the 2.5-D stage kernel; 0: the two-kernel stage
+  // (xy plane tiles, then z pencils through W)
+  int slab_kchunk = 0;
   int timing = 0;
   struct Rec { int cls; hipEvent_t a, b; };
   std::vector<Rec> recs;
@@ -444,9 +449,15 @@ void tstep_t(gm_ctx* c, double dt) {
   const int solver = p.riemann_solver, fcd = p.enable_flux_cd ? 1 : 0;
   const double dtm = dt / 2.0;  // hydro_solver.f90:152
   // predictor: first-order fluxes of U^n, U* = U^n - dt/2 div F (+ dt/2 s)
+  // the 2.5-D stage kernel unless the stage kernels add source terms
+  const int kslab = b.srcin ? 0 : c->slab_kchunk;
   stage_user_s_t(c, b, dtm, 1);
-  timed(c, KC_XY1, [&] { gm::launch_xy(s, b, dtm, 1, solver, fcd, err); });
-  timed(c, KC_Z1, [&] { gm::launch_z(s, b, dtm, 1, solver, fcd, err, sh); });
+  if (kslab) {
+    timed(c, KC_SLAB1, [&] { gm::launch_slab(s, b, dtm, 1, solver, fcd, err, kslab); });
+  } else {
+    timed(c, KC_XY1, [&] { gm::launch_xy(s, b, dtm, 1, solver, fcd, err); });
+    timed(c, KC_Z1, [&] { gm::launch_z(s, b, dtm, 1, solver, fcd, err, sh); });
+  }
   if (fcd) timed(c, KC_BFIELD, [&] { gm::launch_bfield(s, b, dtm, 1, sh); });
   stage_sources_t(c, b, dtm, 1);
   // corrector: limited fluxes of U*, U^{n+1} = U^n - dt div F (+ dt s), in
@@ -454,8 +465,12 @@ void tstep_t(gm_ctx* c, double dt) {
   // boundaryII on U* (src/hydro_solver.f90:169) when ghosts live in memory
   if (b.g.gmem) fill_ghost_t<S>(c, b.Us, 2);
   stage_user_s_t(c, b, dt, 2);
-  timed(c, KC_XY2, [&] { gm::launch_xy(s, b, dt, 2, solver, fcd, err); });
-  timed(c, KC_Z2, [&] { gm::launch_z(s, b, dt, 2, solver, fcd, err, sh); });
+  if (kslab) {
+    timed(c, KC_SLAB2, [&] { gm::launch_slab(s, b, dt, 2, solver, fcd, err, kslab); });
+  } else {
+    timed(c, KC_XY2, [&] { gm::launch_xy(s, b, dt, 2, solver, fcd, err); });
+    timed(c, KC_Z2, [&] { gm::launch_z(s, b, dt, 2, solver, fcd, err, sh); });
+  }
   // when nothing after the field update touches U's interior (no sources,
   // viscosity, cooling, conduction, or boundary hook that may write inside
   // the box), the next get_timestep's minimum is reduced in the same pass
@@ -525,6 +540,13 @@ int gm_create(const gm_params* params, gm_ctx** out) {
   c->ph.limiter = params->slope_limiter;
   c->ph.eos = GM_EOS_ADIABATIC;
   c->ph.tempsc = 1.0;
+  // slab length of the 2.5-D stage kernel: from the grid, or
+  // GUACHO_AMD_SLAB_KCHUNK (planes per slab; 0 selects the two-kernel stage)
+  c->slab_kchunk = gm::slab_kchunk(g);
+  if (const char* e = std::getenv("GUACHO_AMD_SLAB_KCHUNK")) {
+    const long v = std::strtol(e, nullptr, 10);
+    if (v >= 0 && v <= (1 << 20)) c->slab_kchunk = (int)v;
+  }
 
   auto bail = [&](int code) {
     g_create_error = c->msg;

@@ -156,36 +156,39 @@ __device__ __forceinline__ void load_cons(const Geom& g, const S* __restrict__ U
 // Plane tile of the xy kernel: TX x TY interior cells of one z-plane, one
 // thread per cell.  Wave w holds rows 2w (lanes 0-31) and 2w+1 (lanes
 // 32-63), so a row is half a wavefront and x neighbours are adjacent lanes.
-constexpr int TX = 32, TY = 16, XY_THREADS = TX * TY, XY_WAVES = XY_THREADS / 64;
+// The slab kernel (2.5-D) uses the same layout with TYS = 8 rows.
+constexpr int TX = 32, TY = 16, XY_THREADS = TX * TY;
 static_assert(TX == 32 && XY_THREADS == 512, "two tile rows per 64-lane wavefront");
+constexpr int TYS = 8, SL_THREADS = TX * TYS;
 
-// LDS of the xy kernel for reconstruction order ORDER (halo H = ORDER).
-// Primitives [8][RY][RX] of the tile and its x and y halos (corners are
-// not loaded); after the face solves that area holds the y-face fluxes
-// [8][TY][TX].  Then the fluxes of the tile's closing faces: y-face TY of
-// each column [8][TX] and x-face TX of each row [8][TY].  Second order adds
-// reconstructed face states: the upper x state of the halo cell (-1, r),
-// of the tile's last cell (TX-1, r) and the lower x state of the halo cell
-// (TX, r) [8][TY] each; the lower y state of the halo cell (c, TY) [8][TX];
-// and the upper y states of rows -1, 1, 3, .., TY-1 [8][TY/2+1][TX] (the
-// rows an even row, held in the other wavefront, reads below itself).
-template <int ORDER, int NQ = 8>
+// LDS of a plane tile of TYT rows for reconstruction order ORDER (halo
+// H = ORDER).  Primitives [8][RY][RX] of the tile and its x and y halos
+// (corners are not loaded); after the face solves that area holds the
+// y-face fluxes [8][TYT][TX].  Then the fluxes of the tile's closing faces:
+// y-face TYT of each column [8][TX] and x-face TX of each row [8][TYT].
+// Second order adds reconstructed face states: the upper x state of the
+// halo cell (-1, r), of the tile's last cell (TX-1, r) and the lower x
+// state of the halo cell (TX, r) [8][TYT] each; the lower y state of the
+// halo cell (c, TYT) [8][TX]; and the upper y states of rows -1, 1, 3, ..,
+// TYT-1 [8][TYT/2+1][TX] (the rows an even row, held in the other
+// wavefront, reads below itself).
+template <int ORDER, int NQ = 8, int TYT = TY>
 struct XYLds {
   static constexpr int H = ORDER;
-  static constexpr int RX = TX + 2 * H, RY = TY + 2 * H;
+  static constexpr int RX = TX + 2 * H, RY = TYT + 2 * H;
   static constexpr int PS = RX * RY;  // component stride of the primitives
   static constexpr int PRIM = NQ * PS;
-  static constexpr int NLOAD = 2 * H * TX + TY * RX;  // primitives loaded
+  static constexpr int NLOAD = 2 * H * TX + TYT * RX;  // primitives loaded
   static constexpr int TOP = PRIM;
   static constexpr int XEND = TOP + NQ * TX;
-  static constexpr int XL = XEND + NQ * TY;
-  static constexpr int XU = XL + NQ * TY;
-  static constexpr int XR = XU + NQ * TY;
-  static constexpr int YD = XR + NQ * TY;
+  static constexpr int XL = XEND + NQ * TYT;
+  static constexpr int XU = XL + NQ * TYT;
+  static constexpr int XR = XU + NQ * TYT;
+  static constexpr int YD = XR + NQ * TYT;
   static constexpr int YU = YD + NQ * TX;
-  static constexpr int NYU = TY / 2 + 1;
+  static constexpr int NYU = TYT / 2 + 1;
   static constexpr int SIZE = ORDER == 1 ? XL : YU + NQ * NYU * TX;
-  static_assert(NQ * TY * TX <= PRIM, "y-face fluxes must fit the primitive area");
+  static_assert(NQ * TYT * TX <= PRIM, "y-face fluxes must fit the primitive area");
 };
 
 // Reconstructed states of one cell along one axis, from the primitives in
@@ -231,18 +234,203 @@ __device__ __forceinline__ void perm_y_if(bool ydir, T* v) {
   v[6] = f;
 }
 
-// xy kernel.  Thread (c, r) -- lane c + 32 (r & 1) of wave r/2 -- solves the
-// low faces of cell (c, r): x-face c of row r (between cells c-1 and c)
-// and y-face r of column c.  Its high x face is lane c+1's, its high y face
-// row r+1's; the closing faces of a full tile (x-face TX of each row, y-face
-// TY of each column: TY + TX = 48 faces) are solved together by wave 0.
-// In a partial tile the closing faces are the low faces of the halo cells
-// just past it, solved by the threads of those cells.
+// Faces of one plane tile, from the primitives in LDS (stored, then a
+// barrier) to the four fluxes of the thread's own cell: fx (low x face),
+// fxr (high x face), fy (low y face), fyu (high y face).  Thread (c, r) --
+// lane c + 32 (r & 1) of wave r/2 -- solves the low faces of cell (c, r):
+// x-face c of row r (between cells c-1 and c) and y-face r of column c.
+// Its high x face is lane c+1's, its high y face row r+1's; the closing
+// faces of a full tile (x-face TX of each row, y-face TYT of each column)
+// are solved together by wave 0.  In a partial tile the closing faces are
+// the low faces of the halo cells just past it, solved by the threads of
+// those cells.
 //
 // Second order: every thread forms the x and y states of its own cell
-// (cell_states), waves 6 and 7 those of the 2 (TX + TY) halo cells next to
-// the tile.  The x faces are solved before the y states are formed, which
-// keeps the live register set of one axis at a time.
+// (cell_states), the last two waves those of the 2 (TX + TYT) halo cells
+// next to the tile.  The x faces are solved before the y states are formed,
+// which keeps the live register set of one axis at a time.  `mid()` runs
+// once every face is solved, before the flux exchange (callers fetch the
+// cell's state there).  Every thread must call this (it holds barriers);
+// the outputs are meaningful for the tile's own cells only.  Returns false
+// when a solver fell through.  `tid` is threadIdx.x.
+template <int SOLVER, int ORDER, int NQ, int TYT, typename S, typename Mid>
+__device__ __forceinline__ bool xy_faces(const Box<S>& b, S* __restrict__ lds, int wx, int wy, int tid, S fx[NQ],
+                                         S fxr[NQ], S fy[NQ], S fyu[NQ], Mid&& mid) {
+  using L = XYLds<ORDER, NQ, TYT>;
+  constexpr int H = L::H, RX = L::RX, PS = L::PS;
+  constexpr int WAVES = TX * TYT / 64;
+  static_assert(TX + TYT <= 64, "closing faces in one wavefront");
+  static_assert(2 * (TX + TYT) <= 128, "halo states in the last two wavefronts");
+  const int w = tid >> 6, l = tid & 63;
+  const int c = l & (TX - 1), r = 2 * w + (l >> 5);
+  S* __restrict__ lp = lds;
+  S* __restrict__ ltop = lds + L::TOP;
+  S* __restrict__ lend = lds + L::XEND;
+  const int at = (r + H) * RX + c + H;  // own cell in the primitive area
+  const bool vx = r < wy && c <= wx;    // x-face c of row r is the tile's
+  const bool vy = c < wx && r <= wy;    // y-face r of column c is the tile's
+  const bool cx = w == 0 && l < TYT;                   // closing x-face of row l
+  const bool cy = w == 0 && l >= TYT && l < TYT + TX;   // closing y-face of column l - TYT
+  const int crow = cx ? l : 0, ccol = cy ? l - TYT : 0;
+  bool ok = true;
+#pragma unroll
+  for (int q = 0; q < NQ; q++) fx[q] = fy[q] = S(0);
+
+  if constexpr (ORDER == 1) {
+    S pl[NQ], pr[NQ];
+    if (vx) {
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        pl[q] = lp[q * PS + at - 1];
+        pr[q] = lp[q * PS + at];
+      }
+      ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, fx);
+    }
+    if (vy) {
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        pl[q] = lp[q * PS + at - RX];
+        pr[q] = lp[q * PS + at];
+      }
+      axis_perm<1>(pl);
+      axis_perm<1>(pr);
+      ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, fy);
+      axis_perm<1>(fy);
+    }
+    if ((cx && wx == TX && crow < wy) || (cy && wy == TYT && ccol < wx)) {
+      const int hi = cy ? (TYT + H) * RX + ccol + H : (crow + H) * RX + TX + H;
+      const int st = cy ? RX : 1;
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        pl[q] = lp[q * PS + hi - st];
+        pr[q] = lp[q * PS + hi];
+      }
+      S ff[NQ];
+      perm_y_if(cy, pl);
+      perm_y_if(cy, pr);
+      ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, ff);
+      perm_y_if(cy, ff);
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        if (cy) ltop[q * TX + ccol] = ff[q];
+        else lend[q * TYT + crow] = ff[q];
+      }
+    }
+  } else {
+    S* __restrict__ xl = lds + L::XL;
+    S* __restrict__ xu = lds + L::XU;
+    S* __restrict__ xr = lds + L::XR;
+    S* __restrict__ yd = lds + L::YD;
+    S* __restrict__ yu = lds + L::YU;
+    const int lim = b.ph.limiter;
+    // halo cells, into LDS, by the last two waves (h = 0..95): the x
+    // states of (-1, r) and (TX, r), the y states of (c, -1) and (c, TYT)
+    if (w >= WAVES - 2) __builtin_amdgcn_s_setprio(2);
+    if (w >= WAVES - 2) {
+      const int h = (w - (WAVES - 2)) * 64 + l;
+      if (h < 2 * TYT + 2 * TX) {
+        int hat, hst, hdst, hqs;
+        bool hup;
+        if (h < TYT) {
+          hat = (h + H) * RX + H - 1, hst = 1, hup = true, hdst = L::XL + h, hqs = TYT;
+        } else if (h < 2 * TYT) {
+          hat = (h - TYT + H) * RX + H + TX, hst = 1, hup = false, hdst = L::XR + h - TYT, hqs = TYT;
+        } else if (h < 2 * TYT + TX) {
+          hat = (H - 1) * RX + h - 2 * TYT + H, hst = RX, hup = true, hdst = L::YU + h - 2 * TYT, hqs = L::NYU * TX;
+        } else {
+          hat = (H + TYT) * RX + h - 2 * TYT - TX + H, hst = RX, hup = false, hdst = L::YD + h - 2 * TYT - TX, hqs = TX;
+        }
+        S hs[NQ];
+        cell_state<NQ>(lim, lp, PS, hat, hst, hup, hs);
+#pragma unroll
+        for (int q = 0; q < NQ; q++) lds[hdst + q * hqs] = hs[q];
+      }
+    }
+    if (w >= WAVES - 2) __builtin_amdgcn_s_setprio(0);
+    // x: the upper state passes one lane up (lane c-1 holds cell c-1)
+    S pl[NQ], pr[NQ];
+    {
+      S pxu[NQ];
+      cell_states<NQ>(lim, lp, PS, at, 1, pxu, pr);
+      if (c == TX - 1) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++) xu[q * TYT + r] = pxu[q];
+      }
+#pragma unroll
+      for (int q = 0; q < NQ; q++) pl[q] = __shfl_up(pxu[q], 1, 64);
+    }
+    __syncthreads();  // halo states in LDS
+    if (c == 0) {
+#pragma unroll
+      for (int q = 0; q < NQ; q++) pl[q] = xl[q * TYT + r];
+    }
+    if (vx) ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, fx);
+    // y: odd rows hand their upper state to the even row above them (the
+    // next wavefront) through LDS, even rows theirs to the odd row above
+    // (lanes 32 up in the same wavefront)
+    {
+      S pyu[NQ];
+      cell_states<NQ>(lim, lp, PS, at, RX, pyu, pr);
+      if (r & 1) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++) yu[(q * L::NYU + (r + 1) / 2) * TX + c] = pyu[q];
+      }
+#pragma unroll
+      for (int q = 0; q < NQ; q++) pl[q] = __shfl_up(pyu[q], 32, 64);
+    }
+    __syncthreads();  // every read of the primitives is done; y states in LDS
+    if (w == 0) __builtin_amdgcn_s_setprio(2);
+    if (!(r & 1)) {
+#pragma unroll
+      for (int q = 0; q < NQ; q++) pl[q] = yu[(q * L::NYU + r / 2) * TX + c];
+    }
+    if (vy) {
+      axis_perm<1>(pl);
+      axis_perm<1>(pr);
+      ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, fy);
+      axis_perm<1>(fy);
+    }
+    if ((cx && wx == TX && crow < wy) || (cy && wy == TYT && ccol < wx)) {
+      S ff[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        pl[q] = cy ? yu[(q * L::NYU + TYT / 2) * TX + ccol] : xu[q * TYT + crow];
+        pr[q] = cy ? yd[q * TX + ccol] : xr[q * TYT + crow];
+      }
+      perm_y_if(cy, pl);
+      perm_y_if(cy, pr);
+      ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, ff);
+      perm_y_if(cy, ff);
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        if (cy) ltop[q * TX + ccol] = ff[q];
+        else lend[q * TYT + crow] = ff[q];
+      }
+    }
+  }
+  if (w == 0) __builtin_amdgcn_s_setprio(0);
+  mid();
+  if constexpr (ORDER == 1) __syncthreads();  // every read of the primitives is done
+  S* __restrict__ lfy = lds;
+  if (vy) {
+#pragma unroll
+    for (int q = 0; q < NQ; q++) lfy[(q * TYT + r) * TX + c] = fy[q];
+  }
+  __syncthreads();
+
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const S v = __shfl_down(fx[q], 1, 64);
+    fxr[q] = c == TX - 1 ? lend[q * TYT + r] : v;
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; q++) fyu[q] = r == TYT - 1 ? ltop[q * TX + c] : lfy[(q * TYT + r + 1) * TX + c];
+  return ok;
+}
+
+// xy kernel: the primitives of a 32 x 16 plane tile and its halo into LDS,
+// the tile's x and y faces (xy_faces), then the x/y terms of the update,
+// E3 and the x/y parts of E1, E2 into W for the z kernel.
 template <int SOLVER, int ORDER, int FLUXCD, typename S, int NP = 0>
 __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt, int* __restrict__ err,
                                         S* __restrict__ lds) {
@@ -261,8 +449,6 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
   const int c = l & (TX - 1), r = 2 * w + (l >> 5);
   const S* __restrict__ Usrc = ORDER == 1 ? b.U : b.Us;
   S* __restrict__ lp = lds;
-  S* __restrict__ ltop = lds + L::TOP;
-  S* __restrict__ lend = lds + L::XEND;
 
   // primitives of the tile, its x halo on the tile's rows and its y halo
   // on the tile's columns (H rows below, TY rows of RX, H rows above);
@@ -312,152 +498,8 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
   }
   __syncthreads();
 
-  const int at = (r + H) * RX + c + H;  // own cell in the primitive area
-  const bool vx = r < wy && c <= wx;    // x-face c of row r is the tile's
-  const bool vy = c < wx && r <= wy;    // y-face r of column c is the tile's
-  const bool cx = w == 0 && l < TY;                   // closing x-face of row l
-  const bool cy = w == 0 && l >= TY && l < TY + TX;   // closing y-face of column l - TY
-  const int crow = cx ? l : 0, ccol = cy ? l - TY : 0;
-  bool ok = true;
-  S fx[NQ], fy[NQ];
-#pragma unroll
-  for (int q = 0; q < NQ; q++) fx[q] = fy[q] = S(0);
-
-  if constexpr (ORDER == 1) {
-    S pl[NQ], pr[NQ];
-    if (vx) {
-#pragma unroll
-      for (int q = 0; q < NQ; q++) {
-        pl[q] = lp[q * PS + at - 1];
-        pr[q] = lp[q * PS + at];
-      }
-      ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, fx);
-    }
-    if (vy) {
-#pragma unroll
-      for (int q = 0; q < NQ; q++) {
-        pl[q] = lp[q * PS + at - RX];
-        pr[q] = lp[q * PS + at];
-      }
-      axis_perm<1>(pl);
-      axis_perm<1>(pr);
-      ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, fy);
-      axis_perm<1>(fy);
-    }
-    if ((cx && wx == TX && crow < wy) || (cy && wy == TY && ccol < wx)) {
-      const int hi = cy ? (TY + H) * RX + ccol + H : (crow + H) * RX + TX + H;
-      const int st = cy ? RX : 1;
-#pragma unroll
-      for (int q = 0; q < NQ; q++) {
-        pl[q] = lp[q * PS + hi - st];
-        pr[q] = lp[q * PS + hi];
-      }
-      S ff[NQ];
-      perm_y_if(cy, pl);
-      perm_y_if(cy, pr);
-      ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, ff);
-      perm_y_if(cy, ff);
-#pragma unroll
-      for (int q = 0; q < NQ; q++) {
-        if (cy) ltop[q * TX + ccol] = ff[q];
-        else lend[q * TY + crow] = ff[q];
-      }
-    }
-  } else {
-    S* __restrict__ xl = lds + L::XL;
-    S* __restrict__ xu = lds + L::XU;
-    S* __restrict__ xr = lds + L::XR;
-    S* __restrict__ yd = lds + L::YD;
-    S* __restrict__ yu = lds + L::YU;
-    const int lim = b.ph.limiter;
-    // halo cells, into LDS, by the last two waves (h = 0..95): the x
-    // states of (-1, r) and (TX, r), the y states of (c, -1) and (c, TY)
-    if (w >= XY_WAVES - 2) __builtin_amdgcn_s_setprio(2);
-    if (w >= XY_WAVES - 2) {
-      const int h = (w - (XY_WAVES - 2)) * 64 + l;
-      if (h < 2 * TY + 2 * TX) {
-        int hat, hst, hdst, hqs;
-        bool hup;
-        if (h < TY) {
-          hat = (h + H) * RX + H - 1, hst = 1, hup = true, hdst = L::XL + h, hqs = TY;
-        } else if (h < 2 * TY) {
-          hat = (h - TY + H) * RX + H + TX, hst = 1, hup = false, hdst = L::XR + h - TY, hqs = TY;
-        } else if (h < 2 * TY + TX) {
-          hat = (H - 1) * RX + h - 2 * TY + H, hst = RX, hup = true, hdst = L::YU + h - 2 * TY, hqs = L::NYU * TX;
-        } else {
-          hat = (H + TY) * RX + h - 2 * TY - TX + H, hst = RX, hup = false, hdst = L::YD + h - 2 * TY - TX, hqs = TX;
-        }
-        S hs[NQ];
-        cell_state<NQ>(lim, lp, PS, hat, hst, hup, hs);
-#pragma unroll
-        for (int q = 0; q < NQ; q++) lds[hdst + q * hqs] = hs[q];
-      }
-    }
-    if (w >= XY_WAVES - 2) __builtin_amdgcn_s_setprio(0);
-    // x: the upper state passes one lane up (lane c-1 holds cell c-1)
-    S pl[NQ], pr[NQ];
-    {
-      S pxu[NQ];
-      cell_states<NQ>(lim, lp, PS, at, 1, pxu, pr);
-      if (c == TX - 1) {
-#pragma unroll
-        for (int q = 0; q < NQ; q++) xu[q * TY + r] = pxu[q];
-      }
-#pragma unroll
-      for (int q = 0; q < NQ; q++) pl[q] = __shfl_up(pxu[q], 1, 64);
-    }
-    __syncthreads();  // halo states in LDS
-    if (c == 0) {
-#pragma unroll
-      for (int q = 0; q < NQ; q++) pl[q] = xl[q * TY + r];
-    }
-    if (vx) ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, fx);
-    // y: odd rows hand their upper state to the even row above them (the
-    // next wavefront) through LDS, even rows theirs to the odd row above
-    // (lanes 32 up in the same wavefront)
-    {
-      S pyu[NQ];
-      cell_states<NQ>(lim, lp, PS, at, RX, pyu, pr);
-      if (r & 1) {
-#pragma unroll
-        for (int q = 0; q < NQ; q++) yu[(q * L::NYU + (r + 1) / 2) * TX + c] = pyu[q];
-      }
-#pragma unroll
-      for (int q = 0; q < NQ; q++) pl[q] = __shfl_up(pyu[q], 32, 64);
-    }
-    __syncthreads();  // every read of the primitives is done; y states in LDS
-    if (w == 0) __builtin_amdgcn_s_setprio(2);
-    if (!(r & 1)) {
-#pragma unroll
-      for (int q = 0; q < NQ; q++) pl[q] = yu[(q * L::NYU + r / 2) * TX + c];
-    }
-    if (vy) {
-      axis_perm<1>(pl);
-      axis_perm<1>(pr);
-      ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, fy);
-      axis_perm<1>(fy);
-    }
-    if ((cx && wx == TX && crow < wy) || (cy && wy == TY && ccol < wx)) {
-      S ff[NQ];
-#pragma unroll
-      for (int q = 0; q < NQ; q++) {
-        pl[q] = cy ? yu[(q * L::NYU + TY / 2) * TX + ccol] : xu[q * TY + crow];
-        pr[q] = cy ? yd[q * TX + ccol] : xr[q * TY + crow];
-      }
-      perm_y_if(cy, pl);
-      perm_y_if(cy, pr);
-      ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, ff);
-      perm_y_if(cy, ff);
-#pragma unroll
-      for (int q = 0; q < NQ; q++) {
-        if (cy) ltop[q * TX + ccol] = ff[q];
-        else lend[q * TY + crow] = ff[q];
-      }
-    }
-  }
-  if (w == 0) __builtin_amdgcn_s_setprio(0);
-  if (!ok) atomicOr(err, 1);
-  // the cell's state for the update, fetched ahead of the flux exchange
+  S fx[NQ], fy[NQ], fxr[NQ], fyu[NQ];
+  // the cell's state for the update, fetched once the faces are solved
   const bool own = c < wx && r < wy;
   const long long N = g.ncell;
   const long long cc = cidx(g, i0 + c, j0 + r, k);
@@ -466,29 +508,16 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
   // and the z kernel adds it to u in FP64
   constexpr bool INC = sizeof(S) == 4;
   double u0[NH], u0p[NP > 0 ? NP : 1];
-  if (own && !INC) {
+  const bool ok = xy_faces<SOLVER, ORDER, NQ, TY>(b, lds, wx, wy, tid, fx, fxr, fy, fyu, [&] {
+    if (own && !INC) {
 #pragma unroll
-    for (int q = 0; q < NH; q++) u0[q] = (double)b.U[q * N + cc];
+      for (int q = 0; q < NH; q++) u0[q] = (double)b.U[q * N + cc];
 #pragma unroll
-    for (int a = 0; a < NP; a++) u0p[a] = (double)b.U[(8 + a) * N + cc];
-  }
-  if constexpr (ORDER == 1) __syncthreads();  // every read of the primitives is done
-  S* __restrict__ lfy = lds;
-  if (vy) {
-#pragma unroll
-    for (int q = 0; q < NQ; q++) lfy[(q * TY + r) * TX + c] = fy[q];
-  }
-  __syncthreads();
-
-  S fxr[NQ], fyu[NQ];
-#pragma unroll
-  for (int q = 0; q < NQ; q++) {
-    const S v = __shfl_down(fx[q], 1, 64);
-    fxr[q] = c == TX - 1 ? lend[q * TY + r] : v;
-  }
+      for (int a = 0; a < NP; a++) u0p[a] = (double)b.U[(8 + a) * N + cc];
+    }
+  });
+  if (!ok) atomicOr(err, 1);
   if (!own) return;
-#pragma unroll
-  for (int q = 0; q < NQ; q++) fyu[q] = r == TY - 1 ? ltop[q * TX + c] : lfy[(q * TY + r + 1) * TX + c];
   const double dtdx = dt / b.dx, dtdy = dt / b.dy;
   if constexpr (INC) {
     float* __restrict__ Wf = reinterpret_cast<float*>(b.W);
@@ -576,6 +605,8 @@ __device__ __forceinline__ void z_body(const Box<S>& b, unsigned lb, double dt, 
   S pm[NQ], pc[NQ], pu[NQ];  // P(kf), P(kf+1), upper-face state of cell kf
   if constexpr (ORDER == 2) {
     S p0[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) p0[q] = S(0);
     fetch(k0 - 2);
     take(p0);
     fetch(k0 - 1);
@@ -676,6 +707,288 @@ __device__ __forceinline__ void z_body(const Box<S>& b, unsigned lb, double dt, 
 #pragma unroll
     for (int q = 0; q < NQ; q++) fl[q] = ff[q];
   }
+}
+
+// ---------------------------------------------------------------------------
+// Slab kernel (2.5-D blocking): a whole stage of tstep for a 32 x 8 column
+// tile over a slab of `kchunk` planes, marching up z.  Per plane m: the
+// tile's primitives into LDS -- the own cells' from the column window, the
+// halo's from HBM -- then the z face above each own cell (z_body's march,
+// one face per plane), the tile's x and y faces (xy_faces), and the cell's
+// whole update: the x, y and z terms of step in the reference's order, E3
+// and E1, E2 (get_efield's four-term sums), stored once.  The x/y partial
+// sums that the xy kernel hands the z kernel through W stay in registers
+// here, rounded exactly as W holds them (FP64; FP32 increments in the mixed
+// form), so the two paths give the same bits.  A stage then reads U (and
+// U^n's own cells for the corrector) and writes the updated hydro part and
+// E: no W round trip and no second pass over U.
+//
+// The column window: the primitives of the two cells above the current
+// plane sit in two LDS slots per thread (ping-pong by plane parity), the
+// upper-face state of the current cell and the z flux below it in
+// registers; the next cell's state is fetched at the top of each plane
+// with the halo.
+// ---------------------------------------------------------------------------
+template <int ORDER, int NQ, typename S>
+struct SlabLds {
+  using X = XYLds<ORDER, NQ, TYS>;
+  static constexpr int WIN = X::SIZE;             // [2][NQ][SL_THREADS] window slots
+  static constexpr int SIZE = WIN + 2 * NQ * SL_THREADS;
+};
+
+template <int SOLVER, int ORDER, int FLUXCD, typename S, int NP = 0>
+__device__ __forceinline__ void slab_body(const Box<S>& b, unsigned lb, double dt, int kchunk,
+                                          int* __restrict__ err, S* __restrict__ lds) {
+  constexpr int NQ = 8 + NP;  // the MHD variables and the passive scalars
+  constexpr int TYT = TYS;
+  using L = XYLds<ORDER, NQ, TYT>;
+  constexpr int H = L::H, RX = L::RX, PS = L::PS;
+  constexpr int NH = FLUXCD ? 5 : 8;
+  constexpr bool INC = sizeof(S) == 4;  // mixed: FP32 x/y flux increments
+  static_assert(L::NLOAD <= 2 * SL_THREADS, "two halo loads per thread");
+  const Geom& g = b.g;
+  const int ntx = (g.nx + TX - 1) / TX, nty = (g.ny + TYT - 1) / TYT;
+  const unsigned per_plane = (unsigned)(ntx * nty);
+  const int kc = (int)(lb / per_plane);
+  const int rem = (int)(lb - (unsigned)kc * per_plane);
+  const int i0 = (rem % ntx) * TX + 1, j0 = (rem / ntx) * TYT + 1;
+  const int wx = min(TX, g.nx - i0 + 1), wy = min(TYT, g.ny - j0 + 1);
+  const int k0 = kc * kchunk + 1;
+  if (k0 > g.nz) return;  // (whole workgroup: the grid has whole slabs only)
+  const int k1 = min(k0 + kchunk - 1, g.nz);
+  const long long N = g.ncell;
+  const long long sz = (long long)g.NX * g.NY;
+  const double dtdx = dt / b.dx, dtdy = dt / b.dy, dtdz = dt / b.dz;
+  const S* __restrict__ Usrc = ORDER == 1 ? b.U : b.Us;
+  S* __restrict__ Uout = ORDER == 1 ? b.Us : b.U;
+  const int lim = b.ph.limiter;
+  S* __restrict__ lp = lds;
+  S* __restrict__ win = lds + SlabLds<ORDER, NQ, S>::WIN;  // win[(slot * NQ + q) * SL_THREADS + tid]
+  bool ok = true;
+  // lane (c, r) of the tile: c = tid & 31, r = 2 (tid >> 6) + ((tid & 63) >> 5);
+  // cell k of its column at col + (k+1)*sz
+  auto own_of = [&](int t) {
+    const int c = t & (TX - 1), r = 2 * (t >> 6) + ((t & 63) >> 5);
+    return c < wx && r < wy;
+  };
+  auto col_of = [&](int t, bool own) {
+    const int c = t & (TX - 1), r = 2 * (t >> 6) + ((t & 63) >> 5);
+    return cidx(g, own ? i0 + c : i0, own ? j0 + r : j0, -1);
+  };
+
+  // primitives of cell kk of the own column (ghosts by boundary mapping)
+  auto fetch = [&](long long col, int kk, S ur[NQ], bool& flip) {
+    bool f = false;
+    const int sk = ghost_src(g, kk, g.nz, g.bc[4], g.bc[5], f);
+    const long long cc = col + (long long)(sk + 1) * sz;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) ur[q] = Usrc[q * N + cc];
+    flip = f;
+  };
+  auto take = [&](const S ur[NQ], bool flip, S p[NQ]) {
+    double u[NQ], pd[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) u[q] = (double)ur[q];
+    if (flip) u[3] = -u[3];
+    u2prim<NQ>(b.ph, u, pd);
+#pragma unroll
+    for (int q = 0; q < NQ; q++) p[q] = (S)pd[q];
+  };
+  auto wput = [&](int tid, int slot, const S p[NQ]) {
+#pragma unroll
+    for (int q = 0; q < NQ; q++) win[(slot * NQ + q) * SL_THREADS + tid] = p[q];
+  };
+  auto wget = [&](int tid, int slot, S p[NQ]) {
+#pragma unroll
+    for (int q = 0; q < NQ; q++) p[q] = win[(slot * NQ + q) * SL_THREADS + tid];
+  };
+  // z face kf (between cells kf and kf+1) from P(kf) (`lo`), the window
+  // slot holding P(kf+1) and P(kf+ORDER) (`nx`); ORDER 2 moves pu to cell
+  // kf+1's upper state
+  S pu[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) pu[q] = S(0);
+  auto z_face = [&](int tid, const S lo[NQ], int slot1, const S nx[NQ], S ff[NQ]) {
+    S left[NQ], right[NQ];
+    if constexpr (ORDER == 2) {
+      S p1[NQ];
+      wget(tid, slot1, p1);
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        const S sl = lim_average(lim, p1[q] - lo[q], nx[q] - p1[q]);
+        left[q] = pu[q];
+        right[q] = p1[q] - sl * S(0.5);
+        pu[q] = p1[q] + sl * S(0.5);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        left[q] = lo[q];
+        right[q] = nx[q];
+      }
+    }
+    axis_perm<2>(left);
+    axis_perm<2>(right);
+    ok &= riemann<SOLVER, NQ>(b.ph, left, right, ff);
+    axis_perm<2>(ff);
+  };
+
+  // Window before the first plane: slot (k0 & 1) holds P(k0), the other
+  // P(k0 + 1) (ORDER 2); the face below the slab into fl.
+  S fl[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) fl[q] = S(0);
+  {
+    const int tid = (int)threadIdx.x;
+    const bool own = own_of(tid);
+    const long long col = col_of(tid, own);
+    if (own) {
+      S ur[NQ], pa[NQ], pb[NQ];
+      bool f;
+      if constexpr (ORDER == 2) {
+        S p0[NQ];
+        fetch(col, k0 - 2, ur, f);
+        take(ur, f, p0);
+        fetch(col, k0 - 1, ur, f);
+        take(ur, f, pa);                   // P(k0-1)
+        fetch(col, k0, ur, f);
+        take(ur, f, pb);                   // P(k0)
+#pragma unroll
+        for (int q = 0; q < NQ; q++) pu[q] = pa[q] + lim_average(lim, pa[q] - p0[q], pb[q] - pa[q]) * S(0.5);
+        wput(tid, k0 & 1, pb);
+        fetch(col, k0 + 1, ur, f);
+        take(ur, f, p0);                   // P(k0+1)
+        z_face(tid, pa, k0 & 1, p0, fl);   // face k0-1; pu -> upper state of k0
+        wput(tid, (k0 + 1) & 1, p0);
+      } else {
+        fetch(col, k0 - 1, ur, f);
+        take(ur, f, pa);                   // P(k0-1)
+        fetch(col, k0, ur, f);
+        take(ur, f, pb);                   // P(k0)
+        z_face(tid, pa, 0, pb, fl);        // face k0-1
+        wput(tid, k0 & 1, pb);
+      }
+    }
+  }
+
+#pragma unroll 1
+  for (int m = k0; m <= k1; m++) {
+    if (m > k0) __syncthreads();  // the previous plane's reads of LDS are done
+    // Every per-lane index below is recomputed in each plane from an
+    // opaque copy of the lane id: hoisted out of the march, the per-lane
+    // addresses (a 64-bit pointer per variable and access) would hold
+    // registers for the whole loop.
+    int tv = (int)threadIdx.x;
+    asm volatile("" : "+v"(tv));
+    const int tid = tv;
+    const int c = tid & (TX - 1), r = 2 * (tid >> 6) + ((tid & 63) >> 5);
+    const bool own = c < wx && r < wy;
+    const long long col = col_of(tid, own);
+    const int at = (r + H) * RX + c + H;  // own cell in the primitive area
+    // the next cell of the window (P(m+ORDER)), the own cell into the
+    // primitive area, the halo (and, in a partial tile, the cells past its
+    // edge) from HBM; ghosts by boundary mapping
+    S un[NQ];
+    bool unf = false;
+    if (own) fetch(col, m + ORDER, un, unf);
+    if (own) {
+      S pm[NQ];
+      wget(tid, m & 1, pm);
+#pragma unroll
+      for (int q = 0; q < NQ; q++) lp[q * PS + at] = pm[q];
+    }
+    __builtin_amdgcn_s_setprio(3);
+    {
+      double u[2][NQ];
+      int dst[2];
+#pragma unroll
+      for (int it = 0; it < 2; it++) {
+        const int idx = tid + it * SL_THREADS;
+        int rx, ry;
+        if (idx < H * TX) {
+          ry = idx / TX;
+          rx = H + idx % TX;
+        } else if (idx < H * TX + TYT * RX) {
+          const int t = idx - H * TX;
+          ry = H + t / RX;
+          rx = t % RX;
+        } else {
+          const int t = idx - H * TX - TYT * RX;
+          ry = H + TYT + t / TX;
+          rx = H + t % TX;
+        }
+        const bool own_pos = rx >= H && rx < H + wx && ry >= H && ry < H + wy;
+        dst[it] = -1;
+        if (idx < L::NLOAD && rx < wx + 2 * H && ry < wy + 2 * H && !own_pos) {
+          dst[it] = ry * RX + rx;
+          load_cons<NQ>(g, Usrc, i0 - H + rx, j0 - H + ry, m, u[it]);
+        }
+      }
+      __builtin_amdgcn_s_setprio(0);
+#pragma unroll
+      for (int it = 0; it < 2; it++) {
+        if (dst[it] < 0) continue;
+        double p[NQ];
+        u2prim<NQ>(b.ph, u[it], p);
+#pragma unroll
+        for (int q = 0; q < NQ; q++) lp[q * PS + dst[it]] = (S)p[q];
+      }
+    }
+    // z face m (above the own cell): P(m) from the window slot, P(m+ORDER)
+    // just fetched; the window moves up one cell
+    S ff[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) ff[q] = S(0);
+    if (own) {
+      S pm[NQ], pn[NQ];
+      wget(tid, m & 1, pm);
+      take(un, unf, pn);
+      if constexpr (ORDER == 2) {
+        z_face(tid, pm, (m + 1) & 1, pn, ff);
+        wput(tid, m & 1, pn);  // slot of P(m) now holds P(m+2)
+      } else {
+        z_face(tid, pm, 0, pn, ff);
+        wput(tid, (m + 1) & 1, pn);
+      }
+    }
+    __syncthreads();
+
+    const long long cc = col + (long long)(m + 1) * sz;
+    S fx[NQ], fy[NQ], fxr[NQ], fyu[NQ];
+    double u0[NQ];
+    ok &= xy_faces<SOLVER, ORDER, NQ, TYT>(b, lds, wx, wy, tid, fx, fxr, fy, fyu, [&] {
+      if (own) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++)
+          if (q < NH || q >= 8) u0[q] = (double)b.U[q * N + cc];
+      }
+    });
+    if (own) {
+      // the x/y partials, rounded as the xy kernel stores them in W, then
+      // the z term (step, hydro_solver.f90:105-107 / flux_cd_module.f90:299-308)
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        if (q >= NH && q < 8) continue;
+        double wv;
+        if constexpr (INC)
+          wv = u0[q] + (double)(float)((-(dtdx * ((double)fxr[q] - (double)fx[q]))) -
+                                       dtdy * ((double)fyu[q] - (double)fy[q]));
+        else
+          wv = (u0[q] - dtdx * ((double)fxr[q] - (double)fx[q])) - dtdy * ((double)fyu[q] - (double)fy[q]);
+        Uout[q * N + cc] = (S)(wv - dtdz * ((double)ff[q] - (double)fl[q]));
+      }
+      if constexpr (FLUXCD) {
+        const double w5 = (double)(S)((-(double)fy[7]) - (double)fyu[7]);
+        const double w6 = (double)(S)((+(double)fx[7]) + (double)fxr[7]);
+        b.E[2 * N + cc] = (S)(0.25 * ((((-(double)fx[6]) - (double)fxr[6]) + (double)fy[5]) + (double)fyu[5]));
+        b.E[0 * N + cc] = (S)(0.25 * ((w5 + (double)fl[6]) + (double)ff[6]));
+        b.E[1 * N + cc] = (S)(0.25 * ((w6 - (double)fl[5]) - (double)ff[5]));
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; q++) fl[q] = ff[q];
+  }
+  if (!ok) atomicOr(err, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -1328,6 +1641,15 @@ __global__ void __launch_bounds__(256) k_z(Box<S> b, double dt, int kchunk, int*
   z_body<SOLVER, ORDER, FLUXCD, S, NP, W8>(b, xcd_logical_block(), dt, kchunk, err);
 }
 
+// the 2.5-D stage kernel: up to 256 VGPRs (two waves per SIMD, two
+// workgroups per CU) for the column window carried across the plane march
+template <int SOLVER, int ORDER, int FLUXCD, typename S, int NP>
+__global__ void __launch_bounds__(SL_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
+k_slab(Box<S> b, double dt, int kchunk, int* err) {
+  __shared__ S lds[SlabLds<ORDER, 8 + NP, S>::SIZE];
+  slab_body<SOLVER, ORDER, FLUXCD, S, NP>(b, xcd_logical_block(), dt, kchunk, err, lds);
+}
+
 template <typename S>
 __global__ void __launch_bounds__(256) k_bfield(Box<S> b, double dt, int stage, int kchunk) {
   bfield_body(b, xcd_logical_block(), dt, stage, kchunk);
@@ -1617,6 +1939,37 @@ void launch_z(hipStream_t s, const Box<S>& b, double dt, int stage, int solver, 
 #undef GM_CALL
 }
 
+unsigned slab_tiles(const Geom& g, int kchunk) {
+  return ceil_div(g.nx, TX) * ceil_div(g.ny, TYS) * ceil_div(g.nz, kchunk);
+}
+
+// z-planes per slab: about 2048 workgroups (eight per CU), at least 8
+// planes per slab so the window's warm-up (three cells and a face below the
+// slab) stays small, whole grid in one slab when the grid is thin
+int slab_kchunk(const Geom& g) {
+  const long long per_plane = (long long)ceil_div(g.nx, TX) * ceil_div(g.ny, TYS);
+  long long nslab = (2048 + per_plane - 1) / per_plane;
+  const long long max_slab = g.nz / 8 > 1 ? g.nz / 8 : 1;
+  if (nslab > max_slab) nslab = max_slab;
+  if (nslab < 1) nslab = 1;
+  return (int)((g.nz + nslab - 1) / nslab);
+}
+
+template <typename S>
+void launch_slab(hipStream_t s, const Box<S>& b, double dt, int stage, int solver, int flux_cd, int* err,
+                 int kchunk) {
+  const unsigned n = slab_tiles(b.g, kchunk);
+#define GM_CALL(SV, O, F)                                                                                        \
+  {                                                                                                              \
+    if (b.g.npas == 0) hipLaunchKernelGGL((k_slab<SV, O, F, S, 0>), dim3(n), dim3(SL_THREADS), 0, s, b, dt, kchunk, err); \
+    else if (b.g.npas == 1)                                                                                      \
+      hipLaunchKernelGGL((k_slab<SV, O, F, S, 1>), dim3(n), dim3(SL_THREADS), 0, s, b, dt, kchunk, err);        \
+    else hipLaunchKernelGGL((k_slab<SV, O, F, S, 2>), dim3(n), dim3(SL_THREADS), 0, s, b, dt, kchunk, err);    \
+  }
+  GM_STAGE_DISPATCH(GM_CALL)
+#undef GM_CALL
+}
+
 template <typename S>
 void launch_bfield(hipStream_t s, const Box<S>& b, double dt, int stage, const Shape& sh) {
   hipLaunchKernelGGL(k_bfield<S>, dim3(box_tiles(b.g, KC_COL, sh)), dim3(sh.block), 0, s, b, dt, stage, sh.kchunk);
@@ -1773,6 +2126,7 @@ void launch_divb(hipStream_t s, const Geom& g, const S* U, double dx, double dy,
 #define GM_INSTANTIATE(S)                                                                                        \
   template void launch_xy<S>(hipStream_t, const Box<S>&, double, int, int, int, int*);                            \
   template void launch_z<S>(hipStream_t, const Box<S>&, double, int, int, int, int*, const Shape&);               \
+  template void launch_slab<S>(hipStream_t, const Box<S>&, double, int, int, int, int*, int);                     \
   template void launch_bfield<S>(hipStream_t, const Box<S>&, double, int, const Shape&);                          \
   template void launch_cfl<S>(hipStream_t, const Box<S>&, unsigned long long*, int*, const Shape&);               \
   template void launch_bfield_cfl<S>(hipStream_t, const Box<S>&, double, unsigned long long*, int*, const Shape&); \

@@ -102,6 +102,16 @@ void launch_xy(hipStream_t s, const Box<S>& b, double dt, int stage, int solver,
 template <typename S>
 void launch_z(hipStream_t s, const Box<S>& b, double dt, int stage, int solver, int flux_cd, int* err,
               const Shape& sh);
+// 2.5-D form of one stage: launch_xy and launch_z in one kernel that marches
+// 32 x 8 column tiles up slabs of `kchunk` planes (no W round trip); same
+// results bit for bit.  Not for boxes whose stage kernels add source terms
+// (Box::srcin).
+template <typename S>
+void launch_slab(hipStream_t s, const Box<S>& b, double dt, int stage, int solver, int flux_cd, int* err,
+                 int kchunk);
+// default slab length for a grid, and the workgroups of a slab launch
+int slab_kchunk(const Geom& g);
+unsigned slab_tiles(const Geom& g, int kchunk);
 template <typename S>
 void launch_bfield(hipStream_t s, const Box<S>& b, double dt, int stage, const Shape& sh);
 template <typename S>

@@ -139,6 +139,7 @@ int gm_tstep(gm_ctx *ctx, double dt);
 /* Wait for queued work; report any deferred device error. */
 int gm_sync(gm_ctx *ctx);
 
+
 /* Divergence of B with the dump formula of write_BIN
  * (src/Out_BIN_Module.f90:214-222), divb(nx, ny, nz) column-major. */
 int gm_download_divb(gm_ctx *ctx, double *divb);
@@ -148,7 +149,7 @@ int gm_download_divb(gm_ctx *ctx, double *divb);
  * kernel on the context's stream.  gm_kernel_times returns, per kernel
  * class, the accumulated milliseconds and launch count since the last
  * reset, and the kernel names (static strings). */
-#define GM_NKCLASS 13
+#define GM_NKCLASS 15
 int gm_set_timing(gm_ctx *ctx, int enable);
 int gm_kernel_times(gm_ctx *ctx, double *ms, int64_t *launches, const char **names);
 
