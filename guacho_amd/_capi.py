@@ -25,7 +25,7 @@ GM_NKCLASS = 15
 # exported entry points, in the order include/guacho_amd.h declares them
 EXPORTS = (
     "gm_create", "gm_destroy", "gm_last_error", "gm_upload_u", "gm_download_u",
-    "gm_download_primit", "gm_boundary_u", "gm_cfl_dtp", "gm_tstep", "gm_sync",
+    "gm_download_primit", "gm_boundary_u", "gm_cfl_dtp", "gm_tstep", "gm_sync", "gm_tstep_noalloc",
     "gm_download_divb", "gm_set_timing", "gm_kernel_times", "gm_device_bytes",
     "gm_set_time", "gm_set_user_sources", "gm_set_user_bc", "gm_set_cooling", "gm_set_conduction",
     "gm_conduction_substeps", "gm_set_eos",
@@ -95,6 +95,8 @@ def lib() -> C.CDLL:
     L.gm_cfl_dtp.argtypes = [vp, C.POINTER(C.c_double)]
     L.gm_tstep.argtypes = [vp, C.c_double]
     L.gm_sync.argtypes = [vp]
+    L.gm_tstep_noalloc.argtypes = [C.POINTER(GmParams), C.c_void_p, C.c_double, C.c_void_p, C.c_void_p,
+                                   C.c_void_p]
     L.gm_download_divb.argtypes = [vp, dp]
     L.gm_set_timing.argtypes = [vp, C.c_int]
     L.gm_kernel_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.POINTER(C.c_char_p)]

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -515,6 +516,122 @@ int read_errors(gm_ctx* c) {
   return GM_OK;
 }
 
+// ---------------------------------------------------------------------------
+// gm_tstep_noalloc: the step on a caller-held device state with every
+// buffer beyond that state -- U*, the stage partials W, E and the device
+// scalars -- carved from one file-scope device array, so the call allocates
+// nothing.  Each device that runs it holds its own copy of the array (the
+// code object's data, until the process ends); host code reaches it only
+// through the address hipGetSymbolAddress returns on that device.  Calls
+// that overlap on one device share the array and race.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = (size_t)256 << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {nullptr};
+std::mutex g_scratch_mu;
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// device bytes gm_tstep_noalloc needs for params (U*, W, E, scalars)
+size_t noalloc_bytes(const gm_params* p) {
+  const size_t ncell = (size_t)(p->nx + 4) * (p->ny + 4) * (p->nz + 4);
+  const size_t es = p->precision == GM_PRECISION_MIXED ? 4 : 8, nq = (size_t)p->neq;
+  return align256(ncell * nq * es) + align256(ncell * nq * sizeof(double)) + align256(ncell * 3 * es) +
+         align256(sizeof(Scalars));
+}
+
+// address of this device's scratch array (the device is current)
+int scratch_of(int dev, unsigned char** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* a = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !a) {
+      (void)hipGetLastError();
+      g_create_error = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return GM_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = a;
+  }
+  *out = (unsigned char*)g_scratch_ptr[dev];
+  return GM_OK;
+}
+
+__global__ void k_publish(const Scalars* sc, double* dtp, int32_t* err) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    if (dtp) *dtp = __longlong_as_double((long long)sc->dtmin);
+    if (err) *err = sc->err;
+  }
+}
+
+// tstep (src/hydro_solver.f90:134-229) with the add-ons a context adds
+// (user plugins, cooling, conduction) off: the kernels and order of
+// tstep_t, then get_timestep's minimum over the new state, as gm_cfl_dtp
+// computes it.  Scalars are set by kernels, not memsets, so a caller may
+// capture the launches in a graph.
+template <typename S>
+void noalloc_tstep_t(hipStream_t s, const gm_params& p, S* u, unsigned char* scr, double dt) {
+  gm::Box<S> b;
+  b.g.nx = p.nx; b.g.ny = p.ny; b.g.nz = p.nz;
+  b.g.NX = p.nx + 4; b.g.NY = p.ny + 4; b.g.NZ = p.nz + 4;
+  b.g.ncell = (long long)b.g.NX * b.g.NY * b.g.NZ;
+  for (int f = 0; f < 6; f++) b.g.bc[f] = p.bc[f];
+  b.g.gmem = 0;
+  b.g.npas = p.neq - 8;
+  b.ph.cv = p.cv;
+  b.ph.gamma = p.gamma;
+  b.ph.mhd = p.mhd ? 1 : 0;
+  b.ph.pmhd = p.pmhd ? 1 : 0;
+  b.ph.limiter = p.slope_limiter;
+  b.ph.eos = GM_EOS_ADIABATIC;
+  b.ph.tempsc = 1.0;
+  const size_t ncell = (size_t)b.g.ncell, nq = (size_t)p.neq;
+  size_t off = 0;
+  b.U = u;
+  b.Us = (S*)(scr + off);
+  off += align256(ncell * nq * sizeof(S));
+  b.W = (double*)(scr + off);
+  off += align256(ncell * nq * sizeof(double));
+  b.E = (S*)(scr + off);
+  off += align256(ncell * 3 * sizeof(S));
+  Scalars* sc = (Scalars*)(scr + off);
+  b.dx = p.dx; b.dy = p.dy; b.dz = p.dz;
+  b.slot = 0;
+  b.src = p.eight_wave ? gm::SRC_EIGHT_WAVE : 0;
+  b.srcin = (b.src != 0 && b.g.npas == 0) ? 1 : 0;
+  b.usrc = nullptr;
+  const gm::Shape sh = {256, 16};
+  const int solver = p.riemann_solver, fcd = p.enable_flux_cd ? 1 : 0;
+  int* err = &sc->err;
+  const double dtm = dt / 2.0;  // hydro_solver.f90:152
+  const int kslab = b.srcin ? 0 : gm::slab_kchunk(b.g);
+  auto stage = [&](double h, int st) {
+    if (kslab) {
+      gm::launch_slab(s, b, h, st, solver, fcd, err, kslab);
+    } else {
+      gm::launch_xy(s, b, h, st, solver, fcd, err);
+      gm::launch_z(s, b, h, st, solver, fcd, err, sh);
+    }
+  };
+  hipLaunchKernelGGL(k_clear_err, dim3(1), dim3(1), 0, s, sc);
+  stage(dtm, 1);
+  if (fcd) gm::launch_bfield(s, b, dtm, 1, sh);
+  if (b.src && !b.srcin) gm::launch_sources(s, b, dtm, 1, sh);
+  stage(dt, 2);
+  hipLaunchKernelGGL(k_init_dtmin, dim3(1), dim3(1), 0, s, sc);
+  const bool fuse_cfl = fcd && !b.src && p.eta == 0.0;
+  if (fuse_cfl) {
+    gm::launch_bfield_cfl(s, b, dt, &sc->dtmin, err, sh);
+  } else {
+    if (fcd) gm::launch_bfield(s, b, dt, 2, sh);
+    if (b.src && !b.srcin) gm::launch_sources(s, b, dt, 2, sh);
+    if (p.eta != 0.0) gm::launch_viscous(s, b, p.eta);
+    gm::launch_cfl(s, b, &sc->dtmin, err, sh);
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -895,6 +1012,31 @@ int gm_device_bytes(const gm_ctx* c, int64_t* bytes) {
   if (!c || !bytes) return GM_ERR_ARG;
   *bytes = c->bytes;
   return GM_OK;
+}
+
+int gm_tstep_noalloc(const gm_params* params, void* u, double dt, void* stream, double* dtp_next, int32_t* err) {
+  int rc = validate(params);
+  if (rc != GM_OK) return rc;
+  for (int f = 0; f < 6; f++)
+    if (params->bc[f] == GM_BC_OTHER)
+      return fail(nullptr, GM_ERR_CONFIG, "BC_OTHER faces need a context with a user boundary plugin");
+  const size_t need = noalloc_bytes(params);
+  if (need > kScratchBytes)
+    return fail(nullptr, GM_ERR_CONFIG,
+                "grid too large for gm_tstep_noalloc: needs " + std::to_string(need) + " scratch bytes, the limit is " +
+                    std::to_string(kScratchBytes) + " (256 MiB)");
+  if (!u) return fail(nullptr, GM_ERR_ARG, "u is NULL");
+  if (!std::isfinite(dt)) return fail(nullptr, GM_ERR_ARG, "dt is not finite");
+  if (params->device < 0 || params->device >= kMaxDevices) return fail(nullptr, GM_ERR_ARG, "device out of range");
+  if ((rc = hip_check(nullptr, hipSetDevice(params->device), "hipSetDevice")) != GM_OK) return rc;
+  unsigned char* scr = nullptr;
+  if ((rc = scratch_of(params->device, &scr)) != GM_OK) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  if (params->precision == GM_PRECISION_MIXED) noalloc_tstep_t<float>(s, *params, (float*)u, scr, dt);
+  else noalloc_tstep_t<double>(s, *params, (double*)u, scr, dt);
+  const size_t off = noalloc_bytes(params) - align256(sizeof(Scalars));
+  if (dtp_next || err) hipLaunchKernelGGL(k_publish, dim3(1), dim3(1), 0, s, (const Scalars*)(scr + off), dtp_next, err);
+  return hip_check(nullptr, hipGetLastError(), "kernel launch");
 }
 
 }  // extern "C"

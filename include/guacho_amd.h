@@ -139,6 +139,33 @@ int gm_tstep(gm_ctx *ctx, double dt);
 /* Wait for queued work; report any deferred device error. */
 int gm_sync(gm_ctx *ctx);
 
+/* No counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): one gm_tstep plus the next
+ * gm_cfl_dtp's minimum, on a caller-held device state, with no context and
+ * no allocation.  U*, the stage partials, E and the device scalars live in
+ * one file-scope device array of 256 MiB that each GPU running this call
+ * holds until the process ends; a grid needing more (FP64: about 117^3
+ * cells, mixed: 131^3) is refused with GM_ERR_CONFIG before any device
+ * work.  Calls that overlap on one GPU share that array and race.
+ *   params    as for gm_create; BC_OTHER faces are refused (they need a
+ *             context's user boundary plugin); EOS adiabatic
+ *   u         device pointer to the state in the library's device layout:
+ *             variable q of cell (i,j,k) (1-based, two ghost layers) at
+ *             u[q*ncell + ((k+1)*(ny+4) + (j+1))*(nx+4) + (i+1)],
+ *             ncell = (nx+4)(ny+4)(nz+4); doubles for GM_PRECISION_FP64,
+ *             floats for GM_PRECISION_MIXED.  Ghost cells are neither read
+ *             nor written (boundary mapping, as in gm_tstep).
+ *   stream    hipStream_t to launch on, NULL for the null stream; the call
+ *             only enqueues kernels (no memset, copy or synchronisation)
+ *   dtp_next  device double or NULL: get_timestep's minimum
+ *             (src/hydro_core.f90:642-675) over the new state
+ *   err       device int32 or NULL: 0, or the kernels' error bits (1: a
+ *             Riemann solver fell through on a non-finite state)
+ * Errors before any launch: GM_ERR_CONFIG / GM_ERR_ARG (message in
+ * gm_last_error(NULL)). */
+int gm_tstep_noalloc(const gm_params *params, void *u, double dt, void *stream, double *dtp_next,
+                     int32_t *err);
 
 /* Divergence of B with the dump formula of write_BIN
  * (src/Out_BIN_Module.f90:214-222), divb(nx, ny, nz) column-major. */

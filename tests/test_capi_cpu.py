@@ -71,3 +71,17 @@ def test_null_arguments():
     assert L.gm_create(None, None) == _capi.GM_ERR_ARG
     assert L.gm_tstep(None, 0.1) == _capi.GM_ERR_ARG
     assert L.gm_destroy(None) == _capi.GM_ERR_ARG
+
+
+def test_noalloc_refuses_oversized_grid_before_device_work():
+    # 200^3 FP64 needs ~1.3 GB of scratch, past the 256 MiB array; refused
+    # from the arguments alone, before any device call
+    L = _capi.lib()
+    p = _create(nx=200, ny=200, nz=200)
+    rc = L.gm_tstep_noalloc(C.byref(p), C.c_void_p(1), 1e-3, None, None, None)
+    assert rc == _capi.GM_ERR_CONFIG
+    assert "too large" in L.gm_last_error(None).decode()
+    p = _create(bc=(4, 4, 3, 3, 3, 3))
+    assert L.gm_tstep_noalloc(C.byref(p), C.c_void_p(1), 1e-3, None, None, None) == _capi.GM_ERR_CONFIG
+    p = _create()
+    assert L.gm_tstep_noalloc(C.byref(p), None, 1e-3, None, None, None) == _capi.GM_ERR_ARG
