@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision section")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-boxes section")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving section")
+    ap.add_argument("--no-ot", action="store_true", help="skip the Orszag-Tang 128x128x4 section")
     ap.add_argument("--no-extensions", action="store_true",
                     help="skip the 8-wave / user-plugin section")
     ap.add_argument("--serving-requests", type=int, default=24)
@@ -337,6 +338,79 @@ def batch_section(nbox: int, steps: int, seed: int):
             "repeat_bitwise_equal": same_m and same_f}
 
 
+def ot_section(cpu_threads: int):
+    """Orszag-Tang 128x128x4 to t = 0.5 (HLLD, minmod, flux-CD, CFL 0.4,
+    dumps every 0.1 as the reference's driver takes them; the dumps are
+    not written): the reference's host time loop over the single-box path
+    (one CFL read-back and a handful of launches per step: launch-latency
+    bound at 65,536 cells), the same box as a one-box batch whose clock
+    (ramp and t_end clamp) runs on the device, 64 steps per graph launch
+    (its only clamp is onto t = 0.5, so its step count can differ from the
+    host loop's by the dump clamps), and the C restatement on the host
+    cores with the host loop.  FP64 throughout."""
+    import numpy as np
+    from guacho_amd import _capi, problems
+    from guacho_amd.host import timestep_clamp
+    from oracle.oracle import Config, Oracle
+    n, nz, tmax, dtprint = 128, 4, 0.5, 0.1
+    u0, dx, dy, dz = problems.orszag_tang(n, n, nz)
+    prm = _capi.make_params(n, n, nz, dx, dy, dz)
+
+    def host_loop(step_fn, dtp_fn):
+        t, tprint, it = 0.0, dtprint, 1
+        while t <= tmax:
+            dt, dump = timestep_clamp(it, 10, 0.4, dtp_fn(), t, tprint)
+            step_fn(dt)
+            t = t + dt
+            if dump:
+                tprint = tprint + dtprint
+            it += 1
+        return it - 1
+    log("OT 128x128x4 section: host loop, single-box path")
+    with _capi.Subdomain(prm) as dev:
+        dev.upload_u(u0)
+        host_loop(dev.tstep, dev.cfl_dtp)          # warm-up run
+        dev.upload_u(u0)
+        dev.sync()
+        t0 = time.perf_counter()
+        steps = host_loop(dev.tstep, dev.cfl_dtp)
+        dev.sync()
+        el_host = time.perf_counter() - t0
+        u_host = dev.download_u()
+    log("OT 128x128x4 section: one-box batch, device clock, 64 steps per graph launch")
+    el_b = None
+    for rep in range(2):                           # the first run captures the graph
+        with _capi.Batch(precision=_capi.GM_PRECISION_FP64, max_boxes=1, steps_per_launch=64) as B:
+            box = B.add(prm, u0, cfl=0.4, t_end=tmax)
+            t0 = time.perf_counter()
+            while True:
+                B.advance(1)
+                B.sync()
+                _, bsteps, done = B.clock(box)
+                if done:
+                    break
+            el_b = time.perf_counter() - t0
+    log(f"OT 128x128x4 section: C restatement, {cpu_threads} threads")
+    O = Oracle(Config(nx=n, ny=n, nz=nz, dx=dx, dy=dy, dz=dz), nthreads=cpu_threads)
+    O.set_u(u0)
+    O.boundaryI()
+    O.calcprim_u()
+    t0 = time.perf_counter()
+    csteps = host_loop(O.tstep, O.cfl_dtp)
+    el_cpu = time.perf_counter() - t0
+    same = bool(np.array_equal(u_host[:, 2:-2, 2:-2, 2:-2], O.get_u()[:, 2:-2, 2:-2, 2:-2]))
+    O.close()
+    zones = float(n * n * nz)
+    return {"workload": "Orszag-Tang 128x128x4 to t=0.5, FP64 HLLD minmod flux-CD, CFL 0.4", "unit": "zone-updates/s",
+            "steps": steps, "host_loop": {"value": round(zones * steps / el_host, 1),
+                                          "ms_per_step": round(1e3 * el_host / steps, 4), "seconds": round(el_host, 4)},
+            "batch_graph": {"value": round(zones * bsteps / el_b, 1), "ms_per_step": round(1e3 * el_b / bsteps, 4),
+                            "seconds": round(el_b, 4), "steps": bsteps, "steps_per_launch": 64},
+            "cpu_restatement": {"value": round(zones * csteps / el_cpu, 1), "seconds": round(el_cpu, 3),
+                                "threads": cpu_threads, "steps": csteps},
+            "gpu_equals_restatement_at_end": same}
+
+
 def extensions_section(u0, dx, dy, dz, n: int, steps: int):
     """The step with the reference's optional terms, FP64 on the seeded
     turbulence box, against the same box without them: Powell 8-wave
@@ -492,6 +566,8 @@ def main():
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_ach / HBM_PEAK_GBS, 4)},
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items() if v[1]},
     }
+    if not args.no_ot:
+        out["ot128"] = ot_section(int(os.environ.get("OMP_NUM_THREADS") or host_physical_cores() or 1))
     if not args.no_extensions:
         out["extensions"] = extensions_section(u0, dx, dy, dz, n, 10)
     if not args.no_serving:

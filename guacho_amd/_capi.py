@@ -31,7 +31,7 @@ EXPORTS = (
     "gm_conduction_substeps", "gm_set_eos",
     "gm_batch_create", "gm_batch_destroy", "gm_batch_last_error", "gm_batch_box_bytes",
     "gm_batch_free_bytes", "gm_batch_add", "gm_batch_remove", "gm_batch_advance", "gm_batch_sync",
-    "gm_batch_clock", "gm_batch_download_u", "gm_batch_tuning", "gm_batch_active",
+    "gm_batch_query", "gm_batch_clock", "gm_batch_download_u", "gm_batch_tuning", "gm_batch_active",
 )
 
 
@@ -119,6 +119,7 @@ def lib() -> C.CDLL:
     L.gm_batch_remove.argtypes = [vp, C.c_int32]
     L.gm_batch_advance.argtypes = [vp, C.c_int32]
     L.gm_batch_sync.argtypes = [vp]
+    L.gm_batch_query.argtypes = [vp, C.POINTER(C.c_int32)]
     L.gm_batch_clock.argtypes = [vp, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                  C.POINTER(C.c_int32)]
     L.gm_batch_download_u.argtypes = [vp, C.c_int32, dp]
@@ -355,6 +356,13 @@ class Batch:
 
     def sync(self):
         self._check(self.L.gm_batch_sync(self.h))
+
+    def query(self) -> bool:
+        """True once the queued launches have finished (clocks refreshed);
+        never waits."""
+        idle = C.c_int32()
+        self._check(self.L.gm_batch_query(self.h, C.byref(idle)))
+        return bool(idle.value)
 
     def clock(self, box: int):
         t, n, d = C.c_double(), C.c_int64(), C.c_int32()

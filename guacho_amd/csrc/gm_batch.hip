@@ -26,6 +26,12 @@
 
 namespace {
 
+unsigned long long gm_bits_of(double x) {
+  unsigned long long u;
+  std::memcpy(&u, &x, sizeof u);
+  return u;
+}
+
 struct Slot {
   bool used = false;
   gm_params p{};
@@ -44,17 +50,25 @@ constexpr int kMaxBoxes = 4096;
 struct gm_batch {
   gm_batch_params bp{};
   size_t esize = 8;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;  // the steps (graph launches)
+  hipStream_t xfer = nullptr;    // admissions and downloads, beside running steps
   std::vector<Slot> slots;
   int64_t budget = 0;
   int64_t used = 0;
   void* dboxes = nullptr;          // Box<S>[max_boxes], active boxes packed
   unsigned* dprefix = nullptr;     // KC_COUNT x (max_boxes + 1)
+  int* dslots = nullptr;           // clock slots of the packed table
   gm::BoxClock* dclk = nullptr;    // one clock per slot
   gm::BoxClock* hclk = nullptr;    // pinned mirror
   int* derr = nullptr;
   int* herr = nullptr;
-  std::vector<int> order;          // slots of the packed table
+  std::vector<int> order;          // slots of the packed table: admitted boxes not yet done
+  // the launches in flight: their boxes' slots, the event after their clock
+  // read-back, and device buffers whose release waits for them
+  bool in_flight = false;
+  std::vector<int> launched;
+  hipEvent_t ev_launch = nullptr;
+  std::vector<void*> zombies;
   bool any_sources = false;        // some active box has 8-wave source terms
   unsigned tiles[gm::KC_COUNT] = {0};
   bool tables_dirty = true;
@@ -157,7 +171,7 @@ int rebuild_tables_t(gm_batch* b, const gm::Shape& sh) {
   const int cap = b->bp.max_boxes;
   b->order.clear();
   for (int i = 0; i < cap; i++)
-    if (b->slots[i].used) b->order.push_back(i);
+    if (b->slots[i].used && !b->slots[i].clock.done) b->order.push_back(i);
   const int n = (int)b->order.size();
   std::vector<gm::Box<S>> boxes(n);
   std::vector<unsigned> prefix((size_t)gm::KC_COUNT * (cap + 1), 0u);
@@ -180,6 +194,7 @@ int rebuild_tables_t(gm_batch* b, const gm::Shape& sh) {
   }
   BHIP(b, hipStreamSynchronize(b->stream));
   if (n) BHIP(b, hipMemcpy(b->dboxes, boxes.data(), sizeof(gm::Box<S>) * n, hipMemcpyHostToDevice));
+  if (n) BHIP(b, hipMemcpy(b->dslots, b->order.data(), sizeof(int) * n, hipMemcpyHostToDevice));
   BHIP(b, hipMemcpy(b->dprefix, prefix.data(), sizeof(unsigned) * prefix.size(), hipMemcpyHostToDevice));
   b->tables_dirty = false;
   return GM_OK;
@@ -201,14 +216,22 @@ void enqueue_steps_t(gm_batch* b, int nsteps, int tuning) {
   const gm::Shape& sh = b->shape;
   const int solver = b->bp.riemann_solver, fcd = b->bp.enable_flux_cd;
   const unsigned txy = b->tiles[gm::KC_XY], tcol = b->tiles[gm::KC_COL];
+  // With flux-CD and no source pass, each box's CFL minimum for the next
+  // step is folded into the corrector's field update (its clock holds it;
+  // gm_batch_add reduces a new box's first one); otherwise a CFL pass
+  // starts every step.
+  const bool fold = fcd && !b->any_sources;
   for (int st = 0; st < nsteps; st++) {
-    gm::launch_clock_begin(s, b->dclk, cap);
-    gm::launch_cfl_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, b->derr, sh);
-    gm::launch_clock_dt(s, b->dclk, cap, tuning);
+    if (!fold) {
+      gm::launch_clock_begin(s, b->dclk, b->dslots, n);
+      gm::launch_cfl_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, b->derr, sh);
+    }
+    gm::launch_clock_dt(s, b->dclk, b->dslots, n, tuning);
     for (int stage = 1; stage <= 2; stage++) {
       gm::launch_xy_batch(s, boxes, pre(gm::KC_XY), n, txy, b->dclk, stage, solver, fcd, b->derr);
       gm::launch_z_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, stage, solver, fcd, b->derr, sh);
-      if (fcd) gm::launch_bfield_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, stage, sh);
+      if (fold && stage == 2) gm::launch_bfield_cfl_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, b->derr, sh);
+      else if (fcd) gm::launch_bfield_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, stage, sh);
       if (b->any_sources) gm::launch_sources_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, stage, sh);
     }
   }
@@ -280,14 +303,28 @@ int autotune(gm_batch* b) {
   return GM_OK;
 }
 
-int read_back(gm_batch* b) {
-  BHIP(b, hipStreamSynchronize(b->stream));
-  BHIP(b, hipMemcpy(b->hclk, b->dclk, sizeof(gm::BoxClock) * b->bp.max_boxes, hipMemcpyDeviceToHost));
-  BHIP(b, hipMemcpy(b->herr, b->derr, sizeof(int), hipMemcpyDeviceToHost));
-  for (int i = 0; i < b->bp.max_boxes; i++)
-    if (b->slots[i].used) b->slots[i].clock = b->hclk[i];
+void free_zombies(gm_batch* b) {
+  for (void* p : b->zombies) (void)hipFree(p);
+  b->zombies.clear();
+}
+
+// The launches in flight have finished: their boxes' clocks from the
+// read-back, boxes that completed leave the next launch's table, buffers
+// released meanwhile are freed; a deferred device error is reported.
+int finish_launch(gm_batch* b) {
+  if (b->in_flight) {
+    b->in_flight = false;
+    for (int i : b->launched) {
+      if (!b->slots[i].used) continue;
+      b->slots[i].clock = b->hclk[i];
+      if (b->hclk[i].done) b->tables_dirty = true;
+    }
+    b->launched.clear();
+  }
+  free_zombies(b);
   if (*b->herr) {
     const int e = *b->herr;
+    *b->herr = 0;
     BHIP(b, hipMemset(b->derr, 0, sizeof(int)));
     return bfail(b, GM_ERR_NONFINITE,
                  (e & 1) ? "Riemann solver fell through on a non-finite state (hlld.f90:316-317)"
@@ -296,22 +333,36 @@ int read_back(gm_batch* b) {
   return GM_OK;
 }
 
+// Wait for the launches in flight (only if some are).
+int wait_launch(gm_batch* b) {
+  if (!b->in_flight) return GM_OK;
+  BHIP(b, hipStreamSynchronize(b->stream));
+  return finish_launch(b);
+}
+
+bool in_launch(const gm_batch* b, int slot) {
+  if (!b->in_flight) return false;
+  for (int i : b->launched)
+    if (i == slot) return true;
+  return false;
+}
+
 bool valid_id(const gm_batch* b, int32_t id) {
   return id >= 0 && id < b->bp.max_boxes && b->slots[id].used;
 }
 
 template <typename S>
-void ghost_fill_t(gm_batch* b, const Slot& s) {
+void ghost_fill_t(hipStream_t st, const Slot& s) {
   const gm::Geom g = geom_of(s.p);
   S* U = (S*)s.U;
   for (int a = 0; a < 3; a++)
     if (g.bc[2 * a] == GM_BC_PERIODIC) {
-      gm::launch_ghost(b->stream, g, U, a, 0, GM_BC_PERIODIC, 1);
-      gm::launch_ghost(b->stream, g, U, a, 1, GM_BC_PERIODIC, 1);
+      gm::launch_ghost(st, g, U, a, 0, GM_BC_PERIODIC, 1);
+      gm::launch_ghost(st, g, U, a, 1, GM_BC_PERIODIC, 1);
     }
   for (int kind : {GM_BC_CLOSED, GM_BC_OUTFLOW})
     for (int f = 0; f < 6; f++)
-      if (g.bc[f] == kind) gm::launch_ghost(b->stream, g, U, f / 2, f % 2, kind, 1);
+      if (g.bc[f] == kind) gm::launch_ghost(st, g, U, f / 2, f % 2, kind, 1);
 }
 
 }  // namespace
@@ -344,6 +395,10 @@ int gm_batch_create(const gm_batch_params* bp, gm_batch** out) {
   if ((rc = bhip(b, hipSetDevice(bp->device), "hipSetDevice")) != GM_OK) return bail(rc);
   if ((rc = bhip(b, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate")) != GM_OK)
     return bail(rc);
+  if ((rc = bhip(b, hipStreamCreateWithFlags(&b->xfer, hipStreamNonBlocking), "hipStreamCreate")) != GM_OK)
+    return bail(rc);
+  if ((rc = bhip(b, hipEventCreateWithFlags(&b->ev_launch, hipEventDisableTiming), "hipEventCreate")) != GM_OK)
+    return bail(rc);
   size_t freeb = 0, totalb = 0;
   if ((rc = bhip(b, hipMemGetInfo(&freeb, &totalb), "hipMemGetInfo")) != GM_OK) return bail(rc);
   b->budget = bp->memory_budget > 0 ? bp->memory_budget : (int64_t)(0.8 * (double)freeb);
@@ -353,6 +408,7 @@ int gm_batch_create(const gm_batch_params* bp, gm_batch** out) {
       GM_OK)
     return bail(rc);
   if ((rc = bhip(b, hipMalloc((void**)&b->dclk, sizeof(gm::BoxClock) * cap), "hipMalloc")) != GM_OK) return bail(rc);
+  if ((rc = bhip(b, hipMalloc((void**)&b->dslots, sizeof(int) * cap), "hipMalloc")) != GM_OK) return bail(rc);
   if ((rc = bhip(b, hipMalloc((void**)&b->derr, sizeof(int)), "hipMalloc")) != GM_OK) return bail(rc);
   if ((rc = bhip(b, hipHostMalloc((void**)&b->hclk, sizeof(gm::BoxClock) * cap, hipHostMallocDefault),
                  "hipHostMalloc")) != GM_OK)
@@ -368,6 +424,7 @@ int gm_batch_create(const gm_batch_params* bp, gm_batch** out) {
       GM_OK)
     return bail(rc);
   if ((rc = bhip(b, hipMemset(b->derr, 0, sizeof(int)), "hipMemset")) != GM_OK) return bail(rc);
+  *b->herr = 0;
   *out = b;
   return GM_OK;
 }
@@ -376,10 +433,15 @@ int gm_batch_destroy(gm_batch* b) {
   if (!b) return GM_ERR_ARG;
   (void)hipSetDevice(b->bp.device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->xfer) (void)hipStreamSynchronize(b->xfer);
   drop_graph(b);
+  free_zombies(b);
   for (auto& s : b->slots)
     if (s.used) free_slot(s);
   if (b->dboxes) (void)hipFree(b->dboxes);
+  if (b->dslots) (void)hipFree(b->dslots);
+  if (b->ev_launch) (void)hipEventDestroy(b->ev_launch);
+  if (b->xfer) (void)hipStreamDestroy(b->xfer);
   if (b->dprefix) (void)hipFree(b->dprefix);
   if (b->dclk) (void)hipFree(b->dclk);
   if (b->derr) (void)hipFree(b->derr);
@@ -434,7 +496,10 @@ int gm_batch_add(gm_batch* b, const gm_params* p, const double* u, double cfl, d
   const int64_t need = box_bytes_of(*p, b->esize);
   if (b->used + need > b->budget) return bfail(b, GM_ERR_BUSY, "memory budget exhausted");
   BHIP(b, hipSetDevice(b->bp.device));
-  BHIP(b, hipStreamSynchronize(b->stream));
+  // Admission runs on the transfer stream, beside any launch in flight (the
+  // new slot is in none of their tables): allocation, upload, clock and the
+  // first CFL minimum; it has completed when this call returns.
+  hipStream_t x = b->xfer;
   Slot s;
   s.p = *p;
   s.p.device = b->bp.device;
@@ -448,16 +513,16 @@ int gm_batch_add(gm_batch* b, const gm_params* p, const double* u, double cfl, d
       free_slot(s);
       return bfail(b, GM_ERR_OOM, "device allocation failed");
     }
-    (void)hipMemsetAsync(*a.ptr, 0, a.n, b->stream);
+    (void)hipMemsetAsync(*a.ptr, 0, a.n, x);
   }
   s.bytes = need;
   // upload: host AoS doubles -> staging (partials block) -> SoA state
-  BHIP(b, hipMemcpyAsync(s.W, u, (size_t)g.ncell * 8 * sizeof(double), hipMemcpyHostToDevice, b->stream));
-  if (b->esize == 8) gm::launch_aos_to_soa(b->stream, g.ncell, (const double*)s.W, (double*)s.U);
-  else gm::launch_aos_to_soa(b->stream, g.ncell, (const double*)s.W, (float*)s.U);
+  BHIP(b, hipMemcpyAsync(s.W, u, (size_t)g.ncell * 8 * sizeof(double), hipMemcpyHostToDevice, x));
+  if (b->esize == 8) gm::launch_aos_to_soa(x, g.ncell, (const double*)s.W, (double*)s.U);
+  else gm::launch_aos_to_soa(x, g.ncell, (const double*)s.W, (float*)s.U);
   BHIP(b, hipGetLastError());
   gm::BoxClock c{};
-  c.dtmin = 0;
+  c.dtmin = (unsigned long long)gm_bits_of(1e30);
   c.dt = 0.0;
   c.time = 0.0;
   c.t_end = t_end;
@@ -467,9 +532,13 @@ int gm_batch_add(gm_batch* b, const gm_params* p, const double* u, double cfl, d
   c.steps = 0;
   c.max_steps = (int)max_steps;
   c.done = 0;
-  b->hclk[slot] = c;
-  BHIP(b, hipMemcpyAsync(b->dclk + slot, b->hclk + slot, sizeof(gm::BoxClock), hipMemcpyHostToDevice, b->stream));
-  BHIP(b, hipStreamSynchronize(b->stream));
+  BHIP(b, hipMemcpyAsync(b->dclk + slot, &c, sizeof(gm::BoxClock), hipMemcpyHostToDevice, x));
+  // the first step's CFL minimum (get_timestep over the uploaded state),
+  // which the batch's field update folds in for every later step
+  if (b->esize == 8) gm::launch_cfl(x, box_view<double>(s, slot), &b->dclk[slot].dtmin, b->derr, gm::Shape{256, 16});
+  else gm::launch_cfl(x, box_view<float>(s, slot), &b->dclk[slot].dtmin, b->derr, gm::Shape{256, 16});
+  BHIP(b, hipGetLastError());
+  BHIP(b, hipStreamSynchronize(x));
   s.used = true;
   s.clock = c;
   b->slots[slot] = s;
@@ -484,13 +553,22 @@ int gm_batch_remove(gm_batch* b, int32_t id) {
   if (!b) return GM_ERR_ARG;
   if (!valid_id(b, id)) return bfail(b, GM_ERR_ARG, "no such box");
   BHIP(b, hipSetDevice(b->bp.device));
-  BHIP(b, hipStreamSynchronize(b->stream));
+  int rc;
+  // a box that a launch in flight steps is released once it finishes
+  if (in_launch(b, id) && (rc = wait_launch(b)) != GM_OK) return rc;
   gm::BoxClock c{};
   c.done = 1;
-  b->hclk[id] = c;
-  BHIP(b, hipMemcpy(b->dclk + id, b->hclk + id, sizeof(gm::BoxClock), hipMemcpyHostToDevice));
+  BHIP(b, hipMemcpyAsync(b->dclk + id, &c, sizeof(gm::BoxClock), hipMemcpyHostToDevice, b->xfer));
+  BHIP(b, hipStreamSynchronize(b->xfer));
   b->used -= b->slots[id].bytes;
-  free_slot(b->slots[id]);
+  Slot& sl = b->slots[id];
+  if (b->in_flight) {  // freeing could wait for the launches in flight
+    for (void* q : {sl.U, sl.Us, sl.W, sl.E})
+      if (q) b->zombies.push_back(q);
+    sl = Slot();
+  } else {
+    free_slot(sl);
+  }
   b->tables_dirty = true;
   b->graph_dirty = true;
   return GM_OK;
@@ -500,10 +578,14 @@ int gm_batch_advance(gm_batch* b, int32_t nlaunch) {
   if (!b) return GM_ERR_ARG;
   if (nlaunch < 1) return bfail(b, GM_ERR_ARG, "nlaunch must be >= 1");
   BHIP(b, hipSetDevice(b->bp.device));
-  int active = 0;
-  for (auto& s : b->slots) active += s.used ? 1 : 0;
-  if (active == 0) return GM_OK;
   int rc;
+  // new tables or a new graph need the launches in flight to have finished
+  // (their nodes read the tables)
+  if ((b->tables_dirty || b->graph_dirty || (b->bp.autotune && !b->tuned)) && (rc = wait_launch(b)) != GM_OK)
+    return rc;
+  int active = 0;
+  for (auto& s : b->slots) active += (s.used && !s.clock.done) ? 1 : 0;
+  if (active == 0) return GM_OK;
   if (b->bp.autotune && !b->tuned) {
     if ((rc = autotune(b)) != GM_OK) return rc;
   }
@@ -515,13 +597,36 @@ int gm_batch_advance(gm_batch* b, int32_t nlaunch) {
     if ((rc = capture_graph(b)) != GM_OK) return rc;
   }
   for (int i = 0; i < nlaunch; i++) BHIP(b, hipGraphLaunch(b->exec, b->stream));
+  // the clocks and the error flag come back behind the launches
+  BHIP(b, hipMemcpyAsync(b->hclk, b->dclk, sizeof(gm::BoxClock) * b->bp.max_boxes, hipMemcpyDeviceToHost,
+                         b->stream));
+  BHIP(b, hipMemcpyAsync(b->herr, b->derr, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  BHIP(b, hipEventRecord(b->ev_launch, b->stream));
+  b->in_flight = true;
+  b->launched = b->order;
   return GM_OK;
 }
 
 int gm_batch_sync(gm_batch* b) {
   if (!b) return GM_ERR_ARG;
   BHIP(b, hipSetDevice(b->bp.device));
-  return read_back(b);
+  BHIP(b, hipStreamSynchronize(b->stream));
+  BHIP(b, hipStreamSynchronize(b->xfer));
+  return finish_launch(b);
+}
+
+int gm_batch_query(gm_batch* b, int32_t* idle) {
+  if (!b || !idle) return GM_ERR_ARG;
+  BHIP(b, hipSetDevice(b->bp.device));
+  *idle = 1;
+  if (!b->in_flight) return GM_OK;
+  const hipError_t e = hipEventQuery(b->ev_launch);
+  if (e == hipErrorNotReady) {
+    *idle = 0;
+    return GM_OK;
+  }
+  BHIP(b, e);
+  return finish_launch(b);
 }
 
 int gm_batch_clock(gm_batch* b, int32_t id, double* time, int64_t* steps, int32_t* done) {
@@ -539,19 +644,23 @@ int gm_batch_download_u(gm_batch* b, int32_t id, double* u) {
   if (!valid_id(b, id)) return bfail(b, GM_ERR_ARG, "no such box");
   if (!u) return bfail(b, GM_ERR_ARG, "u is NULL");
   BHIP(b, hipSetDevice(b->bp.device));
-  BHIP(b, hipStreamSynchronize(b->stream));
+  int rc;
+  // a box that a launch in flight steps is read once it finishes; any other
+  // box is read on the transfer stream, beside the launches
+  if (in_launch(b, id) && (rc = wait_launch(b)) != GM_OK) return rc;
   const Slot& s = b->slots[id];
   const gm::Geom g = geom_of(s.p);
+  hipStream_t x = b->xfer;
   if (b->esize == 8) {
-    ghost_fill_t<double>(b, s);
-    gm::launch_soa_to_aos(b->stream, g.ncell, (const double*)s.U, (double*)s.W);
+    ghost_fill_t<double>(x, s);
+    gm::launch_soa_to_aos(x, g.ncell, (const double*)s.U, (double*)s.W);
   } else {
-    ghost_fill_t<float>(b, s);
-    gm::launch_soa_to_aos(b->stream, g.ncell, (const float*)s.U, (double*)s.W);
+    ghost_fill_t<float>(x, s);
+    gm::launch_soa_to_aos(x, g.ncell, (const float*)s.U, (double*)s.W);
   }
   BHIP(b, hipGetLastError());
-  BHIP(b, hipMemcpyAsync(u, s.W, (size_t)g.ncell * 8 * sizeof(double), hipMemcpyDeviceToHost, b->stream));
-  BHIP(b, hipStreamSynchronize(b->stream));
+  BHIP(b, hipMemcpyAsync(u, s.W, (size_t)g.ncell * 8 * sizeof(double), hipMemcpyDeviceToHost, x));
+  BHIP(b, hipStreamSynchronize(x));
   return GM_OK;
 }
 

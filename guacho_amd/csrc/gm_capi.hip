@@ -606,7 +606,7 @@ void noalloc_tstep_t(hipStream_t s, const gm_params& p, S* u, unsigned char* scr
   const int solver = p.riemann_solver, fcd = p.enable_flux_cd ? 1 : 0;
   int* err = &sc->err;
   const double dtm = dt / 2.0;  // hydro_solver.f90:152
-  const int kslab = b.srcin ? 0 : gm::slab_kchunk(b.g);
+  const int kslab = (b.srcin || sizeof(S) == 4) ? 0 : gm::slab_kchunk(b.g);  // as gm_create chooses
   auto stage = [&](double h, int st) {
     if (kslab) {
       gm::launch_slab(s, b, h, st, solver, fcd, err, kslab);
@@ -657,9 +657,12 @@ int gm_create(const gm_params* params, gm_ctx** out) {
   c->ph.limiter = params->slope_limiter;
   c->ph.eos = GM_EOS_ADIABATIC;
   c->ph.tempsc = 1.0;
-  // slab length of the 2.5-D stage kernel: from the grid, or
-  // GUACHO_AMD_SLAB_KCHUNK (planes per slab; 0 selects the two-kernel stage)
-  c->slab_kchunk = gm::slab_kchunk(g);
+  // The stage kernels: FP64 runs the 2.5-D slab kernel (4% faster per
+  // step than the two-kernel stage at 256^3, same box), the mixed form the
+  // two-kernel stage (its FP32 xy kernel keeps four waves per SIMD; the
+  // slab's window holds it to two: 11% slower).  GUACHO_AMD_SLAB_KCHUNK
+  // (planes per slab; 0 selects the two-kernel stage) overrides either.
+  c->slab_kchunk = params->precision == GM_PRECISION_FP64 ? gm::slab_kchunk(g) : 0;
   if (const char* e = std::getenv("GUACHO_AMD_SLAB_KCHUNK")) {
     const long v = std::strtol(e, nullptr, 10);
     if (v >= 0 && v <= (1 << 20)) c->slab_kchunk = (int)v;

@@ -1718,6 +1718,23 @@ __global__ void __launch_bounds__(256) kb_bfield(const Box<S>* __restrict__ boxe
   bfield_body(b, lb, stage_dt(clk, b.slot, stage), stage, kchunk);
 }
 
+// the corrector's field update of a batch with each box's next CFL
+// minimum folded in (k_bfield_cfl per box): into the box's clock, which
+// kb_clock_dt reset to 1e30 when it took the step's dt
+template <typename S>
+__global__ void __launch_bounds__(256) kb_bfield_cfl(const Box<S>* __restrict__ boxes,
+                                                     const unsigned* __restrict__ prefix, int nbox,
+                                                     BoxClock* __restrict__ clk, int kchunk, int* err) {
+  int bx;
+  unsigned lb;
+  batch_locate(prefix, nbox, xcd_logical_block(), bx, lb);
+  const Box<S>& b = boxes[bx];
+  double dtp = 1e30;
+  bool bad = false;
+  bfield_body<S, 1, 8>(b, lb, stage_dt(clk, b.slot, 2), 2, kchunk, &dtp, &bad);
+  cfl_reduce(dtp, bad, &clk[b.slot].dtmin, err);
+}
+
 template <typename S>
 __global__ void __launch_bounds__(256) kb_sources(const Box<S>* __restrict__ boxes,
                                                   const unsigned* __restrict__ prefix, int nbox,
@@ -1739,10 +1756,11 @@ __global__ void __launch_bounds__(256) kb_cfl(const Box<S>* __restrict__ boxes, 
 }
 
 
-// Start of a batched step: reset every box's CFL minimum.
-__global__ void kb_clock_begin(BoxClock* clk, int nbox) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < nbox) clk[b].dtmin = (unsigned long long)__double_as_longlong(1e30);
+// Start of a batched step: reset the CFL minimum of every box in the
+// launch (slots[0..nbox), the packed table's clock slots).
+__global__ void kb_clock_begin(BoxClock* clk, const int* __restrict__ slots, int nbox) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nbox) clk[slots[i]].dtmin = (unsigned long long)__double_as_longlong(1e30);
 }
 
 // get_timestep's ramp and clamp on the device (src/hydro_core.f90:677-694):
@@ -1750,9 +1768,10 @@ __global__ void kb_clock_begin(BoxClock* clk, int nbox) {
 // clamped to land on t_end, which completes the box.  A box that is done,
 // or a timing-only launch (tuning != 0), steps with dt = 0: the fluxes are
 // computed in full and the state keeps its values.
-__global__ void kb_clock_dt(BoxClock* clk, int nbox, int tuning) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbox) return;
+__global__ void kb_clock_dt(BoxClock* clk, const int* __restrict__ slots, int nbox, int tuning) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nbox) return;
+  const int b = slots[i];
   BoxClock c = clk[b];
   if (c.done || tuning) {
     clk[b].dt = 0.0;
@@ -1771,6 +1790,7 @@ __global__ void kb_clock_dt(BoxClock* clk, int nbox, int tuning) {
   c.time = c.time + dt;
   c.it += 1;
   c.steps += 1;
+  c.dtmin = (unsigned long long)__double_as_longlong(1e30);  // for the corrector's folded minimum
   if (finish || (c.max_steps > 0 && c.steps >= c.max_steps)) c.done = 1;
   clk[b] = c;
 }
@@ -2081,17 +2101,23 @@ void launch_bfield_batch(hipStream_t s, const Box<S>* boxes, const unsigned* pre
 }
 
 template <typename S>
+void launch_bfield_cfl_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
+                             BoxClock* clk, int* err, const Shape& sh) {
+  hipLaunchKernelGGL(kb_bfield_cfl<S>, dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, clk, sh.kchunk, err);
+}
+
+template <typename S>
 void launch_cfl_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
                       BoxClock* clk, int* err, const Shape& sh) {
   hipLaunchKernelGGL(kb_cfl<S>, dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, sh.kchunk, clk, err);
 }
 
-void launch_clock_begin(hipStream_t s, BoxClock* clk, int nbox) {
-  hipLaunchKernelGGL(kb_clock_begin, dim3(ceil_div(nbox, 64)), dim3(64), 0, s, clk, nbox);
+void launch_clock_begin(hipStream_t s, BoxClock* clk, const int* slots, int nbox) {
+  hipLaunchKernelGGL(kb_clock_begin, dim3(ceil_div(nbox, 64)), dim3(64), 0, s, clk, slots, nbox);
 }
 
-void launch_clock_dt(hipStream_t s, BoxClock* clk, int nbox, int tuning) {
-  hipLaunchKernelGGL(kb_clock_dt, dim3(ceil_div(nbox, 64)), dim3(64), 0, s, clk, nbox, tuning);
+void launch_clock_dt(hipStream_t s, BoxClock* clk, const int* slots, int nbox, int tuning) {
+  hipLaunchKernelGGL(kb_clock_dt, dim3(ceil_div(nbox, 64)), dim3(64), 0, s, clk, slots, nbox, tuning);
 }
 
 template <typename S>
@@ -2143,6 +2169,8 @@ void launch_divb(hipStream_t s, const Geom& g, const S* U, double dx, double dy,
                                   int, int, int, int*, const Shape&);                                             \
   template void launch_bfield_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned,                \
                                        const BoxClock*, int, const Shape&);                                       \
+  template void launch_bfield_cfl_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, BoxClock*,  \
+                                           int*, const Shape&);                                                   \
   template void launch_cfl_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, BoxClock*, int*,  \
                                     const Shape&);                                                                \
   template void launch_ghost<S>(hipStream_t, const Geom&, S*, int, int, int, int);                               \

@@ -162,11 +162,18 @@ void launch_bfield_batch(hipStream_t s, const Box<S>* boxes, const unsigned* pre
 template <typename S>
 void launch_cfl_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
                       BoxClock* clk, int* err, const Shape& sh);
+// the corrector's field update with each box's next CFL minimum folded into
+// its clock (valid when no source pass follows the field update)
+template <typename S>
+void launch_bfield_cfl_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
+                             BoxClock* clk, int* err, const Shape& sh);
 template <typename S>
 void launch_sources_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
                           const BoxClock* clk, int stage, const Shape& sh);
-void launch_clock_begin(hipStream_t s, BoxClock* clk, int nbox);
-void launch_clock_dt(hipStream_t s, BoxClock* clk, int nbox, int tuning);
+// clock kernels over the boxes of a launch: slots[0..nbox) (device) are
+// their clock slots
+void launch_clock_begin(hipStream_t s, BoxClock* clk, const int* slots, int nbox);
+void launch_clock_dt(hipStream_t s, BoxClock* clk, const int* slots, int nbox, int tuning);
 
 // Layout conversion, ghost fill and diagnostics of one box's state.
 // layers 1: one face of boundaryI (src/boundaries.f90:44-247); layers 2:

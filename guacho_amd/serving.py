@@ -1,10 +1,11 @@
 """Continuous-batching server for MHD solves over the batch ABI (gm_batch_*).
 
 Requests arrive over time (Poisson, seeded).  A request is admitted as soon
-as it has arrived and its box fits the device-memory budget; admitted boxes
-stay resident on the GPU and advance together, `steps_per_launch` steps per
-graph launch, each with its own CFL dt.  Between launches completed boxes
-are downloaded and released, which frees memory for waiting requests.
+as it has arrived and its box fits the device-memory budget, also while a
+launch runs; admitted boxes stay resident on the GPU and advance together,
+`steps_per_launch` steps per graph launch, each with its own CFL dt.
+Completed boxes are downloaded and released while the next launch runs,
+which frees memory for waiting requests.
 Reported: time-to-solution per request (completion minus arrival),
 p50/p99, and aggregate zone-updates/s.  The baseline serves the same
 requests one at a time with the plain FP64 single-box path.
@@ -81,8 +82,15 @@ def make_stream(n: int, rate: float, seed: int, precision: int = _capi.GM_PRECIS
 
 def serve(requests: List[Request], steps_per_launch: int = 4, memory_budget: int = 0, keep_results: bool = False,
           max_boxes: int = 256, clock: Callable[[], float] = time.perf_counter,
-          batch_factory: Optional[Callable[..., object]] = None) -> Dict[str, float]:
+          batch_factory: Optional[Callable[..., object]] = None, poll_s: float = 2e-4) -> Dict[str, float]:
     """Serve the requests with continuous batching (mixed precision).
+
+    The host never waits for the GPU while there is work to queue: while a
+    launch runs it admits arrived requests (their uploads run on the batch's
+    transfer stream beside the launch) and polls the launch without
+    blocking; once the launch has finished it queues the next one right
+    away, then downloads and releases the boxes that completed (their
+    buffers are out of the new launch, so the download overlaps it).
 
     Admission: every arrived request whose box fits the free memory and a
     free slot is admitted, in arrival order, and a request that does not fit
@@ -97,9 +105,12 @@ def serve(requests: List[Request], steps_per_launch: int = 4, memory_budget: int
               memory_budget=memory_budget) as B:
         budget = B.free_bytes()
         waiting: List[Request] = []
+        in_flight = False
         t0 = clock()
         qi = 0
-        while qi < len(pending) or waiting or active:
+
+        def admit():
+            nonlocal qi, waiting
             now = clock() - t0
             while qi < len(pending) and pending[qi].arrival <= now:
                 r = pending[qi]
@@ -109,7 +120,6 @@ def serve(requests: List[Request], steps_per_launch: int = 4, memory_budget: int
                     r.finished = now
                     continue
                 waiting.append(r)
-            # admit every waiting request that fits, oldest first
             still: List[Request] = []
             for r in waiting:
                 if len(active) >= max_boxes or B.box_bytes(r.params) > B.free_bytes():
@@ -125,24 +135,31 @@ def serve(requests: List[Request], steps_per_launch: int = 4, memory_budget: int
                 r.admitted = clock() - t0
                 active[box] = r
             waiting = still
-            if not active:
-                if waiting:
+
+        while qi < len(pending) or waiting or active:
+            admit()
+            if in_flight:
+                if not B.query():
+                    time.sleep(poll_s)
+                    continue
+                in_flight = False
+            done = [box for box in active if B.clock(box)[2]]
+            if len(done) < len(active):
+                B.advance(1)              # the boxes still running, without the completed ones
+                in_flight = True
+            for box in done:
+                r = active.pop(box)
+                r.steps = B.clock(box)[1]
+                u = B.download_u(box)
+                if keep_results:
+                    r.result = u
+                B.remove(box)
+                r.finished = clock() - t0
+            if not active and not in_flight:
+                if waiting and not done:
                     raise RuntimeError("waiting requests fit the budget but none can be admitted with no box active")
-                if qi < len(pending):
+                if not waiting and qi < len(pending):
                     time.sleep(max(0.0, pending[qi].arrival - (clock() - t0)))
-                continue
-            B.advance(1)
-            B.sync()
-            for box in list(active):
-                _, steps, done = B.clock(box)
-                if done:
-                    r = active.pop(box)
-                    r.steps = steps
-                    u = B.download_u(box)
-                    if keep_results:
-                        r.result = u
-                    B.remove(box)
-                    r.finished = clock() - t0
         makespan = clock() - t0
     return summarize(requests, makespan)
 

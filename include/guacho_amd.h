@@ -312,19 +312,31 @@ int gm_batch_free_bytes(const gm_batch *batch, int64_t *bytes);
  * starts its clock at time 0, currentIteration 1.  It completes when its
  * dt clamps to t_end (the dump clamp of hydro_core.f90:691-694) or after
  * max_steps steps (0: no limit; at most 2^31 - 1, else GM_ERR_ARG).
- * GM_ERR_BUSY when it does not fit. */
+ * GM_ERR_BUSY when it does not fit.  Runs beside launches in flight (on a
+ * stream of its own) and has finished when it returns; the box joins the
+ * next gm_batch_advance. */
 int gm_batch_add(gm_batch *batch, const gm_params *box, const double *u, double cfl, double t_end,
                  int64_t max_steps, int32_t *box_id);
-/* Release a box and its memory. */
+/* Release a box and its memory (waits only if a launch in flight steps
+ * the box; the memory of a box no launch in flight touches is released
+ * once those launches finish). */
 int gm_batch_remove(gm_batch *batch, int32_t box_id);
-/* Queue nlaunch graph launches of steps_per_launch steps for every active
- * box (completed boxes step with dt = 0).  Asynchronous. */
+/* Queue nlaunch graph launches of steps_per_launch steps for every box that
+ * has not completed (as of the last sync or idle query; a box completing
+ * within the launches steps with dt = 0 after that).  Asynchronous; waits
+ * for launches in flight only when the set of boxes changed. */
 int gm_batch_advance(gm_batch *batch, int32_t nlaunch);
 /* Wait; report deferred device errors (GM_ERR_NONFINITE). */
 int gm_batch_sync(gm_batch *batch);
-/* Clock of a box after the last sync: time, steps taken, completion. */
+/* Without waiting: *idle = 1 when the queued launches have finished (the
+ * clocks are then those of the last launch, as after gm_batch_sync, and a
+ * deferred device error is reported), 0 while they run. */
+int gm_batch_query(gm_batch *batch, int32_t *idle);
+/* Clock of a box after the last sync or idle query: time, steps taken,
+ * completion. */
 int gm_batch_clock(gm_batch *batch, int32_t box_id, double *time, int64_t *steps, int32_t *done);
-/* The box's conserved array after boundaryI's ghost fill (synchronizes). */
+/* The box's conserved array after boundaryI's ghost fill.  Waits only if a
+ * launch in flight steps the box; otherwise runs beside the launches. */
 int gm_batch_download_u(gm_batch *batch, int32_t box_id, double *u);
 /* Launch shape in use and the autotuner's measured step time (0 if not tuned). */
 int gm_batch_tuning(const gm_batch *batch, int32_t *block, int32_t *kchunk, double *ms_per_step);

@@ -11,7 +11,8 @@ from guacho_amd import _capi, serving
 
 class FakeBatch:
     """Boxes of `cells * 10` bytes; every advance() takes one step of each
-    active box; a box is done after its max_steps."""
+    box not yet done, visible from the next query(); a box is done after
+    its max_steps."""
 
     def __init__(self, precision, max_boxes, steps_per_launch, memory_budget):
         self.budget = memory_budget
@@ -47,11 +48,20 @@ class FakeBatch:
         return bid
 
     def advance(self, n):
-        for b in self.boxes.values():
-            b[1] += 1
+        # like the real batch: the launch runs later (here: at the next
+        # query), boxes already completed are not stepped, and clock()
+        # reports the state as of the last query
+        self.pending = [bid for bid, b in self.boxes.items() if b[1] < b[2]]
 
     def sync(self):
-        pass
+        self.query()
+
+    def query(self):
+        for bid in getattr(self, "pending", []):
+            if bid in self.boxes:
+                self.boxes[bid][1] += 1
+        self.pending = []
+        return True
 
     def clock(self, bid):
         need, steps, mx = self.boxes[bid]
