@@ -12,7 +12,9 @@ from typing import Dict, Optional, Sequence
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libguacho_amd.so")
+# the library; GUACHO_AMD_LIB selects another build of it (the host-sanitizer
+# build of `make -C guacho_amd asan`)
+LIB_PATH = os.environ.get("GUACHO_AMD_LIB") or os.path.join(_HERE, "libguacho_amd.so")
 
 GM_OK, GM_ERR_CONFIG, GM_ERR_HIP, GM_ERR_NONFINITE, GM_ERR_OOM, GM_ERR_ARG, GM_ERR_BUSY = 0, 1, 2, 3, 4, 5, 6
 GM_PRECISION_FP64, GM_PRECISION_MIXED = 0, 1

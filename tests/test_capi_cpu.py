@@ -85,3 +85,24 @@ def test_noalloc_refuses_oversized_grid_before_device_work():
     assert L.gm_tstep_noalloc(C.byref(p), C.c_void_p(1), 1e-3, None, None, None) == _capi.GM_ERR_CONFIG
     p = _create()
     assert L.gm_tstep_noalloc(C.byref(p), None, 1e-3, None, None, None) == _capi.GM_ERR_ARG
+
+
+def test_creation_without_a_device_fails_cleanly():
+    # valid configurations: without a GPU creation stops at the first HIP
+    # call and releases what it built (the sanitizer run covers that path);
+    # with one, it succeeds
+    L = _capi.lib()
+    h = C.c_void_p()
+    rc = L.gm_create(C.byref(_create()), C.byref(h))
+    if rc == _capi.GM_OK:
+        assert L.gm_destroy(h) == _capi.GM_OK
+    else:
+        assert rc == _capi.GM_ERR_HIP and not h.value and L.gm_last_error(None)
+    bp = _capi.GmBatchParams()
+    bp.precision, bp.riemann_solver, bp.enable_flux_cd, bp.max_boxes, bp.steps_per_launch = 1, 4, 1, 8, 2
+    hb = C.c_void_p()
+    rc = L.gm_batch_create(C.byref(bp), C.byref(hb))
+    if rc == _capi.GM_OK:
+        assert L.gm_batch_destroy(hb) == _capi.GM_OK
+    else:
+        assert rc == _capi.GM_ERR_HIP and not hb.value and L.gm_batch_last_error(None)
