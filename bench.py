@@ -280,7 +280,7 @@ def batch_section(nbox: int, steps: int, seed: int):
             ids = admit(B)
             B.advance(1)        # first use: autotune (dt = 0 steps), tables, graph capture, the run
             B.sync()
-            block, kchunk, tuned_ms = B.tuning()
+            block, kchunk, slab, tuned_ms = B.tuning()
             u_first = B.download_u(ids[0])
             for i in ids:
                 B.remove(i)
@@ -292,7 +292,7 @@ def batch_section(nbox: int, steps: int, seed: int):
             el = time.perf_counter() - t0
             assert all(B.clock(i) == (B.clock(i)[0], steps, True) for i in ids)
             same = bool(np.array_equal(B.download_u(ids[0]), u_first))
-        return el, (block, kchunk, tuned_ms), same
+        return el, (block, kchunk, slab, tuned_ms), same
 
     def sequential(prec):
         devs = []
@@ -327,7 +327,10 @@ def batch_section(nbox: int, steps: int, seed: int):
     return {"workload": f"{nbox} MHD blasts n~U{{32..128}}^3, outflow, HLLD, mixed, {steps} steps, one graph launch",
             "value": round(zones / el_m, 1), "unit": "zone-updates/s", "seconds": round(el_m, 4),
             "zone_updates": zones,
-            "tuned_shape": {"block": tune_m[0], "kchunk": tune_m[1], "ms_per_step": round(tune_m[2], 4)},
+            "tuned_shape": {"block": tune_m[0], "kchunk": tune_m[1], "slab": tune_m[2],
+                            "ms_per_step": round(tune_m[3], 4)},
+            "tuned_shape_fp64": {"block": tune_f[0], "kchunk": tune_f[1], "slab": tune_f[2],
+                                 "ms_per_step": round(tune_f[3], 4)},
             "batched_fp64_value": round(zones / el_f, 1), "batched_fp64_seconds": round(el_f, 4),
             "sequential_mixed_value": round(zones / seq_m, 1), "sequential_mixed_seconds": round(seq_m, 4),
             "sequential_fp64_value": round(zones / seq_f, 1), "sequential_fp64_seconds": round(seq_f, 4),

@@ -125,7 +125,8 @@ def lib() -> C.CDLL:
     L.gm_batch_clock.argtypes = [vp, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                  C.POINTER(C.c_int32)]
     L.gm_batch_download_u.argtypes = [vp, C.c_int32, dp]
-    L.gm_batch_tuning.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+    L.gm_batch_tuning.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_double)]
     L.gm_batch_active.argtypes = [vp, C.POINTER(C.c_int32)]
     for name in EXPORTS:
         if name not in ("gm_last_error", "gm_batch_last_error"):
@@ -377,9 +378,11 @@ class Batch:
         return u
 
     def tuning(self):
-        b, k, ms = C.c_int32(), C.c_int32(), C.c_double()
-        self._check(self.L.gm_batch_tuning(self.h, C.byref(b), C.byref(k), C.byref(ms)))
-        return b.value, k.value, ms.value
+        """(column tile width, z-pencil length, slab planes (0: two-kernel
+        stage), autotuned ms per step)"""
+        b, k, sl, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
+        self._check(self.L.gm_batch_tuning(self.h, C.byref(b), C.byref(k), C.byref(sl), C.byref(ms)))
+        return b.value, k.value, sl.value, ms.value
 
     def active(self) -> int:
         v = C.c_int32()

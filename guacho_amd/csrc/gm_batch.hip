@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -215,7 +216,7 @@ void enqueue_steps_t(gm_batch* b, int nsteps, int tuning) {
   hipStream_t s = b->stream;
   const gm::Shape& sh = b->shape;
   const int solver = b->bp.riemann_solver, fcd = b->bp.enable_flux_cd;
-  const unsigned txy = b->tiles[gm::KC_XY], tcol = b->tiles[gm::KC_COL];
+  const unsigned txy = b->tiles[gm::KC_XY], tcol = b->tiles[gm::KC_COL], tslab = b->tiles[gm::KC_SLAB];
   // With flux-CD and no source pass, each box's CFL minimum for the next
   // step is folded into the corrector's field update (its clock holds it;
   // gm_batch_add reduces a new box's first one); otherwise a CFL pass
@@ -228,8 +229,12 @@ void enqueue_steps_t(gm_batch* b, int nsteps, int tuning) {
     }
     gm::launch_clock_dt(s, b->dclk, b->dslots, n, tuning);
     for (int stage = 1; stage <= 2; stage++) {
-      gm::launch_xy_batch(s, boxes, pre(gm::KC_XY), n, txy, b->dclk, stage, solver, fcd, b->derr);
-      gm::launch_z_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, stage, solver, fcd, b->derr, sh);
+      if (sh.slab) {
+        gm::launch_slab_batch(s, boxes, pre(gm::KC_SLAB), n, tslab, b->dclk, stage, solver, fcd, b->derr, sh);
+      } else {
+        gm::launch_xy_batch(s, boxes, pre(gm::KC_XY), n, txy, b->dclk, stage, solver, fcd, b->derr);
+        gm::launch_z_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, stage, solver, fcd, b->derr, sh);
+      }
       if (fold && stage == 2) gm::launch_bfield_cfl_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, b->derr, sh);
       else if (fcd) gm::launch_bfield_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, stage, sh);
       if (b->any_sources) gm::launch_sources_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, stage, sh);
@@ -262,12 +267,15 @@ int capture_graph(gm_batch* b) {
 
 bool graph_matches(const gm_batch* b) {
   return b->exec && b->cap_nbox == (int)b->order.size() && b->cap_sources == b->any_sources &&
-         b->cap_shape.block == b->shape.block &&
+         b->cap_shape.block == b->shape.block && b->cap_shape.slab == b->shape.slab &&
          b->cap_shape.kchunk == b->shape.kchunk && std::memcmp(b->cap_tiles, b->tiles, sizeof b->tiles) == 0;
 }
 
-// Time candidate shapes on the admitted boxes; keep the fastest.
+// Time candidate shapes on the admitted boxes; keep the fastest: the stage
+// kernels (the two-kernel stage, or the 2.5-D slab kernel with 16 or 32
+// planes per slab) x the column kernels' tile width x z-pencil length.
 int autotune(gm_batch* b) {
+  static const int slabs[] = {0, 16, 32};
   static const int blocks[] = {64, 128, 256};
   static const int chunks[] = {8, 16, 32};
   hipEvent_t e0, e1;
@@ -275,9 +283,10 @@ int autotune(gm_batch* b) {
   BHIP(b, hipEventCreate(&e1));
   double best = 1e300;
   gm::Shape best_sh = b->shape;
+  for (int sl : slabs)
   for (int blk : blocks)
     for (int kc : chunks) {
-      const gm::Shape sh = {blk, kc};
+      const gm::Shape sh = {blk, kc, sl};
       b->shape = sh;
       int rc = rebuild_tables(b, sh);
       if (rc != GM_OK) return rc;
@@ -384,6 +393,12 @@ int gm_batch_create(const gm_batch_params* bp, gm_batch** out) {
   if (bp->reserved != 0 || bp->memory_budget < 0) return bfail(nullptr, GM_ERR_CONFIG, "bad reserved / budget field");
   gm_batch* b = new gm_batch();
   b->bp = *bp;
+  // stage kernels before (or without) autotuning: the two-kernel stage, or
+  // GUACHO_AMD_BATCH_SLAB planes per slab of the 2.5-D kernel
+  if (const char* e = std::getenv("GUACHO_AMD_BATCH_SLAB")) {
+    const long v = std::strtol(e, nullptr, 10);
+    if (v >= 0 && v <= (1 << 20)) b->shape.slab = (int)v;
+  }
   b->esize = bp->precision == GM_PRECISION_MIXED ? 4 : 8;
   b->slots.resize(bp->max_boxes);
   auto bail = [&](int code) {
@@ -664,10 +679,11 @@ int gm_batch_download_u(gm_batch* b, int32_t id, double* u) {
   return GM_OK;
 }
 
-int gm_batch_tuning(const gm_batch* b, int32_t* block, int32_t* kchunk, double* ms_per_step) {
+int gm_batch_tuning(const gm_batch* b, int32_t* block, int32_t* kchunk, int32_t* slab, double* ms_per_step) {
   if (!b) return GM_ERR_ARG;
   if (block) *block = b->shape.block;
   if (kchunk) *kchunk = b->shape.kchunk;
+  if (slab) *slab = b->shape.slab;
   if (ms_per_step) *ms_per_step = b->tuned ? b->tuned_ms : 0.0;
   return GM_OK;
 }

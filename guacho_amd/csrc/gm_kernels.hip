@@ -1698,6 +1698,18 @@ __global__ void __launch_bounds__(XY_THREADS) __attribute__((amdgpu_waves_per_eu
 }
 
 template <int SOLVER, int ORDER, int FLUXCD, typename S>
+__global__ void __launch_bounds__(SL_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
+kb_slab(const Box<S>* __restrict__ boxes, const unsigned* __restrict__ prefix, int nbox,
+        const BoxClock* __restrict__ clk, int kchunk, int* err) {
+  __shared__ S lds[SlabLds<ORDER, 8, S>::SIZE];
+  int bx;
+  unsigned lb;
+  batch_locate(prefix, nbox, xcd_logical_block(), bx, lb);
+  const Box<S>& b = boxes[bx];
+  slab_body<SOLVER, ORDER, FLUXCD, S, 0>(b, lb, stage_dt(clk, b.slot, ORDER), kchunk, err, lds);
+}
+
+template <int SOLVER, int ORDER, int FLUXCD, typename S>
 __global__ void __launch_bounds__(256) kb_z(const Box<S>* __restrict__ boxes, const unsigned* __restrict__ prefix,
                                             int nbox, const BoxClock* __restrict__ clk, int kchunk, int* err) {
   int bx;
@@ -1910,6 +1922,7 @@ unsigned box_tiles(const Geom& g, int kclass, const Shape& sh) {
   switch (kclass) {
     case KC_XY: return ceil_div(g.nx, TX) * ceil_div(g.ny, TY) * (unsigned)g.nz;
     case KC_COL: return ceil_div((long long)g.nx * g.ny, sh.block) * ceil_div(g.nz, sh.kchunk);
+    case KC_SLAB: return sh.slab > 0 ? slab_tiles(g, sh.slab) : 0u;
   }
   return 0;
 }
@@ -2091,6 +2104,18 @@ void launch_z_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, 
   GM_STAGE_DISPATCH(GM_CALL)
 #undef GM_CALL
 }
+
+template <typename S>
+void launch_slab_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
+                       const BoxClock* clk, int stage, int solver, int flux_cd, int* err, const Shape& sh) {
+#define GM_CALL(SV, O, F)                                                                                       \
+  {                                                                                                              \
+    hipLaunchKernelGGL((kb_slab<SV, O, F, S>), dim3(tiles), dim3(SL_THREADS), 0, s, boxes, prefix, nbox, clk,     \
+                       sh.slab, err);                                                                            \
+  }
+  GM_STAGE_DISPATCH(GM_CALL)
+#undef GM_CALL
+}
 #undef GM_STAGE_OF
 #undef GM_STAGE_DISPATCH
 
@@ -2169,6 +2194,8 @@ void launch_divb(hipStream_t s, const Geom& g, const S* U, double dx, double dy,
                                   int, int, int, int*, const Shape&);                                             \
   template void launch_bfield_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned,                \
                                        const BoxClock*, int, const Shape&);                                       \
+  template void launch_slab_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, const BoxClock*,   \
+                                     int, int, int, int*, const Shape&);                                          \
   template void launch_bfield_cfl_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, BoxClock*,  \
                                            int*, const Shape&);                                                   \
   template void launch_cfl_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, BoxClock*, int*,  \

@@ -78,15 +78,16 @@ struct BoxClock {
 struct Shape {
   int block;
   int kchunk;
+  int slab;  // planes per slab of the 2.5-D stage kernel; 0: the two-kernel stage
 };
 
 // Validates a subdomain configuration; GM_OK or a GM_ERR_* code with msg.
 int check_params(const gm_params* p, std::string& msg);
 
 // Tile classes of one step, for per-class tile tables: 32 x 16 plane tiles
-// of the xy-face kernel, and column pencils of the z-face, field-update and
-// CFL kernels.
-enum KClass { KC_XY = 0, KC_COL, KC_COUNT };
+// of the xy-face kernel, column pencils of the z-face, field-update and CFL
+// kernels, and 32 x 8 column tiles of the slab kernel (Shape::slab planes).
+enum KClass { KC_XY = 0, KC_COL, KC_SLAB, KC_COUNT };
 
 // Tiles (workgroups) one box needs for a kernel class under a shape.
 unsigned box_tiles(const Geom& g, int kclass, const Shape& sh);
@@ -156,6 +157,10 @@ void launch_xy_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix,
 template <typename S>
 void launch_z_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
                     const BoxClock* clk, int stage, int solver, int flux_cd, int* err, const Shape& sh);
+// the 2.5-D stage kernel over a batch (tiles of class KC_SLAB, sh.slab planes)
+template <typename S>
+void launch_slab_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
+                       const BoxClock* clk, int stage, int solver, int flux_cd, int* err, const Shape& sh);
 template <typename S>
 void launch_bfield_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
                          const BoxClock* clk, int stage, const Shape& sh);

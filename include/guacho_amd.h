@@ -294,7 +294,7 @@ typedef struct gm_batch_params {
   int32_t device;
   int32_t max_boxes;        /* capacity of the box table (<= 4096) */
   int32_t steps_per_launch; /* steps captured in one graph launch (>= 1) */
-  int32_t autotune;         /* 1: time tile width x z-pencil candidates on the first advance */
+  int32_t autotune;         /* 1: time stage-kernel x tile width x z-pencil candidates on the first advance */
   int32_t reserved;         /* must be 0 */
   int64_t memory_budget;    /* device bytes the boxes may hold; 0: 80% of free memory */
 } gm_batch_params;
@@ -338,8 +338,10 @@ int gm_batch_clock(gm_batch *batch, int32_t box_id, double *time, int64_t *steps
 /* The box's conserved array after boundaryI's ghost fill.  Waits only if a
  * launch in flight steps the box; otherwise runs beside the launches. */
 int gm_batch_download_u(gm_batch *batch, int32_t box_id, double *u);
-/* Launch shape in use and the autotuner's measured step time (0 if not tuned). */
-int gm_batch_tuning(const gm_batch *batch, int32_t *block, int32_t *kchunk, double *ms_per_step);
+/* Launch shape in use -- the column kernels' tile width and z-pencil length,
+ * the slab kernel's planes per slab (0: the two-kernel stage) -- and the
+ * autotuner's measured step time (0 if not tuned). */
+int gm_batch_tuning(const gm_batch *batch, int32_t *block, int32_t *kchunk, int32_t *slab, double *ms_per_step);
 /* Number of boxes in the batch. */
 int gm_batch_active(const gm_batch *batch, int32_t *n);
 

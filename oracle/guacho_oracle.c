@@ -955,7 +955,7 @@ int orc_fluxes(orc_state *s, int choice) {
   const orc_params *p = &s->p;
   const int nx = p->nx, ny = p->ny, nz = p->nz;
   int bad = 0;
-#pragma omp parallel for schedule(static) reduction(| : bad)
+#pragma omp parallel for collapse(2) schedule(static) reduction(| : bad)
   for (int k = 0; k <= nz; k++)
     for (int j = 0; j <= ny; j++)
       for (int i = 0; i <= nx; i++) {
@@ -1054,7 +1054,7 @@ void orc_get_efield(orc_state *s) {
   const orc_params *p = &s->p;
   const int nx = p->nx, ny = p->ny, nz = p->nz;
   const double *f = s->f, *g = s->g, *h = s->h;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx; i++) {
@@ -1077,7 +1077,7 @@ void orc_step(orc_state *s, double dt) {
   const double *u = s->u, *f = s->f, *g = s->g, *h = s->h;
   double *up = s->up;
   if (p->flux_cd) orc_get_efield(s);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx; i++) {
@@ -1169,7 +1169,7 @@ void orc_viscous_copy(orc_state *s) {
   const double eta = p->eta;
   const double *up = s->up;
   double *u = s->u;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx; i++)
@@ -1205,7 +1205,7 @@ double orc_cfl_dtp(const orc_state *s) {
   const orc_params *p = &s->p;
   const int nx = p->nx, ny = p->ny, nz = p->nz;
   double dtp = 1e30;
-#pragma omp parallel for schedule(static) reduction(min : dtp)
+#pragma omp parallel for collapse(2) schedule(static) reduction(min : dtp)
   for (int k = 1; k <= nz; k++)
     for (int j = 1; j <= ny; j++)
       for (int i = 1; i <= nx; i++) {

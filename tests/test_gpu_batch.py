@@ -138,9 +138,9 @@ def test_autotune_keeps_results_and_picks_a_shape():
         ids = [B.add(p, u, max_steps=6) for p, u in bx]
         B.advance(6)
         B.sync()
-        block, kchunk, ms = B.tuning()
+        block, kchunk, slab, ms = B.tuning()
         got = [B.download_u(i) for i in ids]
-    assert block in (64, 128, 256) and kchunk in (8, 16, 32) and ms > 0
+    assert block in (64, 128, 256) and kchunk in (8, 16, 32) and slab in (0, 16, 32) and ms > 0
     for (p, u), g in zip(bx, got):
         np.testing.assert_array_equal(g, single(p, u, nsteps=6)[0])
 
@@ -166,3 +166,20 @@ def test_budget_and_removal():
         B.sync()
         assert B.clock(b)[2]
         assert B.active() == 1
+
+
+@pytest.mark.parametrize("prec", [_capi.GM_PRECISION_FP64, _capi.GM_PRECISION_MIXED], ids=["fp64", "mixed"])
+def test_batch_slab_kernel_equals_single_box_path(prec, monkeypatch):
+    """The batch with the 2.5-D stage kernel (kb_slab, 8 planes per slab:
+    boxes of one to three slabs) equals the single-box path bit for bit."""
+    say(f"case: batch with the slab stage kernel vs single path, precision {prec}, 5 steps")
+    monkeypatch.setenv("GUACHO_AMD_BATCH_SLAB", "8")
+    bx = boxes(prec)
+    with _capi.Batch(precision=prec, max_boxes=8, steps_per_launch=5) as B:
+        ids = [B.add(p, u, max_steps=5) for p, u in bx]
+        assert B.tuning()[2] == 8
+        B.advance(1)
+        B.sync()
+        got = [B.download_u(i) for i in ids]
+    for (p, u), g in zip(bx, got):
+        np.testing.assert_array_equal(g, single(p, u, nsteps=5)[0])
