@@ -229,7 +229,7 @@ void enqueue_steps_t(gm_batch* b, int nsteps, int tuning) {
     }
     gm::launch_clock_dt(s, b->dclk, b->dslots, n, tuning);
     for (int stage = 1; stage <= 2; stage++) {
-      if (sh.slab) {
+      if (sh.slab && stage == 1) {  // the slab kernel for the predictor, as single boxes run it
         gm::launch_slab_batch(s, boxes, pre(gm::KC_SLAB), n, tslab, b->dclk, stage, solver, fcd, b->derr, sh);
       } else {
         gm::launch_xy_batch(s, boxes, pre(gm::KC_XY), n, txy, b->dclk, stage, solver, fcd, b->derr);
@@ -271,9 +271,10 @@ bool graph_matches(const gm_batch* b) {
          b->cap_shape.kchunk == b->shape.kchunk && std::memcmp(b->cap_tiles, b->tiles, sizeof b->tiles) == 0;
 }
 
-// Time candidate shapes on the admitted boxes; keep the fastest: the stage
-// kernels (the two-kernel stage, or the 2.5-D slab kernel with 16 or 32
-// planes per slab) x the column kernels' tile width x z-pencil length.
+// Time candidate shapes on the admitted boxes; keep the fastest: the
+// predictor's stage kernel (the two-kernel stage, or the 2.5-D slab kernel
+// with 16 or 32 planes per slab) x the column kernels' tile width x
+// z-pencil length.
 int autotune(gm_batch* b) {
   static const int slabs[] = {0, 16, 32};
   static const int blocks[] = {64, 128, 256};

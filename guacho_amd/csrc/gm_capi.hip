@@ -90,9 +90,10 @@ struct gm_ctx {
   gm::Conduction cond{};                // kind 0: no thermal conduction
   int tc_substeps = 0;                  // sub-steps of the last step's conduction
   gm::Shape shape = {256, 16};  // tile width, z-pencil length
-  // planes per slab of the 2.5-D stage kernel; 0: the two-kernel stage
-  // (xy plane tiles, then z pencils through W)
-  int slab_kchunk = 0;
+  // planes per slab of the 2.5-D stage kernel, per stage (predictor,
+  // corrector); 0: the two-kernel stage (xy plane tiles, then z pencils
+  // through W)
+  int slab_kchunk[2] = {0, 0};
   int timing = 0;
   struct Rec { int cls; hipEvent_t a, b; };
   std::vector<Rec> recs;
@@ -104,6 +105,38 @@ struct gm_ctx {
 };
 
 namespace {
+
+// The stage kernels of a step, per stage: planes per slab of the 2.5-D
+// kernel, or 0 for the two-kernel stage.  The predictor runs as a slab
+// kernel and the corrector as the two-kernel stage, in both precisions:
+// at 256^3, same box, the first-order slab predictor takes 1.52-1.57 ms
+// against 1.72 for k_xy + k_z (FP64; mixed 0.78 against 0.80), while the
+// limited slab corrector, held to two waves per SIMD by its window, takes
+// 2.25 against 2.12-2.17 (mixed 1.55 against 1.21); per step FP64 4.44 ms
+// against 4.49 (slab both stages) and 4.66 (two-kernel both), mixed 2.48
+// against 2.80 and 2.51.  GUACHO_AMD_SLAB_KCHUNK (planes per slab; 0:
+// two-kernel) and GUACHO_AMD_SLAB_STAGES ("1", "2", "12": the stages that
+// use the slab kernel) override the choice.
+void stage_policy(int precision, const gm::Geom& g, int kslab[2]) {
+  (void)precision;
+  const int k = gm::slab_kchunk(g);
+  kslab[0] = k;
+  kslab[1] = 0;
+  int len = k;
+  if (const char* e = std::getenv("GUACHO_AMD_SLAB_KCHUNK")) {
+    const long v = std::strtol(e, nullptr, 10);
+    if (v >= 0 && v <= (1 << 20)) {
+      len = (int)v;
+      kslab[0] = len;
+      kslab[1] = kslab[1] ? len : 0;
+    }
+  }
+  if (const char* e = std::getenv("GUACHO_AMD_SLAB_STAGES")) {
+    const std::string st(e);
+    kslab[0] = st.find('1') != std::string::npos ? len : 0;
+    kslab[1] = st.find('2') != std::string::npos ? len : 0;
+  }
+}
 
 int fail(gm_ctx* c, int code, const std::string& m) {
   if (c) c->msg = m;
@@ -451,10 +484,10 @@ void tstep_t(gm_ctx* c, double dt) {
   const double dtm = dt / 2.0;  // hydro_solver.f90:152
   // predictor: first-order fluxes of U^n, U* = U^n - dt/2 div F (+ dt/2 s)
   // the 2.5-D stage kernel unless the stage kernels add source terms
-  const int kslab = b.srcin ? 0 : c->slab_kchunk;
+  const int kslab1 = b.srcin ? 0 : c->slab_kchunk[0], kslab2 = b.srcin ? 0 : c->slab_kchunk[1];
   stage_user_s_t(c, b, dtm, 1);
-  if (kslab) {
-    timed(c, KC_SLAB1, [&] { gm::launch_slab(s, b, dtm, 1, solver, fcd, err, kslab); });
+  if (kslab1) {
+    timed(c, KC_SLAB1, [&] { gm::launch_slab(s, b, dtm, 1, solver, fcd, err, kslab1); });
   } else {
     timed(c, KC_XY1, [&] { gm::launch_xy(s, b, dtm, 1, solver, fcd, err); });
     timed(c, KC_Z1, [&] { gm::launch_z(s, b, dtm, 1, solver, fcd, err, sh); });
@@ -466,8 +499,8 @@ void tstep_t(gm_ctx* c, double dt) {
   // boundaryII on U* (src/hydro_solver.f90:169) when ghosts live in memory
   if (b.g.gmem) fill_ghost_t<S>(c, b.Us, 2);
   stage_user_s_t(c, b, dt, 2);
-  if (kslab) {
-    timed(c, KC_SLAB2, [&] { gm::launch_slab(s, b, dt, 2, solver, fcd, err, kslab); });
+  if (kslab2) {
+    timed(c, KC_SLAB2, [&] { gm::launch_slab(s, b, dt, 2, solver, fcd, err, kslab2); });
   } else {
     timed(c, KC_XY2, [&] { gm::launch_xy(s, b, dt, 2, solver, fcd, err); });
     timed(c, KC_Z2, [&] { gm::launch_z(s, b, dt, 2, solver, fcd, err, sh); });
@@ -606,10 +639,11 @@ void noalloc_tstep_t(hipStream_t s, const gm_params& p, S* u, unsigned char* scr
   const int solver = p.riemann_solver, fcd = p.enable_flux_cd ? 1 : 0;
   int* err = &sc->err;
   const double dtm = dt / 2.0;  // hydro_solver.f90:152
-  const int kslab = (b.srcin || sizeof(S) == 4) ? 0 : gm::slab_kchunk(b.g);  // as gm_create chooses
+  int kslab[2];
+  stage_policy(p.precision, b.g, kslab);  // as gm_create chooses
   auto stage = [&](double h, int st) {
-    if (kslab) {
-      gm::launch_slab(s, b, h, st, solver, fcd, err, kslab);
+    if (!b.srcin && kslab[st - 1]) {
+      gm::launch_slab(s, b, h, st, solver, fcd, err, kslab[st - 1]);
     } else {
       gm::launch_xy(s, b, h, st, solver, fcd, err);
       gm::launch_z(s, b, h, st, solver, fcd, err, sh);
@@ -657,16 +691,7 @@ int gm_create(const gm_params* params, gm_ctx** out) {
   c->ph.limiter = params->slope_limiter;
   c->ph.eos = GM_EOS_ADIABATIC;
   c->ph.tempsc = 1.0;
-  // The stage kernels: FP64 runs the 2.5-D slab kernel (4% faster per
-  // step than the two-kernel stage at 256^3, same box), the mixed form the
-  // two-kernel stage (its FP32 xy kernel keeps four waves per SIMD; the
-  // slab's window holds it to two: 11% slower).  GUACHO_AMD_SLAB_KCHUNK
-  // (planes per slab; 0 selects the two-kernel stage) overrides either.
-  c->slab_kchunk = params->precision == GM_PRECISION_FP64 ? gm::slab_kchunk(g) : 0;
-  if (const char* e = std::getenv("GUACHO_AMD_SLAB_KCHUNK")) {
-    const long v = std::strtol(e, nullptr, 10);
-    if (v >= 0 && v <= (1 << 20)) c->slab_kchunk = (int)v;
-  }
+  stage_policy(params->precision, g, c->slab_kchunk);
 
   auto bail = [&](int code) {
     g_create_error = c->msg;

@@ -1,11 +1,12 @@
 """The 2.5-D stage kernel (k_slab: x, y and z faces and the whole update in
 one pass marching up z) against the two-kernel stage (k_xy plane tiles,
 then k_z pencils through W): the same bits after several steps, every
-step's CFL minimum included, for slab lengths from one plane to the whole
-grid, partial tiles in x and y, every boundary kind, the four solvers,
-passive scalars, FP64 and mixed precision.  (The FP64 path is also checked
-against the oracle bit for bit by the other GPU tests, which run the slab
-kernel by default.)"""
+step's CFL minimum included, with the slab kernel for both stages at slab
+lengths from one plane to the whole grid and with the library's default
+(slab predictor, two-kernel corrector), partial tiles in x and y, every
+boundary kind, the four solvers, passive scalars, FP64 and mixed precision.
+(The FP64 path is also checked against the oracle bit for bit by the other
+GPU tests, which run the default.)"""
 import contextlib
 import os
 
@@ -20,20 +21,27 @@ pytestmark = pytest.mark.gpu
 ENV = "GUACHO_AMD_SLAB_KCHUNK"
 
 
+ENV_STAGES = "GUACHO_AMD_SLAB_STAGES"
+
+
 @contextlib.contextmanager
 def slab_kchunk(v):
-    old = os.environ.get(ENV)
-    if v is None:
-        os.environ.pop(ENV, None)
-    else:
+    """v planes per slab for both stages (0: the two-kernel stage for both;
+    None: the library's default choice)."""
+    old = {e: os.environ.get(e) for e in (ENV, ENV_STAGES)}
+    for e in (ENV, ENV_STAGES):
+        os.environ.pop(e, None)
+    if v is not None:
         os.environ[ENV] = str(v)
+        os.environ[ENV_STAGES] = "12"
     try:
         yield
     finally:
-        if old is None:
-            os.environ.pop(ENV, None)
-        else:
-            os.environ[ENV] = old
+        for e, val in old.items():
+            if val is None:
+                os.environ.pop(e, None)
+            else:
+                os.environ[e] = val
 
 
 def run(prm, u0, steps, kchunk):
