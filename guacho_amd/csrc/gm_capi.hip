@@ -109,18 +109,21 @@ namespace {
 // The stage kernels of a step, per stage: planes per slab of the 2.5-D
 // kernel, or 0 for the two-kernel stage.  The predictor runs as a slab
 // kernel and the corrector as the two-kernel stage, in both precisions:
-// at 256^3, same box, the first-order slab predictor takes 1.52-1.57 ms
-// against 1.72 for k_xy + k_z (FP64; mixed 0.78 against 0.80), while the
-// limited slab corrector, held to two waves per SIMD by its window, takes
-// 2.25 against 2.12-2.17 (mixed 1.55 against 1.21); per step FP64 4.44 ms
-// against 4.49 (slab both stages) and 4.66 (two-kernel both), mixed 2.48
-// against 2.80 and 2.51.  GUACHO_AMD_SLAB_KCHUNK (planes per slab; 0:
-// two-kernel) and GUACHO_AMD_SLAB_STAGES ("1", "2", "12": the stages that
-// use the slab kernel) override the choice.
+// at 256^3, same box, the first-order slab predictor takes 1.51-1.56 ms
+// against 1.59-1.62 for k_xy + k_z (FP64; mixed 0.78 against 0.79), while
+// the limited slab corrector, held to two waves per SIMD by its window,
+// takes 2.19-2.30 against 1.95-2.00 (mixed 1.55 against 1.07); per step
+// FP64 4.26 ms against 4.42 (slab both stages) and 4.36-4.38 (two-kernel
+// both), mixed 2.33 against 2.79 and 2.37-2.38.  With passive scalars
+// both stages run as the two-kernel stage: the slab kernel's registers
+// (238-256 VGPRs without them) do not hold the extra components, and it
+// spills 12-132 bytes a lane.  GUACHO_AMD_SLAB_KCHUNK (planes per slab;
+// 0: two-kernel) and GUACHO_AMD_SLAB_STAGES ("1", "2", "12": the stages
+// that use the slab kernel) override the choice.
 void stage_policy(int precision, const gm::Geom& g, int kslab[2]) {
   (void)precision;
   const int k = gm::slab_kchunk(g);
-  kslab[0] = k;
+  kslab[0] = g.npas == 0 ? k : 0;
   kslab[1] = 0;
   int len = k;
   if (const char* e = std::getenv("GUACHO_AMD_SLAB_KCHUNK")) {

@@ -157,7 +157,7 @@ __device__ __forceinline__ void load_cons(const Geom& g, const S* __restrict__ U
 // thread per cell.  Wave w holds rows 2w (lanes 0-31) and 2w+1 (lanes
 // 32-63), so a row is half a wavefront and x neighbours are adjacent lanes.
 // The slab kernel (2.5-D) uses the same layout with TYS = 8 rows.
-constexpr int TX = 32, TY = 16, XY_THREADS = TX * TY;
+constexpr int TX = 32, TY = 16, XY_THREADS = TX * TY, XY_WAVES = XY_THREADS / 64;
 static_assert(TX == 32 && XY_THREADS == 512, "two tile rows per 64-lane wavefront");
 constexpr int TYS = 8, SL_THREADS = TX * TYS;
 
@@ -248,14 +248,13 @@ __device__ __forceinline__ void perm_y_if(bool ydir, T* v) {
 // Second order: every thread forms the x and y states of its own cell
 // (cell_states), the last two waves those of the 2 (TX + TYT) halo cells
 // next to the tile.  The x faces are solved before the y states are formed,
-// which keeps the live register set of one axis at a time.  `mid()` runs
-// once every face is solved, before the flux exchange (callers fetch the
-// cell's state there).  Every thread must call this (it holds barriers);
-// the outputs are meaningful for the tile's own cells only.  Returns false
-// when a solver fell through.  `tid` is threadIdx.x.
-template <int SOLVER, int ORDER, int NQ, int TYT, typename S, typename Mid>
-__device__ __forceinline__ bool xy_faces(const Box<S>& b, S* __restrict__ lds, int wx, int wy, int tid, S fx[NQ],
-                                         S fxr[NQ], S fy[NQ], S fyu[NQ], Mid&& mid) {
+// which keeps the live register set of one axis at a time.  Every thread
+// must call this (it holds barriers); the outputs are meaningful for the
+// tile's own cells only.  Returns false when a solver fell through.  `tid`
+// is threadIdx.x.
+template <int SOLVER, int ORDER, int NQ, int TYT, typename S>
+__device__ __forceinline__ bool xy_solve(const Box<S>& b, S* __restrict__ lds, int wx, int wy, int tid, S fx[NQ],
+                                         S fy[NQ]) {
   using L = XYLds<ORDER, NQ, TYT>;
   constexpr int H = L::H, RX = L::RX, PS = L::PS;
   constexpr int WAVES = TX * TYT / 64;
@@ -409,7 +408,21 @@ __device__ __forceinline__ bool xy_faces(const Box<S>& b, S* __restrict__ lds, i
     }
   }
   if (w == 0) __builtin_amdgcn_s_setprio(0);
-  mid();
+  return ok;
+}
+
+// ... then the exchange: each thread's high x face (lane c+1's low face, or
+// the row's closing face) into fxr, the tile's y-face fluxes into LDS for
+// xy_fyu.  Every thread calls this (barriers); the cell's state for the
+// update is best fetched between xy_solve and this call.
+template <int ORDER, int NQ, int TYT, typename S>
+__device__ __forceinline__ void xy_exchange(S* __restrict__ lds, int wx, int wy, int tid, const S fx[NQ],
+                                            const S fy[NQ], S fxr[NQ]) {
+  using L = XYLds<ORDER, NQ, TYT>;
+  const int w = tid >> 6, l = tid & 63;
+  const int c = l & (TX - 1), r = 2 * w + (l >> 5);
+  const bool vy = c < wx && r <= wy;
+  S* __restrict__ lend = lds + L::XEND;
   if constexpr (ORDER == 1) __syncthreads();  // every read of the primitives is done
   S* __restrict__ lfy = lds;
   if (vy) {
@@ -417,20 +430,35 @@ __device__ __forceinline__ bool xy_faces(const Box<S>& b, S* __restrict__ lds, i
     for (int q = 0; q < NQ; q++) lfy[(q * TYT + r) * TX + c] = fy[q];
   }
   __syncthreads();
-
 #pragma unroll
   for (int q = 0; q < NQ; q++) {
     const S v = __shfl_down(fx[q], 1, 64);
     fxr[q] = c == TX - 1 ? lend[q * TYT + r] : v;
   }
+}
+
+// the high y face of the thread's cell (row r+1's low face, or the
+// column's closing face), after xy_exchange
+template <int ORDER, int NQ, int TYT, typename S>
+__device__ __forceinline__ void xy_fyu(const S* __restrict__ lds, int tid, S fyu[NQ]) {
+  using L = XYLds<ORDER, NQ, TYT>;
+  const int w = tid >> 6, l = tid & 63;
+  const int c = l & (TX - 1), r = 2 * w + (l >> 5);
+  const S* __restrict__ ltop = lds + L::TOP;
 #pragma unroll
-  for (int q = 0; q < NQ; q++) fyu[q] = r == TYT - 1 ? ltop[q * TX + c] : lfy[(q * TYT + r + 1) * TX + c];
-  return ok;
+  for (int q = 0; q < NQ; q++) fyu[q] = r == TYT - 1 ? ltop[q * TX + c] : lds[(q * TYT + r + 1) * TX + c];
 }
 
 // xy kernel: the primitives of a 32 x 16 plane tile and its halo into LDS,
-// the tile's x and y faces (xy_faces), then the x/y terms of the update,
-// E3 and the x/y parts of E1, E2 into W for the z kernel.
+// the tile's x and y faces, then the x/y terms of the update, E3 and the
+// x/y parts of E1, E2 into W for the z kernel.  The faces are solved as in
+// xy_solve / xy_exchange / xy_fyu (thread (c, r) solves the low faces of
+// cell (c, r), wave 0 the closing faces, the last two waves the halo
+// states), written out here in one piece: at 4 waves per SIMD this kernel
+// sits at 127-128 VGPRs, and the same code through those three functions
+// is scheduled differently and spills 21-22 of them (about 10% slower).
+// The two forms are checked equal bit for bit by the slab-vs-two-kernel
+// tests.
 template <int SOLVER, int ORDER, int FLUXCD, typename S, int NP = 0>
 __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt, int* __restrict__ err,
                                         S* __restrict__ lds) {
@@ -449,6 +477,8 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
   const int c = l & (TX - 1), r = 2 * w + (l >> 5);
   const S* __restrict__ Usrc = ORDER == 1 ? b.U : b.Us;
   S* __restrict__ lp = lds;
+  S* __restrict__ ltop = lds + L::TOP;
+  S* __restrict__ lend = lds + L::XEND;
 
   // primitives of the tile, its x halo on the tile's rows and its y halo
   // on the tile's columns (H rows below, TY rows of RX, H rows above);
@@ -498,8 +528,152 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
   }
   __syncthreads();
 
-  S fx[NQ], fy[NQ], fxr[NQ], fyu[NQ];
-  // the cell's state for the update, fetched once the faces are solved
+  const int at = (r + H) * RX + c + H;  // own cell in the primitive area
+  const bool vx = r < wy && c <= wx;    // x-face c of row r is the tile's
+  const bool vy = c < wx && r <= wy;    // y-face r of column c is the tile's
+  const bool cx = w == 0 && l < TY;                   // closing x-face of row l
+  const bool cy = w == 0 && l >= TY && l < TY + TX;   // closing y-face of column l - TY
+  const int crow = cx ? l : 0, ccol = cy ? l - TY : 0;
+  bool ok = true;
+  S fx[NQ], fy[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) fx[q] = fy[q] = S(0);
+
+  if constexpr (ORDER == 1) {
+    S pl[NQ], pr[NQ];
+    if (vx) {
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        pl[q] = lp[q * PS + at - 1];
+        pr[q] = lp[q * PS + at];
+      }
+      ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, fx);
+    }
+    if (vy) {
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        pl[q] = lp[q * PS + at - RX];
+        pr[q] = lp[q * PS + at];
+      }
+      axis_perm<1>(pl);
+      axis_perm<1>(pr);
+      ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, fy);
+      axis_perm<1>(fy);
+    }
+    if ((cx && wx == TX && crow < wy) || (cy && wy == TY && ccol < wx)) {
+      const int hi = cy ? (TY + H) * RX + ccol + H : (crow + H) * RX + TX + H;
+      const int st = cy ? RX : 1;
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        pl[q] = lp[q * PS + hi - st];
+        pr[q] = lp[q * PS + hi];
+      }
+      S ff[NQ];
+      perm_y_if(cy, pl);
+      perm_y_if(cy, pr);
+      ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, ff);
+      perm_y_if(cy, ff);
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        if (cy) ltop[q * TX + ccol] = ff[q];
+        else lend[q * TY + crow] = ff[q];
+      }
+    }
+  } else {
+    S* __restrict__ xl = lds + L::XL;
+    S* __restrict__ xu = lds + L::XU;
+    S* __restrict__ xr = lds + L::XR;
+    S* __restrict__ yd = lds + L::YD;
+    S* __restrict__ yu = lds + L::YU;
+    const int lim = b.ph.limiter;
+    // halo cells, into LDS, by the last two waves (h = 0..95): the x
+    // states of (-1, r) and (TX, r), the y states of (c, -1) and (c, TY)
+    if (w >= XY_WAVES - 2) __builtin_amdgcn_s_setprio(2);
+    if (w >= XY_WAVES - 2) {
+      const int h = (w - (XY_WAVES - 2)) * 64 + l;
+      if (h < 2 * TY + 2 * TX) {
+        int hat, hst, hdst, hqs;
+        bool hup;
+        if (h < TY) {
+          hat = (h + H) * RX + H - 1, hst = 1, hup = true, hdst = L::XL + h, hqs = TY;
+        } else if (h < 2 * TY) {
+          hat = (h - TY + H) * RX + H + TX, hst = 1, hup = false, hdst = L::XR + h - TY, hqs = TY;
+        } else if (h < 2 * TY + TX) {
+          hat = (H - 1) * RX + h - 2 * TY + H, hst = RX, hup = true, hdst = L::YU + h - 2 * TY, hqs = L::NYU * TX;
+        } else {
+          hat = (H + TY) * RX + h - 2 * TY - TX + H, hst = RX, hup = false, hdst = L::YD + h - 2 * TY - TX, hqs = TX;
+        }
+        S hs[NQ];
+        cell_state<NQ>(lim, lp, PS, hat, hst, hup, hs);
+#pragma unroll
+        for (int q = 0; q < NQ; q++) lds[hdst + q * hqs] = hs[q];
+      }
+    }
+    if (w >= XY_WAVES - 2) __builtin_amdgcn_s_setprio(0);
+    // x: the upper state passes one lane up (lane c-1 holds cell c-1)
+    S pl[NQ], pr[NQ];
+    {
+      S pxu[NQ];
+      cell_states<NQ>(lim, lp, PS, at, 1, pxu, pr);
+      if (c == TX - 1) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++) xu[q * TY + r] = pxu[q];
+      }
+#pragma unroll
+      for (int q = 0; q < NQ; q++) pl[q] = __shfl_up(pxu[q], 1, 64);
+    }
+    __syncthreads();  // halo states in LDS
+    if (c == 0) {
+#pragma unroll
+      for (int q = 0; q < NQ; q++) pl[q] = xl[q * TY + r];
+    }
+    if (vx) ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, fx);
+    // y: odd rows hand their upper state to the even row above them (the
+    // next wavefront) through LDS, even rows theirs to the odd row above
+    // (lanes 32 up in the same wavefront)
+    {
+      S pyu[NQ];
+      cell_states<NQ>(lim, lp, PS, at, RX, pyu, pr);
+      if (r & 1) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++) yu[(q * L::NYU + (r + 1) / 2) * TX + c] = pyu[q];
+      }
+#pragma unroll
+      for (int q = 0; q < NQ; q++) pl[q] = __shfl_up(pyu[q], 32, 64);
+    }
+    __syncthreads();  // every read of the primitives is done; y states in LDS
+    if (w == 0) __builtin_amdgcn_s_setprio(2);
+    if (!(r & 1)) {
+#pragma unroll
+      for (int q = 0; q < NQ; q++) pl[q] = yu[(q * L::NYU + r / 2) * TX + c];
+    }
+    if (vy) {
+      axis_perm<1>(pl);
+      axis_perm<1>(pr);
+      ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, fy);
+      axis_perm<1>(fy);
+    }
+    if ((cx && wx == TX && crow < wy) || (cy && wy == TY && ccol < wx)) {
+      S ff[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        pl[q] = cy ? yu[(q * L::NYU + TY / 2) * TX + ccol] : xu[q * TY + crow];
+        pr[q] = cy ? yd[q * TX + ccol] : xr[q * TY + crow];
+      }
+      perm_y_if(cy, pl);
+      perm_y_if(cy, pr);
+      ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, ff);
+      perm_y_if(cy, ff);
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        if (cy) ltop[q * TX + ccol] = ff[q];
+        else lend[q * TY + crow] = ff[q];
+      }
+    }
+  }
+  if (w == 0) __builtin_amdgcn_s_setprio(0);
+  if (!ok) atomicOr(err, 1);
+  // the cell's state for the update, fetched ahead of the flux exchange
   const bool own = c < wx && r < wy;
   const long long N = g.ncell;
   const long long cc = cidx(g, i0 + c, j0 + r, k);
@@ -508,16 +682,29 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
   // and the z kernel adds it to u in FP64
   constexpr bool INC = sizeof(S) == 4;
   double u0[NH], u0p[NP > 0 ? NP : 1];
-  const bool ok = xy_faces<SOLVER, ORDER, NQ, TY>(b, lds, wx, wy, tid, fx, fxr, fy, fyu, [&] {
-    if (own && !INC) {
+  if (own && !INC) {
 #pragma unroll
-      for (int q = 0; q < NH; q++) u0[q] = (double)b.U[q * N + cc];
+    for (int q = 0; q < NH; q++) u0[q] = (double)b.U[q * N + cc];
 #pragma unroll
-      for (int a = 0; a < NP; a++) u0p[a] = (double)b.U[(8 + a) * N + cc];
-    }
-  });
-  if (!ok) atomicOr(err, 1);
+    for (int a = 0; a < NP; a++) u0p[a] = (double)b.U[(8 + a) * N + cc];
+  }
+  if constexpr (ORDER == 1) __syncthreads();  // every read of the primitives is done
+  S* __restrict__ lfy = lds;
+  if (vy) {
+#pragma unroll
+    for (int q = 0; q < NQ; q++) lfy[(q * TY + r) * TX + c] = fy[q];
+  }
+  __syncthreads();
+
+  S fxr[NQ], fyu[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const S v = __shfl_down(fx[q], 1, 64);
+    fxr[q] = c == TX - 1 ? lend[q * TY + r] : v;
+  }
   if (!own) return;
+#pragma unroll
+  for (int q = 0; q < NQ; q++) fyu[q] = r == TY - 1 ? ltop[q * TX + c] : lfy[(q * TY + r + 1) * TX + c];
   const double dtdx = dt / b.dx, dtdy = dt / b.dy;
   if constexpr (INC) {
     float* __restrict__ Wf = reinterpret_cast<float*>(b.W);
@@ -714,7 +901,7 @@ __device__ __forceinline__ void z_body(const Box<S>& b, unsigned lb, double dt, 
 // tile over a slab of `kchunk` planes, marching up z.  Per plane m: the
 // tile's primitives into LDS -- the own cells' from the column window, the
 // halo's from HBM -- then the z face above each own cell (z_body's march,
-// one face per plane), the tile's x and y faces (xy_faces), and the cell's
+// one face per plane), the tile's x and y faces (xy_solve, xy_exchange), and the cell's
 // whole update: the x, y and z terms of step in the reference's order, E3
 // and E1, E2 (get_efield's four-term sums), stored once.  The x/y partial
 // sums that the xy kernel hands the z kernel through W stay in registers
@@ -956,14 +1143,15 @@ __device__ __forceinline__ void slab_body(const Box<S>& b, unsigned lb, double d
     const long long cc = col + (long long)(m + 1) * sz;
     S fx[NQ], fy[NQ], fxr[NQ], fyu[NQ];
     double u0[NQ];
-    ok &= xy_faces<SOLVER, ORDER, NQ, TYT>(b, lds, wx, wy, tid, fx, fxr, fy, fyu, [&] {
-      if (own) {
-#pragma unroll
-        for (int q = 0; q < NQ; q++)
-          if (q < NH || q >= 8) u0[q] = (double)b.U[q * N + cc];
-      }
-    });
+    ok &= xy_solve<SOLVER, ORDER, NQ, TYT>(b, lds, wx, wy, tid, fx, fy);
     if (own) {
+#pragma unroll
+      for (int q = 0; q < NQ; q++)
+        if (q < NH || q >= 8) u0[q] = (double)b.U[q * N + cc];
+    }
+    xy_exchange<ORDER, NQ, TYT>(lds, wx, wy, tid, fx, fy, fxr);
+    if (own) {
+      xy_fyu<ORDER, NQ, TYT>(lds, tid, fyu);
       // the x/y partials, rounded as the xy kernel stores them in W, then
       // the z term (step, hydro_solver.f90:105-107 / flux_cd_module.f90:299-308)
 #pragma unroll

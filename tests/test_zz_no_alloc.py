@@ -79,3 +79,42 @@ def test_noalloc_refuses_a_grid_past_the_limit():
     rc = L.gm_tstep_noalloc(C.byref(prm), C.c_void_p(1), 1e-3, None, None, None)
     assert rc == _capi.GM_ERR_CONFIG
     assert "too large" in L.gm_last_error(None).decode()
+
+
+def test_noalloc_step_captured_in_a_graph():
+    """What the entry point is for: captured in a HIP graph (torch's capture
+    stream; the call enqueues kernels only), each replay equals the eager
+    call bit for bit -- the state and the next CFL minimum -- with eager
+    work and synchronisation between the replays."""
+    print("case: gm_tstep_noalloc captured in a graph, 3 replays vs 3 eager steps, turbulence 24^3 FP64", flush=True)
+    L = _capi.lib()
+    u0, dx, dy, dz = problems.turbulence(24, seed=21)
+    prm = _capi.make_params(24, 24, 24, dx, dy, dz)
+    dt = context_step(prm, u0)[0]
+    host = to_device_layout(u0)
+    ue = torch.from_numpy(host.copy()).to("cuda:0")
+    ug = torch.from_numpy(host.copy()).to("cuda:0")
+    dtp_e = torch.zeros(1, dtype=torch.float64, device="cuda:0")
+    dtp_g = torch.zeros(1, dtype=torch.float64, device="cuda:0")
+    err = torch.zeros(1, dtype=torch.int32, device="cuda:0")
+
+    def step(u, dtp):
+        rc = L.gm_tstep_noalloc(C.byref(prm), C.c_void_p(u.data_ptr()), dt,
+                                C.c_void_p(torch.cuda.current_stream().cuda_stream), C.c_void_p(dtp.data_ptr()),
+                                C.c_void_p(err.data_ptr()))
+        assert rc == 0, L.gm_last_error(None).decode()
+
+    scratch = torch.zeros_like(ue)
+    step(scratch, dtp_e)            # first call on the device: scratch address resolved outside capture
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        step(ug, dtp_g)
+    for rep in range(3):
+        g.replay()
+        torch.cuda.synchronize()
+        step(ue, dtp_e)             # eager work between the replays
+        torch.cuda.synchronize()
+        assert int(err.item()) == 0
+        assert torch.equal(ug, ue), f"replay {rep}"
+        assert dtp_g.item() == dtp_e.item(), f"replay {rep}"
