@@ -695,6 +695,13 @@ int gm_create(const gm_params* params, gm_ctx** out) {
   c->ph.eos = GM_EOS_ADIABATIC;
   c->ph.tempsc = 1.0;
   stage_policy(params->precision, g, c->slab_kchunk);
+  // GUACHO_AMD_COLUMN_SHAPE="block,kchunk" overrides the column kernels'
+  // tile width (64, 128 or 256 threads) and z-pencil length
+  if (const char* e = std::getenv("GUACHO_AMD_COLUMN_SHAPE")) {
+    int bl = 0, kc = 0;
+    if (std::sscanf(e, "%d,%d", &bl, &kc) == 2 && (bl == 64 || bl == 128 || bl == 256) && kc >= 1 && kc <= 4096)
+      c->shape = {bl, kc, 0};
+  }
 
   auto bail = [&](int code) {
     g_create_error = c->msg;
