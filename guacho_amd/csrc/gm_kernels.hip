@@ -252,9 +252,9 @@ __device__ __forceinline__ void perm_y_if(bool ydir, T* v) {
 // must call this (it holds barriers); the outputs are meaningful for the
 // tile's own cells only.  Returns false when a solver fell through.  `tid`
 // is threadIdx.x.
-template <int SOLVER, int ORDER, int NQ, int TYT, typename S>
+template <int SOLVER, int ORDER, int NQ, int TYT, typename S, bool CS = false>
 __device__ __forceinline__ bool xy_solve(const Box<S>& b, S* __restrict__ lds, int wx, int wy, int tid, S fx[NQ],
-                                         S fy[NQ]) {
+                                         S fy[NQ], const S* __restrict__ cfs = nullptr) {
   using L = XYLds<ORDER, NQ, TYT>;
   constexpr int H = L::H, RX = L::RX, PS = L::PS;
   constexpr int WAVES = TX * TYT / 64;
@@ -276,6 +276,9 @@ __device__ __forceinline__ bool xy_solve(const Box<S>& b, S* __restrict__ lds, i
   for (int q = 0; q < NQ; q++) fx[q] = fy[q] = S(0);
 
   if constexpr (ORDER == 1) {
+    // CS: the cells' side speeds are in LDS beside the primitives (x at
+    // cfs[cell], y at cfs[PS + cell]; side_speed of the cell's primitives
+    // in that axis's order)
     S pl[NQ], pr[NQ];
     if (vx) {
 #pragma unroll
@@ -283,7 +286,8 @@ __device__ __forceinline__ bool xy_solve(const Box<S>& b, S* __restrict__ lds, i
         pl[q] = lp[q * PS + at - 1];
         pr[q] = lp[q * PS + at];
       }
-      ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, fx);
+      if constexpr (CS) ok &= riemann_cs<SOLVER, NQ>(b.ph, pl, pr, cfs[at - 1], cfs[at], fx);
+      else ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, fx);
     }
     if (vy) {
 #pragma unroll
@@ -293,7 +297,8 @@ __device__ __forceinline__ bool xy_solve(const Box<S>& b, S* __restrict__ lds, i
       }
       axis_perm<1>(pl);
       axis_perm<1>(pr);
-      ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, fy);
+      if constexpr (CS) ok &= riemann_cs<SOLVER, NQ>(b.ph, pl, pr, cfs[PS + at - RX], cfs[PS + at], fy);
+      else ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, fy);
       axis_perm<1>(fy);
     }
     if ((cx && wx == TX && crow < wy) || (cy && wy == TYT && ccol < wx)) {
@@ -307,7 +312,12 @@ __device__ __forceinline__ bool xy_solve(const Box<S>& b, S* __restrict__ lds, i
       S ff[NQ];
       perm_y_if(cy, pl);
       perm_y_if(cy, pr);
-      ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, ff);
+      if constexpr (CS) {
+        const S* __restrict__ cf = cfs + (cy ? PS : 0);
+        ok &= riemann_cs<SOLVER, NQ>(b.ph, pl, pr, cf[hi - st], cf[hi], ff);
+      } else {
+        ok &= riemann<SOLVER, NQ>(b.ph, pl, pr, ff);
+      }
       perm_y_if(cy, ff);
 #pragma unroll
       for (int q = 0; q < NQ; q++) {
@@ -808,6 +818,16 @@ __device__ __forceinline__ void z_body(const Box<S>& b, unsigned lb, double dt, 
     take(pc);
     fetch(k0);
   }
+  // first order: the side speed of P(kf), kept from the face below (a
+  // cell's value serves both of its z faces)
+  S cz = S(0);
+  if constexpr (ORDER == 1) {
+    S t[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) t[q] = pc[q];
+    axis_perm<2>(t);
+    cz = side_speed<SOLVER>(b.ph, t);
+  }
   S fl[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; q++) fl[q] = S(0);
@@ -865,7 +885,15 @@ __device__ __forceinline__ void z_body(const Box<S>& b, unsigned lb, double dt, 
     axis_perm<2>(left);
     axis_perm<2>(right);
     S ff[NQ];
-    if (!riemann<SOLVER, NQ>(b.ph, left, right, ff)) atomicOr(err, 1);
+    bool fok;
+    if constexpr (ORDER == 1) {
+      const S csr = side_speed<SOLVER>(b.ph, right);
+      fok = riemann_cs<SOLVER, NQ>(b.ph, left, right, cz, csr, ff);
+      cz = csr;
+    } else {
+      fok = riemann<SOLVER, NQ>(b.ph, left, right, ff);
+    }
+    if (!fok) atomicOr(err, 1);
     axis_perm<2>(ff);
     if (kf >= k0) {
       // source terms (source(), src/sources.f90:188-220): added to the
@@ -920,7 +948,8 @@ template <int ORDER, int NQ, typename S>
 struct SlabLds {
   using X = XYLds<ORDER, NQ, TYS>;
   static constexpr int WIN = X::SIZE;             // [2][NQ][SL_THREADS] window slots
-  static constexpr int SIZE = WIN + 2 * NQ * SL_THREADS;
+  static constexpr int CF = WIN + 2 * NQ * SL_THREADS;  // first order: [2][PS] x and y side speeds
+  static constexpr int SIZE = CF + (ORDER == 1 ? 2 * X::PS : 0);
 };
 
 template <int SOLVER, int ORDER, int FLUXCD, typename S, int NP = 0>
@@ -951,6 +980,7 @@ __device__ __forceinline__ void slab_body(const Box<S>& b, unsigned lb, double d
   const int lim = b.ph.limiter;
   S* __restrict__ lp = lds;
   S* __restrict__ win = lds + SlabLds<ORDER, NQ, S>::WIN;  // win[(slot * NQ + q) * SL_THREADS + tid]
+  S* __restrict__ cfs = lds + SlabLds<ORDER, NQ, S>::CF;   // first order: side speeds, x [PS] then y [PS]
   bool ok = true;
   // lane (c, r) of the tile: c = tid & 31, r = 2 (tid >> 6) + ((tid & 63) >> 5);
   // cell k of its column at col + (k+1)*sz
@@ -992,7 +1022,10 @@ __device__ __forceinline__ void slab_body(const Box<S>& b, unsigned lb, double d
   // z face kf (between cells kf and kf+1) from P(kf) (`lo`), the window
   // slot holding P(kf+1) and P(kf+ORDER) (`nx`); ORDER 2 moves pu to cell
   // kf+1's upper state
+  // first order: cz is the side speed of the cell below the face, kept
+  // from the face below it (a cell's value serves both its z faces)
   S pu[NQ];
+  S cz = S(0);
 #pragma unroll
   for (int q = 0; q < NQ; q++) pu[q] = S(0);
   auto z_face = [&](int tid, const S lo[NQ], int slot1, const S nx[NQ], S ff[NQ]) {
@@ -1016,8 +1049,25 @@ __device__ __forceinline__ void slab_body(const Box<S>& b, unsigned lb, double d
     }
     axis_perm<2>(left);
     axis_perm<2>(right);
-    ok &= riemann<SOLVER, NQ>(b.ph, left, right, ff);
+    if constexpr (ORDER == 1) {
+      const S csr = side_speed<SOLVER>(b.ph, right);
+      ok &= riemann_cs<SOLVER, NQ>(b.ph, left, right, cz, csr, ff);
+      cz = csr;
+    } else {
+      ok &= riemann<SOLVER, NQ>(b.ph, left, right, ff);
+    }
     axis_perm<2>(ff);
+  };
+  // the x and y side speeds of a cell into LDS (first order)
+  auto put_speeds = [&](int cell, const S p[NQ], bool xs, bool ys) {
+    if (xs) cfs[cell] = side_speed<SOLVER>(b.ph, p);
+    if (ys) {
+      S t[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; q++) t[q] = p[q];
+      axis_perm<1>(t);
+      cfs[L::PS + cell] = side_speed<SOLVER>(b.ph, t);
+    }
   };
 
   // Window before the first plane: slot (k0 & 1) holds P(k0), the other
@@ -1052,6 +1102,13 @@ __device__ __forceinline__ void slab_body(const Box<S>& b, unsigned lb, double d
         take(ur, f, pa);                   // P(k0-1)
         fetch(col, k0, ur, f);
         take(ur, f, pb);                   // P(k0)
+        {
+          S t[NQ];
+#pragma unroll
+          for (int q = 0; q < NQ; q++) t[q] = pa[q];
+          axis_perm<2>(t);
+          cz = side_speed<SOLVER>(b.ph, t);
+        }
         z_face(tid, pa, 0, pb, fl);        // face k0-1
         wput(tid, k0 & 1, pb);
       }
@@ -1083,6 +1140,7 @@ __device__ __forceinline__ void slab_body(const Box<S>& b, unsigned lb, double d
       wget(tid, m & 1, pm);
 #pragma unroll
       for (int q = 0; q < NQ; q++) lp[q * PS + at] = pm[q];
+      if constexpr (ORDER == 1) put_speeds(at, pm, true, true);
     }
     __builtin_amdgcn_s_setprio(3);
     {
@@ -1117,8 +1175,19 @@ __device__ __forceinline__ void slab_body(const Box<S>& b, unsigned lb, double d
         if (dst[it] < 0) continue;
         double p[NQ];
         u2prim<NQ>(b.ph, u[it], p);
+        S ps[NQ];
 #pragma unroll
-        for (int q = 0; q < NQ; q++) lp[q * PS + dst[it]] = (S)p[q];
+        for (int q = 0; q < NQ; q++) {
+          ps[q] = (S)p[q];
+          lp[q * PS + dst[it]] = ps[q];
+        }
+        // a halo cell on one of the tile's rows closes an x face, any other
+        // a y face
+        if constexpr (ORDER == 1) {
+          const int hy = dst[it] / RX;
+          const bool xs = hy >= H && hy < H + wy;
+          put_speeds(dst[it], ps, xs, !xs);
+        }
       }
     }
     // z face m (above the own cell): P(m) from the window slot, P(m+ORDER)
@@ -1143,7 +1212,7 @@ __device__ __forceinline__ void slab_body(const Box<S>& b, unsigned lb, double d
     const long long cc = col + (long long)(m + 1) * sz;
     S fx[NQ], fy[NQ], fxr[NQ], fyu[NQ];
     double u0[NQ];
-    ok &= xy_solve<SOLVER, ORDER, NQ, TYT>(b, lds, wx, wy, tid, fx, fy);
+    ok &= xy_solve<SOLVER, ORDER, NQ, TYT, S, ORDER == 1>(b, lds, wx, wy, tid, fx, fy, cfs);
     if (own) {
 #pragma unroll
       for (int q = 0; q < NQ; q++)
@@ -1825,7 +1894,8 @@ k_xy(Box<S> b, double dt, int* err) {
 }
 
 template <int SOLVER, int ORDER, int FLUXCD, typename S, int NP, bool W8 = false>
-__global__ void __launch_bounds__(256) k_z(Box<S> b, double dt, int kchunk, int* err) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ORDER == 1 && FLUXCD && NP == 0 && !W8 ? 3 : 1)))
+k_z(Box<S> b, double dt, int kchunk, int* err) {
   z_body<SOLVER, ORDER, FLUXCD, S, NP, W8>(b, xcd_logical_block(), dt, kchunk, err);
 }
 
@@ -1898,7 +1968,8 @@ kb_slab(const Box<S>* __restrict__ boxes, const unsigned* __restrict__ prefix, i
 }
 
 template <int SOLVER, int ORDER, int FLUXCD, typename S>
-__global__ void __launch_bounds__(256) kb_z(const Box<S>* __restrict__ boxes, const unsigned* __restrict__ prefix,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ORDER == 1 && FLUXCD ? 3 : 1)))
+kb_z(const Box<S>* __restrict__ boxes, const unsigned* __restrict__ prefix,
                                             int nbox, const BoxClock* __restrict__ clk, int kchunk, int* err) {
   int bx;
   unsigned lb;

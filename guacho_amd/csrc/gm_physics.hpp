@@ -274,10 +274,9 @@ __device__ __forceinline__ T hlld_energy(T cv, const T s[8], T bx) {
 // branch for that region: UL* :111-156, UR* :159-204, UL** :255-283,
 // UR** :286-314.
 template <int NQ = 8, typename T>
-__device__ __forceinline__ bool hlld(const Phys& ph, const T pl[NQ], const T pr[NQ], T ff[NQ]) {
+__device__ __forceinline__ bool hlld_cs(const Phys& ph, const T pl[NQ], const T pr[NQ], T csl, T csr, T ff[NQ]) {
   using A = Ar<T>;
   const T cv = (T)ph.cv;
-  const T csl = cfastX(ph, pl), csr = cfastX(ph, pr);
   const T sr = dmax(pl[1] + csl, pr[1] + csr);
   const T sl = dmin(pl[1] - csl, pr[1] - csr);
   if (sl > T(0)) { prim2f<NQ>(ph, pl, ff); return true; }
@@ -355,6 +354,11 @@ __device__ __forceinline__ bool hlld(const Phys& ph, const T pl[NQ], const T pr[
   return true;
 }
 
+template <int NQ = 8, typename T>
+__device__ __forceinline__ bool hlld(const Phys& ph, const T pl[NQ], const T pr[NQ], T ff[NQ]) {
+  return hlld_cs<NQ>(ph, pl, pr, cfastX(ph, pl), cfastX(ph, pr), ff);
+}
+
 // HLL average of prim2fhlle (src/hlle.f90:74-79) and prim2fhll (src/hll.f90:73-78)
 template <int NQ = 8, typename T>
 __device__ __forceinline__ void hll_average(const Phys& ph, const T pl[NQ], const T pr[NQ], T sl, T sr, T ff[NQ]) {
@@ -369,8 +373,7 @@ __device__ __forceinline__ void hll_average(const Phys& ph, const T pl[NQ], cons
 
 // prim2fhlle, src/hlle.f90:48-83
 template <int NQ = 8, typename T>
-__device__ __forceinline__ bool hlle(const Phys& ph, const T pl[NQ], const T pr[NQ], T ff[NQ]) {
-  const T csl = cfastX(ph, pl), csr = cfastX(ph, pr);
+__device__ __forceinline__ bool hlle_cs(const Phys& ph, const T pl[NQ], const T pr[NQ], T csl, T csr, T ff[NQ]) {
   const T sr = dmax(pl[1] + csl, pr[1] + csr);
   const T sl = dmin(pl[1] - csl, pr[1] - csr);
   if (sl > T(0)) { prim2f<NQ>(ph, pl, ff); return true; }
@@ -381,8 +384,7 @@ __device__ __forceinline__ bool hlle(const Phys& ph, const T pl[NQ], const T pr[
 
 // prim2fhll, src/hll.f90:47-82
 template <int NQ = 8, typename T>
-__device__ __forceinline__ bool hll(const Phys& ph, const T pl[NQ], const T pr[NQ], T ff[NQ]) {
-  const T csl = csound(ph, pl[4], pl[0]), csr = csound(ph, pr[4], pr[0]);
+__device__ __forceinline__ bool hll_cs(const Phys& ph, const T pl[NQ], const T pr[NQ], T csl, T csr, T ff[NQ]) {
   const T sr = dmax(pl[1] + csl, pr[1] + csr);
   const T sl = dmin(pl[1] - csl, pr[1] - csr);
   if (sl > T(0)) { prim2f<NQ>(ph, pl, ff); return true; }
@@ -393,9 +395,8 @@ __device__ __forceinline__ bool hll(const Phys& ph, const T pl[NQ], const T pr[N
 
 // prim2fhllc, src/hllc.f90:44-140 (passive B)
 template <int NQ = 8, typename T>
-__device__ __forceinline__ bool hllc(const Phys& ph, const T pl[NQ], const T pr[NQ], T ff[NQ]) {
+__device__ __forceinline__ bool hllc_cs(const Phys& ph, const T pl[NQ], const T pr[NQ], T csl, T csr, T ff[NQ]) {
   using A = Ar<T>;
-  const T csl = csound(ph, pl[4], pl[0]), csr = csound(ph, pr[4], pr[0]);
   const T sr = dmax(pl[1] + csl, pr[1] + csr);
   const T sl = dmin(pl[1] - csl, pr[1] - csr);
   if (sl > T(0)) { prim2f<NQ>(ph, pl, ff); return true; }
@@ -429,12 +430,31 @@ __device__ __forceinline__ bool hllc(const Phys& ph, const T pl[NQ], const T pr[
   return true;
 }
 
+// The wave-speed estimate of one side, from that side's primitives in the
+// solver's axis order: cfastX for HLLD and HLLE (hlld.f90:64-65,
+// hlle.f90:58-59), csound for HLL and HLLC (hll.f90:57-58, hllc.f90:55-56).
+// It depends on that state alone, so in a first-order sweep, where a
+// face's states are the cells' own primitives, a cell's value serves both
+// of its faces on that axis: the same operation on the same operands, the
+// same bits.
+template <int SOLVER, typename T>
+__device__ __forceinline__ T side_speed(const Phys& ph, const T p[8]) {
+  if constexpr (SOLVER == SOLVER_HLLD || SOLVER == SOLVER_HLLE) return cfastX(ph, p);
+  else return csound(ph, p[4], p[0]);
+}
+
+// The interface flux from both sides' states and their side_speed values.
+template <int SOLVER, int NQ = 8, typename T>
+__device__ __forceinline__ bool riemann_cs(const Phys& ph, const T pl[NQ], const T pr[NQ], T csl, T csr, T ff[NQ]) {
+  if constexpr (SOLVER == SOLVER_HLLD) return hlld_cs<NQ>(ph, pl, pr, csl, csr, ff);
+  else if constexpr (SOLVER == SOLVER_HLLE) return hlle_cs<NQ>(ph, pl, pr, csl, csr, ff);
+  else if constexpr (SOLVER == SOLVER_HLL) return hll_cs<NQ>(ph, pl, pr, csl, csr, ff);
+  else return hllc_cs<NQ>(ph, pl, pr, csl, csr, ff);
+}
+
 template <int SOLVER, int NQ = 8, typename T>
 __device__ __forceinline__ bool riemann(const Phys& ph, const T pl[NQ], const T pr[NQ], T ff[NQ]) {
-  if constexpr (SOLVER == SOLVER_HLLD) return hlld<NQ>(ph, pl, pr, ff);
-  else if constexpr (SOLVER == SOLVER_HLLE) return hlle<NQ>(ph, pl, pr, ff);
-  else if constexpr (SOLVER == SOLVER_HLL) return hll<NQ>(ph, pl, pr, ff);
-  else return hllc<NQ>(ph, pl, pr, ff);
+  return riemann_cs<SOLVER, NQ>(ph, pl, pr, side_speed<SOLVER>(ph, pl), side_speed<SOLVER>(ph, pr), ff);
 }
 
 }  // namespace gm
