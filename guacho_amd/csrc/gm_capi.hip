@@ -81,6 +81,11 @@ struct gm_ctx {
   // nothing has changed U's interior or the EOS since
   bool cfl_ready = false;
   bool cfl_pending = false;  // this gm_tstep queued the reduction (ready once its launches succeeded)
+  // Table cooling: the pressure primit carries for the interior cells
+  // (gm::Box::pc), allocated with the first table-cooling setting; live
+  // from a step that ended with table cooling until the next step or upload
+  double* Pc = nullptr;
+  bool pc_live = false;
   double time = 0.0;                    // simulation time the user hooks see
   hipModule_t usrc_mod = nullptr;       // user source plugin
   hipFunction_t usrc_fn = nullptr;
@@ -271,7 +276,7 @@ void release(gm_ctx* c) {
     (void)hipEventDestroy(r.b);
   }
   for (auto e : c->pool) (void)hipEventDestroy(e);
-  void* bufs[] = {c->U, c->Us, c->W, c->E, c->Ssrc, c->Dl};
+  void* bufs[] = {c->U, c->Us, c->W, c->E, c->Ssrc, c->Dl, c->Pc};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->usrc_mod) (void)hipModuleUnload(c->usrc_mod);
@@ -377,6 +382,7 @@ gm::Box<S> box_of(const gm_ctx* c) {
   b.srcin = (b.src != 0 && c->g.npas == 0) ? 1 : 0;
   b.usrc = (b.srcin && c->usrc_fn) ? (const double*)c->Ssrc : nullptr;
   b.g.gmem = (c->ubc_fn || c->cond.kind) ? 1 : 0;
+  b.pc = c->pc_live ? c->Pc : nullptr;
   return b;
 }
 
@@ -524,8 +530,13 @@ void tstep_t(gm_ctx* c, double dt) {
   // viscous_copy (src/hydro_solver.f90:188); with eta = 0 the identity on
   // values, so no pass
   if (p.eta != 0.0) timed(c, KC_VISCOUS, [&] { gm::launch_viscous(s, b, p.eta); });
-  // cooling (src/hydro_solver.f90:206-210)
-  if (c->cool.kind) timed(c, KC_COOLING, [&] { gm::launch_cooling(s, b, c->cool, dt); });
+  // cooling (src/hydro_solver.f90:206-210); table cooling also leaves the
+  // interior cells' primit pressure for the next step (Box::pc)
+  if (c->cool.kind) {
+    gm::Box<S> bc = b;
+    bc.pc = c->cool.kind == GM_COOL_H ? nullptr : c->Pc;
+    timed(c, KC_COOLING, [&] { gm::launch_cooling(s, bc, c->cool, dt); });
+  }
   // boundaryI on U^{n+1} (src/hydro_solver.f90:216) when ghosts live in memory
   if (b.g.gmem) fill_ghost_t<S>(c, b.U, 1);
   // thermal conduction (src/hydro_solver.f90:227)
@@ -752,6 +763,7 @@ int gm_upload_u(gm_ctx* c, const double* u) {
   int rc = set_device(c);
   if (rc) return rc;
   c->cfl_ready = false;
+  c->pc_live = false;  // the host's calcprim of the new state
   const int nq = c->p.neq;
   const size_t n8 = (size_t)c->g.ncell * nq * sizeof(double);
   GM_HIP(c, hipMemcpyAsync(c->W, u, n8, hipMemcpyHostToDevice, c->stream));
@@ -802,8 +814,10 @@ int gm_download_primit(gm_ctx* c, double* primit, double* temp) {
   if (temp && (rc = download_staging(c, &T)) != GM_OK) return rc;
   const int nq = c->p.neq;
   timed(c, KC_LAYOUT, [&] {
-    if (c->esize == 8) gm::launch_prim_all(c->stream, c->g.ncell, c->ph, (const double*)c->U, (double*)c->W, T, nq);
-    else gm::launch_prim_all(c->stream, c->g.ncell, c->ph, (const float*)c->U, (double*)c->W, T, nq);
+    const double* pc = c->pc_live ? c->Pc : nullptr;
+    if (c->esize == 8)
+      gm::launch_prim_all(c->stream, c->g.ncell, c->ph, (const double*)c->U, (double*)c->W, T, nq, c->g, pc);
+    else gm::launch_prim_all(c->stream, c->g.ncell, c->ph, (const float*)c->U, (double*)c->W, T, nq, c->g, pc);
   });
   GM_HIP(c, hipGetLastError());
   GM_HIP(c, hipMemcpyAsync(primit, c->W, (size_t)c->g.ncell * nq * sizeof(double), hipMemcpyDeviceToHost,
@@ -862,6 +876,9 @@ int gm_tstep(gm_ctx* c, double dt) {
   c->cfl_ready = c->cfl_pending = false;
   if (c->esize == 8) tstep_t<double>(c, dt);
   else tstep_t<float>(c, dt);
+  // after table cooling tstep recomputes only the ghost primitives
+  // (hydro_solver.f90:218-223); otherwise all of them
+  c->pc_live = c->cool.kind && c->cool.kind != GM_COOL_H;
   c->ghosts_dirty = c->ubc_fn == nullptr && !c->cond.kind;  // else the step ends with boundaryI
   GM_HIP(c, hipGetLastError());
   c->cfl_ready = c->cfl_pending;
@@ -995,6 +1012,16 @@ int gm_set_cooling(gm_ctx* c, int32_t kind, const double* cooltab, double tempsc
   cl.tsc = tsc;
   cl.hicoef = kind == GM_COOL_DMC ? 0.27e-26 : 0.21e-26;  // cooling_dmc.f90:88, cooling_chi.f90
   if (cooltab) std::memcpy(cl.tab, cooltab, sizeof cl.tab);
+  if (kind != GM_COOL_H && !c->Pc) {
+    int rc = set_device(c);
+    if (rc) return rc;
+    if (hipMalloc((void**)&c->Pc, (size_t)c->g.ncell * sizeof(double)) != hipSuccess) {
+      (void)hipGetLastError();
+      c->Pc = nullptr;
+      return fail(c, GM_ERR_OOM, "device allocation failed (table cooling's primitive pressure)");
+    }
+    c->bytes += (int64_t)c->g.ncell * (int64_t)sizeof(double);
+  }
   c->cool = cl;
   return GM_OK;
 }

@@ -532,6 +532,10 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
       if (dst[it] < 0) continue;
       double p[NQ];
       u2prim<NQ>(b.ph, u[it], p);
+      if (ORDER == 1 && b.pc) {  // Box::pc: interior cells only
+        const int hi = i0 - H + dst[it] % RX, hj = j0 - H + dst[it] / RX;
+        if (hi >= 1 && hi <= g.nx && hj >= 1 && hj <= g.ny) p[4] = b.pc[cidx(g, hi, hj, k)];
+      }
 #pragma unroll
       for (int q = 0; q < NQ; q++) lp[q * PS + dst[it]] = (S)p[q];
     }
@@ -781,6 +785,7 @@ __device__ __forceinline__ void z_body(const Box<S>& b, unsigned lb, double dt, 
 
   S ur[NQ];
   bool uflip = false;
+  int uk = 0;  // cell of the fetched state
   auto fetch = [&](int k) {
     bool f = false;
     const int sk = ghost_src(g, k, g.nz, g.bc[4], g.bc[5], f);
@@ -788,6 +793,7 @@ __device__ __forceinline__ void z_body(const Box<S>& b, unsigned lb, double dt, 
 #pragma unroll
     for (int q = 0; q < NQ; q++) ur[q] = Usrc[q * N + cc];
     uflip = f;
+    uk = k;
   };
   auto take = [&](S p[NQ]) {
     double u[NQ], pd[NQ];
@@ -795,6 +801,9 @@ __device__ __forceinline__ void z_body(const Box<S>& b, unsigned lb, double dt, 
     for (int q = 0; q < NQ; q++) u[q] = (double)ur[q];
     if (uflip) u[3] = -u[3];
     u2prim<NQ>(b.ph, u, pd);
+    // first stage: an interior cell's pressure as primit carries it after
+    // table cooling (Box::pc)
+    if (ORDER == 1 && b.pc && uk >= 1 && uk <= g.nz) pd[4] = b.pc[col + (long long)(uk + 1) * sz];
 #pragma unroll
     for (int q = 0; q < NQ; q++) p[q] = (S)pd[q];
   };
@@ -1002,12 +1011,15 @@ __device__ __forceinline__ void slab_body(const Box<S>& b, unsigned lb, double d
     for (int q = 0; q < NQ; q++) ur[q] = Usrc[q * N + cc];
     flip = f;
   };
-  auto take = [&](const S ur[NQ], bool flip, S p[NQ]) {
+  // kk: the cell's plane; the first stage takes an interior cell's
+  // pressure as primit carries it after table cooling (Box::pc)
+  auto take = [&](const S ur[NQ], bool flip, S p[NQ], long long col, int kk) {
     double u[NQ], pd[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; q++) u[q] = (double)ur[q];
     if (flip) u[3] = -u[3];
     u2prim<NQ>(b.ph, u, pd);
+    if (ORDER == 1 && b.pc && kk >= 1 && kk <= g.nz) pd[4] = b.pc[col + (long long)(kk + 1) * sz];
 #pragma unroll
     for (int q = 0; q < NQ; q++) p[q] = (S)pd[q];
   };
@@ -1085,23 +1097,23 @@ __device__ __forceinline__ void slab_body(const Box<S>& b, unsigned lb, double d
       if constexpr (ORDER == 2) {
         S p0[NQ];
         fetch(col, k0 - 2, ur, f);
-        take(ur, f, p0);
+        take(ur, f, p0, col, k0 - 2);
         fetch(col, k0 - 1, ur, f);
-        take(ur, f, pa);                   // P(k0-1)
+        take(ur, f, pa, col, k0 - 1);      // P(k0-1)
         fetch(col, k0, ur, f);
-        take(ur, f, pb);                   // P(k0)
+        take(ur, f, pb, col, k0);          // P(k0)
 #pragma unroll
         for (int q = 0; q < NQ; q++) pu[q] = pa[q] + lim_average(lim, pa[q] - p0[q], pb[q] - pa[q]) * S(0.5);
         wput(tid, k0 & 1, pb);
         fetch(col, k0 + 1, ur, f);
-        take(ur, f, p0);                   // P(k0+1)
+        take(ur, f, p0, col, k0 + 1);      // P(k0+1)
         z_face(tid, pa, k0 & 1, p0, fl);   // face k0-1; pu -> upper state of k0
         wput(tid, (k0 + 1) & 1, p0);
       } else {
         fetch(col, k0 - 1, ur, f);
-        take(ur, f, pa);                   // P(k0-1)
+        take(ur, f, pa, col, k0 - 1);      // P(k0-1)
         fetch(col, k0, ur, f);
-        take(ur, f, pb);                   // P(k0)
+        take(ur, f, pb, col, k0);          // P(k0)
         {
           S t[NQ];
 #pragma unroll
@@ -1175,6 +1187,10 @@ __device__ __forceinline__ void slab_body(const Box<S>& b, unsigned lb, double d
         if (dst[it] < 0) continue;
         double p[NQ];
         u2prim<NQ>(b.ph, u[it], p);
+        if (ORDER == 1 && b.pc) {  // Box::pc: interior cells only
+          const int hi = i0 - H + dst[it] % RX, hj = j0 - H + dst[it] / RX;
+          if (hi >= 1 && hi <= g.nx && hj >= 1 && hj <= g.ny) p[4] = b.pc[cidx(g, hi, hj, m)];
+        }
         S ps[NQ];
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
@@ -1198,7 +1214,7 @@ __device__ __forceinline__ void slab_body(const Box<S>& b, unsigned lb, double d
     if (own) {
       S pm[NQ], pn[NQ];
       wget(tid, m & 1, pm);
-      take(un, unf, pn);
+      take(un, unf, pn, col, m + ORDER);
       if constexpr (ORDER == 2) {
         z_face(tid, pm, (m + 1) & 1, pn, ff);
         wput(tid, m & 1, pn);  // slot of P(m) now holds P(m+2)
@@ -1274,7 +1290,7 @@ __device__ __forceinline__ int e_nbr(int i, int n, int d, int bc) {
 
 template <int NQ, typename C>
 __device__ __forceinline__ double cfl_cell(const Phys& ph, double dx, double dy, double dz, const double* u,
-                                           double dtp);
+                                           double dtp, const double* pcell = nullptr);
 
 // CFL = 1 (corrector only): also get_timestep's candidate of each updated
 // cell into *dtp, non-finite values into *bad
@@ -1412,13 +1428,17 @@ __global__ void __launch_bounds__(256) k_cooling(Box<S> b, Cooling cl, double dt
 #pragma unroll
   for (int q = 0; q < 10; q++) u[q] = q < 8 + g.npas ? (double)b.U[q * N + c] : 0.0;
   const double T = module_temp(b.ph, u2prim_t<10>(b.ph, u, p), p, cl.tempsc);
-  if (!(T > 10000.)) return;
+  if (!(T > 10000.)) {
+    if (b.pc) b.pc[c] = p[4];  // primit(5) = u2prim's pressure
+    return;
+  }
   const double eth0 = b.ph.cv * p[4];
   const double aloss = cool_lambda(cl, T);
   const double dens = p[0];
   const double ce = (aloss * (dens * dens)) / (eth0 * cl.psc);
   const double pn = p[4] * exp(-(ce * (dt * cl.tsc)));
   b.U[4 * N + c] = (S)((u[4] - eth0) + b.ph.cv * pn);
+  if (b.pc) b.pc[c] = pn;  // primit(5), cooled
 }
 
 // ---------------------------------------------------------------------------
@@ -1811,10 +1831,11 @@ __device__ __forceinline__ double wave_min(double v) {
 // speeds and candidates are C (FP32 in the mixed form).
 template <int NQ, typename C>
 __device__ __forceinline__ double cfl_cell(const Phys& ph, double dx, double dy, double dz, const double* u,
-                                           double dtp) {
+                                           double dtp, const double* pcell) {
   using A = Ar<C>;
   double pd[NQ];
   u2prim<NQ>(ph, u, pd);
+  if (pcell) pd[4] = *pcell;  // the pressure primit carries after table cooling
   C p[8];
 #pragma unroll
   for (int q = 0; q < 8; q++) p[q] = (C)pd[q];
@@ -1875,7 +1896,7 @@ __device__ __forceinline__ void cfl_body(const Box<S>& b, unsigned lb, int kchun
         u[q] = q < 8 + g.npas ? (double)b.U[q * g.ncell + c] : 0.0;
         bad |= !__builtin_isfinite(u[q]);
       }
-      dtp = cfl_cell<NQ, S>(b.ph, b.dx, b.dy, b.dz, u, dtp);
+      dtp = cfl_cell<NQ, S>(b.ph, b.dx, b.dy, b.dz, u, dtp, b.pc ? b.pc + c : nullptr);
     }
   }
   cfl_reduce(dtp, bad, dtmin, err);
@@ -2131,13 +2152,18 @@ __global__ void __launch_bounds__(256) k_soa_to_aos(long long ncell, int nq, con
 // whole array, hydro_core.f90:304-316), AoS output
 template <typename S>
 __global__ void __launch_bounds__(256) k_prim_all(long long ncell, int nq, Phys ph, const S* __restrict__ U,
-                                                  double* __restrict__ Paos, double* __restrict__ T) {
+                                                  double* __restrict__ Paos, double* __restrict__ T, Geom g,
+                                                  const double* __restrict__ pc) {
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < ncell;
        c += (long long)gridDim.x * blockDim.x) {
     double u[10], p[10];
 #pragma unroll
     for (int q = 0; q < 10; q++) u[q] = q < nq ? (double)U[q * ncell + c] : 0.0;
     const double te = u2prim_t<10>(ph, u, p);
+    if (pc) {  // the pressure primit carries after table cooling, interior cells (Box::pc)
+      const long long i = c % g.NX - 1, r = c / g.NX, j = r % g.NY - 1, k = r / g.NY - 1;
+      if (i >= 1 && i <= g.nx && j >= 1 && j <= g.ny && k >= 1 && k <= g.nz) p[4] = pc[c];
+    }
     for (int q = 0; q < nq; q++) Paos[c * nq + q] = p[q];
     if (T) T[c] = te;
   }
@@ -2423,8 +2449,9 @@ void launch_soa_to_aos(hipStream_t s, long long ncell, const S* D, double* A, in
 }
 
 template <typename S>
-void launch_prim_all(hipStream_t s, long long ncell, const Phys& ph, const S* U, double* Paos, double* T, int nq) {
-  hipLaunchKernelGGL(k_prim_all<S>, dim3(grid_for(ncell)), dim3(256), 0, s, ncell, nq, ph, U, Paos, T);
+void launch_prim_all(hipStream_t s, long long ncell, const Phys& ph, const S* U, double* Paos, double* T, int nq,
+                     const Geom& g, const double* pc) {
+  hipLaunchKernelGGL(k_prim_all<S>, dim3(grid_for(ncell)), dim3(256), 0, s, ncell, nq, ph, U, Paos, T, g, pc);
 }
 
 template <typename S>
@@ -2462,7 +2489,8 @@ void launch_divb(hipStream_t s, const Geom& g, const S* U, double dx, double dy,
   template void launch_ghost<S>(hipStream_t, const Geom&, S*, int, int, int, int);                               \
   template void launch_aos_to_soa<S>(hipStream_t, long long, const double*, S*, int);                             \
   template void launch_soa_to_aos<S>(hipStream_t, long long, const S*, double*, int);                             \
-  template void launch_prim_all<S>(hipStream_t, long long, const Phys&, const S*, double*, double*, int);         \
+  template void launch_prim_all<S>(hipStream_t, long long, const Phys&, const S*, double*, double*, int,         \
+                                   const Geom&, const double*);                                               \
   template void launch_divb<S>(hipStream_t, const Geom&, const S*, double, double, double, double*);
 GM_INSTANTIATE(double)
 GM_INSTANTIATE(float)

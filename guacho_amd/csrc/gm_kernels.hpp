@@ -42,6 +42,15 @@ struct Box {
   int srcin;  // 1: the stage kernels add the source terms as they store each
               // cell (no passive scalars); else the source pass does
   const double* usrc;  // with srcin: the user's s of the stage (neq x ncell), or null
+  // The pressure of each interior cell as the reference's primit holds it
+  // after table cooling: coolingdmc / coolingchi set primit(5) of every
+  // interior cell (cooled where T > 1e4 K) and tstep then recomputes only
+  // the ghost primitives (hydro_solver.f90:218-223), so the next
+  // get_timestep and predictor read that pressure rather than one recovered
+  // from the cooled E.  When set, the first stage's primitive recovery and
+  // the CFL minimum take an interior cell's pressure from here (index as
+  // U's); the table-cooling kernel writes it.  Null otherwise.
+  double* pc = nullptr;
 };
 
 enum : int { SRC_EIGHT_WAVE = 1, SRC_USER = 2 };
@@ -189,9 +198,10 @@ template <typename S>
 void launch_aos_to_soa(hipStream_t s, long long ncell, const double* A, S* D, int nq = 8);
 template <typename S>
 void launch_soa_to_aos(hipStream_t s, long long ncell, const S* D, double* A, int nq = 8);
+// pc (Box::pc): interior cells take that pressure; null: every cell's from U
 template <typename S>
-void launch_prim_all(hipStream_t s, long long ncell, const Phys& ph, const S* U, double* Paos, double* T,
-                     int nq = 8);
+void launch_prim_all(hipStream_t s, long long ncell, const Phys& ph, const S* U, double* Paos, double* T, int nq,
+                     const Geom& g, const double* pc);
 template <typename S>
 void launch_divb(hipStream_t s, const Geom& g, const S* U, double dx, double dy, double dz, double* D);
 

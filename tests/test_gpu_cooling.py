@@ -6,9 +6,13 @@ for bit with the reference's own tables (tests/golden/ref_cooling.npz,
 written from oracle/_ref); the tables used here come from that fixture.
 Tolerances: the device's exp and log10 are within 1 ulp of the host's libm,
 so after one step every field but E is bit for bit and E agrees to a few
-ulps (relative 1e-14); over several steps the reference also carries the
-cooled pressure in primit while the device recovers it from E (a rounding
-difference), so the fields agree to relative 1e-10.
+ulps (relative 1e-14).  The reference carries the cooled pressure of each
+interior cell in primit into the next step's get_timestep and predictor
+(coolingdmc sets primit(5); tstep then recomputes only the ghost
+primitives, hydro_solver.f90:218-223); the device keeps that pressure too
+(gm_download_primit returns it) instead of recovering it from the cooled E,
+so over five steps the fields still agree to relative 1e-14 of their scale
+(measured: 1.5e-16; recovering the pressure from E instead left 1e-10).
 """
 import os
 import sys
@@ -54,22 +58,32 @@ def run(kind, nsteps, seed, bc=(1,) * 6, n=16):
             dev.tstep(dt)
             t += dt
         ud = dev.download_u()
+        pd = dev.download_primit()
         dev.set_cooling(_capi.GM_COOL_NONE)
         dev.upload_u(u0)
         dev.tstep(timestep_clamp(1, 10, 0.4, dev.cfl_dtp(), 0.0, 1e30)[0])
         u_nocool = dev.download_u()
     uo = O.get_u()
+    po = O.get_primit()
     O.close()
-    return u0, ud, uo, u_nocool
+    return u0, ud, uo, u_nocool, pd, po
 
 
 @pytest.mark.parametrize("kind", [_capi.GM_COOL_DMC, _capi.GM_COOL_CHI], ids=["dmc", "chianti"])
 def test_cooling_one_step(kind):
     say(f"case: cooling kind {kind}, 16^3 outflow, 1 step")
-    u0, ud, uo, u_nocool = run(kind, 1, 11)
+    u0, ud, uo, u_nocool, pd, po = run(kind, 1, 11)
     for q in (0, 1, 2, 3, 5, 6, 7):
         np.testing.assert_array_equal(ud[q], uo[q], err_msg=f"field {q}")
     np.testing.assert_allclose(ud[4], uo[4], rtol=1e-14, atol=0)
+    # primit as the reference leaves it: the cooled pressure inside (exp's
+    # ulp), every other component and the ghosts' pressure bit for bit
+    inner = (slice(2, -2),) * 3
+    for q in (0, 1, 2, 3, 5, 6, 7):
+        np.testing.assert_array_equal(pd[q], po[q], err_msg=f"primit {q}")
+    rel = np.abs(pd[4][inner] - po[4][inner]) / np.abs(po[4][inner])
+    say(f"  primit pressure: max rel diff {rel.max():.2e}")
+    assert rel.max() <= 1e-14
     inner = (slice(2, -2),) * 3
     cooled = ud[4][inner] < u_nocool[4][inner]
     assert cooled.mean() > 0.9, cooled.mean()          # T ~ 1e6 K everywhere: every cell cools
@@ -78,9 +92,13 @@ def test_cooling_one_step(kind):
 @pytest.mark.parametrize("kind", [_capi.GM_COOL_DMC, _capi.GM_COOL_CHI], ids=["dmc", "chianti"])
 def test_cooling_several_steps(kind):
     say(f"case: cooling kind {kind}, 16^3 periodic, 5 steps")
-    u0, ud, uo, _ = run(kind, 5, 12, bc=(3,) * 6)
+    u0, ud, uo, _, pd, po = run(kind, 5, 12, bc=(3,) * 6)
     scale = np.abs(uo).max(axis=(1, 2, 3), keepdims=True)
-    assert np.max(np.abs(ud - uo) / scale) <= 1e-10
+    err = np.max(np.abs(ud - uo) / scale)
+    perr = np.max(np.abs(pd - po) / np.abs(po).max(axis=(1, 2, 3), keepdims=True))
+    say(f"  5 steps: max field error {err:.2e} of the field scale, primit {perr:.2e}")
+    assert err <= 1e-14
+    assert perr <= 1e-14
 
 
 def test_cooling_rejects_bad_arguments():
