@@ -229,11 +229,15 @@ int gm_set_user_bc(gm_ctx *ctx, const char *code_object, const char *kernel);
  * holds the reference's cooltab(2, 41) as its read_table leaves it (column
  * major: T_1, Lambda_1, T_2, Lambda_2, ...; for DMC already 10**a,
  * 10**(-b)); tempsc, psc, tsc are the parameters module's Tempsc, Psc and
- * tsc.  GM_COOL_NONE turns cooling off (cooltab may then be NULL).  The
- * reference keeps the cooled pressure in primit for the next step; the
- * device recovers it from E, which differs at the rounding level, and its
- * exp/log10 are within 1 ulp of the host's: cooled runs match the reference
- * to a tolerance, not bit for bit. */
+ * tsc.  GM_COOL_NONE turns cooling off (cooltab may then be NULL).  As
+ * the reference keeps the cooled pressure in primit for the next step
+ * (cooling_dmc.f90:145-148; tstep then recomputes only the ghost
+ * primitives), the library keeps each interior cell's pressure after
+ * GM_COOL_DMC / GM_COOL_CHI for the next step's CFL minimum, predictor and
+ * gm_download_primit (one more double per cell, allocated here on the first
+ * table setting; GM_ERR_OOM if that fails).  The device's exp/log10 are
+ * within 1 ulp of the host's, so cooled runs match the reference to a
+ * tolerance (1e-14 of each field's scale over 5 steps), not bit for bit. */
 int gm_set_cooling(gm_ctx *ctx, int32_t kind, const double *cooltab, double tempsc, double psc, double tsc);
 
 #define GM_TC_OFF         0   /* constants.f90:96-98 */
