@@ -16,6 +16,11 @@
 // edge or corner ghost, so a per-axis mapping is exact.
 #include "gm_kernels.hpp"
 
+#ifndef GM_KERNELS_PART
+#define GM_KERNELS_PART 0
+#endif
+#define GM_PART(n) (GM_KERNELS_PART == 0 || GM_KERNELS_PART == (n))
+
 namespace gm {
 
 __device__ __forceinline__ long long cidx(const Geom& g, int i, int j, int k) {
@@ -2048,6 +2053,7 @@ __global__ void __launch_bounds__(256) kb_cfl(const Box<S>* __restrict__ boxes, 
 }
 
 
+#if GM_PART(1)
 // Start of a batched step: reset the CFL minimum of every box in the
 // launch (slots[0..nbox), the packed table's clock slots).
 __global__ void kb_clock_begin(BoxClock* clk, const int* __restrict__ slots, int nbox) {
@@ -2086,6 +2092,7 @@ __global__ void kb_clock_dt(BoxClock* clk, const int* __restrict__ slots, int nb
   if (finish || (c.max_steps > 0 && c.steps >= c.max_steps)) c.done = 1;
   clk[b] = c;
 }
+#endif  // GM_PART(1)
 
 // ---------------------------------------------------------------------------
 // ghost fill of one face of the conserved state, serial branches:
@@ -2203,6 +2210,7 @@ static inline unsigned grid_for(long long n, int block = 256) {
 
 static inline unsigned ceil_div(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
 
+#if GM_PART(1)
 unsigned box_tiles(const Geom& g, int kclass, const Shape& sh) {
   switch (kclass) {
     case KC_XY: return ceil_div(g.nx, TX) * ceil_div(g.ny, TY) * (unsigned)g.nz;
@@ -2211,6 +2219,7 @@ unsigned box_tiles(const Geom& g, int kclass, const Shape& sh) {
   }
   return 0;
 }
+#endif
 
 // solver x order x flux-CD dispatch; CALL(SOLVER, ORDER, FLUXCD)
 #define GM_STAGE_DISPATCH(CALL)                                                      \
@@ -2257,6 +2266,7 @@ void launch_z(hipStream_t s, const Box<S>& b, double dt, int stage, int solver, 
 #undef GM_CALL
 }
 
+#if GM_PART(1)
 unsigned slab_tiles(const Geom& g, int kchunk) {
   return ceil_div(g.nx, TX) * ceil_div(g.ny, TYS) * ceil_div(g.nz, kchunk);
 }
@@ -2272,6 +2282,7 @@ int slab_kchunk(const Geom& g) {
   if (nslab < 1) nslab = 1;
   return (int)((g.nz + nslab - 1) / nslab);
 }
+#endif  // GM_PART(1)
 
 template <typename S>
 void launch_slab(hipStream_t s, const Box<S>& b, double dt, int stage, int solver, int flux_cd, int* err,
@@ -2422,6 +2433,7 @@ void launch_cfl_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix
   hipLaunchKernelGGL(kb_cfl<S>, dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, sh.kchunk, clk, err);
 }
 
+#if GM_PART(1)
 void launch_clock_begin(hipStream_t s, BoxClock* clk, const int* slots, int nbox) {
   hipLaunchKernelGGL(kb_clock_begin, dim3(ceil_div(nbox, 64)), dim3(64), 0, s, clk, slots, nbox);
 }
@@ -2429,6 +2441,7 @@ void launch_clock_begin(hipStream_t s, BoxClock* clk, const int* slots, int nbox
 void launch_clock_dt(hipStream_t s, BoxClock* clk, const int* slots, int nbox, int tuning) {
   hipLaunchKernelGGL(kb_clock_dt, dim3(ceil_div(nbox, 64)), dim3(64), 0, s, clk, slots, nbox, tuning);
 }
+#endif  // GM_PART(1)
 
 template <typename S>
 void launch_ghost(hipStream_t s, const Geom& g, S* U, int axis, int side, int kind, int layers) {
@@ -2460,10 +2473,18 @@ void launch_divb(hipStream_t s, const Geom& g, const S* U, double dx, double dy,
   hipLaunchKernelGGL(k_divb<S>, dim3(grid_for(n)), dim3(256), 0, s, g, U, dx, dy, dz, D);
 }
 
-#define GM_INSTANTIATE(S)                                                                                        \
-  template void launch_xy<S>(hipStream_t, const Box<S>&, double, int, int, int, int*);                            \
-  template void launch_z<S>(hipStream_t, const Box<S>&, double, int, int, int, int*, const Shape&);               \
-  template void launch_slab<S>(hipStream_t, const Box<S>&, double, int, int, int, int*, int);                     \
+// Explicit instantiations, split into parts so the library builds as
+// parallel compilations of this one file: GM_KERNELS_PART 1-4 the FP64
+// kernels (1: plane tiles and the other single-box kernels, 2: z pencils,
+// 3: slabs, 4: batches), 5-8 the same for FP32; 0 (the default) all of
+// them.  Part 1 also holds the non-template kernels and host functions.
+#define GM_INST_XY(S)                                                                                            \
+  template void launch_xy<S>(hipStream_t, const Box<S>&, double, int, int, int, int*);
+#define GM_INST_Z(S)                                                                                             \
+  template void launch_z<S>(hipStream_t, const Box<S>&, double, int, int, int, int*, const Shape&);
+#define GM_INST_SLAB(S)                                                                                          \
+  template void launch_slab<S>(hipStream_t, const Box<S>&, double, int, int, int, int*, int);
+#define GM_INST_REST(S)                                                                                          \
   template void launch_bfield<S>(hipStream_t, const Box<S>&, double, int, const Shape&);                          \
   template void launch_cfl<S>(hipStream_t, const Box<S>&, unsigned long long*, int*, const Shape&);               \
   template void launch_bfield_cfl<S>(hipStream_t, const Box<S>&, double, unsigned long long*, int*, const Shape&); \
@@ -2472,6 +2493,13 @@ void launch_divb(hipStream_t s, const Geom& g, const S* U, double dx, double dy,
   template void launch_viscous<S>(hipStream_t, const Box<S>&, double);                                            \
   template void launch_tc_dtmin<S>(hipStream_t, const Box<S>&, const Conduction&, unsigned long long*);           \
   template void launch_tc_substep<S>(hipStream_t, const Box<S>&, const Conduction&, double);                      \
+  template void launch_ghost<S>(hipStream_t, const Geom&, S*, int, int, int, int);                               \
+  template void launch_aos_to_soa<S>(hipStream_t, long long, const double*, S*, int);                             \
+  template void launch_soa_to_aos<S>(hipStream_t, long long, const S*, double*, int);                             \
+  template void launch_prim_all<S>(hipStream_t, long long, const Phys&, const S*, double*, double*, int,         \
+                                   const Geom&, const double*);                                               \
+  template void launch_divb<S>(hipStream_t, const Geom&, const S*, double, double, double, double*);
+#define GM_INST_BATCH(S)                                                                                         \
   template void launch_sources_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned,               \
                                         const BoxClock*, int, const Shape&);                                      \
   template void launch_xy_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, const BoxClock*,   \
@@ -2485,15 +2513,37 @@ void launch_divb(hipStream_t s, const Geom& g, const S* U, double dx, double dy,
   template void launch_bfield_cfl_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, BoxClock*,  \
                                            int*, const Shape&);                                                   \
   template void launch_cfl_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, BoxClock*, int*,  \
-                                    const Shape&);                                                                \
-  template void launch_ghost<S>(hipStream_t, const Geom&, S*, int, int, int, int);                               \
-  template void launch_aos_to_soa<S>(hipStream_t, long long, const double*, S*, int);                             \
-  template void launch_soa_to_aos<S>(hipStream_t, long long, const S*, double*, int);                             \
-  template void launch_prim_all<S>(hipStream_t, long long, const Phys&, const S*, double*, double*, int,         \
-                                   const Geom&, const double*);                                               \
-  template void launch_divb<S>(hipStream_t, const Geom&, const S*, double, double, double, double*);
-GM_INSTANTIATE(double)
-GM_INSTANTIATE(float)
-#undef GM_INSTANTIATE
+                                    const Shape&);
+#if GM_PART(1)
+GM_INST_XY(double)
+GM_INST_REST(double)
+#endif
+#if GM_PART(2)
+GM_INST_Z(double)
+#endif
+#if GM_PART(3)
+GM_INST_SLAB(double)
+#endif
+#if GM_PART(4)
+GM_INST_BATCH(double)
+#endif
+#if GM_PART(5)
+GM_INST_XY(float)
+GM_INST_REST(float)
+#endif
+#if GM_PART(6)
+GM_INST_Z(float)
+#endif
+#if GM_PART(7)
+GM_INST_SLAB(float)
+#endif
+#if GM_PART(8)
+GM_INST_BATCH(float)
+#endif
+#undef GM_INST_XY
+#undef GM_INST_Z
+#undef GM_INST_SLAB
+#undef GM_INST_REST
+#undef GM_INST_BATCH
 
 }  // namespace gm
