@@ -101,6 +101,23 @@ def test_cooling_several_steps(kind):
     assert perr <= 1e-14
 
 
+@pytest.mark.parametrize("kind", [_capi.GM_COOL_DMC, _capi.GM_COOL_CHI], ids=["dmc", "chianti"])
+def test_cooling_carried_pressure_in_every_stage_kernel(kind, monkeypatch):
+    # the carried primit pressure enters the first stage through the slab
+    # kernel by default and through k_xy + k_z in the two-kernel stage: the
+    # two give the same bits, and both the oracle's values
+    say(f"case: cooling kind {kind}, 16^3 periodic, 4 steps, slab predictor vs two-kernel predictor")
+    monkeypatch.delenv("GUACHO_AMD_SLAB_STAGES", raising=False)
+    monkeypatch.delenv("GUACHO_AMD_SLAB_KCHUNK", raising=False)
+    _, ud_slab, uo, _, pd_slab, _ = run(kind, 4, 13, bc=(3,) * 6)
+    monkeypatch.setenv("GUACHO_AMD_SLAB_KCHUNK", "0")
+    _, ud_two, _, _, pd_two, _ = run(kind, 4, 13, bc=(3,) * 6)
+    np.testing.assert_array_equal(ud_two, ud_slab)
+    np.testing.assert_array_equal(pd_two, pd_slab)
+    scale = np.abs(uo).max(axis=(1, 2, 3), keepdims=True)
+    assert np.max(np.abs(ud_slab - uo) / scale) <= 1e-14
+
+
 def test_cooling_rejects_bad_arguments():
     say("case: cooling argument checks")
     p = _capi.make_params(8, 8, 8, 1 / 8, 1 / 8, 1 / 8)
