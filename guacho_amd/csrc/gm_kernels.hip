@@ -1922,7 +1922,11 @@ k_xy(Box<S> b, double dt, int* err) {
 template <int SOLVER, int ORDER, int FLUXCD, typename S, int NP, bool W8 = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ORDER == 1 && FLUXCD && NP == 0 && !W8 ? 3 : 1)))
 k_z(Box<S> b, double dt, int kchunk, int* err) {
-  z_body<SOLVER, ORDER, FLUXCD, S, NP, W8>(b, xcd_logical_block(), dt, kchunk, err);
+  // tiles in the reverse of the xy kernel's order, so the first pencils
+  // read the partial sums k_xy wrote last, still in the memory-side cache;
+  // the corrector's field update then runs forward over what this kernel
+  // wrote last
+  z_body<SOLVER, ORDER, FLUXCD, S, NP, W8>(b, gridDim.x - 1 - xcd_logical_block(), dt, kchunk, err);
 }
 
 // the 2.5-D stage kernel: up to 256 VGPRs (two waves per SIMD, two
@@ -1936,7 +1940,11 @@ k_slab(Box<S> b, double dt, int kchunk, int* err) {
 
 template <typename S>
 __global__ void __launch_bounds__(256) k_bfield(Box<S> b, double dt, int stage, int kchunk) {
-  bfield_body(b, xcd_logical_block(), dt, stage, kchunk);
+  // the predictor's field update runs its tiles in the reverse of the slab
+  // kernel's order (E written last is read first, from the memory-side
+  // cache); the corrector's, after the reversed z kernel, runs forward
+  const unsigned lb = xcd_logical_block();
+  bfield_body(b, stage == 1 ? gridDim.x - 1 - lb : lb, dt, stage, kchunk);
 }
 
 // the corrector's field update with the next step's CFL minimum folded in:
