@@ -112,8 +112,9 @@ struct gm_ctx {
 namespace {
 
 // The stage kernels of a step, per stage: planes per slab of the 2.5-D
-// kernel, or 0 for the two-kernel stage.  The predictor runs as a slab
-// kernel and the corrector as the two-kernel stage, in both precisions:
+// kernel, or 0 for the two-kernel stage.  On a grid that fills the GPU the
+// predictor runs as a slab kernel and the corrector as the two-kernel
+// stage, in both precisions:
 // at 256^3, same box, the first-order slab predictor takes 1.51-1.56 ms
 // against 1.59-1.62 for k_xy + k_z (FP64; mixed 0.78 against 0.79), while
 // the limited slab corrector, held to two waves per SIMD by its window,
@@ -129,7 +130,14 @@ void stage_policy(int precision, const gm::Geom& g, int kslab[2]) {
   (void)precision;
   const int k = gm::slab_kchunk(g);
   kslab[0] = g.npas == 0 ? k : 0;
-  kslab[1] = 0;
+  // a grid whose corrector tiles (32 x 16 x 1) do not reach two per CU runs
+  // the corrector as a slab kernel too: one kernel instead of two, and the
+  // two-kernel stage's occupancy advantage needs a full GPU.  Per step, FP64
+  // (mixed): 128x128x4 0.077 ms against 0.108 before (0.064), 1024x4x4
+  // 0.066 against 0.096, 32^3 0.092 against 0.193, 64^3 0.137 against
+  // 0.209; 128^3 and up unchanged.
+  const long long xy_tiles = (long long)((g.nx + 31) / 32) * ((g.ny + 15) / 16) * g.nz;
+  kslab[1] = (g.npas == 0 && xy_tiles <= 512) ? k : 0;
   int len = k;
   if (const char* e = std::getenv("GUACHO_AMD_SLAB_KCHUNK")) {
     const long v = std::strtol(e, nullptr, 10);

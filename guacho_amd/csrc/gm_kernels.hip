@@ -2279,14 +2279,19 @@ unsigned slab_tiles(const Geom& g, int kchunk) {
   return ceil_div(g.nx, TX) * ceil_div(g.ny, TYS) * ceil_div(g.nz, kchunk);
 }
 
-// z-planes per slab: about 2048 workgroups (eight per CU), at least 8
-// planes per slab so the window's warm-up (three cells and a face below the
-// slab) stays small, whole grid in one slab when the grid is thin
+// z-planes per slab: about 2048 workgroups (eight per CU, so the tail of
+// the launch is short) on a grid of at least 16384 plane tiles; about 512
+// on a smaller one, down to one plane per slab, since there a workgroup's
+// march up its slab, not the number of workgroups, sets the time (the
+// window's warm-up -- three cells and a face below the slab -- costs less
+// than a march of several planes on a GPU the grid does not fill).
+// Measured at 256^3: 8 to 64 planes within 1%; on 128x128x4, 1024x4x4, 32^3
+// and 64^3 one or two planes per slab are 20-50% faster per step than 4-8.
 int slab_kchunk(const Geom& g) {
   const long long per_plane = (long long)ceil_div(g.nx, TX) * ceil_div(g.ny, TYS);
-  long long nslab = (2048 + per_plane - 1) / per_plane;
-  const long long max_slab = g.nz / 8 > 1 ? g.nz / 8 : 1;
-  if (nslab > max_slab) nslab = max_slab;
+  const long long target = per_plane * g.nz >= 16384 ? 2048 : 512;
+  long long nslab = (target + per_plane - 1) / per_plane;
+  if (nslab > g.nz) nslab = g.nz;
   if (nslab < 1) nslab = 1;
   return (int)((g.nz + nslab - 1) / nslab);
 }

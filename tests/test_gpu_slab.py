@@ -3,7 +3,9 @@ one pass marching up z) against the two-kernel stage (k_xy plane tiles,
 then k_z pencils through W): the same bits after several steps, every
 step's CFL minimum included, with the slab kernel for both stages at slab
 lengths from one plane to the whole grid and with the library's default
-(slab predictor, two-kernel corrector), partial tiles in x and y, every
+(on these grids, which do not fill the GPU, the slab kernel for both stages
+with one or two planes per slab; at 64^3 and up the slab predictor and the
+two-kernel corrector), partial tiles in x and y, every
 boundary kind, the four solvers, passive scalars, FP64 and mixed precision.
 (The FP64 path is also checked against the oracle bit for bit by the other
 GPU tests, which run the default.)"""
