@@ -273,10 +273,10 @@ bool graph_matches(const gm_batch* b) {
 
 // Time candidate shapes on the admitted boxes; keep the fastest: the
 // predictor's stage kernel (the two-kernel stage, or the 2.5-D slab kernel
-// with 16 or 32 planes per slab) x the column kernels' tile width x
+// with 4, 8, 16 or 32 planes per slab) x the column kernels' tile width x
 // z-pencil length.
 int autotune(gm_batch* b) {
-  static const int slabs[] = {0, 16, 32};
+  static const int slabs[] = {0, 4, 8, 16, 32};
   static const int blocks[] = {64, 128, 256};
   static const int chunks[] = {8, 16, 32};
   hipEvent_t e0, e1;
