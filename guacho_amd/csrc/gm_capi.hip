@@ -127,8 +127,9 @@ namespace {
 // 0: two-kernel) and GUACHO_AMD_SLAB_STAGES ("1", "2", "12": the stages
 // that use the slab kernel) override the choice.
 void stage_policy(int precision, const gm::Geom& g, int kslab[2]) {
-  (void)precision;
-  const int k = gm::slab_kchunk(g);
+  // slab workgroups resident at once: two per CU in FP64 (256 VGPRs), three
+  // in the mixed form (the FP32 slab kernels hold 154 and 190)
+  const int k = gm::slab_kchunk(g, precision == GM_PRECISION_MIXED ? 768 : 512);
   kslab[0] = g.npas == 0 ? k : 0;
   // a grid whose corrector tiles (32 x 16 x 1) do not reach two per CU runs
   // the corrector as a slab kernel too: one kernel instead of two, and the

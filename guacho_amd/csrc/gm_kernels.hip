@@ -2279,21 +2279,39 @@ unsigned slab_tiles(const Geom& g, int kchunk) {
   return ceil_div(g.nx, TX) * ceil_div(g.ny, TYS) * ceil_div(g.nz, kchunk);
 }
 
-// z-planes per slab: about 2048 workgroups (eight per CU, so the tail of
-// the launch is short) on a grid of at least 16384 plane tiles; about 512
-// on a smaller one, down to one plane per slab, since there a workgroup's
-// march up its slab, not the number of workgroups, sets the time (the
-// window's warm-up -- three cells and a face below the slab -- costs less
-// than a march of several planes on a GPU the grid does not fill).
-// Measured at 256^3: 8 to 64 planes within 1%; on 128x128x4, 1024x4x4, 32^3
-// and 64^3 one or two planes per slab are 20-50% faster per step than 4-8.
-int slab_kchunk(const Geom& g) {
+// z-planes per slab.  About 2048 workgroups (eight per CU, so the launch's
+// tail is short): at 256^3 that is 32 planes per slab, and 8 to 64 run
+// within 1%; at 96^3 to 192^3, 2 to 13 planes, within 2% of the best length
+// swept.  A grid of no more than two rounds of resident workgroups at one
+// plane per slab (`resident`: the slab workgroups the GPU holds at once,
+// two per CU in FP64, three in the mixed form) does not keep the GPU busy
+// for long, and there the length k minimises
+//   ceil(workgroups(k) / resident) * (k + 1),
+// the rounds of resident workgroups times a workgroup's march (k planes
+// plus about one plane of window warm-up): one round of one- or two-plane
+// marches rather than a second, nearly empty, round.  Per step against the
+// former choice (at least 8 planes per slab): 128x128x4 0.077 ms against
+// 0.108, 1024x4x4 0.066 against 0.096, 32^3 0.092 against 0.193, 64^3
+// 0.137 against 0.209.
+int slab_kchunk(const Geom& g, int resident) {
   const long long per_plane = (long long)ceil_div(g.nx, TX) * ceil_div(g.ny, TYS);
-  const long long target = per_plane * g.nz >= 16384 ? 2048 : 512;
-  long long nslab = (target + per_plane - 1) / per_plane;
-  if (nslab > g.nz) nslab = g.nz;
-  if (nslab < 1) nslab = 1;
-  return (int)((g.nz + nslab - 1) / nslab);
+  if (per_plane * g.nz > 2LL * resident) {
+    long long nslab = (2048 + per_plane - 1) / per_plane;
+    if (nslab > g.nz) nslab = g.nz;
+    if (nslab < 1) nslab = 1;
+    return (int)((g.nz + nslab - 1) / nslab);
+  }
+  int best = 1;
+  long long best_cost = -1;
+  for (int k = 1; k <= g.nz; k++) {
+    const long long wg = per_plane * ((g.nz + k - 1) / k);
+    const long long cost = ((wg + resident - 1) / resident) * (k + 1);
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best = k;
+    }
+  }
+  return best;
 }
 #endif  // GM_PART(1)
 

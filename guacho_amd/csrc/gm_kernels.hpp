@@ -120,7 +120,7 @@ template <typename S>
 void launch_slab(hipStream_t s, const Box<S>& b, double dt, int stage, int solver, int flux_cd, int* err,
                  int kchunk);
 // default slab length for a grid, and the workgroups of a slab launch
-int slab_kchunk(const Geom& g);
+int slab_kchunk(const Geom& g, int resident);
 unsigned slab_tiles(const Geom& g, int kchunk);
 template <typename S>
 void launch_bfield(hipStream_t s, const Box<S>& b, double dt, int stage, const Shape& sh);
