@@ -379,7 +379,8 @@ class Batch:
 
     def tuning(self):
         """(column tile width, z-pencil length, slab planes (0: two-kernel
-        stage), autotuned ms per step)"""
+        stage; positive: slab kernel for the predictor; negative: for both
+        stages, -slab planes), autotuned ms per step)"""
         b, k, sl, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
         self._check(self.L.gm_batch_tuning(self.h, C.byref(b), C.byref(k), C.byref(sl), C.byref(ms)))
         return b.value, k.value, sl.value, ms.value

@@ -229,7 +229,7 @@ void enqueue_steps_t(gm_batch* b, int nsteps, int tuning) {
     }
     gm::launch_clock_dt(s, b->dclk, b->dslots, n, tuning);
     for (int stage = 1; stage <= 2; stage++) {
-      if (sh.slab && stage == 1) {  // the slab kernel for the predictor, as single boxes run it
+      if (sh.slab && (stage == 1 || sh.slab_both)) {  // the slab kernel, as single boxes run it
         gm::launch_slab_batch(s, boxes, pre(gm::KC_SLAB), n, tslab, b->dclk, stage, solver, fcd, b->derr, sh);
       } else {
         gm::launch_xy_batch(s, boxes, pre(gm::KC_XY), n, txy, b->dclk, stage, solver, fcd, b->derr);
@@ -268,15 +268,19 @@ int capture_graph(gm_batch* b) {
 bool graph_matches(const gm_batch* b) {
   return b->exec && b->cap_nbox == (int)b->order.size() && b->cap_sources == b->any_sources &&
          b->cap_shape.block == b->shape.block && b->cap_shape.slab == b->shape.slab &&
+         b->cap_shape.slab_both == b->shape.slab_both &&
          b->cap_shape.kchunk == b->shape.kchunk && std::memcmp(b->cap_tiles, b->tiles, sizeof b->tiles) == 0;
 }
 
 // Time candidate shapes on the admitted boxes; keep the fastest: the
 // predictor's stage kernel (the two-kernel stage, or the 2.5-D slab kernel
 // with 4, 8, 16 or 32 planes per slab) x the column kernels' tile width x
-// z-pencil length.
+// z-pencil length; then, with the best tile width and pencil length, both
+// stages as slab kernels of 1 to 32 planes (what a batch too small to fill
+// the GPU runs fastest, as a single small box does).
 int autotune(gm_batch* b) {
   static const int slabs[] = {0, 4, 8, 16, 32};
+  static const int both_slabs[] = {1, 2, 4, 8, 16, 32};
   static const int blocks[] = {64, 128, 256};
   static const int chunks[] = {8, 16, 32};
   hipEvent_t e0, e1;
@@ -284,27 +288,34 @@ int autotune(gm_batch* b) {
   BHIP(b, hipEventCreate(&e1));
   double best = 1e300;
   gm::Shape best_sh = b->shape;
-  for (int sl : slabs)
-  for (int blk : blocks)
-    for (int kc : chunks) {
-      const gm::Shape sh = {blk, kc, sl};
-      b->shape = sh;
-      int rc = rebuild_tables(b, sh);
-      if (rc != GM_OK) return rc;
-      enqueue_steps(b, 1, 1);  // warm-up
-      (void)hipEventRecord(e0, b->stream);
-      enqueue_steps(b, 2, 1);
-      (void)hipEventRecord(e1, b->stream);
-      BHIP(b, hipEventSynchronize(e1));
-      float ms = 0.f;
-      BHIP(b, hipEventElapsedTime(&ms, e0, e1));
-      if (ms / 2.0 < best) {
-        best = ms / 2.0;
-        best_sh = sh;
-      }
+  auto time_shape = [&](const gm::Shape& sh) -> int {
+    b->shape = sh;
+    int rc = rebuild_tables(b, sh);
+    if (rc != GM_OK) return rc;
+    enqueue_steps(b, 1, 1);  // warm-up
+    (void)hipEventRecord(e0, b->stream);
+    enqueue_steps(b, 2, 1);
+    (void)hipEventRecord(e1, b->stream);
+    BHIP(b, hipEventSynchronize(e1));
+    float ms = 0.f;
+    BHIP(b, hipEventElapsedTime(&ms, e0, e1));
+    if (ms / 2.0 < best) {
+      best = ms / 2.0;
+      best_sh = sh;
     }
+    return GM_OK;
+  };
+  int rc = GM_OK;
+  for (int sl : slabs)
+    for (int blk : blocks)
+      for (int kc : chunks)
+        if (rc == GM_OK) rc = time_shape(gm::Shape{blk, kc, sl, 0});
+  const gm::Shape cols = best_sh;
+  for (int sl : both_slabs)
+    if (rc == GM_OK) rc = time_shape(gm::Shape{cols.block, cols.kchunk, sl, 1});
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (rc != GM_OK) return rc;
   b->shape = best_sh;
   b->tuned = true;
   b->tuned_ms = best;
@@ -395,10 +406,14 @@ int gm_batch_create(const gm_batch_params* bp, gm_batch** out) {
   gm_batch* b = new gm_batch();
   b->bp = *bp;
   // stage kernels before (or without) autotuning: the two-kernel stage, or
-  // GUACHO_AMD_BATCH_SLAB planes per slab of the 2.5-D kernel
+  // GUACHO_AMD_BATCH_SLAB planes per slab of the 2.5-D kernel for the
+  // predictor (negative: for both stages)
   if (const char* e = std::getenv("GUACHO_AMD_BATCH_SLAB")) {
     const long v = std::strtol(e, nullptr, 10);
-    if (v >= 0 && v <= (1 << 20)) b->shape.slab = (int)v;
+    if (v >= -(1 << 20) && v <= (1 << 20)) {
+      b->shape.slab = (int)(v < 0 ? -v : v);
+      b->shape.slab_both = v < 0 ? 1 : 0;
+    }
   }
   b->esize = bp->precision == GM_PRECISION_MIXED ? 4 : 8;
   b->slots.resize(bp->max_boxes);
@@ -684,7 +699,7 @@ int gm_batch_tuning(const gm_batch* b, int32_t* block, int32_t* kchunk, int32_t*
   if (!b) return GM_ERR_ARG;
   if (block) *block = b->shape.block;
   if (kchunk) *kchunk = b->shape.kchunk;
-  if (slab) *slab = b->shape.slab;
+  if (slab) *slab = b->shape.slab_both ? -b->shape.slab : b->shape.slab;
   if (ms_per_step) *ms_per_step = b->tuned ? b->tuned_ms : 0.0;
   return GM_OK;
 }

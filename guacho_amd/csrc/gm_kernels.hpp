@@ -87,7 +87,8 @@ struct BoxClock {
 struct Shape {
   int block;
   int kchunk;
-  int slab;  // planes per slab of the 2.5-D stage kernel; 0: the two-kernel stage
+  int slab;        // planes per slab of the 2.5-D stage kernel; 0: the two-kernel stage
+  int slab_both;   // batches: 1 when the corrector runs as the slab kernel too
 };
 
 // Validates a subdomain configuration; GM_OK or a GM_ERR_* code with msg.

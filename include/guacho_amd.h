@@ -343,8 +343,10 @@ int gm_batch_clock(gm_batch *batch, int32_t box_id, double *time, int64_t *steps
  * launch in flight steps the box; otherwise runs beside the launches. */
 int gm_batch_download_u(gm_batch *batch, int32_t box_id, double *u);
 /* Launch shape in use -- the column kernels' tile width and z-pencil length,
- * the slab kernel's planes per slab (0: the two-kernel stage) -- and the
- * autotuner's measured step time (0 if not tuned). */
+ * the slab kernel's planes per slab (0: the two-kernel stage for both
+ * stages; positive: the slab kernel for the predictor; negative: for both
+ * stages, -slab planes per slab) -- and the autotuner's measured step time
+ * (0 if not tuned). */
 int gm_batch_tuning(const gm_batch *batch, int32_t *block, int32_t *kchunk, int32_t *slab, double *ms_per_step);
 /* Number of boxes in the batch. */
 int gm_batch_active(const gm_batch *batch, int32_t *n);

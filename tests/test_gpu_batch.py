@@ -140,7 +140,7 @@ def test_autotune_keeps_results_and_picks_a_shape():
         B.sync()
         block, kchunk, slab, ms = B.tuning()
         got = [B.download_u(i) for i in ids]
-    assert block in (64, 128, 256) and kchunk in (8, 16, 32) and slab in (0, 4, 8, 16, 32) and ms > 0
+    assert block in (64, 128, 256) and kchunk in (8, 16, 32) and slab in (0, 4, 8, 16, 32, -1, -2, -4, -8, -16, -32) and ms > 0
     for (p, u), g in zip(bx, got):
         np.testing.assert_array_equal(g, single(p, u, nsteps=6)[0])
 
@@ -168,16 +168,18 @@ def test_budget_and_removal():
         assert B.active() == 1
 
 
+@pytest.mark.parametrize("slab", [8, -2], ids=["predictor-8", "both-2"])
 @pytest.mark.parametrize("prec", [_capi.GM_PRECISION_FP64, _capi.GM_PRECISION_MIXED], ids=["fp64", "mixed"])
-def test_batch_slab_kernel_equals_single_box_path(prec, monkeypatch):
-    """The batch with the 2.5-D stage kernel (kb_slab, 8 planes per slab:
-    boxes of one to three slabs) equals the single-box path bit for bit."""
-    say(f"case: batch with the slab stage kernel vs single path, precision {prec}, 5 steps")
-    monkeypatch.setenv("GUACHO_AMD_BATCH_SLAB", "8")
+def test_batch_slab_kernel_equals_single_box_path(prec, slab, monkeypatch):
+    """The batch with the 2.5-D stage kernel (kb_slab) for the predictor (8
+    planes per slab: boxes of one to three slabs) or for both stages (2
+    planes per slab) equals the single-box path bit for bit."""
+    say(f"case: batch with the slab stage kernel ({slab}) vs single path, precision {prec}, 5 steps")
+    monkeypatch.setenv("GUACHO_AMD_BATCH_SLAB", str(slab))
     bx = boxes(prec)
     with _capi.Batch(precision=prec, max_boxes=8, steps_per_launch=5) as B:
         ids = [B.add(p, u, max_steps=5) for p, u in bx]
-        assert B.tuning()[2] == 8
+        assert B.tuning()[2] == slab
         B.advance(1)
         B.sync()
         got = [B.download_u(i) for i in ids]
