@@ -239,6 +239,36 @@ __device__ __forceinline__ void perm_y_if(bool ydir, T* v) {
   v[6] = f;
 }
 
+// Cross-lane moves within one 64-lane wavefront by DPP and v_permlane32_swap
+// (no LDS permute): lane_up1 gives lane l the value of lane l-1 (DPP
+// wave_shr:1), lane_down1 that of lane l+1 (wave_shl:1), lane_up32 gives
+// lanes 32..63 the values of lanes 0..31.  A lane with no source lane (lane
+// 0, lane 63, lanes 0..31 in that order) reads 0, 0 and its own value; the
+// callers replace those.  Every lane of the wavefront must be active.
+template <int CTRL>
+__device__ __forceinline__ unsigned dpp_word(unsigned v) {
+  return (unsigned)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xf, 0xf, true);
+}
+__device__ __forceinline__ unsigned half_word(unsigned v) { return __builtin_amdgcn_permlane32_swap(v, v, false, false)[0]; }
+template <int CTRL, typename S>
+__device__ __forceinline__ S lane_move(S v) {
+  if constexpr (sizeof(S) == 8) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = CTRL ? dpp_word<CTRL>((unsigned)u) : half_word((unsigned)u);
+    const unsigned hi = CTRL ? dpp_word<CTRL>((unsigned)(u >> 32)) : half_word((unsigned)(u >> 32));
+    return __builtin_bit_cast(S, ((unsigned long long)hi << 32) | lo);
+  } else {
+    const unsigned u = __builtin_bit_cast(unsigned, v);
+    return __builtin_bit_cast(S, CTRL ? dpp_word<CTRL>(u) : half_word(u));
+  }
+}
+template <typename S>
+__device__ __forceinline__ S lane_up1(S v) { return lane_move<0x138>(v); }
+template <typename S>
+__device__ __forceinline__ S lane_down1(S v) { return lane_move<0x130>(v); }
+template <typename S>
+__device__ __forceinline__ S lane_up32(S v) { return lane_move<0>(v); }
+
 // Faces of one plane tile, from the primitives in LDS (stored, then a
 // barrier) to the four fluxes of the thread's own cell: fx (low x face),
 // fxr (high x face), fy (low y face), fyu (high y face).  Thread (c, r) --
@@ -371,7 +401,7 @@ __device__ __forceinline__ bool xy_solve(const Box<S>& b, S* __restrict__ lds, i
         for (int q = 0; q < NQ; q++) xu[q * TYT + r] = pxu[q];
       }
 #pragma unroll
-      for (int q = 0; q < NQ; q++) pl[q] = __shfl_up(pxu[q], 1, 64);
+      for (int q = 0; q < NQ; q++) pl[q] = lane_up1(pxu[q]);
     }
     __syncthreads();  // halo states in LDS
     if (c == 0) {
@@ -390,7 +420,7 @@ __device__ __forceinline__ bool xy_solve(const Box<S>& b, S* __restrict__ lds, i
         for (int q = 0; q < NQ; q++) yu[(q * L::NYU + (r + 1) / 2) * TX + c] = pyu[q];
       }
 #pragma unroll
-      for (int q = 0; q < NQ; q++) pl[q] = __shfl_up(pyu[q], 32, 64);
+      for (int q = 0; q < NQ; q++) pl[q] = lane_up32(pyu[q]);
     }
     __syncthreads();  // every read of the primitives is done; y states in LDS
     if (w == 0) __builtin_amdgcn_s_setprio(2);
@@ -447,7 +477,7 @@ __device__ __forceinline__ void xy_exchange(S* __restrict__ lds, int wx, int wy,
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < NQ; q++) {
-    const S v = __shfl_down(fx[q], 1, 64);
+    const S v = lane_down1(fx[q]);
     fxr[q] = c == TX - 1 ? lend[q * TYT + r] : v;
   }
 }
@@ -639,7 +669,7 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
         for (int q = 0; q < NQ; q++) xu[q * TY + r] = pxu[q];
       }
 #pragma unroll
-      for (int q = 0; q < NQ; q++) pl[q] = __shfl_up(pxu[q], 1, 64);
+      for (int q = 0; q < NQ; q++) pl[q] = lane_up1(pxu[q]);
     }
     __syncthreads();  // halo states in LDS
     if (c == 0) {
@@ -658,7 +688,7 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
         for (int q = 0; q < NQ; q++) yu[(q * L::NYU + (r + 1) / 2) * TX + c] = pyu[q];
       }
 #pragma unroll
-      for (int q = 0; q < NQ; q++) pl[q] = __shfl_up(pyu[q], 32, 64);
+      for (int q = 0; q < NQ; q++) pl[q] = lane_up32(pyu[q]);
     }
     __syncthreads();  // every read of the primitives is done; y states in LDS
     if (w == 0) __builtin_amdgcn_s_setprio(2);
@@ -718,7 +748,7 @@ __device__ __forceinline__ void xy_body(const Box<S>& b, unsigned lb, double dt,
   S fxr[NQ], fyu[NQ];
 #pragma unroll
   for (int q = 0; q < NQ; q++) {
-    const S v = __shfl_down(fx[q], 1, 64);
+    const S v = lane_down1(fx[q]);
     fxr[q] = c == TX - 1 ? lend[q * TY + r] : v;
   }
   if (!own) return;
