@@ -489,11 +489,11 @@ def extensions_section(u0, dx, dy, dz, n: int, steps: int):
             "conduction_ms_per_step": res["conduction_iso"]["kernel_ms_per_step"].get("conduction")}
 
 
-def serving_section(nreq: int, seed: int, fractions=(0.5, 1.0, 2.0, 4.0)):
+def serving_section(nreq: int, seed: int, fractions=(0.25, 0.5, 1.0, 2.0, 4.0)):
     """Continuous batching of a Poisson stream of mixed solves (Brio-Wu
     1024x4x4, Orszag-Tang 256^2, blasts 64^3-192^3) against serving the same
     stream one request at a time with the plain FP64 path; arrival rate
-    swept as fractions of the sequential FP64 service rate (at 0.5x and 1x
+    swept as fractions of the sequential FP64 service rate (at 0.25x to 1x
     arrivals bound both; from 2x on, and saturated, the GPU does)."""
     from guacho_amd import serving
     log(f"serving section: {nreq} requests, sequential FP64 service rate")
