@@ -1960,7 +1960,9 @@ k_z(Box<S> b, double dt, int kchunk, int* err) {
 }
 
 // the 2.5-D stage kernel: up to 256 VGPRs (two waves per SIMD, two
-// workgroups per CU) for the column window carried across the plane march
+// workgroups per CU) for the column window carried across the plane march;
+// the FP64 corrector with passive scalars needs more LDS than two
+// workgroups per CU leave, so it runs at one (the compiler says so)
 template <int SOLVER, int ORDER, int FLUXCD, typename S, int NP>
 __global__ void __launch_bounds__(SL_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
 k_slab(Box<S> b, double dt, int kchunk, int* err) {
@@ -2238,7 +2240,7 @@ __global__ void __launch_bounds__(256) k_divb(Geom g, const S* __restrict__ U, d
 // ---------------------------------------------------------------------------
 // host-side launchers, instantiated for FP64 and FP32 storage
 // ---------------------------------------------------------------------------
-static inline unsigned grid_for(long long n, int block = 256) {
+[[maybe_unused]] static inline unsigned grid_for(long long n, int block = 256) {
   long long b = (n + block - 1) / block;
   const long long cap = 256LL * 1024;  // grid-stride loops cover the rest
   if (b > cap) b = cap;
@@ -2246,7 +2248,7 @@ static inline unsigned grid_for(long long n, int block = 256) {
   return (unsigned)b;
 }
 
-static inline unsigned ceil_div(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+[[maybe_unused]] static inline unsigned ceil_div(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
 
 #if GM_PART(1)
 unsigned box_tiles(const Geom& g, int kclass, const Shape& sh) {
