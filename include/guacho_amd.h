@@ -130,10 +130,11 @@ int gm_boundary_u(gm_ctx *ctx);
  * in between makes it reduce over the stored state again. */
 int gm_cfl_dtp(gm_ctx *ctx, double *dtp);
 
-/* Advance the state by dt: tstep (src/hydro_solver.f90:134-229) with the
- * add-ons (cooling, chemistry, radiation, conduction) off.  Asynchronous;
- * a non-finite flux sets an error reported by the next gm_cfl_dtp or
- * gm_sync. */
+/* Advance the state by dt: tstep (src/hydro_solver.f90:134-229), with the
+ * source terms, viscosity, table or H-rate cooling, thermal conduction and
+ * user plugins the context has switched on; chemistry and diffuse radiation
+ * are not part of the library.  Asynchronous; a non-finite flux sets an
+ * error reported by the next gm_cfl_dtp or gm_sync. */
 int gm_tstep(gm_ctx *ctx, double dt);
 
 /* Wait for queued work; report any deferred device error. */
