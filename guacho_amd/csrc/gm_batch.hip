@@ -71,6 +71,7 @@ struct gm_batch {
   hipEvent_t ev_launch = nullptr;
   std::vector<void*> zombies;
   bool any_sources = false;        // some active box has 8-wave source terms
+  bool any_viscous = false;        // some active box has artificial viscosity
   unsigned tiles[gm::KC_COUNT] = {0};
   bool tables_dirty = true;
   bool graph_dirty = true;
@@ -86,6 +87,7 @@ struct gm_batch {
   unsigned cap_tiles[gm::KC_COUNT] = {0};
   gm::Shape cap_shape = {0, 0};
   bool cap_sources = false;
+  bool cap_viscous = false;
   std::string msg;
 };
 
@@ -148,6 +150,7 @@ gm::Box<S> box_view(const Slot& s, int slot) {
   b.src = s.p.eight_wave ? gm::SRC_EIGHT_WAVE : 0;
   b.srcin = 0;
   b.usrc = nullptr;
+  b.eta = s.p.eta;
   return b;
 }
 
@@ -189,9 +192,11 @@ int rebuild_tables_t(gm_batch* b, const gm::Shape& sh) {
     b->tiles[k] = acc;
   }
   b->any_sources = false;
+  b->any_viscous = false;
   for (int i = 0; i < n; i++) {
     boxes[i] = box_view<S>(b->slots[b->order[i]], b->order[i]);
     b->any_sources = b->any_sources || boxes[i].src != 0;
+    b->any_viscous = b->any_viscous || boxes[i].eta != 0.0;
   }
   BHIP(b, hipStreamSynchronize(b->stream));
   if (n) BHIP(b, hipMemcpy(b->dboxes, boxes.data(), sizeof(gm::Box<S>) * n, hipMemcpyHostToDevice));
@@ -217,11 +222,11 @@ void enqueue_steps_t(gm_batch* b, int nsteps, int tuning) {
   const gm::Shape& sh = b->shape;
   const int solver = b->bp.riemann_solver, fcd = b->bp.enable_flux_cd;
   const unsigned txy = b->tiles[gm::KC_XY], tcol = b->tiles[gm::KC_COL], tslab = b->tiles[gm::KC_SLAB];
-  // With flux-CD and no source pass, each box's CFL minimum for the next
-  // step is folded into the corrector's field update (its clock holds it;
-  // gm_batch_add reduces a new box's first one); otherwise a CFL pass
-  // starts every step.
-  const bool fold = fcd && !b->any_sources;
+  // With flux-CD and no source or viscous pass, each box's CFL minimum for
+  // the next step is folded into the corrector's field update (its clock
+  // holds it; gm_batch_add reduces a new box's first one); otherwise a CFL
+  // pass starts every step.
+  const bool fold = fcd && !b->any_sources && !b->any_viscous;
   for (int st = 0; st < nsteps; st++) {
     if (!fold) {
       gm::launch_clock_begin(s, b->dclk, b->dslots, n);
@@ -238,6 +243,9 @@ void enqueue_steps_t(gm_batch* b, int nsteps, int tuning) {
       if (fold && stage == 2) gm::launch_bfield_cfl_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, b->derr, sh);
       else if (fcd) gm::launch_bfield_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, stage, sh);
       if (b->any_sources) gm::launch_sources_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, stage, sh);
+      // viscous_copy (src/hydro_solver.f90:188) after the corrector
+      if (stage == 2 && b->any_viscous)
+        gm::launch_viscous_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, sh);
     }
   }
 }
@@ -262,11 +270,13 @@ int capture_graph(gm_batch* b) {
   std::memcpy(b->cap_tiles, b->tiles, sizeof b->tiles);
   b->cap_shape = b->shape;
   b->cap_sources = b->any_sources;
+  b->cap_viscous = b->any_viscous;
   return GM_OK;
 }
 
 bool graph_matches(const gm_batch* b) {
   return b->exec && b->cap_nbox == (int)b->order.size() && b->cap_sources == b->any_sources &&
+         b->cap_viscous == b->any_viscous &&
          b->cap_shape.block == b->shape.block && b->cap_shape.slab == b->shape.slab &&
          b->cap_shape.slab_both == b->shape.slab_both &&
          b->cap_shape.kchunk == b->shape.kchunk && std::memcmp(b->cap_tiles, b->tiles, sizeof b->tiles) == 0;
@@ -508,7 +518,6 @@ int gm_batch_add(gm_batch* b, const gm_params* p, const double* u, double cfl, d
     if (p->bc[f] == GM_BC_OTHER)
       return bfail(b, GM_ERR_CONFIG, "BC_OTHER (user boundary hooks) is for single-box contexts (gm_set_user_bc)");
   if (p->neq != 8) return bfail(b, GM_ERR_CONFIG, "passive scalars (neq > 8) are for single-box contexts");
-  if (p->eta != 0.0) return bfail(b, GM_ERR_CONFIG, "artificial viscosity (eta != 0) is for single-box contexts");
   if (p->precision != b->bp.precision || p->riemann_solver != b->bp.riemann_solver ||
       (p->enable_flux_cd != 0) != (b->bp.enable_flux_cd != 0))
     return bfail(b, GM_ERR_CONFIG, "box precision, solver and flux-CD must match the batch");

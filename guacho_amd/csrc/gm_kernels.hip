@@ -1616,15 +1616,12 @@ __device__ __forceinline__ double visc_val(const Box<S>& b, const ViscNbr& n, in
   return q == n.flip ? -v : v;
 }
 
+// viscous_copy of interior cell (i, j, k): up + eta * (the six neighbours'
+// sum - 6 up) into W, neighbours outside the domain from U* through the
+// boundary mapping; then W back into U
 template <typename S>
-__global__ void __launch_bounds__(256) k_viscous(Box<S> b, double eta) {
+__device__ __forceinline__ void visc_cell(const Box<S>& b, int i, int j, int k, double eta) {
   const Geom& g = b.g;
-  const long long n = (long long)g.nx * g.ny * g.nz;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int i = (int)(t % g.nx) + 1;
-  const long long r = t / g.nx;
-  const int j = (int)(r % g.ny) + 1, k = (int)(r / g.ny) + 1;
   const long long c = cidx(g, i, j, k), N = g.ncell;
   const ViscNbr xp = visc_nbr(g, i + 1, j, k), xm = visc_nbr(g, i - 1, j, k);
   const ViscNbr yp = visc_nbr(g, i, j + 1, k), ym = visc_nbr(g, i, j - 1, k);
@@ -1638,16 +1635,29 @@ __global__ void __launch_bounds__(256) k_viscous(Box<S> b, double eta) {
 }
 
 template <typename S>
+__device__ __forceinline__ void visc_store_cell(const Box<S>& b, int i, int j, int k) {
+  const long long c = cidx(b.g, i, j, k), N = b.g.ncell;
+  for (int q = 0; q < 8 + b.g.npas; q++) b.U[q * N + c] = (S)b.W[q * N + c];
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_viscous(Box<S> b, double eta) {
+  const Geom& g = b.g;
+  const long long n = (long long)g.nx * g.ny * g.nz;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const long long r = t / g.nx;
+  visc_cell(b, (int)(t % g.nx) + 1, (int)(r % g.ny) + 1, (int)(r / g.ny) + 1, eta);
+}
+
+template <typename S>
 __global__ void __launch_bounds__(256) k_visc_store(Box<S> b) {
   const Geom& g = b.g;
   const long long n = (long long)g.nx * g.ny * g.nz;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const int i = (int)(t % g.nx) + 1;
   const long long r = t / g.nx;
-  const int j = (int)(r % g.ny) + 1, k = (int)(r / g.ny) + 1;
-  const long long c = cidx(g, i, j, k), N = g.ncell;
-  for (int q = 0; q < 8 + g.npas; q++) b.U[q * N + c] = (S)b.W[q * N + c];
+  visc_store_cell(b, (int)(t % g.nx) + 1, (int)(r % g.ny) + 1, (int)(r / g.ny) + 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -2083,6 +2093,30 @@ __global__ void __launch_bounds__(256) kb_sources(const Box<S>* __restrict__ box
   sources_body(b, lb, stage_dt(clk, b.slot, stage), stage, kchunk);
 }
 
+// viscous_copy over a batch's column tiles (STORE: the copy back from W).
+// A box without viscosity (Box::eta = 0), or whose step has dt = 0 (it has
+// completed, or the autotuner is timing shapes), keeps its state: unlike
+// the update, viscous_copy does not scale with dt.
+template <typename S, bool STORE>
+__global__ void __launch_bounds__(256) kb_viscous(const Box<S>* __restrict__ boxes,
+                                                  const unsigned* __restrict__ prefix, int nbox,
+                                                  const BoxClock* __restrict__ clk, int kchunk) {
+  int bx;
+  unsigned lb;
+  batch_locate(prefix, nbox, xcd_logical_block(), bx, lb);
+  const Box<S>& b = boxes[bx];
+  if (b.eta == 0.0 || clk[b.slot].dt == 0.0) return;
+  const Geom& g = b.g;
+  int t, kc;
+  if (!tile_item(lb, tiles_of((unsigned)(g.nx * g.ny)), (unsigned)(g.nx * g.ny), t, kc)) return;
+  const int i = t % g.nx + 1, j = t / g.nx + 1;
+  const int k0 = kc * kchunk + 1, k1 = min(k0 + kchunk - 1, g.nz);
+  for (int k = k0; k <= k1; k++) {
+    if (STORE) visc_store_cell(b, i, j, k);
+    else visc_cell(b, i, j, k, b.eta);
+  }
+}
+
 template <typename S>
 __global__ void __launch_bounds__(256) kb_cfl(const Box<S>* __restrict__ boxes, const unsigned* __restrict__ prefix,
                                               int nbox, int kchunk, BoxClock* clk, int* err) {
@@ -2444,6 +2478,13 @@ void launch_sources_batch(hipStream_t s, const Box<S>* boxes, const unsigned* pr
 }
 
 template <typename S>
+void launch_viscous_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
+                          const BoxClock* clk, const Shape& sh) {
+  hipLaunchKernelGGL((kb_viscous<S, false>), dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, clk, sh.kchunk);
+  hipLaunchKernelGGL((kb_viscous<S, true>), dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, clk, sh.kchunk);
+}
+
+template <typename S>
 void launch_xy_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
                      const BoxClock* clk, int stage, int solver, int flux_cd, int* err) {
 #define GM_CALL(SV, O, F) \
@@ -2565,6 +2606,8 @@ void launch_divb(hipStream_t s, const Geom& g, const S* U, double dx, double dy,
 #define GM_INST_BATCH(S)                                                                                         \
   template void launch_sources_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned,               \
                                         const BoxClock*, int, const Shape&);                                      \
+  template void launch_viscous_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned,               \
+                                        const BoxClock*, const Shape&);                                           \
   template void launch_xy_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, const BoxClock*,   \
                                    int, int, int, int*);                                                          \
   template void launch_z_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, const BoxClock*,    \

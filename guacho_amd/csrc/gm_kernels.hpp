@@ -51,6 +51,10 @@ struct Box {
   // the CFL minimum take an interior cell's pressure from here (index as
   // U's); the table-cooling kernel writes it.  Null otherwise.
   double* pc = nullptr;
+  // artificial viscosity of viscous_copy (src/hydro_solver.f90:47-65) for
+  // batched boxes (launch_viscous_batch); single boxes pass it to
+  // launch_viscous
+  double eta = 0.0;
 };
 
 enum : int { SRC_EIGHT_WAVE = 1, SRC_USER = 2 };
@@ -185,6 +189,11 @@ void launch_bfield_cfl_batch(hipStream_t s, const Box<S>* boxes, const unsigned*
 template <typename S>
 void launch_sources_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
                           const BoxClock* clk, int stage, const Shape& sh);
+// viscous_copy after the corrector for the boxes with Box::eta != 0 whose
+// step has dt > 0 (tiles of class KC_COL)
+template <typename S>
+void launch_viscous_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
+                          const BoxClock* clk, const Shape& sh);
 // clock kernels over the boxes of a launch: slots[0..nbox) (device) are
 // their clock slots
 void launch_clock_begin(hipStream_t s, BoxClock* clk, const int* slots, int nbox);

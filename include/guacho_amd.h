@@ -82,8 +82,8 @@ typedef struct gm_params {
   double cv;              /* specific heat at constant volume */
   double gamma;           /* (cv+1)/cv, passed as the caller computed it */
   double eta;             /* artificial viscosity of viscous_copy
-                             (src/hydro_solver.f90:47-65); finite; batches
-                             take 0 only */
+                             (src/hydro_solver.f90:47-65); finite; per box
+                             in batches */
 } gm_params;
 
 typedef struct gm_ctx gm_ctx;

@@ -107,6 +107,33 @@ def test_alone_equals_in_mix_mixed():
     np.testing.assert_array_equal(alone, mixed)
 
 
+@pytest.mark.parametrize("prec", [_capi.GM_PRECISION_FP64, _capi.GM_PRECISION_MIXED], ids=["fp64", "mixed"])
+@pytest.mark.parametrize("autotune", [False, True], ids=["fixed", "autotuned"])
+def test_viscous_boxes_equal_single_box_path(prec, autotune):
+    """Boxes with artificial viscosity of their own (viscous_copy,
+    src/hydro_solver.f90:47-65, 188) beside one without, in one batch: each
+    equals the single-box path bit for bit.  The graph runs three steps past
+    the boxes' last (5 steps in launches of 4), and the autotuner times
+    shapes with dt = 0 steps; neither may diffuse a box further."""
+    say(f"case: viscous batch, eta 0.02 / 0 / 0.05 / 0.01, precision {prec}, autotune {autotune}, 5 steps")
+    bx = []
+    for n, seed, eta in ((12, 11, 0.02), (16, 12, 0.0), (10, 13, 0.05)):
+        u, dx, dy, dz = problems.blast(n, seed=seed)
+        bx.append((_capi.make_params(n, n, n, dx, dy, dz, bc=(1,) * 6, precision=prec, eta=eta), u))
+    u, dx, dy, dz = problems.turbulence(12, seed=14)
+    bx.append((_capi.make_params(12, 12, 12, dx, dy, dz, bc=(2, 2, 1, 1, 3, 3), precision=prec, eta=0.01), u))
+    with _capi.Batch(precision=prec, max_boxes=8, steps_per_launch=4, autotune=autotune) as B:
+        ids = [B.add(p, u, max_steps=5) for p, u in bx]
+        B.advance(2)
+        B.sync()
+        got = [B.download_u(i) for i in ids]
+        clocks = [B.clock(i) for i in ids]
+    for (p, u), g, (t, n, done) in zip(bx, got, clocks):
+        ref, steps, tref = single(p, u, nsteps=5)
+        np.testing.assert_array_equal(g, ref)
+        assert (n, done) == (5, True) and t == tref
+
+
 def test_t_end_completion_matches_host_loop():
     say("case: OT 32x32x4 to t_end = 0.05 in a batch vs the host loop")
     u, dx, dy, dz = problems.orszag_tang(32, 32, 4)

@@ -103,16 +103,6 @@ def test_viscous_mixed_against_fp64():
     assert rel[0] <= 1e-5 and np.all(rel <= 1e-4), rel
 
 
-def test_batch_refuses_viscosity():
-    say("case: batches refuse eta != 0")
-    p = _capi.make_params(8, 8, 8, 1 / 8, 1 / 8, 1 / 8, eta=0.01)
-    rng = np.random.default_rng(2)
-    with _capi.Batch(precision=_capi.GM_PRECISION_FP64, max_boxes=2) as B:
-        with pytest.raises(_capi.GmError) as ei:
-            B.add(p, random_state(rng, 8, 8, 8), max_steps=1)
-        assert ei.value.code == _capi.GM_ERR_CONFIG
-
-
 def test_viscosity_then_table_cooling():
     """viscous_copy runs before cooling (src/hydro_solver.f90:188, 206-210):
     one step with both against the oracle; every field but E bit for bit, E
