@@ -72,6 +72,7 @@ struct gm_batch {
   std::vector<void*> zombies;
   bool any_sources = false;        // some active box has 8-wave source terms
   bool any_viscous = false;        // some active box has artificial viscosity
+  int neq = 8;                     // variables of every box: 8 + its passive scalars (the first box sets it)
   unsigned tiles[gm::KC_COUNT] = {0};
   bool tables_dirty = true;
   bool graph_dirty = true;
@@ -88,6 +89,7 @@ struct gm_batch {
   gm::Shape cap_shape = {0, 0};
   bool cap_sources = false;
   bool cap_viscous = false;
+  int cap_neq = 0;
   std::string msg;
 };
 
@@ -113,8 +115,8 @@ int bhip(gm_batch* b, hipError_t e, const char* what) {
   } while (0)
 
 int64_t box_bytes_of(const gm_params& p, size_t esize) {
-  const int64_t ncell = (int64_t)(p.nx + 4) * (p.ny + 4) * (p.nz + 4);
-  return ncell * ((8 + 8 + 3) * (int64_t)esize + 8 * (int64_t)sizeof(double));
+  const int64_t ncell = (int64_t)(p.nx + 4) * (p.ny + 4) * (p.nz + 4), nq = p.neq;
+  return ncell * ((nq + nq + 3) * (int64_t)esize + nq * (int64_t)sizeof(double));
 }
 
 gm::Geom geom_of(const gm_params& p) {
@@ -124,7 +126,7 @@ gm::Geom geom_of(const gm_params& p) {
   g.ncell = (long long)g.NX * g.NY * g.NZ;
   for (int k = 0; k < 6; k++) g.bc[k] = p.bc[k];
   g.gmem = 0;
-  g.npas = 0;
+  g.npas = p.neq - 8;
   return g;
 }
 
@@ -222,6 +224,7 @@ void enqueue_steps_t(gm_batch* b, int nsteps, int tuning) {
   const gm::Shape& sh = b->shape;
   const int solver = b->bp.riemann_solver, fcd = b->bp.enable_flux_cd;
   const unsigned txy = b->tiles[gm::KC_XY], tcol = b->tiles[gm::KC_COL], tslab = b->tiles[gm::KC_SLAB];
+  const int npas = b->neq - 8;
   // With flux-CD and no source or viscous pass, each box's CFL minimum for
   // the next step is folded into the corrector's field update (its clock
   // holds it; gm_batch_add reduces a new box's first one); otherwise a CFL
@@ -230,17 +233,17 @@ void enqueue_steps_t(gm_batch* b, int nsteps, int tuning) {
   for (int st = 0; st < nsteps; st++) {
     if (!fold) {
       gm::launch_clock_begin(s, b->dclk, b->dslots, n);
-      gm::launch_cfl_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, b->derr, sh);
+      gm::launch_cfl_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, b->derr, sh, npas);
     }
     gm::launch_clock_dt(s, b->dclk, b->dslots, n, tuning);
     for (int stage = 1; stage <= 2; stage++) {
       if (sh.slab && (stage == 1 || sh.slab_both)) {  // the slab kernel, as single boxes run it
-        gm::launch_slab_batch(s, boxes, pre(gm::KC_SLAB), n, tslab, b->dclk, stage, solver, fcd, b->derr, sh);
+        gm::launch_slab_batch(s, boxes, pre(gm::KC_SLAB), n, tslab, b->dclk, stage, solver, fcd, b->derr, sh, npas);
       } else {
-        gm::launch_xy_batch(s, boxes, pre(gm::KC_XY), n, txy, b->dclk, stage, solver, fcd, b->derr);
-        gm::launch_z_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, stage, solver, fcd, b->derr, sh);
+        gm::launch_xy_batch(s, boxes, pre(gm::KC_XY), n, txy, b->dclk, stage, solver, fcd, b->derr, npas);
+        gm::launch_z_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, stage, solver, fcd, b->derr, sh, npas);
       }
-      if (fold && stage == 2) gm::launch_bfield_cfl_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, b->derr, sh);
+      if (fold && stage == 2) gm::launch_bfield_cfl_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, b->derr, sh, npas);
       else if (fcd) gm::launch_bfield_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, stage, sh);
       if (b->any_sources) gm::launch_sources_batch(s, boxes, pre(gm::KC_COL), n, tcol, b->dclk, stage, sh);
       // viscous_copy (src/hydro_solver.f90:188) after the corrector
@@ -271,12 +274,13 @@ int capture_graph(gm_batch* b) {
   b->cap_shape = b->shape;
   b->cap_sources = b->any_sources;
   b->cap_viscous = b->any_viscous;
+  b->cap_neq = b->neq;
   return GM_OK;
 }
 
 bool graph_matches(const gm_batch* b) {
   return b->exec && b->cap_nbox == (int)b->order.size() && b->cap_sources == b->any_sources &&
-         b->cap_viscous == b->any_viscous &&
+         b->cap_viscous == b->any_viscous && b->cap_neq == b->neq &&
          b->cap_shape.block == b->shape.block && b->cap_shape.slab == b->shape.slab &&
          b->cap_shape.slab_both == b->shape.slab_both &&
          b->cap_shape.kchunk == b->shape.kchunk && std::memcmp(b->cap_tiles, b->tiles, sizeof b->tiles) == 0;
@@ -517,7 +521,10 @@ int gm_batch_add(gm_batch* b, const gm_params* p, const double* u, double cfl, d
   for (int f = 0; f < 6; f++)
     if (p->bc[f] == GM_BC_OTHER)
       return bfail(b, GM_ERR_CONFIG, "BC_OTHER (user boundary hooks) is for single-box contexts (gm_set_user_bc)");
-  if (p->neq != 8) return bfail(b, GM_ERR_CONFIG, "passive scalars (neq > 8) are for single-box contexts");
+  int active = 0;
+  for (const Slot& q : b->slots) active += q.used ? 1 : 0;
+  if (active && p->neq != b->neq)
+    return bfail(b, GM_ERR_CONFIG, "every box of a batch has the same passive scalars (neq)");
   if (p->precision != b->bp.precision || p->riemann_solver != b->bp.riemann_solver ||
       (p->enable_flux_cd != 0) != (b->bp.enable_flux_cd != 0))
     return bfail(b, GM_ERR_CONFIG, "box precision, solver and flux-CD must match the batch");
@@ -544,8 +551,9 @@ int gm_batch_add(gm_batch* b, const gm_params* p, const double* u, double cfl, d
   s.p = *p;
   s.p.device = b->bp.device;
   const gm::Geom g = geom_of(*p);
-  const size_t n8 = (size_t)g.ncell * 8 * b->esize, n3 = (size_t)g.ncell * 3 * b->esize;
-  const size_t nw = (size_t)g.ncell * 8 * sizeof(double);
+  const size_t nq = (size_t)p->neq;
+  const size_t n8 = (size_t)g.ncell * nq * b->esize, n3 = (size_t)g.ncell * 3 * b->esize;
+  const size_t nw = (size_t)g.ncell * nq * sizeof(double);
   struct { void** ptr; size_t n; } al[] = {{&s.U, n8}, {&s.Us, n8}, {&s.W, nw}, {&s.E, n3}};
   for (auto& a : al) {
     if (hipMalloc(a.ptr, a.n) != hipSuccess) {
@@ -557,9 +565,9 @@ int gm_batch_add(gm_batch* b, const gm_params* p, const double* u, double cfl, d
   }
   s.bytes = need;
   // upload: host AoS doubles -> staging (partials block) -> SoA state
-  BHIP(b, hipMemcpyAsync(s.W, u, (size_t)g.ncell * 8 * sizeof(double), hipMemcpyHostToDevice, x));
-  if (b->esize == 8) gm::launch_aos_to_soa(x, g.ncell, (const double*)s.W, (double*)s.U);
-  else gm::launch_aos_to_soa(x, g.ncell, (const double*)s.W, (float*)s.U);
+  BHIP(b, hipMemcpyAsync(s.W, u, (size_t)g.ncell * nq * sizeof(double), hipMemcpyHostToDevice, x));
+  if (b->esize == 8) gm::launch_aos_to_soa(x, g.ncell, (const double*)s.W, (double*)s.U, (int)nq);
+  else gm::launch_aos_to_soa(x, g.ncell, (const double*)s.W, (float*)s.U, (int)nq);
   BHIP(b, hipGetLastError());
   gm::BoxClock c{};
   c.dtmin = (unsigned long long)gm_bits_of(1e30);
@@ -582,6 +590,7 @@ int gm_batch_add(gm_batch* b, const gm_params* p, const double* u, double cfl, d
   s.used = true;
   s.clock = c;
   b->slots[slot] = s;
+  b->neq = p->neq;
   b->used += need;
   b->tables_dirty = true;
   b->graph_dirty = true;
@@ -693,13 +702,13 @@ int gm_batch_download_u(gm_batch* b, int32_t id, double* u) {
   hipStream_t x = b->xfer;
   if (b->esize == 8) {
     ghost_fill_t<double>(x, s);
-    gm::launch_soa_to_aos(x, g.ncell, (const double*)s.U, (double*)s.W);
+    gm::launch_soa_to_aos(x, g.ncell, (const double*)s.U, (double*)s.W, s.p.neq);
   } else {
     ghost_fill_t<float>(x, s);
-    gm::launch_soa_to_aos(x, g.ncell, (const float*)s.U, (double*)s.W);
+    gm::launch_soa_to_aos(x, g.ncell, (const float*)s.U, (double*)s.W, s.p.neq);
   }
   BHIP(b, hipGetLastError());
-  BHIP(b, hipMemcpyAsync(u, s.W, (size_t)g.ncell * 8 * sizeof(double), hipMemcpyDeviceToHost, x));
+  BHIP(b, hipMemcpyAsync(u, s.W, (size_t)g.ncell * s.p.neq * sizeof(double), hipMemcpyDeviceToHost, x));
   BHIP(b, hipStreamSynchronize(x));
   return GM_OK;
 }

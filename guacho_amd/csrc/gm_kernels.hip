@@ -2019,39 +2019,40 @@ __device__ __forceinline__ double stage_dt(const BoxClock* __restrict__ clk, int
   return stage == 1 ? dt / 2.0 : dt;
 }
 
-template <int SOLVER, int ORDER, int FLUXCD, typename S>
-__global__ void __launch_bounds__(XY_THREADS) __attribute__((amdgpu_waves_per_eu(4))) kb_xy(const Box<S>* __restrict__ boxes,
-                                                    const unsigned* __restrict__ prefix, int nbox,
-                                                    const BoxClock* __restrict__ clk, int* err) {
-  __shared__ S lds[XYLds<ORDER>::SIZE];
+// batched kernels: NP passive scalars per box (one count per batch)
+template <int SOLVER, int ORDER, int FLUXCD, typename S, int NP>
+__global__ void __launch_bounds__(XY_THREADS) __attribute__((amdgpu_waves_per_eu(NP == 0 ? 4 : 2)))
+kb_xy(const Box<S>* __restrict__ boxes, const unsigned* __restrict__ prefix, int nbox,
+      const BoxClock* __restrict__ clk, int* err) {
+  __shared__ S lds[XYLds<ORDER, 8 + NP>::SIZE];
   int bx;
   unsigned lb;
   batch_locate(prefix, nbox, xcd_logical_block(), bx, lb);
   const Box<S>& b = boxes[bx];
-  xy_body<SOLVER, ORDER, FLUXCD>(b, lb, stage_dt(clk, b.slot, ORDER), err, lds);
+  xy_body<SOLVER, ORDER, FLUXCD, S, NP>(b, lb, stage_dt(clk, b.slot, ORDER), err, lds);
 }
 
-template <int SOLVER, int ORDER, int FLUXCD, typename S>
+template <int SOLVER, int ORDER, int FLUXCD, typename S, int NP>
 __global__ void __launch_bounds__(SL_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
 kb_slab(const Box<S>* __restrict__ boxes, const unsigned* __restrict__ prefix, int nbox,
         const BoxClock* __restrict__ clk, int kchunk, int* err) {
-  __shared__ S lds[SlabLds<ORDER, 8, S>::SIZE];
+  __shared__ S lds[SlabLds<ORDER, 8 + NP, S>::SIZE];
   int bx;
   unsigned lb;
   batch_locate(prefix, nbox, xcd_logical_block(), bx, lb);
   const Box<S>& b = boxes[bx];
-  slab_body<SOLVER, ORDER, FLUXCD, S, 0>(b, lb, stage_dt(clk, b.slot, ORDER), kchunk, err, lds);
+  slab_body<SOLVER, ORDER, FLUXCD, S, NP>(b, lb, stage_dt(clk, b.slot, ORDER), kchunk, err, lds);
 }
 
-template <int SOLVER, int ORDER, int FLUXCD, typename S>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ORDER == 1 && FLUXCD ? 3 : 1)))
+template <int SOLVER, int ORDER, int FLUXCD, typename S, int NP>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ORDER == 1 && FLUXCD && NP == 0 ? 3 : 1)))
 kb_z(const Box<S>* __restrict__ boxes, const unsigned* __restrict__ prefix,
                                             int nbox, const BoxClock* __restrict__ clk, int kchunk, int* err) {
   int bx;
   unsigned lb;
   batch_locate(prefix, nbox, xcd_logical_block(), bx, lb);
   const Box<S>& b = boxes[bx];
-  z_body<SOLVER, ORDER, FLUXCD>(b, lb, stage_dt(clk, b.slot, ORDER), kchunk, err);
+  z_body<SOLVER, ORDER, FLUXCD, S, NP>(b, lb, stage_dt(clk, b.slot, ORDER), kchunk, err);
 }
 
 template <typename S>
@@ -2068,7 +2069,7 @@ __global__ void __launch_bounds__(256) kb_bfield(const Box<S>* __restrict__ boxe
 // the corrector's field update of a batch with each box's next CFL
 // minimum folded in (k_bfield_cfl per box): into the box's clock, which
 // kb_clock_dt reset to 1e30 when it took the step's dt
-template <typename S>
+template <typename S, int NQ>
 __global__ void __launch_bounds__(256) kb_bfield_cfl(const Box<S>* __restrict__ boxes,
                                                      const unsigned* __restrict__ prefix, int nbox,
                                                      BoxClock* __restrict__ clk, int kchunk, int* err) {
@@ -2078,7 +2079,7 @@ __global__ void __launch_bounds__(256) kb_bfield_cfl(const Box<S>* __restrict__ 
   const Box<S>& b = boxes[bx];
   double dtp = 1e30;
   bool bad = false;
-  bfield_body<S, 1, 8>(b, lb, stage_dt(clk, b.slot, 2), 2, kchunk, &dtp, &bad);
+  bfield_body<S, 1, NQ>(b, lb, stage_dt(clk, b.slot, 2), 2, kchunk, &dtp, &bad);
   cfl_reduce(dtp, bad, &clk[b.slot].dtmin, err);
 }
 
@@ -2117,13 +2118,13 @@ __global__ void __launch_bounds__(256) kb_viscous(const Box<S>* __restrict__ box
   }
 }
 
-template <typename S>
+template <typename S, int NQ>
 __global__ void __launch_bounds__(256) kb_cfl(const Box<S>* __restrict__ boxes, const unsigned* __restrict__ prefix,
                                               int nbox, int kchunk, BoxClock* clk, int* err) {
   int bx;
   unsigned lb;
   batch_locate(prefix, nbox, xcd_logical_block(), bx, lb);
-  cfl_body(boxes[bx], lb, kchunk, &clk[boxes[bx].slot].dtmin, err);
+  cfl_body<S, NQ>(boxes[bx], lb, kchunk, &clk[boxes[bx].slot].dtmin, err);
 }
 
 
@@ -2486,20 +2487,25 @@ void launch_viscous_batch(hipStream_t s, const Box<S>* boxes, const unsigned* pr
 
 template <typename S>
 void launch_xy_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
-                     const BoxClock* clk, int stage, int solver, int flux_cd, int* err) {
-#define GM_CALL(SV, O, F) \
-  { hipLaunchKernelGGL((kb_xy<SV, O, F, S>), dim3(tiles), dim3(XY_THREADS), 0, s, boxes, prefix, nbox, clk, err); }
+                     const BoxClock* clk, int stage, int solver, int flux_cd, int* err, int npas) {
+#define GM_CALL(SV, O, F)                                                                                        \
+  {                                                                                                               \
+    if (npas == 0) hipLaunchKernelGGL((kb_xy<SV, O, F, S, 0>), dim3(tiles), dim3(XY_THREADS), 0, s, boxes, prefix, nbox, clk, err); \
+    else if (npas == 1) hipLaunchKernelGGL((kb_xy<SV, O, F, S, 1>), dim3(tiles), dim3(XY_THREADS), 0, s, boxes, prefix, nbox, clk, err); \
+    else hipLaunchKernelGGL((kb_xy<SV, O, F, S, 2>), dim3(tiles), dim3(XY_THREADS), 0, s, boxes, prefix, nbox, clk, err); \
+  }
   GM_STAGE_DISPATCH(GM_CALL)
 #undef GM_CALL
 }
 
 template <typename S>
 void launch_z_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
-                    const BoxClock* clk, int stage, int solver, int flux_cd, int* err, const Shape& sh) {
+                    const BoxClock* clk, int stage, int solver, int flux_cd, int* err, const Shape& sh, int npas) {
 #define GM_CALL(SV, O, F)                                                                                   \
   {                                                                                                         \
-    hipLaunchKernelGGL((kb_z<SV, O, F, S>), dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, clk,     \
-                       sh.kchunk, err);                                                                     \
+    if (npas == 0) hipLaunchKernelGGL((kb_z<SV, O, F, S, 0>), dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, clk, sh.kchunk, err); \
+    else if (npas == 1) hipLaunchKernelGGL((kb_z<SV, O, F, S, 1>), dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, clk, sh.kchunk, err); \
+    else hipLaunchKernelGGL((kb_z<SV, O, F, S, 2>), dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, clk, sh.kchunk, err); \
   }
   GM_STAGE_DISPATCH(GM_CALL)
 #undef GM_CALL
@@ -2507,11 +2513,12 @@ void launch_z_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, 
 
 template <typename S>
 void launch_slab_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
-                       const BoxClock* clk, int stage, int solver, int flux_cd, int* err, const Shape& sh) {
+                       const BoxClock* clk, int stage, int solver, int flux_cd, int* err, const Shape& sh, int npas) {
 #define GM_CALL(SV, O, F)                                                                                       \
   {                                                                                                              \
-    hipLaunchKernelGGL((kb_slab<SV, O, F, S>), dim3(tiles), dim3(SL_THREADS), 0, s, boxes, prefix, nbox, clk,     \
-                       sh.slab, err);                                                                            \
+    if (npas == 0) hipLaunchKernelGGL((kb_slab<SV, O, F, S, 0>), dim3(tiles), dim3(SL_THREADS), 0, s, boxes, prefix, nbox, clk, sh.slab, err); \
+    else if (npas == 1) hipLaunchKernelGGL((kb_slab<SV, O, F, S, 1>), dim3(tiles), dim3(SL_THREADS), 0, s, boxes, prefix, nbox, clk, sh.slab, err); \
+    else hipLaunchKernelGGL((kb_slab<SV, O, F, S, 2>), dim3(tiles), dim3(SL_THREADS), 0, s, boxes, prefix, nbox, clk, sh.slab, err); \
   }
   GM_STAGE_DISPATCH(GM_CALL)
 #undef GM_CALL
@@ -2527,14 +2534,17 @@ void launch_bfield_batch(hipStream_t s, const Box<S>* boxes, const unsigned* pre
 
 template <typename S>
 void launch_bfield_cfl_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
-                             BoxClock* clk, int* err, const Shape& sh) {
-  hipLaunchKernelGGL(kb_bfield_cfl<S>, dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, clk, sh.kchunk, err);
+                             BoxClock* clk, int* err, const Shape& sh, int npas) {
+  // as launch_bfield_cfl: with passive scalars every variable through NQ = 10
+  if (npas == 0) hipLaunchKernelGGL((kb_bfield_cfl<S, 8>), dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, clk, sh.kchunk, err);
+  else hipLaunchKernelGGL((kb_bfield_cfl<S, 10>), dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, clk, sh.kchunk, err);
 }
 
 template <typename S>
 void launch_cfl_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
-                      BoxClock* clk, int* err, const Shape& sh) {
-  hipLaunchKernelGGL(kb_cfl<S>, dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, sh.kchunk, clk, err);
+                      BoxClock* clk, int* err, const Shape& sh, int npas) {
+  if (npas == 0) hipLaunchKernelGGL((kb_cfl<S, 8>), dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, sh.kchunk, clk, err);
+  else hipLaunchKernelGGL((kb_cfl<S, 10>), dim3(tiles), dim3(sh.block), 0, s, boxes, prefix, nbox, sh.kchunk, clk, err);
 }
 
 #if GM_PART(1)
@@ -2609,17 +2619,17 @@ void launch_divb(hipStream_t s, const Geom& g, const S* U, double dx, double dy,
   template void launch_viscous_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned,               \
                                         const BoxClock*, const Shape&);                                           \
   template void launch_xy_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, const BoxClock*,   \
-                                   int, int, int, int*);                                                          \
+                                   int, int, int, int*, int);                                                     \
   template void launch_z_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, const BoxClock*,    \
-                                  int, int, int, int*, const Shape&);                                             \
+                                  int, int, int, int*, const Shape&, int);                                        \
   template void launch_bfield_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned,                \
                                        const BoxClock*, int, const Shape&);                                       \
   template void launch_slab_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, const BoxClock*,   \
-                                     int, int, int, int*, const Shape&);                                          \
+                                     int, int, int, int*, const Shape&, int);                                     \
   template void launch_bfield_cfl_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, BoxClock*,  \
-                                           int*, const Shape&);                                                   \
+                                           int*, const Shape&, int);                                              \
   template void launch_cfl_batch<S>(hipStream_t, const Box<S>*, const unsigned*, int, unsigned, BoxClock*, int*,  \
-                                    const Shape&);
+                                    const Shape&, int);
 #if GM_PART(1)
 GM_INST_XY(double)
 GM_INST_REST(double)

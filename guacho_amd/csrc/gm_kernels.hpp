@@ -164,28 +164,29 @@ void launch_sources(hipStream_t s, const Box<S>& b, double dt, int stage, const 
 
 // Batched launchers: `tiles` is the total tile count of the class over all
 // boxes and `prefix` (device, nbox+1 entries) its prefix sums.  Each box's
-// dt comes from its clock.
+// dt comes from its clock.  `npas`: the passive scalars of every box of the
+// launch (0..2).
 template <typename S>
 void launch_xy_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
-                     const BoxClock* clk, int stage, int solver, int flux_cd, int* err);
+                     const BoxClock* clk, int stage, int solver, int flux_cd, int* err, int npas);
 template <typename S>
 void launch_z_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
-                    const BoxClock* clk, int stage, int solver, int flux_cd, int* err, const Shape& sh);
+                    const BoxClock* clk, int stage, int solver, int flux_cd, int* err, const Shape& sh, int npas);
 // the 2.5-D stage kernel over a batch (tiles of class KC_SLAB, sh.slab planes)
 template <typename S>
 void launch_slab_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
-                       const BoxClock* clk, int stage, int solver, int flux_cd, int* err, const Shape& sh);
+                       const BoxClock* clk, int stage, int solver, int flux_cd, int* err, const Shape& sh, int npas);
 template <typename S>
 void launch_bfield_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
                          const BoxClock* clk, int stage, const Shape& sh);
 template <typename S>
 void launch_cfl_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
-                      BoxClock* clk, int* err, const Shape& sh);
+                      BoxClock* clk, int* err, const Shape& sh, int npas);
 // the corrector's field update with each box's next CFL minimum folded into
 // its clock (valid when no source pass follows the field update)
 template <typename S>
 void launch_bfield_cfl_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
-                             BoxClock* clk, int* err, const Shape& sh);
+                             BoxClock* clk, int* err, const Shape& sh, int npas);
 template <typename S>
 void launch_sources_batch(hipStream_t s, const Box<S>* boxes, const unsigned* prefix, int nbox, unsigned tiles,
                           const BoxClock* clk, int stage, const Shape& sh);

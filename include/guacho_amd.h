@@ -64,8 +64,8 @@ extern "C" {
 typedef struct gm_params {
   int32_t nx, ny, nz;     /* interior cells of the subdomain */
   int32_t neq;            /* 8 (BFIELD) plus the passive scalars (PASSIVES
-                             builds: npas = 1 or 2), so 8, 9 or 10;
-                             batches take 8 only */
+                             builds: npas = 1 or 2), so 8, 9 or 10; the
+                             boxes of one batch share it */
   int32_t nghost;         /* must be 2 */
   int32_t riemann_solver; /* GM_SOLVER_* */
   int32_t slope_limiter;  /* GM_LIMITER_* */

@@ -12,6 +12,7 @@ import pytest
 
 from guacho_amd import _capi, problems
 from guacho_amd.host import timestep_clamp
+from tests._pairs import random_state
 
 pytestmark = pytest.mark.gpu
 
@@ -132,6 +133,37 @@ def test_viscous_boxes_equal_single_box_path(prec, autotune):
         ref, steps, tref = single(p, u, nsteps=5)
         np.testing.assert_array_equal(g, ref)
         assert (n, done) == (5, True) and t == tref
+
+
+@pytest.mark.parametrize("slab", [0, 8, -2], ids=["two-kernel", "slab-predictor-8", "slab-both-2"])
+@pytest.mark.parametrize("npas", [1, 2])
+@pytest.mark.parametrize("prec", [_capi.GM_PRECISION_FP64, _capi.GM_PRECISION_MIXED], ids=["fp64", "mixed"])
+def test_passive_scalar_boxes_equal_single_box_path(prec, npas, slab, monkeypatch):
+    """Boxes carrying passive scalars (PASSIVES builds, neq = 8 + npas): the
+    batch advances them, 8-wave terms and viscosity included, through each
+    stage-kernel form, and each box equals the single-box path bit for bit.
+    A box with another passive-scalar count is refused while they run."""
+    say(f"case: batch with {npas} passive scalars, stage kernels {slab}, precision {prec}, 6 steps")
+    monkeypatch.setenv("GUACHO_AMD_BATCH_SLAB", str(slab))
+    rng = np.random.default_rng(20 + npas)
+    bx = []
+    for (nx, ny, nz), bc, extra in (((12, 10, 8), (1,) * 6, {}),
+                                    ((16, 16, 6), (3,) * 6, {"eta": 0.02}),
+                                    ((10, 14, 12), (2, 2, 1, 1, 3, 3), {"eight_wave": True})):
+        u = random_state(rng, nx, ny, nz, npas=npas)
+        bx.append((_capi.make_params(nx, ny, nz, 1 / nx, 1 / ny, 1 / nz, bc=bc, precision=prec, npas=npas, **extra), u))
+    with _capi.Batch(precision=prec, max_boxes=4, steps_per_launch=3) as B:
+        ids = [B.add(p, u, max_steps=6) for p, u in bx]
+        p8 = _capi.make_params(8, 8, 8, 1 / 8, 1 / 8, 1 / 8, precision=prec)
+        with pytest.raises(_capi.GmError) as ei:
+            B.add(p8, random_state(rng, 8, 8, 8), max_steps=1)
+        assert ei.value.code == _capi.GM_ERR_CONFIG
+        B.advance(2)
+        B.sync()
+        got = [B.download_u(i) for i in ids]
+    for (p, u), g in zip(bx, got):
+        assert g.shape == u.shape
+        np.testing.assert_array_equal(g, single(p, u, nsteps=6)[0])
 
 
 def test_t_end_completion_matches_host_loop():

@@ -221,10 +221,10 @@ def test_passive_argument_checks():
         _capi.Subdomain(_capi.make_params(8, 8, 8, 1 / 8, 1 / 8, 1 / 8, npas=3))
     assert ei.value.code == _capi.GM_ERR_CONFIG
     rng = np.random.default_rng(2)
-    p2 = _capi.make_params(8, 8, 8, 1 / 8, 1 / 8, 1 / 8, npas=2)
+    p3 = _capi.make_params(8, 8, 8, 1 / 8, 1 / 8, 1 / 8, npas=3)
     with _capi.Batch(precision=_capi.GM_PRECISION_FP64, max_boxes=2) as B:
         with pytest.raises(_capi.GmError) as ei:
-            B.add(p2, random_state(rng, 8, 8, 8, npas=2), max_steps=1)
+            B.add(p3, random_state(rng, 8, 8, 8, npas=3), max_steps=1)
         assert ei.value.code == _capi.GM_ERR_CONFIG
 
 
