@@ -317,7 +317,8 @@ int gm_batch_free_bytes(const gm_batch *batch, int64_t *bytes);
  * starts its clock at time 0, currentIteration 1.  It completes when its
  * dt clamps to t_end (the dump clamp of hydro_core.f90:691-694) or after
  * max_steps steps (0: no limit; at most 2^31 - 1, else GM_ERR_ARG).
- * GM_ERR_BUSY when it does not fit.  Runs beside launches in flight (on a
+ * GM_ERR_BUSY when it does not fit; GM_ERR_CONFIG when its neq differs
+ * from that of the boxes the batch holds.  Runs beside launches in flight (on a
  * stream of its own) and has finished when it returns; the box joins the
  * next gm_batch_advance. */
 int gm_batch_add(gm_batch *batch, const gm_params *box, const double *u, double cfl, double t_end,
