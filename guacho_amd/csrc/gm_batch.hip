@@ -552,9 +552,9 @@ int gm_batch_add(gm_batch* b, const gm_params* p, const double* u, double cfl, d
   s.p.device = b->bp.device;
   const gm::Geom g = geom_of(*p);
   const size_t nq = (size_t)p->neq;
-  const size_t n8 = (size_t)g.ncell * nq * b->esize, n3 = (size_t)g.ncell * 3 * b->esize;
+  const size_t nu = (size_t)g.ncell * nq * b->esize, n3 = (size_t)g.ncell * 3 * b->esize;
   const size_t nw = (size_t)g.ncell * nq * sizeof(double);
-  struct { void** ptr; size_t n; } al[] = {{&s.U, n8}, {&s.Us, n8}, {&s.W, nw}, {&s.E, n3}};
+  struct { void** ptr; size_t n; } al[] = {{&s.U, nu}, {&s.Us, nu}, {&s.W, nw}, {&s.E, n3}};
   for (auto& a : al) {
     if (hipMalloc(a.ptr, a.n) != hipSuccess) {
       (void)hipGetLastError();
