@@ -2032,6 +2032,8 @@ kb_xy(const Box<S>* __restrict__ boxes, const unsigned* __restrict__ prefix, int
   xy_body<SOLVER, ORDER, FLUXCD, S, NP>(b, lb, stage_dt(clk, b.slot, ORDER), err, lds);
 }
 
+// (the FP64 corrector with passive scalars runs at one workgroup per CU,
+// as k_slab does)
 template <int SOLVER, int ORDER, int FLUXCD, typename S, int NP>
 __global__ void __launch_bounds__(SL_THREADS) __attribute__((amdgpu_waves_per_eu(2)))
 kb_slab(const Box<S>* __restrict__ boxes, const unsigned* __restrict__ prefix, int nbox,
