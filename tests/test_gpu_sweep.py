@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 
 from guacho_amd import _capi
+from guacho_amd.host import timestep_clamp
 from tests._pairs import random_state, run_pair
 
 pytestmark = pytest.mark.gpu
@@ -59,3 +60,36 @@ def test_random_configuration_bitwise(case, form, monkeypatch):
                           slope_limiter=lim, bc=bc, enable_flux_cd=fcd)
     ud, uo, _ = run_pair(p, u0, 3)
     np.testing.assert_array_equal(ud, uo)
+
+
+@pytest.mark.parametrize("form", ["default", "two-kernel"])
+@pytest.mark.parametrize("case", cases()[:16], ids=lambda c: f"c{c[0]}")
+def test_random_configuration_mixed_within_tolerance(case, form, monkeypatch):
+    """The mixed form on the same cases, against the FP64 device path with
+    the same dt sequence: density within 1e-5 relative L1, every conserved
+    variable within 1e-4, after 3 steps."""
+    i, nx, ny, nz, solver, mhd, pmhd, lim, bc, fcd = case
+    if form == "two-kernel":
+        monkeypatch.setenv("GUACHO_AMD_SLAB_KCHUNK", "0")
+    say(f"case {i} mixed ({form}): {nx}x{ny}x{nz} solver {solver} limiter {lim} bc {bc} flux-CD {fcd}, 3 steps")
+    u0 = random_state(np.random.default_rng(100 + i), nx, ny, nz)
+    kw = dict(riemann_solver=solver, mhd=mhd, pmhd=pmhd, slope_limiter=lim, bc=bc, enable_flux_cd=fcd)
+    p64 = _capi.make_params(nx, ny, nz, 1.0 / nx, 1.0 / ny, 1.0 / nz, precision=_capi.GM_PRECISION_FP64, **kw)
+    p32 = _capi.make_params(nx, ny, nz, 1.0 / nx, 1.0 / ny, 1.0 / nz, precision=_capi.GM_PRECISION_MIXED, **kw)
+    with _capi.Subdomain(p64) as d64, _capi.Subdomain(p32) as d32:
+        for d in (d64, d32):
+            d.upload_u(u0)
+            d.boundary_u()
+        t = 0.0
+        for it in range(1, 4):
+            dt, _ = timestep_clamp(it, 10, 0.4, d64.cfl_dtp(), t, 1e30)
+            d32.cfl_dtp()
+            d64.tstep(dt)
+            d32.tstep(dt)
+            t += dt
+        a, b = d64.download_u(), d32.download_u()
+    inner = (slice(None), slice(2, -2), slice(2, -2), slice(2, -2))
+    a, b = a[inner], b[inner]
+    assert np.all(np.isfinite(b))
+    rel = np.abs(a - b).sum(axis=(1, 2, 3)) / np.maximum(np.abs(a).sum(axis=(1, 2, 3)), 1e-300)
+    assert rel[0] <= 1e-5 and np.all(rel <= 1e-4), rel
