@@ -115,12 +115,12 @@ namespace {
 // kernel, or 0 for the two-kernel stage.  On a grid that fills the GPU the
 // predictor runs as a slab kernel and the corrector as the two-kernel
 // stage, in both precisions:
-// at 256^3, same box, the first-order slab predictor takes 1.51-1.56 ms
-// against 1.59-1.62 for k_xy + k_z (FP64; mixed 0.78 against 0.79), while
-// the limited slab corrector, held to two waves per SIMD by its window,
-// takes 2.19-2.30 against 1.95-2.00 (mixed 1.55 against 1.07); per step
-// FP64 4.26 ms against 4.42 (slab both stages) and 4.36-4.38 (two-kernel
-// both), mixed 2.33 against 2.79 and 2.37-2.38.  With passive scalars
+// at 256^3, same box, the first-order slab predictor takes 1.43-1.47 ms
+// against 1.55 for k_xy + k_z (FP64; mixed 0.77 against 0.80), while the
+// limited slab corrector, held to two waves per SIMD by its window, takes
+// 2.17 against 1.92 (mixed 1.52 against 1.05); per step FP64 4.13-4.14 ms
+// against 4.32-4.33 (slab both stages) and 4.23-4.24 (two-kernel both),
+// mixed 2.29 against 2.76 and 2.34-2.35.  With passive scalars
 // both stages run as the two-kernel stage: the slab kernel's registers
 // (238-256 VGPRs without them) do not hold the extra components, and it
 // spills 12-132 bytes a lane.  GUACHO_AMD_SLAB_KCHUNK (planes per slab;
