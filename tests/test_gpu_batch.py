@@ -166,6 +166,30 @@ def test_passive_scalar_boxes_equal_single_box_path(prec, npas, slab, monkeypatc
         np.testing.assert_array_equal(g, single(p, u, nsteps=6)[0])
 
 
+def test_passive_scalar_count_follows_the_boxes_held():
+    """A batch takes the passive-scalar count of the boxes it holds: once
+    its last box of one count is removed, a box of another count is admitted
+    and stepped (its graph captured anew) like the single-box path."""
+    say("case: batch empties of npas = 1 boxes, then takes an npas = 0 box, 4 steps each")
+    rng = np.random.default_rng(31)
+    p1 = _capi.make_params(10, 12, 6, 0.1, 1 / 12, 1 / 6, npas=1)
+    u1 = random_state(rng, 10, 12, 6, npas=1)
+    p0 = _capi.make_params(12, 8, 10, 1 / 12, 0.125, 0.1, bc=(1,) * 6)
+    u0 = random_state(rng, 12, 8, 10)
+    with _capi.Batch(precision=_capi.GM_PRECISION_FP64, max_boxes=2, steps_per_launch=4) as B:
+        a = B.add(p1, u1, max_steps=4)
+        B.advance(1)
+        B.sync()
+        g1 = B.download_u(a)
+        B.remove(a)
+        b = B.add(p0, u0, max_steps=4)
+        B.advance(1)
+        B.sync()
+        g0 = B.download_u(b)
+    np.testing.assert_array_equal(g1, single(p1, u1, nsteps=4)[0])
+    np.testing.assert_array_equal(g0, single(p0, u0, nsteps=4)[0])
+
+
 def test_t_end_completion_matches_host_loop():
     say("case: OT 32x32x4 to t_end = 0.05 in a batch vs the host loop")
     u, dx, dy, dz = problems.orszag_tang(32, 32, 4)
